@@ -1,0 +1,191 @@
+"""ctypes mirror of include/dfargo.h and the loader of the HIP library.
+
+The structures below must match include/dfargo.h field for field; a test
+checks their sizes against the C compiler's.
+"""
+import ctypes
+import os
+
+MAX_PLANETS = 8
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdfargo.so")
+
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC, ERR_STATE = range(6)
+FP64, FP32 = 0, 1
+BC_WALL, BC_OPEN = 0, 1
+
+FIELDS = {
+    "potential": 0, "soundspeed": 1, "pressure": 2, "temperature": 3,
+    "divv": 4, "vischeat": 5, "sg_accr": 6, "sg_acct": 7,
+    "taurr": 8, "taurp": 9, "taupp": 10,
+}
+
+c_int, c_long, c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double
+_D8 = c_double * MAX_PLANETS
+_I8 = c_int * MAX_PLANETS
+
+
+class Params(ctypes.Structure):
+    """dfargo_params (src/var.c:15-202 as decoded by src/Interpret.c)."""
+    _fields_ = [
+        ("nrad", c_int), ("nsec", c_int),
+        ("rmin", c_double), ("rmax", c_double),
+        ("log_grid", c_int),
+        ("pmin", c_double), ("pmax", c_double),
+        ("sigma0", c_double), ("sigma_slope", c_double),
+        ("aspect_ratio", c_double), ("flaring_index", c_double),
+        ("viscosity", c_double), ("alpha_viscosity", c_double),
+        ("cavity_radius", c_double), ("cavity_ratio", c_double), ("cavity_width", c_double),
+        ("transition_radius", c_double), ("transition_ratio", c_double),
+        ("transition_width", c_double),
+        ("imposed_disk_drift", c_double),
+        ("add_noise", c_int), ("noise_seed", c_long),
+        ("energy_equation", c_int), ("viscous_heating", c_int), ("beta_cooling", c_int),
+        ("adiabatic_index", c_double), ("beta_cooling_time", c_double),
+        ("beta_cooling_slope", c_double),
+        ("self_gravity", c_int), ("sg_thickness_smoothing", c_double),
+        ("thickness_smoothing", c_double), ("roche_smoothing", c_double),
+        ("indirect_term_smoothing", c_double), ("mass_taper", c_double),
+        ("indirect_term", c_int), ("discard_gas_indirect_term", c_int),
+        ("alternative_gas_indirect_term", c_int),
+        ("exclude_hill", c_int), ("exclude_hill_factor", c_double),
+        ("forced_circular", c_int),
+        ("release_date", c_double), ("release_radius", c_double),
+        ("eccentricity", c_double),
+        ("corotating", c_int), ("omega_frame", c_double),
+        ("fast_transport", c_int), ("add_floors", c_int), ("density_floor", c_double),
+        ("boundary", c_int),
+    ]
+
+    def copy(self):
+        q = Params()
+        ctypes.pointer(q)[0] = self
+        return q
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Planets(ctypes.Structure):
+    """dfargo_planets (PlanetarySystem, src/types.h:97-111)."""
+    _fields_ = [
+        ("nb", c_int),
+        ("mass", _D8), ("x", _D8), ("y", _D8), ("vx", _D8), ("vy", _D8), ("acc", _D8),
+        ("feel_disk", _I8), ("feel_others", _I8), ("binary", _I8),
+        ("initial_mass", _D8), ("final_mass", _D8),
+    ]
+
+
+class Status(ctypes.Structure):
+    """dfargo_status: the scalar globals AlgoGas leaves host-coherent."""
+    _fields_ = [
+        ("physical_time", c_double), ("physical_time_initial", c_double),
+        ("omega_frame", c_double), ("mindt", c_double), ("maxdt", c_double),
+        ("lost_mass", c_double), ("mass_taper", c_double),
+        ("disk_on_primary_x", c_double), ("disk_on_primary_y", c_double),
+        ("disk_on_primary_alt_x", c_double), ("disk_on_primary_alt_y", c_double),
+        ("bary_x", c_double * 3), ("bary_y", c_double * 3),
+        ("substeps_total", c_long), ("last_call_substeps", c_int),
+        ("gas_timesteps_cfl", c_int), ("crashed_density", c_int),
+        ("crashed_energy", c_int), ("crashed", c_int),
+    ]
+
+
+def default_params(**overrides):
+    """dfargo_default_params() restated in Python so that host-only code and
+    CPU tests need no library: the src/var.c:15-202 defaults."""
+    p = Params()
+    p.nrad, p.nsec = 64, 64
+    p.rmin, p.rmax = 1.0, 1.0
+    p.log_grid = 0
+    p.pmin, p.pmax = 0.0, 6.2831853071795864
+    p.sigma0, p.sigma_slope = 173.0, 0.0
+    p.aspect_ratio, p.flaring_index = 0.05, 0.0
+    p.viscosity, p.alpha_viscosity = 0.0, 0.0
+    p.cavity_radius, p.cavity_ratio, p.cavity_width = 0.0, 1.0, 1.0
+    p.transition_radius, p.transition_ratio, p.transition_width = 0.0, 1.0, 1.0
+    p.imposed_disk_drift = 0.0
+    p.add_noise, p.noise_seed = 0, 0
+    p.energy_equation, p.viscous_heating, p.beta_cooling = 0, 1, 0
+    p.adiabatic_index, p.beta_cooling_time, p.beta_cooling_slope = 1.4, 10.0, 0.0
+    p.self_gravity, p.sg_thickness_smoothing = 0, 0.3
+    p.thickness_smoothing, p.roche_smoothing = 0.3, 0.0
+    p.indirect_term_smoothing, p.mass_taper = 0.0, 0.0000001
+    p.indirect_term, p.discard_gas_indirect_term = 1, 0
+    p.alternative_gas_indirect_term = 0
+    p.exclude_hill, p.exclude_hill_factor = 0, 1.0
+    p.forced_circular = 0
+    p.release_date, p.release_radius = 0.0, 0.0
+    p.eccentricity = 0.0
+    p.corotating, p.omega_frame = 0, 0.0
+    p.fast_transport, p.add_floors, p.density_floor = 1, 1, 1e-9
+    p.boundary = BC_WALL
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(f"unknown parameter {k!r}")
+        setattr(p, k, v)
+    return p
+
+
+_lib = None
+
+
+def load(path=None):
+    """Loads libdfargo.so.  Raises if it is missing: there is no fallback."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is missing: build it with __graft_entry__.build() "
+            "(the hydro step has no CPU fallback)")
+    lib = ctypes.CDLL(p)
+    P, PP = ctypes.POINTER(Params), ctypes.POINTER(Planets)
+    dp, vp = ctypes.POINTER(c_double), ctypes.c_void_p
+    sig = {
+        "dfargo_default_params": (None, [P]),
+        "dfargo_check_params": (c_int, [P]),
+        "dfargo_fill_polar_1d_arrays": (c_int, [P, dp, dp, dp, dp]),
+        "dfargo_init_gas_density": (c_int, [P, dp]),
+        "dfargo_init_gas_energy": (c_int, [P, dp]),
+        "dfargo_init_planetary_system": (c_int, [P, c_int, dp, dp, dp,
+                                                 ctypes.POINTER(c_int), ctypes.POINTER(c_int), PP]),
+        "dfargo_aspect_ratio": (c_double, [P, c_double]),
+        "dfargo_create": (c_int, [P, c_int, c_int, ctypes.POINTER(vp)]),
+        "dfargo_destroy": (c_int, [vp]),
+        "dfargo_last_error": (ctypes.c_char_p, []),
+        "dfargo_init_euler": (c_int, [vp, dp, dp, dp, dp, PP]),
+        "dfargo_algogas": (c_int, [vp, c_double, dp, dp, dp, dp, PP]),
+        "dfargo_algogas_device": (c_int, [vp, c_double]),
+        "dfargo_upload_state": (c_int, [vp, dp, dp, dp, dp, PP]),
+        "dfargo_download_state": (c_int, [vp, dp, dp, dp, dp, PP]),
+        "dfargo_get_status": (c_int, [vp, ctypes.POINTER(Status)]),
+        "dfargo_set_time": (c_int, [vp, c_double, c_double, c_double]),
+        "dfargo_get_field": (c_int, [vp, c_int, dp]),
+        "dfargo_compute_selfgravity": (c_int, [vp]),
+        "dfargo_fviscosity": (c_double, [vp, c_double]),
+        "dfargo_get_fargo_shift": (c_int, [vp, ctypes.POINTER(c_long), dp]),
+        "dfargo_set_profiling": (c_int, [vp, c_int]),
+        "dfargo_get_kernel_times": (c_int, [vp, c_int, ctypes.c_void_p, dp,
+                                            ctypes.POINTER(c_long), ctypes.POINTER(c_int)]),
+        "dfargo_stream": (vp, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+EXPORTED = [
+    "dfargo_default_params", "dfargo_check_params", "dfargo_fill_polar_1d_arrays",
+    "dfargo_init_gas_density", "dfargo_init_gas_energy", "dfargo_init_planetary_system",
+    "dfargo_aspect_ratio", "dfargo_create", "dfargo_destroy", "dfargo_last_error",
+    "dfargo_init_euler", "dfargo_algogas", "dfargo_algogas_device", "dfargo_upload_state",
+    "dfargo_download_state", "dfargo_get_status", "dfargo_set_time", "dfargo_get_field",
+    "dfargo_compute_selfgravity", "dfargo_fviscosity", "dfargo_get_fargo_shift",
+    "dfargo_set_profiling", "dfargo_get_kernel_times", "dfargo_stream",
+]

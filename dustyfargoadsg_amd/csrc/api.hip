@@ -1,0 +1,763 @@
+// api.hip -- the dfargo context and the C ABI of include/dfargo.h.
+//
+// One context holds one disc resident in HBM: the state fields in the
+// reference's ring-major layout, the work grids the sub-step needs, the 1-D
+// tables and a device control block.  A sub-step is a fixed list of kernels
+// that read dt and the loop condition from the control block, so the list
+// is captured once in a HIP graph and replayed; the host only reads the
+// control block back between batches of sub-steps.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+#define HIPCHK(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return dfargo_set_error(DFARGO_ERR_HIP, "%s failed: %s (%s:%d)", #call,               \
+                              hipGetErrorString(e_), __FILE__, __LINE__);                   \
+  } while (0)
+
+struct dfargo_ctx {
+  dfargo_params p;
+  int prec = DFARGO_FP64, dev = 0;
+  hipStream_t stream = nullptr;
+  dfargo_tables tab;
+  size_t esz = 8, nfield = 0; // element size, elements per field
+  std::vector<void *> allocs;
+  KArgs a;
+  dfargo_ctrl *d_ctrl = nullptr, *h_ctrl = nullptr;
+  std::vector<int> seq; // kernel ids of one sub-step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int initialized = 0, profiling = 0;
+  double kt_ms[KID_COUNT];
+  long kt_n[KID_COUNT];
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<double> VthetaMed, gcs_host;
+  std::vector<float> fstage;
+};
+
+// ------------------------------------------------------------------------
+
+static int dalloc(dfargo_ctx *c, void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return dfargo_set_error(DFARGO_ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  c->allocs.push_back(*p);
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) return dfargo_set_error(DFARGO_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+  return DFARGO_OK;
+}
+
+static int upload_table(dfargo_ctx *c, const double **dst, const double *src, size_t n) {
+  void *p;
+  int rc = dalloc(c, &p, sizeof(double) * n);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(p, src, sizeof(double) * n, hipMemcpyHostToDevice));
+  *dst = (const double *)p;
+  return DFARGO_OK;
+}
+
+// host double array -> device field of the context's precision
+static int put_field(dfargo_ctx *c, void *dst, const double *src, size_t n) {
+  if (c->prec == DFARGO_FP64) {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    c->fstage.resize(n);
+    for (size_t i = 0; i < n; i++) c->fstage[i] = (float)src[i];
+    HIPCHK(hipMemcpyAsync(dst, c->fstage.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return DFARGO_OK;
+}
+static int get_field_raw(dfargo_ctx *c, double *dst, const void *src, size_t n) {
+  if (c->prec == DFARGO_FP64) {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    c->fstage.resize(n);
+    HIPCHK(hipMemcpyAsync(c->fstage.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) dst[i] = (double)c->fstage[i];
+  }
+  return DFARGO_OK;
+}
+
+static int ctrl_push(dfargo_ctx *c) {
+  HIPCHK(hipMemcpyAsync(c->d_ctrl, c->h_ctrl, sizeof(dfargo_ctrl), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return DFARGO_OK;
+}
+static int ctrl_pull(dfargo_ctx *c) {
+  HIPCHK(hipMemcpyAsync(c->h_ctrl, c->d_ctrl, sizeof(dfargo_ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return DFARGO_OK;
+}
+
+static void planets_to_ctrl(dfargo_ctrl *k, const dfargo_planets *s) {
+  k->nb = s->nb;
+  for (int i = 0; i < DF_MAXPL; i++) {
+    k->pl_m[i] = s->mass[i]; k->pl_x[i] = s->x[i]; k->pl_y[i] = s->y[i];
+    k->pl_vx[i] = s->vx[i]; k->pl_vy[i] = s->vy[i]; k->pl_acc[i] = s->acc[i];
+    k->pl_m0[i] = s->initial_mass[i]; k->pl_m1[i] = s->final_mass[i];
+    k->pl_feel_disk[i] = s->feel_disk[i]; k->pl_feel_others[i] = s->feel_others[i];
+    k->pl_binary[i] = s->binary[i];
+  }
+}
+static void ctrl_to_planets(const dfargo_ctrl *k, dfargo_planets *s) {
+  s->nb = k->nb;
+  for (int i = 0; i < DF_MAXPL; i++) {
+    s->mass[i] = k->pl_m[i]; s->x[i] = k->pl_x[i]; s->y[i] = k->pl_y[i];
+    s->vx[i] = k->pl_vx[i]; s->vy[i] = k->pl_vy[i]; s->acc[i] = k->pl_acc[i];
+    s->initial_mass[i] = k->pl_m0[i]; s->final_mass[i] = k->pl_m1[i];
+    s->feel_disk[i] = k->pl_feel_disk[i]; s->feel_others[i] = k->pl_feel_others[i];
+    s->binary[i] = k->pl_binary[i];
+  }
+}
+
+template <typename T> static void launch_one(dfargo_ctx *c, int id) { launch_kernel<T>(id, c->a, c->stream); }
+
+static void launch_id(dfargo_ctx *c, int id) {
+  if (c->prec == DFARGO_FP64) launch_one<double>(c, id);
+  else launch_one<float>(c, id);
+}
+
+static int run_substep(dfargo_ctx *c) {
+  if (c->profiling) {
+    for (int id : c->seq) {
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      launch_id(c, id);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->kt_ms[id] += ms;
+      c->kt_n[id] += 1;
+    }
+    return DFARGO_OK;
+  }
+  if (!c->gexec) {
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int id : c->seq) launch_id(c, id);
+    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    if (e != hipSuccess) return dfargo_set_error(DFARGO_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  return DFARGO_OK;
+}
+
+// ------------------------------------------------------------------------
+// creation
+
+static int build_context(dfargo_ctx *c) {
+  const dfargo_params &p = c->p;
+  const int nr = p.nrad, ns = p.nsec;
+  int rc = dfargo_host_tables(&p, &c->tab);
+  if (rc) return rc;
+  dfargo_tables &t = c->tab;
+  KArgs &a = c->a;
+  memset(&a, 0, sizeof(a));
+  a.nr = nr;
+  a.ns = ns;
+  a.energy = p.energy_equation;
+  a.visc_on = (p.alpha_viscosity != 0.0) || (p.viscosity != 0.0);
+  a.alpha_on = p.alpha_viscosity != 0.0;
+  a.fast = p.fast_transport;
+  a.floors = p.add_floors;
+  a.bc = p.boundary;
+  a.sg = p.self_gravity;
+  a.beta_cooling = p.beta_cooling;
+  a.viscous_heating = p.viscous_heating;
+  a.gas_indirect = (!p.discard_gas_indirect_term) && (fabs(p.pmax - p.pmin - 2. * M_PI) < 0.01);
+  a.indirect_term = p.indirect_term;
+  a.alt_indirect = p.alternative_gas_indirect_term;
+  a.exclude_hill = p.exclude_hill;
+  a.roche = p.roche_smoothing != 0.0;
+  a.forced_circular = p.forced_circular;
+  a.xcd_remap = 1;
+  a.band = 8;
+  a.pm = p.pmax - p.pmin;
+  a.dphi = a.pm / (double)ns;
+  a.invdphi = 1.0 / a.dphi;
+  a.gamma = p.adiabatic_index;
+  a.alpha = p.alpha_viscosity;
+  a.visc_const = p.viscosity;
+  a.floor_dens = p.density_floor;
+  a.sg_eps = t.sg_eps;
+  a.sg_rstep = t.sg_rstep;
+  a.sg_tstep = t.sg_tstep;
+  a.vh_lr1 = t.vh_lr1;
+  a.vh_lr2 = t.vh_lr2;
+  a.mass_taper_param = p.mass_taper;
+  a.thickness_smoothing = p.thickness_smoothing;
+  a.roche_smoothing = p.roche_smoothing;
+  a.indirect_smoothing = p.indirect_term_smoothing;
+  a.exclude_hill_factor = p.exclude_hill_factor;
+  a.release_date = p.release_date;
+  a.release_radius = p.release_radius;
+  a.aspect_ratio = p.aspect_ratio;
+  a.flaring = p.flaring_index;
+  a.transition_radius = p.transition_radius;
+  a.transition_ratio = p.transition_ratio;
+  a.transition_width = p.transition_width;
+  // tables
+  size_t n1 = (size_t)nr + 2;
+  if ((rc = upload_table(c, &a.Rinf, t.Rinf, n1)) || (rc = upload_table(c, &a.Rsup, t.Rsup, n1)) ||
+      (rc = upload_table(c, &a.Rmed, t.Rmed, n1)) || (rc = upload_table(c, &a.Surf, t.Surf, n1)) ||
+      (rc = upload_table(c, &a.InvRmed, t.InvRmed, n1)) || (rc = upload_table(c, &a.InvSurf, t.InvSurf, n1)) ||
+      (rc = upload_table(c, &a.InvDiffRmed, t.InvDiffRmed, n1)) ||
+      (rc = upload_table(c, &a.InvDiffRsup, t.InvDiffRsup, n1)) ||
+      (rc = upload_table(c, &a.InvRinf, t.InvRinf, n1)) || (rc = upload_table(c, &a.CosAz, t.CosAz, ns)) ||
+      (rc = upload_table(c, &a.SinAz, t.SinAz, ns)) || (rc = upload_table(c, &a.SigmaMed, t.SigmaMed, n1)) ||
+      (rc = upload_table(c, &a.EnergyMed, t.EnergyMed, n1)) || (rc = upload_table(c, &a.cs_iso, t.cs_iso, n1)) ||
+      (rc = upload_table(c, &a.powmin2, t.powmin2, n1)) || (rc = upload_table(c, &a.r15, t.r15, n1)) ||
+      (rc = upload_table(c, &a.visc_mult, t.visc_mult, n1)) || (rc = upload_table(c, &a.beta, t.beta, n1)) ||
+      (rc = upload_table(c, &a.supp_torque, t.supp_torque, n1)) ||
+      (rc = upload_table(c, &a.sg_sqrt, t.sg_sqrt, n1)) || (rc = upload_table(c, &a.sg_normr, t.sg_normr, n1)) ||
+      (rc = upload_table(c, &a.sg_normt, t.sg_normt, n1)))
+    return rc;
+  {
+    double o1 = sqrt(t.Rmed[2] / t.Rmed[1]), o2 = sqrt(t.Rmed[2] / t.Rmed[0]);
+    double o3 = pow(t.Rmed[nr - 1] / t.Rmed[nr - 2], -0.5);
+    if ((rc = upload_table(c, &a.open_vt_in1, &o1, 1)) || (rc = upload_table(c, &a.open_vt_in2, &o2, 1)) ||
+        (rc = upload_table(c, &a.open_vt_out, &o3, 1)))
+      return rc;
+  }
+  c->VthetaMed.assign(n1, 0.0);
+  if ((rc = upload_table(c, &a.VthetaMed, c->VthetaMed.data(), n1))) return rc;
+  // fields: (nr+2) rings so that ring nr (vr outer face) exists with slack
+  c->esz = (c->prec == DFARGO_FP64) ? sizeof(double) : sizeof(float);
+  c->nfield = (size_t)(nr + 2) * ns;
+  void **fields[] = {&a.rho, &a.vr, &a.vt, &a.en, &a.cs, &a.vri, &a.vti, &a.trr, &a.trp, &a.tpp,
+                     &a.divv, &a.vh, &a.t_rho, &a.t_rp, &a.t_rm, &a.t_tp, &a.t_tm, &a.t_en, &a.pot};
+  for (void **f : fields)
+    if ((rc = dalloc(c, f, c->esz * c->nfield))) return rc;
+  void **rings[] = {(void **)&a.vmoy, (void **)&a.ring_dt, (void **)&a.nu_ring, (void **)&a.gcs,
+                    (void **)&a.vmed, (void **)&a.lost_col};
+  for (void **r : rings)
+    if ((rc = dalloc(c, r, sizeof(double) * (size_t)(std::max(nr, ns) + 2)))) return rc;
+  if ((rc = dalloc(c, (void **)&a.ring_part, sizeof(double) * (size_t)(nr + 2) * DF_NPART(DF_MAXTG))) ||
+      (rc = dalloc(c, (void **)&a.nshift, sizeof(long) * (size_t)(nr + 2))) ||
+      (rc = dalloc(c, (void **)&a.ring_crash, sizeof(int) * 2 * (size_t)(nr + 2))) ||
+      (rc = dalloc(c, (void **)&c->d_ctrl, sizeof(dfargo_ctrl))))
+    return rc;
+  a.ctrl = c->d_ctrl;
+  // self-gravity: twiddles and kernel spectra
+  if (p.self_gravity) {
+    const int L2 = 2 * nr;
+    const size_t csz = 2 * c->esz;
+    if ((rc = dalloc(c, &a.sgr, c->esz * c->nfield)) || (rc = dalloc(c, &a.sgt, c->esz * c->nfield)) ||
+        (rc = dalloc(c, &a.sg_z, csz * (size_t)nr * ns)) || (rc = dalloc(c, &a.sg_w, csz * (size_t)L2 * ns)) ||
+        (rc = dalloc(c, &a.sg_kr, csz * (size_t)L2 * ns)) || (rc = dalloc(c, &a.sg_kt, csz * (size_t)L2 * ns)) ||
+        (rc = dalloc(c, &a.tw_ns, csz * ns)) || (rc = dalloc(c, &a.tw_2n, csz * L2)))
+      return rc;
+    // twiddles exp(-2 pi i m / L), FP64 host values
+    std::vector<double> tw(2 * (size_t)std::max(ns, L2));
+    auto put_tw = [&](void *dst, int L) -> int {
+      for (int m = 0; m < L; m++) {
+        tw[2 * m] = cos(2.0 * M_PI * (double)m / (double)L);
+        tw[2 * m + 1] = -sin(2.0 * M_PI * (double)m / (double)L);
+      }
+      return put_field(c, dst, tw.data(), 2 * (size_t)L);
+    };
+    if ((rc = put_tw(a.tw_ns, ns)) || (rc = put_tw(a.tw_2n, L2))) return rc;
+    // kernels (src/sgkernel.c:3-27) in FP64 on the host, spectra on the GPU
+    const double eps = t.sg_eps;
+    std::vector<double> kr(2 * (size_t)L2 * ns, 0.0), kt(2 * (size_t)L2 * ns, 0.0);
+    for (int i = 0; i < L2; i++) {
+      double u = (i < nr) ? log(t.Radii[i] / t.Radii[0]) : -log(t.Radii[2 * nr - i] / t.Radii[0]);
+      for (int j = 0; j < ns; j++) {
+        size_t l = (size_t)i * ns + j;
+        double v = 1.0 + eps * eps - t.CosAz[j] * exp(-u);
+        v *= pow(eps * eps * exp(u) + 2.0 * (cosh(u) - t.CosAz[j]), -1.5);
+        kr[2 * l] = v;
+        double w = t.SinAz[j];
+        w *= pow(eps * eps * exp(u) + 2.0 * (cosh(u) - t.CosAz[j]), -1.5);
+        kt[2 * l] = w;
+      }
+    }
+    void *tmp = nullptr, *twd_ns = nullptr, *twd_2n = nullptr;
+    HIPCHK(hipMalloc(&tmp, sizeof(double) * 2 * (size_t)L2 * ns));
+    HIPCHK(hipMalloc(&twd_ns, sizeof(double) * 2 * ns));
+    HIPCHK(hipMalloc(&twd_2n, sizeof(double) * 2 * L2));
+    for (int m = 0; m < ns; m++) {
+      tw[2 * m] = cos(2.0 * M_PI * (double)m / (double)ns);
+      tw[2 * m + 1] = -sin(2.0 * M_PI * (double)m / (double)ns);
+    }
+    HIPCHK(hipMemcpy(twd_ns, tw.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice));
+    for (int m = 0; m < L2; m++) {
+      tw[2 * m] = cos(2.0 * M_PI * (double)m / (double)L2);
+      tw[2 * m + 1] = -sin(2.0 * M_PI * (double)m / (double)L2);
+    }
+    HIPCHK(hipMemcpy(twd_2n, tw.data(), sizeof(double) * 2 * L2, hipMemcpyHostToDevice));
+    void *dsts[2] = {a.sg_kr, a.sg_kt};
+    std::vector<double> *srcs[2] = {&kr, &kt};
+    for (int q = 0; q < 2; q++) {
+      HIPCHK(hipMemcpy(tmp, srcs[q]->data(), sizeof(double) * 2 * (size_t)L2 * ns, hipMemcpyHostToDevice));
+      launch_fft_rows<double>(tmp, L2, ns, twd_ns, -1, c->stream);
+      launch_fft_cols<double>(tmp, ns, L2, twd_2n, -1, c->stream);
+      HIPCHK(hipGetLastError());
+      if (c->prec == DFARGO_FP64)
+        HIPCHK(hipMemcpyAsync(dsts[q], tmp, sizeof(double) * 2 * (size_t)L2 * ns, hipMemcpyDeviceToDevice, c->stream));
+      else
+        launch_convert_d2f((const double *)tmp, (float *)dsts[q], 2 * (size_t)L2 * ns, c->stream);
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(hipFree(tmp));
+    HIPCHK(hipFree(twd_ns));
+    HIPCHK(hipFree(twd_2n));
+  }
+  // control block
+  HIPCHK(hipHostMalloc((void **)&c->h_ctrl, sizeof(dfargo_ctrl), hipHostMallocDefault));
+  memset(c->h_ctrl, 0, sizeof(dfargo_ctrl));
+  c->h_ctrl->omega_frame = p.omega_frame;
+  c->h_ctrl->mindt = 100.0; /* src/var.c:161-162 defaults */
+  c->h_ctrl->maxdt = 0.00001;
+  c->h_ctrl->active = 0;
+  if ((rc = ctrl_push(c))) return rc;
+  // the kernels of one sub-step, in the order of src/SourceEuler.c:398-716
+  c->seq = {KID_BEGIN, KID_CFL, KID_CONTROL};
+  if (p.energy_equation) c->seq.push_back(KID_EOS);
+  c->seq.push_back(KID_SUBSTEP1);
+  if (p.self_gravity)
+    for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS_FWD, KID_SG_PRODUCT, KID_SG_COLS_INV, KID_SG_ROWS_INV, KID_SG_KICK})
+      c->seq.push_back(id);
+  if (a.visc_on) {
+    c->seq.push_back(KID_VISC_STRESS);
+    c->seq.push_back(KID_VISC_UPDATE);
+  }
+  c->seq.push_back(KID_SUBSTEP2);
+  if (p.energy_equation) c->seq.push_back(KID_ENERGY);
+  for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_VELOCITIES, KID_BOUNDARY}) c->seq.push_back(id);
+  HIPCHK(hipEventCreate(&c->ev0));
+  HIPCHK(hipEventCreate(&c->ev1));
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_create(const dfargo_params *p, int precision, int device, dfargo_ctx **out) {
+  if (!out) return dfargo_set_error(DFARGO_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  int rc = dfargo_check_params(p);
+  if (rc) return rc;
+  if (precision != DFARGO_FP64 && precision != DFARGO_FP32)
+    return dfargo_set_error(DFARGO_ERR_ARG, "precision must be DFARGO_FP64 or DFARGO_FP32");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return dfargo_set_error(DFARGO_ERR_HIP, "no HIP device is visible");
+  if (device < 0 || device >= ndev) return dfargo_set_error(DFARGO_ERR_ARG, "device %d out of range", device);
+  if (ring_kcheck(p->nsec) < 0)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "NSEC=%d cannot be split into <=1024-thread rings", p->nsec);
+  HIPCHK(hipSetDevice(device));
+  dfargo_ctx *c = new dfargo_ctx();
+  c->p = *p;
+  c->prec = precision;
+  c->dev = device;
+  memset(c->kt_ms, 0, sizeof c->kt_ms);
+  memset(c->kt_n, 0, sizeof c->kt_n);
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return dfargo_set_error(DFARGO_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+  }
+  rc = build_context(c);
+  if (rc) {
+    dfargo_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_destroy(dfargo_ctx *c) {
+  if (!c) return DFARGO_OK;
+  hipSetDevice(c->dev);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  for (void *p : c->allocs) hipFree(p);
+  if (c->h_ctrl) hipHostFree(c->h_ctrl);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  dfargo_free_tables(&c->tab);
+  delete c;
+  return DFARGO_OK;
+}
+
+// ------------------------------------------------------------------------
+// InitEuler
+
+extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad, double *vtheta,
+                                 const double *energy, const dfargo_planets *sys_in) {
+  if (!c || !rho || !vrad || !vtheta) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (c->p.energy_equation && !energy)
+    return dfargo_set_error(DFARGO_ERR_ARG, "the energy equation needs an energy array");
+  HIPCHK(hipSetDevice(c->dev));
+  const dfargo_params &p = c->p;
+  const dfargo_tables &t = c->tab;
+  const int nr = p.nrad, ns = p.nsec;
+  const size_t n = (size_t)(nr + 1) * ns;
+  KArgs &a = c->a;
+  int rc;
+  dfargo_planets sys;
+  if (sys_in) sys = *sys_in;
+  else memset(&sys, 0, sizeof sys);
+  std::vector<double> zero(n, 0.0);
+  if ((rc = put_field(c, a.rho, rho, n)) || (rc = put_field(c, a.vr, zero.data(), n)) ||
+      (rc = put_field(c, a.vt, zero.data(), n)))
+    return rc;
+  if (p.energy_equation && (rc = put_field(c, a.en, energy, n))) return rc;
+  // start-up self-gravity (src/main.c:178-186): SG_Accr of the initial
+  // density and its ring mean for the planets and InitGasVelocities
+  std::vector<double> axisg(nr, 0.0);
+  if (p.self_gravity) {
+    if (c->prec == DFARGO_FP64) launch_sg_accel<double>(a, c->stream);
+    else launch_sg_accel<float>(a, c->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> accr(n);
+    if ((rc = get_field_raw(c, accr.data(), a.sgr, (size_t)nr * ns))) return rc;
+    for (int i = 0; i < nr; i++) {
+      double s = 0.;
+      for (int j = 0; j < ns; j++) s += accr[(size_t)i * ns + j];
+      axisg[i] = s / (double)ns;
+    }
+    // init_planetarysys_withSG (src/sgsysinit.c:3-49), non-binary
+    for (int k = 0; k < sys.nb; k++) {
+      double x = sys.x[k], y = sys.y[k], r = sqrt(x * x + y * y), sgacc;
+      double dist = (double)(r / (1. + p.eccentricity));
+      if (dist < t.Rmed[nr - 1]) {
+        int ipl = 0;
+        while ((t.Rmed[ipl] <= dist) && (ipl < nr - 2)) ipl++;
+        double ri = t.Rmed[ipl], rip1 = t.Rmed[ipl + 1], dr = rip1 - ri;
+        sgacc = (dist - ri) * axisg[ipl + 1] + (rip1 - dist) * axisg[ipl];
+        sgacc /= dr;
+      } else
+        sgacc = axisg[nr - 1];
+      sys.vy[k] *= (double)sqrt(1. - dist * dist * sgacc / 1.0 / (1.0 + sys.mass[k]));
+    }
+  }
+  // ComputeSoundSpeed and the ring mean GLOBAL_SoundSpeed
+  std::vector<double> cs(n, 0.0);
+  c->gcs_host.assign(nr + 2, 0.0);
+  for (int i = 0; i < nr; i++) {
+    for (int j = 0; j < ns; j++) {
+      size_t l = (size_t)i * ns + j;
+      cs[l] = p.energy_equation
+                  ? sqrt(p.adiabatic_index * (p.adiabatic_index - 1.0) * energy[l] / rho[l])
+                  : t.cs_iso[i];
+    }
+    double s = 0.;
+    for (int j = 0; j < ns; j++) s += cs[(size_t)i * ns + j];
+    c->gcs_host[i] = s / (double)ns;
+  }
+  if (p.energy_equation && (rc = put_field(c, a.cs, cs.data(), n))) return rc;
+  std::vector<double> nu(nr + 2, 0.0);
+  for (int i = 0; i < nr; i++) {
+    double v = p.viscosity;
+    if (p.alpha_viscosity != 0.0) v = p.alpha_viscosity * c->gcs_host[i] * c->gcs_host[i] * t.r15[i];
+    nu[i] = ((p.alpha_viscosity != 0.0) || (p.viscosity != 0.0)) ? v * t.visc_mult[i] : 0.0;
+  }
+  HIPCHK(hipMemcpy(a.nu_ring, nu.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a.gcs, c->gcs_host.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
+  // InitGasVelocities (src/Pframeforce.c:442-641), no centrifugal balance
+  std::vector<double> vr(n, 0.0), vt(n, 0.0);
+  for (int i = 0; i < nr; i++) {
+    double r = t.Rmed[i], omega = sqrt(1.0 * 1.0 / r / r / r);
+    double myvtheta = omega * r * sqrt(1.0 - pow(p.aspect_ratio, 2.0) * pow(r, 2.0 * p.flaring_index) *
+                                                 (1. + p.sigma_slope - 2.0 * p.flaring_index));
+    for (int j = 0; j < ns; j++) vt[j + (size_t)i * ns] = myvtheta;
+  }
+  if (p.self_gravity)
+    for (int i = 0; i < nr; i++)
+      for (int j = 0; j < ns; j++) {
+        size_t l = (size_t)i * ns + j;
+        double vt2 = vt[l] * vt[l];
+        vt2 -= t.Rmed[i] * axisg[i];
+        vt[l] = sqrt(vt2);
+      }
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) vt[(size_t)i * ns + j] -= p.omega_frame * t.Rmed[i];
+  for (int i = 0; i < nr; i++) {
+    double r = t.Rmed[i], ri = t.Rinf[i], viscosity = nu[i];
+    for (int j = 0; j < ns; j++) {
+      size_t l = (size_t)i * ns + j;
+      if (p.imposed_disk_drift != 0.0)
+        vr[l] = p.imposed_disk_drift * p.sigma0 / t.SigmaInf[i] / ri;
+      else if (p.alpha_viscosity != 0.0)
+        vr[l] -= 3.0 * viscosity / r * (-p.sigma_slope + 2.0 * p.flaring_index + 1.0);
+      else
+        vr[l] -= 3.0 * viscosity / r * (-p.sigma_slope + .5);
+    }
+  }
+  for (int i = 0; i < nr; i++) c->VthetaMed[i] = vt[(size_t)i * ns] + t.Rmed[i] * p.omega_frame;
+  HIPCHK(hipMemcpy((void *)a.VthetaMed, c->VthetaMed.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
+  memcpy(vrad, vr.data(), sizeof(double) * n);
+  memcpy(vtheta, vt.data(), sizeof(double) * n);
+  if ((rc = put_field(c, a.vr, vr.data(), n)) || (rc = put_field(c, a.vt, vt.data(), n))) return rc;
+  // ComputeViscousTerms at start-up (stress grids for output)
+  if (c->prec == DFARGO_FP64) launch_visc_stress_state<double>(a, c->stream);
+  else launch_visc_stress_state<float>(a, c->stream);
+  HIPCHK(hipGetLastError());
+  // control block
+  dfargo_ctrl *k = c->h_ctrl;
+  planets_to_ctrl(k, &sys);
+  k->time = 0.0;
+  k->time0 = 0.0;
+  k->omega_frame = p.omega_frame;
+  if ((rc = ctrl_push(c))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->initialized = 1;
+  return DFARGO_OK;
+}
+
+// ------------------------------------------------------------------------
+// AlgoGas
+
+extern "C" int dfargo_algogas_device(dfargo_ctx *c, double DT) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
+  if (!(DT > 0.0)) return dfargo_set_error(DFARGO_ERR_ARG, "DT must be positive");
+  HIPCHK(hipSetDevice(c->dev));
+  launch_algogas_begin(c->a, DT, c->stream);
+  if (c->p.energy_equation) {
+    if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, c->stream);
+    else launch_eos_pre<float>(c->a, c->stream);
+  }
+  HIPCHK(hipGetLastError());
+  long launched = 0;
+  int batch = 1, rc;
+  const long cap = 50000000L;
+  for (;;) {
+    for (int b = 0; b < batch; b++)
+      if ((rc = run_substep(c))) return rc;
+    launched += batch;
+    if ((rc = ctrl_pull(c))) return rc;
+    const dfargo_ctrl *k = c->h_ctrl;
+    if (!(k->dtemp < 0.999999999 * DT)) break;
+    if (launched > cap)
+      return dfargo_set_error(DFARGO_ERR_STATE, "more than %ld sub-steps in one AlgoGas call", cap);
+    double rem = (DT - k->dtemp) / (k->dt > 0.0 ? k->dt : DT);
+    batch = (int)std::min(std::max(ceil(rem), 1.0), 4096.0);
+  }
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_upload_state(dfargo_ctx *c, const double *rho, const double *vrad, const double *vtheta,
+                                   const double *energy, const dfargo_planets *sys) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  const size_t n = (size_t)(c->p.nrad + 1) * c->p.nsec;
+  int rc;
+  if (rho && (rc = put_field(c, c->a.rho, rho, n))) return rc;
+  if (vrad && (rc = put_field(c, c->a.vr, vrad, n))) return rc;
+  if (vtheta && (rc = put_field(c, c->a.vt, vtheta, n))) return rc;
+  if (energy && c->p.energy_equation && (rc = put_field(c, c->a.en, energy, n))) return rc;
+  if (sys) {
+    if ((rc = ctrl_pull(c))) return rc;
+    planets_to_ctrl(c->h_ctrl, sys);
+    if ((rc = ctrl_push(c))) return rc;
+  }
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_download_state(dfargo_ctx *c, double *rho, double *vrad, double *vtheta, double *energy,
+                                     dfargo_planets *sys) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  const size_t n = (size_t)(c->p.nrad + 1) * c->p.nsec;
+  int rc;
+  if (rho && (rc = get_field_raw(c, rho, c->a.rho, n))) return rc;
+  if (vrad && (rc = get_field_raw(c, vrad, c->a.vr, n))) return rc;
+  if (vtheta && (rc = get_field_raw(c, vtheta, c->a.vt, n))) return rc;
+  if (energy) {
+    if (c->p.energy_equation) {
+      if ((rc = get_field_raw(c, energy, c->a.en, n))) return rc;
+    } else
+      memset(energy, 0, sizeof(double) * n);
+  }
+  if (sys) {
+    if ((rc = ctrl_pull(c))) return rc;
+    ctrl_to_planets(c->h_ctrl, sys);
+  }
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_algogas(dfargo_ctx *c, double DT, double *rho, double *vrad, double *vtheta,
+                              double *energy, dfargo_planets *sys) {
+  int rc = dfargo_upload_state(c, rho, vrad, vtheta, energy, sys);
+  if (rc) return rc;
+  if ((rc = dfargo_algogas_device(c, DT))) return rc;
+  return dfargo_download_state(c, rho, vrad, vtheta, energy, sys);
+}
+
+// ------------------------------------------------------------------------
+// queries
+
+extern "C" int dfargo_get_status(dfargo_ctx *c, dfargo_status *st) {
+  if (!c || !st) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  int rc = ctrl_pull(c);
+  if (rc) return rc;
+  const dfargo_ctrl *k = c->h_ctrl;
+  memset(st, 0, sizeof(*st));
+  st->physical_time = k->time;
+  st->physical_time_initial = k->time0;
+  st->omega_frame = k->omega_frame;
+  st->mindt = k->mindt;
+  st->maxdt = k->maxdt;
+  st->lost_mass = k->lost_mass;
+  st->mass_taper = k->mass_taper;
+  st->disk_on_primary_x = k->dop_x;
+  st->disk_on_primary_y = k->dop_y;
+  st->disk_on_primary_alt_x = k->dopalt_x;
+  st->disk_on_primary_alt_y = k->dopalt_y;
+  for (int q = 0; q < 3; q++) {
+    st->bary_x[q] = k->barx[q];
+    st->bary_y[q] = k->bary[q];
+  }
+  st->substeps_total = k->substeps_total;
+  st->last_call_substeps = k->substeps;
+  st->gas_timesteps_cfl = k->gas_timesteps_cfl;
+  st->crashed_density = k->crashed_d;
+  st->crashed_energy = k->crashed_e;
+  st->crashed = k->crashed_e;
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_set_time(dfargo_ctx *c, double t, double t0, double omega_frame) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  int rc = ctrl_pull(c);
+  if (rc) return rc;
+  c->h_ctrl->time = t;
+  c->h_ctrl->time0 = t0;
+  c->h_ctrl->omega_frame = omega_frame;
+  return ctrl_push(c);
+}
+
+extern "C" int dfargo_get_field(dfargo_ctx *c, int field, double *out) {
+  if (!c || !out) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  const int nr = c->p.nrad, ns = c->p.nsec;
+  const size_t n = (size_t)(nr + 1) * ns;
+  KArgs &a = c->a;
+  int rc;
+  memset(out, 0, sizeof(double) * n);
+  switch (field) {
+    case DFARGO_FIELD_POTENTIAL:
+      if (c->prec == DFARGO_FP64) launch_potential<double>(a, c->stream);
+      else launch_potential<float>(a, c->stream);
+      HIPCHK(hipGetLastError());
+      return get_field_raw(c, out, a.pot, (size_t)nr * ns);
+    case DFARGO_FIELD_SOUNDSPEED:
+      if (c->p.energy_equation) return get_field_raw(c, out, a.cs, (size_t)nr * ns);
+      for (int i = 0; i < nr; i++)
+        for (int j = 0; j < ns; j++) out[(size_t)i * ns + j] = c->tab.cs_iso[i];
+      return DFARGO_OK;
+    case DFARGO_FIELD_PRESSURE:
+    case DFARGO_FIELD_TEMPERATURE: {
+      // from the current state (ComputePressureField / ComputeTemperatureField)
+      std::vector<double> rho(n), e(n, 0.0);
+      if ((rc = get_field_raw(c, rho.data(), a.rho, n))) return rc;
+      if (c->p.energy_equation && (rc = get_field_raw(c, e.data(), a.en, n))) return rc;
+      for (int i = 0; i < nr; i++)
+        for (int j = 0; j < ns; j++) {
+          size_t l = (size_t)i * ns + j;
+          double pr = c->p.energy_equation ? (c->p.adiabatic_index - 1.0) * e[l]
+                                           : rho[l] * c->tab.cs_iso[i] * c->tab.cs_iso[i];
+          out[l] = (field == DFARGO_FIELD_PRESSURE) ? pr : 1.0 / 1.0 * pr / rho[l];
+        }
+      return DFARGO_OK;
+    }
+    case DFARGO_FIELD_DIVV: return get_field_raw(c, out, a.divv, (size_t)nr * ns);
+    case DFARGO_FIELD_VISCHEAT: return get_field_raw(c, out, a.vh, (size_t)nr * ns);
+    case DFARGO_FIELD_SG_ACCR:
+      if (!c->p.self_gravity) return DFARGO_OK;
+      return get_field_raw(c, out, a.sgr, (size_t)nr * ns);
+    case DFARGO_FIELD_SG_ACCT:
+      if (!c->p.self_gravity) return DFARGO_OK;
+      return get_field_raw(c, out, a.sgt, (size_t)nr * ns);
+    case DFARGO_FIELD_TAURR: return get_field_raw(c, out, a.trr, (size_t)nr * ns);
+    case DFARGO_FIELD_TAURP: return get_field_raw(c, out, a.trp, (size_t)nr * ns);
+    case DFARGO_FIELD_TAUPP: return get_field_raw(c, out, a.tpp, (size_t)nr * ns);
+  }
+  return dfargo_set_error(DFARGO_ERR_ARG, "unknown field %d", field);
+}
+
+extern "C" int dfargo_compute_selfgravity(dfargo_ctx *c) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  if (!c->p.self_gravity) return dfargo_set_error(DFARGO_ERR_STATE, "self-gravity is off for this context");
+  HIPCHK(hipSetDevice(c->dev));
+  if (c->prec == DFARGO_FP64) launch_sg_accel<double>(c->a, c->stream);
+  else launch_sg_accel<float>(c->a, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return DFARGO_OK;
+}
+
+extern "C" double dfargo_fviscosity(dfargo_ctx *c, double rad) {
+  if (!c) return 0.0;
+  const dfargo_params &p = c->p;
+  double viscosity = p.viscosity;
+  if (p.alpha_viscosity != 0.0) {
+    std::vector<double> g(p.nrad + 2, 0.0);
+    hipSetDevice(c->dev);
+    if (hipMemcpy(g.data(), c->a.gcs, sizeof(double) * (p.nrad + 2), hipMemcpyDeviceToHost) != hipSuccess)
+      return 0.0;
+    int i = 0;
+    while (i < p.nrad - 1 && c->tab.Rmed[i] < rad) i++;
+    viscosity = p.alpha_viscosity * g[i] * g[i] * pow(rad, 1.5);
+  }
+  double rmin = p.cavity_radius - p.cavity_width * p.aspect_ratio * pow(p.cavity_radius, 1.0 + p.flaring_index);
+  double rmax = p.cavity_radius + p.cavity_width * p.aspect_ratio * pow(p.cavity_radius, 1.0 + p.flaring_index);
+  if (rad < rmin) viscosity *= p.cavity_ratio;
+  if ((rad >= rmin) && (rad <= rmax)) viscosity *= exp((rmax - rad) / (rmax - rmin) * log(p.cavity_ratio));
+  return viscosity;
+}
+
+extern "C" int dfargo_get_fargo_shift(dfargo_ctx *c, long *nshift, double *vmed) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  if (nshift) HIPCHK(hipMemcpy(nshift, c->a.nshift, sizeof(long) * c->p.nrad, hipMemcpyDeviceToHost));
+  if (vmed) HIPCHK(hipMemcpy(vmed, c->a.vmed, sizeof(double) * c->p.nrad, hipMemcpyDeviceToHost));
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_set_profiling(dfargo_ctx *c, int enable) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  c->profiling = enable ? 1 : 0;
+  memset(c->kt_ms, 0, sizeof c->kt_ms);
+  memset(c->kt_n, 0, sizeof c->kt_n);
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_get_kernel_times(dfargo_ctx *c, int max, char (*names)[48], double *total_ms, long *launches,
+                                       int *count) {
+  if (!c || !count) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  int n = 0;
+  for (int id = 0; id < KID_COUNT && n < max; id++) {
+    if (c->kt_n[id] == 0) continue;
+    if (names) {
+      strncpy(names[n], kKernelNames[id], 47);
+      names[n][47] = 0;
+    }
+    if (total_ms) total_ms[n] = c->kt_ms[id];
+    if (launches) launches[n] = c->kt_n[id];
+    n++;
+  }
+  *count = n;
+  return DFARGO_OK;
+}
+
+extern "C" void *dfargo_stream(dfargo_ctx *c) { return c ? (void *)c->stream : nullptr; }

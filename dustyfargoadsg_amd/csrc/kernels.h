@@ -1,0 +1,68 @@
+// kernels.h -- kernel argument block and launcher declarations shared by the
+// HIP translation units of libdfargo.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dfargo_internal.h"
+
+#define DF_G 1.0
+#define DF_PI 3.14159265358979323846 /* M_PI of the host libm headers */
+#define DF_CFLSECURITY 0.5           /* src/SourceEuler.c:10 */
+#define DF_CVNR 1.41                 /* src/SourceEuler.c:13 */
+
+// Everything a kernel needs, passed by value.  Field pointers are untyped:
+// they hold T = double or float depending on the context's precision.
+struct KArgs {
+  int nr, ns;
+  // switches decoded from dfargo_params
+  int energy, visc_on, alpha_on, fast, floors, bc, sg, beta_cooling, viscous_heating;
+  int gas_indirect, indirect_term, alt_indirect, exclude_hill, roche, forced_circular;
+  int xcd_remap, band;
+  // constants
+  double pm, dphi, invdphi, gamma, alpha, visc_const, floor_dens;
+  double sg_eps, sg_rstep, sg_tstep, vh_lr1, vh_lr2;
+  double mass_taper_param, thickness_smoothing, roche_smoothing, indirect_smoothing;
+  double exclude_hill_factor, release_date, release_radius, aspect_ratio, flaring;
+  double transition_radius, transition_ratio, transition_width;
+  // 1-D tables (FP64, device)
+  const double *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed, *InvDiffRsup;
+  const double *InvRinf, *CosAz, *SinAz, *SigmaMed, *EnergyMed, *VthetaMed, *cs_iso;
+  const double *powmin2, *r15, *visc_mult, *beta, *supp_torque, *sg_sqrt, *sg_normr, *sg_normt;
+  const double *open_vt_in1, *open_vt_in2, *open_vt_out;
+  // fields (T)
+  void *rho, *vr, *vt, *en, *cs, *vri, *vti, *trr, *trp, *tpp, *divv, *vh;
+  void *t_rho, *t_rp, *t_rm, *t_tp, *t_tm, *t_en, *sgr, *sgt, *pot;
+  // ring buffers (FP64 unless noted)
+  double *vmoy, *ring_dt, *ring_part, *nu_ring, *gcs, *vmed, *lost_col;
+  long *nshift;
+  int *ring_crash;
+  dfargo_ctrl *ctrl;
+  // self-gravity buffers (complex T)
+  void *sg_z, *sg_w, *sg_kr, *sg_kt, *tw_ns, *tw_2n;
+};
+
+// launchers (step.hip, sg.hip); each enqueues on stream s
+enum KernelId {
+  KID_BEGIN = 0, KID_CFL, KID_CONTROL, KID_EOS, KID_SUBSTEP1, KID_SG_ROWS_FWD, KID_SG_COLS_FWD,
+  KID_SG_PRODUCT, KID_SG_COLS_INV, KID_SG_ROWS_INV, KID_SG_KICK, KID_VISC_STRESS,
+  KID_VISC_UPDATE, KID_SUBSTEP2, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
+  KID_VELOCITIES, KID_BOUNDARY, KID_COUNT
+};
+extern const char *kKernelNames[KID_COUNT];
+
+template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s);
+template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s);
+template <typename T> void launch_eos_pre(const KArgs &a, hipStream_t s);
+template <typename T> void launch_potential(const KArgs &a, hipStream_t s);
+template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s); // rows_fwd..rows_inv
+template <typename T> void launch_visc_stress_state(const KArgs &a, hipStream_t s);
+void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s);
+int ring_kcheck(int ns); // threads-per-ring split used by the ring kernels, -1 if none
+
+// FFT utilities (sg.hip)
+template <typename T>
+void launch_fft_rows(void *data, int nrows, int L, const void *tw, int dir, hipStream_t s);
+template <typename T>
+void launch_fft_cols(void *data, int ncols, int L, const void *tw, int dir, hipStream_t s);
+void launch_convert_d2f(const double *src, float *dst, size_t n, hipStream_t s);

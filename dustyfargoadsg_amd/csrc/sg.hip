@@ -1,0 +1,295 @@
+// sg.hip -- self-gravity by log-polar FFT convolution on CDNA4
+// (compute_selfgravity, src/sgmain.c:3-35; compute_fftdensity,
+// src/sgdens.c:3-86; compute_sgacc, src/sgacc.c:3-162; update_sgvelocity,
+// src/sgupdate.c:3-73).
+//
+// The (2 Nrad) x Nsec real transforms of the reference are done as one
+// complex transform of Sr + i St (the two reduced densities share it), with
+// the zero padding of rows >= Nrad applied on load.  The spectral product
+// separates the two spectra from Z(k) and conj(Z(-k)), multiplies by the
+// kernel spectra and recombines Ar + i At, so one complex inverse transform
+// returns both accelerations.  The 1-D transforms are Stockham radix-4/3/2
+// in LDS, one workgroup per row or column; twiddles come from tables the
+// host fills in FP64.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+template <typename T> struct cx {
+  T x, y;
+};
+template <typename T> __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T> __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) {
+  return {a.x + b.x, a.y + b.y};
+}
+template <typename T> __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) {
+  return {a.x - b.x, a.y - b.y};
+}
+// (s * i) * a for real s
+template <typename T> __device__ __forceinline__ cx<T> cmuli(cx<T> a, T s) {
+  return {-s * a.y, s * a.x};
+}
+
+// In-place-looking Stockham transform of length L = 2^a 3^b held in LDS
+// buffers A and B; returns the buffer holding the result.  dir = -1 is the
+// forward exp(-2 pi i jk/L), +1 the un-normalised inverse.
+template <typename T>
+__device__ cx<T> *fft_lds(cx<T> *A, cx<T> *B, int L, const cx<T> *tw, int dir) {
+  int Ns = 1, rem = L;
+  cx<T> *in = A, *out = B;
+  const T s3 = (T)(dir * 0.86602540378443864676); // dir * sqrt(3)/2
+  while (rem > 1) {
+    const int R = (rem % 3 == 0) ? 3 : ((rem % 4 == 0) ? 4 : 2);
+    const int nb = L / R, span = L / (Ns * R);
+    for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+      const int k = j % Ns;
+      cx<T> v[4];
+      for (int r = 0; r < R; r++) v[r] = in[j + r * nb];
+      for (int r = 1; r < R; r++) {
+        cx<T> w = tw[r * k * span];
+        if (dir > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+      cx<T> y[4];
+      if (R == 2) {
+        y[0] = cadd(v[0], v[1]);
+        y[1] = csub(v[0], v[1]);
+      } else if (R == 4) {
+        cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        cx<T> a2 = cadd(v[1], v[3]), a3 = cmuli(csub(v[1], v[3]), (T)dir);
+        y[0] = cadd(a0, a2);
+        y[2] = csub(a0, a2);
+        y[1] = cadd(a1, a3);
+        y[3] = csub(a1, a3);
+      } else {
+        cx<T> t1 = cadd(v[1], v[2]);
+        cx<T> t2 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
+        cx<T> t3 = cmuli(csub(v[1], v[2]), s3);
+        y[0] = cadd(v[0], t1);
+        y[1] = cadd(t2, t3);
+        y[2] = csub(t2, t3);
+      }
+      const int idxD = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; r++) out[idxD + r * Ns] = y[r];
+    }
+    __syncthreads();
+    cx<T> *tmp = in;
+    in = out;
+    out = tmp;
+    Ns *= R;
+    rem /= R;
+  }
+  return in;
+}
+
+template <typename T> __device__ __forceinline__ double ldd(const void *p, long i) {
+  return (double)((const T *)p)[i];
+}
+
+// compute_fftdensity (src/sgdens.c:54-70): Sr = Sigma sqrt(r/r0),
+// St = Sr r/r0 on the first Nrad rows, then the azimuthal transform
+template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs a, int force) {
+  extern __shared__ unsigned char smem[];
+  if (!force && !a.ctrl->active) return;
+  const int i = blockIdx.x, ns = a.ns;
+  cx<T> *A = (cx<T> *)smem, *B = A + ns;
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    double sr = ldd<T>(a.rho, (long)i * ns + j) * a.sg_sqrt[i];
+    double stv = sr * a.Rmed[i] / a.Rmed[0];
+    A[j] = {(T)sr, (T)stv};
+  }
+  __syncthreads();
+  cx<T> *r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, -1);
+  cx<T> *z = (cx<T> *)a.sg_z + (long)i * ns;
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) z[j] = r[j];
+}
+
+// radial transform of one column with the rows >= Nrad zero-padded
+template <typename T> __global__ void __launch_bounds__(256) k_sg_cols_fwd(KArgs a, int force) {
+  extern __shared__ unsigned char smem[];
+  if (!force && !a.ctrl->active) return;
+  const int k2 = blockIdx.x, ns = a.ns, nr = a.nr, L = 2 * nr;
+  cx<T> *A = (cx<T> *)smem, *B = A + L;
+  const cx<T> *z = (const cx<T> *)a.sg_z;
+  for (int i = threadIdx.x; i < L; i += blockDim.x)
+    A[i] = (i < nr) ? z[(long)i * ns + k2] : cx<T>{(T)0, (T)0};
+  __syncthreads();
+  cx<T> *r = fft_lds<T>(A, B, L, (const cx<T> *)a.tw_2n, -1);
+  cx<T> *w = (cx<T> *)a.sg_w;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) w[(long)i * ns + k2] = r[i];
+}
+
+// spectral product (src/sgacc.c:25-38) on the separated spectra
+template <typename T> __global__ void __launch_bounds__(256) k_sg_product(KArgs a, int force) {
+  if (!force && !a.ctrl->active) return;
+  const int ns = a.ns, L = 2 * a.nr;
+  const long n = (long)L * ns;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int k1 = (int)(idx / ns), k2 = (int)(idx % ns);
+  const int p1 = (L - k1) % L, p2 = (ns - k2) % ns;
+  const long pidx = (long)p1 * ns + p2;
+  if (pidx < idx) return; // the partner's thread does the pair
+  cx<T> *w = (cx<T> *)a.sg_w;
+  const cx<T> *kr = (const cx<T> *)a.sg_kr, *kt = (const cx<T> *)a.sg_kt;
+  const cx<T> z = w[idx], zp = w[pidx];
+  // S_r(k) = (Z(k) + conj Z(-k)) / 2, S_t(k) = (Z(k) - conj Z(-k)) / 2i
+  const cx<T> sr = {(T)0.5 * (z.x + zp.x), (T)0.5 * (z.y - zp.y)};
+  const cx<T> stt = {(T)0.5 * (z.y + zp.y), (T)0.5 * (zp.x - z.x)};
+  const T G = (T)DF_G;
+  {
+    cx<T> K1 = kr[idx], K2 = kt[idx];
+    cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
+    cx<T> at = {-G * (K2.x * stt.x - K2.y * stt.y), -G * (K2.x * stt.y + K2.y * stt.x)};
+    w[idx] = {ar.x - at.y, ar.y + at.x};
+  }
+  if (pidx != idx) {
+    const cx<T> srp = {sr.x, -sr.y}, stp = {stt.x, -stt.y};
+    cx<T> K1 = kr[pidx], K2 = kt[pidx];
+    cx<T> ar = {-G * (K1.x * srp.x - K1.y * srp.y), -G * (K1.x * srp.y + K1.y * srp.x)};
+    cx<T> at = {-G * (K2.x * stp.x - K2.y * stp.y), -G * (K2.x * stp.y + K2.y * stp.x)};
+    w[pidx] = {ar.x - at.y, ar.y + at.x};
+  }
+}
+
+// inverse radial transform; only rows < Nrad are kept
+template <typename T> __global__ void __launch_bounds__(256) k_sg_cols_inv(KArgs a, int force) {
+  extern __shared__ unsigned char smem[];
+  if (!force && !a.ctrl->active) return;
+  const int k2 = blockIdx.x, ns = a.ns, nr = a.nr, L = 2 * nr;
+  cx<T> *A = (cx<T> *)smem, *B = A + L;
+  cx<T> *w = (cx<T> *)a.sg_w;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) A[i] = w[(long)i * ns + k2];
+  __syncthreads();
+  cx<T> *r = fft_lds<T>(A, B, L, (const cx<T> *)a.tw_2n, +1);
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) w[(long)i * ns + k2] = r[i];
+}
+
+// inverse azimuthal transform, normalisation and self-force compensation
+// (src/sgacc.c:137-161)
+template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_inv(KArgs a, int force) {
+  extern __shared__ unsigned char smem[];
+  if (!force && !a.ctrl->active) return;
+  const int i = blockIdx.x, ns = a.ns;
+  cx<T> *A = (cx<T> *)smem, *B = A + ns;
+  const cx<T> *w = (const cx<T> *)a.sg_w + (long)i * ns;
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) A[j] = w[j];
+  __syncthreads();
+  cx<T> *r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, +1);
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    const long l = (long)i * ns + j;
+    double accr = (double)r[j].x * a.sg_normr[i];
+    double acct = (double)r[j].y * a.sg_normt[i];
+    accr += DF_G * ldd<T>(a.rho, l) * a.sg_rstep * a.sg_tstep / a.sg_eps;
+    ((T *)a.sgr)[l] = (T)accr;
+    ((T *)a.sgt)[l] = (T)acct;
+  }
+}
+
+// update_sgvelocity (src/sgupdate.c:30-72) on VradInt / VthetaInt
+template <typename T> __global__ void __launch_bounds__(256) k_sg_kick(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const long l = (long)i * ns + j;
+  const double Dt = c->dt;
+  if (i > 0) {
+    double radsgacc = ((a.Rinf[i] - a.Rmed[i - 1]) * ldd<T>(a.sgr, l) +
+                       (a.Rmed[i] - a.Rinf[i]) * ldd<T>(a.sgr, l - ns)) * a.InvDiffRmed[i];
+    ((T *)a.vri)[l] = (T)(ldd<T>(a.vri, l) + Dt * radsgacc);
+  }
+  const long lm1 = (long)i * ns + (j == 0 ? ns - 1 : j - 1);
+  double azisgacc = 0.5 * (ldd<T>(a.sgt, l) + ldd<T>(a.sgt, lm1));
+  ((T *)a.vti)[l] = (T)(ldd<T>(a.vti, l) + Dt * azisgacc);
+}
+
+// generic in-place batched transforms (kernel spectra at set-up)
+template <typename T>
+__global__ void __launch_bounds__(256) k_fft_rows(cx<T> *data, int L, const cx<T> *tw, int dir) {
+  extern __shared__ unsigned char smem[];
+  cx<T> *A = (cx<T> *)smem, *B = A + L, *row = data + (long)blockIdx.x * L;
+  for (int j = threadIdx.x; j < L; j += blockDim.x) A[j] = row[j];
+  __syncthreads();
+  cx<T> *r = fft_lds<T>(A, B, L, tw, dir);
+  for (int j = threadIdx.x; j < L; j += blockDim.x) row[j] = r[j];
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_fft_cols(cx<T> *data, int ncols, int L, const cx<T> *tw, int dir) {
+  extern __shared__ unsigned char smem[];
+  cx<T> *A = (cx<T> *)smem, *B = A + L;
+  const int c = blockIdx.x;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) A[i] = data[(long)i * ncols + c];
+  __syncthreads();
+  cx<T> *r = fft_lds<T>(A, B, L, tw, dir);
+  for (int i = threadIdx.x; i < L; i += blockDim.x) data[(long)i * ncols + c] = r[i];
+}
+__global__ void k_convert_d2f(const double *src, float *dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+// ------------------------------------------------------------------------
+
+template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
+  const size_t rows_lds = 2 * sizeof(cx<T>) * a.ns, cols_lds = 2 * sizeof(cx<T>) * 2 * a.nr;
+  switch (id) {
+    case KID_SG_ROWS_FWD:
+      hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force);
+      break;
+    case KID_SG_COLS_FWD:
+      hipLaunchKernelGGL(k_sg_cols_fwd<T>, dim3(a.ns), dim3(256), cols_lds, s, a, force);
+      break;
+    case KID_SG_PRODUCT: {
+      long n = 2L * a.nr * a.ns;
+      hipLaunchKernelGGL(k_sg_product<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, force);
+      break;
+    }
+    case KID_SG_COLS_INV:
+      hipLaunchKernelGGL(k_sg_cols_inv<T>, dim3(a.ns), dim3(256), cols_lds, s, a, force);
+      break;
+    case KID_SG_ROWS_INV:
+      hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force);
+      break;
+    case KID_SG_KICK: {
+      int bx = a.ns < 256 ? a.ns : 256;
+      hipLaunchKernelGGL(k_sg_kick<T>, dim3((a.ns + bx - 1) / bx, a.nr), dim3(bx), 0, s, a);
+      break;
+    }
+  }
+}
+
+template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s) {
+  sg_pipeline<T>(id, a, s, 0);
+}
+template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s) {
+  sg_pipeline<T>(KID_SG_ROWS_FWD, a, s, 1);
+  sg_pipeline<T>(KID_SG_COLS_FWD, a, s, 1);
+  sg_pipeline<T>(KID_SG_PRODUCT, a, s, 1);
+  sg_pipeline<T>(KID_SG_COLS_INV, a, s, 1);
+  sg_pipeline<T>(KID_SG_ROWS_INV, a, s, 1);
+}
+template <typename T>
+void launch_fft_rows(void *data, int nrows, int L, const void *tw, int dir, hipStream_t s) {
+  hipLaunchKernelGGL(k_fft_rows<T>, dim3(nrows), dim3(256), 2 * sizeof(cx<T>) * L, s,
+                     (cx<T> *)data, L, (const cx<T> *)tw, dir);
+}
+template <typename T>
+void launch_fft_cols(void *data, int ncols, int L, const void *tw, int dir, hipStream_t s) {
+  hipLaunchKernelGGL(k_fft_cols<T>, dim3(ncols), dim3(256), 2 * sizeof(cx<T>) * L, s,
+                     (cx<T> *)data, ncols, L, (const cx<T> *)tw, dir);
+}
+void launch_convert_d2f(const double *src, float *dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_convert_d2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+}
+
+template void launch_sg_kernel<double>(int, const KArgs &, hipStream_t);
+template void launch_sg_kernel<float>(int, const KArgs &, hipStream_t);
+template void launch_sg_accel<double>(const KArgs &, hipStream_t);
+template void launch_sg_accel<float>(const KArgs &, hipStream_t);
+template void launch_fft_rows<double>(void *, int, int, const void *, int, hipStream_t);
+template void launch_fft_cols<double>(void *, int, int, const void *, int, hipStream_t);
+template void launch_fft_rows<float>(void *, int, int, const void *, int, hipStream_t);
+template void launch_fft_cols<float>(void *, int, int, const void *, int, hipStream_t);

@@ -1,0 +1,1211 @@
+// step.hip -- CDNA4 kernels of one AlgoGas hydro sub-step
+// (src/SourceEuler.c:398-716 and its callees).
+//
+// Storage type T is double or float; arithmetic is double throughout, with
+// the reference's operation order, so that the FP64 build reproduces the
+// reference pointwise (the library is compiled without contraction).
+// Every kernel of a sub-step reads dfargo_ctrl::active and returns at once
+// when the host's DT is already covered, so that a fixed batch of sub-steps
+// can be launched (or replayed from a HIP graph) without a host round trip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "kernels.h"
+
+const char *kKernelNames[KID_COUNT] = {
+    "begin",       "cfl_ring",     "control",      "eos",          "substep1",
+    "sg_rows_fwd", "sg_cols_fwd",  "sg_product",   "sg_cols_inv",  "sg_rows_inv",
+    "sg_kick",     "visc_stress",  "visc_update",  "substep2",     "energy",
+    "transport_rad", "transport_azi", "velocities", "boundary"};
+
+// ------------------------------------------------------------------------
+// helpers
+// ------------------------------------------------------------------------
+
+template <typename T> __device__ __forceinline__ double ld(const void *p, long i) {
+  return (double)((const T *)p)[i];
+}
+template <typename T> __device__ __forceinline__ void st(void *p, long i, double v) {
+  ((T *)p)[i] = (T)v;
+}
+__device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax2(double a, double b) { return (b > a) ? b : a; }
+
+// deterministic block reductions (fixed tree for a given blockDim)
+__device__ double block_sum(double v, double *sh) {
+  int t = threadIdx.x, TB = blockDim.x, P = 1;
+  while (P * 2 <= TB) P *= 2;
+  sh[t] = v;
+  __syncthreads();
+  if (t < TB - P) sh[t] += sh[t + P];
+  __syncthreads();
+  for (int s = P / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+__device__ double block_min(double v, double *sh) {
+  int t = threadIdx.x, TB = blockDim.x, P = 1;
+  while (P * 2 <= TB) P *= 2;
+  sh[t] = v;
+  __syncthreads();
+  if (t < TB - P) sh[t] = dmin2(sh[t], sh[t + P]);
+  __syncthreads();
+  for (int s = P / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = dmin2(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+// Neumaier-compensated running sum
+struct CSum {
+  double s, c;
+  __device__ void add(double x) {
+    double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  __device__ double value() const { return s + c; }
+};
+
+// One block per ring; with xcd_remap the rings of one XCD are contiguous, so
+// the radial neighbours a block reads are in the same L2.
+__device__ __forceinline__ int ring_of_block(const KArgs &a) {
+  int b = blockIdx.x, n = gridDim.x;
+  if (a.xcd_remap && (n % 8 == 0)) return (b % 8) * (n / 8) + b / 8;
+  return b;
+}
+
+// AspectRatio (src/Viscosity.c:107-122) and compute_smoothing
+// (src/Force.c:255-263), LAMBDADOUBLING = 0
+__device__ double d_aspect_ratio(const KArgs &a, double rad) {
+  double ar = a.aspect_ratio;
+  double rmin = a.transition_radius - a.transition_width * a.aspect_ratio;
+  double rmax = a.transition_radius + a.transition_width * a.aspect_ratio;
+  if (rad < rmin) ar *= a.transition_ratio;
+  if ((rad >= rmin) && (rad <= rmax))
+    ar *= exp((rmax - rad) / (rmax - rmin) * log(a.transition_ratio));
+  return ar;
+}
+__device__ double d_smoothing(const KArgs &a, double r) {
+  return a.thickness_smoothing * d_aspect_ratio(a, r) * pow(r, 1.0 + a.flaring);
+}
+
+// gravitational potential at cell (i, j): FillForcesArrays
+// (src/Pframeforce.c:44-129) with the parameters the control kernel stored
+__device__ double pot_at(const KArgs &a, const dfargo_ctrl *c, int i, int j) {
+  double x = a.Rmed[i] * a.CosAz[j], y = a.Rmed[i] * a.SinAz[j];
+  double pot = 0.0;
+  for (int k = 0; k < c->nb; k++) {
+    double xp = c->pot_x[k], yp = c->pot_y[k], mp = c->pot_m[k];
+    double distance = (x - xp) * (x - xp) + (y - yp) * (y - yp);
+    double distancesmooth = sqrt(distance + c->pot_smooth[k]);
+    double pk = -DF_G * mp / distancesmooth;
+    if (a.indirect_term) pk += DF_G * mp * c->pot_invd3[k] * (x * xp + y * yp);
+    pot += pk;
+  }
+  double ps = -DF_G * 1.0 * a.InvRmed[i];
+  if (a.gas_indirect) ps -= c->ind_x * x + c->ind_y * y;
+  pot += ps;
+  return pot;
+}
+
+// ------------------------------------------------------------------------
+// per-call and per-sub-step control
+// ------------------------------------------------------------------------
+
+__global__ void k_algogas_begin(KArgs a, double DT) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    dfargo_ctrl *c = a.ctrl;
+    c->DT = DT;
+    c->dtemp = 0.0;
+    c->substeps = 0;
+    c->gas_timesteps_cfl = 1;
+  }
+}
+
+// start of a sub-step: loop condition, mass taper, n-body limit and the
+// force-sum targets (src/SourceEuler.c:398-440, src/Pframeforce.c:131-161)
+__global__ void k_begin(KArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  dfargo_ctrl *c = a.ctrl;
+  int active = c->dtemp < 0.999999999 * c->DT;
+  c->active = active;
+  if (!active) return;
+  int nb = c->nb;
+  if (a.mass_taper_param > 1e-2) {
+    double mt = (c->time - c->time0) / (a.mass_taper_param * 2.0 * DF_PI);
+    mt = (mt > 1.0 ? 1.0 : pow(sin(mt * DF_PI / 2.0), 2.0));
+    c->mass_taper = mt;
+    if (mt < 1.0)
+      for (int k = 0; k < nb; k++) c->pl_m[k] = c->pl_m0[k] + (c->pl_m1[k] - c->pl_m0[k]) * mt;
+  } else {
+    for (int k = 0; k < nb; k++)
+      if (c->pl_acc[k] < 1e-10) c->pl_m[k] = c->pl_m1[k];
+  }
+  double dtnbody = 100.0;
+  if (nb > 1) {
+    dtnbody = 1e5;
+    for (int k = 0; k < nb; k++)
+      for (int j = k + 1; j < nb; j++) {
+        double dx = c->pl_x[k] - c->pl_x[j], dy = c->pl_y[k] - c->pl_y[j];
+        double dist = sqrt(dx * dx + dy * dy);
+        double buf = 2.0 * DF_PI * sqrt(dist * dist * dist / (c->pl_m[k] + c->pl_m[j] + 1e-8)) / 100.0;
+        dtnbody = dmin2(dtnbody, buf);
+      }
+  }
+  c->dtnbody = dtnbody;
+  // target 0: the star (ComputeAccel at 0,0, src/SourceEuler.c:482)
+  c->tg_x[0] = 0.0;
+  c->tg_y[0] = 0.0;
+  c->tg_rsm[0] = a.indirect_smoothing;
+  c->tg_a[0] = sqrt(0.0 * 0.0 + 0.0 * 0.0);
+  c->tg_rh[0] = pow(0.0 / 3., 1. / 3.) * c->tg_a[0] + 1e-15;
+  c->tg_planet[0] = -1;
+  int nt = 1;
+  for (int k = 0; k < nb; k++) {
+    if (!c->pl_feel_disk[k]) continue;
+    double m = c->pl_m[k], x = c->pl_x[k], y = c->pl_y[k], r = sqrt(x * x + y * y);
+    double sm = a.roche ? r * pow(m / 3., 1. / 3.) * a.roche_smoothing : d_smoothing(a, r);
+    c->tg_x[nt] = x;
+    c->tg_y[nt] = y;
+    c->tg_rsm[nt] = sm;
+    c->tg_a[nt] = sqrt(x * x + y * y);
+    c->tg_rh[nt] = pow(m / 3., 1. / 3.) * c->tg_a[nt] + 1e-15;
+    c->tg_planet[nt] = k;
+    nt++;
+  }
+  c->ntarget = nt;
+}
+
+// ConditionCFL per ring (src/SourceEuler.c:1340-1484) fused with the
+// per-ring partial sums of ComputeForce (src/Force.c:155-214), GasTotalMass
+// and the barycentre (src/SideEuler.c:15-77) and DetectCrash
+// (src/SourceEuler.c:51-69): one read of rho, vr, vt (and cs, e).
+template <typename T, int K> __global__ void __launch_bounds__(1024) k_cfl_ring(KArgs a) {
+  __shared__ double sh[1024];
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = ring_of_block(a), ns = a.ns, nr = a.nr, t = threadIdx.x, TB = blockDim.x;
+  const long base = (long)i * ns;
+  double vt_own[K];
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    vt_own[k] = ld<T>(a.vt, base + t + k * TB);
+    s += vt_own[k];
+  }
+  s = block_sum(s, sh);
+  const double vmoy = s / (double)ns;
+  if (t == 0) a.vmoy[i] = vmoy;
+  double dtmin = 1e30;
+  if (i >= 1) {
+    const double viscosity = a.nu_ring[i];
+    const double dxrad = a.Rsup[i] - a.Rinf[i];
+    const double dxtheta = a.Rmed[i] * a.pm / (double)ns;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int j = t + k * TB;
+      const long l = base + j, lip = l + ns, ljp = (j == ns - 1) ? base : l + 1;
+      double cs = a.energy ? ld<T>(a.cs, l) : a.cs_iso[i];
+      double vres = a.fast ? vt_own[k] - vmoy : vt_own[k];
+      double vrl = ld<T>(a.vr, l);
+      double invdt1 = cs / (dmin2(dxrad, dxtheta));
+      double invdt2 = fabs(vrl) / dxrad;
+      double invdt3 = fabs(vres) / dxtheta;
+      double dvr = ld<T>(a.vr, lip) - vrl;
+      double dvt = ld<T>(a.vt, ljp) - vt_own[k];
+      if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
+      if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
+      double invdt4 = dmax2(dvr / dxrad, dvt / dxtheta);
+      invdt4 *= 4.0 * DF_CVNR * DF_CVNR;
+      double invdt5 = (viscosity != 0.0) ? viscosity * 4.0 / a.powmin2[i] : 1e-10;
+      double invdt6 = 1e-10;
+      double dt = DF_CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 +
+                                        invdt4 * invdt4 + invdt5 * invdt5 + invdt6 * invdt6);
+      dtmin = dmin2(dtmin, dt);
+    }
+  }
+  dtmin = block_min(dtmin, sh);
+  if (t == 0) a.ring_dt[i] = dtmin;
+  // crash scan
+  double crd = 0.0, cre = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long l = base + t + k * TB;
+    if (ld<T>(a.rho, l) < 0.0) crd = 1.0;
+    if (a.energy && ld<T>(a.en, l) < 0.0) cre = 1.0;
+  }
+  crd = block_sum(crd, sh);
+  cre = block_sum(cre, sh);
+  if (t == 0) {
+    a.ring_crash[2 * i] = crd > 0.0;
+    a.ring_crash[2 * i + 1] = cre > 0.0;
+  }
+  // partial sums: disc mass, barycentre, then 8 force sums per target
+  const int NP = DF_NPART(c->ntarget);
+  double *part = a.ring_part + (long)i * DF_NPART(DF_MAXTG);
+  {
+    CSum m = {0, 0}, xb = {0, 0}, yb = {0, 0};
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int j = t + k * TB;
+      double dens = ld<T>(a.rho, base + j);
+      double cm = a.Surf[i] * dens;
+      m.add(cm);
+      xb.add(cm * (a.Rmed[i] * a.CosAz[j]));
+      yb.add(cm * (a.Rmed[i] * a.SinAz[j]));
+    }
+    double v0 = block_sum(m.value(), sh), v1 = block_sum(xb.value(), sh), v2 = block_sum(yb.value(), sh);
+    if (t == 0) { part[0] = v0; part[1] = v1; part[2] = v2; }
+  }
+  for (int tg = 0; tg < c->ntarget; tg++) {
+    const double x = c->tg_x[tg], y = c->tg_y[tg], rsm = c->tg_rsm[tg];
+    const double aa = c->tg_a[tg], rh = c->tg_rh[tg];
+    const int inner = a.Rmed[i] < aa;
+    CSum fx[2] = {{0, 0}, {0, 0}}, fy[2] = {{0, 0}, {0, 0}};
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int j = t + k * TB;
+      double xc = a.Rmed[i] * a.CosAz[j], yc = a.Rmed[i] * a.SinAz[j];
+      double cellmass = a.Surf[i] * ld<T>(a.rho, base + j);
+      double dx = xc - x, dy = yc - y;
+      double d2 = dx * dx + dy * dy;
+      double planet_distance = sqrt(d2);
+      double dist2 = d2 + rsm * rsm;
+      double distance = sqrt(dist2);
+      double InvDist3 = 1.0 / dist2 / distance;
+#pragma unroll
+      for (int kk = 0; kk < 2; kk++) {
+        double hill_cut = 1.;
+        if (kk != 0) {
+          double cutoff = a.exclude_hill_factor * rh;
+          if (planet_distance / cutoff < 0.5) hill_cut = 0.0;
+          else if (planet_distance > cutoff) hill_cut = 1.0;
+          else hill_cut = pow(sin((planet_distance / cutoff - .5) * DF_PI), 2.);
+        }
+        fx[kk].add(DF_G * cellmass * dx * InvDist3 * hill_cut);
+        fy[kk].add(DF_G * cellmass * dy * InvDist3 * hill_cut);
+      }
+    }
+    double r0 = block_sum(fx[0].value(), sh), r1 = block_sum(fx[1].value(), sh);
+    double r2 = block_sum(fy[0].value(), sh), r3 = block_sum(fy[1].value(), sh);
+    if (t == 0) {
+      double *g = part + 3 + 8 * tg;
+      // GlobalForce layout: fxi[0..1], fxo[0..1], fyi[0..1], fyo[0..1]
+      g[0] = inner ? r0 : 0.0; g[1] = inner ? r1 : 0.0;
+      g[2] = inner ? 0.0 : r0; g[3] = inner ? 0.0 : r1;
+      g[4] = inner ? r2 : 0.0; g[5] = inner ? r3 : 0.0;
+      g[6] = inner ? 0.0 : r2; g[7] = inner ? 0.0 : r3;
+    }
+  }
+  (void)NP;
+  (void)nr;
+}
+
+// Cash-Karp RK5 of the star+planet problem (src/RungeKunta.c:14-91)
+__device__ void d_deriv_rk5(const double *q, const double *masses, double *deriv, int n, double dt,
+                            const int *feelothers, int indirect) {
+  const double *x = q, *y = q + n, *vx = y + n, *vy = vx + n;
+  double *dx = deriv, *dy = deriv + n, *dvx = dy + n, *dvy = dvx + n;
+  double Dist[DF_MAXPL];
+  for (int i = 0; i < n; i++) Dist[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
+  for (int i = 0; i < n; i++) {
+    dx[i] = vx[i];
+    dy[i] = vy[i];
+    dvx[i] = -DF_G * 1.0 / Dist[i] / Dist[i] / Dist[i] * x[i];
+    dvy[i] = -DF_G * 1.0 / Dist[i] / Dist[i] / Dist[i] * y[i];
+    for (int j = 0; j < n; j++) {
+      if (indirect) {
+        dvx[i] -= DF_G * masses[j] / Dist[j] / Dist[j] / Dist[j] * x[j];
+        dvy[i] -= DF_G * masses[j] / Dist[j] / Dist[j] / Dist[j] * y[j];
+      }
+      if ((j != i) && feelothers[i]) {
+        double dist = (x[i] - x[j]) * (x[i] - x[j]) + (y[i] - y[j]) * (y[i] - y[j]);
+        dist = sqrt(dist);
+        dvx[i] += DF_G * masses[j] / dist / dist / dist * (x[j] - x[i]);
+        dvy[i] += DF_G * masses[j] / dist / dist / dist * (y[j] - y[i]);
+      }
+    }
+  }
+  for (int i = 0; i < 4 * n; i++) deriv[i] *= dt;
+}
+__device__ void d_translate(const double *q0, const double *k1, const double *k2, const double *k3,
+                            const double *k4, const double *k5, double c1, double c2, double c3,
+                            double c4, double c5, double *q1, int n) {
+  for (int i = 0; i < 4 * n; i++)
+    q1[i] = q0[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i] + c4 * k4[i] + c5 * k5[i];
+}
+__device__ void d_rk5(const double *q0, double dt, const double *m, double *q1, int n,
+                      const int *fo, int ind) {
+  double k1[4 * DF_MAXPL], k2[4 * DF_MAXPL], k3[4 * DF_MAXPL], k4[4 * DF_MAXPL], k5[4 * DF_MAXPL],
+      k6[4 * DF_MAXPL];
+  for (int i = 0; i < 4 * DF_MAXPL; i++) k1[i] = k2[i] = k3[i] = k4[i] = k5[i] = k6[i] = 0.0;
+  d_deriv_rk5(q0, m, k1, n, dt, fo, ind);
+  d_translate(q0, k1, k2, k3, k4, k5, 0.2, 0.0, 0.0, 0.0, 0.0, q1, n);
+  d_deriv_rk5(q1, m, k2, n, dt, fo, ind);
+  d_translate(q0, k1, k2, k3, k4, k5, 0.075, 0.225, 0.0, 0.0, 0.0, q1, n);
+  d_deriv_rk5(q1, m, k3, n, dt, fo, ind);
+  d_translate(q0, k1, k2, k3, k4, k5, 0.3, -0.9, 1.2, 0.0, 0.0, q1, n);
+  d_deriv_rk5(q1, m, k4, n, dt, fo, ind);
+  d_translate(q0, k1, k2, k3, k4, k5, -11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0.0, q1, n);
+  d_deriv_rk5(q1, m, k5, n, dt, fo, ind);
+  d_translate(q0, k1, k2, k3, k4, k5, 1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0,
+              44275.0 / 110592.0, 253.0 / 4096.0, q1, n);
+  d_deriv_rk5(q1, m, k6, n, dt, fo, ind);
+  for (int i = 0; i < 4 * n; i++)
+    q1[i] = q0[i] + 37.0 / 378.0 * k1[i] + 250.0 / 621.0 * k3[i] + 125.0 / 594.0 * k4[i] +
+            512.0 / 1771.0 * k6[i];
+}
+
+// Scalar part of a sub-step, one block: the CFL reduction and sub-step dt
+// (src/SourceEuler.c:443-470, 1485-1551), the gas indirect term and the
+// barycentre acceleration (src/SideEuler.c:91-112), the potential
+// parameters of FillForcesArrays (src/Pframeforce.c:44-90), and the planet
+// update (AdvanceSystemFromDisk, AdvanceSystemRK5, RotatePsys).
+__global__ void __launch_bounds__(256) k_control(KArgs a) {
+  __shared__ double sh[256];
+  __shared__ double sums[DF_NPART(DF_MAXTG)];
+  __shared__ int crash[2];
+  dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int t = threadIdx.x, TB = blockDim.x, nr = a.nr, ns = a.ns;
+  double nd = 1e30;
+  for (int i = 1 + t; i < nr; i += TB) nd = dmin2(nd, a.ring_dt[i]);
+  for (int i = t; i < nr - 1; i += TB) {
+    double dt = a.pm * DF_CFLSECURITY / (double)ns /
+                fabs(a.vmoy[i] * a.InvRmed[i] - a.vmoy[i + 1] * a.InvRmed[i + 1]);
+    nd = dmin2(nd, dt);
+  }
+  nd = block_min(nd, sh);
+  const int NP = DF_NPART(c->ntarget);
+  if (t < NP) {
+    CSum s = {0, 0};
+    for (int i = 0; i < nr; i++) s.add(a.ring_part[(long)i * DF_NPART(DF_MAXTG) + t]);
+    sums[t] = s.value();
+  }
+  if (t < 2) {
+    int f = 0;
+    for (int i = 0; i < nr; i++) f |= a.ring_crash[2 * i + t];
+    crash[t] = f;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  // sub-step dt
+  double deltaT = c->DT - c->dtemp;
+  if (deltaT < nd) nd = deltaT;
+  int gts = (int)(ceil(deltaT / nd));
+  double dthydro = deltaT / (double)gts;
+  double dt = dmin2(dthydro, c->dtnbody);
+  c->dtemp += dt;
+  if (dt < c->mindt) c->mindt = dt;
+  if (dt > c->maxdt) c->maxdt = dt;
+  c->gas_timesteps_cfl = gts;
+  c->dt = dt;
+  c->crashed_d = crash[0];
+  c->crashed_e = crash[1];
+  // gas indirect term: ComputeAccel(star), src/SideEuler.c:282-298
+  const double *g0 = sums + 3;
+  if (a.exclude_hill) {
+    c->dop_x = g0[1] + g0[3];
+    c->dop_y = g0[5] + g0[7];
+  } else {
+    c->dop_x = g0[0] + g0[2];
+    c->dop_y = g0[4] + g0[6];
+  }
+  // barycentre (src/SideEuler.c:42-112)
+  double total = sums[0];
+  c->barx[2] = sums[1] / (1.0 + total);
+  c->bary[2] = sums[2] / (1.0 + total);
+  c->dopalt_x = -(c->barx[2] - 2.0 * c->barx[1] + c->barx[0]) / (dt * dt);
+  c->dopalt_y = -(c->bary[2] - 2.0 * c->bary[1] + c->bary[0]) / (dt * dt);
+  c->barx[0] = c->barx[1]; c->barx[1] = c->barx[2];
+  c->bary[0] = c->bary[1]; c->bary[1] = c->bary[2];
+  // FillForcesArrays parameters
+  double planetmass_taper = 1.0;
+  if (a.mass_taper_param > 1e-2) {
+    double mt = (c->time - c->time0) / (a.mass_taper_param * 2.0 * DF_PI);
+    planetmass_taper = (mt > 1.0 ? 1.0 : .5 * (1.0 - cos(DF_PI * mt)));
+  }
+  c->ind_x = -c->dop_x;
+  c->ind_y = -c->dop_y;
+  if (a.alt_indirect) {
+    c->ind_x = -c->dopalt_x;
+    c->ind_y = -c->dopalt_y;
+  }
+  if (!a.indirect_term) {
+    c->ind_x = 0.0;
+    c->ind_y = 0.0;
+  }
+  const int nb = c->nb;
+  for (int k = 0; k < nb; k++) {
+    double xp = c->pl_x[k], yp = c->pl_y[k], mp = c->pl_m[k] * planetmass_taper;
+    double PlanetDistance = sqrt(xp * xp + yp * yp);
+    double invd3 = 1.0 / PlanetDistance / PlanetDistance / PlanetDistance;
+    double RRoche = PlanetDistance * pow((1.0 / 3.0 * mp), 1.0 / 3.0);
+    double sm = a.roche ? RRoche * a.roche_smoothing : d_smoothing(a, PlanetDistance);
+    c->pot_x[k] = xp;
+    c->pot_y[k] = yp;
+    c->pot_m[k] = mp;
+    c->pot_invd3[k] = invd3;
+    c->pot_smooth[k] = sm * sm;
+  }
+  // AdvanceSystemFromDisk (src/Pframeforce.c:131-161)
+  for (int tg = 1; tg < c->ntarget; tg++) {
+    int k = c->tg_planet[tg];
+    const double *g = sums + 3 + 8 * tg;
+    double gx = a.exclude_hill ? g[1] + g[3] : g[0] + g[2];
+    double gy = a.exclude_hill ? g[5] + g[7] : g[4] + g[6];
+    c->pl_vx[k] += dt * gx;
+    c->pl_vy[k] += dt * gy;
+    c->pl_vx[k] += dt * c->ind_x;
+    c->pl_vy[k] += dt * c->ind_y;
+  }
+  // AdvanceSystemRK5 (src/Pframeforce.c:163-266)
+  if (nb > 0) {
+    double q0[4 * DF_MAXPL], q1[4 * DF_MAXPL], m[DF_MAXPL];
+    if (!a.forced_circular) {
+      for (int i = 0; i < nb; i++) {
+        q0[i] = c->pl_x[i]; q0[i + nb] = c->pl_y[i];
+        q0[i + 2 * nb] = c->pl_vx[i]; q0[i + 3 * nb] = c->pl_vy[i];
+        m[i] = c->pl_m[i];
+      }
+      d_rk5(q0, dt, m, q1, nb, c->pl_feel_others, a.indirect_term);
+    }
+    for (int i = 1 - (c->time >= a.release_date); i < nb; i++) {
+      if (!a.forced_circular) {
+        c->pl_x[i] = q1[i]; c->pl_y[i] = q1[i + nb];
+        c->pl_vx[i] = q1[i + 2 * nb]; c->pl_vy[i] = q1[i + 3 * nb];
+      } else {
+        double x = c->pl_x[i], y = c->pl_y[i], theta = atan2(y, x);
+        double vx = c->pl_vx[i], vy = c->pl_vy[i];
+        double r = sqrt(x * x + y * y), v = sqrt(vx * vx + vy * vy);
+        double omega = (-y * vx + x * vy) / r / r, dtheta = omega * dt;
+        c->pl_x[i] = r * cos(theta + dtheta);
+        c->pl_y[i] = r * sin(theta + dtheta);
+        c->pl_vx[i] = -v * sin(theta + dtheta);
+        c->pl_vy[i] = v * cos(theta + dtheta);
+      }
+    }
+    if (c->time < a.release_date) {
+      double x = c->pl_x[0], y = c->pl_y[0], r = sqrt(x * x + y * y), theta = atan2(y, x);
+      double rdot = (a.release_radius - r) / (a.release_date - c->time);
+      double omega = sqrt((1. + c->pl_m[0]) / r / r / r), new_r = r + rdot * dt, denom = r - new_r;
+      double dtheta = (denom != 0.0) ? 2. * dt * r * omega / denom * (sqrt(r / new_r) - 1.) : omega * dt;
+      double vx = rdot, vy = new_r * sqrt((1. + c->pl_m[0]) / new_r / new_r / new_r);
+      c->pl_x[0] = new_r * cos(dtheta + theta);
+      c->pl_y[0] = new_r * sin(dtheta + theta);
+      c->pl_vx[0] = vx * cos(dtheta + theta) - vy * sin(dtheta + theta);
+      c->pl_vy[0] = vx * sin(dtheta + theta) + vy * cos(dtheta + theta);
+    }
+    // RotatePsys (src/Psys.c:352-370)
+    double angle = c->omega_frame * dt, sint = sin(angle), cost = cos(angle);
+    for (int i = 0; i < nb; i++) {
+      double xt = c->pl_x[i], yt = c->pl_y[i];
+      c->pl_x[i] = xt * cost + yt * sint;
+      c->pl_y[i] = -xt * sint + yt * cost;
+      xt = c->pl_vx[i];
+      yt = c->pl_vy[i];
+      c->pl_vx[i] = xt * cost + yt * sint;
+      c->pl_vy[i] = -xt * sint + yt * cost;
+    }
+  }
+  // PhysicalTime += dt closes the sub-step (src/SourceEuler.c:714); every
+  // other use of PhysicalTime in the sub-step is above
+  c->time += dt;
+  c->substeps += 1;
+  c->substeps_total += 1;
+}
+
+// ComputeSoundSpeed (adiabatic, src/SourceEuler.c:2340) per ring, with the
+// ring mean mpi_make1Dprofile feeds to FViscosity (src/Viscosity.c:30-35)
+template <typename T, int K> __global__ void __launch_bounds__(1024) k_eos(KArgs a, int pre) {
+  __shared__ double sh[1024];
+  if (!pre && !a.ctrl->active) return;
+  const int i = ring_of_block(a), ns = a.ns, t = threadIdx.x, TB = blockDim.x;
+  const long base = (long)i * ns;
+  const double g = a.gamma;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long l = base + t + k * TB;
+    double cs = sqrt(g * (g - 1.0) * ld<T>(a.en, l) / ld<T>(a.rho, l));
+    st<T>(a.cs, l, cs);
+    s += (double)(T)cs;
+  }
+  if (pre) return;
+  s = block_sum(s, sh);
+  if (t == 0) {
+    double gcs = s / (double)ns;
+    a.gcs[i] = gcs;
+    double nu = a.visc_const;
+    if (a.alpha_on) nu = a.alpha * gcs * gcs * a.r15[i];
+    a.nu_ring[i] = a.visc_on ? nu * a.visc_mult[i] : 0.0;
+  }
+}
+
+// SubStep1 (src/SourceEuler.c:865-994): pressure, potential and curvature
+// kicks into VradInt / VthetaInt
+template <typename T> __global__ void __launch_bounds__(256) k_substep1(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const long l = (long)i * ns + j;
+  const double dt = c->dt;
+  const double csi = a.cs_iso[i];
+  auto press = [&](long ll, int ii) -> double {
+    if (!a.energy) {
+      double cs = (double)(T)a.cs_iso[ii];
+      return ld<T>(a.rho, ll) * cs * cs;
+    }
+    return (a.gamma - 1.0) * ld<T>(a.en, ll);
+  };
+  (void)csi;
+  const double rl = ld<T>(a.rho, l), pl = press(l, i), potl = pot_at(a, c, i, j);
+  if (i >= 1) {
+    const long lim = l - ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
+    double gradp = (pl - press(lim, i - 1)) * 2.0 / (rl + ld<T>(a.rho, lim)) * a.InvDiffRmed[i];
+    double gradphi = (potl - pot_at(a, c, i - 1, j)) * a.InvDiffRmed[i];
+    double vt2 = ld<T>(a.vt, l) + ld<T>(a.vt, ljp) + ld<T>(a.vt, lim) + ld<T>(a.vt, ljp - ns);
+    vt2 = vt2 / 4.0 + a.Rinf[i] * c->omega_frame;
+    vt2 = vt2 * vt2;
+    st<T>(a.vri, l, ld<T>(a.vr, l) + dt * (-gradp - gradphi + vt2 * a.InvRinf[i]));
+  } else {
+    st<T>(a.vri, l, 0.0);
+  }
+  const int jm = (j == 0) ? ns - 1 : j - 1;
+  const long ljm = (long)i * ns + jm;
+  double dxtheta = a.pm / (double)ns * a.Rmed[i];
+  double invdxtheta = 1.0 / dxtheta;
+  double gradp = (pl - press(ljm, i)) * 2.0 / (rl + ld<T>(a.rho, ljm)) * invdxtheta;
+  double gradphi = (potl - pot_at(a, c, i, jm)) * invdxtheta;
+  double vti = ld<T>(a.vt, l) - dt * (gradp + gradphi);
+  vti += dt * a.supp_torque[i];
+  st<T>(a.vti, l, vti);
+}
+
+// ComputeViscousTerms stresses (src/Viscosity.c:159-300) on (vr, vt)
+template <typename T>
+__device__ void visc_stress_cell(const KArgs &a, const void *vr, const void *vt, int i, int j,
+                                 double *Trr, double *Tpp, double *Trp, double *divv) {
+  const int ns = a.ns;
+  const long l = (long)i * ns + j, lip = l + ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
+  const double onethird = 1.0 / 3.0, nu = a.nu_ring[i];
+  double vrl = ld<T>(vr, l), vrp = ld<T>(vr, lip), vtl = ld<T>(vt, l), vtp = ld<T>(vt, ljp);
+  double dv = (vrp * a.Rsup[i] - vrl * a.Rinf[i]) * a.InvDiffRsup[i] * a.InvRmed[i];
+  dv += (vtp - vtl) * a.invdphi * a.InvRmed[i];
+  double Drr = (vrp - vrl) * a.InvDiffRsup[i];
+  double Dpp = (vtp - vtl) * a.invdphi * a.InvRmed[i] + 0.5 * (vrp + vrl) * a.InvRmed[i];
+  double rho = ld<T>(a.rho, l);
+  *Trr = 2.0 * rho * nu * (Drr - onethird * dv);
+  *Tpp = 2.0 * rho * nu * (Dpp - onethird * dv);
+  *divv = dv;
+  if (i >= 1) {
+    const long lim = l - ns, ljm = (j == 0) ? (long)i * ns + ns - 1 : l - 1, ljmim = ljm - ns;
+    double Drp = 0.5 * (a.Rinf[i] * (vtl * a.InvRmed[i] - ld<T>(vt, lim) * a.InvRmed[i - 1]) *
+                            a.InvDiffRmed[i] +
+                        (vrl - ld<T>(vr, ljm)) * a.invdphi * a.InvRinf[i]);
+    *Trp = 2.0 * 0.25 * (rho + ld<T>(a.rho, lim) + ld<T>(a.rho, ljm) + ld<T>(a.rho, ljmim)) * nu * Drp;
+  } else {
+    *Trp = 0.0;
+  }
+}
+
+template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs a, int from_state) {
+  if (!from_state && !a.ctrl->active) return;
+  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const long l = (long)i * ns + j;
+  double Trr, Tpp, Trp, dv;
+  visc_stress_cell<T>(a, from_state ? a.vr : a.vri, from_state ? a.vt : a.vti, i, j, &Trr, &Tpp, &Trp, &dv);
+  st<T>(a.trr, l, Trr);
+  st<T>(a.tpp, l, Tpp);
+  st<T>(a.trp, l, Trp);
+}
+
+// UpdateVelocitiesWithViscosity (src/Viscosity.c:302-378) on VradInt/VthetaInt
+template <typename T> __global__ void __launch_bounds__(256) k_visc_update(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const long l = (long)i * ns + j;
+  const double DeltaT = c->dt;
+  const double rho = ld<T>(a.rho, l);
+  if (i >= 1 && i < nr - 1) {
+    const long lip = l + ns, ljm = (j == 0) ? (long)i * ns + ns - 1 : l - 1;
+    double trpl = ld<T>(a.trp, l), trpp = ld<T>(a.trp, lip);
+    double v = ld<T>(a.vti, l);
+    v += DeltaT * a.InvRmed[i] *
+         ((a.Rsup[i] * trpp - a.Rinf[i] * trpl) * a.InvDiffRsup[i] +
+          (ld<T>(a.tpp, l) - ld<T>(a.tpp, ljm)) * a.invdphi + 0.5 * (trpl + trpp)) /
+         (0.5 * (rho + ld<T>(a.rho, ljm)));
+    st<T>(a.vti, l, v);
+  }
+  if (i >= 1) {
+    const long lim = l - ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
+    double v = ld<T>(a.vri, l);
+    v += DeltaT * a.InvRinf[i] *
+         ((a.Rmed[i] * ld<T>(a.trr, l) - a.Rmed[i - 1] * ld<T>(a.trr, lim)) * a.InvDiffRmed[i] +
+          (ld<T>(a.trp, ljp) - ld<T>(a.trp, l)) * a.invdphi - 0.5 * (ld<T>(a.tpp, l) + ld<T>(a.tpp, lim))) /
+         (0.5 * (rho + ld<T>(a.rho, lim)));
+    st<T>(a.vri, l, v);
+  }
+}
+
+// SubStep2 (src/SourceEuler.c:1012-1116): artificial viscosity, written
+// straight into the state (the ActualiseGas copies of src/SourceEuler.c:
+// 598-630); vr of rings 0 and nrad is zero, as VradNew is never written there
+template <typename T> __global__ void __launch_bounds__(256) k_substep2(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const long l = (long)i * ns + j;
+  if (i == nr) {
+    st<T>(a.vr, l, 0.0);
+    st<T>(a.vt, l, 0.0);
+    return;
+  }
+  const double dt = c->dt;
+  const double C2 = DF_CVNR * DF_CVNR;
+  const long lip = l + ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
+  const long ljm = (j == 0) ? (long)i * ns + ns - 1 : l - 1;
+  const double rho = ld<T>(a.rho, l);
+  const double vrl = ld<T>(a.vri, l), vrp = ld<T>(a.vri, lip);
+  const double vtl = ld<T>(a.vti, l), vtp = ld<T>(a.vti, ljp), vtm = ld<T>(a.vti, ljm);
+  double dv = vrp - vrl;
+  const double qr = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
+  dv = vtp - vtl;
+  const double qt = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
+  // qt at j-1
+  const double rhojm = ld<T>(a.rho, ljm);
+  dv = vtl - vtm;
+  const double qtm = (dv < 0.0) ? C2 * rhojm * dv * dv : 0.0;
+  if (i >= 1) {
+    const long lim = l - ns;
+    const double rhoim = ld<T>(a.rho, lim);
+    dv = vrl - ld<T>(a.vri, lim);
+    const double qrm = (dv < 0.0) ? C2 * rhoim * dv * dv : 0.0;
+    st<T>(a.vr, l, vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * a.InvDiffRmed[i]);
+  } else {
+    st<T>(a.vr, l, 0.0);
+  }
+  double dxtheta = a.pm / (double)ns * a.Rmed[i];
+  double invdxtheta = 1.0 / dxtheta;
+  st<T>(a.vt, l, vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta);
+  if (a.energy) {
+    double e = ld<T>(a.en, l);
+    e = e - dt * qr * (vrp - vrl) * a.InvDiffRsup[i] - dt * qt * (vtp - vtl) * invdxtheta;
+    st<T>(a.en, l, e);
+  }
+}
+
+// Energy source terms, per ring: ComputeDivergenceVelocity
+// (src/SourceEuler.c:2373-2411), the stresses and ComputeViscousHeating
+// (src/SourceEuler.c:1554-1606) on the post-SubStep2 velocities, then
+// SubStep3 (src/SourceEuler.c:1147-1198) with beta cooling
+template <typename T>
+__device__ double vischeat_cell(const KArgs &a, int i, int j, double *trr, double *tpp,
+                                double *trp, double *divv) {
+  visc_stress_cell<T>(a, a.vr, a.vt, i, j, trr, tpp, trp, divv);
+  const double viscosity = a.nu_ring[i];
+  if (viscosity == 0.0) return 0.0;
+  const double dens = ld<T>(a.rho, (long)i * a.ns + j);
+  double vh = 0.5 / viscosity / dens * (*trr * *trr + 2.0 * *trp * *trp + *tpp * *tpp);
+  vh += (2.0 / 9.0) * viscosity * dens * *divv * *divv;
+  return vh;
+}
+
+template <typename T, int K> __global__ void __launch_bounds__(1024) k_energy(KArgs a) {
+  __shared__ double sh[1024];
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = ring_of_block(a), ns = a.ns, nr = a.nr, t = threadIdx.x, TB = blockDim.x;
+  const long base = (long)i * ns;
+  const double dt = c->dt;
+  double vh[K], dv[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int j = t + k * TB;
+    double trr, tpp, trp, d;
+    if (a.viscous_heating && i >= 1) {
+      vh[k] = vischeat_cell<T>(a, i, j, &trr, &tpp, &trp, &d);
+    } else {
+      visc_stress_cell<T>(a, a.vr, a.vt, i, j, &trr, &tpp, &trp, &d);
+      vh[k] = 0.0;
+    }
+    if (a.viscous_heating && i == 0) {
+      // power-law extrapolation from rings 1 and 2; the test reads the
+      // viscosity left by the ring loop, i.e. that of ring nrad-1
+      double t1, t2, t3, d1;
+      double q1 = vischeat_cell<T>(a, 1, j, &t1, &t2, &t3, &d1);
+      double q2 = vischeat_cell<T>(a, 2, j, &t1, &t2, &t3, &d1);
+      vh[k] = (a.nu_ring[nr - 1] != 0.0) ? q1 * exp(log(q1 / q2) * a.vh_lr1 / a.vh_lr2) : 0.0;
+    }
+    dv[k] = d;
+    st<T>(a.trr, base + j, trr);
+    st<T>(a.tpp, base + j, tpp);
+    st<T>(a.trp, base + j, trp);
+    st<T>(a.divv, base + j, d);
+    st<T>(a.vh, base + j, vh[k]);
+  }
+  double enew[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long l = base + t + k * TB;
+    double num = ld<T>(a.en, l) + (a.viscous_heating ? 1 : 0) * vh[k] * dt;
+    double den = 1.0 + (a.gamma - 1.0) * dt * dv[k];
+    enew[k] = num / den;
+  }
+  if (a.beta_cooling) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) s += (ld<T>(a.vt, base + t + k * TB) + a.Rmed[i] * c->omega_frame);
+    s = block_sum(s, sh);
+    double omega = s / (double)ns;
+    omega /= a.Rmed[i];
+    double coolingtime = a.beta[i] / omega;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double num = enew[k], den = 1.0 + dt / coolingtime;
+      enew[k] = num / den;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) st<T>(a.en, base + t + k * TB, enew[k]);
+}
+
+// ------------------------------------------------------------------------
+// transport
+// ------------------------------------------------------------------------
+
+// Radial van Leer of the four momenta, energy and density for one cell
+// (ComputeLRMomenta + OneWindRad, src/TransportEuler.c:89-220): the momenta
+// of rings i-2..i+2 are rebuilt from rho, vr, vt, the Work = Q/RhoInt
+// slopes and star values follow ComputeStarRad, and the conservative update
+// uses the fluxes through the faces Rinf[i] and Rsup[i].
+template <typename T> __global__ void __launch_bounds__(256) k_transport_rad(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const double dt = c->dt, Of = c->omega_frame;
+  const int jp = (j == ns - 1) ? 0 : j + 1;
+  const int nq = a.energy ? 6 : 5; // rp, rm, tp, tm, [e], rho
+  // rings r = i-2 .. i+2 -> slot s = r - i + 2
+  double rho[5], W[6][5], vrr[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    int r = i - 2 + s;
+    vrr[s] = (r >= 0 && r <= nr) ? ld<T>(a.vr, (long)r * ns + j) : 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    int r = i - 2 + s;
+    if (r < 0 || r > nr - 1) {
+      rho[s] = 0.0;
+      for (int q = 0; q < 6; q++) W[q][s] = 0.0;
+      continue;
+    }
+    const long l = (long)r * ns + j;
+    double d = ld<T>(a.rho, l);
+    double Q[6];
+    Q[0] = d * vrr[s + 1];
+    Q[1] = d * vrr[s];
+    Q[2] = d * (ld<T>(a.vt, (long)r * ns + jp) + a.Rmed[r] * Of) * a.Rmed[r];
+    Q[3] = d * (ld<T>(a.vt, l) + a.Rmed[r] * Of) * a.Rmed[r];
+    Q[4] = a.energy ? ld<T>(a.en, l) : 0.0;
+    Q[5] = d;
+    rho[s] = d;
+    for (int q = 0; q < 6; q++) W[q][s] = Q[q] / (d + 1e-20);
+  }
+  // slope (ComputeStarRad dq) at slot s (ring i-2+s), zero at rings 0, nr-1
+  auto slope = [&](const double *qb, int s) -> double {
+    int r = i - 2 + s;
+    if (r <= 0 || r >= nr - 1) return 0.0;
+    double dqm = (qb[s] - qb[s - 1]) * a.InvDiffRmed[r];
+    double dqp = (qb[s + 1] - qb[s]) * a.InvDiffRmed[r + 1];
+    return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
+  };
+  // star value at face of ring r = i (slot 2) or i+1 (slot 3)
+  auto star = [&](const double *qb, int s) -> double {
+    int r = i - 2 + s;
+    if (r <= 0 || r >= nr) return 0.0;
+    double v = vrr[s];
+    if (v > 0.0) return qb[s - 1] + (a.Rmed[r] - a.Rmed[r - 1] - v * dt) * 0.5 * slope(qb, s - 1);
+    return qb[s] - (a.Rmed[r + 1] - a.Rmed[r] + v * dt) * 0.5 * slope(qb, s);
+  };
+  const double dtheta = a.pm / (double)ns;
+  const double rs0 = star(rho, 2), rs1 = star(rho, 3);
+  const long l = (long)i * ns + j;
+  void *outp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
+  for (int q = 0; q < 6; q++) {
+    if (q == 4 && !a.energy) continue;
+    double qs0 = star(W[q], 2), qs1 = star(W[q], 3);
+    double varq = dt * dtheta * a.Rinf[i] * qs0 * rs0 * vrr[2];
+    varq -= dt * dtheta * a.Rsup[i] * qs1 * rs1 * vrr[3];
+    // Q itself = W * (rho + 1e-20) is not reconstructed: reload it
+    double Qi;
+    const double d = rho[2];
+    if (q == 0) Qi = d * vrr[3];
+    else if (q == 1) Qi = d * vrr[2];
+    else if (q == 2) Qi = d * (ld<T>(a.vt, (long)i * ns + jp) + a.Rmed[i] * Of) * a.Rmed[i];
+    else if (q == 3) Qi = d * (ld<T>(a.vt, l) + a.Rmed[i] * Of) * a.Rmed[i];
+    else if (q == 4) Qi = ld<T>(a.en, l);
+    else Qi = d;
+    Qi += varq * a.InvSurf[i];
+    st<T>(outp[q], l, Qi);
+    if (q == 5 && i == 0) a.lost_col[j] = varq;
+  }
+  (void)nq;
+}
+
+// Azimuthal van Leer with FARGO (OneWindTheta, src/TransportEuler.c:225-465)
+// for one ring, in registers; Work and slope arrays go through LDS for the
+// j-1 / j+1 neighbours.  The integer shift is applied on the final store.
+template <typename T, int K> struct AziRing {
+  double *sW, *sD;
+  int t, TB, ns;
+  __device__ int jj(int k) const { return t + k * TB; }
+};
+
+template <typename T, int K>
+__device__ void azi_star(AziRing<T, K> &R, const double (&W)[K], const double (&v)[K], double dt,
+                         double dxtheta, double invdxtheta, double (&out)[K]) {
+  const int ns = R.ns;
+#pragma unroll
+  for (int k = 0; k < K; k++) R.sW[R.jj(k)] = W[k];
+  __syncthreads();
+  double dq[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int j = R.jj(k), jm = (j == 0) ? ns - 1 : j - 1, jp = (j == ns - 1) ? 0 : j + 1;
+    double dqm = (W[k] - R.sW[jm]), dqp = (R.sW[jp] - W[k]);
+    dq[k] = (dqp * dqm > 0.0) ? dqp * dqm / (dqp + dqm) * invdxtheta : 0.0;
+    R.sD[j] = dq[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int j = R.jj(k), jm = (j == 0) ? ns - 1 : j - 1;
+    double ksi = v[k] * dt;
+    if (ksi > 0.0) out[k] = R.sW[jm] + (dxtheta - ksi) * R.sD[jm];
+    else out[k] = W[k] - (dxtheta + ksi) * dq[k];
+  }
+  __syncthreads();
+}
+
+template <typename T, int K>
+__device__ void azi_vanleer(AziRing<T, K> &R, double (&Q)[K], const double (&RI)[K],
+                            const double (&RS)[K], const double (&v)[K], double dt, double dxtheta,
+                            double invdxtheta, double dxrad, double invsurf) {
+  const int ns = R.ns;
+  double W[K], qs[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) W[k] = Q[k] / (RI[k] + 1e-20);
+  azi_star<T, K>(R, W, v, dt, dxtheta, invdxtheta, qs);
+  double f[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    f[k] = dxrad * qs[k] * RS[k] * v[k];
+    R.sD[R.jj(k)] = f[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int j = R.jj(k), jp = (j == ns - 1) ? 0 : j + 1;
+    double varq = f[k];
+    varq -= R.sD[jp];
+    Q[k] += varq * invsurf;
+  }
+  __syncthreads();
+}
+
+template <typename T, int K> __global__ void __launch_bounds__(1024) k_transport_azi(KArgs a) {
+  extern __shared__ double lds[];
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = ring_of_block(a), ns = a.ns, t = threadIdx.x, TB = blockDim.x;
+  AziRing<T, K> R;
+  R.sW = lds;
+  R.sD = lds + ns;
+  R.t = t;
+  R.TB = TB;
+  R.ns = ns;
+  double *sh = lds + 2 * ns;
+  const long base = (long)i * ns;
+  const double dt = c->dt, invdt = 1.0 / dt;
+  // lost mass through the inner edge (VanLeerRadial's LostByDisk)
+  if (i == 0) {
+    double s = 0.0;
+    for (int j = t; j < ns; j += TB) s += a.lost_col[j];
+    s = block_sum(s, sh);
+    if (t == 0) a.ctrl->lost_mass += s;
+  }
+  // ComputeThetaElongations / ComputeAverageThetaVelocities / Residuals
+  double vres[K];
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double elong = ld<T>(a.vt, base + R.jj(k)) * dt;
+    vres[k] = elong;
+    s += elong * invdt;
+  }
+  s = block_sum(s, sh);
+  const double vmed = s / (double)ns;
+#pragma unroll
+  for (int k = 0; k < K; k++) vres[k] = vres[k] * invdt - vmed;
+  // ComputeConstantResidual (src/TransportEuler.c:292-348)
+  const double dpinvns = a.pm / (double)ns;
+  const double Ntilde = vmed * a.InvRmed[i] * dt * (double)ns / a.pm;
+  const double Nround = floor(Ntilde + 0.5);
+  const long nshift = (long)Nround;
+  double u = (Ntilde - Nround) * a.Rmed[i] * invdt * dpinvns;
+  if (t == 0) {
+    a.vmed[i] = vmed;
+    a.nshift[i] = nshift;
+  }
+  if (!a.fast) {
+#pragma unroll
+    for (int k = 0; k < K; k++) vres[k] = u + vres[k];
+    u = 0.0;
+  }
+  const double dxtheta = a.pm / (double)ns * a.Rmed[i], invdxtheta = 1.0 / dxtheta;
+  const double dxrad = (a.Rsup[i] - a.Rinf[i]) * dt, invsurf = 1.0 / a.Surf[i];
+  // quantities after the radial pass: rp, rm, tp, tm, [e], rho
+  void *qp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
+  double Q[6][K], RI[K], RS[K];
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      Q[q][k] = (q == 4 && !a.energy) ? 0.0 : ld<T>(qp[q], base + R.jj(k));
+  // residual pass: QuantitiesAdvection(Rho, VthetaRes)
+#pragma unroll
+  for (int k = 0; k < K; k++) RI[k] = Q[5][k];
+  azi_star<T, K>(R, RI, vres, dt, dxtheta, invdxtheta, RS);
+  for (int q = 0; q < 6; q++) {
+    if (q == 4 && !a.energy) continue;
+    azi_vanleer<T, K>(R, Q[q], RI, RS, vres, dt, dxtheta, invdxtheta, dxrad, invsurf);
+  }
+  // uniform pass: QuantitiesAdvection(Rho, Vtheta = constant residual)
+  if (a.fast) {
+    double uu[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      RI[k] = Q[5][k];
+      uu[k] = u;
+    }
+    azi_star<T, K>(R, RI, uu, dt, dxtheta, invdxtheta, RS);
+    for (int q = 0; q < 6; q++) {
+      if (q == 4 && !a.energy) continue;
+      azi_vanleer<T, K>(R, Q[q], RI, RS, uu, dt, dxtheta, invdxtheta, dxrad, invsurf);
+    }
+  }
+  // AdvectSHIFT (src/TransportEuler.c:350-373): new[j] = old[j - Nshift]
+  long sh_mod = nshift % ns;
+  if (sh_mod < 0) sh_mod += ns;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    if (q == 4 && !a.energy) continue;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      long jn = R.jj(k) + sh_mod;
+      if (jn >= ns) jn -= ns;
+      st<T>(qp[q], base + jn, Q[q][k]);
+    }
+  }
+}
+
+// ComputeVelocities (src/TransportEuler.c:515-547) and the floors
+// (src/SourceEuler.c:2506-2541): rebuilds vr, vt from the transported
+// momenta and writes the new state
+template <typename T> __global__ void __launch_bounds__(256) k_velocities(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const long l = (long)i * ns + j, lim = l - ns, ljm = (j == 0) ? (long)i * ns + ns - 1 : l - 1;
+  const double rho = ld<T>(a.t_rho, l);
+  double vr = 0.0;
+  if (i != 0) vr = (ld<T>(a.t_rp, lim) + ld<T>(a.t_rm, l)) / (rho + ld<T>(a.t_rho, lim) + 1e-20);
+  double vt = (ld<T>(a.t_tp, ljm) + ld<T>(a.t_tm, l)) / (rho + ld<T>(a.t_rho, ljm) + 1e-20) / a.Rmed[i] -
+              a.Rmed[i] * c->omega_frame;
+  st<T>(a.vr, l, vr);
+  st<T>(a.vt, l, vt);
+  double d = rho;
+  if (a.floors && d < a.floor_dens) d = a.floor_dens;
+  st<T>(a.rho, l, d);
+  if (a.energy) {
+    double e = ld<T>(a.t_en, l);
+    double e_bottom = 1e-3 * a.EnergyMed[i];
+    if (a.floors && e < e_bottom) e = e_bottom;
+    st<T>(a.en, l, e);
+  }
+}
+
+// ApplyBoundaryCondition (src/SideEuler.c:1102-1131): wall or open edges,
+// and the sub-Keplerian reset of vt in rings 0 and nrad-1, which runs for
+// every boundary type (src/SideEuler.c:1110-1112)
+template <typename T> __global__ void __launch_bounds__(256) k_boundary(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  if (a.bc == DFARGO_BC_WALL) {
+    int i = 2;
+    st<T>(a.vr, (long)(i - 1) * ns + j, 0.0);
+    st<T>(a.vr, (long)(i - 2) * ns + j, -ld<T>(a.vr, (long)i * ns + j));
+    st<T>(a.rho, (long)(i - 1) * ns + j, a.SigmaMed[i - 1]);
+    st<T>(a.rho, (long)(i - 2) * ns + j, a.SigmaMed[i - 2]);
+    i = nr - 3;
+    st<T>(a.vr, (long)(i + 2) * ns + j, 0.0);
+    st<T>(a.vr, (long)(i + 3) * ns + j, -ld<T>(a.vr, (long)(i + 1) * ns + j));
+    st<T>(a.rho, (long)(i + 1) * ns + j, a.SigmaMed[i + 1]);
+    st<T>(a.rho, (long)(i + 2) * ns + j, a.SigmaMed[i + 2]);
+  }
+  st<T>(a.vt, j, a.VthetaMed[0] - a.Rmed[0] * c->omega_frame);
+  st<T>(a.vt, (long)(nr - 1) * ns + j, a.VthetaMed[nr - 1] - a.Rmed[nr - 1] * c->omega_frame);
+  if (a.bc == DFARGO_BC_OPEN) {
+    // OpenBoundary (src/SideEuler.c:300-408), standard prescription
+    int i = 2;
+    long l = (long)i * ns + j;
+    double rl = ld<T>(a.rho, l);
+    st<T>(a.rho, l - ns, rl);
+    st<T>(a.rho, l - 2 * ns, rl);
+    if (a.energy) {
+      double e = ld<T>(a.en, l);
+      st<T>(a.en, l - ns, e);
+      st<T>(a.en, l - 2 * ns, e);
+    }
+    double vtl = ld<T>(a.vt, l);
+    st<T>(a.vt, l - ns, vtl * a.open_vt_in1[0]);
+    st<T>(a.vt, l - 2 * ns, vtl * a.open_vt_in2[0]);
+    double vrp = ld<T>(a.vr, l + ns);
+    if ((vrp >= 0.0) || (rl < a.SigmaMed[0])) {
+      st<T>(a.vr, l, 0.0);
+      st<T>(a.vr, l - ns, 0.0);
+    } else {
+      st<T>(a.vr, l, vrp);
+      st<T>(a.vr, l - ns, vrp);
+    }
+    i = nr - 1;
+    l = (long)i * ns + j;
+    double rm = ld<T>(a.rho, l - ns);
+    st<T>(a.rho, l, rm);
+    if (a.energy) st<T>(a.en, l, ld<T>(a.en, l - ns));
+    st<T>(a.vt, l, ld<T>(a.vt, l - ns) * a.open_vt_out[0]);
+    double vrm = ld<T>(a.vr, l - ns);
+    if ((vrm < 0.0) || (rm < a.SigmaMed[nr - 2])) st<T>(a.vr, l, 0.0);
+    else st<T>(a.vr, l, vrm);
+  }
+}
+
+// Potential grid on demand (the field of FillForcesArrays)
+template <typename T> __global__ void k_potential(KArgs a) {
+  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  st<T>(a.pot, (long)i * ns + j, pot_at(a, a.ctrl, i, j));
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+
+static int ring_k(int ns) {
+  static const int ks[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  for (int k : ks)
+    if (ns % k == 0 && ns / k <= 1024) return k;
+  return -1;
+}
+
+int ring_kcheck(int ns) { return ring_k(ns); }
+
+#define RING_DISPATCH(KERNEL, T, a, s, lds, ...)                                            \
+  do {                                                                                      \
+    int K_ = ring_k((a).ns);                                                                \
+    dim3 g_((a).nr), b_((a).ns / K_);                                                       \
+    switch (K_) {                                                                           \
+      case 1: hipLaunchKernelGGL((KERNEL<T, 1>), g_, b_, lds, s, a, ##__VA_ARGS__); break;  \
+      case 2: hipLaunchKernelGGL((KERNEL<T, 2>), g_, b_, lds, s, a, ##__VA_ARGS__); break;  \
+      case 3: hipLaunchKernelGGL((KERNEL<T, 3>), g_, b_, lds, s, a, ##__VA_ARGS__); break;  \
+      case 4: hipLaunchKernelGGL((KERNEL<T, 4>), g_, b_, lds, s, a, ##__VA_ARGS__); break;  \
+      case 6: hipLaunchKernelGGL((KERNEL<T, 6>), g_, b_, lds, s, a, ##__VA_ARGS__); break;  \
+      case 8: hipLaunchKernelGGL((KERNEL<T, 8>), g_, b_, lds, s, a, ##__VA_ARGS__); break;  \
+      case 12: hipLaunchKernelGGL((KERNEL<T, 12>), g_, b_, lds, s, a, ##__VA_ARGS__); break; \
+      case 16: hipLaunchKernelGGL((KERNEL<T, 16>), g_, b_, lds, s, a, ##__VA_ARGS__); break; \
+    }                                                                                       \
+  } while (0)
+
+static dim3 cell_grid(const KArgs &a, int rings) {
+  int bx = a.ns < 256 ? a.ns : 256;
+  return dim3((a.ns + bx - 1) / bx, rings);
+}
+static dim3 cell_block(const KArgs &a) { return dim3(a.ns < 256 ? a.ns : 256); }
+
+template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) {
+  switch (id) {
+    case KID_BEGIN: hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, s, a); break;
+    case KID_CFL: RING_DISPATCH(k_cfl_ring, T, a, s, 0); break;
+    case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s, a); break;
+    case KID_EOS: RING_DISPATCH(k_eos, T, a, s, 0, 0); break;
+    case KID_SUBSTEP1:
+      hipLaunchKernelGGL(k_substep1<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
+      break;
+    case KID_VISC_STRESS:
+      hipLaunchKernelGGL(k_visc_stress<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a, 0);
+      break;
+    case KID_VISC_UPDATE:
+      hipLaunchKernelGGL(k_visc_update<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
+      break;
+    case KID_SUBSTEP2:
+      hipLaunchKernelGGL(k_substep2<T>, cell_grid(a, a.nr + 1), cell_block(a), 0, s, a);
+      break;
+    case KID_ENERGY: RING_DISPATCH(k_energy, T, a, s, 0); break;
+    case KID_TRANSPORT_RAD:
+      hipLaunchKernelGGL(k_transport_rad<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
+      break;
+    case KID_TRANSPORT_AZI: {
+      size_t lds = sizeof(double) * (2 * (size_t)a.ns + 1024);
+      RING_DISPATCH(k_transport_azi, T, a, s, lds);
+      break;
+    }
+    case KID_VELOCITIES:
+      hipLaunchKernelGGL(k_velocities<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
+      break;
+    case KID_BOUNDARY:
+      hipLaunchKernelGGL(k_boundary<T>, dim3((a.ns + 255) / 256), dim3(256), 0, s, a);
+      break;
+    default: launch_sg_kernel<T>(id, a, s); break;
+  }
+}
+
+template <typename T> void launch_eos_pre(const KArgs &a, hipStream_t s) {
+  RING_DISPATCH(k_eos, T, a, s, 0, 1);
+}
+template <typename T> void launch_potential(const KArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_potential<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
+}
+template <typename T> void launch_visc_stress_state(const KArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_visc_stress<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a, 1);
+}
+void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s) {
+  hipLaunchKernelGGL(k_algogas_begin, dim3(1), dim3(64), 0, s, a, DT);
+}
+
+template void launch_kernel<double>(int, const KArgs &, hipStream_t);
+template void launch_kernel<float>(int, const KArgs &, hipStream_t);
+template void launch_eos_pre<double>(const KArgs &, hipStream_t);
+template void launch_eos_pre<float>(const KArgs &, hipStream_t);
+template void launch_potential<double>(const KArgs &, hipStream_t);
+template void launch_potential<float>(const KArgs &, hipStream_t);
+template void launch_visc_stress_state<double>(const KArgs &, hipStream_t);
+template void launch_visc_stress_state<float>(const KArgs &, hipStream_t);

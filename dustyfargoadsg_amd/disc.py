@@ -1,0 +1,172 @@
+"""Host-side mirror of the reference's hydro entry points over the C ABI.
+
+A `Disc` owns one dfargo context: the reference's InitEuler / AlgoGas /
+compute_selfgravity calls map to its methods, with the state as numpy FP64
+arrays in the reference's polar layout (l = j + i*nsec, (nrad+1)*nsec).
+Errors raise DfargoError with the library's message; there is no CPU
+fallback, so a missing or unloadable library raises at construction.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import Params, Planets, Status, FIELDS
+
+
+class DfargoError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("fields must be C-contiguous float64 arrays")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(lib, rc):
+    if rc != _capi.OK:
+        raise DfargoError(f"dfargo error {rc}: {lib.dfargo_last_error().decode()}")
+
+
+def make_planets(params, spec, lib=None):
+    """InitPlanetarySystem (src/Psys.c:96-196) from .cfg columns."""
+    lib = lib or _capi.load()
+    nb = len(spec["dist"])
+    out = Planets()
+    d = np.array(list(spec["dist"]) + [0.0], dtype=np.float64)
+    m = np.array(list(spec["mass"]) + [0.0], dtype=np.float64)
+    a = np.array(list(spec["acc"]) + [0.0], dtype=np.float64)
+    fd = (ctypes.c_int * (nb + 1))(*(list(spec["feel_disk"]) + [0]))
+    fo = (ctypes.c_int * (nb + 1))(*(list(spec["feel_others"]) + [0]))
+    _check(lib, lib.dfargo_init_planetary_system(ctypes.byref(params), nb, _ptr(d), _ptr(m),
+                                                  _ptr(a), fd, fo, ctypes.byref(out)))
+    return out
+
+
+def initial_fields(params, lib=None):
+    """InitGasDensity and InitGasEnergy (src/Pframeforce.c:297-440)."""
+    lib = lib or _capi.load()
+    n = (params.nrad + 1) * params.nsec
+    rho, e = np.zeros(n), np.zeros(n)
+    _check(lib, lib.dfargo_init_gas_density(ctypes.byref(params), _ptr(rho)))
+    if params.energy_equation:
+        _check(lib, lib.dfargo_init_gas_energy(ctypes.byref(params), _ptr(e)))
+    return rho, e
+
+
+class Disc:
+    """One disc on one GPU."""
+
+    def __init__(self, params, precision=_capi.FP64, device=0, lib_path=None):
+        self.lib = _capi.load(lib_path)
+        self.p = params.copy()
+        self.nr, self.ns = params.nrad, params.nsec
+        self.n = (self.nr + 1) * self.ns
+        self.precision = precision
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.dfargo_create(ctypes.byref(self.p), precision, device, ctypes.byref(h)))
+        self.ctx = h
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.dfargo_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def zeros(self):
+        return np.zeros(self.n, dtype=np.float64)
+
+    # -- reference entry points ------------------------------------------
+    def init_euler(self, rho, energy=None, planets=None):
+        """InitEuler with the start-up self-gravity call; returns (vr, vt)."""
+        vr, vt = self.zeros(), self.zeros()
+        _check(self.lib, self.lib.dfargo_init_euler(
+            self.ctx, _ptr(np.ascontiguousarray(rho, dtype=np.float64)), _ptr(vr), _ptr(vt),
+            _ptr(None if energy is None else np.ascontiguousarray(energy, dtype=np.float64)),
+            ctypes.byref(planets) if planets is not None else None))
+        return vr, vt
+
+    def initial_state(self, planet_spec=None):
+        rho, e = initial_fields(self.p, self.lib)
+        sys_ = make_planets(self.p, planet_spec or dict(dist=[], mass=[], acc=[], feel_disk=[],
+                                                        feel_others=[]), self.lib)
+        vr, vt = self.init_euler(rho, e if self.p.energy_equation else None, sys_)
+        return dict(rho=rho, vr=vr, vt=vt, energy=e, planets=sys_)
+
+    def algogas(self, DT, state=None):
+        """AlgoGas: with `state` the host arrays are uploaded and updated in
+        place (the reference's calling convention); without it the device
+        state is advanced with no transfer."""
+        if state is None:
+            _check(self.lib, self.lib.dfargo_algogas_device(self.ctx, DT))
+            return
+        e = state.get("energy")
+        _check(self.lib, self.lib.dfargo_algogas(
+            self.ctx, DT, _ptr(state["rho"]), _ptr(state["vr"]), _ptr(state["vt"]),
+            _ptr(e if (e is not None and self.p.energy_equation) else None),
+            ctypes.byref(state["planets"]) if state.get("planets") is not None else None))
+
+    def set_state(self, st):
+        e = st.get("energy")
+        _check(self.lib, self.lib.dfargo_upload_state(
+            self.ctx, _ptr(st["rho"]), _ptr(st["vr"]), _ptr(st["vt"]),
+            _ptr(e if (e is not None and self.p.energy_equation) else None),
+            ctypes.byref(st["planets"]) if st.get("planets") is not None else None))
+
+    def state(self):
+        rho, vr, vt, e = self.zeros(), self.zeros(), self.zeros(), self.zeros()
+        sys_ = Planets()
+        _check(self.lib, self.lib.dfargo_download_state(self.ctx, _ptr(rho), _ptr(vr), _ptr(vt),
+                                                        _ptr(e), ctypes.byref(sys_)))
+        return dict(rho=rho, vr=vr, vt=vt, energy=e, planets=sys_)
+
+    def status(self):
+        st = Status()
+        _check(self.lib, self.lib.dfargo_get_status(self.ctx, ctypes.byref(st)))
+        return st
+
+    def set_time(self, t, t0=0.0, omega_frame=0.0):
+        _check(self.lib, self.lib.dfargo_set_time(self.ctx, t, t0, omega_frame))
+
+    def field(self, name):
+        out = self.zeros()
+        _check(self.lib, self.lib.dfargo_get_field(self.ctx, FIELDS[name], _ptr(out)))
+        return out
+
+    def compute_selfgravity(self):
+        _check(self.lib, self.lib.dfargo_compute_selfgravity(self.ctx))
+        return self.field("sg_accr"), self.field("sg_acct")
+
+    def fargo_shift(self):
+        ns = (ctypes.c_long * self.nr)()
+        vm = np.zeros(self.nr)
+        _check(self.lib, self.lib.dfargo_get_fargo_shift(self.ctx, ns, _ptr(vm)))
+        return np.array(ns[:], dtype=np.int64), vm
+
+    def fviscosity(self, r):
+        return self.lib.dfargo_fviscosity(self.ctx, r)
+
+    # -- measurement --------------------------------------------------------
+    def set_profiling(self, on):
+        _check(self.lib, self.lib.dfargo_set_profiling(self.ctx, 1 if on else 0))
+
+    def kernel_times(self):
+        maxn = 32
+        names = (ctypes.c_char * 48 * maxn)()
+        tot = np.zeros(maxn)
+        cnt = (ctypes.c_long * maxn)()
+        n = ctypes.c_int()
+        _check(self.lib, self.lib.dfargo_get_kernel_times(self.ctx, maxn, names, _ptr(tot), cnt,
+                                                          ctypes.byref(n)))
+        return {bytes(names[k]).split(b"\0")[0].decode(): (tot[k], cnt[k]) for k in range(n.value)}
+
+    def stream(self):
+        return self.lib.dfargo_stream(self.ctx)

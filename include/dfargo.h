@@ -1,0 +1,247 @@
+/*
+ * dfargo.h -- C ABI of the MI355X-native Dusty FARGO-ADSG gas hydro step.
+ *
+ * Every entry point takes plain pointers, sizes and the parameter block
+ * below; no HIP or torch type appears in a signature.  Field arrays use the
+ * reference's polar layout: ring-major doubles, l = j + i*nsec, with
+ * (nrad+1)*nsec entries (ring nrad holds the outer-interface radial velocity).
+ *
+ * Each entry point names the reference function it replaces (file:line in
+ * the Dusty FARGO-ADSG sources).  Errors: a non-zero DFARGO_ERR_* return and
+ * a message from dfargo_last_error(); nothing calls exit().
+ */
+#ifndef DFARGO_H
+#define DFARGO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFARGO_MAX_PLANETS 8
+
+enum dfargo_status_code {
+  DFARGO_OK = 0,
+  DFARGO_ERR_ARG = 1,          /* invalid argument or size                  */
+  DFARGO_ERR_UNSUPPORTED = 2,  /* parameter combination not implemented     */
+  DFARGO_ERR_HIP = 3,          /* HIP runtime failure (message has detail)   */
+  DFARGO_ERR_ALLOC = 4,        /* host or device allocation failed           */
+  DFARGO_ERR_STATE = 5         /* call made before the required set-up call  */
+};
+
+enum dfargo_precision {
+  DFARGO_FP64 = 0,   /* fields stored and stenciled in double            */
+  DFARGO_FP32 = 1    /* fields in float; FP64 for ring means, force sums,
+                        planets, dt and budgets                          */
+};
+
+enum dfargo_boundary {
+  DFARGO_BC_WALL = 0,  /* INNERBOUNDARY WALL (default, src/var.c:20)       */
+  DFARGO_BC_OPEN = 1   /* INNERBOUNDARY OPEN (src/SideEuler.c:300-408)     */
+};
+
+/* Parameter block: the .par keys the hydro step reads (src/var.c:15-202),
+ * already decoded the way src/Interpret.c:165-476 decodes them. */
+typedef struct dfargo_params {
+  /* grid */
+  int    nrad, nsec;                 /* NRAD, NSEC                          */
+  double rmin, rmax;                 /* RMIN, RMAX                          */
+  int    log_grid;                   /* RADIALSPACING starts with L         */
+  double pmin, pmax;                 /* PMIN, PMAX                          */
+  /* disc */
+  double sigma0, sigma_slope;        /* SIGMA0, SIGMASLOPE                  */
+  double aspect_ratio, flaring_index;/* ASPECTRATIO, FLARINGINDEX           */
+  double viscosity, alpha_viscosity; /* VISCOSITY, ALPHAVISCOSITY           */
+  double cavity_radius, cavity_ratio, cavity_width;          /* CAVITY*    */
+  double transition_radius, transition_ratio, transition_width;
+  double imposed_disk_drift;         /* IMPOSEDDISKDRIFT                    */
+  int    add_noise;                  /* ADDNOISE                            */
+  long   noise_seed;                 /* value srand48() is seeded with      */
+  /* thermodynamics */
+  int    energy_equation;            /* ENERGYEQUATION                      */
+  int    viscous_heating;            /* VISCOUSHEATING (default YES)        */
+  int    beta_cooling;               /* BETACOOLING                         */
+  double adiabatic_index;            /* ADIABATICINDEX                      */
+  double beta_cooling_time, beta_cooling_slope;
+  /* self-gravity */
+  int    self_gravity;               /* SELFGRAVITY YES (Z mode not here)   */
+  double sg_thickness_smoothing;     /* SGTHICKNESSSMOOTHING                */
+  /* planets and forces */
+  double thickness_smoothing;        /* THICKNESSSMOOTHING                  */
+  double roche_smoothing;            /* ROCHESMOOTHING                      */
+  double indirect_term_smoothing;    /* INDIRECTTERMSMOOTHING               */
+  double mass_taper;                 /* MASSTAPER (orbits)                  */
+  int    indirect_term;              /* INDIRECTTERM (default YES)          */
+  int    discard_gas_indirect_term;  /* DISCARDGASINDIRECTTERM              */
+  int    alternative_gas_indirect_term;
+  int    exclude_hill;               /* EXCLUDEHILL                         */
+  double exclude_hill_factor;        /* EXCLUDEHILLFACTOR                   */
+  int    forced_circular;            /* FORCEDCIRCULAR                      */
+  double release_date, release_radius;
+  double eccentricity;               /* ECCENTRICITY                        */
+  /* frame */
+  int    corotating;                 /* FRAME C (only 0 is implemented)     */
+  double omega_frame;                /* OMEGAFRAME                          */
+  /* numerics */
+  int    fast_transport;             /* TRANSPORT FARGO (1) or STANDARD (0) */
+  int    add_floors;                 /* ADDFLOORS (default YES)             */
+  double density_floor;              /* DENSITYFLOOR                        */
+  int    boundary;                   /* enum dfargo_boundary                */
+} dfargo_params;
+
+/* PlanetarySystem (src/types.h:97-111) plus the two mass tables that
+ * src/global.h:33 keeps beside it. */
+typedef struct dfargo_planets {
+  int    nb;
+  double mass[DFARGO_MAX_PLANETS], x[DFARGO_MAX_PLANETS], y[DFARGO_MAX_PLANETS];
+  double vx[DFARGO_MAX_PLANETS], vy[DFARGO_MAX_PLANETS], acc[DFARGO_MAX_PLANETS];
+  int    feel_disk[DFARGO_MAX_PLANETS], feel_others[DFARGO_MAX_PLANETS];
+  int    binary[DFARGO_MAX_PLANETS];
+  double initial_mass[DFARGO_MAX_PLANETS], final_mass[DFARGO_MAX_PLANETS];
+} dfargo_planets;
+
+/* Scalar globals AlgoGas leaves host-coherent (src/global.h:32-42,
+ * src/SourceEuler.c:38, src/TransportEuler.c:27). */
+typedef struct dfargo_status {
+  double physical_time, physical_time_initial, omega_frame;
+  double mindt, maxdt, lost_mass, mass_taper;
+  double disk_on_primary_x, disk_on_primary_y;         /* DiskOnPrimaryAcceleration     */
+  double disk_on_primary_alt_x, disk_on_primary_alt_y; /* DiskOnPrimaryAcceleration_alt */
+  double bary_x[3], bary_y[3];       /* stardiscbarycenter_x/y               */
+  long   substeps_total;             /* hydro sub-steps since create         */
+  int    last_call_substeps;         /* sub-steps of the last AlgoGas call   */
+  int    gas_timesteps_cfl;          /* GasTimeStepsCFL of the last sub-step */
+  int    crashed_density;            /* DetectCrash(Rho), last sub-step      */
+  int    crashed_energy;             /* DetectCrash(Energy), last sub-step   */
+  int    crashed;                    /* the reference's Crashed flag (the
+                                        energy scan overwrites the density
+                                        scan, src/SourceEuler.c:545-548)     */
+} dfargo_status;
+
+/* Derived grids the host may read back (src/Output.c:234-284). */
+enum dfargo_field {
+  DFARGO_FIELD_POTENTIAL = 0,
+  DFARGO_FIELD_SOUNDSPEED = 1,
+  DFARGO_FIELD_PRESSURE = 2,
+  DFARGO_FIELD_TEMPERATURE = 3,
+  DFARGO_FIELD_DIVV = 4,
+  DFARGO_FIELD_VISCHEAT = 5,
+  DFARGO_FIELD_SG_ACCR = 6,      /* gr     (src/sgacc.c:157)  */
+  DFARGO_FIELD_SG_ACCT = 7,      /* gtheta (src/sgacc.c:158)  */
+  DFARGO_FIELD_TAURR = 8,
+  DFARGO_FIELD_TAURP = 9,
+  DFARGO_FIELD_TAUPP = 10
+};
+
+typedef struct dfargo_ctx dfargo_ctx;
+
+/* ---- host-side reference functions (pure CPU, glibc math) ---------------- */
+
+/* Defaults of src/var.c:15-202 for every field of dfargo_params. */
+void dfargo_default_params(dfargo_params *p);
+
+/* Validates a parameter block; returns DFARGO_ERR_UNSUPPORTED for options
+ * this build does not implement.  Replaces the checks in
+ * src/Interpret.c:399-476. */
+int dfargo_check_params(const dfargo_params *p);
+
+/* FillPolar1DArrays (src/SourceEuler.c:71-188) without the file output.
+ * radii: nrad+1; rmed, surf: nrad; azimuth: nsec. */
+int dfargo_fill_polar_1d_arrays(const dfargo_params *p, double *radii,
+                                double *rmed, double *surf, double *azimuth);
+
+/* InitGasDensity (src/Pframeforce.c:297-341), standard power law with the
+ * optional AddNoise draw (drand48 seeded with p->noise_seed). */
+int dfargo_init_gas_density(const dfargo_params *p, double *rho);
+
+/* InitGasEnergy (src/Pframeforce.c:422-440). */
+int dfargo_init_gas_energy(const dfargo_params *p, double *energy);
+
+/* InitPlanetarySystem (src/Psys.c:96-196) after the .cfg file has been
+ * parsed into per-planet columns. */
+int dfargo_init_planetary_system(const dfargo_params *p, int nb,
+                                 const double *dist, const double *mass,
+                                 const double *acc, const int *feel_disk,
+                                 const int *feel_others, dfargo_planets *out);
+
+/* AspectRatio (src/Viscosity.c:107-122) at PhysicalTime = initial time. */
+double dfargo_aspect_ratio(const dfargo_params *p, double r);
+
+/* ---- GPU context ----------------------------------------------------------- */
+
+/* Allocates the device state for one disc and uploads the 1-D tables:
+ * the device half of FillPolar1DArrays plus the allocations of InitEuler
+ * (src/SourceEuler.c:190-291), InitTransport (src/TransportEuler.c:32-59)
+ * and InitViscosity (src/Viscosity.c:141-157).  precision: enum
+ * dfargo_precision.  device: HIP device ordinal. */
+int dfargo_create(const dfargo_params *p, int precision, int device,
+                  dfargo_ctx **out);
+int dfargo_destroy(dfargo_ctx *ctx);
+const char *dfargo_last_error(void);
+
+/* InitEuler (src/SourceEuler.c:190-291) together with the start-up
+ * self-gravity call of src/main.c:178-186: computes sound speed, pressure,
+ * temperature, the initial velocities (InitGasVelocities,
+ * src/Pframeforce.c:442-641, written into vrad/vtheta) and the viscous
+ * stresses, and makes rho, vrad, vtheta, energy and sys the device state. */
+int dfargo_init_euler(dfargo_ctx *ctx, const double *rho, double *vrad,
+                      double *vtheta, const double *energy,
+                      const dfargo_planets *sys);
+
+/* AlgoGas (src/SourceEuler.c:326-719): advances the state by DT with as
+ * many CFL sub-steps as the reference takes.  The host arrays are uploaded
+ * before and written back after the call, as the reference host expects
+ * (src/main.c:387).  energy may be NULL when the energy equation is off;
+ * sys may be NULL when there are no planets. */
+int dfargo_algogas(dfargo_ctx *ctx, double DT, double *rho, double *vrad,
+                   double *vtheta, double *energy, dfargo_planets *sys);
+
+/* AlgoGas on the device-resident state, with no host transfer.  Used by a
+ * host that keeps the state in HBM between calls. */
+int dfargo_algogas_device(dfargo_ctx *ctx, double DT);
+
+/* Explicit transfers of the state (rho, vrad, vtheta, energy: (nrad+1)*nsec
+ * doubles each, energy may be NULL) and of the planetary system. */
+int dfargo_upload_state(dfargo_ctx *ctx, const double *rho, const double *vrad,
+                        const double *vtheta, const double *energy,
+                        const dfargo_planets *sys);
+int dfargo_download_state(dfargo_ctx *ctx, double *rho, double *vrad,
+                          double *vtheta, double *energy, dfargo_planets *sys);
+
+/* Scalar globals after the last call; set_time overrides PhysicalTime,
+ * PhysicalTimeInitial and OmegaFrame (restart path, src/main.c:262-272). */
+int dfargo_get_status(dfargo_ctx *ctx, dfargo_status *st);
+int dfargo_set_time(dfargo_ctx *ctx, double physical_time,
+                    double physical_time_initial, double omega_frame);
+
+/* Copies a derived grid (enum dfargo_field) to out ((nrad+1)*nsec doubles).
+ * Temperature is computed on demand (ComputeTemperatureField,
+ * src/SourceEuler.c:2436-2460), the others hold their values from the last
+ * sub-step. */
+int dfargo_get_field(dfargo_ctx *ctx, int field, double *out);
+
+/* compute_selfgravity(..., SGUpdate = NO) (src/sgmain.c:3-35) on the
+ * device-resident density: refreshes gr/gtheta without a velocity kick. */
+int dfargo_compute_selfgravity(dfargo_ctx *ctx);
+
+/* FViscosity (src/Viscosity.c:23-69) with the current ring-mean sound
+ * speed. */
+double dfargo_fviscosity(dfargo_ctx *ctx, double r);
+
+/* The integer FARGO shifts Nshift[i] (src/TransportEuler.c:310-316) and the
+ * ring-mean azimuthal velocities VMed[i] of the last sub-step (nrad each). */
+int dfargo_get_fargo_shift(dfargo_ctx *ctx, long *nshift, double *vmed);
+
+/* Per-kernel timing with HIP events on the context's stream.  While
+ * enabled, sub-steps launch eagerly and every kernel is bracketed by a pair
+ * of events.  names: up to max entries of at most 47 characters. */
+int dfargo_set_profiling(dfargo_ctx *ctx, int enable);
+int dfargo_get_kernel_times(dfargo_ctx *ctx, int max, char (*names)[48],
+                            double *total_ms, long *launches, int *count);
+
+/* The hipStream_t the context launches on, as an opaque pointer. */
+void *dfargo_stream(dfargo_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFARGO_H */

@@ -1,0 +1,1671 @@
+/*
+ * oracle.c -- FP64 CPU restatement of Dusty FARGO-ADSG's AlgoGas sub-step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Not linked into the product.
+ *
+ * Every function follows the reference line by line, with the same
+ * operation order, so that pointwise results are bitwise reproducible by a
+ * device build compiled without contraction.  Citations are to the
+ * Dusty FARGO-ADSG sources (src/...).
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   tables          closed forms of Rmed, Surf, log spacing
+ *   init            equilibrium of the initial state (vr stays ~h^2 small)
+ *   CFL             closed-form dt for a uniform, static disc
+ *   SubStep1        sign/scale: a pure pressure ramp kicks vr by -dt*dP/dr/rho
+ *   viscosity       solid-body rotation is stress-free; Keplerian shear
+ *                   stress matches -3/2 nu Sigma Omega to O(dr^2)
+ *   SubStep2        von Neumann-Richtmyer q = C^2 rho dv^2 on a single jump
+ *   SubStep3        closed-form beta cooling e/(1+dt/t_c)
+ *   transport       mass conservation, zero-velocity identity, an integer
+ *                   solid-body rotation equals numpy.roll, TVD in 1-D
+ *   FARGO shift     Nshift = round(Omega dt Ns / 2pi) on solid-body rings
+ *   SG              brute-force direct convolution on tiny grids,
+ *                   axisymmetric disc gives g_theta = 0, mirror antisymmetry
+ *   FFT             naive DFT on mixed-radix sizes
+ *   RK5             Cash-Karp tableau identities, 5th-order convergence,
+ *                   Kepler energy and angular momentum
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define G 1.0
+#define CFLSECURITY 0.5 /* src/SourceEuler.c:10 */
+#define CVNR 1.41       /* src/SourceEuler.c:13 */
+#define MAXPL DFARGO_MAX_PLANETS
+
+typedef struct { double x, y; } pair_t;
+
+struct or_ctx {
+  dfargo_params p;
+  int nr, ns;
+  size_t gsize; /* doubles per grid, as CreatePolarGrid (src/LowTasks.c:97) */
+  /* 1-D tables (src/global.h:29-37) */
+  double *Radii, *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed;
+  double *InvDiffRsup, *InvRinf, *GlobalRmed, *Azimuth, *CosAz, *SinAz;
+  double *SigmaMed, *SigmaInf, *EnergyMed, *VthetaMed, *VradMed, *GlobalCs;
+  double *AxiSG;
+  double *cabs, *cord; /* CellAbscissa / CellOrdinate */
+  /* state */
+  double *rho, *vr, *vt, *en;
+  dfargo_planets sys;
+  /* module-static work grids of the reference */
+  double *cs, *pres, *temp, *pot, *vradint, *vthetaint, *vradnew, *vthetanew;
+  double *energyint, *energynew, *rhoint, *temperint, *divv, *drr, *drp, *dpp;
+  double *trr, *trp, *tpp, *vischeat;
+  double *rp, *rm, *tp, *tm, *rhostar, *work, *qrstar, *vthetares, *elong;
+  double *dq, *tempshift;
+  long *nshift;
+  double *vmed, *vmoy;
+  int *nosplit;
+  /* self-gravity */
+  int sg_ready;
+  double sg_eps, sg_rstep, sg_tstep;
+  double *kr_re, *kr_im, *kt_re, *kt_im;
+  double *sg_accr, *sg_acct, *gr, *gtheta;
+  /* scalars */
+  double PhysicalTime, PhysicalTimeInitial, OmegaFrame, MINDT, MAXDT, LostMass;
+  double MassTaper;
+  pair_t dop, dop_alt, indirect;
+  double barx[3], bary[3];
+  int GasTimeStepsCFL, last_substeps, crashed_d, crashed_e, crashed;
+  long substeps;
+};
+
+static double min2(double a, double b) { return (b < a) ? b : a; }
+static double max2(double a, double b) { return (b > a) ? b : a; }
+
+static double *grid(or_ctx *c) { return (double *)calloc(c->gsize, sizeof(double)); }
+
+/* ------------------------------------------------------------------------ */
+/* profiles: Sigma, Energy, AspectRatio, FViscosity, compute_smoothing       */
+/* ------------------------------------------------------------------------ */
+
+/* Sigma (src/Theo.c:13-60), power law with optional cavity, ScalingFactor 1 */
+static double sigma_profile(const dfargo_params *p, double r) {
+  double cavity = 1.0;
+  if (r < p->cavity_radius) cavity = 1.0 / p->cavity_ratio;
+  return cavity * 1.0 * p->sigma0 * pow(r, -p->sigma_slope);
+}
+
+/* Energy (src/Theo.c:156-170) with R = MU = 1 */
+static double energy_profile(const dfargo_params *p, double r) {
+  double energy0 = 1.0 / 1.0 / (p->adiabatic_index - 1.0) * sigma_profile(p, r) *
+                   pow(p->aspect_ratio, 2.0) * pow(r, -1.0 + 2.0 * p->flaring_index);
+  return 1.0 * 1.0 * energy0;
+}
+
+/* AspectRatio (src/Viscosity.c:107-122), LAMBDADOUBLING = 0 */
+static double aspect_ratio(const dfargo_params *p, double rad) {
+  double ar = p->aspect_ratio;
+  double rmin = p->transition_radius - p->transition_width * p->aspect_ratio;
+  double rmax = p->transition_radius + p->transition_width * p->aspect_ratio;
+  double scale = 1.0;
+  rmin *= scale;
+  rmax *= scale;
+  if (rad < rmin) ar *= p->transition_ratio;
+  if ((rad >= rmin) && (rad <= rmax))
+    ar *= exp((rmax - rad) / (rmax - rmin) * log(p->transition_ratio));
+  return ar;
+}
+
+/* FViscosity (src/Viscosity.c:23-69) */
+double or_fviscosity(or_ctx *c, double rad) {
+  const dfargo_params *p = &c->p;
+  double viscosity = p->viscosity, rmin, rmax;
+  int i = 0;
+  if (p->alpha_viscosity != 0.0) {
+    while (c->GlobalRmed[i] < rad) i++;
+    viscosity = p->alpha_viscosity * c->GlobalCs[i] * c->GlobalCs[i] * pow(rad, 1.5);
+  }
+  rmin = p->cavity_radius -
+         p->cavity_width * p->aspect_ratio * pow(p->cavity_radius, 1.0 + p->flaring_index);
+  rmax = p->cavity_radius +
+         p->cavity_width * p->aspect_ratio * pow(p->cavity_radius, 1.0 + p->flaring_index);
+  if (rad < rmin) viscosity *= p->cavity_ratio;
+  if ((rad >= rmin) && (rad <= rmax))
+    viscosity *= exp((rmax - rad) / (rmax - rmin) * log(p->cavity_ratio));
+  return viscosity;
+}
+
+static int has_viscosity(const dfargo_params *p) {
+  return (p->alpha_viscosity != 0.0) || (p->viscosity != 0.0);
+}
+
+/* compute_smoothing (src/Force.c:255-263) */
+static double compute_smoothing(const dfargo_params *p, double r) {
+  return p->thickness_smoothing * aspect_ratio(p, r) * pow(r, 1.0 + p->flaring_index);
+}
+
+/* mpi_make1Dprofile (src/axilib.c:5-57), one rank */
+static void make_1d_profile(or_ctx *c, const double *f, double *axi) {
+  for (int i = 0; i < c->nr; i++) {
+    double s = 0.;
+    for (int j = 0; j < c->ns; j++) s += f[i * c->ns + j];
+    s /= (double)c->ns;
+    axi[i] = s;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* creation: FillPolar1DArrays, InitComputeAccel, allocations                */
+/* ------------------------------------------------------------------------ */
+
+or_ctx *or_create(const dfargo_params *p) {
+  or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
+  int nr = p->nrad, ns = p->nsec, i, j;
+  c->p = *p;
+  c->nr = nr;
+  c->ns = ns;
+  c->gsize = (size_t)(nr + 3) * (size_t)(ns + 1) + 5;
+#define T1(n) (double *)calloc((size_t)(n), sizeof(double))
+  c->Radii = T1(nr + 2); c->Rinf = T1(nr + 2); c->Rsup = T1(nr + 2);
+  c->Rmed = T1(nr + 2); c->Surf = T1(nr + 2); c->InvRmed = T1(nr + 2);
+  c->InvSurf = T1(nr + 2); c->InvDiffRmed = T1(nr + 2); c->InvDiffRsup = T1(nr + 2);
+  c->InvRinf = T1(nr + 2); c->GlobalRmed = T1(nr + 2); c->Azimuth = T1(ns);
+  c->CosAz = T1(ns); c->SinAz = T1(ns); c->SigmaMed = T1(nr + 2);
+  c->SigmaInf = T1(nr + 2); c->EnergyMed = T1(nr + 2); c->VthetaMed = T1(nr + 2);
+  c->VradMed = T1(nr + 2); c->GlobalCs = T1(nr + 2); c->AxiSG = T1(nr + 2);
+  c->vmed = T1(nr + 2); c->vmoy = T1(nr + 2);
+#undef T1
+  c->nshift = (long *)calloc((size_t)nr + 2, sizeof(long));
+  c->nosplit = (int *)calloc((size_t)nr + 2, sizeof(int));
+  /* FillPolar1DArrays (src/SourceEuler.c:71-188), no radii.dat */
+  double drrsep = (p->rmax - p->rmin) / (double)nr;
+  for (i = 0; i <= nr; i++) {
+    if (p->log_grid)
+      c->Radii[i] = p->rmin * exp((double)i / (double)nr * log(p->rmax / p->rmin));
+    else
+      c->Radii[i] = p->rmin + drrsep * (double)(i);
+  }
+  for (i = 0; i < nr; i++) {
+    double *R = c->Radii;
+    c->GlobalRmed[i] = 2.0 / 3.0 * (R[i + 1] * R[i + 1] * R[i + 1] - R[i] * R[i] * R[i]);
+    c->GlobalRmed[i] = c->GlobalRmed[i] / (R[i + 1] * R[i + 1] - R[i] * R[i]);
+  }
+  for (i = 0; i < nr; i++) {
+    c->Rinf[i] = c->Radii[i];
+    c->Rsup[i] = c->Radii[i + 1];
+    c->Rmed[i] = 2.0 / 3.0 * (c->Rsup[i] * c->Rsup[i] * c->Rsup[i] - c->Rinf[i] * c->Rinf[i] * c->Rinf[i]);
+    c->Rmed[i] = c->Rmed[i] / (c->Rsup[i] * c->Rsup[i] - c->Rinf[i] * c->Rinf[i]);
+    c->Surf[i] = 0.5 * (p->pmax - p->pmin) * (c->Rsup[i] * c->Rsup[i] - c->Rinf[i] * c->Rinf[i]) / (double)ns;
+    c->InvRmed[i] = 1.0 / c->Rmed[i];
+    c->InvSurf[i] = 1.0 / c->Surf[i];
+    c->InvDiffRsup[i] = 1.0 / (c->Rsup[i] - c->Rinf[i]);
+    c->InvRinf[i] = 1.0 / c->Rinf[i];
+  }
+  c->Rinf[nr] = c->Radii[nr];
+  for (i = 1; i < nr; i++) c->InvDiffRmed[i] = 1.0 / (c->Rmed[i] - c->Rmed[i - 1]);
+  for (j = 0; j < ns; j++) {
+    c->Azimuth[j] = p->pmin + (p->pmax - p->pmin) * (double)j / (double)ns;
+    c->CosAz[j] = cos(c->Azimuth[j]);
+    c->SinAz[j] = sin(c->Azimuth[j]);
+  }
+  /* grids */
+  c->rho = grid(c); c->vr = grid(c); c->vt = grid(c); c->en = grid(c);
+  c->cs = grid(c); c->pres = grid(c); c->temp = grid(c); c->pot = grid(c);
+  c->vradint = grid(c); c->vthetaint = grid(c); c->vradnew = grid(c);
+  c->vthetanew = grid(c); c->energyint = grid(c); c->energynew = grid(c);
+  c->rhoint = grid(c); c->temperint = grid(c); c->divv = grid(c); c->drr = grid(c);
+  c->drp = grid(c); c->dpp = grid(c); c->trr = grid(c); c->trp = grid(c);
+  c->tpp = grid(c); c->vischeat = grid(c); c->rp = grid(c); c->rm = grid(c);
+  c->tp = grid(c); c->tm = grid(c); c->rhostar = grid(c); c->work = grid(c);
+  c->qrstar = grid(c); c->vthetares = grid(c); c->elong = grid(c);
+  c->dq = grid(c); c->tempshift = grid(c); c->cabs = grid(c); c->cord = grid(c);
+  c->sg_accr = grid(c); c->sg_acct = grid(c); c->gr = grid(c); c->gtheta = grid(c);
+  /* InitComputeAccel (src/SideEuler.c:263-280) */
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      c->cabs[j + i * ns] = c->Rmed[i] * cos(c->Azimuth[j]);
+      c->cord[j + i * ns] = c->Rmed[i] * sin(c->Azimuth[j]);
+    }
+  /* FillSigma (src/Theo.c:85-91); FillEnergy (src/Theo.c:172-176) */
+  for (i = 0; i < nr; i++) {
+    c->SigmaMed[i] = sigma_profile(p, c->Rmed[i]);
+    c->SigmaInf[i] = sigma_profile(p, c->Rinf[i]);
+    if (p->energy_equation) c->EnergyMed[i] = energy_profile(p, c->Rmed[i]);
+  }
+  c->OmegaFrame = p->omega_frame;
+  c->MINDT = 100.0; /* src/var.c:161 defaults */
+  c->MAXDT = 0.00001;
+  return c;
+}
+
+void or_destroy(or_ctx *c) {
+  if (!c) return;
+  double **all[] = {&c->Radii, &c->Rinf, &c->Rsup, &c->Rmed, &c->Surf, &c->InvRmed,
+                    &c->InvSurf, &c->InvDiffRmed, &c->InvDiffRsup, &c->InvRinf,
+                    &c->GlobalRmed, &c->Azimuth, &c->CosAz, &c->SinAz, &c->SigmaMed,
+                    &c->SigmaInf, &c->EnergyMed, &c->VthetaMed, &c->VradMed,
+                    &c->GlobalCs, &c->AxiSG, &c->vmed, &c->vmoy, &c->rho, &c->vr, &c->vt,
+                    &c->en, &c->cs, &c->pres, &c->temp, &c->pot, &c->vradint,
+                    &c->vthetaint, &c->vradnew, &c->vthetanew, &c->energyint,
+                    &c->energynew, &c->rhoint, &c->temperint, &c->divv, &c->drr,
+                    &c->drp, &c->dpp, &c->trr, &c->trp, &c->tpp, &c->vischeat,
+                    &c->rp, &c->rm, &c->tp, &c->tm, &c->rhostar, &c->work,
+                    &c->qrstar, &c->vthetares, &c->elong, &c->dq, &c->tempshift,
+                    &c->cabs, &c->cord, &c->sg_accr, &c->sg_acct, &c->gr,
+                    &c->gtheta, &c->kr_re, &c->kr_im, &c->kt_re, &c->kt_im};
+  for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); k++) free(*all[k]);
+  free(c->nshift);
+  free(c->nosplit);
+  free(c);
+}
+
+void or_get_tables(or_ctx *c, double *radii, double *rmed, double *surf, double *azimuth) {
+  if (radii) memcpy(radii, c->Radii, sizeof(double) * (c->nr + 1));
+  if (rmed) memcpy(rmed, c->Rmed, sizeof(double) * c->nr);
+  if (surf) memcpy(surf, c->Surf, sizeof(double) * c->nr);
+  if (azimuth) memcpy(azimuth, c->Azimuth, sizeof(double) * c->ns);
+}
+
+/* ------------------------------------------------------------------------ */
+/* host initial state                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* InitGasDensity (src/Pframeforce.c:297-341), drand48 stream of srand48(seed) */
+void or_init_gas_density(or_ctx *c, double *dens) {
+  unsigned short xs[3];
+  xs[0] = 0x330E;
+  xs[1] = (unsigned short)(c->p.noise_seed & 0xFFFF);
+  xs[2] = (unsigned short)((c->p.noise_seed >> 16) & 0xFFFF);
+  memset(dens, 0, sizeof(double) * (size_t)(c->nr + 1) * c->ns);
+  for (int i = 0; i < c->nr; i++)
+    for (int j = 0; j < c->ns; j++) {
+      int l = j + i * c->ns;
+      dens[l] = c->SigmaMed[i];
+      if (c->p.add_noise) {
+        double randomnb = 2.0 * erand48(xs) - 1.0;
+        dens[l] += 1e-3 * c->SigmaMed[i] * randomnb;
+      }
+    }
+}
+
+/* InitGasEnergy (src/Pframeforce.c:422-440) */
+void or_init_gas_energy(or_ctx *c, double *energy) {
+  memset(energy, 0, sizeof(double) * (size_t)(c->nr + 1) * c->ns);
+  for (int i = 0; i < c->nr; i++)
+    for (int j = 0; j < c->ns; j++) energy[j + i * c->ns] = energy_profile(&c->p, c->Rmed[i]);
+}
+
+/* InitPlanetarySystem (src/Psys.c:96-196); the .cfg columns are read with
+ * %f, so each value is rounded to float first (src/Psys.c:118) */
+void or_init_planetary_system(or_ctx *c, int nb, const double *dist_in,
+                              const double *mass_in, const double *acc_in,
+                              const int *feel_disk, const int *feel_others,
+                              dfargo_planets *sys) {
+  memset(sys, 0, sizeof(*sys));
+  sys->nb = nb;
+  for (int i = 0; i < nb; i++) {
+    float dist = (float)dist_in[i], mass = (float)mass_in[i], accret = (float)acc_in[i];
+    double massinvelocity;
+    sys->mass[i] = (double)mass;
+    sys->initial_mass[i] = 0.0;
+    sys->final_mass[i] = sys->mass[i];
+    if (c->p.mass_taper > 1e-3)
+      massinvelocity = 0.0;
+    else
+      massinvelocity = sys->final_mass[i];
+    sys->x[i] = (double)dist * (1.0 + c->p.eccentricity);
+    sys->y[i] = 0.0;
+    sys->vy[i] = (double)sqrt(G * (1.0 + massinvelocity) / dist) *
+                 sqrt((1.0 - c->p.eccentricity) / (1.0 + c->p.eccentricity));
+    sys->vx[i] = -0.0000000001 * sys->vy[i];
+    sys->acc[i] = accret;
+    sys->feel_disk[i] = feel_disk[i];
+    sys->feel_others[i] = feel_others[i];
+    sys->binary[i] = 0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* equation of state                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeSoundSpeed (src/SourceEuler.c:2323-2347) */
+static void compute_sound_speed(or_ctx *c) {
+  const dfargo_params *p = &c->p;
+  for (int i = 0; i < c->nr; i++) {
+    double cs_buf = 0.;
+    if (!p->energy_equation)
+      cs_buf = aspect_ratio(p, c->Rmed[i]) * sqrt(G * 1.0 / c->Rmed[i]) * pow(c->Rmed[i], p->flaring_index);
+    for (int j = 0; j < c->ns; j++) {
+      int l = i * c->ns + j;
+      if (!p->energy_equation)
+        c->cs[l] = cs_buf;
+      else
+        c->cs[l] = sqrt(p->adiabatic_index * (p->adiabatic_index - 1.0) * c->en[l] / c->rho[l]);
+    }
+  }
+}
+
+/* ComputePressureField (src/SourceEuler.c:2414-2434) */
+static void compute_pressure(or_ctx *c) {
+  for (int i = 0; i < c->nr; i++)
+    for (int j = 0; j < c->ns; j++) {
+      int l = i * c->ns + j;
+      if (!c->p.energy_equation)
+        c->pres[l] = c->rho[l] * c->cs[l] * c->cs[l];
+      else
+        c->pres[l] = (c->p.adiabatic_index - 1.0) * c->en[l];
+    }
+}
+
+/* ComputeTemperatureField (src/SourceEuler.c:2436-2460), MU = R = 1 */
+static void compute_temperature(or_ctx *c) {
+  for (int i = 0; i < c->nr; i++)
+    for (int j = 0; j < c->ns; j++) {
+      int l = i * c->ns + j;
+      if (!c->p.energy_equation)
+        c->temp[l] = 1.0 / 1.0 * c->pres[l] / c->rho[l];
+      else
+        c->temp[l] = 1.0 / 1.0 * (c->p.adiabatic_index - 1.0) * c->en[l] / c->rho[l];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* viscosity                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeDivergenceVelocity (src/SourceEuler.c:2373-2411) */
+static void compute_divergence(or_ctx *c, const double *vr, const double *vt) {
+  int nr = c->nr, ns = c->ns;
+  double dphi = (c->p.pmax - c->p.pmin) / (double)ns, invdphi = 1.0 / dphi;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      c->divv[l] = (vr[lip] * c->Rsup[i] - vr[l] * c->Rinf[i]) * c->InvDiffRsup[i] * c->InvRmed[i];
+      c->divv[l] += (vt[ljp] - vt[l]) * invdphi * c->InvRmed[i];
+    }
+}
+
+/* ComputeViscousTerms (src/Viscosity.c:159-300) */
+static void compute_viscous_terms(or_ctx *c, const double *vr, const double *vt) {
+  int nr = c->nr, ns = c->ns;
+  const double *rho = c->rho;
+  double dphi = (c->p.pmax - c->p.pmin) / (double)ns, invdphi = 1.0 / dphi;
+  double onethird = 1.0 / 3.0, viscosity;
+  compute_divergence(c, vr, vt);
+  if (c->p.alpha_viscosity != 0.0) make_1d_profile(c, c->cs, c->GlobalCs);
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      c->drr[l] = (vr[lip] - vr[l]) * c->InvDiffRsup[i];
+      c->dpp[l] = (vt[ljp] - vt[l]) * invdphi * c->InvRmed[i] + 0.5 * (vr[lip] + vr[l]) * c->InvRmed[i];
+    }
+  for (int i = 1; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1, lim = l - ns;
+      c->drp[l] = 0.5 * (c->Rinf[i] * (vt[l] * c->InvRmed[i] - vt[lim] * c->InvRmed[i - 1]) * c->InvDiffRmed[i] +
+                         (vr[l] - vr[ljm]) * invdphi * c->InvRinf[i]);
+    }
+  for (int i = 0; i < nr; i++) {
+    viscosity = has_viscosity(&c->p) ? or_fviscosity(c, c->Rmed[i]) : 0.0;
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns;
+      c->trr[l] = 2.0 * rho[l] * viscosity * (c->drr[l] - onethird * c->divv[l]);
+      c->tpp[l] = 2.0 * rho[l] * viscosity * (c->dpp[l] - onethird * c->divv[l]);
+    }
+  }
+  for (int i = 1; i < nr; i++) {
+    viscosity = has_viscosity(&c->p) ? or_fviscosity(c, c->Rmed[i]) : 0.0;
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lim = l - ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      int ljmim = ljm - ns;
+      c->trp[l] = 2.0 * 0.25 * (rho[l] + rho[lim] + rho[ljm] + rho[ljmim]) * viscosity * c->drp[l];
+    }
+  }
+}
+
+/* UpdateVelocitiesWithViscosity (src/Viscosity.c:302-378) */
+static void update_velocities_viscosity(or_ctx *c, double *vr, double *vt, double DeltaT) {
+  int nr = c->nr, ns = c->ns;
+  const double *rho = c->rho, *Trr = c->trr, *Trp = c->trp, *Tpp = c->tpp;
+  double dphi = (c->p.pmax - c->p.pmin) / (double)ns, invdphi = 1.0 / dphi;
+  for (int i = 1; i < nr - 1; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lip = l + ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      vt[l] += DeltaT * c->InvRmed[i] *
+               ((c->Rsup[i] * Trp[lip] - c->Rinf[i] * Trp[l]) * c->InvDiffRsup[i] +
+                (Tpp[l] - Tpp[ljm]) * invdphi + 0.5 * (Trp[l] + Trp[lip])) /
+               (0.5 * (rho[l] + rho[ljm]));
+    }
+  for (int i = 1; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lim = l - ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      vr[l] += DeltaT * c->InvRinf[i] *
+               ((c->Rmed[i] * Trr[l] - c->Rmed[i - 1] * Trr[lim]) * c->InvDiffRmed[i] +
+                (Trp[ljp] - Trp[l]) * invdphi - 0.5 * (Tpp[l] + Tpp[lim])) /
+               (0.5 * (rho[l] + rho[lim]));
+    }
+}
+
+void or_viscous(or_ctx *c, double dt) {
+  compute_viscous_terms(c, c->vr, c->vt);
+  update_velocities_viscosity(c, c->vr, c->vt, dt);
+}
+
+/* ComputeViscousHeating (src/SourceEuler.c:1554-1606) */
+static void compute_viscous_heating(or_ctx *c) {
+  int nr = c->nr, ns = c->ns;
+  const double *dens = c->rho;
+  double viscosity = 0.0;
+  for (int i = 1; i < nr; i++) {
+    if (has_viscosity(&c->p)) viscosity = or_fviscosity(c, c->Rmed[i]);
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns;
+      if (viscosity != 0.0) {
+        c->vischeat[l] = 0.5 / viscosity / dens[l] *
+                         (c->trr[l] * c->trr[l] + 2.0 * c->trp[l] * c->trp[l] + c->tpp[l] * c->tpp[l]);
+        c->vischeat[l] += (2.0 / 9.0) * viscosity * dens[l] * c->divv[l] * c->divv[l];
+      } else
+        c->vischeat[l] = 0.0;
+    }
+  }
+  double r = c->Rmed[0], rip = c->Rmed[1], ri2p = c->Rmed[2];
+  for (int j = 0; j < ns; j++) {
+    int l = j, lip = l + ns, li2p = lip + ns;
+    double qpip = c->vischeat[lip], qpi2p = c->vischeat[li2p];
+    if (viscosity != 0.0)
+      c->vischeat[l] = qpip * exp(log(qpip / qpi2p) * log(r / rip) / log(rip / ri2p));
+    else
+      c->vischeat[l] = 0.0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* forces, potential, planets                                                */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeForce (src/Force.c:77-253) with dimfxy = 2; gf as GlobalForce */
+static void compute_force(or_ctx *c, const double *dens, double x, double y,
+                          double rsmoothing, double mass, double gf[8]) {
+  int nr = c->nr, ns = c->ns;
+  double fxi[2] = {0., 0.}, fxo[2] = {0., 0.}, fyi[2] = {0., 0.}, fyo[2] = {0., 0.};
+  double a = sqrt(x * x + y * y);
+  double rh = pow(mass / 3., 1. / 3.) * a + 1e-15;
+  int binary = (c->sys.nb > 1) && c->sys.binary[0];
+  double xb = 0., yb = 0.;
+  if (binary) {
+    double m0 = c->sys.mass[0], m1 = c->sys.mass[1];
+    xb = (m0 * c->sys.x[0] + m1 * c->sys.x[1]) / (m0 + m1);
+    yb = (m0 * c->sys.y[0] + m1 * c->sys.y[1]) / (m0 + m1);
+  }
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns;
+      double xc = c->cabs[l], yc = c->cord[l];
+      double cellmass = c->Surf[i] * dens[l];
+      double dx = xc - x, dy = yc - y;
+      double d2 = dx * dx + dy * dy;
+      double planet_distance = sqrt(d2);
+      double dist2 = d2 + rsmoothing * rsmoothing;
+      double distance = sqrt(dist2);
+      double InvDist3 = 1.0 / dist2 / distance;
+      if (binary) {
+        d2 = (xc - xb) * (xc - xb) + (yc - yb) * (yc - yb);
+        planet_distance = sqrt(d2);
+      }
+      for (int k = 0; k < 2; k++) {
+        double hill_cut;
+        if (k != 0) {
+          double cutoff = c->p.exclude_hill_factor * rh;
+          if (planet_distance / cutoff < 0.5)
+            hill_cut = 0.0;
+          else if (planet_distance > cutoff)
+            hill_cut = 1.0;
+          else
+            hill_cut = pow(sin((planet_distance / cutoff - .5) * M_PI), 2.);
+        } else
+          hill_cut = 1.;
+        if (c->Rmed[i] < a) {
+          fxi[k] += G * cellmass * dx * InvDist3 * hill_cut;
+          fyi[k] += G * cellmass * dy * InvDist3 * hill_cut;
+        } else {
+          fxo[k] += G * cellmass * dx * InvDist3 * hill_cut;
+          fyo[k] += G * cellmass * dy * InvDist3 * hill_cut;
+        }
+      }
+    }
+  gf[0] = fxi[0]; gf[1] = fxi[1]; gf[2] = fxo[0]; gf[3] = fxo[1];
+  gf[4] = fyi[0]; gf[5] = fyi[1]; gf[6] = fyo[0]; gf[7] = fyo[1];
+}
+
+/* ComputeAccel (src/SideEuler.c:282-298) */
+static pair_t compute_accel(or_ctx *c, const double *dens, double x, double y,
+                            double rsmoothing, double mass) {
+  double gf[8];
+  pair_t a;
+  compute_force(c, dens, x, y, rsmoothing, mass, gf);
+  if (c->p.exclude_hill) {
+    a.x = gf[1] + gf[3];
+    a.y = gf[5] + gf[7];
+  } else {
+    a.x = gf[0] + gf[2];
+    a.y = gf[4] + gf[6];
+  }
+  return a;
+}
+
+/* GasTotalMass, ComputeBarycenterPosition, ComputeOppositeAcceleration
+ * OfBarycenter (src/SideEuler.c:15-112) */
+static pair_t opposite_acc_barycenter(or_ctx *c, double timestep) {
+  int nr = c->nr, ns = c->ns;
+  double total = 0.0, lxb = 0.0, lyb = 0.0;
+  pair_t acc;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) total += c->Surf[i] * c->rho[j + i * ns];
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = i * ns + j;
+      lxb += c->Surf[i] * c->rho[l] * c->cabs[l];
+      lyb += c->Surf[i] * c->rho[l] * c->cord[l];
+    }
+  c->barx[2] = lxb / (1.0 + total);
+  c->bary[2] = lyb / (1.0 + total);
+  acc.x = -(c->barx[2] - 2.0 * c->barx[1] + c->barx[0]) / (timestep * timestep);
+  acc.y = -(c->bary[2] - 2.0 * c->bary[1] + c->bary[0]) / (timestep * timestep);
+  c->barx[0] = c->barx[1]; c->barx[1] = c->barx[2];
+  c->bary[0] = c->bary[1]; c->bary[1] = c->bary[2];
+  return acc;
+}
+
+/* ComputeIndirectTerm + FillForcesArrays (src/Pframeforce.c:25-129) */
+static void fill_forces_arrays(or_ctx *c) {
+  const dfargo_params *p = &c->p;
+  int nr = c->nr, ns = c->ns, nb = c->sys.nb;
+  double planetmass_taper;
+  if (p->mass_taper > 1e-2) {
+    double mt = (c->PhysicalTime - c->PhysicalTimeInitial) / (p->mass_taper * 2.0 * M_PI);
+    planetmass_taper = (mt > 1.0 ? 1.0 : .5 * (1.0 - cos(M_PI * mt)));
+  } else
+    planetmass_taper = 1.0;
+  c->indirect.x = -c->dop.x;
+  c->indirect.y = -c->dop.y;
+  if (p->alternative_gas_indirect_term) {
+    c->indirect.x = -c->dop_alt.x;
+    c->indirect.y = -c->dop_alt.y;
+  }
+  if (!p->indirect_term) {
+    c->indirect.x = 0.0;
+    c->indirect.y = 0.0;
+  }
+  for (int l = 0; l < (nr + 1) * ns; l++) c->pot[l] = 0.0;
+  for (int k = 0; k < nb; k++) {
+    double xplanet = c->sys.x[k], yplanet = c->sys.y[k];
+    double mplanet = c->sys.mass[k] * planetmass_taper;
+    double PlanetDistance = sqrt(xplanet * xplanet + yplanet * yplanet);
+    double InvPlanetDistance3 = 1.0 / PlanetDistance / PlanetDistance / PlanetDistance;
+    double RRoche = PlanetDistance * pow((1.0 / 3.0 * mplanet), 1.0 / 3.0);
+    double smoothing = (p->roche_smoothing != 0.0) ? RRoche * p->roche_smoothing
+                                                     : compute_smoothing(p, PlanetDistance);
+    double smooth = smoothing * smoothing;
+    for (int i = 0; i < nr; i++)
+      for (int j = 0; j < ns; j++) {
+        int l = j + i * ns;
+        double x = c->Rmed[i] * c->CosAz[j], y = c->Rmed[i] * c->SinAz[j];
+        double distance = (x - xplanet) * (x - xplanet) + (y - yplanet) * (y - yplanet);
+        double distancesmooth = sqrt(distance + smooth);
+        double pot = -G * mplanet / distancesmooth;
+        if (p->indirect_term) pot += G * mplanet * InvPlanetDistance3 * (x * xplanet + y * yplanet);
+        c->pot[l] += pot;
+      }
+  }
+  for (int i = 0; i < nr; i++) {
+    double InvDistance = 1.0 / c->Rmed[i];
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns;
+      double x = c->Rmed[i] * c->CosAz[j], y = c->Rmed[i] * c->SinAz[j];
+      double pot = -G * 1.0 * InvDistance;
+      if ((!p->discard_gas_indirect_term) && (fabs(p->pmax - p->pmin - 2. * M_PI) < 0.01))
+        pot -= c->indirect.x * x + c->indirect.y * y;
+      c->pot[l] += pot;
+    }
+  }
+}
+
+/* AdvanceSystemFromDisk (src/Pframeforce.c:131-161) */
+static void advance_system_from_disk(or_ctx *c, double dt) {
+  for (int k = 0; k < c->sys.nb; k++) {
+    if (!c->sys.feel_disk[k]) continue;
+    double m = c->sys.mass[k], x = c->sys.x[k], y = c->sys.y[k];
+    double r = sqrt(x * x + y * y), smoothing;
+    if (c->p.roche_smoothing != 0.0)
+      smoothing = r * pow(m / 3., 1. / 3.) * c->p.roche_smoothing;
+    else
+      smoothing = compute_smoothing(&c->p, r);
+    pair_t gamma = compute_accel(c, c->rho, x, y, smoothing, m);
+    c->sys.vx[k] += dt * gamma.x;
+    c->sys.vy[k] += dt * gamma.y;
+    c->sys.vx[k] += dt * c->indirect.x;
+    c->sys.vy[k] += dt * c->indirect.y;
+  }
+}
+
+/* DerivMotionRK5 (src/RungeKunta.c:14-53) */
+static void deriv_motion_rk5(const double *q, const double *masses, double *deriv,
+                             int n, double dt, const int *feelothers, int indirect) {
+  const double *x = q, *y = q + n, *vx = y + n, *vy = vx + n;
+  double *dx = deriv, *dy = deriv + n, *dvx = dy + n, *dvy = dvx + n;
+  double Dist[MAXPL];
+  for (int i = 0; i < n; i++) Dist[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
+  for (int i = 0; i < n; i++) {
+    dx[i] = vx[i];
+    dy[i] = vy[i];
+    dvx[i] = -G * 1.0 / Dist[i] / Dist[i] / Dist[i] * x[i];
+    dvy[i] = -G * 1.0 / Dist[i] / Dist[i] / Dist[i] * y[i];
+    for (int j = 0; j < n; j++) {
+      if (indirect) {
+        dvx[i] -= G * masses[j] / Dist[j] / Dist[j] / Dist[j] * x[j];
+        dvy[i] -= G * masses[j] / Dist[j] / Dist[j] / Dist[j] * y[j];
+      }
+      if ((j != i) && feelothers[i]) {
+        double dist = (x[i] - x[j]) * (x[i] - x[j]) + (y[i] - y[j]) * (y[i] - y[j]);
+        dist = sqrt(dist);
+        dvx[i] += G * masses[j] / dist / dist / dist * (x[j] - x[i]);
+        dvy[i] += G * masses[j] / dist / dist / dist * (y[j] - y[i]);
+      }
+    }
+  }
+  for (int i = 0; i < 4 * n; i++) deriv[i] *= dt;
+}
+
+static void translate_rk5(const double *qold, const double *k1, const double *k2,
+                          const double *k3, const double *k4, const double *k5,
+                          double c1, double c2, double c3, double c4, double c5,
+                          double *qnew, int n) {
+  for (int i = 0; i < 4 * n; i++)
+    qnew[i] = qold[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i] + c4 * k4[i] + c5 * k5[i];
+}
+
+/* RungeKunta (src/RungeKunta.c:64-91), Cash-Karp tableau */
+void or_rk5(const double *q0, double dt, const double *masses, double *q1, int n,
+            const int *feelothers, int indirect) {
+  double k1[4 * MAXPL], k2[4 * MAXPL], k3[4 * MAXPL], k4[4 * MAXPL], k5[4 * MAXPL], k6[4 * MAXPL];
+  memset(k1, 0, sizeof k1); memset(k2, 0, sizeof k2); memset(k3, 0, sizeof k3);
+  memset(k4, 0, sizeof k4); memset(k5, 0, sizeof k5); memset(k6, 0, sizeof k6);
+  deriv_motion_rk5(q0, masses, k1, n, dt, feelothers, indirect);
+  translate_rk5(q0, k1, k2, k3, k4, k5, 0.2, 0.0, 0.0, 0.0, 0.0, q1, n);
+  deriv_motion_rk5(q1, masses, k2, n, dt, feelothers, indirect);
+  translate_rk5(q0, k1, k2, k3, k4, k5, 0.075, 0.225, 0.0, 0.0, 0.0, q1, n);
+  deriv_motion_rk5(q1, masses, k3, n, dt, feelothers, indirect);
+  translate_rk5(q0, k1, k2, k3, k4, k5, 0.3, -0.9, 1.2, 0.0, 0.0, q1, n);
+  deriv_motion_rk5(q1, masses, k4, n, dt, feelothers, indirect);
+  translate_rk5(q0, k1, k2, k3, k4, k5, -11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0.0, q1, n);
+  deriv_motion_rk5(q1, masses, k5, n, dt, feelothers, indirect);
+  translate_rk5(q0, k1, k2, k3, k4, k5, 1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0,
+                44275.0 / 110592.0, 253.0 / 4096.0, q1, n);
+  deriv_motion_rk5(q1, masses, k6, n, dt, feelothers, indirect);
+  for (int i = 0; i < 4 * n; i++)
+    q1[i] = q0[i] + 37.0 / 378.0 * k1[i] + 250.0 / 621.0 * k3[i] + 125.0 / 594.0 * k4[i] +
+            512.0 / 1771.0 * k6[i];
+}
+
+/* AdvanceSystemRK5 (src/Pframeforce.c:163-266) without ForcedInnerCircular */
+static void advance_system_rk5(or_ctx *c, double dt) {
+  dfargo_planets *s = &c->sys;
+  int n = s->nb;
+  double q0[4 * MAXPL], q1[4 * MAXPL], masses[MAXPL];
+  if (n == 0) return;
+  if (!c->p.forced_circular) {
+    for (int i = 0; i < n; i++) {
+      q0[i] = s->x[i]; q0[i + n] = s->y[i]; q0[i + 2 * n] = s->vx[i]; q0[i + 3 * n] = s->vy[i];
+      masses[i] = s->mass[i];
+    }
+    or_rk5(q0, dt, masses, q1, n, s->feel_others, c->p.indirect_term);
+  }
+  for (int i = 1 - (c->PhysicalTime >= c->p.release_date); i < n; i++) {
+    if (!c->p.forced_circular) {
+      s->x[i] = q1[i]; s->y[i] = q1[i + n]; s->vx[i] = q1[i + 2 * n]; s->vy[i] = q1[i + 3 * n];
+    } else {
+      double x = s->x[i], y = s->y[i], theta = atan2(y, x), vx = s->vx[i], vy = s->vy[i];
+      double r = sqrt(x * x + y * y), v = sqrt(vx * vx + vy * vy);
+      double omega = (-y * vx + x * vy) / r / r, dtheta = omega * dt;
+      s->x[i] = r * cos(theta + dtheta);
+      s->y[i] = r * sin(theta + dtheta);
+      s->vx[i] = -v * sin(theta + dtheta);
+      s->vy[i] = v * cos(theta + dtheta);
+    }
+  }
+  if (c->PhysicalTime < c->p.release_date) {
+    double x = s->x[0], y = s->y[0], r = sqrt(x * x + y * y), theta = atan2(y, x);
+    double rdot = (c->p.release_radius - r) / (c->p.release_date - c->PhysicalTime);
+    double omega = sqrt((1. + s->mass[0]) / r / r / r), new_r = r + rdot * dt, denom = r - new_r;
+    double dtheta = (denom != 0.0) ? 2. * dt * r * omega / denom * (sqrt(r / new_r) - 1.) : omega * dt;
+    double vx = rdot, vy = new_r * sqrt((1. + s->mass[0]) / new_r / new_r / new_r);
+    s->x[0] = new_r * cos(dtheta + theta);
+    s->y[0] = new_r * sin(dtheta + theta);
+    s->vx[0] = vx * cos(dtheta + theta) - vy * sin(dtheta + theta);
+    s->vy[0] = vx * sin(dtheta + theta) + vy * cos(dtheta + theta);
+  }
+}
+
+/* RotatePsys (src/Psys.c:352-370) */
+static void rotate_psys(or_ctx *c, double angle) {
+  double sint = sin(angle), cost = cos(angle);
+  for (int i = 0; i < c->sys.nb; i++) {
+    double xt = c->sys.x[i], yt = c->sys.y[i];
+    c->sys.x[i] = xt * cost + yt * sint;
+    c->sys.y[i] = -xt * sint + yt * cost;
+    xt = c->sys.vx[i];
+    yt = c->sys.vy[i];
+    c->sys.vx[i] = xt * cost + yt * sint;
+    c->sys.vy[i] = -xt * sint + yt * cost;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* self-gravity                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Mixed-radix (2, 3, else naive) recursive DFT, unnormalized,
+ * y[k] = sum_n x[n] exp(sign*2*pi*i*k*n/N).  Pinned against a naive DFT. */
+static void fft_rec(const double *xr, const double *xi, int stride, double *yr,
+                    double *yi, int n, int sign, const double *twr, const double *twi,
+                    int tstride) {
+  if (n == 1) {
+    yr[0] = xr[0];
+    yi[0] = xi[0];
+    return;
+  }
+  int p = (n % 2 == 0) ? 2 : ((n % 3 == 0) ? 3 : n);
+  int m = n / p;
+  if (p == n) { /* naive */
+    for (int k = 0; k < n; k++) {
+      double sr = 0., si = 0.;
+      for (int t = 0; t < n; t++) {
+        long idx = ((long)k * t) % n * tstride;
+        double wr = twr[idx], wi = sign * twi[idx];
+        sr += xr[t * stride] * wr - xi[t * stride] * wi;
+        si += xr[t * stride] * wi + xi[t * stride] * wr;
+      }
+      yr[k] = sr;
+      yi[k] = si;
+    }
+    return;
+  }
+  double *tr = (double *)malloc(sizeof(double) * n), *ti = (double *)malloc(sizeof(double) * n);
+  for (int r = 0; r < p; r++)
+    fft_rec(xr + r * stride, xi + r * stride, stride * p, tr + r * m, ti + r * m, m, sign, twr,
+            twi, tstride * p);
+  for (int k = 0; k < m; k++)
+    for (int q = 0; q < p; q++) {
+      int kk = k + q * m;
+      double sr = 0., si = 0.;
+      for (int r = 0; r < p; r++) {
+        long idx = ((long)r * kk) % n * tstride;
+        double wr = twr[idx], wi = sign * twi[idx];
+        sr += tr[r * m + k] * wr - ti[r * m + k] * wi;
+        si += tr[r * m + k] * wi + ti[r * m + k] * wr;
+      }
+      yr[kk] = sr;
+      yi[kk] = si;
+    }
+  free(tr);
+  free(ti);
+}
+
+void or_fft(double *re, double *im, int n, int sign) {
+  double *twr = (double *)malloc(sizeof(double) * n), *twi = (double *)malloc(sizeof(double) * n);
+  double *yr = (double *)malloc(sizeof(double) * n), *yi = (double *)malloc(sizeof(double) * n);
+  for (int k = 0; k < n; k++) {
+    twr[k] = cos(2.0 * M_PI * (double)k / (double)n);
+    twi[k] = sin(2.0 * M_PI * (double)k / (double)n);
+  }
+  fft_rec(re, im, 1, yr, yi, n, sign, twr, twi, 1);
+  memcpy(re, yr, sizeof(double) * n);
+  memcpy(im, yi, sizeof(double) * n);
+  free(twr); free(twi); free(yr); free(yi);
+}
+
+/* 2-D complex transform of an nx-by-ny row-major array */
+static void fft2d(double *re, double *im, int nx, int ny, int sign) {
+  int mx = nx > ny ? nx : ny;
+  double *br = (double *)malloc(sizeof(double) * mx), *bi = (double *)malloc(sizeof(double) * mx);
+  for (int i = 0; i < nx; i++) or_fft(re + (size_t)i * ny, im + (size_t)i * ny, ny, sign);
+  for (int j = 0; j < ny; j++) {
+    for (int i = 0; i < nx; i++) { br[i] = re[(size_t)i * ny + j]; bi[i] = im[(size_t)i * ny + j]; }
+    or_fft(br, bi, nx, sign);
+    for (int i = 0; i < nx; i++) { re[(size_t)i * ny + j] = br[i]; im[(size_t)i * ny + j] = bi[i]; }
+  }
+  free(br);
+  free(bi);
+}
+
+/* init_sg + compute_fftkernel (src/sginit.c:3-13, src/sgkernel.c:3-27):
+ * kernels on the (2 Nrad) x Nsec log-polar mesh, then their spectra */
+static void sg_init(or_ctx *c) {
+  int nr = c->nr, ns = c->ns, n2 = 2 * nr;
+  size_t tot = (size_t)n2 * ns;
+  c->sg_eps = c->p.sg_thickness_smoothing * c->p.aspect_ratio;
+  c->sg_rstep = log(c->Radii[nr] / c->Radii[0]) / (double)nr;
+  c->sg_tstep = (c->p.pmax - c->p.pmin) / (double)ns;
+  c->kr_re = (double *)calloc(tot, sizeof(double)); c->kr_im = (double *)calloc(tot, sizeof(double));
+  c->kt_re = (double *)calloc(tot, sizeof(double)); c->kt_im = (double *)calloc(tot, sizeof(double));
+  double eps = c->sg_eps;
+  for (int i = 0; i < n2; i++) {
+    double u = (i < nr) ? log(c->Radii[i] / c->Radii[0]) : -log(c->Radii[2 * nr - i] / c->Radii[0]);
+    for (int j = 0; j < ns; j++) {
+      size_t l = (size_t)i * ns + j;
+      c->kr_re[l] = 1.0 + eps * eps - c->CosAz[j] * exp(-u);
+      c->kr_re[l] *= pow(eps * eps * exp(u) + 2.0 * (cosh(u) - c->CosAz[j]), -1.5);
+      c->kt_re[l] = c->SinAz[j];
+      c->kt_re[l] *= pow(eps * eps * exp(u) + 2.0 * (cosh(u) - c->CosAz[j]), -1.5);
+    }
+  }
+  fft2d(c->kr_re, c->kr_im, n2, ns, -1);
+  fft2d(c->kt_re, c->kt_im, n2, ns, -1);
+  c->sg_ready = 1;
+}
+
+/* normalisation and self-force (src/sgacc.c:137-161) applied to the raw
+ * transform output raw_r/raw_t (the un-normalised inverse transform) */
+static void sg_finish(or_ctx *c, const double *rho, const double *raw_r, const double *raw_t,
+                      double *accr, double *acct) {
+  int nr = c->nr, ns = c->ns;
+  for (int i = 0; i < nr; i++) {
+    double normaccr = c->sg_rstep * c->sg_tstep / ((double)(2 * nr) * (double)ns);
+    double normacct = normaccr;
+    normaccr /= sqrt(c->Rmed[i] / c->GlobalRmed[0]);
+    normacct /= (c->Rmed[i] / c->GlobalRmed[0] * sqrt(c->Rmed[i] / c->GlobalRmed[0]));
+    for (int j = 0; j < ns; j++) {
+      int l = i * ns + j;
+      accr[l] = raw_r[l] * normaccr;
+      acct[l] = raw_t[l] * normacct;
+    }
+  }
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = i * ns + j;
+      accr[l] += G * rho[l] * c->sg_rstep * c->sg_tstep / c->sg_eps;
+    }
+}
+
+/* compute_fftdensity + compute_sgacc (src/sgdens.c:3-86, src/sgacc.c:3-162) */
+void or_sg_accel(or_ctx *c, const double *rho, double *accr, double *acct) {
+  int nr = c->nr, ns = c->ns, n2 = 2 * nr;
+  size_t tot = (size_t)n2 * ns;
+  if (!c->sg_ready) sg_init(c);
+  double *sr = (double *)calloc(tot, sizeof(double)), *si = (double *)calloc(tot, sizeof(double));
+  double *tr = (double *)calloc(tot, sizeof(double)), *ti = (double *)calloc(tot, sizeof(double));
+  double *ar = (double *)calloc(tot, sizeof(double)), *ai = (double *)calloc(tot, sizeof(double));
+  double *br = (double *)calloc(tot, sizeof(double)), *bi = (double *)calloc(tot, sizeof(double));
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      size_t l = (size_t)i * ns + j;
+      sr[l] = rho[i * ns + j] * sqrt(c->Rmed[i] / c->GlobalRmed[0]);
+      tr[l] = sr[l] * c->Rmed[i] / c->GlobalRmed[0];
+    }
+  fft2d(sr, si, n2, ns, -1);
+  fft2d(tr, ti, n2, ns, -1);
+  for (size_t l = 0; l < tot; l++) {
+    ar[l] = -G * (c->kr_re[l] * sr[l] - c->kr_im[l] * si[l]);
+    ai[l] = -G * (c->kr_re[l] * si[l] + c->kr_im[l] * sr[l]);
+    br[l] = -G * (c->kt_re[l] * tr[l] - c->kt_im[l] * ti[l]);
+    bi[l] = -G * (c->kt_re[l] * ti[l] + c->kt_im[l] * tr[l]);
+  }
+  fft2d(ar, ai, n2, ns, +1);
+  fft2d(br, bi, n2, ns, +1);
+  sg_finish(c, rho, ar, br, accr, acct);
+  free(sr); free(si); free(tr); free(ti); free(ar); free(ai); free(br); free(bi);
+}
+
+/* the same acceleration by direct summation of the cyclic convolution, for
+ * tiny grids: raw[i][j] = -G * 2Nr*Ns * sum K[i-i'][j-j'] S[i'][j'] */
+void or_sg_accel_direct(or_ctx *c, const double *rho, double *accr, double *acct) {
+  int nr = c->nr, ns = c->ns, n2 = 2 * nr;
+  double eps, *rawr, *rawt;
+  if (!c->sg_ready) sg_init(c);
+  eps = c->sg_eps;
+  rawr = (double *)calloc((size_t)nr * ns, sizeof(double));
+  rawt = (double *)calloc((size_t)nr * ns, sizeof(double));
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      long double sumr = 0., sumt = 0.;
+      for (int ip = 0; ip < nr; ip++)
+        for (int jp = 0; jp < ns; jp++) {
+          int di = ((i - ip) % n2 + n2) % n2, dj = ((j - jp) % ns + ns) % ns;
+          double u = (di < nr) ? log(c->Radii[di] / c->Radii[0]) : -log(c->Radii[2 * nr - di] / c->Radii[0]);
+          double base = pow(eps * eps * exp(u) + 2.0 * (cosh(u) - c->CosAz[dj]), -1.5);
+          double kr = (1.0 + eps * eps - c->CosAz[dj] * exp(-u)) * base;
+          double kt = c->SinAz[dj] * base;
+          double s_r = rho[ip * ns + jp] * sqrt(c->Rmed[ip] / c->GlobalRmed[0]);
+          double s_t = s_r * c->Rmed[ip] / c->GlobalRmed[0];
+          sumr += (long double)kr * s_r;
+          sumt += (long double)kt * s_t;
+        }
+      rawr[i * ns + j] = -G * (double)sumr * (double)n2 * (double)ns;
+      rawt[i * ns + j] = -G * (double)sumt * (double)n2 * (double)ns;
+    }
+  sg_finish(c, rho, rawr, rawt, accr, acct);
+  free(rawr);
+  free(rawt);
+}
+
+/* compute_selfgravity (src/sgmain.c:3-35) + update_sgvelocity
+ * (src/sgupdate.c:3-73) */
+static void compute_selfgravity(or_ctx *c, double *vrad, double *vtheta, double Dt, int update) {
+  int nr = c->nr, ns = c->ns;
+  or_sg_accel(c, c->rho, c->sg_accr, c->sg_acct);
+  memcpy(c->gr, c->sg_accr, sizeof(double) * nr * ns);
+  memcpy(c->gtheta, c->sg_acct, sizeof(double) * nr * ns);
+  if (!update) return;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = i * ns + j;
+      if (i > 0) {
+        double radsgacc = ((c->Rinf[i] - c->Rmed[i - 1]) * c->sg_accr[l] +
+                           (c->Rmed[i] - c->Rinf[i]) * c->sg_accr[l - ns]) * c->InvDiffRmed[i];
+        vrad[l] += Dt * radsgacc;
+      }
+      int lm1 = i * ns + (j == 0 ? ns - 1 : j - 1);
+      double azisgacc = 0.5 * (c->sg_acct[l] + c->sg_acct[lm1]);
+      vtheta[l] += Dt * azisgacc;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* InitEuler + InitGasVelocities                                              */
+/* ------------------------------------------------------------------------ */
+
+/* InitGasVelocities (src/Pframeforce.c:442-641), no centrifugal balance */
+static void init_gas_velocities(or_ctx *c) {
+  const dfargo_params *p = &c->p;
+  int nr = c->nr, ns = c->ns;
+  double *vr = c->vr, *vt = c->vt;
+  for (int i = 0; i < nr; i++) {
+    double r = c->Rmed[i], omega = sqrt(G * 1.0 / r / r / r);
+    double myvtheta = omega * r * sqrt(1.0 - pow(p->aspect_ratio, 2.0) * pow(r, 2.0 * p->flaring_index) *
+                                                 (1. + p->sigma_slope - 2.0 * p->flaring_index));
+    for (int j = 0; j < ns; j++) vt[j + i * ns] = myvtheta;
+  }
+  if (p->self_gravity) {
+    make_1d_profile(c, c->sg_accr, c->AxiSG);
+    for (int i = 0; i < nr; i++) {
+      double r = c->Rmed[i];
+      for (int j = 0; j < ns; j++) {
+        int l = i * ns + j;
+        double vt2 = vt[l] * vt[l];
+        vt2 -= r * c->AxiSG[i];
+        vt[l] = sqrt(vt2);
+      }
+    }
+  }
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) vt[i * ns + j] -= c->OmegaFrame * c->Rmed[i];
+  if (p->alpha_viscosity != 0.0) make_1d_profile(c, c->cs, c->GlobalCs);
+  for (int i = 0; i < nr; i++) {
+    double r = c->Rmed[i], ri = c->Rinf[i];
+    double viscosity = has_viscosity(p) ? or_fviscosity(c, r) : 0.0;
+    for (int j = 0; j < ns; j++) {
+      int l = i * ns + j;
+      if (p->imposed_disk_drift != 0.0)
+        vr[l] = p->imposed_disk_drift * p->sigma0 / c->SigmaInf[i] / ri;
+      else if (p->alpha_viscosity != 0.0)
+        vr[l] -= 3.0 * viscosity / r * (-p->sigma_slope + 2.0 * p->flaring_index + 1.0);
+      else
+        vr[l] -= 3.0 * viscosity / r * (-p->sigma_slope + .5);
+    }
+  }
+  for (int i = 0; i < nr; i++) {
+    c->VthetaMed[i] = vt[i * ns] + c->Rmed[i] * c->OmegaFrame;
+    c->VradMed[i] = vr[i * ns];
+  }
+}
+
+/* init_planetarysys_withSG (src/sgsysinit.c:3-49), non-binary */
+static void init_planetarysys_with_sg(or_ctx *c) {
+  make_1d_profile(c, c->sg_accr, c->AxiSG);
+  for (int k = 0; k < c->sys.nb; k++) {
+    double x = c->sys.x[k], y = c->sys.y[k], r = sqrt(x * x + y * y), sgacc;
+    double dist = (double)(r / (1. + c->p.eccentricity));
+    if (dist < c->GlobalRmed[c->nr - 1]) {
+      int ipl = 0;
+      while ((c->GlobalRmed[ipl] <= dist) && (ipl < c->nr - 2)) ipl++;
+      double ri = c->GlobalRmed[ipl], rip1 = c->GlobalRmed[ipl + 1], dr = rip1 - ri;
+      sgacc = (dist - ri) * c->AxiSG[ipl + 1] + (rip1 - dist) * c->AxiSG[ipl];
+      sgacc /= dr;
+    } else
+      sgacc = c->AxiSG[c->nr - 1];
+    c->sys.vy[k] *= (double)sqrt(1. - dist * dist * sgacc / G / (1.0 + c->sys.mass[k]));
+  }
+}
+
+int or_init_euler(or_ctx *c, const double *rho, double *vrad, double *vtheta,
+                  const double *energy, const dfargo_planets *sys) {
+  size_t n = (size_t)(c->nr + 1) * c->ns;
+  memcpy(c->rho, rho, sizeof(double) * n);
+  if (energy) memcpy(c->en, energy, sizeof(double) * n);
+  if (sys) c->sys = *sys; else memset(&c->sys, 0, sizeof(c->sys));
+  memset(c->vr, 0, sizeof(double) * c->gsize);
+  memset(c->vt, 0, sizeof(double) * c->gsize);
+  if (c->p.self_gravity) { /* src/main.c:178-186 */
+    compute_selfgravity(c, c->vr, c->vt, 0.0, 0);
+    init_planetarysys_with_sg(c);
+  }
+  compute_sound_speed(c);
+  compute_pressure(c);
+  compute_temperature(c);
+  init_gas_velocities(c);
+  compute_viscous_terms(c, c->vr, c->vt);
+  if (c->p.energy_equation && c->p.viscous_heating) compute_viscous_heating(c);
+  memcpy(vrad, c->vr, sizeof(double) * n);
+  memcpy(vtheta, c->vt, sizeof(double) * n);
+  return 0;
+}
+
+void or_set_state(or_ctx *c, const double *rho, const double *vrad, const double *vtheta,
+                  const double *energy, const dfargo_planets *sys) {
+  size_t n = (size_t)(c->nr + 1) * c->ns;
+  if (rho) memcpy(c->rho, rho, sizeof(double) * n);
+  if (vrad) memcpy(c->vr, vrad, sizeof(double) * n);
+  if (vtheta) memcpy(c->vt, vtheta, sizeof(double) * n);
+  if (energy) memcpy(c->en, energy, sizeof(double) * n);
+  if (sys) c->sys = *sys;
+}
+
+void or_get_state(or_ctx *c, double *rho, double *vrad, double *vtheta, double *energy,
+                  dfargo_planets *sys) {
+  size_t n = (size_t)(c->nr + 1) * c->ns;
+  if (rho) memcpy(rho, c->rho, sizeof(double) * n);
+  if (vrad) memcpy(vrad, c->vr, sizeof(double) * n);
+  if (vtheta) memcpy(vtheta, c->vt, sizeof(double) * n);
+  if (energy) memcpy(energy, c->en, sizeof(double) * n);
+  if (sys) *sys = c->sys;
+}
+
+void or_set_time(or_ctx *c, double t, double t0, double omega_frame) {
+  c->PhysicalTime = t;
+  c->PhysicalTimeInitial = t0;
+  c->OmegaFrame = omega_frame;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CFL                                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* ConditionCFL (src/SourceEuler.c:1304-1552), gas only, no diffusion */
+int or_condition_cfl(or_ctx *c, double deltaT) {
+  int nr = c->nr, ns = c->ns;
+  double *Vmoy = c->vmoy, newdt = 1e30, viscosity = 0.0;
+  double *Vres = (double *)malloc(sizeof(double) * (ns + 1));
+  const double *vt = c->vt, *vr = c->vr, pm = c->p.pmax - c->p.pmin;
+  for (int i = 0; i < nr; i++) {
+    Vmoy[i] = 0.0;
+    for (int j = 0; j < ns; j++) Vmoy[i] += vt[j + i * ns];
+    Vmoy[i] /= (double)ns;
+  }
+  for (int i = 1; i < nr; i++) {
+    if (has_viscosity(&c->p)) viscosity = or_fviscosity(c, c->Rmed[i]);
+    double dxrad = c->Rsup[i] - c->Rinf[i];
+    double dxtheta = c->Rmed[i] * pm / (double)ns;
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns;
+      Vres[j] = c->p.fast_transport ? vt[l] - Vmoy[i] : vt[l];
+    }
+    Vres[ns] = Vres[0];
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      double cs = c->cs[l];
+      double invdt1 = cs / (min2(dxrad, dxtheta));
+      double invdt2 = fabs(vr[l]) / dxrad;
+      double invdt3 = fabs(Vres[j]) / dxtheta;
+      double dvr = vr[lip] - vr[l], dvt = vt[ljp] - vt[l], invdt4, invdt5, invdt6 = 1e-10, dt;
+      if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
+      if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
+      invdt4 = max2(dvr / dxrad, dvt / dxtheta);
+      invdt4 *= 4.0 * CVNR * CVNR;
+      if (viscosity != 0.0)
+        invdt5 = viscosity * 4.0 / pow(min2(dxrad, dxtheta), 2.0);
+      else
+        invdt5 = 1e-10;
+      dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 +
+                              invdt4 * invdt4 + invdt5 * invdt5 + invdt6 * invdt6);
+      if (dt < newdt) newdt = dt;
+    }
+  }
+  for (int i = 0; i < nr - 1; i++) {
+    double dt = pm * CFLSECURITY / (double)c->ns /
+                fabs(Vmoy[i] * c->InvRmed[i] - Vmoy[i + 1] * c->InvRmed[i + 1]);
+    if (dt < newdt) newdt = dt;
+  }
+  if (deltaT < newdt) newdt = deltaT;
+  free(Vres);
+  return (int)(ceil(deltaT / newdt));
+}
+
+/* ------------------------------------------------------------------------ */
+/* source steps                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* SubStep1 (src/SourceEuler.c:765-1009), gas without dust fluid */
+void or_substep1(or_ctx *c, double dt) {
+  const dfargo_params *p = &c->p;
+  int nr = c->nr, ns = c->ns;
+  const double *rho = c->rho, *vrad = c->vr, *vtheta = c->vt, *Pot = c->pot, *Press = c->pres;
+  double *vradint = c->vradint, *vthetaint = c->vthetaint;
+  for (int i = 1; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lim = l - ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      double gradp = (Press[l] - Press[lim]) * 2.0 / (rho[l] + rho[lim]) * c->InvDiffRmed[i];
+      double gradphi = (Pot[l] - Pot[lim]) * c->InvDiffRmed[i];
+      double vt2 = vtheta[l] + vtheta[ljp] + vtheta[lim] + vtheta[ljp - ns];
+      vt2 = vt2 / 4.0 + c->Rinf[i] * c->OmegaFrame;
+      vt2 = vt2 * vt2;
+      vradint[l] = vrad[l] + dt * (-gradp - gradphi + vt2 * c->InvRinf[i]);
+    }
+  for (int i = 0; i < nr; i++) {
+    double dxtheta = (p->pmax - p->pmin) / (double)ns * c->Rmed[i];
+    double invdxtheta = 1.0 / dxtheta;
+    double supp_torque = p->imposed_disk_drift * 0.5 * pow(c->Rmed[i], -2.5) * p->sigma0 / c->SigmaMed[i];
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      double gradp = (Press[l] - Press[ljm]) * 2.0 / (rho[l] + rho[ljm]) * invdxtheta;
+      double gradphi = (Pot[l] - Pot[ljm]) * invdxtheta;
+      vthetaint[l] = vtheta[l] - dt * (gradp + gradphi);
+      vthetaint[l] += dt * supp_torque;
+    }
+  }
+  if (p->self_gravity) compute_selfgravity(c, vradint, vthetaint, dt, 1);
+  if (has_viscosity(p)) {
+    compute_viscous_terms(c, vradint, vthetaint);
+    update_velocities_viscosity(c, vradint, vthetaint, dt);
+  }
+}
+
+/* SubStep2 (src/SourceEuler.c:1012-1116) */
+static void substep2(or_ctx *c, double dt) {
+  int nr = c->nr, ns = c->ns;
+  const double *rho = c->rho, *vrad = c->vradint, *vtheta = c->vthetaint, *energy = c->en;
+  double *qr = c->rhoint, *qt = c->temperint;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      double dv = vrad[lip] - vrad[l];
+      qr[l] = (dv < 0.0) ? CVNR * CVNR * rho[l] * dv * dv : 0.0;
+      dv = vtheta[ljp] - vtheta[l];
+      qt[l] = (dv < 0.0) ? CVNR * CVNR * rho[l] * dv * dv : 0.0;
+    }
+  for (int i = 1; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lim = l - ns;
+      c->vradnew[l] = vrad[l] - dt * 2.0 / (rho[l] + rho[lim]) * (qr[l] - qr[lim]) * c->InvDiffRmed[i];
+    }
+  for (int i = 0; i < nr; i++) {
+    double dxtheta = (c->p.pmax - c->p.pmin) / (double)ns * c->Rmed[i];
+    double invdxtheta = 1.0 / dxtheta;
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lip = l + ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      int ljp = (j == ns - 1) ? i * ns : l + 1;
+      c->vthetanew[l] = vtheta[l] - dt * 2.0 / (rho[l] + rho[ljm]) * (qt[l] - qt[ljm]) * invdxtheta;
+      if (c->p.energy_equation)
+        c->energyint[l] = energy[l] - dt * qr[l] * (vrad[lip] - vrad[l]) * c->InvDiffRsup[i] -
+                          dt * qt[l] * (vtheta[ljp] - vtheta[l]) * invdxtheta;
+    }
+  }
+}
+
+/* SubStep3 (src/SourceEuler.c:1119-1302), no thermal cooling/TempPresc */
+static void substep3(or_ctx *c, double dt) {
+  const dfargo_params *p = &c->p;
+  int nr = c->nr, ns = c->ns;
+  const double *energy = c->energyint, *vtheta = c->vt;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns;
+      double num = energy[l] + (p->viscous_heating ? 1 : 0) * c->vischeat[l] * dt;
+      double den = 1.0 + (p->adiabatic_index - 1.0) * dt * c->divv[l];
+      c->energynew[l] = num / den;
+    }
+  if (p->beta_cooling) {
+    for (int i = 0; i < nr; i++) {
+      double omega = 0.0;
+      for (int j = 0; j < ns; j++) omega += (vtheta[j + i * ns] + c->Rmed[i] * c->OmegaFrame);
+      omega /= (double)ns;
+      omega /= c->Rmed[i];
+      double beta = p->beta_cooling_time * pow(c->Rmed[i], -p->beta_cooling_slope);
+      double coolingtime = beta / omega;
+      for (int j = 0; j < ns; j++) {
+        int l = j + i * ns;
+        double num = c->energynew[l], den = 1.0 + dt / coolingtime;
+        c->energynew[l] = num / den;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* transport                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeLRMomenta (src/TransportEuler.c:89-118) */
+static void compute_lr_momenta(or_ctx *c) {
+  int nr = c->nr, ns = c->ns;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      c->rp[l] = c->rho[l] * c->vr[lip];
+      c->rm[l] = c->rho[l] * c->vr[l];
+      c->tp[l] = c->rho[l] * (c->vt[ljp] + c->Rmed[i] * c->OmegaFrame) * c->Rmed[i];
+      c->tm[l] = c->rho[l] * (c->vt[l] + c->Rmed[i] * c->OmegaFrame) * c->Rmed[i];
+    }
+}
+
+/* ComputeStarRad (src/TransportEuler.c:142-180) */
+static void compute_star_rad(or_ctx *c, const double *qb, const double *vr, double *qs, double dt) {
+  int nr = c->nr, ns = c->ns;
+  double *dq = c->dq;
+  for (int j = 0; j < ns; j++) {
+    for (int i = 0; i < nr; i++) {
+      int l = j + i * ns, lq = i + j * nr, lip = l + ns, lim = l - ns;
+      if ((i == 0) || (i == nr - 1))
+        dq[lq] = 0.0;
+      else {
+        double dqm = (qb[l] - qb[lim]) * c->InvDiffRmed[i];
+        double dqp = (qb[lip] - qb[l]) * c->InvDiffRmed[i + 1];
+        dq[lq] = (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
+      }
+    }
+    for (int i = 1; i < nr; i++) {
+      int l = j + i * ns, lq = i + j * nr, lim = l - ns;
+      if (vr[l] > 0.0)
+        qs[l] = qb[lim] + (c->Rmed[i] - c->Rmed[i - 1] - vr[l] * dt) * 0.5 * dq[lq - 1];
+      else
+        qs[l] = qb[l] - (c->Rmed[i + 1] - c->Rmed[i] + vr[l] * dt) * 0.5 * dq[lq];
+    }
+    qs[j] = qs[j + ns * nr] = 0.0;
+  }
+}
+
+/* DivisePolarGrid (src/SideEuler.c:244-261) over rings 0..nr-1 */
+static void divise(or_ctx *c, const double *num, const double *den, double *res) {
+  for (int l = 0; l < c->nr * c->ns; l++) res[l] = num[l] / (den[l] + 1e-20);
+}
+
+/* VanLeerRadial (src/TransportEuler.c:182-220) */
+static double van_leer_radial(or_ctx *c, double *qb, double dt) {
+  int nr = c->nr, ns = c->ns;
+  double dtheta = (c->p.pmax - c->p.pmin) / (double)ns, lost = 0.0;
+  divise(c, qb, c->rhoint, c->work);
+  compute_star_rad(c, c->work, c->vr, c->qrstar, dt);
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lip = l + ns;
+      double varq = dt * dtheta * c->Rinf[i] * c->qrstar[l] * c->rhostar[l] * c->vr[l];
+      varq -= dt * dtheta * c->Rsup[i] * c->qrstar[lip] * c->rhostar[lip] * c->vr[lip];
+      qb[l] += varq * c->InvSurf[i];
+      if (i == 0) lost += varq;
+    }
+  return lost;
+}
+
+/* ComputeStarTheta (src/TransportEuler.c:419-444) */
+static void compute_star_theta(or_ctx *c, const double *qb, const double *vt, double *qs, double dt) {
+  int nr = c->nr, ns = c->ns;
+  double *dq = c->dq;
+  for (int i = 0; i < nr; i++) {
+    double dxtheta = (c->p.pmax - c->p.pmin) / (double)ns * c->Rmed[i];
+    double invdxtheta = 1.0 / dxtheta;
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, ljp = (j == ns - 1) ? i * ns : l + 1, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      double dqm = (qb[l] - qb[ljm]), dqp = (qb[ljp] - qb[l]);
+      dq[l] = (dqp * dqm > 0.0) ? dqp * dqm / (dqp + dqm) * invdxtheta : 0.0;
+    }
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, ljm = (j == 0 ? ns - 1 : j - 1) + i * ns;
+      double ksi = vt[l] * dt;
+      if (ksi > 0.0)
+        qs[l] = qb[ljm] + (dxtheta - ksi) * dq[ljm];
+      else
+        qs[l] = qb[l] - (dxtheta + ksi) * dq[l];
+    }
+  }
+}
+
+/* VanLeerTheta (src/TransportEuler.c:446-465) */
+static void van_leer_theta(or_ctx *c, const double *vt, double *qb, double dt, int uniform) {
+  int nr = c->nr, ns = c->ns;
+  divise(c, qb, c->rhoint, c->work);
+  compute_star_theta(c, c->work, vt, c->qrstar, dt);
+  for (int i = 0; i < nr; i++) {
+    double dxrad = (c->Rsup[i] - c->Rinf[i]) * dt, invsurf = 1.0 / c->Surf[i];
+    if (!uniform || !c->nosplit[i]) {
+      for (int j = 0; j < ns; j++) {
+        int l = j + i * ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+        double varq = dxrad * c->qrstar[l] * c->rhostar[l] * vt[l];
+        varq -= dxrad * c->qrstar[ljp] * c->rhostar[ljp] * vt[ljp];
+        qb[l] += varq * invsurf;
+      }
+    }
+  }
+}
+
+/* QuantitiesAdvection (src/TransportEuler.c:404-417) */
+static void quantities_advection(or_ctx *c, const double *vt, double dt, int uniform) {
+  compute_star_theta(c, c->rho, vt, c->rhostar, dt);
+  memcpy(c->rhoint, c->rho, sizeof(double) * (c->nr + 1) * c->ns);
+  van_leer_theta(c, vt, c->rp, dt, uniform);
+  van_leer_theta(c, vt, c->rm, dt, uniform);
+  van_leer_theta(c, vt, c->tp, dt, uniform);
+  van_leer_theta(c, vt, c->tm, dt, uniform);
+  if (c->p.energy_equation) van_leer_theta(c, vt, c->en, dt, uniform);
+  van_leer_theta(c, vt, c->rho, dt, uniform);
+}
+
+/* AdvectSHIFT (src/TransportEuler.c:350-373) */
+static void advect_shift(or_ctx *c, double *val) {
+  int nr = c->nr, ns = c->ns;
+  for (int i = 0; i < nr; i++) {
+    for (int j = 0; j < ns; j++) {
+      long ji = j - c->nshift[i];
+      while (ji < 0) ji += ns;
+      while (ji >= ns) ji -= ns;
+      c->tempshift[j + i * ns] = val[ji + i * ns];
+    }
+    for (int j = 0; j < ns; j++) val[j + i * ns] = c->tempshift[j + i * ns];
+  }
+}
+
+/* OneWindTheta with its helpers (src/TransportEuler.c:225-348) */
+static void one_wind_theta(or_ctx *c, double dt) {
+  int nr = c->nr, ns = c->ns;
+  double invdt = 1.0 / dt, pm = c->p.pmax - c->p.pmin, dpinvns = pm / (double)ns;
+  for (int l = 0; l < nr * ns; l++) c->elong[l] = c->vt[l] * dt;
+  for (int i = 0; i < nr; i++) {
+    double moy = 0.0;
+    for (int j = 0; j < ns; j++) moy += c->elong[j + i * ns] * invdt;
+    c->vmed[i] = moy / (double)ns;
+  }
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) c->vthetares[j + i * ns] = c->elong[j + i * ns] * invdt - c->vmed[i];
+  for (int i = 0; i < nr; i++) {
+    double Ntilde = c->vmed[i] * c->InvRmed[i] * dt * (double)ns / pm;
+    double Nround = floor(Ntilde + 0.5);
+    c->nshift[i] = (long)Nround;
+    for (int j = 0; j < ns; j++) c->vt[j + i * ns] = (Ntilde - Nround) * c->Rmed[i] * invdt * dpinvns;
+    if (!c->p.fast_transport) {
+      c->nosplit[i] = 1;
+      for (int j = 0; j < ns; j++) {
+        int l = j + i * ns;
+        c->vthetares[l] = c->vt[l] + c->vthetares[l];
+        c->vt[l] = 0.0;
+      }
+    } else
+      c->nosplit[i] = 0;
+  }
+  quantities_advection(c, c->vthetares, dt, 0);
+  quantities_advection(c, c->vt, dt, 1);
+  advect_shift(c, c->rp);
+  advect_shift(c, c->rm);
+  advect_shift(c, c->tp);
+  advect_shift(c, c->tm);
+  if (c->p.energy_equation) advect_shift(c, c->en);
+  advect_shift(c, c->rho);
+}
+
+/* ComputeVelocities (src/TransportEuler.c:515-547) */
+static void compute_velocities(or_ctx *c) {
+  int nr = c->nr, ns = c->ns;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns, lim = l - ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      if (i == 0)
+        c->vr[l] = 0.0;
+      else
+        c->vr[l] = (c->rp[lim] + c->rm[l]) / (c->rho[l] + c->rho[lim] + 1e-20);
+      c->vt[l] = (c->tp[ljm] + c->tm[l]) / (c->rho[l] + c->rho[ljm] + 1e-20) / c->Rmed[i] -
+                 c->Rmed[i] * c->OmegaFrame;
+    }
+}
+
+/* Transport (src/TransportEuler.c:65-84) */
+void or_transport(or_ctx *c, double dt) {
+  compute_lr_momenta(c);
+  /* OneWindRad (src/TransportEuler.c:123-137) */
+  compute_star_rad(c, c->rho, c->vr, c->rhostar, dt);
+  memcpy(c->rhoint, c->rho, sizeof(double) * (c->nr + 1) * c->ns);
+  van_leer_radial(c, c->rp, dt);
+  van_leer_radial(c, c->rm, dt);
+  van_leer_radial(c, c->tp, dt);
+  van_leer_radial(c, c->tm, dt);
+  if (c->p.energy_equation) van_leer_radial(c, c->en, dt);
+  c->LostMass += van_leer_radial(c, c->rho, dt);
+  one_wind_theta(c, dt);
+  compute_velocities(c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* boundaries, floors, crash                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* ApplySubKeplerianBoundary (src/SideEuler.c:1008-1044) */
+static void subkeplerian_boundary(or_ctx *c) {
+  int nr = c->nr, ns = c->ns;
+  for (int j = 0; j < ns; j++) c->vt[j] = c->VthetaMed[0] - c->Rmed[0] * c->OmegaFrame;
+  for (int j = 0; j < ns; j++)
+    c->vt[(nr - 1) * ns + j] = c->VthetaMed[nr - 1] - c->Rmed[nr - 1] * c->OmegaFrame;
+}
+
+/* ApplyWallBoundaries (src/SideEuler.c:1047-1099) */
+static void wall_boundaries(or_ctx *c) {
+  int nr = c->nr, ns = c->ns, i = 2;
+  for (int j = 0; j < ns; j++) {
+    c->vr[(i - 1) * ns + j] = 0.0;
+    c->vr[(i - 2) * ns + j] = -c->vr[i * ns + j];
+    c->rho[(i - 1) * ns + j] = c->SigmaMed[i - 1];
+    c->rho[(i - 2) * ns + j] = c->SigmaMed[i - 2];
+  }
+  i = nr - 3;
+  for (int j = 0; j < ns; j++) {
+    c->vr[(i + 2) * ns + j] = 0.0;
+    c->vr[(i + 3) * ns + j] = -c->vr[(i + 1) * ns + j];
+    c->rho[(i + 1) * ns + j] = c->SigmaMed[i + 1];
+    c->rho[(i + 2) * ns + j] = c->SigmaMed[i + 2];
+  }
+}
+
+/* OpenBoundary (src/SideEuler.c:300-408), standard (non-KN) prescription */
+static void open_boundary(or_ctx *c) {
+  int nr = c->nr, ns = c->ns, i = 2;
+  double *rho = c->rho, *vr = c->vr, *vt = c->vt, *energy = c->en;
+  for (int j = 0; j < ns; j++) {
+    int l = j + i * ns;
+    rho[l - ns] = rho[l];
+    rho[l - 2 * ns] = rho[l];
+    if (c->p.energy_equation) {
+      energy[l - ns] = energy[l];
+      energy[l - 2 * ns] = energy[l];
+    }
+    vt[l - ns] = vt[l] * sqrt(c->Rmed[i] / c->Rmed[i - 1]);
+    vt[l - 2 * ns] = vt[l] * sqrt(c->Rmed[i] / c->Rmed[i - 2]);
+    if ((vr[l + ns] >= 0.0) || (rho[l] < c->SigmaMed[0])) {
+      vr[l] = 0.0;
+      vr[l - ns] = 0.0;
+    } else {
+      vr[l] = vr[l + ns];
+      vr[l - ns] = vr[l + ns];
+    }
+  }
+  i = nr - 1;
+  for (int j = 0; j < ns; j++) {
+    int l = j + i * ns;
+    rho[l] = rho[l - ns];
+    if (c->p.energy_equation) energy[l] = energy[l - ns];
+    vt[l] = vt[l - ns] * pow(c->Rmed[i] / c->Rmed[i - 1], -0.5);
+    if ((vr[l - ns] < 0.0) || (rho[l] < c->SigmaMed[nr - 2]))
+      vr[l] = 0.0;
+    else
+      vr[l] = vr[l - ns];
+  }
+}
+
+/* ApplyBoundaryCondition (src/SideEuler.c:1102-1131) */
+static void apply_boundary_condition(or_ctx *c) {
+  if (c->p.boundary == DFARGO_BC_WALL) wall_boundaries(c);
+  subkeplerian_boundary(c); /* unconditional: src/SideEuler.c:1110-1112 */
+  if (c->p.boundary == DFARGO_BC_OPEN) open_boundary(c);
+}
+
+/* AddFloorDensity, AddFloorEnergy (src/SourceEuler.c:2506-2541) */
+static void add_floors(or_ctx *c) {
+  for (int i = 0; i < c->nr; i++)
+    for (int j = 0; j < c->ns; j++) {
+      int l = i * c->ns + j;
+      if (c->rho[l] < c->p.density_floor) c->rho[l] = c->p.density_floor;
+    }
+  for (int i = 0; i < c->nr; i++) {
+    double e_bottom = 1e-3 * c->EnergyMed[i];
+    for (int j = 0; j < c->ns; j++) {
+      int l = i * c->ns + j;
+      if (c->en[l] < e_bottom) c->en[l] = e_bottom;
+    }
+  }
+}
+
+/* DetectCrash (src/SourceEuler.c:51-69) */
+static int detect_crash(or_ctx *c, const double *a) {
+  int crashed = 0;
+  for (int l = 0; l < c->nr * c->ns; l++)
+    if (a[l] < 0.0) crashed = 1;
+  return crashed;
+}
+
+/* ------------------------------------------------------------------------ */
+/* AlgoGas                                                                    */
+/* ------------------------------------------------------------------------ */
+
+int or_algogas(or_ctx *c, double DT) {
+  const dfargo_params *p = &c->p;
+  dfargo_planets *sys = &c->sys;
+  int nb = sys->nb, nsub = 0;
+  double dtemp = 0.0, dt, dthydro, dtnbody;
+  size_t n = (size_t)(c->nr + 1) * c->ns;
+  if (p->energy_equation) compute_sound_speed(c);
+  c->GasTimeStepsCFL = 1;
+  while (dtemp < 0.999999999 * DT) {
+    /* mass taper (src/SourceEuler.c:400-418) */
+    if (p->mass_taper > 1e-2) {
+      c->MassTaper = (c->PhysicalTime - c->PhysicalTimeInitial) / (p->mass_taper * 2.0 * M_PI);
+      c->MassTaper = (c->MassTaper > 1.0 ? 1.0 : pow(sin(c->MassTaper * M_PI / 2.0), 2.0));
+      if (c->MassTaper < 1.0)
+        for (int k = 0; k < nb; k++)
+          sys->mass[k] = sys->initial_mass[k] + (sys->final_mass[k] - sys->initial_mass[k]) * c->MassTaper;
+    } else {
+      for (int k = 0; k < nb; k++)
+        if (sys->acc[k] < 1e-10) sys->mass[k] = sys->final_mass[k];
+    }
+    /* n-body limit (src/SourceEuler.c:421-440) */
+    if (nb > 1) {
+      dtnbody = 1e5;
+      for (int k = 0; k < nb; k++)
+        for (int j = k + 1; j < nb; j++) {
+          double xk = sys->x[k], xj = sys->x[j], yk = sys->y[k], yj = sys->y[j];
+          double mk = sys->mass[k], mj = sys->mass[j];
+          double dist = sqrt((xk - xj) * (xk - xj) + (yk - yj) * (yk - yj));
+          double buf = 2.0 * M_PI * sqrt(dist * dist * dist / (mk + mj + 1e-8)) / 100.0;
+          dtnbody = min2(dtnbody, buf);
+        }
+    } else
+      dtnbody = 100.0;
+    c->GasTimeStepsCFL = or_condition_cfl(c, DT - dtemp);
+    dthydro = (DT - dtemp) / (double)c->GasTimeStepsCFL;
+    dt = min2(dthydro, dtnbody);
+    dtemp += dt;
+    if (dt < c->MINDT) c->MINDT = dt;
+    if (dt > c->MAXDT) c->MAXDT = dt;
+    /* forces and planets (src/SourceEuler.c:472-500) */
+    c->dop = compute_accel(c, c->rho, 0.0, 0.0, p->indirect_term_smoothing, 0.0);
+    c->dop_alt = opposite_acc_barycenter(c, dt);
+    fill_forces_arrays(c);
+    advance_system_from_disk(c, dt);
+    advance_system_rk5(c, dt);
+    rotate_psys(c, c->OmegaFrame * dt);
+    /* crash scan (src/SourceEuler.c:545-548) */
+    c->crashed_d = detect_crash(c, c->rho);
+    c->crashed_e = detect_crash(c, c->en);
+    c->crashed = c->crashed_e;
+    compute_sound_speed(c);
+    compute_pressure(c);
+    or_substep1(c, dt);
+    /* ActualiseGas round trips (src/SourceEuler.c:598-612) */
+    memcpy(c->vr, c->vradint, sizeof(double) * n);
+    memcpy(c->vt, c->vthetaint, sizeof(double) * n);
+    substep2(c, dt);
+    memcpy(c->vr, c->vradnew, sizeof(double) * n);
+    memcpy(c->vt, c->vthetanew, sizeof(double) * n);
+    if (p->energy_equation) memcpy(c->en, c->energyint, sizeof(double) * n);
+    compute_divergence(c, c->vr, c->vt);
+    if (p->energy_equation) {
+      memcpy(c->energyint, c->en, sizeof(double) * n);
+      if (p->viscous_heating) {
+        compute_viscous_terms(c, c->vr, c->vt);
+        compute_viscous_heating(c);
+      }
+      substep3(c, dt);
+      memcpy(c->en, c->energynew, sizeof(double) * n);
+    }
+    or_transport(c, dt);
+    if (p->add_floors) add_floors(c);
+    apply_boundary_condition(c);
+    compute_temperature(c);
+    c->PhysicalTime += dt;
+    nsub++;
+  }
+  c->last_substeps = nsub;
+  c->substeps += nsub;
+  return nsub;
+}
+
+void or_get_status(or_ctx *c, dfargo_status *st) {
+  memset(st, 0, sizeof(*st));
+  st->physical_time = c->PhysicalTime;
+  st->physical_time_initial = c->PhysicalTimeInitial;
+  st->omega_frame = c->OmegaFrame;
+  st->mindt = c->MINDT;
+  st->maxdt = c->MAXDT;
+  st->lost_mass = c->LostMass;
+  st->mass_taper = c->MassTaper;
+  st->disk_on_primary_x = c->dop.x;
+  st->disk_on_primary_y = c->dop.y;
+  st->disk_on_primary_alt_x = c->dop_alt.x;
+  st->disk_on_primary_alt_y = c->dop_alt.y;
+  for (int k = 0; k < 3; k++) {
+    st->bary_x[k] = c->barx[k];
+    st->bary_y[k] = c->bary[k];
+  }
+  st->substeps_total = c->substeps;
+  st->last_call_substeps = c->last_substeps;
+  st->gas_timesteps_cfl = c->GasTimeStepsCFL;
+  st->crashed_density = c->crashed_d;
+  st->crashed_energy = c->crashed_e;
+  st->crashed = c->crashed;
+}
+
+void or_get_field(or_ctx *c, int field, double *out) {
+  size_t n = (size_t)(c->nr + 1) * c->ns;
+  const double *src = NULL;
+  switch (field) {
+    case DFARGO_FIELD_POTENTIAL: src = c->pot; break;
+    case DFARGO_FIELD_SOUNDSPEED: src = c->cs; break;
+    case DFARGO_FIELD_PRESSURE: src = c->pres; break;
+    case DFARGO_FIELD_TEMPERATURE: src = c->temp; break;
+    case DFARGO_FIELD_DIVV: src = c->divv; break;
+    case DFARGO_FIELD_VISCHEAT: src = c->vischeat; break;
+    case DFARGO_FIELD_SG_ACCR: src = c->gr; break;
+    case DFARGO_FIELD_SG_ACCT: src = c->gtheta; break;
+    case DFARGO_FIELD_TAURR: src = c->trr; break;
+    case DFARGO_FIELD_TAURP: src = c->trp; break;
+    case DFARGO_FIELD_TAUPP: src = c->tpp; break;
+    default: memset(out, 0, sizeof(double) * n); return;
+  }
+  memcpy(out, src, sizeof(double) * n);
+}
+
+void or_get_fargo_shift(or_ctx *c, long *nshift, double *vmed) {
+  if (nshift) memcpy(nshift, c->nshift, sizeof(long) * c->nr);
+  if (vmed) memcpy(vmed, c->vmed, sizeof(double) * c->nr);
+}

@@ -1,0 +1,72 @@
+/*
+ * oracle.h -- FP64 CPU restatement of Dusty FARGO-ADSG's gas hydro step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links or loads this
+ * code; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg use it, as the checker the HIP path is compared against.
+ *
+ * The reference cannot be built here: its sources include headers that
+ * src/varparser.pl generates at build time (param.h, global_ex.h, sg_ex.h),
+ * and its self-gravity needs FFTW-MPI.  This file restates the algorithm
+ * from the sources instead; every function cites the reference lines it
+ * follows.  How each function is pinned is listed in oracle.c's header.
+ */
+#ifndef DFARGO_ORACLE_H
+#define DFARGO_ORACLE_H
+
+#include "../include/dfargo.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_ctx or_ctx;
+
+or_ctx *or_create(const dfargo_params *p);
+void or_destroy(or_ctx *c);
+
+/* tables of FillPolar1DArrays: radii (nrad+1), rmed, surf (nrad), azimuth (nsec) */
+void or_get_tables(or_ctx *c, double *radii, double *rmed, double *surf,
+                   double *azimuth);
+
+/* host initial state (InitGasDensity, InitGasEnergy, InitPlanetarySystem) */
+void or_init_gas_density(or_ctx *c, double *rho);
+void or_init_gas_energy(or_ctx *c, double *energy);
+void or_init_planetary_system(or_ctx *c, int nb, const double *dist,
+                              const double *mass, const double *acc,
+                              const int *feel_disk, const int *feel_others,
+                              dfargo_planets *out);
+
+/* start-up SG call + InitEuler; writes the initial velocities */
+int or_init_euler(or_ctx *c, const double *rho, double *vrad, double *vtheta,
+                  const double *energy, const dfargo_planets *sys);
+
+void or_set_state(or_ctx *c, const double *rho, const double *vrad,
+                  const double *vtheta, const double *energy,
+                  const dfargo_planets *sys);
+void or_get_state(or_ctx *c, double *rho, double *vrad, double *vtheta,
+                  double *energy, dfargo_planets *sys);
+void or_set_time(or_ctx *c, double t, double t0, double omega_frame);
+
+/* AlgoGas; returns the number of sub-steps taken */
+int or_algogas(or_ctx *c, double DT);
+void or_get_status(or_ctx *c, dfargo_status *st);
+void or_get_field(or_ctx *c, int field, double *out);
+void or_get_fargo_shift(or_ctx *c, long *nshift, double *vmed);
+
+/* pieces, for pinning tests */
+int or_condition_cfl(or_ctx *c, double deltaT);
+void or_transport(or_ctx *c, double dt);
+void or_substep1(or_ctx *c, double dt);
+void or_viscous(or_ctx *c, double dt);     /* ComputeViscousTerms + Update on Vrad/Vtheta */
+void or_sg_accel(or_ctx *c, const double *rho, double *accr, double *acct);
+void or_sg_accel_direct(or_ctx *c, const double *rho, double *accr, double *acct);
+void or_fft(double *re, double *im, int n, int sign);
+void or_rk5(const double *q0, double dt, const double *masses, double *q1,
+            int n, const int *feelothers, int indirect_term);
+double or_fviscosity(or_ctx *c, double r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

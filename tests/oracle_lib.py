@@ -1,0 +1,154 @@
+"""numpy wrapper of oracle/liboracle.so (the FP64 CPU restatement).
+
+Test infrastructure: used only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from dustyfargoadsg_amd._capi import Params, Planets, Status, FIELDS
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = ctypes.CDLL(ORACLE_SO)
+        P, PP = ctypes.POINTER(Params), ctypes.POINTER(Planets)
+        dp, vp, ip = ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
+        sig = {
+            "or_create": (vp, [P]), "or_destroy": (None, [vp]),
+            "or_get_tables": (None, [vp, dp, dp, dp, dp]),
+            "or_init_gas_density": (None, [vp, dp]),
+            "or_init_gas_energy": (None, [vp, dp]),
+            "or_init_planetary_system": (None, [vp, ctypes.c_int, dp, dp, dp, ip, ip, PP]),
+            "or_init_euler": (ctypes.c_int, [vp, dp, dp, dp, dp, PP]),
+            "or_set_state": (None, [vp, dp, dp, dp, dp, PP]),
+            "or_get_state": (None, [vp, dp, dp, dp, dp, PP]),
+            "or_set_time": (None, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "or_algogas": (ctypes.c_int, [vp, ctypes.c_double]),
+            "or_get_status": (None, [vp, ctypes.POINTER(Status)]),
+            "or_get_field": (None, [vp, ctypes.c_int, dp]),
+            "or_get_fargo_shift": (None, [vp, ctypes.POINTER(ctypes.c_long), dp]),
+            "or_condition_cfl": (ctypes.c_int, [vp, ctypes.c_double]),
+            "or_transport": (None, [vp, ctypes.c_double]),
+            "or_substep1": (None, [vp, ctypes.c_double]),
+            "or_viscous": (None, [vp, ctypes.c_double]),
+            "or_sg_accel": (None, [vp, dp, dp, dp]),
+            "or_sg_accel_direct": (None, [vp, dp, dp, dp]),
+            "or_fft": (None, [dp, dp, ctypes.c_int, ctypes.c_int]),
+            "or_rk5": (None, [dp, ctypes.c_double, dp, dp, ctypes.c_int, ip, ctypes.c_int]),
+            "or_fviscosity": (ctypes.c_double, [vp, ctypes.c_double]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_planets(ctx_lib, ctx, spec):
+    nb = len(spec["dist"])
+    out = Planets()
+    d = np.array(spec["dist"] + [0.0], dtype=np.float64)
+    m = np.array(spec["mass"] + [0.0], dtype=np.float64)
+    a = np.array(spec["acc"] + [0.0], dtype=np.float64)
+    fd = (ctypes.c_int * (nb + 1))(*(spec["feel_disk"] + [0]))
+    fo = (ctypes.c_int * (nb + 1))(*(spec["feel_others"] + [0]))
+    ctx_lib.or_init_planetary_system(ctx, nb, ptr(d), ptr(m), ptr(a), fd, fo, ctypes.byref(out))
+    return out
+
+
+class Oracle:
+    """One disc on the CPU oracle."""
+
+    def __init__(self, params):
+        self.p = params.copy()
+        self.nr, self.ns = params.nrad, params.nsec
+        self.L = lib()
+        self.ctx = self.L.or_create(ctypes.byref(self.p))
+        self.n = (self.nr + 1) * self.ns
+
+    def __del__(self):
+        try:
+            self.L.or_destroy(self.ctx)
+        except Exception:
+            pass
+
+    def zeros(self):
+        return np.zeros(self.n, dtype=np.float64)
+
+    def tables(self):
+        radii = np.zeros(self.nr + 1); rmed = np.zeros(self.nr)
+        surf = np.zeros(self.nr); az = np.zeros(self.ns)
+        self.L.or_get_tables(self.ctx, ptr(radii), ptr(rmed), ptr(surf), ptr(az))
+        return radii, rmed, surf, az
+
+    def initial_state(self, planet_spec=None):
+        """InitGasDensity, InitGasEnergy, InitPlanetarySystem, InitEuler."""
+        rho, e = self.zeros(), self.zeros()
+        self.L.or_init_gas_density(self.ctx, ptr(rho))
+        if self.p.energy_equation:
+            self.L.or_init_gas_energy(self.ctx, ptr(e))
+        sys_ = make_planets(self.L, self.ctx, planet_spec or dict(
+            dist=[], mass=[], acc=[], feel_disk=[], feel_others=[]))
+        vr, vt = self.zeros(), self.zeros()
+        self.L.or_init_euler(self.ctx, ptr(rho), ptr(vr), ptr(vt), ptr(e), ctypes.byref(sys_))
+        return dict(rho=rho, vr=vr, vt=vt, energy=e, planets=sys_)
+
+    def set_state(self, st):
+        self.L.or_set_state(self.ctx, ptr(st["rho"]), ptr(st["vr"]), ptr(st["vt"]),
+                            ptr(st.get("energy")),
+                            ctypes.byref(st["planets"]) if st.get("planets") is not None else None)
+
+    def state(self):
+        rho, vr, vt, e = self.zeros(), self.zeros(), self.zeros(), self.zeros()
+        sys_ = Planets()
+        self.L.or_get_state(self.ctx, ptr(rho), ptr(vr), ptr(vt), ptr(e), ctypes.byref(sys_))
+        return dict(rho=rho, vr=vr, vt=vt, energy=e, planets=sys_)
+
+    def algogas(self, DT):
+        return self.L.or_algogas(self.ctx, DT)
+
+    def status(self):
+        st = Status()
+        self.L.or_get_status(self.ctx, ctypes.byref(st))
+        return st
+
+    def field(self, name):
+        out = self.zeros()
+        self.L.or_get_field(self.ctx, FIELDS[name], ptr(out))
+        return out
+
+    def fargo_shift(self):
+        ns = (ctypes.c_long * self.nr)()
+        vm = np.zeros(self.nr)
+        self.L.or_get_fargo_shift(self.ctx, ns, ptr(vm))
+        return np.array(ns[:], dtype=np.int64), vm
+
+    def sg_accel(self, rho, direct=False):
+        ar, at = self.zeros(), self.zeros()
+        f = self.L.or_sg_accel_direct if direct else self.L.or_sg_accel
+        f(self.ctx, ptr(np.ascontiguousarray(rho, dtype=np.float64)), ptr(ar), ptr(at))
+        return ar, at
