@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Hydro sub-step throughput on one MI355X.
+
+A step is one hydro sub-step of AlgoGas (one pass of the reference's
+`while (dtemp < DT)` body) over the whole disc; the metric is cell-updates
+per second with the state resident in HBM.  The default workload is the
+adiabatic self-gravitating disc at 512 x 1536 (energy equation, beta cooling
+10, alpha 1e-3, SG by 1024 x 1536 FFT), timed with FP32 storage (the
+headline `value`) and with FP64 storage beside it.
+
+Prints one JSON line.  Per-kernel times come from HIP events on the
+library's stream in a second pass over the same sub-steps; the roofline
+entry is the kernel with the largest share of the step time.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def kernel_bytes_per_cell(name, w, energy, sg):
+    """Compulsory HBM bytes per cell of one launch of each kernel: every
+    field it must read once and write once (tables and scalars count 0)."""
+    ns = 4 if energy else 3                  # Sigma, vr, vt (+ e)
+    nq = 6 if energy else 5                  # transported quantities
+    table = {
+        "cfl_ring": (3 + (2 if energy else 0)) * w,     # rho, vr, vt (+cs, e)
+        "eos": (2 + 1) * w,                              # rho, e -> cs
+        "substep1": (ns + 2) * w,                        # state in, vr*, vt* out
+        "visc_stress": 3 * w + 3 * w,                    # rho, vr*, vt* -> 3 stresses
+        "visc_update": 3 * w + 2 * w + 2 * w,            # rho + 3 stresses, update vr*, vt*
+        "substep2": (3 + (1 if energy else 0)) * w + (2 + (1 if energy else 0)) * w,
+        "energy": 4 * w + 6 * w,                         # rho, vr, vt, e -> e, divv, vh, 3 tau
+        "transport_rad": ns * w + nq * w,
+        "transport_azi": (nq + 1) * w + nq * w,
+        "velocities": nq * w + ns * w,
+        "sg_rows_fwd": w + 2 * w,                        # Sigma -> complex row spectra
+        # Z columns in, the two half kernel spectra ((Ns/2+1) x 2 Nrad complex
+        # each, 4 words per cell), W columns out
+        "sg_cols": 2 * w + 4 * w + 2 * w,
+        "sg_rows_inv": 2 * w + w + 2 * w,                # W row, Sigma -> g_r, g_theta
+        "sg_kick": 4 * w + 2 * w,
+        "boundary": 0, "begin": 0, "control": 0,
+    }
+    return table.get(name, 0)
+
+
+def step_bytes_per_cell(w, energy, sg):
+    """Algorithmic bytes per cell per sub-step: each state field read and
+    written once, plus the two kernel spectra of self-gravity (2 x 2Nr x
+    (Ns/2+1) complex values, ~4 words per cell)."""
+    ns = 4 if energy else 3
+    return 2 * ns * w + (4 * w if sg else 0)
+
+
+def run_gpu(params, planets, precision, steps, warmup, prof_steps):
+    from dustyfargoadsg_amd.disc import Disc
+    d = Disc(params, precision)
+    d.initial_state(planets)
+    DT = 1.0e3  # an AlgoGas call much longer than the CFL step
+    d.run_substeps(DT, warmup)
+    t0 = time.perf_counter()
+    d.run_substeps(DT, steps)  # returns after the stream has drained
+    t1 = time.perf_counter()
+    st = d.status()
+    assert st.last_call_substeps == steps, (st.last_call_substeps, steps)
+    d.set_profiling(True)
+    d.run_substeps(DT, prof_steps)
+    kt = d.kernel_times()
+    d.set_profiling(False)
+    d.close()
+    return t1 - t0, kt
+
+
+def cpu_baseline(params, planets, budget_s):
+    """The FP64 oracle (a single-threaded C restatement of the sub-step) on
+    the host, one sub-step per AlgoGas call, for about budget_s seconds."""
+    from tests.oracle_lib import Oracle
+    o = Oracle(params)
+    o.initial_state(planets)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.algogas(1e-7)  # shorter than the CFL step: one sub-step per call
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s and n >= 2:
+            break
+    return n * params.nrad * params.nsec / el, n, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="energy_sg")
+    ap.add_argument("--nrad", type=int, default=None)
+    ap.add_argument("--nsec", type=int, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs one disc on one GPU; multi-GPU runs are not "
+              "implemented (the ring-slab sharding is a design only)", file=sys.stderr)
+        sys.exit(2)
+    from dustyfargoadsg_amd import _capi
+    from dustyfargoadsg_amd.workloads import WORKLOADS
+    kw = {}
+    if args.nrad:
+        kw["nrad"] = args.nrad
+    if args.nsec:
+        kw["nsec"] = args.nsec
+    params, planets = WORKLOADS[args.workload](**kw)
+    cells = params.nrad * params.nsec
+    prof_steps = min(args.steps, 60)
+    res = {}
+    for prec, name in ((_capi.FP64, "fp64"), (_capi.FP32, "fp32")):
+        el, kt = run_gpu(params, planets, prec, args.steps, args.warmup, prof_steps)
+        res[name] = dict(elapsed=el, kt=kt)
+    w32, w64 = 4, 8
+    energy, sg = bool(params.energy_equation), bool(params.self_gravity)
+
+    def summary(name, w):
+        el, kt = res[name]["elapsed"], res[name]["kt"]
+        value = args.steps * cells / el
+        ms = el / args.steps * 1e3
+        # the roofline kernel: the largest share of step time among kernels
+        # that move field data (the single-block control kernel moves none)
+        dom = max(((k, v) for k, v in kt.items() if kernel_bytes_per_cell(k, w, energy, sg) > 0),
+                  key=lambda kv: kv[1][0])
+        kname, (tot_ms, nl) = dom
+        avg_s = tot_ms / nl * 1e-3
+        kbytes = kernel_bytes_per_cell(kname, w, energy, sg) * cells
+        sbytes = step_bytes_per_cell(w, energy, sg) * cells
+        prof_total = sum(v[0] for v in kt.values()) / prof_steps
+        kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 3), "launches": int(v[1]),
+                       "share": round(v[0] / prof_steps / prof_total, 4)}
+                   for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])}
+        return value, ms, kname, avg_s, kbytes, sbytes, kernels
+
+    v32, ms32, k32, a32, kb32, sb32, kern32 = summary("fp32", w32)
+    v64, ms64, k64, a64, kb64, sb64, kern64 = summary("fp64", w64)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{params.nrad}x{params.nsec}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic = tj.get("fp32", {}).get(k32)
+    ach = kb32 / a32 / 1e9
+    line = {
+        "metric": "cell-updates/s per hydro step (% HBM roofline), FP32 vs FP64 at <=1e-5 error",
+        "value": v32,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "storage_dtype": "f32",
+        "data": "synthetic (seeded AddNoise initial disc)",
+        "config": {"workload": f"{args.workload} {params.nrad}x{params.nsec}",
+                   "nrad": params.nrad, "nsec": params.nsec,
+                   "energy_equation": energy, "self_gravity": sg,
+                   "alpha_viscosity": params.alpha_viscosity, "planets": len(planets["dist"])},
+        "roofline": {"bound": "hbm", "kernel": k32, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": kb32, "avg_launch_us": a32 * 1e6},
+        "roofline_step": {"bytes_per_cell": step_bytes_per_cell(w32, energy, sg),
+                          "achieved": sb32 / (ms32 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": sb32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "fp64": {"value": v64, "ms_per_step": ms64, "dominant_kernel": k64,
+                 "roofline_step_frac": sb64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "fp32_over_fp64": v32 / v64,
+        "kernels_fp32": kern32,
+        "kernels_fp64": kern64,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        cv, n, el = cpu_baseline(params, planets, args.cpu_seconds)
+        line["cpu_baseline"] = {"value": cv, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                                "sample": f"{n} sub-steps of the same {params.nrad}x{params.nsec} "
+                                          f"disc by the FP64 C oracle, {el:.1f} s, 1 thread"}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

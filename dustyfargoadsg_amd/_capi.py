@@ -158,6 +158,7 @@ def load(path=None):
         "dfargo_init_euler": (c_int, [vp, dp, dp, dp, dp, PP]),
         "dfargo_algogas": (c_int, [vp, c_double, dp, dp, dp, dp, PP]),
         "dfargo_algogas_device": (c_int, [vp, c_double]),
+        "dfargo_run_substeps": (c_int, [vp, c_double, c_int]),
         "dfargo_upload_state": (c_int, [vp, dp, dp, dp, dp, PP]),
         "dfargo_download_state": (c_int, [vp, dp, dp, dp, dp, PP]),
         "dfargo_get_status": (c_int, [vp, ctypes.POINTER(Status)]),
@@ -184,7 +185,7 @@ EXPORTED = [
     "dfargo_default_params", "dfargo_check_params", "dfargo_fill_polar_1d_arrays",
     "dfargo_init_gas_density", "dfargo_init_gas_energy", "dfargo_init_planetary_system",
     "dfargo_aspect_ratio", "dfargo_create", "dfargo_destroy", "dfargo_last_error",
-    "dfargo_init_euler", "dfargo_algogas", "dfargo_algogas_device", "dfargo_upload_state",
+    "dfargo_init_euler", "dfargo_algogas", "dfargo_algogas_device", "dfargo_run_substeps", "dfargo_upload_state",
     "dfargo_download_state", "dfargo_get_status", "dfargo_set_time", "dfargo_get_field",
     "dfargo_compute_selfgravity", "dfargo_fviscosity", "dfargo_get_fargo_shift",
     "dfargo_set_profiling", "dfargo_get_kernel_times", "dfargo_stream",

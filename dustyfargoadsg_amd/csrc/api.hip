@@ -41,6 +41,7 @@ struct dfargo_ctx {
   double kt_ms[KID_COUNT];
   long kt_n[KID_COUNT];
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> evpool;
   std::vector<double> VthetaMed, gcs_host;
   std::vector<float> fstage;
 };
@@ -133,16 +134,25 @@ static void launch_id(dfargo_ctx *c, int id) {
 
 static int run_substep(dfargo_ctx *c) {
   if (c->profiling) {
-    for (int id : c->seq) {
-      HIPCHK(hipEventRecord(c->ev0, c->stream));
-      launch_id(c, id);
+    // one event pair per kernel from a pool; collected once per sub-step
+    const size_t nk = c->seq.size();
+    while (c->evpool.size() < 2 * nk) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->evpool.push_back(e);
+    }
+    for (size_t q = 0; q < nk; q++) {
+      HIPCHK(hipEventRecord(c->evpool[2 * q], c->stream));
+      launch_id(c, c->seq[q]);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(c->ev1, c->stream));
-      HIPCHK(hipEventSynchronize(c->ev1));
+      HIPCHK(hipEventRecord(c->evpool[2 * q + 1], c->stream));
+    }
+    HIPCHK(hipEventSynchronize(c->evpool[2 * nk - 1]));
+    for (size_t q = 0; q < nk; q++) {
       float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      c->kt_ms[id] += ms;
-      c->kt_n[id] += 1;
+      HIPCHK(hipEventElapsedTime(&ms, c->evpool[2 * q], c->evpool[2 * q + 1]));
+      c->kt_ms[c->seq[q]] += ms;
+      c->kt_n[c->seq[q]] += 1;
     }
     return DFARGO_OK;
   }
@@ -186,7 +196,14 @@ static int build_context(dfargo_ctx *c) {
   a.roche = p.roche_smoothing != 0.0;
   a.forced_circular = p.forced_circular;
   a.xcd_remap = 1;
-  a.band = 8;
+  {
+    // ring band of the radial sweep: about 3072 one-wavefront blocks (12
+    // per CU, three per SIMD at the sweep's register use), 2..32 rings per
+    // band (each band re-reads three halo rings)
+    int colblk = (ns + 63) / 64;
+    int band = (nr * colblk) / 3072;
+    a.band = band < 2 ? 2 : (band > 32 ? 32 : band);
+  }
   a.pm = p.pmax - p.pmin;
   a.dphi = a.pm / (double)ns;
   a.invdphi = 1.0 / a.dphi;
@@ -249,7 +266,6 @@ static int build_context(dfargo_ctx *c) {
     if ((rc = dalloc(c, r, sizeof(double) * (size_t)(std::max(nr, ns) + 2)))) return rc;
   if ((rc = dalloc(c, (void **)&a.ring_part, sizeof(double) * (size_t)(nr + 2) * DF_NPART(DF_MAXTG))) ||
       (rc = dalloc(c, (void **)&a.nshift, sizeof(long) * (size_t)(nr + 2))) ||
-      (rc = dalloc(c, (void **)&a.ring_crash, sizeof(int) * 2 * (size_t)(nr + 2))) ||
       (rc = dalloc(c, (void **)&c->d_ctrl, sizeof(dfargo_ctrl))))
     return rc;
   a.ctrl = c->d_ctrl;
@@ -258,8 +274,9 @@ static int build_context(dfargo_ctx *c) {
     const int L2 = 2 * nr;
     const size_t csz = 2 * c->esz;
     if ((rc = dalloc(c, &a.sgr, c->esz * c->nfield)) || (rc = dalloc(c, &a.sgt, c->esz * c->nfield)) ||
-        (rc = dalloc(c, &a.sg_z, csz * (size_t)nr * ns)) || (rc = dalloc(c, &a.sg_w, csz * (size_t)L2 * ns)) ||
-        (rc = dalloc(c, &a.sg_kr, csz * (size_t)L2 * ns)) || (rc = dalloc(c, &a.sg_kt, csz * (size_t)L2 * ns)) ||
+        (rc = dalloc(c, &a.sg_z, csz * (size_t)nr * ns)) || (rc = dalloc(c, &a.sg_w, csz * (size_t)nr * ns)) ||
+        (rc = dalloc(c, &a.sg_kr, csz * (size_t)(ns / 2 + 1) * L2)) ||
+        (rc = dalloc(c, &a.sg_kt, csz * (size_t)(ns / 2 + 1) * L2)) ||
         (rc = dalloc(c, &a.tw_ns, csz * ns)) || (rc = dalloc(c, &a.tw_2n, csz * L2)))
       return rc;
     // twiddles exp(-2 pi i m / L), FP64 host values
@@ -301,18 +318,25 @@ static int build_context(dfargo_ctx *c) {
       tw[2 * m + 1] = -sin(2.0 * M_PI * (double)m / (double)L2);
     }
     HIPCHK(hipMemcpy(twd_2n, tw.data(), sizeof(double) * 2 * L2, hipMemcpyHostToDevice));
+    // the k_sg_cols kernel reads wavenumbers 0..Ns/2 only, one contiguous
+    // column of 2 Nrad values per wavenumber: keep that half, transposed
     void *dsts[2] = {a.sg_kr, a.sg_kt};
     std::vector<double> *srcs[2] = {&kr, &kt};
+    const int nh = ns / 2 + 1;
+    std::vector<double> half(2 * (size_t)nh * L2);
     for (int q = 0; q < 2; q++) {
       HIPCHK(hipMemcpy(tmp, srcs[q]->data(), sizeof(double) * 2 * (size_t)L2 * ns, hipMemcpyHostToDevice));
       launch_fft_rows<double>(tmp, L2, ns, twd_ns, -1, c->stream);
       launch_fft_cols<double>(tmp, ns, L2, twd_2n, -1, c->stream);
       HIPCHK(hipGetLastError());
-      if (c->prec == DFARGO_FP64)
-        HIPCHK(hipMemcpyAsync(dsts[q], tmp, sizeof(double) * 2 * (size_t)L2 * ns, hipMemcpyDeviceToDevice, c->stream));
-      else
-        launch_convert_d2f((const double *)tmp, (float *)dsts[q], 2 * (size_t)L2 * ns, c->stream);
       HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipMemcpy(srcs[q]->data(), tmp, sizeof(double) * 2 * (size_t)L2 * ns, hipMemcpyDeviceToHost));
+      for (int k2 = 0; k2 < nh; k2++)
+        for (int k1 = 0; k1 < L2; k1++) {
+          half[2 * ((size_t)k2 * L2 + k1)] = (*srcs[q])[2 * ((size_t)k1 * ns + k2)];
+          half[2 * ((size_t)k2 * L2 + k1) + 1] = (*srcs[q])[2 * ((size_t)k1 * ns + k2) + 1];
+        }
+      if ((rc = put_field(c, dsts[q], half.data(), half.size()))) return rc;
     }
     HIPCHK(hipFree(tmp));
     HIPCHK(hipFree(twd_ns));
@@ -326,12 +350,15 @@ static int build_context(dfargo_ctx *c) {
   c->h_ctrl->maxdt = 0.00001;
   c->h_ctrl->active = 0;
   if ((rc = ctrl_push(c))) return rc;
+  if (c->prec == DFARGO_FP64) prepare_kernels<double>(a);
+  else prepare_kernels<float>(a);
+  HIPCHK(hipGetLastError());
   // the kernels of one sub-step, in the order of src/SourceEuler.c:398-716
   c->seq = {KID_BEGIN, KID_CFL, KID_CONTROL};
   if (p.energy_equation) c->seq.push_back(KID_EOS);
   c->seq.push_back(KID_SUBSTEP1);
   if (p.self_gravity)
-    for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS_FWD, KID_SG_PRODUCT, KID_SG_COLS_INV, KID_SG_ROWS_INV, KID_SG_KICK})
+    for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV, KID_SG_KICK})
       c->seq.push_back(id);
   if (a.visc_on) {
     c->seq.push_back(KID_VISC_STRESS);
@@ -352,6 +379,15 @@ extern "C" int dfargo_create(const dfargo_params *p, int precision, int device, 
   if (rc) return rc;
   if (precision != DFARGO_FP64 && precision != DFARGO_FP32)
     return dfargo_set_error(DFARGO_ERR_ARG, "precision must be DFARGO_FP64 or DFARGO_FP32");
+  if (p->self_gravity) {
+    // the radial transforms of one wavenumber pair hold four 2*NRAD complex
+    // columns in the 160 KiB of LDS
+    size_t lds = 4 * 2 * (size_t)p->nrad * 2 * (precision == DFARGO_FP64 ? 8 : 4);
+    if (lds > 160 * 1024)
+      return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
+                              "self-gravity with NRAD=%d needs %zu bytes of LDS (> 160 KiB) at this precision",
+                              p->nrad, lds);
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return dfargo_set_error(DFARGO_ERR_HIP, "no HIP device is visible");
@@ -389,6 +425,7 @@ extern "C" int dfargo_destroy(dfargo_ctx *c) {
   if (c->h_ctrl) hipHostFree(c->h_ctrl);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   dfargo_free_tables(&c->tab);
   delete c;
@@ -550,6 +587,24 @@ extern "C" int dfargo_algogas_device(dfargo_ctx *c, double DT) {
     double rem = (DT - k->dtemp) / (k->dt > 0.0 ? k->dt : DT);
     batch = (int)std::min(std::max(ceil(rem), 1.0), 4096.0);
   }
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_run_substeps(dfargo_ctx *c, double DT, int nsub) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
+  if (!(DT > 0.0) || nsub < 0) return dfargo_set_error(DFARGO_ERR_ARG, "need DT > 0 and nsub >= 0");
+  HIPCHK(hipSetDevice(c->dev));
+  launch_algogas_begin(c->a, DT, c->stream);
+  if (c->p.energy_equation) {
+    if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, c->stream);
+    else launch_eos_pre<float>(c->a, c->stream);
+  }
+  HIPCHK(hipGetLastError());
+  int rc;
+  for (int b = 0; b < nsub; b++)
+    if ((rc = run_substep(c))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
   return DFARGO_OK;
 }
 

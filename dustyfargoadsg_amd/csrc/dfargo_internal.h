@@ -46,7 +46,7 @@ int dfargo_set_error(int code, const char *fmt, ...);
 
 #define DF_MAXPL DFARGO_MAX_PLANETS
 #define DF_MAXTG (DF_MAXPL + 1) /* force-sum targets: the star + planets */
-#define DF_NPART(ntg) (3 + 8 * (ntg))
+#define DF_NPART(ntg) (5 + 4 * (ntg))
 
 /* Device control block: the scalars of AlgoGas that evolve sub-step by
  * sub-step.  Lives in device memory; one copy per context. */

@@ -36,22 +36,25 @@ struct KArgs {
   // ring buffers (FP64 unless noted)
   double *vmoy, *ring_dt, *ring_part, *nu_ring, *gcs, *vmed, *lost_col;
   long *nshift;
-  int *ring_crash;
   dfargo_ctrl *ctrl;
   // self-gravity buffers (complex T)
+  // sg_kr, sg_kt: kernel spectra for wavenumbers 0..Ns/2, [k2][k1] (transposed)
   void *sg_z, *sg_w, *sg_kr, *sg_kt, *tw_ns, *tw_2n;
 };
 
 // launchers (step.hip, sg.hip); each enqueues on stream s
 enum KernelId {
-  KID_BEGIN = 0, KID_CFL, KID_CONTROL, KID_EOS, KID_SUBSTEP1, KID_SG_ROWS_FWD, KID_SG_COLS_FWD,
-  KID_SG_PRODUCT, KID_SG_COLS_INV, KID_SG_ROWS_INV, KID_SG_KICK, KID_VISC_STRESS,
+  KID_BEGIN = 0, KID_CFL, KID_CONTROL, KID_EOS, KID_SUBSTEP1, KID_SG_ROWS_FWD, KID_SG_COLS,
+  KID_SG_ROWS_INV, KID_SG_KICK, KID_VISC_STRESS,
   KID_VISC_UPDATE, KID_SUBSTEP2, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
   KID_VELOCITIES, KID_BOUNDARY, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 
 template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s);
+// one-time kernel attributes (dynamic LDS above 64 KiB); call before any capture
+template <typename T> void prepare_kernels(const KArgs &a);
+template <typename T> void sg_prepare(const KArgs &a);
 template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s);
 template <typename T> void launch_eos_pre(const KArgs &a, hipStream_t s);
 template <typename T> void launch_potential(const KArgs &a, hipStream_t s);

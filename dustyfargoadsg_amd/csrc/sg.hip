@@ -32,48 +32,62 @@ template <typename T> __device__ __forceinline__ cx<T> cmuli(cx<T> a, T s) {
   return {-s * a.y, s * a.x};
 }
 
-// In-place-looking Stockham transform of length L = 2^a 3^b held in LDS
-// buffers A and B; returns the buffer holding the result.  dir = -1 is the
-// forward exp(-2 pi i jk/L), +1 the un-normalised inverse.
+// One radix-R pass of a Stockham transform: nb = L/R butterflies, each
+// reading R inputs at stride nb and writing R outputs at stride Ns.
+template <typename T, int R>
+__device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int Ns, int span,
+                                         const cx<T> *tw, int dir) {
+  const int nb = L / R;
+  const T s3 = (T)(dir * 0.86602540378443864676); // dir * sqrt(3)/2
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    const int k = j % Ns;
+    cx<T> v[R], y[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = in[j + r * nb];
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      cx<T> w = tw[r * k * span];
+      if (dir > 0) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+    if constexpr (R == 2) {
+      y[0] = cadd(v[0], v[1]);
+      y[1] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+      cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+      cx<T> a2 = cadd(v[1], v[3]), a3 = cmuli(csub(v[1], v[3]), (T)dir);
+      y[0] = cadd(a0, a2);
+      y[2] = csub(a0, a2);
+      y[1] = cadd(a1, a3);
+      y[3] = csub(a1, a3);
+    } else {
+      cx<T> t1 = cadd(v[1], v[2]);
+      cx<T> t2 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
+      cx<T> t3 = cmuli(csub(v[1], v[2]), s3);
+      y[0] = cadd(v[0], t1);
+      y[1] = cadd(t2, t3);
+      y[2] = csub(t2, t3);
+    }
+    const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) out[idxD + r * Ns] = y[r];
+  }
+}
+
+// Stockham transform of length L = 2^a 3^b held in LDS buffers A and B
+// (radix 3 passes first, then radix 4, then one radix 2 if needed); returns
+// the buffer holding the result.  dir = -1 is the forward exp(-2 pi i jk/L),
+// +1 the un-normalised inverse.  tw holds exp(-2 pi i m/L), m < L.
 template <typename T>
-__device__ cx<T> *fft_lds(cx<T> *A, cx<T> *B, int L, const cx<T> *tw, int dir) {
+__device__ __forceinline__ cx<T> *fft_lds(cx<T> *A, cx<T> *B, int L, const cx<T> *tw, int dir) {
   int Ns = 1, rem = L;
   cx<T> *in = A, *out = B;
-  const T s3 = (T)(dir * 0.86602540378443864676); // dir * sqrt(3)/2
   while (rem > 1) {
     const int R = (rem % 3 == 0) ? 3 : ((rem % 4 == 0) ? 4 : 2);
-    const int nb = L / R, span = L / (Ns * R);
-    for (int j = threadIdx.x; j < nb; j += blockDim.x) {
-      const int k = j % Ns;
-      cx<T> v[4];
-      for (int r = 0; r < R; r++) v[r] = in[j + r * nb];
-      for (int r = 1; r < R; r++) {
-        cx<T> w = tw[r * k * span];
-        if (dir > 0) w.y = -w.y;
-        v[r] = cmul(v[r], w);
-      }
-      cx<T> y[4];
-      if (R == 2) {
-        y[0] = cadd(v[0], v[1]);
-        y[1] = csub(v[0], v[1]);
-      } else if (R == 4) {
-        cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-        cx<T> a2 = cadd(v[1], v[3]), a3 = cmuli(csub(v[1], v[3]), (T)dir);
-        y[0] = cadd(a0, a2);
-        y[2] = csub(a0, a2);
-        y[1] = cadd(a1, a3);
-        y[3] = csub(a1, a3);
-      } else {
-        cx<T> t1 = cadd(v[1], v[2]);
-        cx<T> t2 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
-        cx<T> t3 = cmuli(csub(v[1], v[2]), s3);
-        y[0] = cadd(v[0], t1);
-        y[1] = cadd(t2, t3);
-        y[2] = csub(t2, t3);
-      }
-      const int idxD = (j / Ns) * Ns * R + k;
-      for (int r = 0; r < R; r++) out[idxD + r * Ns] = y[r];
-    }
+    const int span = L / (Ns * R);
+    if (R == 4) fft_pass<T, 4>(in, out, L, Ns, span, tw, dir);
+    else if (R == 3) fft_pass<T, 3>(in, out, L, Ns, span, tw, dir);
+    else fft_pass<T, 2>(in, out, L, Ns, span, tw, dir);
     __syncthreads();
     cx<T> *tmp = in;
     in = out;
@@ -106,65 +120,66 @@ template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs
   for (int j = threadIdx.x; j < ns; j += blockDim.x) z[j] = r[j];
 }
 
-// radial transform of one column with the rows >= Nrad zero-padded
-template <typename T> __global__ void __launch_bounds__(256) k_sg_cols_fwd(KArgs a, int force) {
+// Radial transforms and spectral product (src/sgacc.c:25-42) for one pair
+// of azimuthal wavenumbers (k2, Ns-k2), whose spectra are tied by the
+// Hermitian symmetry of the two real reduced densities.  One workgroup:
+//   1. loads both columns of Z (rows >= Nrad are the zero padding),
+//   2. transforms them over 2 Nrad,
+//   3. separates S_r(k) = (Z(k) + conj Z(-k))/2, S_t(k) = (Z(k) - conj Z(-k))/2i,
+//      multiplies by the kernel spectra (K(k1, Ns-k2) = conj K(-k1, k2), so
+//      only wavenumbers 0..Ns/2 are stored, transposed) and recombines
+//      A_r + i A_t,
+//   4. inverse-transforms both columns and keeps rows < Nrad.
+template <typename T> __global__ void __launch_bounds__(256) k_sg_cols(KArgs a, int force) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->active) return;
-  const int k2 = blockIdx.x, ns = a.ns, nr = a.nr, L = 2 * nr;
-  cx<T> *A = (cx<T> *)smem, *B = A + L;
+  const int ns = a.ns, nr = a.nr, L = 2 * nr, p = blockIdx.x;
+  const int ka = p, kb = (ns - p) % ns;
+  const bool self = (ka == kb);
+  cx<T> *A1 = (cx<T> *)smem, *B1 = A1 + L, *A2 = B1 + L, *B2 = A2 + L;
   const cx<T> *z = (const cx<T> *)a.sg_z;
-  for (int i = threadIdx.x; i < L; i += blockDim.x)
-    A[i] = (i < nr) ? z[(long)i * ns + k2] : cx<T>{(T)0, (T)0};
-  __syncthreads();
-  cx<T> *r = fft_lds<T>(A, B, L, (const cx<T> *)a.tw_2n, -1);
-  cx<T> *w = (cx<T> *)a.sg_w;
-  for (int i = threadIdx.x; i < L; i += blockDim.x) w[(long)i * ns + k2] = r[i];
-}
-
-// spectral product (src/sgacc.c:25-38) on the separated spectra
-template <typename T> __global__ void __launch_bounds__(256) k_sg_product(KArgs a, int force) {
-  if (!force && !a.ctrl->active) return;
-  const int ns = a.ns, L = 2 * a.nr;
-  const long n = (long)L * ns;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  const int k1 = (int)(idx / ns), k2 = (int)(idx % ns);
-  const int p1 = (L - k1) % L, p2 = (ns - k2) % ns;
-  const long pidx = (long)p1 * ns + p2;
-  if (pidx < idx) return; // the partner's thread does the pair
-  cx<T> *w = (cx<T> *)a.sg_w;
-  const cx<T> *kr = (const cx<T> *)a.sg_kr, *kt = (const cx<T> *)a.sg_kt;
-  const cx<T> z = w[idx], zp = w[pidx];
-  // S_r(k) = (Z(k) + conj Z(-k)) / 2, S_t(k) = (Z(k) - conj Z(-k)) / 2i
-  const cx<T> sr = {(T)0.5 * (z.x + zp.x), (T)0.5 * (z.y - zp.y)};
-  const cx<T> stt = {(T)0.5 * (z.y + zp.y), (T)0.5 * (zp.x - z.x)};
-  const T G = (T)DF_G;
-  {
-    cx<T> K1 = kr[idx], K2 = kt[idx];
-    cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
-    cx<T> at = {-G * (K2.x * stt.x - K2.y * stt.y), -G * (K2.x * stt.y + K2.y * stt.x)};
-    w[idx] = {ar.x - at.y, ar.y + at.x};
+  const cx<T> zero = {(T)0, (T)0};
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    A1[i] = (i < nr) ? z[(long)i * ns + ka] : zero;
+    if (!self) A2[i] = (i < nr) ? z[(long)i * ns + kb] : zero;
   }
-  if (pidx != idx) {
-    const cx<T> srp = {sr.x, -sr.y}, stp = {stt.x, -stt.y};
-    cx<T> K1 = kr[pidx], K2 = kt[pidx];
-    cx<T> ar = {-G * (K1.x * srp.x - K1.y * srp.y), -G * (K1.x * srp.y + K1.y * srp.x)};
-    cx<T> at = {-G * (K2.x * stp.x - K2.y * stp.y), -G * (K2.x * stp.y + K2.y * stp.x)};
-    w[pidx] = {ar.x - at.y, ar.y + at.x};
-  }
-}
-
-// inverse radial transform; only rows < Nrad are kept
-template <typename T> __global__ void __launch_bounds__(256) k_sg_cols_inv(KArgs a, int force) {
-  extern __shared__ unsigned char smem[];
-  if (!force && !a.ctrl->active) return;
-  const int k2 = blockIdx.x, ns = a.ns, nr = a.nr, L = 2 * nr;
-  cx<T> *A = (cx<T> *)smem, *B = A + L;
-  cx<T> *w = (cx<T> *)a.sg_w;
-  for (int i = threadIdx.x; i < L; i += blockDim.x) A[i] = w[(long)i * ns + k2];
   __syncthreads();
-  cx<T> *r = fft_lds<T>(A, B, L, (const cx<T> *)a.tw_2n, +1);
-  for (int i = threadIdx.x; i < nr; i += blockDim.x) w[(long)i * ns + k2] = r[i];
+  const cx<T> *tw = (const cx<T> *)a.tw_2n;
+  cx<T> *ra = fft_lds<T>(A1, B1, L, tw, -1);
+  cx<T> *rb = self ? ra : fft_lds<T>(A2, B2, L, tw, -1);
+  cx<T> *wa = (ra == A1) ? B1 : A1;
+  cx<T> *wb = self ? wa : ((rb == A2) ? B2 : A2);
+  const cx<T> *khr = (const cx<T> *)a.sg_kr + (long)p * L, *kht = (const cx<T> *)a.sg_kt + (long)p * L;
+  const T G = (T)DF_G, h = (T)0.5;
+  for (int k1 = threadIdx.x; k1 < L; k1 += blockDim.x) {
+    const int m1 = (k1 == 0) ? 0 : L - k1;
+    {
+      const cx<T> zz = ra[k1], zp = rb[m1];
+      const cx<T> sr = {h * (zz.x + zp.x), h * (zz.y - zp.y)};
+      const cx<T> st = {h * (zz.y + zp.y), h * (zp.x - zz.x)};
+      const cx<T> K1 = khr[k1], K2 = kht[k1];
+      const cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
+      const cx<T> at = {-G * (K2.x * st.x - K2.y * st.y), -G * (K2.x * st.y + K2.y * st.x)};
+      wa[k1] = {ar.x - at.y, ar.y + at.x};
+    }
+    if (!self) {
+      const cx<T> zz = rb[k1], zp = ra[m1];
+      const cx<T> sr = {h * (zz.x + zp.x), h * (zz.y - zp.y)};
+      const cx<T> st = {h * (zz.y + zp.y), h * (zp.x - zz.x)};
+      const cx<T> K1 = {khr[m1].x, -khr[m1].y}, K2 = {kht[m1].x, -kht[m1].y};
+      const cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
+      const cx<T> at = {-G * (K2.x * st.x - K2.y * st.y), -G * (K2.x * st.y + K2.y * st.x)};
+      wb[k1] = {ar.x - at.y, ar.y + at.x};
+    }
+  }
+  __syncthreads();
+  cx<T> *oa = fft_lds<T>(wa, (wa == A1) ? B1 : A1, L, tw, +1);
+  cx<T> *ob = self ? oa : fft_lds<T>(wb, (wb == A2) ? B2 : A2, L, tw, +1);
+  cx<T> *w = (cx<T> *)a.sg_w;
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+    w[(long)i * ns + ka] = oa[i];
+    if (!self) w[(long)i * ns + kb] = ob[i];
+  }
 }
 
 // inverse azimuthal transform, normalisation and self-force compensation
@@ -233,22 +248,19 @@ __global__ void k_convert_d2f(const double *src, float *dst, size_t n) {
 
 // ------------------------------------------------------------------------
 
+template <typename K> static void allow_lds(K kernel, size_t bytes) {
+  // dynamic LDS above 64 KiB must be requested (gfx950 has 160 KiB per CU)
+  if (bytes > 65536) hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
-  const size_t rows_lds = 2 * sizeof(cx<T>) * a.ns, cols_lds = 2 * sizeof(cx<T>) * 2 * a.nr;
+  const size_t rows_lds = 2 * sizeof(cx<T>) * a.ns, cols_lds = 4 * sizeof(cx<T>) * 2 * a.nr;
   switch (id) {
     case KID_SG_ROWS_FWD:
       hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force);
       break;
-    case KID_SG_COLS_FWD:
-      hipLaunchKernelGGL(k_sg_cols_fwd<T>, dim3(a.ns), dim3(256), cols_lds, s, a, force);
-      break;
-    case KID_SG_PRODUCT: {
-      long n = 2L * a.nr * a.ns;
-      hipLaunchKernelGGL(k_sg_product<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, force);
-      break;
-    }
-    case KID_SG_COLS_INV:
-      hipLaunchKernelGGL(k_sg_cols_inv<T>, dim3(a.ns), dim3(256), cols_lds, s, a, force);
+    case KID_SG_COLS:
+      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(256), cols_lds, s, a, force);
       break;
     case KID_SG_ROWS_INV:
       hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force);
@@ -261,23 +273,32 @@ template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_
   }
 }
 
+// Raise the dynamic LDS limit of the kernels that need more than 64 KiB,
+// once at set-up (not inside a stream capture).
+template <typename T> void sg_prepare(const KArgs &a) {
+  const size_t rows_lds = 2 * sizeof(cx<T>) * a.ns, cols_lds = 4 * sizeof(cx<T>) * 2 * a.nr;
+  allow_lds(k_sg_rows_fwd<T>, rows_lds);
+  allow_lds(k_sg_cols<T>, cols_lds);
+  allow_lds(k_sg_rows_inv<T>, rows_lds);
+}
+
 template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s) {
   sg_pipeline<T>(id, a, s, 0);
 }
 template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s) {
   sg_pipeline<T>(KID_SG_ROWS_FWD, a, s, 1);
-  sg_pipeline<T>(KID_SG_COLS_FWD, a, s, 1);
-  sg_pipeline<T>(KID_SG_PRODUCT, a, s, 1);
-  sg_pipeline<T>(KID_SG_COLS_INV, a, s, 1);
+  sg_pipeline<T>(KID_SG_COLS, a, s, 1);
   sg_pipeline<T>(KID_SG_ROWS_INV, a, s, 1);
 }
 template <typename T>
 void launch_fft_rows(void *data, int nrows, int L, const void *tw, int dir, hipStream_t s) {
+  allow_lds(k_fft_rows<T>, 2 * sizeof(cx<T>) * L);
   hipLaunchKernelGGL(k_fft_rows<T>, dim3(nrows), dim3(256), 2 * sizeof(cx<T>) * L, s,
                      (cx<T> *)data, L, (const cx<T> *)tw, dir);
 }
 template <typename T>
 void launch_fft_cols(void *data, int ncols, int L, const void *tw, int dir, hipStream_t s) {
+  allow_lds(k_fft_cols<T>, 2 * sizeof(cx<T>) * L);
   hipLaunchKernelGGL(k_fft_cols<T>, dim3(ncols), dim3(256), 2 * sizeof(cx<T>) * L, s,
                      (cx<T> *)data, ncols, L, (const cx<T> *)tw, dir);
 }
@@ -285,6 +306,8 @@ void launch_convert_d2f(const double *src, float *dst, size_t n, hipStream_t s) 
   hipLaunchKernelGGL(k_convert_d2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
 }
 
+template void sg_prepare<double>(const KArgs &);
+template void sg_prepare<float>(const KArgs &);
 template void launch_sg_kernel<double>(int, const KArgs &, hipStream_t);
 template void launch_sg_kernel<float>(int, const KArgs &, hipStream_t);
 template void launch_sg_accel<double>(const KArgs &, hipStream_t);

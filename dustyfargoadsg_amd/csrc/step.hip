@@ -14,8 +14,8 @@
 
 const char *kKernelNames[KID_COUNT] = {
     "begin",       "cfl_ring",     "control",      "eos",          "substep1",
-    "sg_rows_fwd", "sg_cols_fwd",  "sg_product",   "sg_cols_inv",  "sg_rows_inv",
-    "sg_kick",     "visc_stress",  "visc_update",  "substep2",     "energy",
+    "sg_rows_fwd", "sg_cols",      "sg_rows_inv",  "sg_kick",      "visc_stress",
+    "visc_update", "substep2",     "energy",
     "transport_rad", "transport_azi", "velocities", "boundary"};
 
 // ------------------------------------------------------------------------
@@ -62,6 +62,57 @@ __device__ double block_min(double v, double *sh) {
   __syncthreads();
   return r;
 }
+// N sums at once: wavefront shuffle trees, then one pass over the wave
+// partials (blocks that are whole wavefronts), else the LDS tree.  sh must
+// hold 17*N doubles (and blockDim doubles for the fallback).
+template <int N> __device__ void block_sum_n(double (&v)[N], double *sh) {
+  const int TB = blockDim.x;
+  if (TB % 64 != 0) {
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = block_sum(v[k], sh);
+    return;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = TB >> 6;
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; k++) sh[wave * N + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += sh[w * N + threadIdx.x];
+    sh[16 * N + threadIdx.x] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) v[k] = sh[16 * N + k];
+  __syncthreads();
+}
+__device__ double block_sum_fast(double v, double *sh) {
+  double w[1] = {v};
+  block_sum_n<1>(w, sh);
+  return w[0];
+}
+__device__ double block_min_fast(double v, double *sh) {
+  const int TB = blockDim.x;
+  if (TB % 64 != 0) return block_min(v, sh);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = TB >> 6;
+  for (int off = 32; off > 0; off >>= 1) v = dmin2(v, __shfl_down(v, off, 64));
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh[0];
+    for (int w = 1; w < nw; w++) m = dmin2(m, sh[w]);
+    sh[16] = m;
+  }
+  __syncthreads();
+  double r = sh[16];
+  __syncthreads();
+  return r;
+}
+
 // Neumaier-compensated running sum
 struct CSum {
   double s, c;
@@ -73,6 +124,11 @@ struct CSum {
   }
   __device__ double value() const { return s + c; }
 };
+
+// Ring kernels hold K cells of a ring per thread.  Up to K = 2 a block may
+// have 1024 threads (128 VGPRs per lane); from K = 3 on the block is capped
+// at 512 threads so that 256 VGPRs hold the K-wide working set unspilled.
+#define RING_LB(K) ((K) <= 2 ? 1024 : 512)
 
 // One block per ring; with xcd_remap the rings of one XCD are contiguous, so
 // the radial neighbours a block reads are in the same L2.
@@ -188,7 +244,7 @@ __global__ void k_begin(KArgs a) {
 // per-ring partial sums of ComputeForce (src/Force.c:155-214), GasTotalMass
 // and the barycentre (src/SideEuler.c:15-77) and DetectCrash
 // (src/SourceEuler.c:51-69): one read of rho, vr, vt (and cs, e).
-template <typename T, int K> __global__ void __launch_bounds__(1024) k_cfl_ring(KArgs a) {
+template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl_ring(KArgs a) {
   __shared__ double sh[1024];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
@@ -201,7 +257,7 @@ template <typename T, int K> __global__ void __launch_bounds__(1024) k_cfl_ring(
     vt_own[k] = ld<T>(a.vt, base + t + k * TB);
     s += vt_own[k];
   }
-  s = block_sum(s, sh);
+  s = block_sum_fast(s, sh);
   const double vmoy = s / (double)ns;
   if (t == 0) a.vmoy[i] = vmoy;
   double dtmin = 1e30;
@@ -232,43 +288,37 @@ template <typename T, int K> __global__ void __launch_bounds__(1024) k_cfl_ring(
       dtmin = dmin2(dtmin, dt);
     }
   }
-  dtmin = block_min(dtmin, sh);
+  dtmin = block_min_fast(dtmin, sh);
   if (t == 0) a.ring_dt[i] = dtmin;
-  // crash scan
-  double crd = 0.0, cre = 0.0;
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    const long l = base + t + k * TB;
-    if (ld<T>(a.rho, l) < 0.0) crd = 1.0;
-    if (a.energy && ld<T>(a.en, l) < 0.0) cre = 1.0;
-  }
-  crd = block_sum(crd, sh);
-  cre = block_sum(cre, sh);
-  if (t == 0) {
-    a.ring_crash[2 * i] = crd > 0.0;
-    a.ring_crash[2 * i + 1] = cre > 0.0;
-  }
-  // partial sums: disc mass, barycentre, then 8 force sums per target
-  const int NP = DF_NPART(c->ntarget);
-  double *part = a.ring_part + (long)i * DF_NPART(DF_MAXTG);
+  // per-ring partials, stored partial-major (ring_part[p * nr + i]):
+  // negative-density and negative-energy counts, disc mass, barycentre
+  double *part = a.ring_part + i;
   {
     CSum m = {0, 0}, xb = {0, 0}, yb = {0, 0};
+    double crd = 0.0, cre = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int j = t + k * TB;
       double dens = ld<T>(a.rho, base + j);
+      if (dens < 0.0) crd += 1.0;
+      if (a.energy && ld<T>(a.en, base + j) < 0.0) cre += 1.0;
       double cm = a.Surf[i] * dens;
       m.add(cm);
       xb.add(cm * (a.Rmed[i] * a.CosAz[j]));
       yb.add(cm * (a.Rmed[i] * a.SinAz[j]));
     }
-    double v0 = block_sum(m.value(), sh), v1 = block_sum(xb.value(), sh), v2 = block_sum(yb.value(), sh);
-    if (t == 0) { part[0] = v0; part[1] = v1; part[2] = v2; }
+    double v[5] = {crd, cre, m.value(), xb.value(), yb.value()};
+    block_sum_n<5>(v, sh);
+    if (t == 0)
+#pragma unroll
+      for (int k = 0; k < 5; k++) part[(long)k * nr] = v[k];
   }
+  // then fx, fx (Hill-cut), fy, fy (Hill-cut) per force target; the
+  // inner/outer split of GlobalForce is only ever summed, so it is not kept
+  const bool hill = a.exclude_hill;
   for (int tg = 0; tg < c->ntarget; tg++) {
     const double x = c->tg_x[tg], y = c->tg_y[tg], rsm = c->tg_rsm[tg];
-    const double aa = c->tg_a[tg], rh = c->tg_rh[tg];
-    const int inner = a.Rmed[i] < aa;
+    const double rh = c->tg_rh[tg];
     CSum fx[2] = {{0, 0}, {0, 0}}, fy[2] = {{0, 0}, {0, 0}};
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -277,36 +327,34 @@ template <typename T, int K> __global__ void __launch_bounds__(1024) k_cfl_ring(
       double cellmass = a.Surf[i] * ld<T>(a.rho, base + j);
       double dx = xc - x, dy = yc - y;
       double d2 = dx * dx + dy * dy;
-      double planet_distance = sqrt(d2);
       double dist2 = d2 + rsm * rsm;
       double distance = sqrt(dist2);
       double InvDist3 = 1.0 / dist2 / distance;
-#pragma unroll
-      for (int kk = 0; kk < 2; kk++) {
-        double hill_cut = 1.;
-        if (kk != 0) {
-          double cutoff = a.exclude_hill_factor * rh;
-          if (planet_distance / cutoff < 0.5) hill_cut = 0.0;
-          else if (planet_distance > cutoff) hill_cut = 1.0;
-          else hill_cut = pow(sin((planet_distance / cutoff - .5) * DF_PI), 2.);
-        }
-        fx[kk].add(DF_G * cellmass * dx * InvDist3 * hill_cut);
-        fy[kk].add(DF_G * cellmass * dy * InvDist3 * hill_cut);
+      fx[0].add(DF_G * cellmass * dx * InvDist3 * 1.0);
+      fy[0].add(DF_G * cellmass * dy * InvDist3 * 1.0);
+      if (hill) {
+        double planet_distance = sqrt(d2);
+        double cutoff = a.exclude_hill_factor * rh, hill_cut;
+        if (planet_distance / cutoff < 0.5) hill_cut = 0.0;
+        else if (planet_distance > cutoff) hill_cut = 1.0;
+        else hill_cut = pow(sin((planet_distance / cutoff - .5) * DF_PI), 2.);
+        fx[1].add(DF_G * cellmass * dx * InvDist3 * hill_cut);
+        fy[1].add(DF_G * cellmass * dy * InvDist3 * hill_cut);
       }
     }
-    double r0 = block_sum(fx[0].value(), sh), r1 = block_sum(fx[1].value(), sh);
-    double r2 = block_sum(fy[0].value(), sh), r3 = block_sum(fy[1].value(), sh);
-    if (t == 0) {
-      double *g = part + 3 + 8 * tg;
-      // GlobalForce layout: fxi[0..1], fxo[0..1], fyi[0..1], fyo[0..1]
-      g[0] = inner ? r0 : 0.0; g[1] = inner ? r1 : 0.0;
-      g[2] = inner ? 0.0 : r0; g[3] = inner ? 0.0 : r1;
-      g[4] = inner ? r2 : 0.0; g[5] = inner ? r3 : 0.0;
-      g[6] = inner ? 0.0 : r2; g[7] = inner ? 0.0 : r3;
+    double *g = part + (long)(5 + 4 * tg) * nr;
+    if (hill) {
+      double v[4] = {fx[0].value(), fx[1].value(), fy[0].value(), fy[1].value()};
+      block_sum_n<4>(v, sh);
+      if (t == 0)
+#pragma unroll
+        for (int k = 0; k < 4; k++) g[(long)k * nr] = v[k];
+    } else {
+      double v[2] = {fx[0].value(), fy[0].value()};
+      block_sum_n<2>(v, sh);
+      if (t == 0) { g[0] = v[0]; g[2 * (long)nr] = v[1]; }
     }
   }
-  (void)NP;
-  (void)nr;
 }
 
 // Cash-Karp RK5 of the star+planet problem (src/RungeKunta.c:14-91)
@@ -369,10 +417,9 @@ __device__ void d_rk5(const double *q0, double dt, const double *m, double *q1, 
 // barycentre acceleration (src/SideEuler.c:91-112), the potential
 // parameters of FillForcesArrays (src/Pframeforce.c:44-90), and the planet
 // update (AdvanceSystemFromDisk, AdvanceSystemRK5, RotatePsys).
-__global__ void __launch_bounds__(256) k_control(KArgs a) {
-  __shared__ double sh[256];
+__global__ void __launch_bounds__(1024) k_control(KArgs a) {
+  __shared__ double sh[1024];
   __shared__ double sums[DF_NPART(DF_MAXTG)];
-  __shared__ int crash[2];
   dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
   const int t = threadIdx.x, TB = blockDim.x, nr = a.nr, ns = a.ns;
@@ -383,17 +430,17 @@ __global__ void __launch_bounds__(256) k_control(KArgs a) {
                 fabs(a.vmoy[i] * a.InvRmed[i] - a.vmoy[i + 1] * a.InvRmed[i + 1]);
     nd = dmin2(nd, dt);
   }
-  nd = block_min(nd, sh);
-  const int NP = DF_NPART(c->ntarget);
-  if (t < NP) {
+  nd = block_min_fast(nd, sh);
+  // each wavefront sums whole partials: lanes stride the rings with a
+  // compensated sum, then a fixed shuffle tree (deterministic)
+  const int NP = DF_NPART(c->ntarget), lane = t & 63, nw = TB >> 6;
+  for (int p = t >> 6; p < NP; p += nw) {
+    const double *src = a.ring_part + (long)p * nr;
     CSum s = {0, 0};
-    for (int i = 0; i < nr; i++) s.add(a.ring_part[(long)i * DF_NPART(DF_MAXTG) + t]);
-    sums[t] = s.value();
-  }
-  if (t < 2) {
-    int f = 0;
-    for (int i = 0; i < nr; i++) f |= a.ring_crash[2 * i + t];
-    crash[t] = f;
+    for (int i = lane; i < nr; i += 64) s.add(src[i]);
+    double v = s.value();
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) sums[p] = v;
   }
   __syncthreads();
   if (t != 0) return;
@@ -408,21 +455,16 @@ __global__ void __launch_bounds__(256) k_control(KArgs a) {
   if (dt > c->maxdt) c->maxdt = dt;
   c->gas_timesteps_cfl = gts;
   c->dt = dt;
-  c->crashed_d = crash[0];
-  c->crashed_e = crash[1];
+  c->crashed_d = sums[0] > 0.0;
+  c->crashed_e = sums[1] > 0.0;
   // gas indirect term: ComputeAccel(star), src/SideEuler.c:282-298
-  const double *g0 = sums + 3;
-  if (a.exclude_hill) {
-    c->dop_x = g0[1] + g0[3];
-    c->dop_y = g0[5] + g0[7];
-  } else {
-    c->dop_x = g0[0] + g0[2];
-    c->dop_y = g0[4] + g0[6];
-  }
+  const double *g0 = sums + 5;
+  c->dop_x = a.exclude_hill ? g0[1] : g0[0];
+  c->dop_y = a.exclude_hill ? g0[3] : g0[2];
   // barycentre (src/SideEuler.c:42-112)
-  double total = sums[0];
-  c->barx[2] = sums[1] / (1.0 + total);
-  c->bary[2] = sums[2] / (1.0 + total);
+  double total = sums[2];
+  c->barx[2] = sums[3] / (1.0 + total);
+  c->bary[2] = sums[4] / (1.0 + total);
   c->dopalt_x = -(c->barx[2] - 2.0 * c->barx[1] + c->barx[0]) / (dt * dt);
   c->dopalt_y = -(c->bary[2] - 2.0 * c->bary[1] + c->bary[0]) / (dt * dt);
   c->barx[0] = c->barx[1]; c->barx[1] = c->barx[2];
@@ -459,9 +501,9 @@ __global__ void __launch_bounds__(256) k_control(KArgs a) {
   // AdvanceSystemFromDisk (src/Pframeforce.c:131-161)
   for (int tg = 1; tg < c->ntarget; tg++) {
     int k = c->tg_planet[tg];
-    const double *g = sums + 3 + 8 * tg;
-    double gx = a.exclude_hill ? g[1] + g[3] : g[0] + g[2];
-    double gy = a.exclude_hill ? g[5] + g[7] : g[4] + g[6];
+    const double *g = sums + 5 + 4 * tg;
+    double gx = a.exclude_hill ? g[1] : g[0];
+    double gy = a.exclude_hill ? g[3] : g[2];
     c->pl_vx[k] += dt * gx;
     c->pl_vy[k] += dt * gy;
     c->pl_vx[k] += dt * c->ind_x;
@@ -525,7 +567,7 @@ __global__ void __launch_bounds__(256) k_control(KArgs a) {
 
 // ComputeSoundSpeed (adiabatic, src/SourceEuler.c:2340) per ring, with the
 // ring mean mpi_make1Dprofile feeds to FViscosity (src/Viscosity.c:30-35)
-template <typename T, int K> __global__ void __launch_bounds__(1024) k_eos(KArgs a, int pre) {
+template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_eos(KArgs a, int pre) {
   __shared__ double sh[1024];
   if (!pre && !a.ctrl->active) return;
   const int i = ring_of_block(a), ns = a.ns, t = threadIdx.x, TB = blockDim.x;
@@ -540,7 +582,7 @@ template <typename T, int K> __global__ void __launch_bounds__(1024) k_eos(KArgs
     s += (double)(T)cs;
   }
   if (pre) return;
-  s = block_sum(s, sh);
+  s = block_sum_fast(s, sh);
   if (t == 0) {
     double gcs = s / (double)ns;
     a.gcs[i] = gcs;
@@ -724,7 +766,7 @@ __device__ double vischeat_cell(const KArgs &a, int i, int j, double *trr, doubl
   return vh;
 }
 
-template <typename T, int K> __global__ void __launch_bounds__(1024) k_energy(KArgs a) {
+template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_energy(KArgs a) {
   __shared__ double sh[1024];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
@@ -769,7 +811,7 @@ template <typename T, int K> __global__ void __launch_bounds__(1024) k_energy(KA
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) s += (ld<T>(a.vt, base + t + k * TB) + a.Rmed[i] * c->omega_frame);
-    s = block_sum(s, sh);
+    s = block_sum_fast(s, sh);
     double omega = s / (double)ns;
     omega /= a.Rmed[i];
     double coolingtime = a.beta[i] / omega;
@@ -787,85 +829,108 @@ template <typename T, int K> __global__ void __launch_bounds__(1024) k_energy(KA
 // transport
 // ------------------------------------------------------------------------
 
-// Radial van Leer of the four momenta, energy and density for one cell
-// (ComputeLRMomenta + OneWindRad, src/TransportEuler.c:89-220): the momenta
-// of rings i-2..i+2 are rebuilt from rho, vr, vt, the Work = Q/RhoInt
-// slopes and star values follow ComputeStarRad, and the conservative update
-// uses the fluxes through the faces Rinf[i] and Rsup[i].
+// Radial van Leer of the four momenta, energy and density
+// (ComputeLRMomenta + OneWindRad, src/TransportEuler.c:89-220) as a column
+// sweep: one thread owns column j over a band of rings and walks the faces
+// outward, so every Work = Q/RhoInt value and every ComputeStarRad slope is
+// formed once per cell (plus two halo rings per band), as in the reference.
+// Quantity slots: 0 rp, 1 rm, 2 tp, 3 tm, 4 e, 5 rho (the Work of the
+// density itself), 6 rho as its own base for RhoStar.
+template <typename T>
+__device__ __forceinline__ void rad_ring(const KArgs &a, int r, int j, int jp, double Of,
+                                         double (&q)[7], double (&Q)[6], double &vr_face) {
+  const int ns = a.ns, nr = a.nr;
+  if (r < 0 || r > nr - 1) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) q[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) Q[k] = 0.0;
+    vr_face = (r == nr) ? ld<T>(a.vr, (long)nr * ns + j) : 0.0;
+    return;
+  }
+  const long l = (long)r * ns + j;
+  const double d = ld<T>(a.rho, l), vr0 = ld<T>(a.vr, l), vr1 = ld<T>(a.vr, l + ns);
+  Q[0] = d * vr1;
+  Q[1] = d * vr0;
+  Q[2] = d * (ld<T>(a.vt, (long)r * ns + jp) + a.Rmed[r] * Of) * a.Rmed[r];
+  Q[3] = d * (ld<T>(a.vt, l) + a.Rmed[r] * Of) * a.Rmed[r];
+  Q[4] = a.energy ? ld<T>(a.en, l) : 0.0;
+  Q[5] = d;
+#pragma unroll
+  for (int k = 0; k < 6; k++) q[k] = Q[k] / (d + 1e-20);
+  q[6] = d;
+  vr_face = vr0;
+}
+
+__device__ __forceinline__ double rad_slope(const KArgs &a, int r, double qm, double q0, double qp) {
+  if (r <= 0 || r >= a.nr - 1) return 0.0;
+  double dqm = (q0 - qm) * a.InvDiffRmed[r];
+  double dqp = (qp - q0) * a.InvDiffRmed[r + 1];
+  return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
+}
+
 template <typename T> __global__ void __launch_bounds__(256) k_transport_rad(KArgs a) {
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
-  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ns) return;
-  const double dt = c->dt, Of = c->omega_frame;
+  const int i0 = blockIdx.y * a.band, i1 = min(i0 + a.band, nr);
+  const double dt = c->dt, Of = c->omega_frame, dtheta = a.pm / (double)ns;
   const int jp = (j == ns - 1) ? 0 : j + 1;
-  const int nq = a.energy ? 6 : 5; // rp, rm, tp, tm, [e], rho
-  // rings r = i-2 .. i+2 -> slot s = r - i + 2
-  double rho[5], W[6][5], vrr[6];
-#pragma unroll
-  for (int s = 0; s < 6; s++) {
-    int r = i - 2 + s;
-    vrr[s] = (r >= 0 && r <= nr) ? ld<T>(a.vr, (long)r * ns + j) : 0.0;
-  }
-#pragma unroll
-  for (int s = 0; s < 5; s++) {
-    int r = i - 2 + s;
-    if (r < 0 || r > nr - 1) {
-      rho[s] = 0.0;
-      for (int q = 0; q < 6; q++) W[q][s] = 0.0;
-      continue;
-    }
-    const long l = (long)r * ns + j;
-    double d = ld<T>(a.rho, l);
-    double Q[6];
-    Q[0] = d * vrr[s + 1];
-    Q[1] = d * vrr[s];
-    Q[2] = d * (ld<T>(a.vt, (long)r * ns + jp) + a.Rmed[r] * Of) * a.Rmed[r];
-    Q[3] = d * (ld<T>(a.vt, l) + a.Rmed[r] * Of) * a.Rmed[r];
-    Q[4] = a.energy ? ld<T>(a.en, l) : 0.0;
-    Q[5] = d;
-    rho[s] = d;
-    for (int q = 0; q < 6; q++) W[q][s] = Q[q] / (d + 1e-20);
-  }
-  // slope (ComputeStarRad dq) at slot s (ring i-2+s), zero at rings 0, nr-1
-  auto slope = [&](const double *qb, int s) -> double {
-    int r = i - 2 + s;
-    if (r <= 0 || r >= nr - 1) return 0.0;
-    double dqm = (qb[s] - qb[s - 1]) * a.InvDiffRmed[r];
-    double dqp = (qb[s + 1] - qb[s]) * a.InvDiffRmed[r + 1];
-    return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
-  };
-  // star value at face of ring r = i (slot 2) or i+1 (slot 3)
-  auto star = [&](const double *qb, int s) -> double {
-    int r = i - 2 + s;
-    if (r <= 0 || r >= nr) return 0.0;
-    double v = vrr[s];
-    if (v > 0.0) return qb[s - 1] + (a.Rmed[r] - a.Rmed[r - 1] - v * dt) * 0.5 * slope(qb, s - 1);
-    return qb[s] - (a.Rmed[r + 1] - a.Rmed[r] + v * dt) * 0.5 * slope(qb, s);
-  };
-  const double dtheta = a.pm / (double)ns;
-  const double rs0 = star(rho, 2), rs1 = star(rho, 3);
-  const long l = (long)i * ns + j;
   void *outp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
-  for (int q = 0; q < 6; q++) {
-    if (q == 4 && !a.energy) continue;
-    double qs0 = star(W[q], 2), qs1 = star(W[q], 3);
-    double varq = dt * dtheta * a.Rinf[i] * qs0 * rs0 * vrr[2];
-    varq -= dt * dtheta * a.Rsup[i] * qs1 * rs1 * vrr[3];
-    // Q itself = W * (rho + 1e-20) is not reconstructed: reload it
-    double Qi;
-    const double d = rho[2];
-    if (q == 0) Qi = d * vrr[3];
-    else if (q == 1) Qi = d * vrr[2];
-    else if (q == 2) Qi = d * (ld<T>(a.vt, (long)i * ns + jp) + a.Rmed[i] * Of) * a.Rmed[i];
-    else if (q == 3) Qi = d * (ld<T>(a.vt, l) + a.Rmed[i] * Of) * a.Rmed[i];
-    else if (q == 4) Qi = ld<T>(a.en, l);
-    else Qi = d;
-    Qi += varq * a.InvSurf[i];
-    st<T>(outp[q], l, Qi);
-    if (q == 5 && i == 0) a.lost_col[j] = varq;
+  // window: rings s-2 (qa), s-1 (qb), s (qc), s+1 (qd)
+  double qa[7], qb[7], qc[7], qd[7], Qb[6], Qc[6], Qd[6], Qx[6];
+  double vb, vc, vd, vx;
+  rad_ring<T>(a, i0 - 2, j, jp, Of, qa, Qx, vx);
+  rad_ring<T>(a, i0 - 1, j, jp, Of, qb, Qb, vb);
+  rad_ring<T>(a, i0, j, jp, Of, qc, Qc, vc);
+  double dq_prev[7], star_prev[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) dq_prev[k] = rad_slope(a, i0 - 1, qa[k], qb[k], qc[k]);
+  for (int s = i0; s <= i1; s++) {
+    rad_ring<T>(a, s + 1, j, jp, Of, qd, Qd, vd);
+    double dq_cur[7], star[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      dq_cur[k] = rad_slope(a, s, qb[k], qc[k], qd[k]);
+      // ComputeStarRad at face s (src/TransportEuler.c:170-178)
+      if (s <= 0 || s >= nr)
+        star[k] = 0.0;
+      else if (vc > 0.0)
+        star[k] = qb[k] + (a.Rmed[s] - a.Rmed[s - 1] - vc * dt) * 0.5 * dq_prev[k];
+      else
+        star[k] = qc[k] - (a.Rmed[s + 1] - a.Rmed[s] + vc * dt) * 0.5 * dq_cur[k];
+    }
+    if (s > i0) {
+      // VanLeerRadial update of ring i = s-1 (src/TransportEuler.c:205-217)
+      const int i = s - 1;
+      const long l = (long)i * ns + j;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        if (k == 4 && !a.energy) continue;
+        double varq = dt * dtheta * a.Rinf[i] * star_prev[k] * star_prev[6] * vb;
+        varq -= dt * dtheta * a.Rsup[i] * star[k] * star[6] * vc;
+        st<T>(outp[k], l, Qb[k] + varq * a.InvSurf[i]);
+        if (k == 5 && i == 0) a.lost_col[j] = varq;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      star_prev[k] = star[k];
+      dq_prev[k] = dq_cur[k];
+      qa[k] = qb[k];
+      qb[k] = qc[k];
+      qc[k] = qd[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      Qb[k] = Qc[k];
+      Qc[k] = Qd[k];
+    }
+    vb = vc;
+    vc = vd;
   }
-  (void)nq;
+  (void)vx;
 }
 
 // Azimuthal van Leer with FARGO (OneWindTheta, src/TransportEuler.c:225-465)
@@ -929,7 +994,7 @@ __device__ void azi_vanleer(AziRing<T, K> &R, double (&Q)[K], const double (&RI)
   __syncthreads();
 }
 
-template <typename T, int K> __global__ void __launch_bounds__(1024) k_transport_azi(KArgs a) {
+template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_transport_azi(KArgs a) {
   extern __shared__ double lds[];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
@@ -947,7 +1012,7 @@ template <typename T, int K> __global__ void __launch_bounds__(1024) k_transport
   if (i == 0) {
     double s = 0.0;
     for (int j = t; j < ns; j += TB) s += a.lost_col[j];
-    s = block_sum(s, sh);
+    s = block_sum_fast(s, sh);
     if (t == 0) a.ctrl->lost_mass += s;
   }
   // ComputeThetaElongations / ComputeAverageThetaVelocities / Residuals
@@ -959,7 +1024,7 @@ template <typename T, int K> __global__ void __launch_bounds__(1024) k_transport
     vres[k] = elong;
     s += elong * invdt;
   }
-  s = block_sum(s, sh);
+  s = block_sum_fast(s, sh);
   const double vmed = s / (double)ns;
 #pragma unroll
   for (int k = 0; k < K; k++) vres[k] = vres[k] * invdt - vmed;
@@ -1123,7 +1188,7 @@ template <typename T> __global__ void k_potential(KArgs a) {
 static int ring_k(int ns) {
   static const int ks[] = {1, 2, 3, 4, 6, 8, 12, 16};
   for (int k : ks)
-    if (ns % k == 0 && ns / k <= 1024) return k;
+    if (ns % k == 0 && ns / k <= RING_LB(k)) return k;
   return -1;
 }
 
@@ -1155,7 +1220,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
   switch (id) {
     case KID_BEGIN: hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, s, a); break;
     case KID_CFL: RING_DISPATCH(k_cfl_ring, T, a, s, 0); break;
-    case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s, a); break;
+    case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
     case KID_EOS: RING_DISPATCH(k_eos, T, a, s, 0, 0); break;
     case KID_SUBSTEP1:
       hipLaunchKernelGGL(k_substep1<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
@@ -1170,9 +1235,13 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       hipLaunchKernelGGL(k_substep2<T>, cell_grid(a, a.nr + 1), cell_block(a), 0, s, a);
       break;
     case KID_ENERGY: RING_DISPATCH(k_energy, T, a, s, 0); break;
-    case KID_TRANSPORT_RAD:
-      hipLaunchKernelGGL(k_transport_rad<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
+    case KID_TRANSPORT_RAD: {
+      // one wavefront of columns per block, bands of a.band rings
+      int bx = a.ns < 64 ? a.ns : 64;
+      dim3 g((a.ns + bx - 1) / bx, (a.nr + a.band - 1) / a.band);
+      hipLaunchKernelGGL(k_transport_rad<T>, g, dim3(bx), 0, s, a);
       break;
+    }
     case KID_TRANSPORT_AZI: {
       size_t lds = sizeof(double) * (2 * (size_t)a.ns + 1024);
       RING_DISPATCH(k_transport_azi, T, a, s, lds);
@@ -1201,6 +1270,26 @@ void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s) {
   hipLaunchKernelGGL(k_algogas_begin, dim3(1), dim3(64), 0, s, a, DT);
 }
 
+template <typename T> void prepare_kernels(const KArgs &a) {
+  const size_t lds = sizeof(double) * (2 * (size_t)a.ns + 1024);
+  if (lds > 65536) {
+    const void *f = nullptr;
+    switch (ring_k(a.ns)) {
+      case 1: f = (const void *)k_transport_azi<T, 1>; break;
+      case 2: f = (const void *)k_transport_azi<T, 2>; break;
+      case 3: f = (const void *)k_transport_azi<T, 3>; break;
+      case 4: f = (const void *)k_transport_azi<T, 4>; break;
+      case 6: f = (const void *)k_transport_azi<T, 6>; break;
+      case 8: f = (const void *)k_transport_azi<T, 8>; break;
+      case 12: f = (const void *)k_transport_azi<T, 12>; break;
+      case 16: f = (const void *)k_transport_azi<T, 16>; break;
+    }
+    if (f) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  }
+  if (a.sg) sg_prepare<T>(a);
+}
+template void prepare_kernels<double>(const KArgs &);
+template void prepare_kernels<float>(const KArgs &);
 template void launch_kernel<double>(int, const KArgs &, hipStream_t);
 template void launch_kernel<float>(int, const KArgs &, hipStream_t);
 template void launch_eos_pre<double>(const KArgs &, hipStream_t);

@@ -114,6 +114,10 @@ class Disc:
             _ptr(e if (e is not None and self.p.energy_equation) else None),
             ctypes.byref(state["planets"]) if state.get("planets") is not None else None))
 
+    def run_substeps(self, DT, nsub):
+        """Exactly nsub sub-steps of an AlgoGas call of length DT."""
+        _check(self.lib, self.lib.dfargo_run_substeps(self.ctx, DT, nsub))
+
     def set_state(self, st):
         e = st.get("energy")
         _check(self.lib, self.lib.dfargo_upload_state(

@@ -199,6 +199,12 @@ int dfargo_algogas(dfargo_ctx *ctx, double DT, double *rho, double *vrad,
  * host that keeps the state in HBM between calls. */
 int dfargo_algogas_device(dfargo_ctx *ctx, double DT);
 
+/* Measurement entry point (no counterpart in the reference): starts an
+ * AlgoGas call of length DT on the device state and runs exactly nsub
+ * sub-step passes, fewer only if DT is covered first; the call is left open
+ * (dtemp < DT).  Used to time a fixed number of hydro sub-steps. */
+int dfargo_run_substeps(dfargo_ctx *ctx, double DT, int nsub);
+
 /* Explicit transfers of the state (rho, vrad, vtheta, energy: (nrad+1)*nsec
  * doubles each, energy may be NULL) and of the planetary system. */
 int dfargo_upload_state(dfargo_ctx *ctx, const double *rho, const double *vrad,

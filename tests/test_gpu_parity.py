@@ -1,0 +1,200 @@
+"""Parity of the HIP hydro step with the FP64 oracle, through the C ABI.
+
+Both sides start from the same host initial state (InitGasDensity with the
+drand48 noise stream, InitEuler), take AlgoGas calls of the same DT and are
+compared field by field.  Bars:
+  * integer work is bit-exact: FARGO shifts Nshift[i], the number of CFL
+    sub-steps per call;
+  * FP64 fields: relative L-infinity error (max |gpu - oracle| / max |oracle|)
+    <= 1e-12 after a few calls and <= 1e-11 after 10 orbits, for Sigma,
+    v_theta and energy; the radial velocity is measured on the velocity
+    scale max |v_theta| with the same bars;
+  * FP32 storage: <= 2e-5 (Sigma), 2e-6 (velocities) after a few calls.
+Differences come only from reduction order (ring means, force sums, FFT), as
+pointwise arithmetic is the reference's, built without contraction.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from dustyfargoadsg_amd import _capi
+from dustyfargoadsg_amd.disc import Disc, DfargoError
+from dustyfargoadsg_amd.workloads import gas_planet, energy_sg
+from tests.oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+DT = 2 * math.pi / 20  # one .par DT: twenty per orbit at r = 1
+
+
+def rel(a, b, nr, ns):
+    a, b = a[: nr * ns], b[: nr * ns]
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def pair(params, planets, precision=_capi.FP64):
+    print(f"case {params.nrad}x{params.nsec} prec={precision}", flush=True)
+    o = Oracle(params)
+    so = o.initial_state(planets)
+    d = Disc(params, precision)
+    sg = d.initial_state(planets)
+    return o, so, d, sg
+
+
+def compare(o, d, p, tol_rho, tol_vr, tol_vt, tol_e=None):
+    """vr is compared on the velocity scale max|v_theta|: its own maximum is
+    ~1e-3 of that, so a max|vr|-relative figure would mostly measure how
+    small vr is."""
+    go, oo = d.state(), o.state()
+    errs = {k: rel(go[k], oo[k], p.nrad, p.nsec) for k in ("rho", "vt", "energy")}
+    n = p.nrad * p.nsec
+    errs["vr"] = np.abs(go["vr"][:n] - oo["vr"][:n]).max() / np.abs(oo["vt"][:n]).max()
+    errs["vr_own_scale"] = rel(go["vr"], oo["vr"], p.nrad, p.nsec)
+    assert errs["rho"] <= tol_rho, errs
+    assert errs["vr"] <= tol_vr, errs
+    assert errs["vt"] <= tol_vt, errs
+    if p.energy_equation:
+        assert errs["energy"] <= (tol_e or tol_rho), errs
+    return errs, go, oo
+
+
+@pytest.mark.parametrize("nr,ns,calls", [(32, 96, 3), (128, 384, 2)])
+def test_gas_planet_fp64(nr, ns, calls):
+    p, pl = gas_planet(nrad=nr, nsec=ns)
+    o, so, d, sg = pair(p, pl)
+    for k in ("rho", "vr", "vt"):
+        assert np.array_equal(sg[k], so[k]), k
+    for c in range(calls):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        st, sto = d.status(), o.status()
+        assert st.last_call_substeps == n_o
+        nsh_g, vm_g = d.fargo_shift()
+        nsh_o, vm_o = o.fargo_shift()
+        assert np.array_equal(nsh_g, nsh_o)
+        assert np.abs(vm_g - vm_o).max() <= 1e-13 * np.abs(vm_o).max()
+        assert abs(st.physical_time - sto.physical_time) <= 1e-15
+        assert abs(st.lost_mass - sto.lost_mass) <= 1e-10 * abs(sto.lost_mass)
+    errs, go, oo = compare(o, d, p, 1e-12, 1e-12, 1e-12)
+    assert abs(go["planets"].x[0] - oo["planets"].x[0]) <= 1e-14
+    assert abs(go["planets"].y[0] - oo["planets"].y[0]) <= 1e-14
+
+
+@pytest.mark.parametrize("nr,ns,calls", [(32, 96, 2), (64, 192, 1)])
+def test_energy_sg_fp64(nr, ns, calls):
+    p, pl = energy_sg(nrad=nr, nsec=ns)
+    o, so, d, sg = pair(p, pl)
+    assert rel(sg["vt"], so["vt"], nr, ns) <= 1e-14
+    for c in range(calls):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+        assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
+    compare(o, d, p, 1e-12, 1e-12, 1e-12, 1e-12)
+    for f in ("divv", "vischeat", "sg_accr", "sg_acct"):
+        assert rel(d.field(f), o.field(f), nr, ns) <= 1e-10, f
+
+
+def test_sg_acceleration_matches_oracle():
+    p, pl = energy_sg(nrad=64, nsec=192)
+    o, so, d, sg = pair(p, pl)
+    ar, at = d.compute_selfgravity()
+    ar_o, at_o = o.sg_accel(so["rho"])
+    assert rel(ar, ar_o, 64, 192) <= 1e-13
+    assert rel(at, at_o, 64, 192) <= 1e-11
+
+
+@pytest.mark.parametrize("variant", ["open", "standard", "feel_disk", "taper", "no_indirect"])
+def test_parameter_variants_fp64(variant):
+    """Boundary, transport and planet options against the oracle."""
+    if variant == "feel_disk":
+        p, pl = gas_planet(nrad=32, nsec=96, forced_circular=0)
+        pl = dict(pl, feel_disk=[1], feel_others=[1])
+    elif variant == "taper":
+        p, pl = gas_planet(nrad=32, nsec=96, forced_circular=0, mass_taper=0.5)
+        pl = dict(pl, feel_disk=[1], feel_others=[0])
+    else:
+        over = {"open": dict(boundary=_capi.BC_OPEN), "standard": dict(fast_transport=0),
+                "no_indirect": dict(indirect_term=0)}[variant]
+        p, pl = gas_planet(nrad=32, nsec=96, **over)
+    o, so, d, sg = pair(p, pl)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+        assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
+    errs, go, oo = compare(o, d, p, 1e-12, 1e-12, 1e-12)
+    for k in ("x", "y", "vx", "vy", "mass"):
+        assert abs(getattr(go["planets"], k)[0] - getattr(oo["planets"], k)[0]) <= 1e-13, k
+
+
+def test_fp32_storage_within_bound():
+    p, pl = gas_planet(nrad=128, nsec=384)
+    o, so, d, sg = pair(p, pl, _capi.FP32)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+    compare(o, d, p, 2e-5, 2e-6, 2e-6)
+
+
+def test_graph_replay_equals_eager():
+    """The sub-step graph replayed and the same kernels launched eagerly
+    (profiling mode) give bit-identical states."""
+    p, pl = energy_sg(nrad=32, nsec=96)
+    d1, d2 = Disc(p), Disc(p)
+    d1.initial_state(pl)
+    d2.initial_state(pl)
+    d2.set_profiling(True)
+    for _ in range(3):
+        d1.algogas(DT)
+        d2.algogas(DT)
+    s1, s2 = d1.state(), d2.state()
+    for k in ("rho", "vr", "vt", "energy"):
+        assert np.array_equal(s1[k], s2[k]), k
+    times = d2.kernel_times()
+    assert "transport_azi" in times and times["transport_azi"][1] > 0
+
+
+def test_host_arrays_round_trip():
+    """AlgoGas with host arrays (upload, step, download) equals the
+    device-resident call."""
+    p, pl = gas_planet(nrad=32, nsec=96)
+    d1, d2 = Disc(p), Disc(p)
+    st = d1.initial_state(pl)
+    d2.initial_state(pl)
+    d1.algogas(DT, state=st)
+    d2.algogas(DT)
+    s2 = d2.state()
+    for k in ("rho", "vr", "vt"):
+        assert np.array_equal(st[k], s2[k]), k
+    assert st["planets"].x[0] == s2["planets"].x[0]
+
+
+def test_run_substeps_exact_count_and_errors():
+    p, pl = gas_planet(nrad=32, nsec=96)
+    d = Disc(p)
+    with pytest.raises(DfargoError):
+        d.algogas(DT)  # before InitEuler
+    d.initial_state(pl)
+    d.run_substeps(1e6, 5)
+    assert d.status().last_call_substeps == 5
+    with pytest.raises(DfargoError):
+        d.algogas(-1.0)
+
+
+def test_gas_planet_ten_orbits_fp64():
+    """The 128x384 planet disc over 10 orbits (200 AlgoGas calls): every
+    field within 1e-11 of the oracle and every FARGO shift identical."""
+    p, pl = gas_planet()
+    o, so, d, sg = pair(p, pl)
+    for c in range(200):
+        o.algogas(DT)
+        d.algogas(DT)
+        if c % 20 == 19:
+            assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0]), c
+            print(f"call {c + 1}", flush=True)
+    assert d.status().substeps_total == o.status().substeps_total
+    errs, _, _ = compare(o, d, p, 1e-11, 1e-11, 1e-11)
+    print("10-orbit errors", errs, flush=True)
