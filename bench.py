@@ -31,23 +31,23 @@ def kernel_bytes_per_cell(name, w, energy, sg):
     ns = 4 if energy else 3                  # Sigma, vr, vt (+ e)
     nq = 6 if energy else 5                  # transported quantities
     table = {
-        "cfl_ring": (3 + (2 if energy else 0)) * w,     # rho, vr, vt (+cs, e)
-        "eos": (2 + 1) * w,                              # rho, e -> cs
+        # CFL, force/mass partial sums, crash scan (+ sound speed: e in, cs out)
+        "cfl_ring": (3 + (2 if energy else 0)) * w,
         "substep1": (ns + 2) * w,                        # state in, vr*, vt* out
         "visc_stress": 3 * w + 3 * w,                    # rho, vr*, vt* -> 3 stresses
         "visc_update": 3 * w + 2 * w + 2 * w,            # rho + 3 stresses, update vr*, vt*
         "substep2": (3 + (1 if energy else 0)) * w + (2 + (1 if energy else 0)) * w,
         "energy": 4 * w + 6 * w,                         # rho, vr, vt, e -> e, divv, vh, 3 tau
-        "transport_rad": ns * w + nq * w,
-        "transport_azi": (nq + 1) * w + nq * w,
-        "velocities": nq * w + ns * w,
+        "transport_rad": ns * w + nq * w,                # state in, radially transported quantities out
+        "transport_azi": (nq + 1) * w + nq * w,          # quantities + vt in, shifted quantities out
+        "finish": nq * w + ns * w,                       # momenta in, state out
         "sg_rows_fwd": w + 2 * w,                        # Sigma -> complex row spectra
         # Z columns in, the two half kernel spectra ((Ns/2+1) x 2 Nrad complex
         # each, 4 words per cell), W columns out
         "sg_cols": 2 * w + 4 * w + 2 * w,
         "sg_rows_inv": 2 * w + w + 2 * w,                # W row, Sigma -> g_r, g_theta
         "sg_kick": 4 * w + 2 * w,
-        "boundary": 0, "begin": 0, "control": 0,
+        "control": 0,
     }
     return table.get(name, 0)
 

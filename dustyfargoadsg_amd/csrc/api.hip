@@ -196,14 +196,6 @@ static int build_context(dfargo_ctx *c) {
   a.roche = p.roche_smoothing != 0.0;
   a.forced_circular = p.forced_circular;
   a.xcd_remap = 1;
-  {
-    // ring band of the radial sweep: about 3072 one-wavefront blocks (12
-    // per CU, three per SIMD at the sweep's register use), 2..32 rings per
-    // band (each band re-reads three halo rings)
-    int colblk = (ns + 63) / 64;
-    int band = (nr * colblk) / 3072;
-    a.band = band < 2 ? 2 : (band > 32 ? 32 : band);
-  }
   a.pm = p.pmax - p.pmin;
   a.dphi = a.pm / (double)ns;
   a.invdphi = 1.0 / a.dphi;
@@ -354,8 +346,7 @@ static int build_context(dfargo_ctx *c) {
   else prepare_kernels<float>(a);
   HIPCHK(hipGetLastError());
   // the kernels of one sub-step, in the order of src/SourceEuler.c:398-716
-  c->seq = {KID_BEGIN, KID_CFL, KID_CONTROL};
-  if (p.energy_equation) c->seq.push_back(KID_EOS);
+  c->seq = {KID_CFL, KID_CONTROL};
   c->seq.push_back(KID_SUBSTEP1);
   if (p.self_gravity)
     for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV, KID_SG_KICK})
@@ -366,7 +357,7 @@ static int build_context(dfargo_ctx *c) {
   }
   c->seq.push_back(KID_SUBSTEP2);
   if (p.energy_equation) c->seq.push_back(KID_ENERGY);
-  for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_VELOCITIES, KID_BOUNDARY}) c->seq.push_back(id);
+  for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_FINISH}) c->seq.push_back(id);
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   return DFARGO_OK;

@@ -67,7 +67,7 @@ typedef struct dfargo_ctrl {
   double tg_x[DF_MAXTG], tg_y[DF_MAXTG], tg_rsm[DF_MAXTG], tg_a[DF_MAXTG], tg_rh[DF_MAXTG];
   int tg_planet[DF_MAXTG];
   int ntarget;
-  int active, gas_timesteps_cfl, substeps, crashed_d, crashed_e;
+  int go, active, gas_timesteps_cfl, substeps, crashed_d, crashed_e;
   long substeps_total;
 } dfargo_ctrl;
 

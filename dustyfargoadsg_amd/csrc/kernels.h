@@ -18,7 +18,7 @@ struct KArgs {
   // switches decoded from dfargo_params
   int energy, visc_on, alpha_on, fast, floors, bc, sg, beta_cooling, viscous_heating;
   int gas_indirect, indirect_term, alt_indirect, exclude_hill, roche, forced_circular;
-  int xcd_remap, band;
+  int xcd_remap;
   // constants
   double pm, dphi, invdphi, gamma, alpha, visc_const, floor_dens;
   double sg_eps, sg_rstep, sg_tstep, vh_lr1, vh_lr2;
@@ -44,10 +44,10 @@ struct KArgs {
 
 // launchers (step.hip, sg.hip); each enqueues on stream s
 enum KernelId {
-  KID_BEGIN = 0, KID_CFL, KID_CONTROL, KID_EOS, KID_SUBSTEP1, KID_SG_ROWS_FWD, KID_SG_COLS,
+  KID_CFL = 0, KID_CONTROL, KID_SUBSTEP1, KID_SG_ROWS_FWD, KID_SG_COLS,
   KID_SG_ROWS_INV, KID_SG_KICK, KID_VISC_STRESS,
   KID_VISC_UPDATE, KID_SUBSTEP2, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
-  KID_VELOCITIES, KID_BOUNDARY, KID_COUNT
+  KID_FINISH, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 

@@ -13,10 +13,10 @@
 #include "kernels.h"
 
 const char *kKernelNames[KID_COUNT] = {
-    "begin",       "cfl_ring",     "control",      "eos",          "substep1",
+    "cfl_ring",     "control",      "substep1",
     "sg_rows_fwd", "sg_cols",      "sg_rows_inv",  "sg_kick",      "visc_stress",
     "visc_update", "substep2",     "energy",
-    "transport_rad", "transport_azi", "velocities", "boundary"};
+    "transport_rad", "transport_azi", "finish"};
 
 // ------------------------------------------------------------------------
 // helpers
@@ -113,6 +113,43 @@ __device__ double block_min_fast(double v, double *sh) {
   return r;
 }
 
+// N sums and one minimum in one pass (same scheme as block_sum_n); sh
+// must hold 17*(N+1) doubles (and blockDim doubles for the fallback).
+template <int N> __device__ void block_sum_min_n(double (&v)[N], double &mn, double *sh) {
+  const int TB = blockDim.x;
+  if (TB % 64 != 0) {
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = block_sum(v[k], sh);
+    mn = block_min(mn, sh);
+    return;
+  }
+  const int M = N + 1, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = TB >> 6;
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  for (int off = 32; off > 0; off >>= 1) mn = dmin2(mn, __shfl_down(mn, off, 64));
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k++) sh[wave * M + k] = v[k];
+    sh[wave * M + N] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += sh[w * M + threadIdx.x];
+    sh[16 * M + threadIdx.x] = s;
+  } else if (threadIdx.x == N) {
+    double m = sh[N];
+    for (int w = 1; w < nw; w++) m = dmin2(m, sh[w * M + N]);
+    sh[16 * M + N] = m;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) v[k] = sh[16 * M + k];
+  mn = sh[16 * M + N];
+  __syncthreads();
+}
+
 // Neumaier-compensated running sum
 struct CSum {
   double s, c;
@@ -125,10 +162,11 @@ struct CSum {
   __device__ double value() const { return s + c; }
 };
 
-// Ring kernels hold K cells of a ring per thread.  Up to K = 2 a block may
-// have 1024 threads (128 VGPRs per lane); from K = 3 on the block is capped
-// at 512 threads so that 256 VGPRs hold the K-wide working set unspilled.
-#define RING_LB(K) ((K) <= 2 ? 1024 : 512)
+// Ring kernels hold K cells of a ring per thread.  With K = 1 a block may
+// have 1024 threads (128 VGPRs per lane), with K = 2 768 threads (168
+// VGPRs), from K = 3 on 512 threads (256 VGPRs), so that the K-wide working
+// set stays in registers.
+#define RING_LB(K) ((K) == 1 ? 1024 : ((K) == 2 ? 768 : 512))
 
 // One block per ring; with xcd_remap the rings of one XCD are contiguous, so
 // the radial neighbours a block reads are in the same L2.
@@ -176,24 +214,10 @@ __device__ double pot_at(const KArgs &a, const dfargo_ctrl *c, int i, int j) {
 // per-call and per-sub-step control
 // ------------------------------------------------------------------------
 
-__global__ void k_algogas_begin(KArgs a, double DT) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    dfargo_ctrl *c = a.ctrl;
-    c->DT = DT;
-    c->dtemp = 0.0;
-    c->substeps = 0;
-    c->gas_timesteps_cfl = 1;
-  }
-}
-
-// start of a sub-step: loop condition, mass taper, n-body limit and the
-// force-sum targets (src/SourceEuler.c:398-440, src/Pframeforce.c:131-161)
-__global__ void k_begin(KArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  dfargo_ctrl *c = a.ctrl;
-  int active = c->dtemp < 0.999999999 * c->DT;
-  c->active = active;
-  if (!active) return;
+// start of a sub-step: mass taper, n-body limit and the force-sum targets
+// (src/SourceEuler.c:398-440, src/Pframeforce.c:131-161); one thread, run
+// once the loop condition holds
+__device__ void begin_substep(const KArgs &a, dfargo_ctrl *c) {
   int nb = c->nb;
   if (a.mass_taper_param > 1e-2) {
     double mt = (c->time - c->time0) / (a.mass_taper_param * 2.0 * DF_PI);
@@ -240,6 +264,26 @@ __global__ void k_begin(KArgs a) {
   c->ntarget = nt;
 }
 
+// AlgoGas entry: loop variables, and the first sub-step's start.  The
+// sub-step kernels read two flags of the control block: `go` (the loop
+// condition, read by the CFL kernel that opens a sub-step and by the
+// control kernel) and `active` (set by the control kernel from `go`, read
+// by every kernel after it).  The control kernel re-evaluates `go` for the
+// next sub-step at its end, so the flags never change under a kernel that
+// reads them.
+__global__ void k_algogas_begin(KArgs a, double DT) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    dfargo_ctrl *c = a.ctrl;
+    c->DT = DT;
+    c->dtemp = 0.0;
+    c->substeps = 0;
+    c->gas_timesteps_cfl = 1;
+    c->active = 0;
+    c->go = c->dtemp < 0.999999999 * c->DT;
+    if (c->go) begin_substep(a, c);
+  }
+}
+
 // ConditionCFL per ring (src/SourceEuler.c:1340-1484) fused with the
 // per-ring partial sums of ComputeForce (src/Force.c:155-214), GasTotalMass
 // and the barycentre (src/SideEuler.c:15-77) and DetectCrash
@@ -247,7 +291,7 @@ __global__ void k_begin(KArgs a) {
 template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl_ring(KArgs a) {
   __shared__ double sh[1024];
   const dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
+  if (!c->go) return;
   const int i = ring_of_block(a), ns = a.ns, nr = a.nr, t = threadIdx.x, TB = blockDim.x;
   const long base = (long)i * ns;
   double vt_own[K];
@@ -288,30 +332,48 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
       dtmin = dmin2(dtmin, dt);
     }
   }
-  dtmin = block_min_fast(dtmin, sh);
-  if (t == 0) a.ring_dt[i] = dtmin;
   // per-ring partials, stored partial-major (ring_part[p * nr + i]):
-  // negative-density and negative-energy counts, disc mass, barycentre
+  // negative-density and negative-energy counts, disc mass, barycentre.
+  // With the energy equation the same pass is ComputeSoundSpeed
+  // (src/SourceEuler.c:2340, after the CFL above read the previous cs) and
+  // the ring mean of cs that mpi_make1Dprofile feeds to FViscosity
+  // (src/Viscosity.c:30-35) for the kernels after this one.
   double *part = a.ring_part + i;
   {
     CSum m = {0, 0}, xb = {0, 0}, yb = {0, 0};
-    double crd = 0.0, cre = 0.0;
+    double crd = 0.0, cre = 0.0, css = 0.0;
+    const double g = a.gamma;
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int j = t + k * TB;
       double dens = ld<T>(a.rho, base + j);
       if (dens < 0.0) crd += 1.0;
-      if (a.energy && ld<T>(a.en, base + j) < 0.0) cre += 1.0;
+      if (a.energy) {
+        const double en = ld<T>(a.en, base + j);
+        if (en < 0.0) cre += 1.0;
+        const double cs = sqrt(g * (g - 1.0) * en / dens);
+        st<T>(a.cs, base + j, cs);
+        css += (double)(T)cs;
+      }
       double cm = a.Surf[i] * dens;
       m.add(cm);
       xb.add(cm * (a.Rmed[i] * a.CosAz[j]));
       yb.add(cm * (a.Rmed[i] * a.SinAz[j]));
     }
-    double v[5] = {crd, cre, m.value(), xb.value(), yb.value()};
-    block_sum_n<5>(v, sh);
-    if (t == 0)
+    double v[6] = {crd, cre, m.value(), xb.value(), yb.value(), css};
+    block_sum_min_n<6>(v, dtmin, sh);
+    if (t == 0) {
 #pragma unroll
       for (int k = 0; k < 5; k++) part[(long)k * nr] = v[k];
+      a.ring_dt[i] = dtmin;
+      if (a.energy) {
+        const double gcs = v[5] / (double)ns;
+        a.gcs[i] = gcs;
+        double nu = a.visc_const;
+        if (a.alpha_on) nu = a.alpha * gcs * gcs * a.r15[i];
+        a.nu_ring[i] = a.visc_on ? nu * a.visc_mult[i] : 0.0;
+      }
+    }
   }
   // then fx, fx (Hill-cut), fy, fy (Hill-cut) per force target; the
   // inner/outer split of GlobalForce is only ever summed, so it is not kept
@@ -421,8 +483,11 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
   __shared__ double sh[1024];
   __shared__ double sums[DF_NPART(DF_MAXTG)];
   dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
   const int t = threadIdx.x, TB = blockDim.x, nr = a.nr, ns = a.ns;
+  if (!c->go) {
+    if (t == 0) c->active = 0;
+    return;
+  }
   double nd = 1e30;
   for (int i = 1 + t; i < nr; i += TB) nd = dmin2(nd, a.ring_dt[i]);
   for (int i = t; i < nr - 1; i += TB) {
@@ -563,32 +628,22 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
   c->time += dt;
   c->substeps += 1;
   c->substeps_total += 1;
+  c->active = 1;
+  // loop condition and start of the next sub-step
+  c->go = c->dtemp < 0.999999999 * c->DT;
+  if (c->go) begin_substep(a, c);
 }
 
-// ComputeSoundSpeed (adiabatic, src/SourceEuler.c:2340) per ring, with the
-// ring mean mpi_make1Dprofile feeds to FViscosity (src/Viscosity.c:30-35)
-template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_eos(KArgs a, int pre) {
-  __shared__ double sh[1024];
-  if (!pre && !a.ctrl->active) return;
+// ComputeSoundSpeed at the start of AlgoGas (src/SourceEuler.c:2340); in
+// the sub-steps the CFL kernel recomputes cs with its ring means
+template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_eos(KArgs a) {
   const int i = ring_of_block(a), ns = a.ns, t = threadIdx.x, TB = blockDim.x;
   const long base = (long)i * ns;
   const double g = a.gamma;
-  double s = 0.0;
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const long l = base + t + k * TB;
-    double cs = sqrt(g * (g - 1.0) * ld<T>(a.en, l) / ld<T>(a.rho, l));
-    st<T>(a.cs, l, cs);
-    s += (double)(T)cs;
-  }
-  if (pre) return;
-  s = block_sum_fast(s, sh);
-  if (t == 0) {
-    double gcs = s / (double)ns;
-    a.gcs[i] = gcs;
-    double nu = a.visc_const;
-    if (a.alpha_on) nu = a.alpha * gcs * gcs * a.r15[i];
-    a.nu_ring[i] = a.visc_on ? nu * a.visc_mult[i] : 0.0;
+    st<T>(a.cs, l, sqrt(g * (g - 1.0) * ld<T>(a.en, l) / ld<T>(a.rho, l)));
   }
 }
 
@@ -829,169 +884,187 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
 // transport
 // ------------------------------------------------------------------------
 
-// Radial van Leer of the four momenta, energy and density
-// (ComputeLRMomenta + OneWindRad, src/TransportEuler.c:89-220) as a column
-// sweep: one thread owns column j over a band of rings and walks the faces
-// outward, so every Work = Q/RhoInt value and every ComputeStarRad slope is
-// formed once per cell (plus two halo rings per band), as in the reference.
-// Quantity slots: 0 rp, 1 rm, 2 tp, 3 tm, 4 e, 5 rho (the Work of the
-// density itself), 6 rho as its own base for RhoStar.
-template <typename T>
-__device__ __forceinline__ void rad_ring(const KArgs &a, int r, int j, int jp, double Of,
-                                         double (&q)[7], double (&Q)[6], double &vr_face) {
-  const int ns = a.ns, nr = a.nr;
-  if (r < 0 || r > nr - 1) {
-#pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < 6; k++) Q[k] = 0.0;
-    vr_face = (r == nr) ? ld<T>(a.vr, (long)nr * ns + j) : 0.0;
-    return;
-  }
-  const long l = (long)r * ns + j;
-  const double d = ld<T>(a.rho, l), vr0 = ld<T>(a.vr, l), vr1 = ld<T>(a.vr, l + ns);
-  Q[0] = d * vr1;
-  Q[1] = d * vr0;
-  Q[2] = d * (ld<T>(a.vt, (long)r * ns + jp) + a.Rmed[r] * Of) * a.Rmed[r];
-  Q[3] = d * (ld<T>(a.vt, l) + a.Rmed[r] * Of) * a.Rmed[r];
-  Q[4] = a.energy ? ld<T>(a.en, l) : 0.0;
-  Q[5] = d;
-#pragma unroll
-  for (int k = 0; k < 6; k++) q[k] = Q[k] / (d + 1e-20);
-  q[6] = d;
-  vr_face = vr0;
-}
+// Radial transport (ComputeLRMomenta + OneWindRad, src/TransportEuler.c:
+// 89-220) on tiles of RT_R rings x RT_C columns.  The Work values
+// Q/(rho+1e-20) of the tile's rings and of two halo rings either side, then
+// the ComputeStarRad slopes, go through LDS, so each is formed once per cell
+// (plus the halo); every thread then updates one cell.  Quantity slots:
+// 0 rp, 1 rm, 2 tp, 3 tm, 4 e, 5 rho (the Work of the density itself),
+// 6 rho as its own base for RhoStar.
+#define RT_C 32
+#define RT_R 8
 
+// slope of ComputeStarRad at ring r (zero on the first and last ring)
 __device__ __forceinline__ double rad_slope(const KArgs &a, int r, double qm, double q0, double qp) {
   if (r <= 0 || r >= a.nr - 1) return 0.0;
   double dqm = (q0 - qm) * a.InvDiffRmed[r];
   double dqp = (qp - q0) * a.InvDiffRmed[r + 1];
   return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
 }
+// QRStar at face r (between rings r-1 and r) from the Work values of rings
+// r-1, r and the slopes there; zero on faces 0 and nr
+__device__ __forceinline__ double rad_star(const KArgs &a, int r, double v, double dt, double wm, double w0,
+                                           double dqm, double dq0) {
+  if (r <= 0 || r >= a.nr) return 0.0;
+  if (v > 0.0) return wm + (a.Rmed[r] - a.Rmed[r - 1] - v * dt) * 0.5 * dqm;
+  return w0 - (a.Rmed[r + 1] - a.Rmed[r] + v * dt) * 0.5 * dq0;
+}
+// the six transported quantities of cell (r, j) before the pass
+template <typename T>
+__device__ __forceinline__ void rad_quantities(const KArgs &a, int r, int j, double Of, double (&Q)[6]) {
+  const int ns = a.ns;
+  const int jp = (j == ns - 1) ? 0 : j + 1;
+  const long l = (long)r * ns + j;
+  const double d = ld<T>(a.rho, l), Rm = a.Rmed[r];
+  Q[0] = d * ld<T>(a.vr, l + ns);
+  Q[1] = d * ld<T>(a.vr, l);
+  Q[2] = d * (ld<T>(a.vt, (long)r * ns + jp) + Rm * Of) * Rm;
+  Q[3] = d * (ld<T>(a.vt, l) + Rm * Of) * Rm;
+  Q[4] = a.energy ? ld<T>(a.en, l) : 0.0;
+  Q[5] = d;
+}
 
-template <typename T> __global__ void __launch_bounds__(256) k_transport_rad(KArgs a) {
+template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport_rad(KArgs a) {
+  __shared__ double sW[7][RT_R + 4][RT_C];
+  __shared__ double sD[7][RT_R + 2][RT_C];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
-  const int ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
-  const int i0 = blockIdx.y * a.band, i1 = min(i0 + a.band, nr);
+  const int ns = a.ns, nr = a.nr, tx = threadIdx.x, ty = threadIdx.y;
+  const int j = blockIdx.x * RT_C + tx, i0 = blockIdx.y * RT_R;
+  const int jc = (j < ns) ? j : ns - 1; // columns past the edge only fill the tile
   const double dt = c->dt, Of = c->omega_frame, dtheta = a.pm / (double)ns;
-  const int jp = (j == ns - 1) ? 0 : j + 1;
+  // Work values of rings i0-2 .. i0+RT_R+1
+  for (int q = ty; q < RT_R + 4; q += RT_R) {
+    const int r = i0 - 2 + q;
+    if (r < 0 || r > nr - 1) {
+#pragma unroll
+      for (int k = 0; k < 7; k++) sW[k][q][tx] = 0.0;
+      continue;
+    }
+    double Q[6];
+    rad_quantities<T>(a, r, jc, Of, Q);
+#pragma unroll
+    for (int k = 0; k < 6; k++) sW[k][q][tx] = Q[k] / (Q[5] + 1e-20);
+    sW[6][q][tx] = Q[5];
+  }
+  __syncthreads();
+  // slopes at rings i0-1 .. i0+RT_R
+  for (int q = ty; q < RT_R + 2; q += RT_R) {
+    const int r = i0 - 1 + q;
+#pragma unroll
+    for (int k = 0; k < 7; k++) sD[k][q][tx] = rad_slope(a, r, sW[k][q][tx], sW[k][q + 1][tx], sW[k][q + 2][tx]);
+  }
+  __syncthreads();
+  const int i = i0 + ty;
+  if (j >= ns || i >= nr) return;
+  const long l = (long)i * ns + j;
+  const double v0 = ld<T>(a.vr, l), v1 = ld<T>(a.vr, l + ns);
+  // stars at faces i (Work of rings i-1, i: slots ty+1, ty+2; slopes of
+  // rings i-1, i: slots ty, ty+1) and i+1
+  const double rs0 = rad_star(a, i, v0, dt, sW[6][ty + 1][tx], sW[6][ty + 2][tx], sD[6][ty][tx], sD[6][ty + 1][tx]);
+  const double rs1 =
+      rad_star(a, i + 1, v1, dt, sW[6][ty + 2][tx], sW[6][ty + 3][tx], sD[6][ty + 1][tx], sD[6][ty + 2][tx]);
+  double Q[6];
+  rad_quantities<T>(a, i, j, Of, Q);
+  const double fin = dt * dtheta * a.Rinf[i], fout = dt * dtheta * a.Rsup[i];
   void *outp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
-  // window: rings s-2 (qa), s-1 (qb), s (qc), s+1 (qd)
-  double qa[7], qb[7], qc[7], qd[7], Qb[6], Qc[6], Qd[6], Qx[6];
-  double vb, vc, vd, vx;
-  rad_ring<T>(a, i0 - 2, j, jp, Of, qa, Qx, vx);
-  rad_ring<T>(a, i0 - 1, j, jp, Of, qb, Qb, vb);
-  rad_ring<T>(a, i0, j, jp, Of, qc, Qc, vc);
-  double dq_prev[7], star_prev[7];
 #pragma unroll
-  for (int k = 0; k < 7; k++) dq_prev[k] = rad_slope(a, i0 - 1, qa[k], qb[k], qc[k]);
-  for (int s = i0; s <= i1; s++) {
-    rad_ring<T>(a, s + 1, j, jp, Of, qd, Qd, vd);
-    double dq_cur[7], star[7];
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-      dq_cur[k] = rad_slope(a, s, qb[k], qc[k], qd[k]);
-      // ComputeStarRad at face s (src/TransportEuler.c:170-178)
-      if (s <= 0 || s >= nr)
-        star[k] = 0.0;
-      else if (vc > 0.0)
-        star[k] = qb[k] + (a.Rmed[s] - a.Rmed[s - 1] - vc * dt) * 0.5 * dq_prev[k];
-      else
-        star[k] = qc[k] - (a.Rmed[s + 1] - a.Rmed[s] + vc * dt) * 0.5 * dq_cur[k];
-    }
-    if (s > i0) {
-      // VanLeerRadial update of ring i = s-1 (src/TransportEuler.c:205-217)
-      const int i = s - 1;
-      const long l = (long)i * ns + j;
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-        if (k == 4 && !a.energy) continue;
-        double varq = dt * dtheta * a.Rinf[i] * star_prev[k] * star_prev[6] * vb;
-        varq -= dt * dtheta * a.Rsup[i] * star[k] * star[6] * vc;
-        st<T>(outp[k], l, Qb[k] + varq * a.InvSurf[i]);
-        if (k == 5 && i == 0) a.lost_col[j] = varq;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-      star_prev[k] = star[k];
-      dq_prev[k] = dq_cur[k];
-      qa[k] = qb[k];
-      qb[k] = qc[k];
-      qc[k] = qd[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      Qb[k] = Qc[k];
-      Qc[k] = Qd[k];
-    }
-    vb = vc;
-    vc = vd;
+  for (int k = 0; k < 6; k++) {
+    if (k == 4 && !a.energy) continue;
+    const double q0 = rad_star(a, i, v0, dt, sW[k][ty + 1][tx], sW[k][ty + 2][tx], sD[k][ty][tx], sD[k][ty + 1][tx]);
+    const double q1 =
+        rad_star(a, i + 1, v1, dt, sW[k][ty + 2][tx], sW[k][ty + 3][tx], sD[k][ty + 1][tx], sD[k][ty + 2][tx]);
+    // VanLeerRadial (src/TransportEuler.c:205-217)
+    double varq = fin * q0 * rs0 * v0;
+    varq -= fout * q1 * rs1 * v1;
+    st<T>(outp[k], l, Q[k] + varq * a.InvSurf[i]);
+    if (k == 5 && i == 0) a.lost_col[j] = varq;
   }
-  (void)vx;
 }
 
-// Azimuthal van Leer with FARGO (OneWindTheta, src/TransportEuler.c:225-465)
-// for one ring, in registers; Work and slope arrays go through LDS for the
-// j-1 / j+1 neighbours.  The integer shift is applied on the final store.
-template <typename T, int K> struct AziRing {
-  double *sW, *sD;
-  int t, TB, ns;
-  __device__ int jj(int k) const { return t + k * TB; }
-};
-
-template <typename T, int K>
-__device__ void azi_star(AziRing<T, K> &R, const double (&W)[K], const double (&v)[K], double dt,
-                         double dxtheta, double invdxtheta, double (&out)[K]) {
-  const int ns = R.ns;
-#pragma unroll
-  for (int k = 0; k < K; k++) R.sW[R.jj(k)] = W[k];
-  __syncthreads();
-  double dq[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    int j = R.jj(k), jm = (j == 0) ? ns - 1 : j - 1, jp = (j == ns - 1) ? 0 : j + 1;
-    double dqm = (W[k] - R.sW[jm]), dqp = (R.sW[jp] - W[k]);
-    dq[k] = (dqp * dqm > 0.0) ? dqp * dqm / (dqp + dqm) * invdxtheta : 0.0;
-    R.sD[j] = dq[k];
+// Azimuthal part (OneWindTheta, src/TransportEuler.c:225-465).  The j-1 /
+// j+1 neighbours of a thread's cells belong to other threads: each
+// exchange publishes the first two and last two cells of every thread in
+// one of two ring-sized LDS buffers (alternating, so one barrier per
+// exchange suffices) and reads back the two cells either side.
+template <int K>
+__device__ __forceinline__ void ring_halo(double *buf, const double (&X)[K], int j0, int ns, double (&H)[4]) {
+  buf[j0] = X[0];
+  buf[j0 + K - 1] = X[K - 1];
+  if (K > 1) {
+    buf[j0 + 1] = X[K > 1 ? 1 : 0];
+    buf[j0 + K - 2] = X[K > 1 ? K - 2 : 0];
   }
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    int j = R.jj(k), jm = (j == 0) ? ns - 1 : j - 1;
-    double ksi = v[k] * dt;
-    if (ksi > 0.0) out[k] = R.sW[jm] + (dxtheta - ksi) * R.sD[jm];
-    else out[k] = W[k] - (dxtheta + ksi) * dq[k];
-  }
-  __syncthreads();
+  const int jm2 = (j0 >= 2) ? j0 - 2 : j0 - 2 + ns, jm1 = (j0 >= 1) ? j0 - 1 : j0 - 1 + ns;
+  const int jk0 = (j0 + K < ns) ? j0 + K : j0 + K - ns, jk1 = (j0 + K + 1 < ns) ? j0 + K + 1 : j0 + K + 1 - ns;
+  H[0] = buf[jm2];
+  H[1] = buf[jm1];
+  H[2] = buf[jk0];
+  H[3] = buf[jk1];
 }
 
-template <typename T, int K>
-__device__ void azi_vanleer(AziRing<T, K> &R, double (&Q)[K], const double (&RI)[K],
-                            const double (&RS)[K], const double (&v)[K], double dt, double dxtheta,
-                            double invdxtheta, double dxrad, double invsurf) {
-  const int ns = R.ns;
-  double W[K], qs[K];
+// ComputeStarTheta (src/TransportEuler.c:419-444) at the cells j0 .. j0+K
+// (the K own cells and the first cell of the next thread) from the own
+// values X and the halo H = {X[j0-2], X[j0-1], X[j0+K], X[j0+K+1]}.
+template <int K>
+__device__ __forceinline__ void azi_star(const double (&X)[K], const double (&H)[4], const double (&v)[K + 1],
+                                         double dt, double dxtheta, double invdxtheta, double (&S)[K + 1]) {
+  // values at cells j0-2 .. j0+K+1 (offset 2)
+  double x[K + 4];
+  x[0] = H[0];
+  x[1] = H[1];
 #pragma unroll
-  for (int k = 0; k < K; k++) W[k] = Q[k] / (RI[k] + 1e-20);
-  azi_star<T, K>(R, W, v, dt, dxtheta, invdxtheta, qs);
-  double f[K];
+  for (int k = 0; k < K; k++) x[k + 2] = X[k];
+  x[K + 2] = H[2];
+  x[K + 3] = H[3];
+  // slopes at cells j0-1 .. j0+K (offset 1)
+  double dq[K + 2];
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    f[k] = dxrad * qs[k] * RS[k] * v[k];
-    R.sD[R.jj(k)] = f[k];
+  for (int c = 0; c < K + 2; c++) {
+    const double dqm = x[c + 1] - x[c], dqp = x[c + 2] - x[c + 1];
+    dq[c] = (dqp * dqm > 0.0) ? dqp * dqm / (dqp + dqm) * invdxtheta : 0.0;
   }
-  __syncthreads();
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    int j = R.jj(k), jp = (j == ns - 1) ? 0 : j + 1;
-    double varq = f[k];
-    varq -= R.sD[jp];
-    Q[k] += varq * invsurf;
+  for (int c = 0; c < K + 1; c++) {
+    const double ksi = v[c] * dt;
+    if (ksi > 0.0) S[c] = x[c + 1] + (dxtheta - ksi) * dq[c];
+    else S[c] = x[c + 2] - (dxtheta + ksi) * dq[c + 1];
   }
-  __syncthreads();
+}
+
+// QuantitiesAdvection (src/TransportEuler.c:375-400) with velocity v at the
+// cells j0 .. j0+K: the density star first, then every quantity's star,
+// flux and update.  Q[5] is the density; RI is its value before the pass.
+template <int K>
+__device__ __forceinline__ void azi_pass(const KArgs &a, double *lds, int &flip, int j0, double (&Q)[6][K],
+                                         const double (&v)[K + 1], double dt, double dxtheta, double invdxtheta,
+                                         double dxrad, double invsurf) {
+  const int ns = a.ns;
+  double RI[K], RS[K + 1], H[4];
+#pragma unroll
+  for (int k = 0; k < K; k++) RI[k] = Q[5][k];
+  ring_halo<K>(lds + flip * ns, RI, j0, ns, H);
+  flip ^= 1;
+  azi_star<K>(RI, H, v, dt, dxtheta, invdxtheta, RS);
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    if (q == 4 && !a.energy) continue;
+    double W[K], S[K + 1], f[K + 1];
+#pragma unroll
+    for (int k = 0; k < K; k++) W[k] = Q[q][k] / (RI[k] + 1e-20);
+    ring_halo<K>(lds + flip * ns, W, j0, ns, H);
+    flip ^= 1;
+    azi_star<K>(W, H, v, dt, dxtheta, invdxtheta, S);
+#pragma unroll
+    for (int c = 0; c < K + 1; c++) f[c] = dxrad * S[c] * RS[c] * v[c];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double varq = f[k];
+      varq -= f[k + 1];
+      Q[q][k] += varq * invsurf;
+    }
+  }
 }
 
 template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_transport_azi(KArgs a) {
@@ -999,35 +1072,38 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
   const int i = ring_of_block(a), ns = a.ns, t = threadIdx.x, TB = blockDim.x;
-  AziRing<T, K> R;
-  R.sW = lds;
-  R.sD = lds + ns;
-  R.t = t;
-  R.TB = TB;
-  R.ns = ns;
+  const int j0 = t * K;
   double *sh = lds + 2 * ns;
   const long base = (long)i * ns;
   const double dt = c->dt, invdt = 1.0 / dt;
-  // lost mass through the inner edge (VanLeerRadial's LostByDisk)
+  // mass through the inner edge (VanLeerRadial's LostByDisk, ring 0)
   if (i == 0) {
-    double s = 0.0;
-    for (int j = t; j < ns; j += TB) s += a.lost_col[j];
-    s = block_sum_fast(s, sh);
-    if (t == 0) a.ctrl->lost_mass += s;
+    double lost = 0.0;
+    for (int j = t; j < ns; j += TB) lost += a.lost_col[j];
+    lost = block_sum_fast(lost, sh);
+    if (t == 0) a.ctrl->lost_mass += lost;
   }
-  // ComputeThetaElongations / ComputeAverageThetaVelocities / Residuals
-  double vres[K];
+  // quantities after the radial pass: rp, rm, tp, tm, [e], rho
+  void *qp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
+  double Q[6][K];
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+#pragma unroll
+    for (int k = 0; k < K; k++) Q[q][k] = (q == 4 && !a.energy) ? 0.0 : ld<T>(qp[q], base + j0 + k);
+  // ComputeThetaElongations / ComputeAverageThetaVelocities / Residuals on
+  // the velocity before transport; v[K] is the first cell of the next thread
+  double vres[K + 1];
   double s = 0.0;
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    double elong = ld<T>(a.vt, base + R.jj(k)) * dt;
-    vres[k] = elong;
-    s += elong * invdt;
+  for (int k = 0; k < K + 1; k++) {
+    const int j = (j0 + k < ns) ? j0 + k : j0 + k - ns;
+    vres[k] = ld<T>(a.vt, base + j) * dt;
+    if (k < K) s += vres[k] * invdt;
   }
   s = block_sum_fast(s, sh);
   const double vmed = s / (double)ns;
 #pragma unroll
-  for (int k = 0; k < K; k++) vres[k] = vres[k] * invdt - vmed;
+  for (int k = 0; k < K + 1; k++) vres[k] = vres[k] * invdt - vmed;
   // ComputeConstantResidual (src/TransportEuler.c:292-348)
   const double dpinvns = a.pm / (double)ns;
   const double Ntilde = vmed * a.InvRmed[i] * dt * (double)ns / a.pm;
@@ -1040,40 +1116,19 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
   }
   if (!a.fast) {
 #pragma unroll
-    for (int k = 0; k < K; k++) vres[k] = u + vres[k];
-    u = 0.0;
+    for (int k = 0; k < K + 1; k++) vres[k] = u + vres[k];
   }
   const double dxtheta = a.pm / (double)ns * a.Rmed[i], invdxtheta = 1.0 / dxtheta;
   const double dxrad = (a.Rsup[i] - a.Rinf[i]) * dt, invsurf = 1.0 / a.Surf[i];
-  // quantities after the radial pass: rp, rm, tp, tm, [e], rho
-  void *qp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
-  double Q[6][K], RI[K], RS[K];
-#pragma unroll
-  for (int q = 0; q < 6; q++)
-#pragma unroll
-    for (int k = 0; k < K; k++)
-      Q[q][k] = (q == 4 && !a.energy) ? 0.0 : ld<T>(qp[q], base + R.jj(k));
-  // residual pass: QuantitiesAdvection(Rho, VthetaRes)
-#pragma unroll
-  for (int k = 0; k < K; k++) RI[k] = Q[5][k];
-  azi_star<T, K>(R, RI, vres, dt, dxtheta, invdxtheta, RS);
-  for (int q = 0; q < 6; q++) {
-    if (q == 4 && !a.energy) continue;
-    azi_vanleer<T, K>(R, Q[q], RI, RS, vres, dt, dxtheta, invdxtheta, dxrad, invsurf);
-  }
-  // uniform pass: QuantitiesAdvection(Rho, Vtheta = constant residual)
+  int flip = 0;
+  // residual pass (the only pass of the standard transport)
+  azi_pass<K>(a, lds, flip, j0, Q, vres, dt, dxtheta, invdxtheta, dxrad, invsurf);
+  // uniform pass with the constant residual
   if (a.fast) {
-    double uu[K];
+    double uu[K + 1];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-      RI[k] = Q[5][k];
-      uu[k] = u;
-    }
-    azi_star<T, K>(R, RI, uu, dt, dxtheta, invdxtheta, RS);
-    for (int q = 0; q < 6; q++) {
-      if (q == 4 && !a.energy) continue;
-      azi_vanleer<T, K>(R, Q[q], RI, RS, uu, dt, dxtheta, invdxtheta, dxrad, invsurf);
-    }
+    for (int k = 0; k < K + 1; k++) uu[k] = u;
+    azi_pass<K>(a, lds, flip, j0, Q, uu, dt, dxtheta, invdxtheta, dxrad, invsurf);
   }
   // AdvectSHIFT (src/TransportEuler.c:350-373): new[j] = old[j - Nshift]
   long sh_mod = nshift % ns;
@@ -1083,7 +1138,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
     if (q == 4 && !a.energy) continue;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      long jn = R.jj(k) + sh_mod;
+      long jn = j0 + k + sh_mod;
       if (jn >= ns) jn -= ns;
       st<T>(qp[q], base + jn, Q[q][k]);
     }
@@ -1091,87 +1146,100 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
 }
 
 // ComputeVelocities (src/TransportEuler.c:515-547) and the floors
-// (src/SourceEuler.c:2506-2541): rebuilds vr, vt from the transported
-// momenta and writes the new state
-template <typename T> __global__ void __launch_bounds__(256) k_velocities(KArgs a) {
-  const dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
-  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
+// (src/SourceEuler.c:2506-2541) of one cell from the transported momenta,
+// rounded to the storage type as the stored state would be
+struct CellState {
+  double rho, vr, vt, e;
+};
+template <typename T> __device__ CellState vel_cell(const KArgs &a, const dfargo_ctrl *c, int i, int j) {
+  const int ns = a.ns;
   const long l = (long)i * ns + j, lim = l - ns, ljm = (j == 0) ? (long)i * ns + ns - 1 : l - 1;
   const double rho = ld<T>(a.t_rho, l);
-  double vr = 0.0;
-  if (i != 0) vr = (ld<T>(a.t_rp, lim) + ld<T>(a.t_rm, l)) / (rho + ld<T>(a.t_rho, lim) + 1e-20);
-  double vt = (ld<T>(a.t_tp, ljm) + ld<T>(a.t_tm, l)) / (rho + ld<T>(a.t_rho, ljm) + 1e-20) / a.Rmed[i] -
-              a.Rmed[i] * c->omega_frame;
-  st<T>(a.vr, l, vr);
-  st<T>(a.vt, l, vt);
-  double d = rho;
-  if (a.floors && d < a.floor_dens) d = a.floor_dens;
-  st<T>(a.rho, l, d);
+  CellState s;
+  s.vr = 0.0;
+  if (i != 0) s.vr = (ld<T>(a.t_rp, lim) + ld<T>(a.t_rm, l)) / (rho + ld<T>(a.t_rho, lim) + 1e-20);
+  s.vt = (ld<T>(a.t_tp, ljm) + ld<T>(a.t_tm, l)) / (rho + ld<T>(a.t_rho, ljm) + 1e-20) / a.Rmed[i] -
+         a.Rmed[i] * c->omega_frame;
+  s.rho = rho;
+  if (a.floors && s.rho < a.floor_dens) s.rho = a.floor_dens;
+  s.e = 0.0;
   if (a.energy) {
-    double e = ld<T>(a.t_en, l);
-    double e_bottom = 1e-3 * a.EnergyMed[i];
-    if (a.floors && e < e_bottom) e = e_bottom;
-    st<T>(a.en, l, e);
+    s.e = ld<T>(a.t_en, l);
+    const double e_bottom = 1e-3 * a.EnergyMed[i];
+    if (a.floors && s.e < e_bottom) s.e = e_bottom;
   }
+  s.rho = (double)(T)s.rho;
+  s.vr = (double)(T)s.vr;
+  s.vt = (double)(T)s.vt;
+  s.e = (double)(T)s.e;
+  return s;
 }
 
-// ApplyBoundaryCondition (src/SideEuler.c:1102-1131): wall or open edges,
+// End of the sub-step: ComputeVelocities and the floors, then
+// ApplyBoundaryCondition (src/SideEuler.c:1102-1131) -- wall or open edges
 // and the sub-Keplerian reset of vt in rings 0 and nrad-1, which runs for
-// every boundary type (src/SideEuler.c:1110-1112)
-template <typename T> __global__ void __launch_bounds__(256) k_boundary(KArgs a) {
+// every boundary type (src/SideEuler.c:1110-1112).  Each cell of the edge
+// rings takes the values the boundary routines would copy into it from the
+// rings they read (2, 3 and nrad-2), recomputed here from the momenta.
+// Rings 0 .. nrad, the last only for the wall's vr at the outer face.
+template <typename T> __global__ void __launch_bounds__(256) k_finish(KArgs a) {
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
-  const int ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ns) return;
-  if (a.bc == DFARGO_BC_WALL) {
-    int i = 2;
-    st<T>(a.vr, (long)(i - 1) * ns + j, 0.0);
-    st<T>(a.vr, (long)(i - 2) * ns + j, -ld<T>(a.vr, (long)i * ns + j));
-    st<T>(a.rho, (long)(i - 1) * ns + j, a.SigmaMed[i - 1]);
-    st<T>(a.rho, (long)(i - 2) * ns + j, a.SigmaMed[i - 2]);
-    i = nr - 3;
-    st<T>(a.vr, (long)(i + 2) * ns + j, 0.0);
-    st<T>(a.vr, (long)(i + 3) * ns + j, -ld<T>(a.vr, (long)(i + 1) * ns + j));
-    st<T>(a.rho, (long)(i + 1) * ns + j, a.SigmaMed[i + 1]);
-    st<T>(a.rho, (long)(i + 2) * ns + j, a.SigmaMed[i + 2]);
+  const long l = (long)i * ns + j;
+  if (i == nr) {
+    if (a.bc == DFARGO_BC_WALL) st<T>(a.vr, l, -vel_cell<T>(a, c, nr - 2, j).vr);
+    return;
   }
-  st<T>(a.vt, j, a.VthetaMed[0] - a.Rmed[0] * c->omega_frame);
-  st<T>(a.vt, (long)(nr - 1) * ns + j, a.VthetaMed[nr - 1] - a.Rmed[nr - 1] * c->omega_frame);
+  CellState s = vel_cell<T>(a, c, i, j);
+  if (a.bc == DFARGO_BC_WALL) {
+    if (i == 0) {
+      s.vr = -vel_cell<T>(a, c, 2, j).vr;
+      s.rho = a.SigmaMed[0];
+    } else if (i == 1) {
+      s.vr = 0.0;
+      s.rho = a.SigmaMed[1];
+    }
+    if (i == nr - 2) s.rho = a.SigmaMed[nr - 2];
+    if (i == nr - 1) {
+      s.vr = 0.0;
+      s.rho = a.SigmaMed[nr - 1];
+    }
+  }
+  if (i == 0) s.vt = a.VthetaMed[0] - a.Rmed[0] * c->omega_frame;
+  if (i == nr - 1) s.vt = a.VthetaMed[nr - 1] - a.Rmed[nr - 1] * c->omega_frame;
   if (a.bc == DFARGO_BC_OPEN) {
     // OpenBoundary (src/SideEuler.c:300-408), standard prescription
-    int i = 2;
-    long l = (long)i * ns + j;
-    double rl = ld<T>(a.rho, l);
-    st<T>(a.rho, l - ns, rl);
-    st<T>(a.rho, l - 2 * ns, rl);
-    if (a.energy) {
-      double e = ld<T>(a.en, l);
-      st<T>(a.en, l - ns, e);
-      st<T>(a.en, l - 2 * ns, e);
+    if (i <= 2) {
+      const CellState s2 = (i == 2) ? s : vel_cell<T>(a, c, 2, j);
+      const double vrp = vel_cell<T>(a, c, 3, j).vr;
+      const double vr2 = ((vrp >= 0.0) || (s2.rho < a.SigmaMed[0])) ? 0.0 : vrp;
+      if (i == 2) s.vr = vr2;
+      if (i == 1) {
+        s.rho = s2.rho;
+        s.e = s2.e;
+        s.vt = s2.vt * a.open_vt_in1[0];
+        s.vr = vr2;
+      }
+      if (i == 0) {
+        s.rho = s2.rho;
+        s.e = s2.e;
+        s.vt = s2.vt * a.open_vt_in2[0];
+      }
     }
-    double vtl = ld<T>(a.vt, l);
-    st<T>(a.vt, l - ns, vtl * a.open_vt_in1[0]);
-    st<T>(a.vt, l - 2 * ns, vtl * a.open_vt_in2[0]);
-    double vrp = ld<T>(a.vr, l + ns);
-    if ((vrp >= 0.0) || (rl < a.SigmaMed[0])) {
-      st<T>(a.vr, l, 0.0);
-      st<T>(a.vr, l - ns, 0.0);
-    } else {
-      st<T>(a.vr, l, vrp);
-      st<T>(a.vr, l - ns, vrp);
+    if (i == nr - 1) {
+      const CellState sm = vel_cell<T>(a, c, nr - 2, j);
+      s.rho = sm.rho;
+      s.e = sm.e;
+      s.vt = sm.vt * a.open_vt_out[0];
+      s.vr = ((sm.vr < 0.0) || (sm.rho < a.SigmaMed[nr - 2])) ? 0.0 : sm.vr;
     }
-    i = nr - 1;
-    l = (long)i * ns + j;
-    double rm = ld<T>(a.rho, l - ns);
-    st<T>(a.rho, l, rm);
-    if (a.energy) st<T>(a.en, l, ld<T>(a.en, l - ns));
-    st<T>(a.vt, l, ld<T>(a.vt, l - ns) * a.open_vt_out[0]);
-    double vrm = ld<T>(a.vr, l - ns);
-    if ((vrm < 0.0) || (rm < a.SigmaMed[nr - 2])) st<T>(a.vr, l, 0.0);
-    else st<T>(a.vr, l, vrm);
   }
+  st<T>(a.rho, l, s.rho);
+  st<T>(a.vr, l, s.vr);
+  st<T>(a.vt, l, s.vt);
+  if (a.energy) st<T>(a.en, l, s.e);
 }
 
 // Potential grid on demand (the field of FillForcesArrays)
@@ -1218,10 +1286,8 @@ static dim3 cell_block(const KArgs &a) { return dim3(a.ns < 256 ? a.ns : 256); }
 
 template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) {
   switch (id) {
-    case KID_BEGIN: hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, s, a); break;
     case KID_CFL: RING_DISPATCH(k_cfl_ring, T, a, s, 0); break;
     case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
-    case KID_EOS: RING_DISPATCH(k_eos, T, a, s, 0, 0); break;
     case KID_SUBSTEP1:
       hipLaunchKernelGGL(k_substep1<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
       break;
@@ -1236,10 +1302,8 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       break;
     case KID_ENERGY: RING_DISPATCH(k_energy, T, a, s, 0); break;
     case KID_TRANSPORT_RAD: {
-      // one wavefront of columns per block, bands of a.band rings
-      int bx = a.ns < 64 ? a.ns : 64;
-      dim3 g((a.ns + bx - 1) / bx, (a.nr + a.band - 1) / a.band);
-      hipLaunchKernelGGL(k_transport_rad<T>, g, dim3(bx), 0, s, a);
+      dim3 g((a.ns + RT_C - 1) / RT_C, (a.nr + RT_R - 1) / RT_R);
+      hipLaunchKernelGGL(k_transport_rad<T>, g, dim3(RT_C, RT_R), 0, s, a);
       break;
     }
     case KID_TRANSPORT_AZI: {
@@ -1247,18 +1311,15 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       RING_DISPATCH(k_transport_azi, T, a, s, lds);
       break;
     }
-    case KID_VELOCITIES:
-      hipLaunchKernelGGL(k_velocities<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
-      break;
-    case KID_BOUNDARY:
-      hipLaunchKernelGGL(k_boundary<T>, dim3((a.ns + 255) / 256), dim3(256), 0, s, a);
+    case KID_FINISH:
+      hipLaunchKernelGGL(k_finish<T>, cell_grid(a, a.nr + 1), cell_block(a), 0, s, a);
       break;
     default: launch_sg_kernel<T>(id, a, s); break;
   }
 }
 
 template <typename T> void launch_eos_pre(const KArgs &a, hipStream_t s) {
-  RING_DISPATCH(k_eos, T, a, s, 0, 1);
+  RING_DISPATCH(k_eos, T, a, s, 0);
 }
 template <typename T> void launch_potential(const KArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(k_potential<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
