@@ -32,34 +32,65 @@ template <typename T> __device__ __forceinline__ cx<T> cmuli(cx<T> a, T s) {
   return {-s * a.y, s * a.x};
 }
 
+// DFT of length 4 in the direction dir (-1 forward, +1 inverse)
+template <typename T> __device__ __forceinline__ void dft4(const cx<T> (&x)[4], cx<T> (&y)[4], T dir) {
+  const cx<T> a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+  const cx<T> a2 = cadd(x[1], x[3]), a3 = cmuli(csub(x[1], x[3]), dir);
+  y[0] = cadd(a0, a2);
+  y[2] = csub(a0, a2);
+  y[1] = cadd(a1, a3);
+  y[3] = csub(a1, a3);
+}
+
 // One radix-R pass of a Stockham transform: nb = L/R butterflies, each
-// reading R inputs at stride nb and writing R outputs at stride Ns.
+// reading R inputs at stride nb and writing R outputs at stride Ns.  The
+// quotient j / Ns comes from a 32-bit reciprocal (mg = 2^32/Ns rounded up,
+// exact for j < 2^12 and Ns <= 2^12), not an integer division.
 template <typename T, int R>
-__device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int Ns, int span,
+__device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int Ns, unsigned mg, int span,
                                          const cx<T> *tw, int dir) {
   const int nb = L / R;
   const T s3 = (T)(dir * 0.86602540378443864676); // dir * sqrt(3)/2
+  const T c8 = (T)0.70710678118654752440;         // sqrt(1/2)
+  const T fd = (T)dir;
   for (int j = threadIdx.x; j < nb; j += blockDim.x) {
-    const int k = j % Ns;
+    const int g = (Ns == 1) ? j : (int)__umulhi((unsigned)j, mg);
+    const int k = j - g * Ns;
     cx<T> v[R], y[R];
 #pragma unroll
     for (int r = 0; r < R; r++) v[r] = in[j + r * nb];
+    if (Ns > 1) {
 #pragma unroll
-    for (int r = 1; r < R; r++) {
-      cx<T> w = tw[r * k * span];
-      if (dir > 0) w.y = -w.y;
-      v[r] = cmul(v[r], w);
+      for (int r = 1; r < R; r++) {
+        cx<T> w = tw[r * k * span];
+        if (dir > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
     }
     if constexpr (R == 2) {
       y[0] = cadd(v[0], v[1]);
       y[1] = csub(v[0], v[1]);
     } else if constexpr (R == 4) {
-      cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-      cx<T> a2 = cadd(v[1], v[3]), a3 = cmuli(csub(v[1], v[3]), (T)dir);
-      y[0] = cadd(a0, a2);
-      y[2] = csub(a0, a2);
-      y[1] = cadd(a1, a3);
-      y[3] = csub(a1, a3);
+      dft4<T>(v, y, fd);
+    } else if constexpr (R == 8) {
+      // y[2m] = DFT4(v[k] + v[k+4]), y[2m+1] = DFT4((v[k] - v[k+4]) W8^k)
+      cx<T> e[4], o[4], E[4], O[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        e[q] = cadd(v[q], v[q + 4]);
+        o[q] = csub(v[q], v[q + 4]);
+      }
+      // W8 = exp(dir i pi/4): (c8, dir c8); W8^2 = dir i; W8^3 = (-c8, dir c8)
+      o[1] = {c8 * (o[1].x - fd * o[1].y), c8 * (o[1].y + fd * o[1].x)};
+      o[2] = cmuli(o[2], fd);
+      o[3] = {-c8 * (o[3].x + fd * o[3].y), c8 * (fd * o[3].x - o[3].y)};
+      dft4<T>(e, E, fd);
+      dft4<T>(o, O, fd);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        y[2 * q] = E[q];
+        y[2 * q + 1] = O[q];
+      }
     } else {
       cx<T> t1 = cadd(v[1], v[2]);
       cx<T> t2 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
@@ -68,26 +99,29 @@ __device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int
       y[1] = cadd(t2, t3);
       y[2] = csub(t2, t3);
     }
-    const int idxD = (j / Ns) * Ns * R + k;
+    const int idxD = g * Ns * R + k;
 #pragma unroll
     for (int r = 0; r < R; r++) out[idxD + r * Ns] = y[r];
   }
 }
 
-// Stockham transform of length L = 2^a 3^b held in LDS buffers A and B
-// (radix 3 passes first, then radix 4, then one radix 2 if needed); returns
-// the buffer holding the result.  dir = -1 is the forward exp(-2 pi i jk/L),
-// +1 the un-normalised inverse.  tw holds exp(-2 pi i m/L), m < L.
+// Stockham transform of length L = 2^a 3^b <= 4096 held in LDS buffers A
+// and B (radix 3 passes first, then radix 8, then one radix 4 or 2 if
+// needed); returns the buffer holding the result.  dir = -1 is the forward
+// exp(-2 pi i jk/L), +1 the un-normalised inverse.  tw holds
+// exp(-2 pi i m/L), m < L.
 template <typename T>
 __device__ __forceinline__ cx<T> *fft_lds(cx<T> *A, cx<T> *B, int L, const cx<T> *tw, int dir) {
   int Ns = 1, rem = L;
   cx<T> *in = A, *out = B;
   while (rem > 1) {
-    const int R = (rem % 3 == 0) ? 3 : ((rem % 4 == 0) ? 4 : 2);
+    const int R = (rem % 3 == 0) ? 3 : ((rem % 8 == 0) ? 8 : ((rem % 4 == 0) ? 4 : 2));
     const int span = L / (Ns * R);
-    if (R == 4) fft_pass<T, 4>(in, out, L, Ns, span, tw, dir);
-    else if (R == 3) fft_pass<T, 3>(in, out, L, Ns, span, tw, dir);
-    else fft_pass<T, 2>(in, out, L, Ns, span, tw, dir);
+    const unsigned mg = (Ns > 1) ? 0xFFFFFFFFu / (unsigned)Ns + 1u : 0u;
+    if (R == 8) fft_pass<T, 8>(in, out, L, Ns, mg, span, tw, dir);
+    else if (R == 4) fft_pass<T, 4>(in, out, L, Ns, mg, span, tw, dir);
+    else if (R == 3) fft_pass<T, 3>(in, out, L, Ns, mg, span, tw, dir);
+    else fft_pass<T, 2>(in, out, L, Ns, mg, span, tw, dir);
     __syncthreads();
     cx<T> *tmp = in;
     in = out;

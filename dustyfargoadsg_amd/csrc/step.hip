@@ -28,6 +28,10 @@ template <typename T> __device__ __forceinline__ double ld(const void *p, long i
 template <typename T> __device__ __forceinline__ void st(void *p, long i, double v) {
   ((T *)p)[i] = (T)v;
 }
+// a / b in the arithmetic of the storage type: the IEEE quotient for
+// double; for float the hardware reciprocal (1 ulp) times a
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax2(double a, double b) { return (b > a) ? b : a; }
 
@@ -828,6 +832,9 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
   const int i = ring_of_block(a), ns = a.ns, nr = a.nr, t = threadIdx.x, TB = blockDim.x;
   const long base = (long)i * ns;
   const double dt = c->dt;
+  // the stresses, divergence and heating are diagnostics read between
+  // AlgoGas calls: stored on the last sub-step of a call only
+  const bool last = !c->go;
   double vh[K], dv[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -848,11 +855,13 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
       vh[k] = (a.nu_ring[nr - 1] != 0.0) ? q1 * exp(log(q1 / q2) * a.vh_lr1 / a.vh_lr2) : 0.0;
     }
     dv[k] = d;
-    st<T>(a.trr, base + j, trr);
-    st<T>(a.tpp, base + j, tpp);
-    st<T>(a.trp, base + j, trp);
-    st<T>(a.divv, base + j, d);
-    st<T>(a.vh, base + j, vh[k]);
+    if (last) {
+      st<T>(a.trr, base + j, trr);
+      st<T>(a.tpp, base + j, tpp);
+      st<T>(a.trp, base + j, trp);
+      st<T>(a.divv, base + j, d);
+      st<T>(a.vh, base + j, vh[k]);
+    }
   }
   double enew[K];
 #pragma unroll
@@ -890,61 +899,65 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
 // the ComputeStarRad slopes, go through LDS, so each is formed once per cell
 // (plus the halo); every thread then updates one cell.  Quantity slots:
 // 0 rp, 1 rm, 2 tp, 3 tm, 4 e, 5 rho (the Work of the density itself),
-// 6 rho as its own base for RhoStar.
+// 6 rho as its own base for RhoStar.  The arithmetic is that of the storage
+// type R (the reference's double with FP64 storage).
 #define RT_C 32
 #define RT_R 8
 
 // slope of ComputeStarRad at ring r (zero on the first and last ring)
-__device__ __forceinline__ double rad_slope(const KArgs &a, int r, double qm, double q0, double qp) {
-  if (r <= 0 || r >= a.nr - 1) return 0.0;
-  double dqm = (q0 - qm) * a.InvDiffRmed[r];
-  double dqp = (qp - q0) * a.InvDiffRmed[r + 1];
-  return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
+template <typename R>
+__device__ __forceinline__ R rad_slope(const KArgs &a, int r, R qm, R q0, R qp) {
+  if (r <= 0 || r >= a.nr - 1) return R(0);
+  R dqm = (q0 - qm) * (R)a.InvDiffRmed[r];
+  R dqp = (qp - q0) * (R)a.InvDiffRmed[r + 1];
+  return (dqp * dqm > R(0)) ? qdiv(R(2) * dqp * dqm, dqp + dqm) : R(0);
 }
 // QRStar at face r (between rings r-1 and r) from the Work values of rings
 // r-1, r and the slopes there; zero on faces 0 and nr
-__device__ __forceinline__ double rad_star(const KArgs &a, int r, double v, double dt, double wm, double w0,
-                                           double dqm, double dq0) {
-  if (r <= 0 || r >= a.nr) return 0.0;
-  if (v > 0.0) return wm + (a.Rmed[r] - a.Rmed[r - 1] - v * dt) * 0.5 * dqm;
-  return w0 - (a.Rmed[r + 1] - a.Rmed[r] + v * dt) * 0.5 * dq0;
+template <typename R>
+__device__ __forceinline__ R rad_star(const KArgs &a, int r, R v, R dt, R wm, R w0, R dqm, R dq0) {
+  if (r <= 0 || r >= a.nr) return R(0);
+  if (v > R(0)) return wm + ((R)a.Rmed[r] - (R)a.Rmed[r - 1] - v * dt) * R(0.5) * dqm;
+  return w0 - ((R)a.Rmed[r + 1] - (R)a.Rmed[r] + v * dt) * R(0.5) * dq0;
 }
 // the six transported quantities of cell (r, j) before the pass
-template <typename T>
-__device__ __forceinline__ void rad_quantities(const KArgs &a, int r, int j, double Of, double (&Q)[6]) {
+template <typename R>
+__device__ __forceinline__ void rad_quantities(const KArgs &a, int r, int j, R Of, R (&Q)[6]) {
   const int ns = a.ns;
   const int jp = (j == ns - 1) ? 0 : j + 1;
-  const long l = (long)r * ns + j;
-  const double d = ld<T>(a.rho, l), Rm = a.Rmed[r];
-  Q[0] = d * ld<T>(a.vr, l + ns);
-  Q[1] = d * ld<T>(a.vr, l);
-  Q[2] = d * (ld<T>(a.vt, (long)r * ns + jp) + Rm * Of) * Rm;
-  Q[3] = d * (ld<T>(a.vt, l) + Rm * Of) * Rm;
-  Q[4] = a.energy ? ld<T>(a.en, l) : 0.0;
+  const R *rho = (const R *)a.rho + (long)r * ns, *vr = (const R *)a.vr + (long)r * ns;
+  const R *vt = (const R *)a.vt + (long)r * ns;
+  const R d = rho[j], Rm = (R)a.Rmed[r];
+  Q[0] = d * vr[j + ns];
+  Q[1] = d * vr[j];
+  Q[2] = d * (vt[jp] + Rm * Of) * Rm;
+  Q[3] = d * (vt[j] + Rm * Of) * Rm;
+  Q[4] = a.energy ? ((const R *)a.en)[(long)r * ns + j] : R(0);
   Q[5] = d;
 }
 
 template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport_rad(KArgs a) {
-  __shared__ double sW[7][RT_R + 4][RT_C];
-  __shared__ double sD[7][RT_R + 2][RT_C];
+  using R = T;
+  __shared__ R sW[7][RT_R + 4][RT_C];
+  __shared__ R sD[7][RT_R + 2][RT_C];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
   const int ns = a.ns, nr = a.nr, tx = threadIdx.x, ty = threadIdx.y;
   const int j = blockIdx.x * RT_C + tx, i0 = blockIdx.y * RT_R;
   const int jc = (j < ns) ? j : ns - 1; // columns past the edge only fill the tile
-  const double dt = c->dt, Of = c->omega_frame, dtheta = a.pm / (double)ns;
+  const R dt = (R)c->dt, Of = (R)c->omega_frame, dtheta = (R)(a.pm / (double)ns);
   // Work values of rings i0-2 .. i0+RT_R+1
   for (int q = ty; q < RT_R + 4; q += RT_R) {
     const int r = i0 - 2 + q;
     if (r < 0 || r > nr - 1) {
 #pragma unroll
-      for (int k = 0; k < 7; k++) sW[k][q][tx] = 0.0;
+      for (int k = 0; k < 7; k++) sW[k][q][tx] = R(0);
       continue;
     }
-    double Q[6];
-    rad_quantities<T>(a, r, jc, Of, Q);
+    R Q[6];
+    rad_quantities<R>(a, r, jc, Of, Q);
 #pragma unroll
-    for (int k = 0; k < 6; k++) sW[k][q][tx] = Q[k] / (Q[5] + 1e-20);
+    for (int k = 0; k < 6; k++) sW[k][q][tx] = qdiv(Q[k], Q[5] + R(1e-20));
     sW[6][q][tx] = Q[5];
   }
   __syncthreads();
@@ -952,43 +965,44 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
   for (int q = ty; q < RT_R + 2; q += RT_R) {
     const int r = i0 - 1 + q;
 #pragma unroll
-    for (int k = 0; k < 7; k++) sD[k][q][tx] = rad_slope(a, r, sW[k][q][tx], sW[k][q + 1][tx], sW[k][q + 2][tx]);
+    for (int k = 0; k < 7; k++) sD[k][q][tx] = rad_slope<R>(a, r, sW[k][q][tx], sW[k][q + 1][tx], sW[k][q + 2][tx]);
   }
   __syncthreads();
   const int i = i0 + ty;
   if (j >= ns || i >= nr) return;
   const long l = (long)i * ns + j;
-  const double v0 = ld<T>(a.vr, l), v1 = ld<T>(a.vr, l + ns);
+  const R v0 = ((const R *)a.vr)[l], v1 = ((const R *)a.vr)[l + ns];
   // stars at faces i (Work of rings i-1, i: slots ty+1, ty+2; slopes of
   // rings i-1, i: slots ty, ty+1) and i+1
-  const double rs0 = rad_star(a, i, v0, dt, sW[6][ty + 1][tx], sW[6][ty + 2][tx], sD[6][ty][tx], sD[6][ty + 1][tx]);
-  const double rs1 =
-      rad_star(a, i + 1, v1, dt, sW[6][ty + 2][tx], sW[6][ty + 3][tx], sD[6][ty + 1][tx], sD[6][ty + 2][tx]);
-  double Q[6];
-  rad_quantities<T>(a, i, j, Of, Q);
-  const double fin = dt * dtheta * a.Rinf[i], fout = dt * dtheta * a.Rsup[i];
+  const R rs0 = rad_star<R>(a, i, v0, dt, sW[6][ty + 1][tx], sW[6][ty + 2][tx], sD[6][ty][tx], sD[6][ty + 1][tx]);
+  const R rs1 =
+      rad_star<R>(a, i + 1, v1, dt, sW[6][ty + 2][tx], sW[6][ty + 3][tx], sD[6][ty + 1][tx], sD[6][ty + 2][tx]);
+  R Q[6];
+  rad_quantities<R>(a, i, j, Of, Q);
+  const R fin = dt * dtheta * (R)a.Rinf[i], fout = dt * dtheta * (R)a.Rsup[i], invs = (R)a.InvSurf[i];
   void *outp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     if (k == 4 && !a.energy) continue;
-    const double q0 = rad_star(a, i, v0, dt, sW[k][ty + 1][tx], sW[k][ty + 2][tx], sD[k][ty][tx], sD[k][ty + 1][tx]);
-    const double q1 =
-        rad_star(a, i + 1, v1, dt, sW[k][ty + 2][tx], sW[k][ty + 3][tx], sD[k][ty + 1][tx], sD[k][ty + 2][tx]);
+    const R q0 = rad_star<R>(a, i, v0, dt, sW[k][ty + 1][tx], sW[k][ty + 2][tx], sD[k][ty][tx], sD[k][ty + 1][tx]);
+    const R q1 =
+        rad_star<R>(a, i + 1, v1, dt, sW[k][ty + 2][tx], sW[k][ty + 3][tx], sD[k][ty + 1][tx], sD[k][ty + 2][tx]);
     // VanLeerRadial (src/TransportEuler.c:205-217)
-    double varq = fin * q0 * rs0 * v0;
+    R varq = fin * q0 * rs0 * v0;
     varq -= fout * q1 * rs1 * v1;
-    st<T>(outp[k], l, Q[k] + varq * a.InvSurf[i]);
-    if (k == 5 && i == 0) a.lost_col[j] = varq;
+    ((R *)outp[k])[l] = Q[k] + varq * invs;
+    if (k == 5 && i == 0) a.lost_col[j] = (double)varq;
   }
 }
 
-// Azimuthal part (OneWindTheta, src/TransportEuler.c:225-465).  The j-1 /
+// Azimuthal transport (OneWindTheta, src/TransportEuler.c:225-465), one ring
+// per workgroup, K consecutive cells j0 .. j0+K-1 per thread.  The j-1 /
 // j+1 neighbours of a thread's cells belong to other threads: each
 // exchange publishes the first two and last two cells of every thread in
 // one of two ring-sized LDS buffers (alternating, so one barrier per
 // exchange suffices) and reads back the two cells either side.
-template <int K>
-__device__ __forceinline__ void ring_halo(double *buf, const double (&X)[K], int j0, int ns, double (&H)[4]) {
+template <typename R, int K>
+__device__ __forceinline__ void ring_halo(R *buf, const R (&X)[K], int j0, int ns, R (&H)[4]) {
   buf[j0] = X[0];
   buf[j0 + K - 1] = X[K - 1];
   if (K > 1) {
@@ -1007,11 +1021,11 @@ __device__ __forceinline__ void ring_halo(double *buf, const double (&X)[K], int
 // ComputeStarTheta (src/TransportEuler.c:419-444) at the cells j0 .. j0+K
 // (the K own cells and the first cell of the next thread) from the own
 // values X and the halo H = {X[j0-2], X[j0-1], X[j0+K], X[j0+K+1]}.
-template <int K>
-__device__ __forceinline__ void azi_star(const double (&X)[K], const double (&H)[4], const double (&v)[K + 1],
-                                         double dt, double dxtheta, double invdxtheta, double (&S)[K + 1]) {
+template <typename R, int K>
+__device__ __forceinline__ void azi_star(const R (&X)[K], const R (&H)[4], const R (&v)[K + 1], R dt, R dxtheta,
+                                         R invdxtheta, R (&S)[K + 1]) {
   // values at cells j0-2 .. j0+K+1 (offset 2)
-  double x[K + 4];
+  R x[K + 4];
   x[0] = H[0];
   x[1] = H[1];
 #pragma unroll
@@ -1019,16 +1033,16 @@ __device__ __forceinline__ void azi_star(const double (&X)[K], const double (&H)
   x[K + 2] = H[2];
   x[K + 3] = H[3];
   // slopes at cells j0-1 .. j0+K (offset 1)
-  double dq[K + 2];
+  R dq[K + 2];
 #pragma unroll
   for (int c = 0; c < K + 2; c++) {
-    const double dqm = x[c + 1] - x[c], dqp = x[c + 2] - x[c + 1];
-    dq[c] = (dqp * dqm > 0.0) ? dqp * dqm / (dqp + dqm) * invdxtheta : 0.0;
+    const R dqm = x[c + 1] - x[c], dqp = x[c + 2] - x[c + 1];
+    dq[c] = (dqp * dqm > R(0)) ? qdiv(dqp * dqm, dqp + dqm) * invdxtheta : R(0);
   }
 #pragma unroll
   for (int c = 0; c < K + 1; c++) {
-    const double ksi = v[c] * dt;
-    if (ksi > 0.0) S[c] = x[c + 1] + (dxtheta - ksi) * dq[c];
+    const R ksi = v[c] * dt;
+    if (ksi > R(0)) S[c] = x[c + 1] + (dxtheta - ksi) * dq[c];
     else S[c] = x[c + 2] - (dxtheta + ksi) * dq[c + 1];
   }
 }
@@ -1036,44 +1050,46 @@ __device__ __forceinline__ void azi_star(const double (&X)[K], const double (&H)
 // QuantitiesAdvection (src/TransportEuler.c:375-400) with velocity v at the
 // cells j0 .. j0+K: the density star first, then every quantity's star,
 // flux and update.  Q[5] is the density; RI is its value before the pass.
-template <int K>
-__device__ __forceinline__ void azi_pass(const KArgs &a, double *lds, int &flip, int j0, double (&Q)[6][K],
-                                         const double (&v)[K + 1], double dt, double dxtheta, double invdxtheta,
-                                         double dxrad, double invsurf) {
+template <typename R, int K>
+__device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int j0, R (&Q)[6][K], const R (&v)[K + 1],
+                                         R dt, R dxtheta, R invdxtheta, R dxrad, R invsurf) {
   const int ns = a.ns;
-  double RI[K], RS[K + 1], H[4];
+  R RI[K], RS[K + 1], H[4];
 #pragma unroll
   for (int k = 0; k < K; k++) RI[k] = Q[5][k];
-  ring_halo<K>(lds + flip * ns, RI, j0, ns, H);
+  ring_halo<R, K>(lds + flip * ns, RI, j0, ns, H);
   flip ^= 1;
-  azi_star<K>(RI, H, v, dt, dxtheta, invdxtheta, RS);
+  azi_star<R, K>(RI, H, v, dt, dxtheta, invdxtheta, RS);
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     if (q == 4 && !a.energy) continue;
-    double W[K], S[K + 1], f[K + 1];
+    R W[K], S[K + 1], f[K + 1];
 #pragma unroll
-    for (int k = 0; k < K; k++) W[k] = Q[q][k] / (RI[k] + 1e-20);
-    ring_halo<K>(lds + flip * ns, W, j0, ns, H);
+    for (int k = 0; k < K; k++) W[k] = qdiv(Q[q][k], RI[k] + R(1e-20));
+    ring_halo<R, K>(lds + flip * ns, W, j0, ns, H);
     flip ^= 1;
-    azi_star<K>(W, H, v, dt, dxtheta, invdxtheta, S);
+    azi_star<R, K>(W, H, v, dt, dxtheta, invdxtheta, S);
 #pragma unroll
     for (int c = 0; c < K + 1; c++) f[c] = dxrad * S[c] * RS[c] * v[c];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      double varq = f[k];
+      R varq = f[k];
       varq -= f[k + 1];
       Q[q][k] += varq * invsurf;
     }
   }
 }
 
+// The FARGO mean velocity, the integer shift and the constant residual stay
+// in FP64 whatever the storage type.
 template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_transport_azi(KArgs a) {
+  using R = T;
   extern __shared__ double lds[];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
   const int i = ring_of_block(a), ns = a.ns, t = threadIdx.x, TB = blockDim.x;
   const int j0 = t * K;
-  double *sh = lds + 2 * ns;
+  double *sh = lds + 2 * ns * sizeof(R) / sizeof(double); // after the two ring buffers of R
   const long base = (long)i * ns;
   const double dt = c->dt, invdt = 1.0 / dt;
   // mass through the inner edge (VanLeerRadial's LostByDisk, ring 0)
@@ -1085,25 +1101,23 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
   }
   // quantities after the radial pass: rp, rm, tp, tm, [e], rho
   void *qp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
-  double Q[6][K];
+  R Q[6][K];
 #pragma unroll
   for (int q = 0; q < 6; q++)
 #pragma unroll
-    for (int k = 0; k < K; k++) Q[q][k] = (q == 4 && !a.energy) ? 0.0 : ld<T>(qp[q], base + j0 + k);
+    for (int k = 0; k < K; k++) Q[q][k] = (q == 4 && !a.energy) ? R(0) : ((const R *)qp[q])[base + j0 + k];
   // ComputeThetaElongations / ComputeAverageThetaVelocities / Residuals on
   // the velocity before transport; v[K] is the first cell of the next thread
-  double vres[K + 1];
+  double vel[K + 1];
   double s = 0.0;
 #pragma unroll
   for (int k = 0; k < K + 1; k++) {
     const int j = (j0 + k < ns) ? j0 + k : j0 + k - ns;
-    vres[k] = ld<T>(a.vt, base + j) * dt;
-    if (k < K) s += vres[k] * invdt;
+    vel[k] = ld<T>(a.vt, base + j) * dt;
+    if (k < K) s += vel[k] * invdt;
   }
   s = block_sum_fast(s, sh);
   const double vmed = s / (double)ns;
-#pragma unroll
-  for (int k = 0; k < K + 1; k++) vres[k] = vres[k] * invdt - vmed;
   // ComputeConstantResidual (src/TransportEuler.c:292-348)
   const double dpinvns = a.pm / (double)ns;
   const double Ntilde = vmed * a.InvRmed[i] * dt * (double)ns / a.pm;
@@ -1114,21 +1128,26 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
     a.vmed[i] = vmed;
     a.nshift[i] = nshift;
   }
-  if (!a.fast) {
+  R vres[K + 1];
 #pragma unroll
-    for (int k = 0; k < K + 1; k++) vres[k] = u + vres[k];
+  for (int k = 0; k < K + 1; k++) {
+    double v = vel[k] * invdt - vmed;
+    if (!a.fast) v = u + v;
+    vres[k] = (R)v;
   }
-  const double dxtheta = a.pm / (double)ns * a.Rmed[i], invdxtheta = 1.0 / dxtheta;
-  const double dxrad = (a.Rsup[i] - a.Rinf[i]) * dt, invsurf = 1.0 / a.Surf[i];
+  const double dxth = a.pm / (double)ns * a.Rmed[i];
+  const R dxtheta = (R)dxth, invdxtheta = (R)(1.0 / dxth);
+  const R dxrad = (R)((a.Rsup[i] - a.Rinf[i]) * dt), invsurf = (R)(1.0 / a.Surf[i]), rdt = (R)dt;
+  R *ring = (R *)lds;
   int flip = 0;
   // residual pass (the only pass of the standard transport)
-  azi_pass<K>(a, lds, flip, j0, Q, vres, dt, dxtheta, invdxtheta, dxrad, invsurf);
+  azi_pass<R, K>(a, ring, flip, j0, Q, vres, rdt, dxtheta, invdxtheta, dxrad, invsurf);
   // uniform pass with the constant residual
   if (a.fast) {
-    double uu[K + 1];
+    R uu[K + 1];
 #pragma unroll
-    for (int k = 0; k < K + 1; k++) uu[k] = u;
-    azi_pass<K>(a, lds, flip, j0, Q, uu, dt, dxtheta, invdxtheta, dxrad, invsurf);
+    for (int k = 0; k < K + 1; k++) uu[k] = (R)u;
+    azi_pass<R, K>(a, ring, flip, j0, Q, uu, rdt, dxtheta, invdxtheta, dxrad, invsurf);
   }
   // AdvectSHIFT (src/TransportEuler.c:350-373): new[j] = old[j - Nshift]
   long sh_mod = nshift % ns;
@@ -1140,7 +1159,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
     for (int k = 0; k < K; k++) {
       long jn = j0 + k + sh_mod;
       if (jn >= ns) jn -= ns;
-      st<T>(qp[q], base + jn, Q[q][k]);
+      ((R *)qp[q])[base + jn] = Q[q][k];
     }
   }
 }
@@ -1307,7 +1326,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       break;
     }
     case KID_TRANSPORT_AZI: {
-      size_t lds = sizeof(double) * (2 * (size_t)a.ns + 1024);
+      size_t lds = sizeof(T) * 2 * (size_t)a.ns + sizeof(double) * 1024;
       RING_DISPATCH(k_transport_azi, T, a, s, lds);
       break;
     }
@@ -1332,7 +1351,7 @@ void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s) {
 }
 
 template <typename T> void prepare_kernels(const KArgs &a) {
-  const size_t lds = sizeof(double) * (2 * (size_t)a.ns + 1024);
+  const size_t lds = sizeof(T) * 2 * (size_t)a.ns + sizeof(double) * 1024;
   if (lds > 65536) {
     const void *f = nullptr;
     switch (ring_k(a.ns)) {
