@@ -33,11 +33,9 @@ def kernel_bytes_per_cell(name, w, energy, sg):
     table = {
         # CFL, force/mass partial sums, crash scan (+ sound speed: e in, cs out)
         "cfl_ring": (3 + (2 if energy else 0)) * w,
-        "substep1": (ns + 2) * w,                        # state in, vr*, vt* out
-        "visc_stress": 3 * w + 3 * w,                    # rho, vr*, vt* -> 3 stresses
-        "visc_update": 3 * w + 2 * w + 2 * w,            # rho + 3 stresses, update vr*, vt*
-        "substep2": (3 + (1 if energy else 0)) * w + (2 + (1 if energy else 0)) * w,
-        "energy": 4 * w + 6 * w,                         # rho, vr, vt, e -> e, divv, vh, 3 tau
+        # SubStep1 + viscosity + SubStep2: state (+ g_r, g_theta) in, new velocities (+ e) out
+        "source": (ns + (2 if sg else 0)) * w + (2 + (1 if energy else 0)) * w,
+        "energy": 4 * w + 1 * w,                         # rho, vr, vt, e in; e out
         "transport_rad": ns * w + nq * w,                # state in, radially transported quantities out
         "transport_azi": (nq + 1) * w + nq * w,          # quantities + vt in, shifted quantities out
         "finish": nq * w + ns * w,                       # momenta in, state out
@@ -46,7 +44,8 @@ def kernel_bytes_per_cell(name, w, energy, sg):
         # each, 4 words per cell), W columns out
         "sg_cols": 2 * w + 4 * w + 2 * w,
         "sg_rows_inv": 2 * w + w + 2 * w,                # W row, Sigma -> g_r, g_theta
-        "sg_kick": 4 * w + 2 * w,
+        "sg_transpose_z": 2 * w + 2 * w,                 # Z row-major -> column-major
+        "sg_transpose_w": 2 * w + 2 * w,
         "control": 0,
     }
     return table.get(name, 0)
@@ -152,7 +151,10 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        traffic = tj.get("fp32", {}).get(k32)
+        entry = tj.get("fp32", {}).get(k32)
+        # HBM bytes per launch of the roofline kernel (FETCH_SIZE + WRITE_SIZE,
+        # scaled by a plain copy of known size)
+        traffic = entry["bytes"] if entry else None
     ach = kb32 / a32 / 1e9
     line = {
         "metric": "cell-updates/s per hydro step (% HBM roofline), FP32 vs FP64 at <=1e-5 error",

@@ -18,7 +18,9 @@ FIELDS = {
     "potential": 0, "soundspeed": 1, "pressure": 2, "temperature": 3,
     "divv": 4, "vischeat": 5, "sg_accr": 6, "sg_acct": 7,
     "taurr": 8, "taurp": 9, "taupp": 10,
+    "dust_density": 11, "fb_accr": 12, "fb_acct": 13, "fb_edot": 14,
 }
+INTERP_NGP, INTERP_CIC, INTERP_TSC = 0, 1, 2
 
 c_int, c_long, c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double
 _D8 = c_double * MAX_PLANETS
@@ -89,6 +91,66 @@ class Status(ctypes.Structure):
         ("gas_timesteps_cfl", c_int), ("crashed_density", c_int),
         ("crashed_energy", c_int), ("crashed", c_int),
     ]
+
+
+class DustParams(ctypes.Structure):
+    """dfargo_dust_params (the particle switches of src/var.c:136-181)."""
+    _fields_ = [
+        ("nbpart", c_int),
+        ("sizeminpart", c_double), ("sizemaxpart", c_double), ("sizepartslope", c_double),
+        ("rhopart", c_double), ("dustslope", c_double),
+        ("rmindust", c_double), ("rmaxdust", c_double),
+        ("interpolation", c_int),
+        ("dust_feel_disk", c_int), ("dust_feel_sg", c_int), ("dust_feel_planets", c_int),
+        ("dust_feedback", c_int), ("short_friction_time_approximation", c_int),
+        ("no_dt_constraint_by_particles", c_int), ("dust_timestep", c_double),
+        ("factor_unit_mass", c_double), ("factor_unit_length", c_double),
+        ("particle_mass", c_double), ("seed", c_long),
+        ("dust_feel_turb", c_int), ("dust_growth", c_int), ("remove_dust_from_hill", c_int),
+        ("zz_integrator", c_int), ("dust_feel_sg_zero_mode", c_int),
+    ]
+
+    def copy(self):
+        q = DustParams()
+        ctypes.pointer(q)[0] = self
+        return q
+
+
+class DustState(ctypes.Structure):
+    """dfargo_dust_state: pointers to the nbpart-long particle arrays."""
+    _fields_ = [(n, ctypes.POINTER(c_double)) for n in
+                ("r", "th", "vr", "vth", "l", "dustsize", "dustsize_init", "stokesnb")]
+
+
+DUST_ARRAYS = ("r", "th", "vr", "vth", "l", "dustsize", "dustsize_init", "stokesnb")
+
+
+def dust_state(arrays):
+    """DustState pointing at a dict of float64 numpy arrays (kept alive by the caller)."""
+    st = DustState()
+    for n in DUST_ARRAYS:
+        setattr(st, n, arrays[n].ctypes.data_as(ctypes.POINTER(c_double)))
+    return st
+
+
+def dust_default_params(nbpart=0, **overrides):
+    """dfargo_dust_default_params() restated (src/var.c:136-181 defaults)."""
+    d = DustParams()
+    d.nbpart = nbpart
+    d.sizeminpart, d.sizemaxpart, d.sizepartslope = 0.001, 0.001, 3.0
+    d.rhopart, d.dustslope = 1.0, 0.5
+    d.rmindust, d.rmaxdust = 0.0, 0.0
+    d.interpolation = INTERP_TSC
+    d.dust_feel_disk, d.dust_feel_sg, d.dust_feel_planets = 1, 1, 1
+    d.dust_feedback, d.short_friction_time_approximation = 0, 1
+    d.no_dt_constraint_by_particles, d.dust_timestep = 1, 0.0
+    d.factor_unit_mass, d.factor_unit_length = 1.0, 1.0
+    d.particle_mass, d.seed = 0.0, 0
+    for k, v in overrides.items():
+        if not hasattr(d, k):
+            raise KeyError(f"unknown dust parameter {k!r}")
+        setattr(d, k, v)
+    return d
 
 
 def default_params(**overrides):
@@ -171,6 +233,12 @@ def load(path=None):
         "dfargo_get_kernel_times": (c_int, [vp, c_int, ctypes.c_void_p, dp,
                                             ctypes.POINTER(c_long), ctypes.POINTER(c_int)]),
         "dfargo_stream": (vp, [vp]),
+        "dfargo_dust_default_params": (None, [ctypes.POINTER(DustParams), c_int]),
+        "dfargo_check_dust_params": (c_int, [P, ctypes.POINTER(DustParams)]),
+        "dfargo_init_dust_system": (c_int, [P, ctypes.POINTER(DustParams), dp, ctypes.POINTER(DustState)]),
+        "dfargo_attach_dust": (c_int, [vp, ctypes.POINTER(DustParams), ctypes.POINTER(DustState), c_double]),
+        "dfargo_download_dust": (c_int, [vp, ctypes.POINTER(DustState)]),
+        "dfargo_upload_dust": (c_int, [vp, ctypes.POINTER(DustState)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -189,4 +257,6 @@ EXPORTED = [
     "dfargo_download_state", "dfargo_get_status", "dfargo_set_time", "dfargo_get_field",
     "dfargo_compute_selfgravity", "dfargo_fviscosity", "dfargo_get_fargo_shift",
     "dfargo_set_profiling", "dfargo_get_kernel_times", "dfargo_stream",
+    "dfargo_dust_default_params", "dfargo_check_dust_params", "dfargo_init_dust_system",
+    "dfargo_attach_dust", "dfargo_download_dust", "dfargo_upload_dust",
 ]

@@ -44,6 +44,10 @@ struct dfargo_ctx {
   std::vector<hipEvent_t> evpool;
   std::vector<double> VthetaMed, gcs_host;
   std::vector<float> fstage;
+  // dust particles (dfargo_attach_dust)
+  int dust_on = 0;
+  dfargo_dust_params dp;
+  std::vector<double> dustsize_init;
 };
 
 // ------------------------------------------------------------------------
@@ -170,6 +174,26 @@ static int run_substep(dfargo_ctx *c) {
 // ------------------------------------------------------------------------
 // creation
 
+// The kernels of one sub-step, in the order of src/SourceEuler.c:398-716.
+// Self-gravity only reads the density at the start of the sub-step: its
+// transforms run first, and SubStep1 applies the kick (update_sgvelocity).
+// The particles move after the planets (src/SourceEuler.c:504-531), i.e.
+// after the control kernel, and before the source terms that read their
+// feedback.
+static void build_sequence(dfargo_ctx *c) {
+  c->seq.clear();
+  if (c->p.self_gravity)
+    for (int id : {KID_SG_ROWS_FWD, KID_SG_TRANSPOSE_Z, KID_SG_COLS, KID_SG_TRANSPOSE_W, KID_SG_ROWS_INV})
+      c->seq.push_back(id);
+  for (int id : {KID_CFL, KID_CONTROL}) c->seq.push_back(id);
+  if (c->dust_on)
+    for (int id : {KID_DUST_MOVE, KID_DUST_COUNT, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_DEPOSIT})
+      c->seq.push_back(id);
+  c->seq.push_back(KID_SOURCE);
+  if (c->p.energy_equation) c->seq.push_back(KID_ENERGY);
+  for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_FINISH}) c->seq.push_back(id);
+}
+
 static int build_context(dfargo_ctx *c) {
   const dfargo_params &p = c->p;
   const int nr = p.nrad, ns = p.nsec;
@@ -249,7 +273,7 @@ static int build_context(dfargo_ctx *c) {
   c->esz = (c->prec == DFARGO_FP64) ? sizeof(double) : sizeof(float);
   c->nfield = (size_t)(nr + 2) * ns;
   void **fields[] = {&a.rho, &a.vr, &a.vt, &a.en, &a.cs, &a.vri, &a.vti, &a.trr, &a.trp, &a.tpp,
-                     &a.divv, &a.vh, &a.t_rho, &a.t_rp, &a.t_rm, &a.t_tp, &a.t_tm, &a.t_en, &a.pot};
+                     &a.divv, &a.vh, &a.t_rho, &a.t_rp, &a.t_rm, &a.t_tp, &a.t_tm, &a.t_en, &a.pot, &a.e_int};
   for (void **f : fields)
     if ((rc = dalloc(c, f, c->esz * c->nfield))) return rc;
   void **rings[] = {(void **)&a.vmoy, (void **)&a.ring_dt, (void **)&a.nu_ring, (void **)&a.gcs,
@@ -345,19 +369,7 @@ static int build_context(dfargo_ctx *c) {
   if (c->prec == DFARGO_FP64) prepare_kernels<double>(a);
   else prepare_kernels<float>(a);
   HIPCHK(hipGetLastError());
-  // the kernels of one sub-step, in the order of src/SourceEuler.c:398-716
-  c->seq = {KID_CFL, KID_CONTROL};
-  c->seq.push_back(KID_SUBSTEP1);
-  if (p.self_gravity)
-    for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV, KID_SG_KICK})
-      c->seq.push_back(id);
-  if (a.visc_on) {
-    c->seq.push_back(KID_VISC_STRESS);
-    c->seq.push_back(KID_VISC_UPDATE);
-  }
-  c->seq.push_back(KID_SUBSTEP2);
-  if (p.energy_equation) c->seq.push_back(KID_ENERGY);
-  for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_FINISH}) c->seq.push_back(id);
+  build_sequence(c);
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   return DFARGO_OK;
@@ -373,7 +385,7 @@ extern "C" int dfargo_create(const dfargo_params *p, int precision, int device, 
   if (p->self_gravity) {
     // the radial transforms of one wavenumber pair hold four 2*NRAD complex
     // columns in the 160 KiB of LDS
-    size_t lds = 4 * 2 * (size_t)p->nrad * 2 * (precision == DFARGO_FP64 ? 8 : 4);
+    size_t L = 2 * (size_t)p->nrad, lds = 4 * (L + L / 16 + 1) * 2 * (precision == DFARGO_FP64 ? 8 : 4);
     if (lds > 160 * 1024)
       return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
                               "self-gravity with NRAD=%d needs %zu bytes of LDS (> 160 KiB) at this precision",
@@ -737,6 +749,18 @@ extern "C" int dfargo_get_field(dfargo_ctx *c, int field, double *out) {
     case DFARGO_FIELD_TAURR: return get_field_raw(c, out, a.trr, (size_t)nr * ns);
     case DFARGO_FIELD_TAURP: return get_field_raw(c, out, a.trp, (size_t)nr * ns);
     case DFARGO_FIELD_TAUPP: return get_field_raw(c, out, a.tpp, (size_t)nr * ns);
+    case DFARGO_FIELD_DUST_DENSITY:
+      if (!c->dust_on) return DFARGO_OK;
+      return get_field_raw(c, out, a.dustdens, (size_t)nr * ns);
+    case DFARGO_FIELD_FB_ACCR:
+      if (!c->dust_on) return DFARGO_OK;
+      return get_field_raw(c, out, a.radfbacc, (size_t)nr * ns);
+    case DFARGO_FIELD_FB_ACCT:
+      if (!c->dust_on) return DFARGO_OK;
+      return get_field_raw(c, out, a.azifbacc, (size_t)nr * ns);
+    case DFARGO_FIELD_FB_EDOT:
+      if (!c->dust_on) return DFARGO_OK;
+      return get_field_raw(c, out, a.fbedot, (size_t)nr * ns);
   }
   return dfargo_set_error(DFARGO_ERR_ARG, "unknown field %d", field);
 }
@@ -807,3 +831,142 @@ extern "C" int dfargo_get_kernel_times(dfargo_ctx *c, int max, char (*names)[48]
 }
 
 extern "C" void *dfargo_stream(dfargo_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+// ------------------------------------------------------------------------
+// dust particles
+
+static int put_doubles(dfargo_ctx *c, double *dst, const double *src, size_t n) {
+  if (!src || n == 0) return DFARGO_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return DFARGO_OK;
+}
+static int get_doubles(dfargo_ctx *c, double *dst, const double *src, size_t n) {
+  if (!dst || n == 0) return DFARGO_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, const dfargo_dust_state *st,
+                                  double timestep) {
+  if (!c || !d || !st) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
+  if (c->dust_on) return dfargo_set_error(DFARGO_ERR_STATE, "particles are already attached to this context");
+  int rc = dfargo_check_dust_params(&c->p, d);
+  if (rc) return rc;
+  const int n = d->nbpart;
+  if (n == 0) return DFARGO_OK; // NBPART = 0: no particles, nothing changes
+  if (!st->r || !st->th || !st->vr || !st->vth || !st->l || !st->dustsize)
+    return dfargo_set_error(DFARGO_ERR_ARG, "the dust state needs r, th, vr, vth, l and dustsize");
+  if (!(timestep > 0.0)) return dfargo_set_error(DFARGO_ERR_ARG, "the start-up time step must be positive");
+  const dfargo_params &p = c->p;
+  const int nr = p.nrad, ns = p.nsec;
+  if (ns < 4) return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "particles need NSEC >= 4");
+  HIPCHK(hipSetDevice(c->dev));
+  KArgs &a = c->a;
+  const dfargo_tables &t = c->tab;
+  const size_t np = (size_t)n, ncell = (size_t)nr * ns;
+  double **pd[] = {&a.p_r, &a.p_th, &a.p_vr, &a.p_vth, &a.p_l, &a.p_size, &a.p_stokes,
+                   &a.p_raddrag, &a.p_azidrag, &a.p_ts, &a.p_dv2};
+  for (double **q : pd)
+    if ((rc = dalloc(c, (void **)q, sizeof(double) * np))) return rc;
+  int **pi[] = {&a.p_cell, &a.p_sorted};
+  for (int **q : pi)
+    if ((rc = dalloc(c, (void **)q, sizeof(int) * np))) return rc;
+  int **ci[] = {&a.cell_count, &a.cell_start, &a.cell_fill};
+  for (int **q : ci)
+    if ((rc = dalloc(c, (void **)q, sizeof(int) * ncell))) return rc;
+  if ((rc = dalloc(c, (void **)&a.scan_sums, sizeof(int) * (size_t)dust_scan_blocks((long)ncell)))) return rc;
+  void **grids[] = {&a.radgradp, &a.azigradp, &a.radindacc, &a.aziindacc, &a.radsgacc, &a.azisgacc,
+                    &a.radfbacc, &a.azifbacc, &a.fbedot, &a.dustdens, &a.cs_old};
+  for (void **g : grids)
+    if ((rc = dalloc(c, g, c->esz * c->nfield))) return rc;
+  if ((rc = upload_table(c, &a.Azimuth, t.Azimuth, (size_t)ns))) return rc;
+  // particle state
+  if ((rc = put_doubles(c, a.p_r, st->r, np)) || (rc = put_doubles(c, a.p_th, st->th, np)) ||
+      (rc = put_doubles(c, a.p_vr, st->vr, np)) || (rc = put_doubles(c, a.p_vth, st->vth, np)) ||
+      (rc = put_doubles(c, a.p_l, st->l, np)) || (rc = put_doubles(c, a.p_size, st->dustsize, np)))
+    return rc;
+  c->dustsize_init.assign(st->dustsize_init ? st->dustsize_init : st->dustsize,
+                          (st->dustsize_init ? st->dustsize_init : st->dustsize) + np);
+  // switches and constants (ComputeCodeUnits, src/LowTasks.c:204-212)
+  a.d_on = 1;
+  a.d_np = n;
+  a.d_interp = d->interpolation;
+  a.d_feel_disk = d->dust_feel_disk;
+  a.d_feel_sg = d->dust_feel_sg;
+  a.d_feel_planets = d->dust_feel_planets;
+  a.d_feedback = d->dust_feedback;
+  a.d_sfta = d->short_friction_time_approximation;
+  a.d_fixed_dt = !d->no_dt_constraint_by_particles;
+  a.log_grid = p.log_grid;
+  a.d_dust_timestep = d->dust_timestep;
+  a.d_unit_mass = 1.9891e30 * d->factor_unit_mass;
+  a.d_unit_length = 1.49598e11 * d->factor_unit_length;
+  a.d_rho_code = d->rhopart * 1000.0 * pow(a.d_unit_length, 3.) / a.d_unit_mass;
+  const double half = 0.5 * (p.pmax - p.pmin) / (double)ns;
+  a.aziinf0 = t.Azimuth[0] - half;
+  a.azisup_last = t.Azimuth[ns - 1] + half;
+  a.delta_phi = (t.Azimuth[0] + half) - (t.Azimuth[0] - half);
+  a.delta_r = p.log_grid ? log(t.Rsup[0] / t.Rinf[0]) : t.Rsup[0] - t.Rinf[0];
+  a.rmin = p.rmin;
+  a.rmax = p.rmax;
+  // control block: super-particle mass and a fresh minimum stopping time
+  if ((rc = ctrl_pull(c))) return rc;
+  c->h_ctrl->particles_mass = d->particle_mass;
+  c->h_ctrl->particles_mass0 = d->particle_mass;
+  c->h_ctrl->dust_mp = d->particle_mass;
+  double big = 1e6;
+  memcpy(&c->h_ctrl->min_ts_bits, &big, sizeof big);
+  if ((rc = ctrl_push(c))) return rc;
+  // start-up interpolation (src/main.c:287)
+  if (c->prec == DFARGO_FP64) launch_dust_startup<double>(a, timestep, c->stream);
+  else launch_dust_startup<float>(a, timestep, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->dp = *d;
+  c->dust_on = 1;
+  // the sub-step now holds the particle kernels: capture it again
+  if (c->gexec) {
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  if (c->graph) {
+    HIPCHK(hipGraphDestroy(c->graph));
+    c->graph = nullptr;
+  }
+  build_sequence(c);
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_download_dust(dfargo_ctx *c, dfargo_dust_state *st) {
+  if (!c || !st) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (!c->dust_on) return dfargo_set_error(DFARGO_ERR_STATE, "no particles are attached to this context");
+  HIPCHK(hipSetDevice(c->dev));
+  const KArgs &a = c->a;
+  const size_t np = (size_t)a.d_np;
+  int rc;
+  if ((rc = get_doubles(c, st->r, a.p_r, np)) || (rc = get_doubles(c, st->th, a.p_th, np)) ||
+      (rc = get_doubles(c, st->vr, a.p_vr, np)) || (rc = get_doubles(c, st->vth, a.p_vth, np)) ||
+      (rc = get_doubles(c, st->l, a.p_l, np)) || (rc = get_doubles(c, st->dustsize, a.p_size, np)) ||
+      (rc = get_doubles(c, st->stokesnb, a.p_stokes, np)))
+    return rc;
+  if (st->dustsize_init) memcpy(st->dustsize_init, c->dustsize_init.data(), sizeof(double) * np);
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_upload_dust(dfargo_ctx *c, const dfargo_dust_state *st) {
+  if (!c || !st) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (!c->dust_on) return dfargo_set_error(DFARGO_ERR_STATE, "no particles are attached to this context");
+  HIPCHK(hipSetDevice(c->dev));
+  const KArgs &a = c->a;
+  const size_t np = (size_t)a.d_np;
+  int rc;
+  if ((rc = put_doubles(c, a.p_r, st->r, np)) || (rc = put_doubles(c, a.p_th, st->th, np)) ||
+      (rc = put_doubles(c, a.p_vr, st->vr, np)) || (rc = put_doubles(c, a.p_vth, st->vth, np)) ||
+      (rc = put_doubles(c, a.p_l, st->l, np)) || (rc = put_doubles(c, a.p_size, st->dustsize, np)))
+    return rc;
+  if (st->dustsize_init) c->dustsize_init.assign(st->dustsize_init, st->dustsize_init + np);
+  return DFARGO_OK;
+}

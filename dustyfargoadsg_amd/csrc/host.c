@@ -315,3 +315,128 @@ int dfargo_set_error(int code, const char *fmt, ...) {
 }
 
 const char *dfargo_last_error(void) { return g_err; }
+
+/* ------------------------------------------------------------------------ */
+/* dust particles                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* src/var.c:136-181 and the booleans of src/Interpret.c:39-45 after the
+ * default .par strings are decoded */
+void dfargo_dust_default_params(dfargo_dust_params *d, int nbpart) {
+  memset(d, 0, sizeof(*d));
+  d->nbpart = nbpart;
+  d->sizeminpart = 0.001;
+  d->sizemaxpart = 0.001;
+  d->sizepartslope = 3.0;
+  d->rhopart = 1.0;
+  d->dustslope = 0.5;
+  d->rmindust = 0.0;
+  d->rmaxdust = 0.0;
+  d->interpolation = DFARGO_INTERP_TSC;
+  d->dust_feel_disk = 1;
+  d->dust_feel_sg = 1;
+  d->dust_feel_planets = 1;
+  d->dust_feedback = 0;
+  d->short_friction_time_approximation = 1;
+  d->no_dt_constraint_by_particles = 1;
+  d->dust_timestep = 0.0;
+  d->factor_unit_mass = 1.0;
+  d->factor_unit_length = 1.0;
+  d->particle_mass = 0.0;
+  d->seed = 0;
+}
+
+int dfargo_check_dust_params(const dfargo_params *p, const dfargo_dust_params *d) {
+  if (!p || !d) return dfargo_set_error(DFARGO_ERR_ARG, "null parameter block");
+  int rc = dfargo_check_params(p);
+  if (rc) return rc;
+  if (d->nbpart < 0) return dfargo_set_error(DFARGO_ERR_ARG, "NBPART must be >= 0");
+  if (d->nbpart > 0 && (d->sizeminpart == d->sizemaxpart) && (d->sizepartslope != 0.0)) /* src/Dsys.c:154-158 */
+    return dfargo_set_error(DFARGO_ERR_ARG, "SIZEMINPART = SIZEMAXPART needs SIZEPARTSLOPE = 0");
+  if (d->interpolation < DFARGO_INTERP_NGP || d->interpolation > DFARGO_INTERP_TSC)
+    return dfargo_set_error(DFARGO_ERR_ARG, "INTERPOLATION must be NGP, CIC or TSC");
+  if (!(d->sizeminpart > 0.0) || !(d->sizemaxpart > 0.0) || !(d->rhopart > 0.0))
+    return dfargo_set_error(DFARGO_ERR_ARG, "particle sizes and RHOPART must be positive");
+  if (!d->no_dt_constraint_by_particles && !(d->dust_timestep > 0.0))
+    return dfargo_set_error(DFARGO_ERR_ARG, "a particle time step constraint needs DUSTTIMESTEP > 0");
+  if (d->dust_feel_turb)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "DUSTFEELTURB (stochastic kicks) is not implemented");
+  if (d->dust_growth) return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "DUSTGROWTH is not implemented");
+  if (d->remove_dust_from_hill)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "REMOVEDUSTFROMPLANETSHILLRADIUS is not implemented");
+  if (d->zz_integrator) return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "ZZINTEGRATOR is not implemented");
+  if (d->dust_feel_sg_zero_mode)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "DUSTFEELSG Z (axisymmetric SG only) is not implemented");
+  if (p->omega_frame != 0.0)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
+                            "particles in a rotating frame (OMEGAFRAME != 0) are not implemented");
+  return DFARGO_OK;
+}
+
+/* InitDustSystem (src/Dsys.c:128-216), one process: sizes, then radii, then
+ * azimuths from one drand48 stream seeded like srand48(seed) */
+int dfargo_init_dust_system(const dfargo_params *p, const dfargo_dust_params *d, const double *axi_sg_accr,
+                            dfargo_dust_state *out) {
+  int rc = dfargo_check_dust_params(p, d);
+  if (rc) return rc;
+  if (!out || !out->r || !out->th || !out->vr || !out->vth || !out->l || !out->dustsize || !out->dustsize_init)
+    return dfargo_set_error(DFARGO_ERR_ARG, "null particle array");
+  if (p->self_gravity && !axi_sg_accr)
+    return dfargo_set_error(DFARGO_ERR_ARG, "self-gravity needs the axisymmetric SG acceleration");
+  dfargo_tables t;
+  rc = dfargo_host_tables(p, &t);
+  if (rc) return rc;
+  const int n = d->nbpart, nr = p->nrad, ns = p->nsec;
+  const double unit_mass = 1.9891e30 * d->factor_unit_mass, unit_length = 1.49598e11 * d->factor_unit_length;
+  const double rmindust = (d->rmindust == 0.0) ? p->rmin : d->rmindust;
+  const double rmaxdust = (d->rmaxdust == 0.0) ? p->rmax : d->rmaxdust;
+  (void)unit_mass;
+  unsigned short xs[3];
+  xs[0] = 0x330E;
+  xs[1] = (unsigned short)(d->seed & 0xFFFF);
+  xs[2] = (unsigned short)((d->seed >> 16) & 0xFFFF);
+  double C1, C2;
+  if (d->sizeminpart != d->sizemaxpart) {
+    if (d->sizepartslope != 1.0) {
+      C1 = pow(d->sizemaxpart, 1.0 - d->sizepartslope);
+      C2 = pow(d->sizeminpart, 1.0 - d->sizepartslope);
+      for (int i = 0; i < n; i++)
+        out->dustsize[i] = pow(C2 + (C1 - C2) * erand48(xs), 1.0 / (1.0 - d->sizepartslope)) / unit_length;
+    } else {
+      for (int i = 0; i < n; i++)
+        out->dustsize[i] = d->sizeminpart * exp(erand48(xs) * log(d->sizemaxpart / d->sizeminpart)) / unit_length;
+    }
+  } else {
+    for (int i = 0; i < n; i++) out->dustsize[i] = d->sizeminpart / unit_length;
+  }
+  if (d->dustslope != 1.0) {
+    C1 = pow(rmaxdust, 1.0 - d->dustslope);
+    C2 = pow(rmindust, 1.0 - d->dustslope);
+    for (int i = 0; i < n; i++) out->r[i] = pow(C2 + (C1 - C2) * erand48(xs), 1.0 / (1.0 - d->dustslope));
+  } else {
+    for (int i = 0; i < n; i++) out->r[i] = rmindust * exp(erand48(xs) * log(rmaxdust / rmindust));
+  }
+  /* AziInf[0] and AziSup[ns-1] (src/SourceEuler.c:172-173) */
+  const double aziinf0 = t.Azimuth[0] - 0.5 * (p->pmax - p->pmin) / (double)ns;
+  const double azisup = t.Azimuth[ns - 1] + 0.5 * (p->pmax - p->pmin) / (double)ns;
+  for (int i = 0; i < n; i++) out->th[i] = aziinf0 + erand48(xs) * (azisup - aziinf0);
+  for (int i = 0; i < n; i++) {
+    out->dustsize_init[i] = out->dustsize[i];
+    out->vr[i] = 0.0;
+    if (!p->self_gravity) {
+      out->vth[i] = pow(out->r[i], -0.5);
+    } else {
+      const double dist = out->r[i];
+      int ipl = 0;
+      while ((t.Rmed[ipl] <= dist) && (ipl < nr - 2)) ipl++;
+      const double ri = t.Rmed[ipl], rip1 = t.Rmed[ipl + 1], dr = rip1 - ri;
+      double sgacc = (dist - ri) * axi_sg_accr[ipl + 1] + (rip1 - dist) * axi_sg_accr[ipl];
+      sgacc /= dr;
+      out->vth[i] = pow(out->r[i], -0.5) * sqrt(1.0 - dist * dist * sgacc);
+    }
+    out->l[i] = out->r[i] * out->vth[i];
+    if (out->stokesnb) out->stokesnb[i] = 0.0;
+  }
+  dfargo_free_tables(&t);
+  return DFARGO_OK;
+}

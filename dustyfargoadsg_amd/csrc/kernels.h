@@ -32,11 +32,26 @@ struct KArgs {
   const double *open_vt_in1, *open_vt_in2, *open_vt_out;
   // fields (T)
   void *rho, *vr, *vt, *en, *cs, *vri, *vti, *trr, *trp, *tpp, *divv, *vh;
-  void *t_rho, *t_rp, *t_rm, *t_tp, *t_tm, *t_en, *sgr, *sgt, *pot;
+  void *t_rho, *t_rp, *t_rm, *t_tp, *t_tm, *t_en, *sgr, *sgt, *pot, *e_int;
   // ring buffers (FP64 unless noted)
   double *vmoy, *ring_dt, *ring_part, *nu_ring, *gcs, *vmed, *lost_col;
   long *nshift;
   dfargo_ctrl *ctrl;
+  // dust particles (FP64 whatever the field type) and the grids they share
+  // with the gas (T)
+  int d_on, d_np, d_interp, d_feel_disk, d_feel_sg, d_feel_planets, d_feedback, d_sfta;
+  int d_fixed_dt, log_grid;
+  double d_dust_timestep, d_rho_code, d_unit_mass, d_unit_length;
+  double aziinf0, azisup_last, delta_r, delta_phi, rmin, rmax;
+  const double *Azimuth;
+  double *p_r, *p_th, *p_vr, *p_vth, *p_l, *p_size, *p_stokes;
+  double *p_raddrag, *p_azidrag, *p_ts, *p_dv2;
+  int *p_cell, *cell_count, *cell_start, *cell_fill, *p_sorted, *scan_sums;
+  void *radgradp, *azigradp, *radindacc, *aziindacc, *radsgacc, *azisgacc;
+  void *radfbacc, *azifbacc, *fbedot, *dustdens;
+  // energy equation: the sound speed the sub-step started with, kept for
+  // the particles when the CFL kernel recomputes cs
+  void *cs_old;
   // self-gravity buffers (complex T)
   // sg_kr, sg_kt: kernel spectra for wavenumbers 0..Ns/2, [k2][k1] (transposed)
   void *sg_z, *sg_w, *sg_kr, *sg_kt, *tw_ns, *tw_2n;
@@ -44,10 +59,9 @@ struct KArgs {
 
 // launchers (step.hip, sg.hip); each enqueues on stream s
 enum KernelId {
-  KID_CFL = 0, KID_CONTROL, KID_SUBSTEP1, KID_SG_ROWS_FWD, KID_SG_COLS,
-  KID_SG_ROWS_INV, KID_SG_KICK, KID_VISC_STRESS,
-  KID_VISC_UPDATE, KID_SUBSTEP2, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
-  KID_FINISH, KID_COUNT
+  KID_SG_ROWS_FWD = 0, KID_SG_TRANSPOSE_Z, KID_SG_COLS, KID_SG_TRANSPOSE_W, KID_SG_ROWS_INV,
+  KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
+  KID_FINISH, KID_DUST_MOVE, KID_DUST_COUNT, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_DEPOSIT, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 
@@ -56,6 +70,11 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s);
 template <typename T> void prepare_kernels(const KArgs &a);
 template <typename T> void sg_prepare(const KArgs &a);
 template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s);
+// dust.hip: the particle kernels of a sub-step (after the control kernel),
+// and the start-up interpolation outside sub-steps
+template <typename T> void launch_dust_kernel(int id, const KArgs &a, hipStream_t s);
+template <typename T> void launch_dust_startup(const KArgs &a, double timestep, hipStream_t s);
+int dust_scan_blocks(long ncell); // block totals of the bucket scan (scan_sums length)
 template <typename T> void launch_eos_pre(const KArgs &a, hipStream_t s);
 template <typename T> void launch_potential(const KArgs &a, hipStream_t s);
 template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s); // rows_fwd..rows_inv

@@ -32,6 +32,12 @@ template <typename T> __device__ __forceinline__ cx<T> cmuli(cx<T> a, T s) {
   return {-s * a.y, s * a.x};
 }
 
+// LDS layout of a transform buffer: one spare slot after every 16
+// elements, so that the power-of-two strides of the Stockham passes spread
+// over the LDS banks.  A buffer of L elements takes fpsz(L) slots.
+__device__ __forceinline__ int fp(int i) { return i + (i >> 4); }
+__host__ __device__ inline int fpsz(int L) { return L + (L >> 4) + 1; }
+
 // DFT of length 4 in the direction dir (-1 forward, +1 inverse)
 template <typename T> __device__ __forceinline__ void dft4(const cx<T> (&x)[4], cx<T> (&y)[4], T dir) {
   const cx<T> a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
@@ -58,7 +64,7 @@ __device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int
     const int k = j - g * Ns;
     cx<T> v[R], y[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) v[r] = in[j + r * nb];
+    for (int r = 0; r < R; r++) v[r] = in[fp(j + r * nb)];
     if (Ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; r++) {
@@ -101,12 +107,12 @@ __device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int
     }
     const int idxD = g * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; r++) out[idxD + r * Ns] = y[r];
+    for (int r = 0; r < R; r++) out[fp(idxD + r * Ns)] = y[r];
   }
 }
 
 // Stockham transform of length L = 2^a 3^b <= 4096 held in LDS buffers A
-// and B (radix 3 passes first, then radix 8, then one radix 4 or 2 if
+// and B (padded layout fp) (radix 3 passes first, then radix 8, then one radix 4 or 2 if
 // needed); returns the buffer holding the result.  dir = -1 is the forward
 // exp(-2 pi i jk/L), +1 the un-normalised inverse.  tw holds
 // exp(-2 pi i m/L), m < L.
@@ -137,26 +143,49 @@ template <typename T> __device__ __forceinline__ double ldd(const void *p, long 
 }
 
 // compute_fftdensity (src/sgdens.c:54-70): Sr = Sigma sqrt(r/r0),
-// St = Sr r/r0 on the first Nrad rows, then the azimuthal transform
+// St = Sr r/r0 on the first Nrad rows, then the azimuthal transform; Z is
+// stored row-major [Nrad][Nsec]
 template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs a, int force) {
   extern __shared__ unsigned char smem[];
-  if (!force && !a.ctrl->active) return;
+  if (!force && !a.ctrl->go) return;
   const int i = blockIdx.x, ns = a.ns;
-  cx<T> *A = (cx<T> *)smem, *B = A + ns;
+  cx<T> *A = (cx<T> *)smem, *B = A + fpsz(ns);
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
     double sr = ldd<T>(a.rho, (long)i * ns + j) * a.sg_sqrt[i];
     double stv = sr * a.Rmed[i] / a.Rmed[0];
-    A[j] = {(T)sr, (T)stv};
+    A[fp(j)] = {(T)sr, (T)stv};
   }
   __syncthreads();
   cx<T> *r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, -1);
   cx<T> *z = (cx<T> *)a.sg_z + (long)i * ns;
-  for (int j = threadIdx.x; j < ns; j += blockDim.x) z[j] = r[j];
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) z[j] = r[fp(j)];
+}
+
+// dst[c][r] = src[r][c] for a rows x cols complex array, through 32 x 32
+// LDS tiles so that both sides are read and written along rows
+#define TP_T 32
+template <typename T>
+__global__ void __launch_bounds__(TP_T * 8) k_sg_transpose(KArgs a, int force, const cx<T> *src, cx<T> *dst,
+                                                           int rows, int cols) {
+  __shared__ cx<T> tile[TP_T][TP_T + 1];
+  if (!force && !a.ctrl->go) return;
+  const int c0 = blockIdx.x * TP_T, r0 = blockIdx.y * TP_T, tx = threadIdx.x, ty = threadIdx.y;
+  for (int q = ty; q < TP_T; q += 8) {
+    const int r = r0 + q, c = c0 + tx;
+    if (r < rows && c < cols) tile[q][tx] = src[(long)r * cols + c];
+  }
+  __syncthreads();
+  for (int q = ty; q < TP_T; q += 8) {
+    const int c = c0 + q, r = r0 + tx;
+    if (r < rows && c < cols) dst[(long)c * rows + r] = tile[tx][q];
+  }
 }
 
 // Radial transforms and spectral product (src/sgacc.c:25-42) for one pair
 // of azimuthal wavenumbers (k2, Ns-k2), whose spectra are tied by the
-// Hermitian symmetry of the two real reduced densities.  One workgroup:
+// Hermitian symmetry of the two real reduced densities.  Z and W are stored
+// transposed here ([Nsec][Nrad]), so each column is contiguous.  One
+// workgroup:
 //   1. loads both columns of Z (rows >= Nrad are the zero padding),
 //   2. transforms them over 2 Nrad,
 //   3. separates S_r(k) = (Z(k) + conj Z(-k))/2, S_t(k) = (Z(k) - conj Z(-k))/2i,
@@ -164,18 +193,19 @@ template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs
 //      only wavenumbers 0..Ns/2 are stored, transposed) and recombines
 //      A_r + i A_t,
 //   4. inverse-transforms both columns and keeps rows < Nrad.
-template <typename T> __global__ void __launch_bounds__(256) k_sg_cols(KArgs a, int force) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_sg_cols(KArgs a, int force, const cx<T> *zt, cx<T> *wt) {
   extern __shared__ unsigned char smem[];
-  if (!force && !a.ctrl->active) return;
-  const int ns = a.ns, nr = a.nr, L = 2 * nr, p = blockIdx.x;
+  if (!force && !a.ctrl->go) return;
+  const int ns = a.ns, nr = a.nr, L = 2 * nr, p = blockIdx.x, S = fpsz(L);
   const int ka = p, kb = (ns - p) % ns;
   const bool self = (ka == kb);
-  cx<T> *A1 = (cx<T> *)smem, *B1 = A1 + L, *A2 = B1 + L, *B2 = A2 + L;
-  const cx<T> *z = (const cx<T> *)a.sg_z;
+  cx<T> *A1 = (cx<T> *)smem, *B1 = A1 + S, *A2 = B1 + S, *B2 = A2 + S;
+  const cx<T> *za = zt + (long)ka * nr, *zb = zt + (long)kb * nr;
   const cx<T> zero = {(T)0, (T)0};
   for (int i = threadIdx.x; i < L; i += blockDim.x) {
-    A1[i] = (i < nr) ? z[(long)i * ns + ka] : zero;
-    if (!self) A2[i] = (i < nr) ? z[(long)i * ns + kb] : zero;
+    A1[fp(i)] = (i < nr) ? za[i] : zero;
+    if (!self) A2[fp(i)] = (i < nr) ? zb[i] : zero;
   }
   __syncthreads();
   const cx<T> *tw = (const cx<T> *)a.tw_2n;
@@ -188,92 +218,74 @@ template <typename T> __global__ void __launch_bounds__(256) k_sg_cols(KArgs a, 
   for (int k1 = threadIdx.x; k1 < L; k1 += blockDim.x) {
     const int m1 = (k1 == 0) ? 0 : L - k1;
     {
-      const cx<T> zz = ra[k1], zp = rb[m1];
+      const cx<T> zz = ra[fp(k1)], zp = rb[fp(m1)];
       const cx<T> sr = {h * (zz.x + zp.x), h * (zz.y - zp.y)};
       const cx<T> st = {h * (zz.y + zp.y), h * (zp.x - zz.x)};
       const cx<T> K1 = khr[k1], K2 = kht[k1];
       const cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
       const cx<T> at = {-G * (K2.x * st.x - K2.y * st.y), -G * (K2.x * st.y + K2.y * st.x)};
-      wa[k1] = {ar.x - at.y, ar.y + at.x};
+      wa[fp(k1)] = {ar.x - at.y, ar.y + at.x};
     }
     if (!self) {
-      const cx<T> zz = rb[k1], zp = ra[m1];
+      const cx<T> zz = rb[fp(k1)], zp = ra[fp(m1)];
       const cx<T> sr = {h * (zz.x + zp.x), h * (zz.y - zp.y)};
       const cx<T> st = {h * (zz.y + zp.y), h * (zp.x - zz.x)};
       const cx<T> K1 = {khr[m1].x, -khr[m1].y}, K2 = {kht[m1].x, -kht[m1].y};
       const cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
       const cx<T> at = {-G * (K2.x * st.x - K2.y * st.y), -G * (K2.x * st.y + K2.y * st.x)};
-      wb[k1] = {ar.x - at.y, ar.y + at.x};
+      wb[fp(k1)] = {ar.x - at.y, ar.y + at.x};
     }
   }
   __syncthreads();
   cx<T> *oa = fft_lds<T>(wa, (wa == A1) ? B1 : A1, L, tw, +1);
   cx<T> *ob = self ? oa : fft_lds<T>(wb, (wb == A2) ? B2 : A2, L, tw, +1);
-  cx<T> *w = (cx<T> *)a.sg_w;
+  cx<T> *wpa = wt + (long)ka * nr, *wpb = wt + (long)kb * nr;
   for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-    w[(long)i * ns + ka] = oa[i];
-    if (!self) w[(long)i * ns + kb] = ob[i];
+    wpa[i] = oa[fp(i)];
+    if (!self) wpb[i] = ob[fp(i)];
   }
 }
 
 // inverse azimuthal transform, normalisation and self-force compensation
-// (src/sgacc.c:137-161)
-template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_inv(KArgs a, int force) {
+// (src/sgacc.c:137-161); W row-major
+template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_inv(KArgs a, int force, const cx<T> *wrow) {
   extern __shared__ unsigned char smem[];
-  if (!force && !a.ctrl->active) return;
+  if (!force && !a.ctrl->go) return;
   const int i = blockIdx.x, ns = a.ns;
-  cx<T> *A = (cx<T> *)smem, *B = A + ns;
-  const cx<T> *w = (const cx<T> *)a.sg_w + (long)i * ns;
-  for (int j = threadIdx.x; j < ns; j += blockDim.x) A[j] = w[j];
+  cx<T> *A = (cx<T> *)smem, *B = A + fpsz(ns);
+  const cx<T> *w = wrow + (long)i * ns;
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) A[fp(j)] = w[j];
   __syncthreads();
   cx<T> *r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, +1);
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
     const long l = (long)i * ns + j;
-    double accr = (double)r[j].x * a.sg_normr[i];
-    double acct = (double)r[j].y * a.sg_normt[i];
+    double accr = (double)r[fp(j)].x * a.sg_normr[i];
+    double acct = (double)r[fp(j)].y * a.sg_normt[i];
     accr += DF_G * ldd<T>(a.rho, l) * a.sg_rstep * a.sg_tstep / a.sg_eps;
     ((T *)a.sgr)[l] = (T)accr;
     ((T *)a.sgt)[l] = (T)acct;
   }
 }
 
-// update_sgvelocity (src/sgupdate.c:30-72) on VradInt / VthetaInt
-template <typename T> __global__ void __launch_bounds__(256) k_sg_kick(KArgs a) {
-  const dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
-  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
-  const long l = (long)i * ns + j;
-  const double Dt = c->dt;
-  if (i > 0) {
-    double radsgacc = ((a.Rinf[i] - a.Rmed[i - 1]) * ldd<T>(a.sgr, l) +
-                       (a.Rmed[i] - a.Rinf[i]) * ldd<T>(a.sgr, l - ns)) * a.InvDiffRmed[i];
-    ((T *)a.vri)[l] = (T)(ldd<T>(a.vri, l) + Dt * radsgacc);
-  }
-  const long lm1 = (long)i * ns + (j == 0 ? ns - 1 : j - 1);
-  double azisgacc = 0.5 * (ldd<T>(a.sgt, l) + ldd<T>(a.sgt, lm1));
-  ((T *)a.vti)[l] = (T)(ldd<T>(a.vti, l) + Dt * azisgacc);
-}
-
 // generic in-place batched transforms (kernel spectra at set-up)
 template <typename T>
 __global__ void __launch_bounds__(256) k_fft_rows(cx<T> *data, int L, const cx<T> *tw, int dir) {
   extern __shared__ unsigned char smem[];
-  cx<T> *A = (cx<T> *)smem, *B = A + L, *row = data + (long)blockIdx.x * L;
-  for (int j = threadIdx.x; j < L; j += blockDim.x) A[j] = row[j];
+  cx<T> *A = (cx<T> *)smem, *B = A + fpsz(L), *row = data + (long)blockIdx.x * L;
+  for (int j = threadIdx.x; j < L; j += blockDim.x) A[fp(j)] = row[j];
   __syncthreads();
   cx<T> *r = fft_lds<T>(A, B, L, tw, dir);
-  for (int j = threadIdx.x; j < L; j += blockDim.x) row[j] = r[j];
+  for (int j = threadIdx.x; j < L; j += blockDim.x) row[j] = r[fp(j)];
 }
 template <typename T>
 __global__ void __launch_bounds__(256) k_fft_cols(cx<T> *data, int ncols, int L, const cx<T> *tw, int dir) {
   extern __shared__ unsigned char smem[];
-  cx<T> *A = (cx<T> *)smem, *B = A + L;
+  cx<T> *A = (cx<T> *)smem, *B = A + fpsz(L);
   const int c = blockIdx.x;
-  for (int i = threadIdx.x; i < L; i += blockDim.x) A[i] = data[(long)i * ncols + c];
+  for (int i = threadIdx.x; i < L; i += blockDim.x) A[fp(i)] = data[(long)i * ncols + c];
   __syncthreads();
   cx<T> *r = fft_lds<T>(A, B, L, tw, dir);
-  for (int i = threadIdx.x; i < L; i += blockDim.x) data[(long)i * ncols + c] = r[i];
+  for (int i = threadIdx.x; i < L; i += blockDim.x) data[(long)i * ncols + c] = r[fp(i)];
 }
 __global__ void k_convert_d2f(const double *src, float *dst, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -287,30 +299,39 @@ template <typename K> static void allow_lds(K kernel, size_t bytes) {
   if (bytes > 65536) hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+static size_t rows_lds_bytes(const KArgs &a, size_t csz) { return 2 * csz * fpsz(a.ns); }
+static size_t cols_lds_bytes(const KArgs &a, size_t csz) { return 4 * csz * fpsz(2 * a.nr); }
+
+// Z (row-major, sg_z) -> Z^T (sg_w) -> pair transforms -> W^T (sg_z) -> W
+// (row-major, sg_w) -> rows_inv
 template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
-  const size_t rows_lds = 2 * sizeof(cx<T>) * a.ns, cols_lds = 4 * sizeof(cx<T>) * 2 * a.nr;
+  const size_t rows_lds = rows_lds_bytes(a, sizeof(cx<T>)), cols_lds = cols_lds_bytes(a, sizeof(cx<T>));
+  cx<T> *z = (cx<T> *)a.sg_z, *w = (cx<T> *)a.sg_w;
   switch (id) {
     case KID_SG_ROWS_FWD:
       hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force);
       break;
+    case KID_SG_TRANSPOSE_Z:
+      hipLaunchKernelGGL(k_sg_transpose<T>, dim3((a.ns + TP_T - 1) / TP_T, (a.nr + TP_T - 1) / TP_T),
+                         dim3(TP_T, 8), 0, s, a, force, (const cx<T> *)z, w, a.nr, a.ns);
+      break;
     case KID_SG_COLS:
-      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(256), cols_lds, s, a, force);
+      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(256), cols_lds, s, a, force, (const cx<T> *)w, z);
+      break;
+    case KID_SG_TRANSPOSE_W:
+      hipLaunchKernelGGL(k_sg_transpose<T>, dim3((a.nr + TP_T - 1) / TP_T, (a.ns + TP_T - 1) / TP_T),
+                         dim3(TP_T, 8), 0, s, a, force, (const cx<T> *)z, w, a.ns, a.nr);
       break;
     case KID_SG_ROWS_INV:
-      hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force);
+      hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force, (const cx<T> *)w);
       break;
-    case KID_SG_KICK: {
-      int bx = a.ns < 256 ? a.ns : 256;
-      hipLaunchKernelGGL(k_sg_kick<T>, dim3((a.ns + bx - 1) / bx, a.nr), dim3(bx), 0, s, a);
-      break;
-    }
   }
 }
 
 // Raise the dynamic LDS limit of the kernels that need more than 64 KiB,
 // once at set-up (not inside a stream capture).
 template <typename T> void sg_prepare(const KArgs &a) {
-  const size_t rows_lds = 2 * sizeof(cx<T>) * a.ns, cols_lds = 4 * sizeof(cx<T>) * 2 * a.nr;
+  const size_t rows_lds = rows_lds_bytes(a, sizeof(cx<T>)), cols_lds = cols_lds_bytes(a, sizeof(cx<T>));
   allow_lds(k_sg_rows_fwd<T>, rows_lds);
   allow_lds(k_sg_cols<T>, cols_lds);
   allow_lds(k_sg_rows_inv<T>, rows_lds);
@@ -320,20 +341,19 @@ template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t 
   sg_pipeline<T>(id, a, s, 0);
 }
 template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s) {
-  sg_pipeline<T>(KID_SG_ROWS_FWD, a, s, 1);
-  sg_pipeline<T>(KID_SG_COLS, a, s, 1);
-  sg_pipeline<T>(KID_SG_ROWS_INV, a, s, 1);
+  for (int id : {KID_SG_ROWS_FWD, KID_SG_TRANSPOSE_Z, KID_SG_COLS, KID_SG_TRANSPOSE_W, KID_SG_ROWS_INV})
+    sg_pipeline<T>(id, a, s, 1);
 }
 template <typename T>
 void launch_fft_rows(void *data, int nrows, int L, const void *tw, int dir, hipStream_t s) {
-  allow_lds(k_fft_rows<T>, 2 * sizeof(cx<T>) * L);
-  hipLaunchKernelGGL(k_fft_rows<T>, dim3(nrows), dim3(256), 2 * sizeof(cx<T>) * L, s,
+  allow_lds(k_fft_rows<T>, 2 * sizeof(cx<T>) * fpsz(L));
+  hipLaunchKernelGGL(k_fft_rows<T>, dim3(nrows), dim3(256), 2 * sizeof(cx<T>) * fpsz(L), s,
                      (cx<T> *)data, L, (const cx<T> *)tw, dir);
 }
 template <typename T>
 void launch_fft_cols(void *data, int ncols, int L, const void *tw, int dir, hipStream_t s) {
-  allow_lds(k_fft_cols<T>, 2 * sizeof(cx<T>) * L);
-  hipLaunchKernelGGL(k_fft_cols<T>, dim3(ncols), dim3(256), 2 * sizeof(cx<T>) * L, s,
+  allow_lds(k_fft_cols<T>, 2 * sizeof(cx<T>) * fpsz(L));
+  hipLaunchKernelGGL(k_fft_cols<T>, dim3(ncols), dim3(256), 2 * sizeof(cx<T>) * fpsz(L), s,
                      (cx<T> *)data, ncols, L, (const cx<T> *)tw, dir);
 }
 void launch_convert_d2f(const double *src, float *dst, size_t n, hipStream_t s) {

@@ -13,10 +13,10 @@
 #include "kernels.h"
 
 const char *kKernelNames[KID_COUNT] = {
-    "cfl_ring",     "control",      "substep1",
-    "sg_rows_fwd", "sg_cols",      "sg_rows_inv",  "sg_kick",      "visc_stress",
-    "visc_update", "substep2",     "energy",
-    "transport_rad", "transport_azi", "finish"};
+    "sg_rows_fwd", "sg_transpose_z", "sg_cols", "sg_transpose_w", "sg_rows_inv",
+    "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
+    "transport_azi", "finish",      "dust_move",      "dust_count", "dust_scan",
+    "dust_scatter",  "dust_deposit"};
 
 // ------------------------------------------------------------------------
 // helpers
@@ -214,6 +214,20 @@ __device__ double pot_at(const KArgs &a, const dfargo_ctrl *c, int i, int j) {
   return pot;
 }
 
+// indirect part of the potential felt by the particles (the IndPot grid of
+// FillForcesArrays, src/Pframeforce.c:60-88): the planets' term is
+// assigned, not summed, so the last planet's stands, as in the reference
+__device__ double indpot_at(const KArgs &a, const dfargo_ctrl *c, int i, int j) {
+  const double x = a.Rmed[i] * a.CosAz[j], y = a.Rmed[i] * a.SinAz[j];
+  double ip = 0.0;
+  if (a.indirect_term && a.d_feel_planets && c->nb > 0) {
+    const int k = c->nb - 1;
+    ip = DF_G * c->pot_m[k] * c->pot_invd3[k] * (x * c->pot_x[k] + y * c->pot_y[k]);
+  }
+  if (a.gas_indirect && a.d_feel_disk) ip -= c->ind_x * x + c->ind_y * y;
+  return ip;
+}
+
 // ------------------------------------------------------------------------
 // per-call and per-sub-step control
 // ------------------------------------------------------------------------
@@ -229,9 +243,11 @@ __device__ void begin_substep(const KArgs &a, dfargo_ctrl *c) {
     c->mass_taper = mt;
     if (mt < 1.0)
       for (int k = 0; k < nb; k++) c->pl_m[k] = c->pl_m0[k] + (c->pl_m1[k] - c->pl_m0[k]) * mt;
+    c->particles_mass = c->particles_mass0 * mt;
   } else {
     for (int k = 0; k < nb; k++)
       if (c->pl_acc[k] < 1e-10) c->pl_m[k] = c->pl_m1[k];
+    c->particles_mass = c->particles_mass0;
   }
   double dtnbody = 100.0;
   if (nb > 1) {
@@ -298,42 +314,46 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
   if (!c->go) return;
   const int i = ring_of_block(a), ns = a.ns, nr = a.nr, t = threadIdx.x, TB = blockDim.x;
   const long base = (long)i * ns;
-  double vt_own[K];
+  // the CFL arithmetic is that of the storage type R (the ring mean is FP64)
+  using R = T;
+  const R *vt = (const R *)a.vt + base, *vr = (const R *)a.vr + base;
+  R vt_own[K];
   double s = 0.0;
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    vt_own[k] = ld<T>(a.vt, base + t + k * TB);
-    s += vt_own[k];
+    vt_own[k] = vt[t + k * TB];
+    s += (double)vt_own[k];
   }
   s = block_sum_fast(s, sh);
   const double vmoy = s / (double)ns;
   if (t == 0) a.vmoy[i] = vmoy;
   double dtmin = 1e30;
   if (i >= 1) {
-    const double viscosity = a.nu_ring[i];
-    const double dxrad = a.Rsup[i] - a.Rinf[i];
-    const double dxtheta = a.Rmed[i] * a.pm / (double)ns;
+    const R viscosity = (R)a.nu_ring[i];
+    const R dxrad = (R)(a.Rsup[i] - a.Rinf[i]);
+    const R dxtheta = (R)(a.Rmed[i] * a.pm / (double)ns);
+    const R vm = (R)vmoy, dxmin = (dxrad < dxtheta) ? dxrad : dxtheta;
+    const R invdt5 = (viscosity != R(0)) ? qdiv(viscosity * R(4), (R)a.powmin2[i]) : R(1e-10);
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const int j = t + k * TB;
-      const long l = base + j, lip = l + ns, ljp = (j == ns - 1) ? base : l + 1;
-      double cs = a.energy ? ld<T>(a.cs, l) : a.cs_iso[i];
-      double vres = a.fast ? vt_own[k] - vmoy : vt_own[k];
-      double vrl = ld<T>(a.vr, l);
-      double invdt1 = cs / (dmin2(dxrad, dxtheta));
-      double invdt2 = fabs(vrl) / dxrad;
-      double invdt3 = fabs(vres) / dxtheta;
-      double dvr = ld<T>(a.vr, lip) - vrl;
-      double dvt = ld<T>(a.vt, ljp) - vt_own[k];
-      if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
-      if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
-      double invdt4 = dmax2(dvr / dxrad, dvt / dxtheta);
-      invdt4 *= 4.0 * DF_CVNR * DF_CVNR;
-      double invdt5 = (viscosity != 0.0) ? viscosity * 4.0 / a.powmin2[i] : 1e-10;
-      double invdt6 = 1e-10;
-      double dt = DF_CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 +
-                                        invdt4 * invdt4 + invdt5 * invdt5 + invdt6 * invdt6);
-      dtmin = dmin2(dtmin, dt);
+      const int j = t + k * TB, jp = (j == ns - 1) ? 0 : j + 1;
+      const R cs = a.energy ? ((const R *)a.cs)[base + j] : (R)a.cs_iso[i];
+      const R vres = a.fast ? vt_own[k] - vm : vt_own[k];
+      const R vrl = vr[j];
+      const R invdt1 = qdiv(cs, dxmin);
+      const R invdt2 = qdiv(fabs(vrl), dxrad);
+      const R invdt3 = qdiv(fabs(vres), dxtheta);
+      R dvr = vr[j + ns] - vrl;
+      R dvt = vt[jp] - vt_own[k];
+      if (dvr >= R(0)) dvr = R(1e-10); else dvr = -dvr;
+      if (dvt >= R(0)) dvt = R(1e-10); else dvt = -dvt;
+      const R q4 = qdiv(dvr, dxrad), q5 = qdiv(dvt, dxtheta);
+      R invdt4 = (q5 > q4) ? q5 : q4;
+      invdt4 *= R(4.0 * DF_CVNR * DF_CVNR);
+      const R invdt6 = R(1e-10);
+      const R dt = qdiv(R(DF_CFLSECURITY), sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 +
+                                                   invdt4 * invdt4 + invdt5 * invdt5 + invdt6 * invdt6));
+      dtmin = dmin2(dtmin, (double)dt);
     }
   }
   // per-ring partials, stored partial-major (ring_part[p * nr + i]):
@@ -356,6 +376,8 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
         const double en = ld<T>(a.en, base + j);
         if (en < 0.0) cre += 1.0;
         const double cs = sqrt(g * (g - 1.0) * en / dens);
+        // the particles of this sub-step still read the previous cs
+        if (a.d_on) ((T *)a.cs_old)[base + j] = ((const T *)a.cs)[base + j];
         st<T>(a.cs, base + j, cs);
         css += (double)(T)cs;
       }
@@ -428,17 +450,17 @@ __device__ void d_deriv_rk5(const double *q, const double *masses, double *deriv
                             const int *feelothers, int indirect) {
   const double *x = q, *y = q + n, *vx = y + n, *vy = vx + n;
   double *dx = deriv, *dy = deriv + n, *dvx = dy + n, *dvy = dvx + n;
-  double Dist[DF_MAXPL];
-  for (int i = 0; i < n; i++) Dist[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
   for (int i = 0; i < n; i++) {
+    const double Di = sqrt(x[i] * x[i] + y[i] * y[i]);
     dx[i] = vx[i];
     dy[i] = vy[i];
-    dvx[i] = -DF_G * 1.0 / Dist[i] / Dist[i] / Dist[i] * x[i];
-    dvy[i] = -DF_G * 1.0 / Dist[i] / Dist[i] / Dist[i] * y[i];
+    dvx[i] = -DF_G * 1.0 / Di / Di / Di * x[i];
+    dvy[i] = -DF_G * 1.0 / Di / Di / Di * y[i];
     for (int j = 0; j < n; j++) {
       if (indirect) {
-        dvx[i] -= DF_G * masses[j] / Dist[j] / Dist[j] / Dist[j] * x[j];
-        dvy[i] -= DF_G * masses[j] / Dist[j] / Dist[j] / Dist[j] * y[j];
+        const double Dj = sqrt(x[j] * x[j] + y[j] * y[j]);
+        dvx[i] -= DF_G * masses[j] / Dj / Dj / Dj * x[j];
+        dvy[i] -= DF_G * masses[j] / Dj / Dj / Dj * y[j];
       }
       if ((j != i) && feelothers[i]) {
         double dist = (x[i] - x[j]) * (x[i] - x[j]) + (y[i] - y[j]) * (y[i] - y[j]);
@@ -456,10 +478,12 @@ __device__ void d_translate(const double *q0, const double *k1, const double *k2
   for (int i = 0; i < 4 * n; i++)
     q1[i] = q0[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i] + c4 * k4[i] + c5 * k5[i];
 }
+// k holds the six stages, 6 x 4*DF_MAXPL doubles (LDS: the control kernel
+// keeps per-thread arrays out of scratch memory)
 __device__ void d_rk5(const double *q0, double dt, const double *m, double *q1, int n,
-                      const int *fo, int ind) {
-  double k1[4 * DF_MAXPL], k2[4 * DF_MAXPL], k3[4 * DF_MAXPL], k4[4 * DF_MAXPL], k5[4 * DF_MAXPL],
-      k6[4 * DF_MAXPL];
+                      const int *fo, int ind, double *k) {
+  double *k1 = k, *k2 = k1 + 4 * DF_MAXPL, *k3 = k2 + 4 * DF_MAXPL, *k4 = k3 + 4 * DF_MAXPL,
+         *k5 = k4 + 4 * DF_MAXPL, *k6 = k5 + 4 * DF_MAXPL;
   for (int i = 0; i < 4 * DF_MAXPL; i++) k1[i] = k2[i] = k3[i] = k4[i] = k5[i] = k6[i] = 0.0;
   d_deriv_rk5(q0, m, k1, n, dt, fo, ind);
   d_translate(q0, k1, k2, k3, k4, k5, 0.2, 0.0, 0.0, 0.0, 0.0, q1, n);
@@ -514,15 +538,30 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
   __syncthreads();
   if (t != 0) return;
   // sub-step dt
-  double deltaT = c->DT - c->dtemp;
-  if (deltaT < nd) nd = deltaT;
-  int gts = (int)(ceil(deltaT / nd));
-  double dthydro = deltaT / (double)gts;
-  double dt = dmin2(dthydro, c->dtnbody);
+  double dt;
+  if (!a.d_fixed_dt) {
+    double deltaT = c->DT - c->dtemp;
+    // the minimum stopping time of the last interpolation, resolved by two
+    // steps (src/SourceEuler.c:1501-1507)
+    if (a.d_on && a.d_np > 0 && a.d_feedback) {
+      double dt_fb = 0.5 * __longlong_as_double((long long)c->min_ts_bits);
+      if (dt_fb == 0.0) dt_fb = 1e10;
+      if (dt_fb < nd) nd = dt_fb;
+    }
+    if (deltaT < nd) nd = deltaT;
+    int gts = (int)(ceil(deltaT / nd));
+    double dthydro = deltaT / (double)gts;
+    dt = dmin2(dthydro, c->dtnbody);
+    c->gas_timesteps_cfl = gts;
+  } else {
+    // the particles' fixed time step (src/SourceEuler.c:457-463)
+    dt = a.d_dust_timestep;
+    c->mindt = a.d_dust_timestep;
+    c->maxdt = a.d_dust_timestep;
+  }
   c->dtemp += dt;
   if (dt < c->mindt) c->mindt = dt;
   if (dt > c->maxdt) c->maxdt = dt;
-  c->gas_timesteps_cfl = gts;
   c->dt = dt;
   c->crashed_d = sums[0] > 0.0;
   c->crashed_e = sums[1] > 0.0;
@@ -580,14 +619,14 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
   }
   // AdvanceSystemRK5 (src/Pframeforce.c:163-266)
   if (nb > 0) {
-    double q0[4 * DF_MAXPL], q1[4 * DF_MAXPL], m[DF_MAXPL];
+    double *q0 = sh, *q1 = q0 + 4 * DF_MAXPL, *m = q1 + 4 * DF_MAXPL, *kst = m + DF_MAXPL;
     if (!a.forced_circular) {
       for (int i = 0; i < nb; i++) {
         q0[i] = c->pl_x[i]; q0[i + nb] = c->pl_y[i];
         q0[i + 2 * nb] = c->pl_vx[i]; q0[i + 3 * nb] = c->pl_vy[i];
         m[i] = c->pl_m[i];
       }
-      d_rk5(q0, dt, m, q1, nb, c->pl_feel_others, a.indirect_term);
+      d_rk5(q0, dt, m, q1, nb, c->pl_feel_others, a.indirect_term, kst);
     }
     for (int i = 1 - (c->time >= a.release_date); i < nb; i++) {
       if (!a.forced_circular) {
@@ -633,6 +672,13 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
   c->substeps += 1;
   c->substeps_total += 1;
   c->active = 1;
+  if (a.d_on) {
+    // masses of this sub-step for the particle kernels, and a fresh
+    // minimum for their interpolation (src/Dsys.c:300)
+    c->dust_mp = c->particles_mass;
+    for (int k = 0; k < nb; k++) c->dust_pl_m[k] = c->pl_m[k];
+    c->min_ts_bits = (unsigned long long)__double_as_longlong(1e6);
+  }
   // loop condition and start of the next sub-step
   c->go = c->dtemp < 0.999999999 * c->DT;
   if (c->go) begin_substep(a, c);
@@ -649,47 +695,6 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_eos
     const long l = base + t + k * TB;
     st<T>(a.cs, l, sqrt(g * (g - 1.0) * ld<T>(a.en, l) / ld<T>(a.rho, l)));
   }
-}
-
-// SubStep1 (src/SourceEuler.c:865-994): pressure, potential and curvature
-// kicks into VradInt / VthetaInt
-template <typename T> __global__ void __launch_bounds__(256) k_substep1(KArgs a) {
-  const dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
-  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
-  const long l = (long)i * ns + j;
-  const double dt = c->dt;
-  const double csi = a.cs_iso[i];
-  auto press = [&](long ll, int ii) -> double {
-    if (!a.energy) {
-      double cs = (double)(T)a.cs_iso[ii];
-      return ld<T>(a.rho, ll) * cs * cs;
-    }
-    return (a.gamma - 1.0) * ld<T>(a.en, ll);
-  };
-  (void)csi;
-  const double rl = ld<T>(a.rho, l), pl = press(l, i), potl = pot_at(a, c, i, j);
-  if (i >= 1) {
-    const long lim = l - ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
-    double gradp = (pl - press(lim, i - 1)) * 2.0 / (rl + ld<T>(a.rho, lim)) * a.InvDiffRmed[i];
-    double gradphi = (potl - pot_at(a, c, i - 1, j)) * a.InvDiffRmed[i];
-    double vt2 = ld<T>(a.vt, l) + ld<T>(a.vt, ljp) + ld<T>(a.vt, lim) + ld<T>(a.vt, ljp - ns);
-    vt2 = vt2 / 4.0 + a.Rinf[i] * c->omega_frame;
-    vt2 = vt2 * vt2;
-    st<T>(a.vri, l, ld<T>(a.vr, l) + dt * (-gradp - gradphi + vt2 * a.InvRinf[i]));
-  } else {
-    st<T>(a.vri, l, 0.0);
-  }
-  const int jm = (j == 0) ? ns - 1 : j - 1;
-  const long ljm = (long)i * ns + jm;
-  double dxtheta = a.pm / (double)ns * a.Rmed[i];
-  double invdxtheta = 1.0 / dxtheta;
-  double gradp = (pl - press(ljm, i)) * 2.0 / (rl + ld<T>(a.rho, ljm)) * invdxtheta;
-  double gradphi = (potl - pot_at(a, c, i, jm)) * invdxtheta;
-  double vti = ld<T>(a.vt, l) - dt * (gradp + gradphi);
-  vti += dt * a.supp_torque[i];
-  st<T>(a.vti, l, vti);
 }
 
 // ComputeViscousTerms stresses (src/Viscosity.c:159-300) on (vr, vt)
@@ -719,8 +724,8 @@ __device__ void visc_stress_cell(const KArgs &a, const void *vr, const void *vt,
   }
 }
 
+// stresses of the current state, for the diagnostics outside a sub-step
 template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs a, int from_state) {
-  if (!from_state && !a.ctrl->active) return;
   const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ns) return;
   const long l = (long)i * ns + j;
@@ -731,81 +736,222 @@ template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs
   st<T>(a.trp, l, Trp);
 }
 
-// UpdateVelocitiesWithViscosity (src/Viscosity.c:302-378) on VradInt/VthetaInt
-template <typename T> __global__ void __launch_bounds__(256) k_visc_update(KArgs a) {
-  const dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
-  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
-  const long l = (long)i * ns + j;
-  const double DeltaT = c->dt;
-  const double rho = ld<T>(a.rho, l);
-  if (i >= 1 && i < nr - 1) {
-    const long lip = l + ns, ljm = (j == 0) ? (long)i * ns + ns - 1 : l - 1;
-    double trpl = ld<T>(a.trp, l), trpp = ld<T>(a.trp, lip);
-    double v = ld<T>(a.vti, l);
-    v += DeltaT * a.InvRmed[i] *
-         ((a.Rsup[i] * trpp - a.Rinf[i] * trpl) * a.InvDiffRsup[i] +
-          (ld<T>(a.tpp, l) - ld<T>(a.tpp, ljm)) * a.invdphi + 0.5 * (trpl + trpp)) /
-         (0.5 * (rho + ld<T>(a.rho, ljm)));
-    st<T>(a.vti, l, v);
+// Source terms of one sub-step on tiles of SRC_TR rings x SRC_TC columns:
+// SubStep1 (src/SourceEuler.c:865-994, with the self-gravity kick of
+// update_sgvelocity, src/sgupdate.c:30-72), the viscous stresses and
+// velocity update (ComputeViscousTerms + UpdateVelocitiesWithViscosity,
+// src/Viscosity.c:159-378) and SubStep2 (src/SourceEuler.c:1012-1116), in
+// that order, with the intermediate VradInt / VthetaInt and stresses in
+// LDS.  Each stage is evaluated on the tile and the halo the next stage
+// reads: potential on a 4-cell halo, SubStep1 on 3, stresses on 2, the
+// viscous update on 1.  The state is read only; the post-SubStep2
+// velocities go to vri / vti and the energy to e_int (the ActualiseGas
+// copies of src/SourceEuler.c:598-630 are deferred to the end of the
+// sub-step), so no workgroup overwrites what another one still reads.
+#define SRC_TR 8
+#define SRC_TC 64
+#define SRC_PR (SRC_TR + 6)
+#define SRC_PC (SRC_TC + 6)
+#define SRC_VR (SRC_TR + 5)
+#define SRC_VC (SRC_TC + 5)
+#define SRC_SR (SRC_TR + 3)
+#define SRC_SC (SRC_TC + 3)
+
+template <typename T> __device__ __forceinline__ double src_press(const KArgs &a, long l, int i) {
+  if (!a.energy) {
+    const double cs = (double)(T)a.cs_iso[i];
+    return ld<T>(a.rho, l) * cs * cs;
   }
-  if (i >= 1) {
-    const long lim = l - ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
-    double v = ld<T>(a.vri, l);
-    v += DeltaT * a.InvRinf[i] *
-         ((a.Rmed[i] * ld<T>(a.trr, l) - a.Rmed[i - 1] * ld<T>(a.trr, lim)) * a.InvDiffRmed[i] +
-          (ld<T>(a.trp, ljp) - ld<T>(a.trp, l)) * a.invdphi - 0.5 * (ld<T>(a.tpp, l) + ld<T>(a.tpp, lim))) /
-         (0.5 * (rho + ld<T>(a.rho, lim)));
-    st<T>(a.vri, l, v);
-  }
+  return (a.gamma - 1.0) * ld<T>(a.en, l);
 }
 
-// SubStep2 (src/SourceEuler.c:1012-1116): artificial viscosity, written
-// straight into the state (the ActualiseGas copies of src/SourceEuler.c:
-// 598-630); vr of rings 0 and nrad is zero, as VradNew is never written there
-template <typename T> __global__ void __launch_bounds__(256) k_substep2(KArgs a) {
+template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
+  __shared__ double sPot[SRC_PR][SRC_PC];
+  __shared__ T sVr[SRC_VR][SRC_VC], sVt[SRC_VR][SRC_VC];
+  __shared__ T sTrr[SRC_SR][SRC_SC], sTpp[SRC_SR][SRC_SC], sTrp[SRC_SR][SRC_SC];
+  __shared__ int sCol[SRC_PC];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
-  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
-  const long l = (long)i * ns + j;
-  if (i == nr) {
-    st<T>(a.vr, l, 0.0);
-    st<T>(a.vt, l, 0.0);
-    return;
-  }
+  const int ns = a.ns, nr = a.nr, tid = threadIdx.x, TB = blockDim.x;
+  const int j0 = blockIdx.x * SRC_TC, i0 = blockIdx.y * SRC_TR;
   const double dt = c->dt;
-  const double C2 = DF_CVNR * DF_CVNR;
-  const long lip = l + ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
-  const long ljm = (j == 0) ? (long)i * ns + ns - 1 : l - 1;
-  const double rho = ld<T>(a.rho, l);
-  const double vrl = ld<T>(a.vri, l), vrp = ld<T>(a.vri, lip);
-  const double vtl = ld<T>(a.vti, l), vtp = ld<T>(a.vti, ljp), vtm = ld<T>(a.vti, ljm);
-  double dv = vrp - vrl;
-  const double qr = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
-  dv = vtp - vtl;
-  const double qt = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
-  // qt at j-1
-  const double rhojm = ld<T>(a.rho, ljm);
-  dv = vtl - vtm;
-  const double qtm = (dv < 0.0) ? C2 * rhojm * dv * dv : 0.0;
-  if (i >= 1) {
-    const long lim = l - ns;
-    const double rhoim = ld<T>(a.rho, lim);
-    dv = vrl - ld<T>(a.vri, lim);
-    const double qrm = (dv < 0.0) ? C2 * rhoim * dv * dv : 0.0;
-    st<T>(a.vr, l, vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * a.InvDiffRmed[i]);
-  } else {
-    st<T>(a.vr, l, 0.0);
+  // columns j0-4 .. j0+SRC_TC+1 of the potential region, wrapped
+  for (int p = tid; p < SRC_PC; p += TB) {
+    int jj = (j0 - 4 + p) % ns;
+    sCol[p] = (jj < 0) ? jj + ns : jj;
   }
-  double dxtheta = a.pm / (double)ns * a.Rmed[i];
-  double invdxtheta = 1.0 / dxtheta;
-  st<T>(a.vt, l, vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta);
-  if (a.energy) {
-    double e = ld<T>(a.en, l);
-    e = e - dt * qr * (vrp - vrl) * a.InvDiffRsup[i] - dt * qt * (vtp - vtl) * invdxtheta;
-    st<T>(a.en, l, e);
+  __syncthreads();
+  // stage 0: potential on rings i0-4 .. i0+SRC_TR+1
+  for (int idx = tid; idx < SRC_PR * SRC_PC; idx += TB) {
+    const int q = idx / SRC_PC, p = idx - q * SRC_PC, r = i0 - 4 + q;
+    sPot[q][p] = (r >= 0 && r <= nr - 1) ? pot_at(a, c, r, sCol[p]) : 0.0;
+  }
+  __syncthreads();
+  // stage 1: SubStep1 on rings i0-3 .. i0+SRC_TR+1 (VradInt and VthetaInt
+  // of ring nrad are zero, as in the reference where they are never set)
+  for (int idx = tid; idx < SRC_VR * SRC_VC; idx += TB) {
+    const int q = idx / SRC_VC, p = idx - q * SRC_VC, i = i0 - 3 + q, j = sCol[p + 1];
+    if (i < 0 || i > nr - 1) {
+      sVr[q][p] = (T)0;
+      sVt[q][p] = (T)0;
+      continue;
+    }
+    const long l = (long)i * ns + j;
+    const double rl = ld<T>(a.rho, l), pl = src_press<T>(a, l, i), potl = sPot[q + 1][p + 1];
+    // the particles' grids (pressure gradients, indirect and self-gravity
+    // accelerations, src/SourceEuler.c:893-948) on the tile's own cells
+    const int jt = j0 - 3 + p;
+    const bool dgrid = a.d_on && q >= 3 && q < 3 + SRC_TR && jt >= j0 && jt < j0 + SRC_TC && jt < ns;
+    double vri = 0.0;
+    if (i >= 1) {
+      const long lim = l - ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
+      const double plim = src_press<T>(a, lim, i - 1);
+      double gradp = (pl - plim) * 2.0 / (rl + ld<T>(a.rho, lim)) * a.InvDiffRmed[i];
+      double gradphi = (potl - sPot[q][p + 1]) * a.InvDiffRmed[i];
+      double vt2 = ld<T>(a.vt, l) + ld<T>(a.vt, ljp) + ld<T>(a.vt, lim) + ld<T>(a.vt, ljp - ns);
+      vt2 = vt2 / 4.0 + a.Rinf[i] * c->omega_frame;
+      vt2 = vt2 * vt2;
+      vri = (double)(T)(ld<T>(a.vr, l) + dt * (-gradp - gradphi + vt2 * a.InvRinf[i]));
+      if (a.d_on && a.d_feedback) vri = (double)(T)(vri + dt * ld<T>(a.radfbacc, l));
+      if (dgrid) {
+        st<T>(a.radgradp, l, (pl - plim) * a.InvDiffRmed[i]);
+        if (a.indirect_term)
+          st<T>(a.radindacc, l, -(indpot_at(a, c, i, j) - indpot_at(a, c, i - 1, j)) * a.InvDiffRmed[i]);
+      }
+      if (a.sg) {
+        double radsgacc = ((a.Rinf[i] - a.Rmed[i - 1]) * ld<T>(a.sgr, l) +
+                           (a.Rmed[i] - a.Rinf[i]) * ld<T>(a.sgr, lim)) * a.InvDiffRmed[i];
+        if (dgrid) st<T>(a.radsgacc, l, radsgacc);
+        vri = vri + dt * radsgacc;
+      }
+    }
+    const long ljm = (long)i * ns + sCol[p];
+    const double dxtheta = a.pm / (double)ns * a.Rmed[i], invdxtheta = 1.0 / dxtheta;
+    const double pljm = src_press<T>(a, ljm, i);
+    double gradp = (pl - pljm) * 2.0 / (rl + ld<T>(a.rho, ljm)) * invdxtheta;
+    double gradphi = (potl - sPot[q + 1][p]) * invdxtheta;
+    double vti = ld<T>(a.vt, l) - dt * (gradp + gradphi);
+    vti += dt * a.supp_torque[i];
+    if (a.d_on && a.d_feedback) vti += dt * ld<T>(a.azifbacc, l);
+    if (dgrid) {
+      st<T>(a.azigradp, l, (pl - pljm) * invdxtheta);
+      if (a.indirect_term)
+        st<T>(a.aziindacc, l, -(indpot_at(a, c, i, j) - indpot_at(a, c, i, sCol[p])) * invdxtheta);
+    }
+    if (a.sg) {
+      const double azisgacc = 0.5 * (ld<T>(a.sgt, l) + ld<T>(a.sgt, ljm));
+      if (dgrid) st<T>(a.azisgacc, l, azisgacc);
+      vti = (double)(T)vti;
+      vti = vti + dt * azisgacc;
+    }
+    sVr[q][p] = (T)vri;
+    sVt[q][p] = (T)vti;
+  }
+  __syncthreads();
+  if (a.visc_on) {
+    // stage 2: stresses on rings i0-2 .. i0+SRC_TR (V slot = S slot + 1)
+    const double onethird = 1.0 / 3.0;
+    for (int idx = tid; idx < SRC_SR * SRC_SC; idx += TB) {
+      const int q = idx / SRC_SC, p = idx - q * SRC_SC, i = i0 - 2 + q, j = sCol[p + 2];
+      if (i < 0 || i > nr - 1) {
+        sTrr[q][p] = sTpp[q][p] = sTrp[q][p] = (T)0;
+        continue;
+      }
+      const long l = (long)i * ns + j;
+      const double nu = a.nu_ring[i];
+      const double vrl = (double)sVr[q + 1][p + 1], vrp = (double)sVr[q + 2][p + 1];
+      const double vtl = (double)sVt[q + 1][p + 1], vtp = (double)sVt[q + 1][p + 2];
+      double dv = (vrp * a.Rsup[i] - vrl * a.Rinf[i]) * a.InvDiffRsup[i] * a.InvRmed[i];
+      dv += (vtp - vtl) * a.invdphi * a.InvRmed[i];
+      const double Drr = (vrp - vrl) * a.InvDiffRsup[i];
+      const double Dpp = (vtp - vtl) * a.invdphi * a.InvRmed[i] + 0.5 * (vrp + vrl) * a.InvRmed[i];
+      const double rho = ld<T>(a.rho, l);
+      sTrr[q][p] = (T)(2.0 * rho * nu * (Drr - onethird * dv));
+      sTpp[q][p] = (T)(2.0 * rho * nu * (Dpp - onethird * dv));
+      double Trp = 0.0;
+      if (i >= 1) {
+        const long lim = l - ns, ljm = (long)i * ns + sCol[p + 1], ljmim = ljm - ns;
+        const double Drp = 0.5 * (a.Rinf[i] * (vtl * a.InvRmed[i] - (double)sVt[q][p + 1] * a.InvRmed[i - 1]) *
+                                      a.InvDiffRmed[i] +
+                                  (vrl - (double)sVr[q + 1][p]) * a.invdphi * a.InvRinf[i]);
+        Trp = 2.0 * 0.25 * (rho + ld<T>(a.rho, lim) + ld<T>(a.rho, ljm) + ld<T>(a.rho, ljmim)) * nu * Drp;
+      }
+      sTrp[q][p] = (T)Trp;
+    }
+    __syncthreads();
+    // stage 3: the viscous update in place, VthetaInt on rings i0 ..
+    // i0+SRC_TR-1 x columns j0-1 .. j0+SRC_TC, VradInt on rings i0-1 ..
+    // i0+SRC_TR x columns j0 .. j0+SRC_TC-1 (S slot = V slot - 1)
+    for (int idx = tid; idx < SRC_TR * (SRC_TC + 2); idx += TB) {
+      const int q = idx / (SRC_TC + 2), p = idx - q * (SRC_TC + 2), i = i0 + q;
+      const int qv = q + 3, pv = p + 2, qs = qv - 1, ps = pv - 1;
+      if (i >= 1 && i < nr - 1) {
+        const long l = (long)i * ns + sCol[pv + 1], ljm = (long)i * ns + sCol[pv];
+        const double rho = ld<T>(a.rho, l);
+        const double trpl = (double)sTrp[qs][ps], trpp = (double)sTrp[qs + 1][ps];
+        double v = (double)sVt[qv][pv];
+        v += dt * a.InvRmed[i] *
+             ((a.Rsup[i] * trpp - a.Rinf[i] * trpl) * a.InvDiffRsup[i] +
+              ((double)sTpp[qs][ps] - (double)sTpp[qs][ps - 1]) * a.invdphi + 0.5 * (trpl + trpp)) /
+             (0.5 * (rho + ld<T>(a.rho, ljm)));
+        sVt[qv][pv] = (T)v;
+      }
+    }
+    for (int idx = tid; idx < (SRC_TR + 2) * SRC_TC; idx += TB) {
+      const int q = idx / SRC_TC, p = idx - q * SRC_TC, i = i0 - 1 + q;
+      const int qv = q + 2, pv = p + 3, qs = qv - 1, ps = pv - 1;
+      if (i >= 1 && i <= nr - 1) {
+        const long l = (long)i * ns + sCol[pv + 1], lim = l - ns;
+        const double rho = ld<T>(a.rho, l);
+        double v = (double)sVr[qv][pv];
+        v += dt * a.InvRinf[i] *
+             ((a.Rmed[i] * (double)sTrr[qs][ps] - a.Rmed[i - 1] * (double)sTrr[qs - 1][ps]) * a.InvDiffRmed[i] +
+              ((double)sTrp[qs][ps + 1] - (double)sTrp[qs][ps]) * a.invdphi -
+              0.5 * ((double)sTpp[qs][ps] + (double)sTpp[qs - 1][ps])) /
+             (0.5 * (rho + ld<T>(a.rho, lim)));
+        sVr[qv][pv] = (T)v;
+      }
+    }
+    __syncthreads();
+  }
+  // stage 4: SubStep2 on the tile (V slot = tile slot + 3)
+  const double C2 = DF_CVNR * DF_CVNR;
+  for (int idx = tid; idx < SRC_TR * SRC_TC; idx += TB) {
+    const int q = idx / SRC_TC, p = idx - q * SRC_TC, i = i0 + q, jt = j0 + p;
+    if (i > nr - 1 || jt >= ns) continue;
+    const int qv = q + 3, pv = p + 3, j = sCol[pv + 1];
+    const long l = (long)i * ns + j, ljm = (long)i * ns + sCol[pv];
+    const double rho = ld<T>(a.rho, l);
+    const double vrl = (double)sVr[qv][pv], vrp = (double)sVr[qv + 1][pv];
+    const double vtl = (double)sVt[qv][pv], vtp = (double)sVt[qv][pv + 1], vtm = (double)sVt[qv][pv - 1];
+    double dv = vrp - vrl;
+    const double qr = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
+    dv = vtp - vtl;
+    const double qt = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
+    const double rhojm = ld<T>(a.rho, ljm);
+    dv = vtl - vtm;
+    const double qtm = (dv < 0.0) ? C2 * rhojm * dv * dv : 0.0;
+    double vrn = 0.0;
+    if (i >= 1) {
+      const long lim = l - ns;
+      const double rhoim = ld<T>(a.rho, lim);
+      dv = vrl - (double)sVr[qv - 1][pv];
+      const double qrm = (dv < 0.0) ? C2 * rhoim * dv * dv : 0.0;
+      vrn = vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * a.InvDiffRmed[i];
+    }
+    const double dxtheta = a.pm / (double)ns * a.Rmed[i], invdxtheta = 1.0 / dxtheta;
+    st<T>(a.vri, l, vrn);
+    st<T>(a.vti, l, vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta);
+    if (a.energy) {
+      double e = ld<T>(a.en, l);
+      e = e - dt * qr * (vrp - vrl) * a.InvDiffRsup[i] - dt * qt * (vtp - vtl) * invdxtheta;
+      st<T>(a.e_int, l, e);
+    }
+    if (i == nr - 1) {
+      // vr and vt of ring nrad are zero after ActualiseGas
+      st<T>(a.vr, l + ns, 0.0);
+      st<T>(a.vt, l + ns, 0.0);
+    }
   }
 }
 
@@ -816,7 +962,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_substep2(KArgs a)
 template <typename T>
 __device__ double vischeat_cell(const KArgs &a, int i, int j, double *trr, double *tpp,
                                 double *trp, double *divv) {
-  visc_stress_cell<T>(a, a.vr, a.vt, i, j, trr, tpp, trp, divv);
+  visc_stress_cell<T>(a, a.vri, a.vti, i, j, trr, tpp, trp, divv);
   const double viscosity = a.nu_ring[i];
   if (viscosity == 0.0) return 0.0;
   const double dens = ld<T>(a.rho, (long)i * a.ns + j);
@@ -843,7 +989,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
     if (a.viscous_heating && i >= 1) {
       vh[k] = vischeat_cell<T>(a, i, j, &trr, &tpp, &trp, &d);
     } else {
-      visc_stress_cell<T>(a, a.vr, a.vt, i, j, &trr, &tpp, &trp, &d);
+      visc_stress_cell<T>(a, a.vri, a.vti, i, j, &trr, &tpp, &trp, &d);
       vh[k] = 0.0;
     }
     if (a.viscous_heating && i == 0) {
@@ -867,14 +1013,14 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const long l = base + t + k * TB;
-    double num = ld<T>(a.en, l) + (a.viscous_heating ? 1 : 0) * vh[k] * dt;
+    double num = ld<T>(a.e_int, l) + (a.viscous_heating ? 1 : 0) * vh[k] * dt;
     double den = 1.0 + (a.gamma - 1.0) * dt * dv[k];
     enew[k] = num / den;
   }
   if (a.beta_cooling) {
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < K; k++) s += (ld<T>(a.vt, base + t + k * TB) + a.Rmed[i] * c->omega_frame);
+    for (int k = 0; k < K; k++) s += (ld<T>(a.vti, base + t + k * TB) + a.Rmed[i] * c->omega_frame);
     s = block_sum_fast(s, sh);
     double omega = s / (double)ns;
     omega /= a.Rmed[i];
@@ -884,6 +1030,11 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
       double num = enew[k], den = 1.0 + dt / coolingtime;
       enew[k] = num / den;
     }
+  }
+  // heating by the dust feedback (src/SourceEuler.c:1291-1299)
+  if (a.d_on && a.d_feedback) {
+#pragma unroll
+    for (int k = 0; k < K; k++) enew[k] += ld<T>(a.fbedot, base + t + k * TB) * dt;
   }
 #pragma unroll
   for (int k = 0; k < K; k++) st<T>(a.en, base + t + k * TB, enew[k]);
@@ -925,8 +1076,8 @@ template <typename R>
 __device__ __forceinline__ void rad_quantities(const KArgs &a, int r, int j, R Of, R (&Q)[6]) {
   const int ns = a.ns;
   const int jp = (j == ns - 1) ? 0 : j + 1;
-  const R *rho = (const R *)a.rho + (long)r * ns, *vr = (const R *)a.vr + (long)r * ns;
-  const R *vt = (const R *)a.vt + (long)r * ns;
+  const R *rho = (const R *)a.rho + (long)r * ns, *vr = (const R *)a.vri + (long)r * ns;
+  const R *vt = (const R *)a.vti + (long)r * ns;
   const R d = rho[j], Rm = (R)a.Rmed[r];
   Q[0] = d * vr[j + ns];
   Q[1] = d * vr[j];
@@ -971,7 +1122,7 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
   const int i = i0 + ty;
   if (j >= ns || i >= nr) return;
   const long l = (long)i * ns + j;
-  const R v0 = ((const R *)a.vr)[l], v1 = ((const R *)a.vr)[l + ns];
+  const R v0 = ((const R *)a.vri)[l], v1 = ((const R *)a.vri)[l + ns];
   // stars at faces i (Work of rings i-1, i: slots ty+1, ty+2; slopes of
   // rings i-1, i: slots ty, ty+1) and i+1
   const R rs0 = rad_star<R>(a, i, v0, dt, sW[6][ty + 1][tx], sW[6][ty + 2][tx], sD[6][ty][tx], sD[6][ty + 1][tx]);
@@ -1113,7 +1264,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
 #pragma unroll
   for (int k = 0; k < K + 1; k++) {
     const int j = (j0 + k < ns) ? j0 + k : j0 + k - ns;
-    vel[k] = ld<T>(a.vt, base + j) * dt;
+    vel[k] = ld<T>(a.vti, base + j) * dt;
     if (k < K) s += vel[k] * invdt;
   }
   s = block_sum_fast(s, sh);
@@ -1307,17 +1458,9 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
   switch (id) {
     case KID_CFL: RING_DISPATCH(k_cfl_ring, T, a, s, 0); break;
     case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
-    case KID_SUBSTEP1:
-      hipLaunchKernelGGL(k_substep1<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
-      break;
-    case KID_VISC_STRESS:
-      hipLaunchKernelGGL(k_visc_stress<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a, 0);
-      break;
-    case KID_VISC_UPDATE:
-      hipLaunchKernelGGL(k_visc_update<T>, cell_grid(a, a.nr), cell_block(a), 0, s, a);
-      break;
-    case KID_SUBSTEP2:
-      hipLaunchKernelGGL(k_substep2<T>, cell_grid(a, a.nr + 1), cell_block(a), 0, s, a);
+    case KID_SOURCE:
+      hipLaunchKernelGGL(k_source<T>, dim3((a.ns + SRC_TC - 1) / SRC_TC, (a.nr + SRC_TR - 1) / SRC_TR), dim3(256), 0, s,
+                         a);
       break;
     case KID_ENERGY: RING_DISPATCH(k_energy, T, a, s, 0); break;
     case KID_TRANSPORT_RAD: {
@@ -1333,6 +1476,11 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
     case KID_FINISH:
       hipLaunchKernelGGL(k_finish<T>, cell_grid(a, a.nr + 1), cell_block(a), 0, s, a);
       break;
+    case KID_DUST_MOVE:
+    case KID_DUST_COUNT:
+    case KID_DUST_SCAN:
+    case KID_DUST_SCATTER:
+    case KID_DUST_DEPOSIT: launch_dust_kernel<T>(id, a, s); break;
     default: launch_sg_kernel<T>(id, a, s); break;
   }
 }

@@ -158,6 +158,33 @@ class Disc:
     def fviscosity(self, r):
         return self.lib.dfargo_fviscosity(self.ctx, r)
 
+    # -- dust particles (src/Dsys.c, src/DustUpdate.c) ---------------------
+    def attach_dust(self, d, parts, timestep):
+        """Particles from a dict of nbpart-long arrays (r, th, vr, vth, l,
+        dustsize) and the start-up interpolation with `timestep`."""
+        n = d.nbpart
+        arrs = {k: np.ascontiguousarray(parts.get(k, parts["dustsize"] if k == "dustsize_init"
+                                                  else np.zeros(n)), dtype=np.float64)
+                for k in _capi.DUST_ARRAYS}
+        self._dust_keep = arrs
+        self.dust_n = n
+        _check(self.lib, self.lib.dfargo_attach_dust(self.ctx, ctypes.byref(d), ctypes.byref(_capi.dust_state(arrs)),
+                                                     timestep))
+
+    def dust_state(self):
+        """The particle arrays (r, th, vr, vth, l, dustsize, dustsize_init,
+        stokesnb) as a dict of float64 arrays."""
+        arrs = {k: np.zeros(self.dust_n) for k in _capi.DUST_ARRAYS}
+        _check(self.lib, self.lib.dfargo_download_dust(self.ctx, ctypes.byref(_capi.dust_state(arrs))))
+        return arrs
+
+    def set_dust_state(self, parts):
+        arrs = {k: np.ascontiguousarray(parts[k], dtype=np.float64) for k in _capi.DUST_ARRAYS if k in parts}
+        st = _capi.DustState()
+        for k, v in arrs.items():
+            setattr(st, k, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        _check(self.lib, self.lib.dfargo_upload_dust(self.ctx, ctypes.byref(st)))
+
     # -- measurement --------------------------------------------------------
     def set_profiling(self, on):
         _check(self.lib, self.lib.dfargo_set_profiling(self.ctx, 1 if on else 0))
@@ -174,3 +201,14 @@ class Disc:
 
     def stream(self):
         return self.lib.dfargo_stream(self.ctx)
+
+
+def init_dust_system(params, d, axi_sg_accr=None, lib=None):
+    """InitDustSystem (src/Dsys.c:128-216) on the host: dict of particle arrays."""
+    lib = lib or _capi.load()
+    n = d.nbpart
+    arrs = {k: np.zeros(max(n, 1)) for k in _capi.DUST_ARRAYS}
+    _check(lib, lib.dfargo_init_dust_system(ctypes.byref(params), ctypes.byref(d), _ptr(axi_sg_accr),
+                                            ctypes.byref(_capi.dust_state(arrs))))
+    return {k: v[:n].copy() for k, v in arrs.items()}
+

@@ -64,6 +64,42 @@ def gas_sg_large(nrad=1024, nsec=3072, seed=DEFAULT_SEED, **over):
     return p, planets
 
 
+def dust_particles(params, nbpart, dust_to_gas=0.01, **over):
+    """Particle switches for `nbpart` super-particles over [rmindust,
+    rmaxdust] (default: the grid), with the super-particle mass of
+    src/main.c:190-203: the dust disc mass between RMINDUST and RMAXDUST
+    over NBPART, the dust density being dust_to_gas x Sigma
+    (DUSTTOGASDENSITYRATIO, src/Theo.c:66-72) taken without its noise."""
+    import ctypes
+
+    import numpy as np
+
+    from . import _capi
+    d = _capi.dust_default_params(nbpart, sizepartslope=0.0)
+    for k, v in over.items():
+        if not hasattr(d, k):
+            raise KeyError(f"unknown dust parameter {k!r}")
+        setattr(d, k, v)
+    if "particle_mass" not in over and nbpart > 0:
+        lib = _capi.load()
+        nr, ns = params.nrad, params.nsec
+        radii, rmed, surf, az = np.zeros(nr + 1), np.zeros(nr), np.zeros(nr), np.zeros(ns)
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        lib.dfargo_fill_polar_1d_arrays(ctypes.byref(params), dp(radii), dp(rmed), dp(surf), dp(az))
+        rmin = d.rmindust if d.rmindust != 0.0 else params.rmin
+        rmax = d.rmaxdust if d.rmaxdust != 0.0 else params.rmax
+        mass = 0.0
+        for i in range(nr):
+            if rmin <= radii[i] <= rmax:
+                r = rmed[i]
+                sig = params.sigma0 * r ** (-params.sigma_slope)
+                if r < params.cavity_radius:
+                    sig /= params.cavity_ratio
+                mass += sig * dust_to_gas * 0.5 * (params.pmax - params.pmin) * (radii[i + 1] ** 2 - radii[i] ** 2)
+        d.particle_mass = mass / nbpart
+    return d
+
+
 WORKLOADS = {
     "gas_planet": gas_planet,
     "energy_sg": energy_sg,

@@ -129,8 +129,50 @@ enum dfargo_field {
   DFARGO_FIELD_SG_ACCT = 7,      /* gtheta (src/sgacc.c:158)  */
   DFARGO_FIELD_TAURR = 8,
   DFARGO_FIELD_TAURP = 9,
-  DFARGO_FIELD_TAUPP = 10
+  DFARGO_FIELD_TAUPP = 10,
+  DFARGO_FIELD_DUST_DENSITY = 11, /* dustpcdens (src/Dsys.c:708)   */
+  DFARGO_FIELD_FB_ACCR = 12,      /* RadFBAcc (src/Dsys.c:877)     */
+  DFARGO_FIELD_FB_ACCT = 13,      /* AziFBAcc (src/Dsys.c:972)     */
+  DFARGO_FIELD_FB_EDOT = 14       /* FBedot (src/Dsys.c:1066)      */
 };
+
+/* ---- Lagrangian dust particles (src/Dsys.c, src/DustUpdate.c) ----------- */
+
+/* Interpolation kernels of `interpolation` (INTERPOLATION, src/Interpret.c:318-336). */
+enum dfargo_interp { DFARGO_INTERP_NGP = 0, DFARGO_INTERP_CIC = 1, DFARGO_INTERP_TSC = 2 };
+
+/* The particle switches of the .par file, decoded as src/Interpret.c does;
+ * defaults of src/var.c:136-181 (dfargo_dust_default_params).  Options the
+ * GPU path does not implement are fields so that they are rejected
+ * loudly: dust_feel_turb, dust_growth, remove_dust_from_hill,
+ * zz_integrator, dust_feel_sg_zero_mode must be 0. */
+typedef struct dfargo_dust_params {
+  int    nbpart;                     /* NBPART                                */
+  double sizeminpart, sizemaxpart;   /* SIZEMINPART, SIZEMAXPART (m)          */
+  double sizepartslope;              /* SIZEPARTSLOPE                         */
+  double rhopart;                    /* RHOPART (g/cm^3)                      */
+  double dustslope;                  /* DUSTSLOPE                             */
+  double rmindust, rmaxdust;         /* RMINDUST, RMAXDUST (0: RMIN, RMAX)    */
+  int    interpolation;              /* enum dfargo_interp                    */
+  int    dust_feel_disk, dust_feel_sg, dust_feel_planets;
+  int    dust_feedback;              /* DUSTFEEDBACK                          */
+  int    short_friction_time_approximation; /* SFTAPPROX                     */
+  int    no_dt_constraint_by_particles;     /* NODTCONSTRAINTBYPCS          */
+  double dust_timestep;              /* DUSTTIMESTEP (fixed dt otherwise)     */
+  double factor_unit_mass, factor_unit_length; /* FACTORUNITMASS/LENGTH      */
+  double particle_mass;              /* Particles_Mass_Initial (src/main.c:190-223) */
+  long   seed;                       /* seed of InitDustSystem's srand48      */
+  int    dust_feel_turb, dust_growth, remove_dust_from_hill, zz_integrator;
+  int    dust_feel_sg_zero_mode;
+} dfargo_dust_params;
+
+/* The particle arrays of DustSystem (src/types.h:114-146) the step reads
+ * and writes, nbpart entries each. */
+typedef struct dfargo_dust_state {
+  double *r, *th, *vr, *vth, *l;     /* position, velocity, R*vth            */
+  double *dustsize, *dustsize_init;  /* code units                            */
+  double *stokesnb;                  /* Stokes number of the last interpolation */
+} dfargo_dust_state;
 
 typedef struct dfargo_ctx dfargo_ctx;
 
@@ -246,6 +288,34 @@ int dfargo_get_kernel_times(dfargo_ctx *ctx, int max, char (*names)[48],
 
 /* The hipStream_t the context launches on, as an opaque pointer. */
 void *dfargo_stream(dfargo_ctx *ctx);
+
+/* ---- dust particles ------------------------------------------------------- */
+
+/* Defaults of src/var.c:136-181 (NODTCONSTRAINTBYPCS YES, DUSTFEEDBACK NO,
+ * INTERPOLATION TSC, SFTAPPROX YES) with nbpart particles. */
+void dfargo_dust_default_params(dfargo_dust_params *d, int nbpart);
+/* Checks the particle switches; DFARGO_ERR_UNSUPPORTED for the options the
+ * GPU path does not implement (and for a rotating frame, whose
+ * interpolation moves vtheta by +R*OmegaFrame and back). */
+int dfargo_check_dust_params(const dfargo_params *p, const dfargo_dust_params *d);
+/* InitDustSystem (src/Dsys.c:128-216): sizes, radii and azimuths drawn
+ * with drand48 seeded by d->seed (all sizes, then all radii, then all
+ * azimuths), vr = 0, vth Keplerian.  With self-gravity the axisymmetric
+ * radial SG acceleration (nrad values, GLOBAL_AxiSGAccr) corrects vth;
+ * pass NULL without self-gravity. */
+int dfargo_init_dust_system(const dfargo_params *p, const dfargo_dust_params *d,
+                            const double *axi_sg_accr, dfargo_dust_state *out);
+/* Attaches particles to a context (after dfargo_init_euler) and runs the
+ * start-up `interpolation` (src/main.c:287, with the output time step DT)
+ * that sets the gas values at the particles, the feedback grids and the
+ * minimum stopping time.  From
+ * then on every sub-step of AlgoGas moves the particles as
+ * src/SourceEuler.c:504-531 does. */
+int dfargo_attach_dust(dfargo_ctx *ctx, const dfargo_dust_params *d, const dfargo_dust_state *st,
+                       double timestep);
+/* Particle arrays back to the host / from the host (nbpart entries). */
+int dfargo_download_dust(dfargo_ctx *ctx, dfargo_dust_state *st);
+int dfargo_upload_dust(dfargo_ctx *ctx, const dfargo_dust_state *st);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,17 @@ struct or_ctx {
   double barx[3], bary[3];
   int GasTimeStepsCFL, last_substeps, crashed_d, crashed_e, crashed;
   long substeps;
+  /* gas grids the particles read and write (src/SourceEuler.c:210-219) */
+  double *indpot, *radgradp, *azigradp, *radindacc, *aziindacc, *radsgacc, *azisgacc;
+  double *radfbacc, *azifbacc, *fbedot, *dustdens;
+  /* dust particles (src/types.h:114-146) */
+  int dust_on, np;
+  dfargo_dust_params dp;
+  double *d_size, *d_size_init, *d_r, *d_th, *d_vr, *d_vth, *d_l;
+  double *d_vrg, *d_vtg, *d_densg, *d_csg, *d_radgradp, *d_azigradp, *d_radind, *d_aziind;
+  double *d_radsg, *d_azisg, *d_raddrag, *d_azidrag, *d_stokes;
+  double *AziInf, *AziSup;
+  double particles_mass_initial, particles_mass, min_stopping_time, unit_mass, unit_length;
 };
 
 static double min2(double a, double b) { return (b < a) ? b : a; }
@@ -217,6 +228,9 @@ or_ctx *or_create(const dfargo_params *p) {
   c->qrstar = grid(c); c->vthetares = grid(c); c->elong = grid(c);
   c->dq = grid(c); c->tempshift = grid(c); c->cabs = grid(c); c->cord = grid(c);
   c->sg_accr = grid(c); c->sg_acct = grid(c); c->gr = grid(c); c->gtheta = grid(c);
+  c->indpot = grid(c); c->radgradp = grid(c); c->azigradp = grid(c); c->radindacc = grid(c);
+  c->aziindacc = grid(c); c->radsgacc = grid(c); c->azisgacc = grid(c); c->radfbacc = grid(c);
+  c->azifbacc = grid(c); c->fbedot = grid(c); c->dustdens = grid(c);
   /* InitComputeAccel (src/SideEuler.c:263-280) */
   for (i = 0; i < nr; i++)
     for (j = 0; j < ns; j++) {
@@ -249,7 +263,14 @@ void or_destroy(or_ctx *c) {
                     &c->rp, &c->rm, &c->tp, &c->tm, &c->rhostar, &c->work,
                     &c->qrstar, &c->vthetares, &c->elong, &c->dq, &c->tempshift,
                     &c->cabs, &c->cord, &c->sg_accr, &c->sg_acct, &c->gr,
-                    &c->gtheta, &c->kr_re, &c->kr_im, &c->kt_re, &c->kt_im};
+                    &c->gtheta, &c->kr_re, &c->kr_im, &c->kt_re, &c->kt_im,
+                    &c->indpot, &c->radgradp, &c->azigradp, &c->radindacc, &c->aziindacc,
+                    &c->radsgacc, &c->azisgacc, &c->radfbacc, &c->azifbacc, &c->fbedot,
+                    &c->dustdens, &c->d_size, &c->d_size_init, &c->d_r, &c->d_th, &c->d_vr,
+                    &c->d_vth, &c->d_l, &c->d_vrg, &c->d_vtg, &c->d_densg, &c->d_csg,
+                    &c->d_radgradp, &c->d_azigradp, &c->d_radind, &c->d_aziind, &c->d_radsg,
+                    &c->d_azisg, &c->d_raddrag, &c->d_azidrag, &c->d_stokes, &c->AziInf,
+                    &c->AziSup};
   for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); k++) free(*all[k]);
   free(c->nshift);
   free(c->nosplit);
@@ -595,6 +616,8 @@ static void fill_forces_arrays(or_ctx *c) {
     c->indirect.y = 0.0;
   }
   for (int l = 0; l < (nr + 1) * ns; l++) c->pot[l] = 0.0;
+  for (int l = 0; l < (nr + 1) * ns; l++) c->indpot[l] = 0.0;
+  const int dust_planets = c->dust_on && c->dp.dust_feel_planets, dust_disk = c->dust_on && c->dp.dust_feel_disk;
   for (int k = 0; k < nb; k++) {
     double xplanet = c->sys.x[k], yplanet = c->sys.y[k];
     double mplanet = c->sys.mass[k] * planetmass_taper;
@@ -611,7 +634,11 @@ static void fill_forces_arrays(or_ctx *c) {
         double distance = (x - xplanet) * (x - xplanet) + (y - yplanet) * (y - yplanet);
         double distancesmooth = sqrt(distance + smooth);
         double pot = -G * mplanet / distancesmooth;
-        if (p->indirect_term) pot += G * mplanet * InvPlanetDistance3 * (x * xplanet + y * yplanet);
+        if (p->indirect_term) {
+          pot += G * mplanet * InvPlanetDistance3 * (x * xplanet + y * yplanet);
+          /* assigned, not summed, as in the reference */
+          if (dust_planets) c->indpot[l] = G * mplanet * InvPlanetDistance3 * (x * xplanet + y * yplanet);
+        }
         c->pot[l] += pot;
       }
   }
@@ -621,8 +648,10 @@ static void fill_forces_arrays(or_ctx *c) {
       int l = j + i * ns;
       double x = c->Rmed[i] * c->CosAz[j], y = c->Rmed[i] * c->SinAz[j];
       double pot = -G * 1.0 * InvDistance;
-      if ((!p->discard_gas_indirect_term) && (fabs(p->pmax - p->pmin - 2. * M_PI) < 0.01))
+      if ((!p->discard_gas_indirect_term) && (fabs(p->pmax - p->pmin - 2. * M_PI) < 0.01)) {
         pot -= c->indirect.x * x + c->indirect.y * y;
+        if (dust_disk) c->indpot[l] -= c->indirect.x * x + c->indirect.y * y;
+      }
       c->pot[l] += pot;
     }
   }
@@ -960,10 +989,12 @@ static void compute_selfgravity(or_ctx *c, double *vrad, double *vtheta, double 
       if (i > 0) {
         double radsgacc = ((c->Rinf[i] - c->Rmed[i - 1]) * c->sg_accr[l] +
                            (c->Rmed[i] - c->Rinf[i]) * c->sg_accr[l - ns]) * c->InvDiffRmed[i];
+        c->radsgacc[l] = radsgacc;
         vrad[l] += Dt * radsgacc;
       }
       int lm1 = i * ns + (j == 0 ? ns - 1 : j - 1);
       double azisgacc = 0.5 * (c->sg_acct[l] + c->sg_acct[lm1]);
+      c->azisgacc[l] = azisgacc;
       vtheta[l] += Dt * azisgacc;
     }
 }
@@ -1133,6 +1164,13 @@ int or_condition_cfl(or_ctx *c, double deltaT) {
                 fabs(Vmoy[i] * c->InvRmed[i] - Vmoy[i + 1] * c->InvRmed[i + 1]);
     if (dt < newdt) newdt = dt;
   }
+  /* the minimum stopping time of the last interpolation resolved by two
+     steps (src/SourceEuler.c:1501-1507) */
+  if (c->dust_on && c->np != 0 && c->dp.dust_feedback) {
+    double dt_fb = 0.5 * c->min_stopping_time;
+    if (dt_fb == 0.0) dt_fb = 1e10;
+    if (dt_fb < newdt) newdt = dt_fb;
+  }
   if (deltaT < newdt) newdt = deltaT;
   free(Vres);
   return (int)(ceil(deltaT / newdt));
@@ -1152,11 +1190,14 @@ void or_substep1(or_ctx *c, double dt) {
     for (int j = 0; j < ns; j++) {
       int l = j + i * ns, lim = l - ns, ljp = (j == ns - 1) ? i * ns : l + 1;
       double gradp = (Press[l] - Press[lim]) * 2.0 / (rho[l] + rho[lim]) * c->InvDiffRmed[i];
+      c->radgradp[l] = (Press[l] - Press[lim]) * c->InvDiffRmed[i];
       double gradphi = (Pot[l] - Pot[lim]) * c->InvDiffRmed[i];
+      c->radindacc[l] = -(c->indpot[l] - c->indpot[lim]) * c->InvDiffRmed[i];
       double vt2 = vtheta[l] + vtheta[ljp] + vtheta[lim] + vtheta[ljp - ns];
       vt2 = vt2 / 4.0 + c->Rinf[i] * c->OmegaFrame;
       vt2 = vt2 * vt2;
       vradint[l] = vrad[l] + dt * (-gradp - gradphi + vt2 * c->InvRinf[i]);
+      if (c->dust_on && c->dp.dust_feedback) vradint[l] += dt * c->radfbacc[l];
     }
   for (int i = 0; i < nr; i++) {
     double dxtheta = (p->pmax - p->pmin) / (double)ns * c->Rmed[i];
@@ -1165,9 +1206,12 @@ void or_substep1(or_ctx *c, double dt) {
     for (int j = 0; j < ns; j++) {
       int l = j + i * ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
       double gradp = (Press[l] - Press[ljm]) * 2.0 / (rho[l] + rho[ljm]) * invdxtheta;
+      c->azigradp[l] = (Press[l] - Press[ljm]) * invdxtheta;
       double gradphi = (Pot[l] - Pot[ljm]) * invdxtheta;
+      c->aziindacc[l] = -(c->indpot[l] - c->indpot[ljm]) * invdxtheta;
       vthetaint[l] = vtheta[l] - dt * (gradp + gradphi);
       vthetaint[l] += dt * supp_torque;
+      if (c->dust_on && c->dp.dust_feedback) vthetaint[l] += dt * c->azifbacc[l];
     }
   }
   if (p->self_gravity) compute_selfgravity(c, vradint, vthetaint, dt, 1);
@@ -1236,6 +1280,9 @@ static void substep3(or_ctx *c, double dt) {
       }
     }
   }
+  /* heating by the dust feedback (src/SourceEuler.c:1291-1299) */
+  if (c->dust_on && c->dp.dust_feedback)
+    for (int l = 0; l < nr * ns; l++) c->energynew[l] += c->fbedot[l] * dt;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1538,6 +1585,448 @@ static int detect_crash(or_ctx *c, const double *a) {
 /* AlgoGas                                                                    */
 /* ------------------------------------------------------------------------ */
 
+
+/* ------------------------------------------------------------------------ */
+/* Lagrangian dust particles (src/Dsys.c, src/DustUpdate.c)                  */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeCodeUnits (src/LowTasks.c:204-212): unit mass in kg, unit length
+ * in m */
+static void dust_units(const dfargo_dust_params *d, double *unit_mass, double *unit_length) {
+  *unit_mass = 1.9891e30 * d->factor_unit_mass;
+  *unit_length = 1.49598e11 * d->factor_unit_length;
+}
+
+/* InitDustSystem (src/Dsys.c:128-216), single process: all sizes, then all
+ * radii, then all azimuths from one drand48 stream */
+int or_dust_init_system(const dfargo_params *p, const dfargo_dust_params *d, const double *axi_sg_accr,
+                        double *size, double *r, double *th, double *vr, double *vth, double *l) {
+  or_ctx *c = or_create(p);
+  if (!c) return -1;
+  int nr = c->nr, ns = c->ns, n = d->nbpart;
+  double unit_mass, unit_length, C1, C2;
+  dust_units(d, &unit_mass, &unit_length);
+  double rmindust = (d->rmindust == 0.0) ? p->rmin : d->rmindust;
+  double rmaxdust = (d->rmaxdust == 0.0) ? p->rmax : d->rmaxdust;
+  unsigned short xs[3];
+  xs[0] = 0x330E;
+  xs[1] = (unsigned short)(d->seed & 0xFFFF);
+  xs[2] = (unsigned short)((d->seed >> 16) & 0xFFFF);
+  if (d->sizeminpart != d->sizemaxpart) {
+    if (d->sizepartslope != 1.0) {
+      C1 = pow(d->sizemaxpart, 1.0 - d->sizepartslope);
+      C2 = pow(d->sizeminpart, 1.0 - d->sizepartslope);
+      for (int i = 0; i < n; i++)
+        size[i] = pow(C2 + (C1 - C2) * erand48(xs), 1.0 / (1.0 - d->sizepartslope)) / unit_length;
+    } else {
+      for (int i = 0; i < n; i++)
+        size[i] = d->sizeminpart * exp(erand48(xs) * log(d->sizemaxpart / d->sizeminpart)) / unit_length;
+    }
+  } else {
+    for (int i = 0; i < n; i++) size[i] = d->sizeminpart / unit_length;
+  }
+  if (d->dustslope != 1.0) {
+    C1 = pow(rmaxdust, 1.0 - d->dustslope);
+    C2 = pow(rmindust, 1.0 - d->dustslope);
+    for (int i = 0; i < n; i++) r[i] = pow(C2 + (C1 - C2) * erand48(xs), 1.0 / (1.0 - d->dustslope));
+  } else {
+    for (int i = 0; i < n; i++) r[i] = rmindust * exp(erand48(xs) * log(rmaxdust / rmindust));
+  }
+  double aziinf0 = c->Azimuth[0] - 0.5 * (p->pmax - p->pmin) / (double)ns;
+  double azisup = c->Azimuth[ns - 1] + 0.5 * (p->pmax - p->pmin) / (double)ns;
+  for (int i = 0; i < n; i++) th[i] = aziinf0 + erand48(xs) * (azisup - aziinf0);
+  for (int i = 0; i < n; i++) {
+    vr[i] = 0.0;
+    if (!p->self_gravity) {
+      vth[i] = pow(r[i], -0.5);
+    } else {
+      double dist = r[i];
+      int ipl = 0;
+      while ((c->Rmed[ipl] <= dist) && (ipl < nr - 2)) ipl++;
+      double ri = c->Rmed[ipl], rip1 = c->Rmed[ipl + 1], dr = rip1 - ri;
+      double sgacc = (dist - ri) * axi_sg_accr[ipl + 1] + (rip1 - dist) * axi_sg_accr[ipl];
+      sgacc /= dr;
+      vth[i] = pow(r[i], -0.5) * sqrt(1.0 - dist * dist * sgacc);
+    }
+    l[i] = r[i] * vth[i];
+  }
+  or_destroy(c);
+  return 0;
+}
+
+int or_dust_attach(or_ctx *c, const dfargo_dust_params *d, const double *size, const double *r,
+                   const double *th, const double *vr, const double *vth, const double *l, double timestep) {
+  int n = d->nbpart, ns = c->ns;
+  c->dust_on = 1;
+  c->dp = *d;
+  c->np = n;
+#define DA(x) c->x = (double *)calloc((size_t)n + 1, sizeof(double))
+  DA(d_size); DA(d_size_init); DA(d_r); DA(d_th); DA(d_vr); DA(d_vth); DA(d_l);
+  DA(d_vrg); DA(d_vtg); DA(d_densg); DA(d_csg); DA(d_radgradp); DA(d_azigradp);
+  DA(d_radind); DA(d_aziind); DA(d_radsg); DA(d_azisg); DA(d_raddrag); DA(d_azidrag); DA(d_stokes);
+#undef DA
+  memcpy(c->d_size, size, sizeof(double) * n);
+  memcpy(c->d_size_init, size, sizeof(double) * n);
+  memcpy(c->d_r, r, sizeof(double) * n);
+  memcpy(c->d_th, th, sizeof(double) * n);
+  memcpy(c->d_vr, vr, sizeof(double) * n);
+  memcpy(c->d_vth, vth, sizeof(double) * n);
+  memcpy(c->d_l, l, sizeof(double) * n);
+  c->AziInf = (double *)calloc((size_t)ns, sizeof(double));
+  c->AziSup = (double *)calloc((size_t)ns, sizeof(double));
+  for (int j = 0; j < ns; j++) {
+    c->AziInf[j] = c->Azimuth[j] - 0.5 * (c->p.pmax - c->p.pmin) / (double)ns;
+    c->AziSup[j] = c->Azimuth[j] + 0.5 * (c->p.pmax - c->p.pmin) / (double)ns;
+  }
+  dust_units(d, &c->unit_mass, &c->unit_length);
+  c->particles_mass_initial = d->particle_mass;
+  c->particles_mass = d->particle_mass;
+  /* start-up interpolation (src/main.c:287), with the sound speed of
+     InitEuler */
+  c->min_stopping_time = 1e6;
+  or_dust_interpolation(c, timestep);
+  return 0;
+}
+
+/* weights of `interpolation` for the cell at distances (dr, dphi); kind is
+ * the stencil: 0 the vr one (radial NGP cut), 1 the vtheta one (azimuthal
+ * NGP cut), 2 the cell-centred one (src/Dsys.c:407-441,490-524,640-665) */
+static void dust_weights(int scheme, int kind, double dr, double dphi, double delta_r, double delta_phi,
+                         double *wr, double *wt) {
+  if (scheme == DFARGO_INTERP_NGP) {
+    *wr = 1.0;
+    *wt = 1.0;
+    if (kind == 0) *wr = (dr < 0.5 * delta_r) ? 1.0 : 0.0;
+    if (kind == 1) *wt = (dphi < 0.5 * delta_phi) ? 1.0 : 0.0;
+  } else if (scheme == DFARGO_INTERP_CIC) {
+    *wr = 1.0 - dr / delta_r;
+    *wt = 1.0 - dphi / delta_phi;
+  } else {
+    if (dr < 0.5 * delta_r) *wr = 0.75 - dr * dr / delta_r / delta_r;
+    if ((dr >= 0.5 * delta_r) && (dr <= 1.5 * delta_r)) *wr = 0.5 * (1.5 - dr / delta_r) * (1.5 - dr / delta_r);
+    if (dr > 1.5 * delta_r) *wr = 0.0;
+    if (dphi < 0.5 * delta_phi) *wt = 0.75 - dphi * dphi / delta_phi / delta_phi;
+    if ((dphi >= 0.5 * delta_phi) && (dphi <= 1.5 * delta_phi))
+      *wt = 0.5 * (1.5 - dphi / delta_phi) * (1.5 - dphi / delta_phi);
+    if (dphi > 1.5 * delta_phi) *wt = 0.0;
+  }
+}
+
+/* index ranges of the three stencils for the particle in cell (ip, jp) */
+static void dust_stencil(or_ctx *c, int scheme, int kind, int ip, int jp, double rp, double tp, int *imin,
+                         int *imax, int *jmin, int *jmax) {
+  int myip = ip, myjp = jp;
+  if (scheme == DFARGO_INTERP_NGP) {
+    *imin = ip; *imax = ip + (kind == 0 ? 1 : 0);
+    *jmin = jp; *jmax = jp + (kind == 1 ? 1 : 0);
+  } else if (scheme == DFARGO_INTERP_CIC) {
+    if (kind != 1) myjp = (tp < c->Azimuth[jp]) ? jp - 1 : jp;
+    if (kind != 0) myip = (rp < c->Rmed[ip]) ? ip - 1 : ip;
+    *imin = myip; *imax = myip + 1;
+    *jmin = myjp; *jmax = myjp + 1;
+  } else {
+    if (kind == 1) myjp = (tp > c->Azimuth[jp]) ? jp + 1 : jp;
+    *imin = ip - 1; *imax = ip + 1;
+    *jmin = myjp - 1; *jmax = myjp + 1;
+  }
+}
+
+/* wrapped azimuthal index and the stencil's reference azimuth of column j */
+static int dust_col(or_ctx *c, int kind, int j, double delta_phi, double *az) {
+  int ns = c->ns;
+  const double *A = (kind == 1) ? c->AziInf : c->Azimuth;
+  if (j >= 0 && j < ns) {
+    *az = A[j];
+    return j;
+  }
+  if (j < 0) {
+    *az = A[0] + j * delta_phi;
+    return j + ns;
+  }
+  *az = A[ns - 1] + (j + 1 - ns) * delta_phi;
+  return j - ns;
+}
+
+/* radial distance of the stencil: to Rinf for the vr stencil, Rmed else */
+static double dust_dr(or_ctx *c, int kind, int i, double rp) {
+  if (i < 0 || i > c->nr - 1) return 0.0;
+  double ref = (kind == 0) ? c->Rinf[i] : c->Rmed[i];
+  return c->p.log_grid ? fabs(log(rp / ref)) : fabs(rp - ref);
+}
+
+/* interpolation (src/Dsys.c:265-1149), single process: gas values at the
+ * particles, Stokes numbers and drag accelerations, feedback grids and the
+ * dust density.  vtheta is read in the fixed frame (the reference adds
+ * Rmed*OmegaFrame to the gas grid and removes it afterwards; a rotating
+ * frame is not supported with particles). */
+void or_dust_interpolation(or_ctx *c, double timestep) {
+  const dfargo_params *p = &c->p;
+  const dfargo_dust_params *d = &c->dp;
+  int nr = c->nr, ns = c->ns, scheme = d->interpolation;
+  const double *vr = c->vr, *vt = c->vt, *dens = c->rho, *cs = c->cs;
+  double dust_density_codeunits = d->rhopart * 1000.0 * pow(c->unit_length, 3.) / c->unit_mass;
+  for (int l = 0; l < nr * ns; l++) {
+    if (d->dust_feedback) {
+      c->radfbacc[l] = 0.0;
+      c->azifbacc[l] = 0.0;
+      c->fbedot[l] = 0.0;
+    }
+    c->dustdens[l] = 0.0;
+  }
+  double delta_r = p->log_grid ? log(c->Rsup[0] / c->Rinf[0]) : c->Rsup[0] - c->Rinf[0];
+  double delta_phi = c->AziSup[0] - c->AziInf[0];
+  c->min_stopping_time = 1e6;
+  for (int k = 0; k < c->np; k++) {
+    c->d_vrg[k] = c->d_vtg[k] = c->d_radgradp[k] = c->d_azigradp[k] = 0.0;
+    c->d_radind[k] = c->d_radsg[k] = c->d_aziind[k] = c->d_azisg[k] = 0.0;
+    c->d_densg[k] = c->d_csg[k] = 0.0;
+    double rp = c->d_r[k];
+    if (!((rp >= c->Rinf[0]) && (rp < c->Rsup[nr - 1]))) continue;
+    int ip = 0;
+    while (rp >= c->Rinf[ip]) ip++;
+    ip--;
+    double tp = c->d_th[k];
+    int jp = (int)floor(ns * (tp - c->AziInf[0]) / (c->AziSup[ns - 1] - c->AziInf[0]));
+    if (jp == ns) jp = 0;
+    int imin, imax, jmin, jmax;
+    double wr = 0.0, wt = 0.0, az;
+    /* 2a) vr stencil */
+    dust_stencil(c, scheme, 0, ip, jp, rp, tp, &imin, &imax, &jmin, &jmax);
+    for (int j = jmin; j <= jmax; j++) {
+      int myj = dust_col(c, 0, j, delta_phi, &az);
+      double dphi = fabs(tp - az);
+      for (int i = imin; i <= imax; i++) {
+        dust_weights(scheme, 0, dust_dr(c, 0, i, rp), dphi, delta_r, delta_phi, &wr, &wt);
+        if (i >= 0 && i <= nr - 1) {
+          int l = i * ns + myj;
+          c->d_vrg[k] += (wr * wt * vr[l]);
+          c->d_radgradp[k] += (wr * wt * c->radgradp[l]);
+          if (p->indirect_term) c->d_radind[k] += (wr * wt * c->radindacc[l]);
+          if (p->self_gravity) c->d_radsg[k] += (wr * wt * c->radsgacc[l]);
+        }
+      }
+    }
+    /* 2b) vtheta stencil */
+    dust_stencil(c, scheme, 1, ip, jp, rp, tp, &imin, &imax, &jmin, &jmax);
+    for (int j = jmin; j <= jmax; j++) {
+      int myj = dust_col(c, 1, j, delta_phi, &az);
+      double dphi = fabs(tp - az);
+      for (int i = imin; i <= imax; i++) {
+        dust_weights(scheme, 1, dust_dr(c, 1, i, rp), dphi, delta_r, delta_phi, &wr, &wt);
+        if (i >= 0 && i <= nr - 1) {
+          int l = i * ns + myj;
+          c->d_vtg[k] += (wr * wt * vt[l]);
+          c->d_azigradp[k] += (wr * wt * c->azigradp[l]);
+          if (p->indirect_term) c->d_aziind[k] += (wr * wt * c->aziindacc[l]);
+          if (p->self_gravity) c->d_azisg[k] += (wr * wt * c->azisgacc[l]);
+        }
+      }
+    }
+    /* 2c) cell-centred stencil: density, sound speed, dust density */
+    dust_stencil(c, scheme, 2, ip, jp, rp, tp, &imin, &imax, &jmin, &jmax);
+    for (int j = jmin; j <= jmax; j++) {
+      int myj = dust_col(c, 2, j, delta_phi, &az);
+      double dphi = fabs(tp - az);
+      for (int i = imin; i <= imax; i++) {
+        dust_weights(scheme, 2, dust_dr(c, 2, i, rp), dphi, delta_r, delta_phi, &wr, &wt);
+        if (i >= 0 && i <= nr - 1) {
+          int l = i * ns + myj;
+          c->d_densg[k] += (wr * wt * dens[l]);
+          c->d_csg[k] += (wr * wt * cs[l]);
+          c->dustdens[l] += wr * wt * c->particles_mass / c->Surf[i];
+        }
+      }
+    }
+    /* 2d) Stokes number (Paardekooper 2007) */
+    double vrp = c->d_vr[k], vtp = c->d_vth[k], vrg = c->d_vrg[k], vtg = c->d_vtg[k];
+    double csg = c->d_csg[k], densg = c->d_densg[k];
+    double densg_cgs = 0.1 * densg * c->unit_mass * pow(c->unit_length, -2.0);
+    double lambda = 3.34e-8 * pow(densg_cgs, -1.) * aspect_ratio(p, rp) * rp;
+    double Kn = 0.5 * lambda / c->d_size[k];
+    double Mach = sqrt((vrp - vrg) * (vrp - vrg) + (vtp - vtg) * (vtp - vtg)) / csg;
+    double f_D = sqrt(1.0 + 9.0 * M_PI * Mach * Mach / 128.0);
+    double Re = 3.0 * sqrt(M_PI / 8.0) * Mach / Kn;
+    double k_D = 0.0;
+    if (Re <= 500.0)
+      k_D = 1.0 + 0.15 * pow(Re, 0.687);
+    else {
+      if ((Re > 500.0) && (Re <= 1500.0)) k_D = 3.96e-6 * pow(Re, 2.4);
+      if (Re > 1500.0) k_D = 0.11 * Re;
+    }
+    double Cdrag = pow(3.0 * Kn + 1.0, 2.0) * pow(9.0 * Kn * Kn * f_D + 3.0 * Kn * k_D, -1.0);
+    double St = 0.5 * M_PI * Cdrag * c->d_size[k] * dust_density_codeunits / densg;
+    double Ts = St * pow(c->d_r[k], 1.5);
+    if (Ts < c->min_stopping_time) c->min_stopping_time = Ts;
+    c->d_stokes[k] = St;
+    /* 3) drag acceleration */
+    if ((Ts < timestep) && d->short_friction_time_approximation) {
+      vrp = vrg + Ts * c->d_radgradp[k] / densg;
+      vtp = vtg + Ts * c->d_azigradp[k] / densg;
+    }
+    c->d_raddrag[k] = -(vrp - vrg) / Ts;
+    c->d_azidrag[k] = -(vtp - vtg) / Ts;
+    if (!d->dust_feedback) continue;
+    double mp = c->particles_mass;
+    /* 4a) radial feedback, vr stencil */
+    dust_stencil(c, scheme, 0, ip, jp, rp, tp, &imin, &imax, &jmin, &jmax);
+    for (int j = jmin; j <= jmax; j++) {
+      int myj = dust_col(c, 0, j, delta_phi, &az);
+      double dphi = fabs(tp - az);
+      for (int i = imin; i <= imax; i++) {
+        dust_weights(scheme, 0, dust_dr(c, 0, i, rp), dphi, delta_r, delta_phi, &wr, &wt);
+        if (i >= 0 && i <= nr - 1) {
+          int l = i * ns + myj;
+          double mg = dens[l] * c->Surf[i];
+          c->radfbacc[l] -= (wr * wt * c->d_raddrag[k]) * mp / mg;
+        }
+      }
+    }
+    /* 4b) azimuthal feedback, vtheta stencil */
+    dust_stencil(c, scheme, 1, ip, jp, rp, tp, &imin, &imax, &jmin, &jmax);
+    for (int j = jmin; j <= jmax; j++) {
+      int myj = dust_col(c, 1, j, delta_phi, &az);
+      double dphi = fabs(tp - az);
+      for (int i = imin; i <= imax; i++) {
+        dust_weights(scheme, 1, dust_dr(c, 1, i, rp), dphi, delta_r, delta_phi, &wr, &wt);
+        if (i >= 0 && i <= nr - 1) {
+          int l = i * ns + myj;
+          double mg = dens[l] * c->Surf[i];
+          c->azifbacc[l] -= (wr * wt * c->d_azidrag[k]) * mp / mg;
+        }
+      }
+    }
+    /* 4c) heating rate, cell-centred stencil */
+    if (p->energy_equation) {
+      dust_stencil(c, scheme, 2, ip, jp, rp, tp, &imin, &imax, &jmin, &jmax);
+      for (int j = jmin; j <= jmax; j++) {
+        int myj = dust_col(c, 2, j, delta_phi, &az);
+        double dphi = fabs(tp - az);
+        for (int i = imin; i <= imax; i++) {
+          dust_weights(scheme, 2, dust_dr(c, 2, i, rp), dphi, delta_r, delta_phi, &wr, &wt);
+          if (i >= 0 && i <= nr - 1) {
+            int l = i * ns + myj;
+            double rhop = mp / c->Surf[i];
+            double dv2 = (vrp - vrg) * (vrp - vrg) + (vtp - vtg) * (vtp - vtg);
+            c->fbedot[l] += rhop * wr * wt * dv2 / Ts;
+          }
+        }
+      }
+    }
+  }
+}
+
+/* keep an azimuth in [AziInf[0], AziSup[ns-1]] (src/DustUpdate.c:38-45) */
+static double dust_wrap(or_ctx *c, double th) {
+  int ns = c->ns;
+  if (th < c->AziInf[0]) th += (c->AziSup[ns - 1] - c->AziInf[0]);
+  if (th > c->AziSup[ns - 1]) th -= (c->AziSup[ns - 1] - c->AziInf[0]);
+  return th;
+}
+
+/* SemiUpdateDustPositions (src/DustUpdate.c:6-138), single process, no
+ * turbulent kicks and no Hill-sphere removal */
+static void dust_semi_update_positions(or_ctx *c, double timestep) {
+  int nr = c->nr;
+  for (int k = 0; k < c->np; k++) {
+    double r = c->d_r[k];
+    if (!((r >= c->Rinf[0]) && (r < c->Rsup[nr - 1]))) continue;
+    c->d_r[k] += c->d_vr[k] * 0.5 * timestep;
+    c->d_th[k] += c->d_vth[k] * r * 0.25 * timestep * (pow(r, -2.) + pow(c->d_r[k], -2.));
+    c->d_th[k] = dust_wrap(c, c->d_th[k]);
+    c->d_th[k] = dust_wrap(c, c->d_th[k]);
+  }
+}
+
+/* UpdateDustVelocities (src/DustUpdate.c:267-435), semi-implicit
+ * integrator or the short-friction-time limit */
+static void dust_update_velocities(or_ctx *c, double timestep) {
+  const dfargo_params *p = &c->p;
+  const dfargo_dust_params *d = &c->dp;
+  int nr = c->nr;
+  or_dust_interpolation(c, timestep);
+  for (int k = 0; k < c->np; k++) {
+    double rd = c->d_r[k];
+    if (!((rd >= c->Rinf[0]) && (rd < c->Rsup[nr - 1]))) continue;
+    double td = c->d_th[k], vrd = c->d_vr[k], vtd = c->d_vth[k];
+    double vrg = c->d_vrg[k], vtg = c->d_vtg[k], St = c->d_stokes[k];
+    int local_feel_disk = d->dust_feel_disk;
+    if ((rd < p->rmin) || (rd > p->rmax)) local_feel_disk = 0;
+    double stoptime = St * pow(rd, 1.5);
+    if ((stoptime < timestep) && d->short_friction_time_approximation) {
+      c->d_vr[k] = vrg + stoptime * c->d_radgradp[k] / c->d_densg[k];
+      c->d_vth[k] = vtg + stoptime * c->d_azigradp[k] / c->d_densg[k];
+      c->d_l[k] = rd * c->d_vth[k];
+    } else {
+      double FgravR = -pow(rd, -2.), FgravTh = 0.;
+      if ((fabs(FgravR + vtd * vtd / rd) < 1e-15) && (!d->dust_feel_planets) && (!d->dust_feel_disk) &&
+          (!d->dust_feel_sg))
+        FgravR = -vtd * vtd / rd;
+      if (d->dust_feel_planets) {
+        for (int i = 0; i < c->sys.nb; i++) {
+          double xp = c->sys.x[i], yp = c->sys.y[i];
+          double rp = sqrt(xp * xp + yp * yp), tp = atan2(yp, xp), mp = c->sys.mass[i];
+          double eps = p->thickness_smoothing * p->aspect_ratio * pow(0.5 * rp, 1.0 + p->flaring_index) *
+                       pow(2.0, 1.0 + p->flaring_index);
+          double dd = sqrt(rd * rd + rp * rp - 2.0 * rd * rp * cos(td - tp) + eps * eps);
+          FgravR += mp * (rp * cos(td - tp) - rd) * pow(dd, -3.0);
+          FgravTh += -mp * rp * sin(td - tp) * pow(dd, -3.0);
+        }
+      }
+      if (p->indirect_term) {
+        FgravR += c->d_radind[k];
+        FgravTh += c->d_aziind[k];
+      }
+      if (p->self_gravity && d->dust_feel_sg) {
+        FgravR += c->d_radsg[k];
+        FgravTh += c->d_azisg[k];
+      }
+      double den = 1.0 + local_feel_disk * timestep / stoptime;
+      double numTh = (vtd - vtg) + timestep * (FgravTh - vrd * vtd / rd);
+      c->d_vth[k] = vtg + numTh / den;
+      double numR = (vrd - vrg) + timestep * (FgravR + (c->d_vth[k] * c->d_vth[k]) / rd);
+      c->d_vr[k] = vrg + numR / den;
+    }
+  }
+}
+
+/* RotateDsys (src/Dsys.c:1153-1169) */
+static void dust_rotate(or_ctx *c, double angle) {
+  for (int k = 0; k < c->np; k++) c->d_th[k] = dust_wrap(c, c->d_th[k] - angle);
+}
+
+void or_dust_get(or_ctx *c, double *r, double *th, double *vr, double *vth, double *l, double *stokes) {
+  size_t n = sizeof(double) * (size_t)c->np;
+  if (r) memcpy(r, c->d_r, n);
+  if (th) memcpy(th, c->d_th, n);
+  if (vr) memcpy(vr, c->d_vr, n);
+  if (vth) memcpy(vth, c->d_vth, n);
+  if (l) memcpy(l, c->d_l, n);
+  if (stokes) memcpy(stokes, c->d_stokes, n);
+}
+
+void or_dust_get_gas_at_particles(or_ctx *c, double *vrg, double *vtg, double *densg, double *csg) {
+  size_t n = sizeof(double) * (size_t)c->np;
+  if (vrg) memcpy(vrg, c->d_vrg, n);
+  if (vtg) memcpy(vtg, c->d_vtg, n);
+  if (densg) memcpy(densg, c->d_densg, n);
+  if (csg) memcpy(csg, c->d_csg, n);
+}
+
+/* the particle part of one sub-step over a frozen gas state, for pinning */
+void or_dust_step(or_ctx *c, double dt) {
+  dust_semi_update_positions(c, 2. * dt);
+  dust_update_velocities(c, dt);
+}
+
+void or_dust_get_drag(or_ctx *c, double *raddrag, double *azidrag) {
+  size_t n = sizeof(double) * (size_t)c->np;
+  if (raddrag) memcpy(raddrag, c->d_raddrag, n);
+  if (azidrag) memcpy(azidrag, c->d_azidrag, n);
+}
+
+double or_dust_min_stopping_time(or_ctx *c) { return c->min_stopping_time; }
+double or_dust_particles_mass(or_ctx *c) { return c->particles_mass; }
+
 int or_algogas(or_ctx *c, double DT) {
   const dfargo_params *p = &c->p;
   dfargo_planets *sys = &c->sys;
@@ -1554,9 +2043,11 @@ int or_algogas(or_ctx *c, double DT) {
       if (c->MassTaper < 1.0)
         for (int k = 0; k < nb; k++)
           sys->mass[k] = sys->initial_mass[k] + (sys->final_mass[k] - sys->initial_mass[k]) * c->MassTaper;
+      c->particles_mass = c->particles_mass_initial * c->MassTaper;
     } else {
       for (int k = 0; k < nb; k++)
         if (sys->acc[k] < 1e-10) sys->mass[k] = sys->final_mass[k];
+      c->particles_mass = c->particles_mass_initial;
     }
     /* n-body limit (src/SourceEuler.c:421-440) */
     if (nb > 1) {
@@ -1571,9 +2062,16 @@ int or_algogas(or_ctx *c, double DT) {
         }
     } else
       dtnbody = 100.0;
-    c->GasTimeStepsCFL = or_condition_cfl(c, DT - dtemp);
-    dthydro = (DT - dtemp) / (double)c->GasTimeStepsCFL;
-    dt = min2(dthydro, dtnbody);
+    if (!c->dust_on || c->np == 0 || c->dp.no_dt_constraint_by_particles) {
+      c->GasTimeStepsCFL = or_condition_cfl(c, DT - dtemp);
+      dthydro = (DT - dtemp) / (double)c->GasTimeStepsCFL;
+      dt = min2(dthydro, dtnbody);
+    } else {
+      /* fixed particle time step (src/SourceEuler.c:457-463) */
+      dt = c->dp.dust_timestep;
+      c->MINDT = c->dp.dust_timestep;
+      c->MAXDT = c->dp.dust_timestep;
+    }
     dtemp += dt;
     if (dt < c->MINDT) c->MINDT = dt;
     if (dt > c->MAXDT) c->MAXDT = dt;
@@ -1583,7 +2081,14 @@ int or_algogas(or_ctx *c, double DT) {
     fill_forces_arrays(c);
     advance_system_from_disk(c, dt);
     advance_system_rk5(c, dt);
+    /* particles (src/SourceEuler.c:504-513): positions over dt, then
+       velocities over dt */
+    if (c->dust_on && c->np != 0) {
+      dust_semi_update_positions(c, 2. * dt);
+      dust_update_velocities(c, dt);
+    }
     rotate_psys(c, c->OmegaFrame * dt);
+    if (c->dust_on && c->np != 0) dust_rotate(c, c->OmegaFrame * dt);
     /* crash scan (src/SourceEuler.c:545-548) */
     c->crashed_d = detect_crash(c, c->rho);
     c->crashed_e = detect_crash(c, c->en);
@@ -1660,6 +2165,10 @@ void or_get_field(or_ctx *c, int field, double *out) {
     case DFARGO_FIELD_TAURR: src = c->trr; break;
     case DFARGO_FIELD_TAURP: src = c->trp; break;
     case DFARGO_FIELD_TAUPP: src = c->tpp; break;
+    case DFARGO_FIELD_DUST_DENSITY: src = c->dustdens; break;
+    case DFARGO_FIELD_FB_ACCR: src = c->radfbacc; break;
+    case DFARGO_FIELD_FB_ACCT: src = c->azifbacc; break;
+    case DFARGO_FIELD_FB_EDOT: src = c->fbedot; break;
     default: memset(out, 0, sizeof(double) * n); return;
   }
   memcpy(out, src, sizeof(double) * n);

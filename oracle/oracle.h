@@ -66,6 +66,21 @@ void or_rk5(const double *q0, double dt, const double *masses, double *q1,
             int n, const int *feelothers, int indirect_term);
 double or_fviscosity(or_ctx *c, double r);
 
+/* dust particles */
+int or_dust_init_system(const dfargo_params *p, const dfargo_dust_params *d, const double *axi_sg_accr,
+                        double *size, double *r, double *th, double *vr, double *vth, double *l);
+/* attach after or_init_euler; runs the start-up interpolation with the
+   given time step (src/main.c:287 passes DT) */
+int or_dust_attach(or_ctx *c, const dfargo_dust_params *d, const double *size, const double *r,
+                   const double *th, const double *vr, const double *vth, const double *l, double timestep);
+void or_dust_interpolation(or_ctx *c, double timestep);
+void or_dust_get(or_ctx *c, double *r, double *th, double *vr, double *vth, double *l, double *stokes);
+void or_dust_get_gas_at_particles(or_ctx *c, double *vrg, double *vtg, double *densg, double *csg);
+void or_dust_step(or_ctx *c, double dt);
+void or_dust_get_drag(or_ctx *c, double *raddrag, double *azidrag);
+double or_dust_min_stopping_time(or_ctx *c);
+double or_dust_particles_mass(or_ctx *c);
+
 #ifdef __cplusplus
 }
 #endif

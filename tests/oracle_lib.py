@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from dustyfargoadsg_amd._capi import Params, Planets, Status, FIELDS
+from dustyfargoadsg_amd._capi import Params, Planets, Status, FIELDS, DustParams
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
@@ -53,6 +53,16 @@ def lib():
             "or_fft": (None, [dp, dp, ctypes.c_int, ctypes.c_int]),
             "or_rk5": (None, [dp, ctypes.c_double, dp, dp, ctypes.c_int, ip, ctypes.c_int]),
             "or_fviscosity": (ctypes.c_double, [vp, ctypes.c_double]),
+            "or_dust_init_system": (ctypes.c_int, [P, ctypes.POINTER(DustParams), dp, dp, dp, dp, dp, dp, dp]),
+            "or_dust_attach": (ctypes.c_int, [vp, ctypes.POINTER(DustParams), dp, dp, dp, dp, dp, dp,
+                                              ctypes.c_double]),
+            "or_dust_interpolation": (None, [vp, ctypes.c_double]),
+            "or_dust_get": (None, [vp, dp, dp, dp, dp, dp, dp]),
+            "or_dust_get_gas_at_particles": (None, [vp, dp, dp, dp, dp]),
+            "or_dust_get_drag": (None, [vp, dp, dp]),
+            "or_dust_step": (None, [vp, ctypes.c_double]),
+            "or_dust_min_stopping_time": (ctypes.c_double, [vp]),
+            "or_dust_particles_mass": (ctypes.c_double, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -152,3 +162,49 @@ class Oracle:
         f = self.L.or_sg_accel_direct if direct else self.L.or_sg_accel
         f(self.ctx, ptr(np.ascontiguousarray(rho, dtype=np.float64)), ptr(ar), ptr(at))
         return ar, at
+
+    # ---- dust particles ----
+    def dust_init(self, dp, axi_sg_accr=None):
+        """InitDustSystem restated: dict of particle arrays."""
+        n = dp.nbpart
+        a = {k: np.zeros(n + 1) for k in ("dustsize", "r", "th", "vr", "vth", "l")}
+        rc = self.L.or_dust_init_system(ctypes.byref(self.p), ctypes.byref(dp), ptr(axi_sg_accr),
+                                        ptr(a["dustsize"]), ptr(a["r"]), ptr(a["th"]), ptr(a["vr"]),
+                                        ptr(a["vth"]), ptr(a["l"]))
+        assert rc == 0
+        return {k: v[:n].copy() for k, v in a.items()}
+
+    def dust_attach(self, dp, parts, timestep):
+        self.dp = dp.copy()
+        c = {k: np.ascontiguousarray(parts[k], dtype=np.float64) for k in ("dustsize", "r", "th", "vr", "vth", "l")}
+        self.L.or_dust_attach(self.ctx, ctypes.byref(self.dp), ptr(c["dustsize"]), ptr(c["r"]), ptr(c["th"]),
+                              ptr(c["vr"]), ptr(c["vth"]), ptr(c["l"]), timestep)
+
+    def dust_state(self):
+        n = self.dp.nbpart
+        a = {k: np.zeros(n) for k in ("r", "th", "vr", "vth", "l", "stokesnb")}
+        self.L.or_dust_get(self.ctx, ptr(a["r"]), ptr(a["th"]), ptr(a["vr"]), ptr(a["vth"]), ptr(a["l"]),
+                           ptr(a["stokesnb"]))
+        return a
+
+    def dust_gas_at_particles(self):
+        n = self.dp.nbpart
+        a = {k: np.zeros(n) for k in ("vrg", "vtg", "densg", "csg")}
+        self.L.or_dust_get_gas_at_particles(self.ctx, ptr(a["vrg"]), ptr(a["vtg"]), ptr(a["densg"]), ptr(a["csg"]))
+        return a
+
+    def dust_interpolation(self, timestep):
+        self.L.or_dust_interpolation(self.ctx, timestep)
+
+    def dust_min_stopping_time(self):
+        return self.L.or_dust_min_stopping_time(self.ctx)
+
+    def dust_drag(self):
+        n = self.dp.nbpart
+        r, t = np.zeros(n), np.zeros(n)
+        self.L.or_dust_get_drag(self.ctx, ptr(r), ptr(t))
+        return r, t
+
+    def dust_step(self, dt):
+        """particle part of a sub-step over the frozen gas state"""
+        self.L.or_dust_step(self.ctx, dt)

@@ -130,3 +130,82 @@ def test_create_fails_loudly_without_device(lib):
         return
     d.close()
     pytest.skip("a HIP device is visible; covered by the gpu tests")
+
+
+def test_dust_struct_layout_and_defaults(lib, tmp_path):
+    src = tmp_path / "dlayout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "dfargo.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu\\n\", sizeof(dfargo_dust_params),"
+        " sizeof(dfargo_dust_state), offsetof(dfargo_dust_params, seed),"
+        " offsetof(dfargo_dust_params, dust_feel_sg_zero_mode));return 0;}\n")
+    exe = tmp_path / "dlayout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    want = [ctypes.sizeof(_capi.DustParams), ctypes.sizeof(_capi.DustState),
+            _capi.DustParams.seed.offset, _capi.DustParams.dust_feel_sg_zero_mode.offset]
+    assert got == want
+    d = _capi.DustParams()
+    lib.dfargo_dust_default_params(ctypes.byref(d), 123)
+    q = _capi.dust_default_params(123)
+    for name, _ in _capi.DustParams._fields_:
+        assert getattr(d, name) == getattr(q, name), name
+
+
+@pytest.mark.parametrize("case", [
+    dict(sizepartslope=3.5, dustslope=0.5),
+    dict(sizepartslope=1.0, dustslope=1.0),
+    dict(sizeminpart=0.01, sizemaxpart=0.01, sizepartslope=0.0, rmindust=0.0, rmaxdust=0.0),
+])
+def test_host_dust_init_matches_oracle(lib, oracle_built, case):
+    """InitDustSystem on the host (sizes, radii, azimuths from one drand48
+    stream, Keplerian vtheta) is bit-identical to the oracle's."""
+    from dustyfargoadsg_amd.disc import init_dust_system
+    p, _ = gas_planet(nrad=32, nsec=96)
+    kw = dict(sizeminpart=1e-3, sizemaxpart=1e-1, rmindust=0.6, rmaxdust=2.2, particle_mass=1e-9, seed=7)
+    kw.update(case)
+    d = _capi.dust_default_params(2000, **kw)
+    o = Oracle(p)
+    want = o.dust_init(d)
+    got = init_dust_system(p, d, None, lib)
+    for k in ("dustsize", "r", "th", "vr", "vth", "l"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(got["dustsize_init"], want["dustsize"])
+
+
+def test_host_dust_init_with_selfgravity_matches_oracle(lib, oracle_built):
+    """With self-gravity vtheta takes the axisymmetric SG acceleration
+    interpolated at the particle (src/Dsys.c:203-212)."""
+    from dustyfargoadsg_amd.disc import init_dust_system
+    p, _ = energy_sg(nrad=32, nsec=96)
+    axi = -1e-3 * np.linspace(1.0, 2.0, p.nrad)
+    d = _capi.dust_default_params(500, sizeminpart=1e-3, sizemaxpart=1e-2, seed=3, particle_mass=1e-9)
+    want = Oracle(p).dust_init(d, axi)
+    got = init_dust_system(p, d, axi, lib)
+    for k in ("r", "th", "vth", "l"):
+        assert np.array_equal(got[k], want[k]), k
+    assert not np.array_equal(got["vth"], got["r"] ** -0.5)
+
+
+def test_dust_parameter_checks(lib):
+    p, _ = gas_planet(nrad=16, nsec=48)
+
+    def check(**kw):
+        base = dict(sizepartslope=0.0)
+        base.update(kw)
+        d = _capi.dust_default_params(10, **base)
+        return lib.dfargo_check_dust_params(ctypes.byref(p), ctypes.byref(d))
+    assert check() == _capi.OK
+    # the defaults of src/var.c:136-138 themselves are inconsistent, and the
+    # reference refuses them too (src/Dsys.c:151-155)
+    assert check(sizepartslope=3.0) == _capi.ERR_ARG
+    assert check(interpolation=3) == _capi.ERR_ARG
+    assert check(rhopart=0.0) == _capi.ERR_ARG
+    assert check(no_dt_constraint_by_particles=0, dust_timestep=0.0) == _capi.ERR_ARG
+    for k in ("dust_feel_turb", "dust_growth", "remove_dust_from_hill", "zz_integrator",
+              "dust_feel_sg_zero_mode"):
+        assert check(**{k: 1}) == _capi.ERR_UNSUPPORTED, k
+    q = p.copy()
+    q.omega_frame = 1.0
+    d = _capi.dust_default_params(10, sizepartslope=0.0)
+    assert lib.dfargo_check_dust_params(ctypes.byref(q), ctypes.byref(d)) == _capi.ERR_UNSUPPORTED
