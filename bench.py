@@ -6,7 +6,9 @@ A step is one hydro sub-step of AlgoGas (one pass of the reference's
 per second with the state resident in HBM.  The default workload is the
 adiabatic self-gravitating disc at 512 x 1536 (energy equation, beta cooling
 10, alpha 1e-3, SG by 1024 x 1536 FFT), timed with FP32 storage (the
-headline `value`) and with FP64 storage beside it.
+headline `value`) and with FP64 storage beside it.  `--workload gas_dust`
+runs the 1024 x 3072 gas disc with 10^6 Lagrangian particles (drag
+feedback on), whose sub-step also moves and deposits the particles.
 
 Prints one JSON line.  Per-kernel times come from HIP events on the
 library's stream in a second pass over the same sub-steps; the roofline
@@ -51,6 +53,37 @@ def kernel_bytes_per_cell(name, w, energy, sg):
     return table.get(name, 0)
 
 
+def kernel_bytes_per_launch(name, w, energy, sg, cells, npart):
+    """Compulsory HBM bytes of one launch: the per-cell figure times the
+    cells, and for the particle kernels the particle arrays each must read
+    and write once (FP64 particles, int32 cells and buckets)."""
+    if name.startswith("dust_"):
+        per_particle = {
+            # r, th, vr, vth, size in; r, th, vr, vth, l, St, drag x2, t_s,
+            # |dv|^2, the CIC/TSC weights (12-18 doubles) out; cell and slot
+            "dust_move": 5 * 8 + 10 * 8 + 12 * 8 + 4 + 4,
+            "dust_scan": 0,
+            "dust_scatter": 4 + 4 + 4,             # cell, slot in; bucket entry out
+            "dust_sort": 0,
+            # bucket entry, r, th, cell, drag x2, four weights per stencil (+ t_s,
+            # |dv|^2 with the energy equation)
+            "dust_deposit": 8 + 4 + 4 + 16 + 8 * 8 + (16 if energy else 0),
+            # bucket entry in; seven FP64 fields and the id read and written
+            "dust_reorder": 8 + 2 * (7 * 8 + 4),
+        }[name]
+        per_cell = {
+            "dust_move": 0,
+            "dust_scan": 4 + 4,                    # counts in, ring-local starts out
+            "dust_scatter": 4,                     # ring-local start of the particle's cell
+            "dust_sort": 4 + 4 + 4,                # local start, count in; start out
+            # starts, counts, rho in; two feedback grids (+ FBedot) out
+            "dust_deposit": 8 + w + 2 * w + (w if energy else 0),
+            "dust_reorder": 0,
+        }[name]
+        return per_particle * npart + per_cell * cells
+    return kernel_bytes_per_cell(name, w, energy, sg) * cells
+
+
 def step_bytes_per_cell(w, energy, sg):
     """Algorithmic bytes per cell per sub-step: each state field read and
     written once, plus the two kernel spectra of self-gravity (2 x 2Nr x
@@ -59,10 +92,12 @@ def step_bytes_per_cell(w, energy, sg):
     return 2 * ns * w + (4 * w if sg else 0)
 
 
-def run_gpu(params, planets, precision, steps, warmup, prof_steps):
-    from dustyfargoadsg_amd.disc import Disc
+def run_gpu(params, planets, precision, steps, warmup, prof_steps, dust=None):
+    from dustyfargoadsg_amd.disc import Disc, init_dust_system
     d = Disc(params, precision)
     d.initial_state(planets)
+    if dust is not None:
+        d.attach_dust(dust, init_dust_system(params, dust), 2 * math.pi / 20)
     DT = 1.0e3  # an AlgoGas call much longer than the CFL step
     d.run_substeps(DT, warmup)
     t0 = time.perf_counter()
@@ -78,12 +113,15 @@ def run_gpu(params, planets, precision, steps, warmup, prof_steps):
     return t1 - t0, kt
 
 
-def cpu_baseline(params, planets, budget_s):
+def cpu_baseline(params, planets, budget_s, dust=None):
     """The FP64 oracle (a single-threaded C restatement of the sub-step) on
     the host, one sub-step per AlgoGas call, for about budget_s seconds."""
     from tests.oracle_lib import Oracle
     o = Oracle(params)
     o.initial_state(planets)
+    if dust is not None:
+        from dustyfargoadsg_amd.disc import init_dust_system
+        o.dust_attach(dust, init_dust_system(params, dust), 2 * math.pi / 20)
     n, t0 = 0, time.perf_counter()
     while True:
         o.algogas(1e-7)  # shorter than the CFL step: one sub-step per call
@@ -117,11 +155,14 @@ def main():
     if args.nsec:
         kw["nsec"] = args.nsec
     params, planets = WORKLOADS[args.workload](**kw)
+    from dustyfargoadsg_amd.workloads import DUST_WORKLOADS
+    dust = DUST_WORKLOADS[args.workload](params) if args.workload in DUST_WORKLOADS else None
+    npart = dust.nbpart if dust is not None else 0
     cells = params.nrad * params.nsec
     prof_steps = min(args.steps, 60)
     res = {}
     for prec, name in ((_capi.FP64, "fp64"), (_capi.FP32, "fp32")):
-        el, kt = run_gpu(params, planets, prec, args.steps, args.warmup, prof_steps)
+        el, kt = run_gpu(params, planets, prec, args.steps, args.warmup, prof_steps, dust)
         res[name] = dict(elapsed=el, kt=kt)
     w32, w64 = 4, 8
     energy, sg = bool(params.energy_equation), bool(params.self_gravity)
@@ -132,11 +173,11 @@ def main():
         ms = el / args.steps * 1e3
         # the roofline kernel: the largest share of step time among kernels
         # that move field data (the single-block control kernel moves none)
-        dom = max(((k, v) for k, v in kt.items() if kernel_bytes_per_cell(k, w, energy, sg) > 0),
-                  key=lambda kv: kv[1][0])
+        nbytes = lambda k: kernel_bytes_per_launch(k, w, energy, sg, cells, npart)
+        dom = max(((k, v) for k, v in kt.items() if nbytes(k) > 0), key=lambda kv: kv[1][0])
         kname, (tot_ms, nl) = dom
         avg_s = tot_ms / nl * 1e-3
-        kbytes = kernel_bytes_per_cell(kname, w, energy, sg) * cells
+        kbytes = nbytes(kname)
         sbytes = step_bytes_per_cell(w, energy, sg) * cells
         prof_total = sum(v[0] for v in kt.values()) / prof_steps
         kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 3), "launches": int(v[1]),
@@ -173,7 +214,8 @@ def main():
         "config": {"workload": f"{args.workload} {params.nrad}x{params.nsec}",
                    "nrad": params.nrad, "nsec": params.nsec,
                    "energy_equation": energy, "self_gravity": sg,
-                   "alpha_viscosity": params.alpha_viscosity, "planets": len(planets["dist"])},
+                   "alpha_viscosity": params.alpha_viscosity, "planets": len(planets["dist"]),
+                   "particles": npart},
         "roofline": {"bound": "hbm", "kernel": k32, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kb32, "avg_launch_us": a32 * 1e6},
@@ -188,7 +230,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        cv, n, el = cpu_baseline(params, planets, args.cpu_seconds)
+        cv, n, el = cpu_baseline(params, planets, args.cpu_seconds, dust)
         line["cpu_baseline"] = {"value": cv, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                                 "sample": f"{n} sub-steps of the same {params.nrad}x{params.nsec} "
                                           f"disc by the FP64 C oracle, {el:.1f} s, 1 thread"}

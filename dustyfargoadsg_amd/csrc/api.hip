@@ -187,7 +187,8 @@ static void build_sequence(dfargo_ctx *c) {
       c->seq.push_back(id);
   for (int id : {KID_CFL, KID_CONTROL}) c->seq.push_back(id);
   if (c->dust_on)
-    for (int id : {KID_DUST_MOVE, KID_DUST_COUNT, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_DEPOSIT})
+    for (int id : {KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT,
+                   KID_DUST_REORDER})
       c->seq.push_back(id);
   c->seq.push_back(KID_SOURCE);
   if (c->p.energy_equation) c->seq.push_back(KID_ENERGY);
@@ -848,6 +849,20 @@ static int get_doubles(dfargo_ctx *c, double *dst, const double *src, size_t n) 
   return DFARGO_OK;
 }
 
+// particle arrays of the caller (index = particle id) into a set, ids in
+// order; a NULL array leaves that field as it is
+static int put_dust_set(dfargo_ctx *c, const DustSet &P, const dfargo_dust_state *st, size_t np) {
+  int rc;
+  if ((rc = put_doubles(c, P.r, st->r, np)) || (rc = put_doubles(c, P.th, st->th, np)) ||
+      (rc = put_doubles(c, P.vr, st->vr, np)) || (rc = put_doubles(c, P.vth, st->vth, np)) ||
+      (rc = put_doubles(c, P.l, st->l, np)) || (rc = put_doubles(c, P.size, st->dustsize, np)))
+    return rc;
+  std::vector<int> ids(np);
+  for (size_t k = 0; k < np; k++) ids[k] = (int)k;
+  HIPCHK(hipMemcpy(P.id, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+  return DFARGO_OK;
+}
+
 extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, const dfargo_dust_state *st,
                                   double timestep) {
   if (!c || !d || !st) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
@@ -867,27 +882,36 @@ extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, co
   KArgs &a = c->a;
   const dfargo_tables &t = c->tab;
   const size_t np = (size_t)n, ncell = (size_t)nr * ns;
-  double **pd[] = {&a.p_r, &a.p_th, &a.p_vr, &a.p_vth, &a.p_l, &a.p_size, &a.p_stokes,
-                   &a.p_raddrag, &a.p_azidrag, &a.p_ts, &a.p_dv2};
+  for (int set = 0; set < 2; set++) {
+    DustSet &P = a.pset[set];
+    double **pd[] = {&P.r, &P.th, &P.vr, &P.vth, &P.l, &P.size, &P.stokes};
+    for (double **q : pd)
+      if ((rc = dalloc(c, (void **)q, sizeof(double) * np))) return rc;
+    if ((rc = dalloc(c, (void **)&P.id, sizeof(int) * np))) return rc;
+  }
+  double **pd[] = {&a.p_raddrag, &a.p_azidrag, &a.p_ts, &a.p_dv2};
   for (double **q : pd)
     if ((rc = dalloc(c, (void **)q, sizeof(double) * np))) return rc;
-  int **pi[] = {&a.p_cell, &a.p_sorted};
+  if ((rc = dalloc(c, (void **)&a.p_sorted, sizeof(unsigned long long) * np))) return rc;
+  if ((rc = dalloc(c, (void **)&a.p_w, sizeof(double) * 18 * np)) ||
+      (rc = dalloc(c, (void **)&a.p_tsmin, sizeof(unsigned long long) * ((np + 255) / 256))))
+    return rc;
+  int **pi[] = {&a.p_cell, &a.p_slot, &a.p_base, &a.out_list};
   for (int **q : pi)
     if ((rc = dalloc(c, (void **)q, sizeof(int) * np))) return rc;
-  int **ci[] = {&a.cell_count, &a.cell_start, &a.cell_fill};
+  int **ci[] = {&a.cell_count, &a.cell_local, &a.cell_start};
   for (int **q : ci)
     if ((rc = dalloc(c, (void **)q, sizeof(int) * ncell))) return rc;
-  if ((rc = dalloc(c, (void **)&a.scan_sums, sizeof(int) * (size_t)dust_scan_blocks((long)ncell)))) return rc;
+  if ((rc = dalloc(c, (void **)&a.ring_tot, sizeof(int) * (size_t)(nr + 1))) ||
+      (rc = dalloc(c, (void **)&a.ring_off, sizeof(int) * (size_t)(nr + 1))))
+    return rc;
   void **grids[] = {&a.radgradp, &a.azigradp, &a.radindacc, &a.aziindacc, &a.radsgacc, &a.azisgacc,
                     &a.radfbacc, &a.azifbacc, &a.fbedot, &a.dustdens, &a.cs_old};
   for (void **g : grids)
     if ((rc = dalloc(c, g, c->esz * c->nfield))) return rc;
   if ((rc = upload_table(c, &a.Azimuth, t.Azimuth, (size_t)ns))) return rc;
-  // particle state
-  if ((rc = put_doubles(c, a.p_r, st->r, np)) || (rc = put_doubles(c, a.p_th, st->th, np)) ||
-      (rc = put_doubles(c, a.p_vr, st->vr, np)) || (rc = put_doubles(c, a.p_vth, st->vth, np)) ||
-      (rc = put_doubles(c, a.p_l, st->l, np)) || (rc = put_doubles(c, a.p_size, st->dustsize, np)))
-    return rc;
+  // particle state into set 0, in the caller's order
+  if ((rc = put_dust_set(c, a.pset[0], st, np))) return rc;
   c->dustsize_init.assign(st->dustsize_init ? st->dustsize_init : st->dustsize,
                           (st->dustsize_init ? st->dustsize_init : st->dustsize) + np);
   // switches and constants (ComputeCodeUnits, src/LowTasks.c:204-212)
@@ -903,8 +927,9 @@ extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, co
   a.log_grid = p.log_grid;
   a.d_dust_timestep = d->dust_timestep;
   a.d_unit_mass = 1.9891e30 * d->factor_unit_mass;
-  a.d_unit_length = 1.49598e11 * d->factor_unit_length;
-  a.d_rho_code = d->rhopart * 1000.0 * pow(a.d_unit_length, 3.) / a.d_unit_mass;
+  const double unit_length = 1.49598e11 * d->factor_unit_length;
+  a.d_len_m2 = pow(unit_length, -2.0);
+  a.d_rho_code = d->rhopart * 1000.0 * pow(unit_length, 3.) / a.d_unit_mass;
   const double half = 0.5 * (p.pmax - p.pmin) / (double)ns;
   a.aziinf0 = t.Azimuth[0] - half;
   a.azisup_last = t.Azimuth[ns - 1] + half;
@@ -912,13 +937,14 @@ extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, co
   a.delta_r = p.log_grid ? log(t.Rsup[0] / t.Rinf[0]) : t.Rsup[0] - t.Rinf[0];
   a.rmin = p.rmin;
   a.rmax = p.rmax;
+  a.d_ring_step = p.log_grid ? log(t.Rinf[1] / t.Rinf[0]) : t.Rinf[1] - t.Rinf[0];
   // control block: super-particle mass and a fresh minimum stopping time
   if ((rc = ctrl_pull(c))) return rc;
   c->h_ctrl->particles_mass = d->particle_mass;
   c->h_ctrl->particles_mass0 = d->particle_mass;
   c->h_ctrl->dust_mp = d->particle_mass;
-  double big = 1e6;
-  memcpy(&c->h_ctrl->min_ts_bits, &big, sizeof big);
+  c->h_ctrl->pset = 0;
+  c->h_ctrl->out_count = 0;
   if ((rc = ctrl_push(c))) return rc;
   // start-up interpolation (src/main.c:287)
   if (c->prec == DFARGO_FP64) launch_dust_startup<double>(a, timestep, c->stream);
@@ -944,14 +970,21 @@ extern "C" int dfargo_download_dust(dfargo_ctx *c, dfargo_dust_state *st) {
   if (!c || !st) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
   if (!c->dust_on) return dfargo_set_error(DFARGO_ERR_STATE, "no particles are attached to this context");
   HIPCHK(hipSetDevice(c->dev));
-  const KArgs &a = c->a;
-  const size_t np = (size_t)a.d_np;
-  int rc;
-  if ((rc = get_doubles(c, st->r, a.p_r, np)) || (rc = get_doubles(c, st->th, a.p_th, np)) ||
-      (rc = get_doubles(c, st->vr, a.p_vr, np)) || (rc = get_doubles(c, st->vth, a.p_vth, np)) ||
-      (rc = get_doubles(c, st->l, a.p_l, np)) || (rc = get_doubles(c, st->dustsize, a.p_size, np)) ||
-      (rc = get_doubles(c, st->stokesnb, a.p_stokes, np)))
-    return rc;
+  int rc = ctrl_pull(c);
+  if (rc) return rc;
+  const DustSet &P = c->a.pset[c->h_ctrl->pset];
+  const size_t np = (size_t)c->a.d_np;
+  // the device holds the particles sorted by cell: back to the caller's order
+  std::vector<int> ids(np);
+  HIPCHK(hipMemcpy(ids.data(), P.id, sizeof(int) * np, hipMemcpyDeviceToHost));
+  std::vector<double> buf(np);
+  const double *src[] = {P.r, P.th, P.vr, P.vth, P.l, P.size, P.stokes};
+  double *dst[] = {st->r, st->th, st->vr, st->vth, st->l, st->dustsize, st->stokesnb};
+  for (int f = 0; f < 7; f++) {
+    if (!dst[f]) continue;
+    if ((rc = get_doubles(c, buf.data(), src[f], np))) return rc;
+    for (size_t s = 0; s < np; s++) dst[f][ids[s]] = buf[s];
+  }
   if (st->dustsize_init) memcpy(st->dustsize_init, c->dustsize_init.data(), sizeof(double) * np);
   return DFARGO_OK;
 }
@@ -960,13 +993,24 @@ extern "C" int dfargo_upload_dust(dfargo_ctx *c, const dfargo_dust_state *st) {
   if (!c || !st) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
   if (!c->dust_on) return dfargo_set_error(DFARGO_ERR_STATE, "no particles are attached to this context");
   HIPCHK(hipSetDevice(c->dev));
-  const KArgs &a = c->a;
-  const size_t np = (size_t)a.d_np;
-  int rc;
-  if ((rc = put_doubles(c, a.p_r, st->r, np)) || (rc = put_doubles(c, a.p_th, st->th, np)) ||
-      (rc = put_doubles(c, a.p_vr, st->vr, np)) || (rc = put_doubles(c, a.p_vth, st->vth, np)) ||
-      (rc = put_doubles(c, a.p_l, st->l, np)) || (rc = put_doubles(c, a.p_size, st->dustsize, np)))
-    return rc;
+  const size_t np = (size_t)c->a.d_np;
+  // the current state in the caller's order, overlaid with what is given,
+  // written back in that order (the Stokes numbers of the last
+  // interpolation follow their particles)
+  dfargo_dust_state cur;
+  std::vector<double> r(np), th(np), vr(np), vth(np), l(np), sz(np), stk(np);
+  cur.r = r.data(); cur.th = th.data(); cur.vr = vr.data(); cur.vth = vth.data(); cur.l = l.data();
+  cur.dustsize = sz.data(); cur.stokesnb = stk.data(); cur.dustsize_init = nullptr;
+  int rc = dfargo_download_dust(c, &cur);
+  if (rc) return rc;
+  if (st->r) r.assign(st->r, st->r + np);
+  if (st->th) th.assign(st->th, st->th + np);
+  if (st->vr) vr.assign(st->vr, st->vr + np);
+  if (st->vth) vth.assign(st->vth, st->vth + np);
+  if (st->l) l.assign(st->l, st->l + np);
+  if (st->dustsize) sz.assign(st->dustsize, st->dustsize + np);
+  const DustSet &P = c->a.pset[c->h_ctrl->pset];
+  if ((rc = put_dust_set(c, P, &cur, np)) || (rc = put_doubles(c, P.stokes, stk.data(), np))) return rc;
   if (st->dustsize_init) c->dustsize_init.assign(st->dustsize_init, st->dustsize_init + np);
   return DFARGO_OK;
 }

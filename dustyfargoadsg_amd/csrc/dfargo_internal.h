@@ -69,15 +69,15 @@ typedef struct dfargo_ctrl {
   int ntarget;
   int go, active, gas_timesteps_cfl, substeps, crashed_d, crashed_e;
   long substeps_total;
-  /* dust particles: super-particle mass (tapered like the planets) and
-     the minimum stopping time of the last interpolation, as the bits of a
-     positive double so that an integer atomicMin orders it */
+  /* dust particles: super-particle mass (tapered like the planets) */
   double particles_mass, particles_mass0;
-  unsigned long long min_ts_bits;
   /* the super-particle and planet masses of the current sub-step, kept by
      the control kernel for the particle kernels that run after it (it
      already tapers the masses for the next sub-step) */
   double dust_mp, dust_pl_m[DF_MAXPL];
+  /* the particle set holding the current state, and the particles outside
+     the grid counted by this sub-step's move */
+  int pset, out_count;
 } dfargo_ctrl;
 
 #endif

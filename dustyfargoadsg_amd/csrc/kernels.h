@@ -11,6 +11,14 @@
 #define DF_CFLSECURITY 0.5           /* src/SourceEuler.c:10 */
 #define DF_CVNR 1.41                 /* src/SourceEuler.c:13 */
 
+// The particle state arrays (FP64, int ids).  Two sets: each sub-step
+// reads the particles from one, in cell order, and writes them re-sorted by
+// cell to the other; id is the particle's index in the reference's arrays.
+struct DustSet {
+  double *r, *th, *vr, *vth, *l, *size, *stokes;
+  int *id;
+};
+
 // Everything a kernel needs, passed by value.  Field pointers are untyped:
 // they hold T = double or float depending on the context's precision.
 struct KArgs {
@@ -41,12 +49,30 @@ struct KArgs {
   // with the gas (T)
   int d_on, d_np, d_interp, d_feel_disk, d_feel_sg, d_feel_planets, d_feedback, d_sfta;
   int d_fixed_dt, log_grid;
-  double d_dust_timestep, d_rho_code, d_unit_mass, d_unit_length;
+  double d_dust_timestep, d_rho_code, d_unit_mass, d_len_m2; // d_len_m2 = unit_length^-2
   double aziinf0, azisup_last, delta_r, delta_phi, rmin, rmax;
   const double *Azimuth;
-  double *p_r, *p_th, *p_vr, *p_vth, *p_l, *p_size, *p_stokes;
+  DustSet pset[2];
+  // per particle position of the current sub-step: drag, stopping time,
+  // |dv|^2 for the heating
   double *p_raddrag, *p_azidrag, *p_ts, *p_dv2;
-  int *p_cell, *cell_count, *cell_start, *cell_fill, *p_sorted, *scan_sums;
+  // per particle the separable stencil weights of its three stencils:
+  // p_w[(kind * 6 + q) * np + k], q < 3 radial, 3 + q azimuthal
+  double *p_w;
+  // buckets: particle cell and slot in it, per-cell counts and starts
+  // (ring-local, then global), per-ring totals and offsets (ring_off[nr]:
+  // the particles inside the grid); bucket entries are (id << 32 | position)
+  int *p_cell, *p_slot, *cell_count, *cell_local, *cell_start, *ring_tot, *ring_off;
+  unsigned long long *p_sorted;
+  int *out_list; // positions of the particles outside the grid
+  // first node of each stencil relative to the particle's cell, 2 + 2 bits
+  // per stencil (ring, column; offset + 1)
+  int *p_base;
+  // minimum stopping time per particle block of the last interpolation
+  // (bits of a positive double; all ones for a block without particles in
+  // the grid)
+  unsigned long long *p_tsmin;
+  double d_ring_step; // log(Rinf[1]/Rinf[0]) on a logarithmic grid, Rinf[1]-Rinf[0] otherwise
   void *radgradp, *azigradp, *radindacc, *aziindacc, *radsgacc, *azisgacc;
   void *radfbacc, *azifbacc, *fbedot, *dustdens;
   // energy equation: the sound speed the sub-step started with, kept for
@@ -61,7 +87,7 @@ struct KArgs {
 enum KernelId {
   KID_SG_ROWS_FWD = 0, KID_SG_TRANSPOSE_Z, KID_SG_COLS, KID_SG_TRANSPOSE_W, KID_SG_ROWS_INV,
   KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
-  KID_FINISH, KID_DUST_MOVE, KID_DUST_COUNT, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_DEPOSIT, KID_COUNT
+  KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 
@@ -74,7 +100,6 @@ template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t 
 // and the start-up interpolation outside sub-steps
 template <typename T> void launch_dust_kernel(int id, const KArgs &a, hipStream_t s);
 template <typename T> void launch_dust_startup(const KArgs &a, double timestep, hipStream_t s);
-int dust_scan_blocks(long ncell); // block totals of the bucket scan (scan_sums length)
 template <typename T> void launch_eos_pre(const KArgs &a, hipStream_t s);
 template <typename T> void launch_potential(const KArgs &a, hipStream_t s);
 template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s); // rows_fwd..rows_inv

@@ -15,8 +15,8 @@
 const char *kKernelNames[KID_COUNT] = {
     "sg_rows_fwd", "sg_transpose_z", "sg_cols", "sg_transpose_w", "sg_rows_inv",
     "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
-    "transport_azi", "finish",      "dust_move",      "dust_count", "dust_scan",
-    "dust_scatter",  "dust_deposit"};
+    "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
+    "dust_sort",     "dust_deposit", "dust_reorder"};
 
 // ------------------------------------------------------------------------
 // helpers
@@ -381,6 +381,8 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
         st<T>(a.cs, base + j, cs);
         css += (double)(T)cs;
       }
+      // the particle buckets of this sub-step start empty
+      if (a.d_on) a.cell_count[base + j] = 0;
       double cm = a.Surf[i] * dens;
       m.add(cm);
       xb.add(cm * (a.Rmed[i] * a.CosAz[j]));
@@ -524,6 +526,25 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
     nd = dmin2(nd, dt);
   }
   nd = block_min_fast(nd, sh);
+  // minimum stopping time of the last particle interpolation (reset to 1e6
+  // at each interpolation, src/Dsys.c:300): the minimum of the per-block
+  // minima, compared as bit patterns of positive doubles
+  double dust_min_ts = 1e6;
+  if (a.d_on && a.d_np > 0 && a.d_feedback) {
+    unsigned long long m = (unsigned long long)__double_as_longlong(1e6);
+    const int nb = (a.d_np + 255) / 256;
+    for (int b = t; b < nb; b += TB) m = (a.p_tsmin[b] < m) ? a.p_tsmin[b] : m;
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = (unsigned long long)__shfl_xor((long)m, off, 64);
+      m = (o < m) ? o : m;
+    }
+    __shared__ unsigned long long tsw[16];
+    if ((t & 63) == 0) tsw[t >> 6] = m;
+    __syncthreads();
+    m = tsw[0];
+    for (int w = 1; w < TB / 64; w++) m = (tsw[w] < m) ? tsw[w] : m;
+    dust_min_ts = __longlong_as_double((long long)m);
+  }
   // each wavefront sums whole partials: lanes stride the rings with a
   // compensated sum, then a fixed shuffle tree (deterministic)
   const int NP = DF_NPART(c->ntarget), lane = t & 63, nw = TB >> 6;
@@ -544,7 +565,7 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
     // the minimum stopping time of the last interpolation, resolved by two
     // steps (src/SourceEuler.c:1501-1507)
     if (a.d_on && a.d_np > 0 && a.d_feedback) {
-      double dt_fb = 0.5 * __longlong_as_double((long long)c->min_ts_bits);
+      double dt_fb = 0.5 * dust_min_ts;
       if (dt_fb == 0.0) dt_fb = 1e10;
       if (dt_fb < nd) nd = dt_fb;
     }
@@ -677,7 +698,7 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
     // minimum for their interpolation (src/Dsys.c:300)
     c->dust_mp = c->particles_mass;
     for (int k = 0; k < nb; k++) c->dust_pl_m[k] = c->pl_m[k];
-    c->min_ts_bits = (unsigned long long)__double_as_longlong(1e6);
+    c->out_count = 0;
   }
   // loop condition and start of the next sub-step
   c->go = c->dtemp < 0.999999999 * c->DT;
@@ -775,6 +796,8 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
   const int ns = a.ns, nr = a.nr, tid = threadIdx.x, TB = blockDim.x;
   const int j0 = blockIdx.x * SRC_TC, i0 = blockIdx.y * SRC_TR;
   const double dt = c->dt;
+  // the particles of this sub-step were re-sorted into the other set
+  if (a.d_on && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) a.ctrl->pset ^= 1;
   // columns j0-4 .. j0+SRC_TC+1 of the potential region, wrapped
   for (int p = tid; p < SRC_PC; p += TB) {
     int jj = (j0 - 4 + p) % ns;
@@ -1477,10 +1500,11 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       hipLaunchKernelGGL(k_finish<T>, cell_grid(a, a.nr + 1), cell_block(a), 0, s, a);
       break;
     case KID_DUST_MOVE:
-    case KID_DUST_COUNT:
     case KID_DUST_SCAN:
     case KID_DUST_SCATTER:
-    case KID_DUST_DEPOSIT: launch_dust_kernel<T>(id, a, s); break;
+    case KID_DUST_SORT:
+    case KID_DUST_DEPOSIT:
+    case KID_DUST_REORDER: launch_dust_kernel<T>(id, a, s); break;
     default: launch_sg_kernel<T>(id, a, s); break;
   }
 }

@@ -100,8 +100,29 @@ def dust_particles(params, nbpart, dust_to_gas=0.01, **over):
     return d
 
 
+def gas_dust(nrad=1024, nsec=3072, seed=DEFAULT_SEED, **over):
+    """Gas plus Lagrangian dust: the gas disc of gas_sg_large without
+    self-gravity (alpha 1e-3, one q=1e-3 planet at r=1, h = 0.05) at
+    1024x3072; the particles are given by gas_dust_particles."""
+    return gas_sg_large(nrad=nrad, nsec=nsec, seed=seed, self_gravity=0, **over)
+
+
+def gas_dust_particles(params, nbpart=1_000_000, seed=DEFAULT_SEED, **over):
+    """10^6 particles uniform in r over [0.5, 2.0] (DUSTSLOPE 0) and in
+    azimuth, sizes 1 um - 1 cm with dn/ds ~ s^-3.5, internal density
+    2.7 g/cc (Stokes numbers ~1e-5 - 1), CIC interpolation, drag feedback
+    on the gas, dust-to-gas mass ratio 0.01."""
+    kw = dict(sizeminpart=1e-6, sizemaxpart=1e-2, sizepartslope=3.5, rhopart=2.7, dustslope=0.0,
+              rmindust=0.5, rmaxdust=2.0, interpolation=1, dust_feedback=1, seed=seed)
+    kw.update(over)
+    return dust_particles(params, nbpart, **kw)
+
+
+DUST_WORKLOADS = {"gas_dust": gas_dust_particles}
+
 WORKLOADS = {
     "gas_planet": gas_planet,
     "energy_sg": energy_sg,
     "gas_sg_large": gas_sg_large,
+    "gas_dust": gas_dust,
 }

@@ -13,9 +13,10 @@ AlgoGas calls of the same DT, and are compared.  Bars:
   * FP32 field storage (particles stay FP64): positions <= 1e-6, particle
     velocities <= 1e-5 of max |v_theta|, Stokes numbers <= 1e-3 (they go
     through the Mach number of the dust-gas velocity lag).
-The device sums each cell's deposit in particle order, as the oracle's
-sequential loop does; the differences left are those of the device's libm
-(pow, exp, log) against glibc's and of the gas fields' reduction order.
+The device sums each cell's deposit bucket by bucket (a fixed,
+deterministic order) where the oracle's sequential loop goes by particle
+index; that, the device's libm (log, pow) against glibc's and the gas
+fields' reduction order are the only sources of difference.
 """
 import math
 
