@@ -113,6 +113,67 @@ def run_gpu(params, planets, precision, steps, warmup, prof_steps, dust=None):
     return t1 - t0, kt
 
 
+def precision_error(params, planets, orbits, control=True):
+    """FP32 against FP64 storage from the same initial disc after `orbits`
+    orbits at r = 1 (AlgoGas calls of DT = 2 pi / 20): relative L-infinity
+    and L2 errors per field (Sigma, v_theta, energy on their own scale; v_r
+    on the v_theta scale), the relative L-infinity error of the ring-mean
+    Sigma(r) and of the disc mass.  With `control`, a second FP64 run whose
+    initial Sigma is perturbed by 1e-14 (relative) gives the same figures
+    for FP64 against itself: where the flow amplifies perturbations
+    (gravitoturbulence), that is the floor any pointwise comparison has."""
+    import numpy as np
+    from dustyfargoadsg_amd import _capi
+    from dustyfargoadsg_amd.disc import Disc
+    DT = 2 * math.pi / 20
+    nr, ns = params.nrad, params.nsec
+    n = nr * ns
+
+    def run(prec, perturb):
+        d = Disc(params, prec)
+        st = d.initial_state(planets)
+        if perturb:
+            st["rho"][:n] *= 1.0 + 1e-14
+            d.set_state(dict(rho=st["rho"], vr=st["vr"], vt=st["vt"], energy=st["energy"]))
+        for c in range(int(round(orbits * 20))):
+            d.algogas(DT)
+            if c % 200 == 199:
+                print(f"precision run {'fp64' if prec == _capi.FP64 else 'fp32'}{' perturbed' if perturb else ''}:"
+                      f" {(c + 1) / 20:g} orbits", file=sys.stderr, flush=True)
+        out = d.state()
+        out["substeps"] = d.status().substeps_total
+        d.close()
+        return out
+
+    def compare(a, b):
+        def linf(k, scale=None):
+            ref = b[scale or k][:n]
+            return float(np.abs(a[k][:n] - b[k][:n]).max() / np.abs(ref).max())
+
+        def l2(k, scale=None):
+            ref = b[scale or k][:n]
+            return float(np.linalg.norm(a[k][:n] - b[k][:n]) / np.linalg.norm(ref))
+        pa, pb = a["rho"][:n].reshape(nr, ns).mean(axis=1), b["rho"][:n].reshape(nr, ns).mean(axis=1)
+        out = {"sigma_linf": linf("rho"), "sigma_l2": l2("rho"),
+               "sigma_ring_mean_linf": float(np.abs(pa - pb).max() / np.abs(pb).max()),
+               "mass": float(abs(a["rho"][:n].sum() / b["rho"][:n].sum() - 1.0)),
+               "vtheta_linf": linf("vt"), "vr_linf_on_vtheta": linf("vr", "vt"), "vr_l2_on_vtheta": l2("vr", "vt")}
+        if params.energy_equation:
+            out["energy_linf"] = linf("energy")
+            out["energy_l2"] = l2("energy")
+        out["substeps"] = [int(a["substeps"]), int(b["substeps"])]
+        qa, qb = a["planets"], b["planets"]
+        if qb.nb > 0:
+            out["planet_position"] = max(max(abs(qa.x[k] - qb.x[k]), abs(qa.y[k] - qb.y[k])) for k in range(qb.nb))
+        return out
+
+    ref = run(_capi.FP64, False)
+    res = {"orbits": orbits, "fp32_vs_fp64": compare(run(_capi.FP32, False), ref)}
+    if control:
+        res["fp64_perturbed_1e-14_vs_fp64"] = compare(run(_capi.FP64, True), ref)
+    return res
+
+
 def cpu_baseline(params, planets, budget_s, dust=None):
     """The FP64 oracle (a single-threaded C restatement of the sub-step) on
     the host, one sub-step per AlgoGas call, for about budget_s seconds."""
@@ -142,6 +203,8 @@ def main():
     ap.add_argument("--nsec", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--error-orbits", type=float, default=1.0,
+                    help="orbits over which the FP32 error against FP64 is measured (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one disc on one GPU; multi-GPU runs are not "
@@ -229,6 +292,8 @@ def main():
         "kernels_fp64": kern64,
         "cpu_baseline": None,
     }
+    if args.error_orbits > 0 and dust is None:
+        line["fp32_error_vs_fp64"] = precision_error(params, planets, args.error_orbits, control=False)
     if not args.no_cpu_baseline:
         cv, n, el = cpu_baseline(params, planets, args.cpu_seconds, dust)
         line["cpu_baseline"] = {"value": cv, "unit": "cell-updates/s", "cores": 1, "kind": "port",

@@ -28,6 +28,54 @@ template <typename T> __device__ __forceinline__ double ld(const void *p, long i
 template <typename T> __device__ __forceinline__ void st(void *p, long i, double v) {
   ((T *)p)[i] = (T)v;
 }
+// Unbiased stochastic rounding of the state to FP32 storage.  A value
+// between two floats is stored as the upper one with probability equal to
+// its distance from the lower one, in ulps; the draw is a hash of the cell,
+// the sub-step count and a per-field salt, so the result is deterministic
+// (a graph replay and an eager run agree).  Round-to-nearest would drop the
+// same sub-ulp increment of a slowly evolving field every sub-step and let
+// the error grow linearly; with unbiased rounding it grows as a random
+// walk.  FP64 storage stores the value as it is.
+__device__ __forceinline__ unsigned mix32(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ unsigned sr_key(const dfargo_ctrl *c, long l, unsigned salt) {
+  return mix32((unsigned)l * 0x9E3779B9U ^ mix32((unsigned)c->substeps_total * 0x85EBCA6BU + salt));
+}
+template <typename T> __device__ __forceinline__ T sr_round(double v, unsigned key);
+template <> __device__ __forceinline__ double sr_round<double>(double v, unsigned) { return v; }
+template <> __device__ __forceinline__ float sr_round<float>(double v, unsigned key) {
+  const float f = (float)v;
+  const double df = (double)f;
+  if (df == v || !(fabs(v) < 3.0e38)) return f;
+  const unsigned u = __float_as_uint(f);
+  float lo, hi;
+  if (df < v) {
+    lo = f;
+    hi = (f == 0.0f) ? __uint_as_float(1u) : __uint_as_float(f > 0.0f ? u + 1u : u - 1u);
+  } else {
+    hi = f;
+    lo = (f == 0.0f) ? -__uint_as_float(1u) : __uint_as_float(f > 0.0f ? u - 1u : u + 1u);
+  }
+  const double p_up = (v - (double)lo) / ((double)hi - (double)lo);
+  const double draw = (double)(mix32(key) >> 8) * (1.0 / 16777216.0);
+  return (draw < p_up) ? hi : lo;
+}
+template <typename T> __device__ __forceinline__ void st_sr(void *p, long i, double v, unsigned key) {
+  ((T *)p)[i] = sr_round<T>(v, key);
+}
+// salts of the stochastically rounded stores
+enum { SR_VR = 1, SR_VT, SR_E, SR_EN, SR_RAD = 8, SR_AZI = 16 };
+
+// arithmetic type of the transport stencils for a storage type
+template <typename T> struct TransportArith { using type = T; };
+template <> struct TransportArith<float> { using type = double; };
+
 // a / b in the arithmetic of the storage type: the IEEE quotient for
 // double; for float the hardware reciprocal (1 ulp) times a
 __device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
@@ -314,14 +362,16 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
   if (!c->go) return;
   const int i = ring_of_block(a), ns = a.ns, nr = a.nr, t = threadIdx.x, TB = blockDim.x;
   const long base = (long)i * ns;
-  // the CFL arithmetic is that of the storage type R (the ring mean is FP64)
-  using R = T;
-  const R *vt = (const R *)a.vt + base, *vr = (const R *)a.vr + base;
+  // the CFL arithmetic is FP64 whatever the storage type: the sub-step
+  // count ceil(DT / dt_min) then follows the FP64 run's, and FP32 storage
+  // changes the solution, not the time discretisation
+  using R = double;
+  const T *vt = (const T *)a.vt + base, *vr = (const T *)a.vr + base;
   R vt_own[K];
   double s = 0.0;
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    vt_own[k] = vt[t + k * TB];
+    vt_own[k] = (R)vt[t + k * TB];
     s += (double)vt_own[k];
   }
   s = block_sum_fast(s, sh);
@@ -337,14 +387,14 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int j = t + k * TB, jp = (j == ns - 1) ? 0 : j + 1;
-      const R cs = a.energy ? ((const R *)a.cs)[base + j] : (R)a.cs_iso[i];
+      const R cs = a.energy ? (R)((const T *)a.cs)[base + j] : (R)a.cs_iso[i];
       const R vres = a.fast ? vt_own[k] - vm : vt_own[k];
-      const R vrl = vr[j];
+      const R vrl = (R)vr[j];
       const R invdt1 = qdiv(cs, dxmin);
       const R invdt2 = qdiv(fabs(vrl), dxrad);
       const R invdt3 = qdiv(fabs(vres), dxtheta);
-      R dvr = vr[j + ns] - vrl;
-      R dvt = vt[jp] - vt_own[k];
+      R dvr = (R)vr[j + ns] - vrl;
+      R dvt = (R)vt[jp] - vt_own[k];
       if (dvr >= R(0)) dvr = R(1e-10); else dvr = -dvr;
       if (dvt >= R(0)) dvt = R(1e-10); else dvt = -dvt;
       const R q4 = qdiv(dvr, dxrad), q5 = qdiv(dvt, dxtheta);
@@ -788,7 +838,9 @@ template <typename T> __device__ __forceinline__ double src_press(const KArgs &a
 
 template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
   __shared__ double sPot[SRC_PR][SRC_PC];
-  __shared__ T sVr[SRC_VR][SRC_VC], sVt[SRC_VR][SRC_VC];
+  // VradInt / VthetaInt in FP64 whatever the storage type: their sub-step
+  // increments are kept in full until the final, unbiased rounding
+  __shared__ double sVr[SRC_VR][SRC_VC], sVt[SRC_VR][SRC_VC];
   __shared__ T sTrr[SRC_SR][SRC_SC], sTpp[SRC_SR][SRC_SC], sTrp[SRC_SR][SRC_SC];
   __shared__ int sCol[SRC_PC];
   const dfargo_ctrl *c = a.ctrl;
@@ -815,8 +867,8 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
   for (int idx = tid; idx < SRC_VR * SRC_VC; idx += TB) {
     const int q = idx / SRC_VC, p = idx - q * SRC_VC, i = i0 - 3 + q, j = sCol[p + 1];
     if (i < 0 || i > nr - 1) {
-      sVr[q][p] = (T)0;
-      sVt[q][p] = (T)0;
+      sVr[q][p] = 0.0;
+      sVt[q][p] = 0.0;
       continue;
     }
     const long l = (long)i * ns + j;
@@ -834,8 +886,8 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
       double vt2 = ld<T>(a.vt, l) + ld<T>(a.vt, ljp) + ld<T>(a.vt, lim) + ld<T>(a.vt, ljp - ns);
       vt2 = vt2 / 4.0 + a.Rinf[i] * c->omega_frame;
       vt2 = vt2 * vt2;
-      vri = (double)(T)(ld<T>(a.vr, l) + dt * (-gradp - gradphi + vt2 * a.InvRinf[i]));
-      if (a.d_on && a.d_feedback) vri = (double)(T)(vri + dt * ld<T>(a.radfbacc, l));
+      vri = ld<T>(a.vr, l) + dt * (-gradp - gradphi + vt2 * a.InvRinf[i]);
+      if (a.d_on && a.d_feedback) vri += dt * ld<T>(a.radfbacc, l);
       if (dgrid) {
         st<T>(a.radgradp, l, (pl - plim) * a.InvDiffRmed[i]);
         if (a.indirect_term)
@@ -864,11 +916,10 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
     if (a.sg) {
       const double azisgacc = 0.5 * (ld<T>(a.sgt, l) + ld<T>(a.sgt, ljm));
       if (dgrid) st<T>(a.azisgacc, l, azisgacc);
-      vti = (double)(T)vti;
       vti = vti + dt * azisgacc;
     }
-    sVr[q][p] = (T)vri;
-    sVt[q][p] = (T)vti;
+    sVr[q][p] = vri;
+    sVt[q][p] = vti;
   }
   __syncthreads();
   if (a.visc_on) {
@@ -917,7 +968,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
              ((a.Rsup[i] * trpp - a.Rinf[i] * trpl) * a.InvDiffRsup[i] +
               ((double)sTpp[qs][ps] - (double)sTpp[qs][ps - 1]) * a.invdphi + 0.5 * (trpl + trpp)) /
              (0.5 * (rho + ld<T>(a.rho, ljm)));
-        sVt[qv][pv] = (T)v;
+        sVt[qv][pv] = v;
       }
     }
     for (int idx = tid; idx < (SRC_TR + 2) * SRC_TC; idx += TB) {
@@ -932,7 +983,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
               ((double)sTrp[qs][ps + 1] - (double)sTrp[qs][ps]) * a.invdphi -
               0.5 * ((double)sTpp[qs][ps] + (double)sTpp[qs - 1][ps])) /
              (0.5 * (rho + ld<T>(a.rho, lim)));
-        sVr[qv][pv] = (T)v;
+        sVr[qv][pv] = v;
       }
     }
     __syncthreads();
@@ -963,12 +1014,12 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
       vrn = vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * a.InvDiffRmed[i];
     }
     const double dxtheta = a.pm / (double)ns * a.Rmed[i], invdxtheta = 1.0 / dxtheta;
-    st<T>(a.vri, l, vrn);
-    st<T>(a.vti, l, vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta);
+    st_sr<T>(a.vri, l, vrn, sr_key(c, l, SR_VR));
+    st_sr<T>(a.vti, l, vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta, sr_key(c, l, SR_VT));
     if (a.energy) {
       double e = ld<T>(a.en, l);
       e = e - dt * qr * (vrp - vrl) * a.InvDiffRsup[i] - dt * qt * (vtp - vtl) * invdxtheta;
-      st<T>(a.e_int, l, e);
+      st_sr<T>(a.e_int, l, e, sr_key(c, l, SR_E));
     }
     if (i == nr - 1) {
       // vr and vt of ring nrad are zero after ActualiseGas
@@ -1060,7 +1111,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
     for (int k = 0; k < K; k++) enew[k] += ld<T>(a.fbedot, base + t + k * TB) * dt;
   }
 #pragma unroll
-  for (int k = 0; k < K; k++) st<T>(a.en, base + t + k * TB, enew[k]);
+  for (int k = 0; k < K; k++) st_sr<T>(a.en, base + t + k * TB, enew[k], sr_key(c, base + t + k * TB, SR_EN));
 }
 
 // ------------------------------------------------------------------------
@@ -1095,23 +1146,23 @@ __device__ __forceinline__ R rad_star(const KArgs &a, int r, R v, R dt, R wm, R 
   return w0 - ((R)a.Rmed[r + 1] - (R)a.Rmed[r] + v * dt) * R(0.5) * dq0;
 }
 // the six transported quantities of cell (r, j) before the pass
-template <typename R>
+template <typename T, typename R>
 __device__ __forceinline__ void rad_quantities(const KArgs &a, int r, int j, R Of, R (&Q)[6]) {
   const int ns = a.ns;
   const int jp = (j == ns - 1) ? 0 : j + 1;
-  const R *rho = (const R *)a.rho + (long)r * ns, *vr = (const R *)a.vri + (long)r * ns;
-  const R *vt = (const R *)a.vti + (long)r * ns;
-  const R d = rho[j], Rm = (R)a.Rmed[r];
-  Q[0] = d * vr[j + ns];
-  Q[1] = d * vr[j];
-  Q[2] = d * (vt[jp] + Rm * Of) * Rm;
-  Q[3] = d * (vt[j] + Rm * Of) * Rm;
-  Q[4] = a.energy ? ((const R *)a.en)[(long)r * ns + j] : R(0);
+  const T *rho = (const T *)a.rho + (long)r * ns, *vr = (const T *)a.vri + (long)r * ns;
+  const T *vt = (const T *)a.vti + (long)r * ns;
+  const R d = (R)rho[j], Rm = (R)a.Rmed[r];
+  Q[0] = d * (R)vr[j + ns];
+  Q[1] = d * (R)vr[j];
+  Q[2] = d * ((R)vt[jp] + Rm * Of) * Rm;
+  Q[3] = d * ((R)vt[j] + Rm * Of) * Rm;
+  Q[4] = a.energy ? (R)((const T *)a.en)[(long)r * ns + j] : R(0);
   Q[5] = d;
 }
 
 template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport_rad(KArgs a) {
-  using R = T;
+  using R = typename TransportArith<T>::type;
   __shared__ R sW[7][RT_R + 4][RT_C];
   __shared__ R sD[7][RT_R + 2][RT_C];
   const dfargo_ctrl *c = a.ctrl;
@@ -1129,7 +1180,7 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
       continue;
     }
     R Q[6];
-    rad_quantities<R>(a, r, jc, Of, Q);
+    rad_quantities<T, R>(a, r, jc, Of, Q);
 #pragma unroll
     for (int k = 0; k < 6; k++) sW[k][q][tx] = qdiv(Q[k], Q[5] + R(1e-20));
     sW[6][q][tx] = Q[5];
@@ -1145,14 +1196,14 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
   const int i = i0 + ty;
   if (j >= ns || i >= nr) return;
   const long l = (long)i * ns + j;
-  const R v0 = ((const R *)a.vri)[l], v1 = ((const R *)a.vri)[l + ns];
+  const R v0 = (R)((const T *)a.vri)[l], v1 = (R)((const T *)a.vri)[l + ns];
   // stars at faces i (Work of rings i-1, i: slots ty+1, ty+2; slopes of
   // rings i-1, i: slots ty, ty+1) and i+1
   const R rs0 = rad_star<R>(a, i, v0, dt, sW[6][ty + 1][tx], sW[6][ty + 2][tx], sD[6][ty][tx], sD[6][ty + 1][tx]);
   const R rs1 =
       rad_star<R>(a, i + 1, v1, dt, sW[6][ty + 2][tx], sW[6][ty + 3][tx], sD[6][ty + 1][tx], sD[6][ty + 2][tx]);
   R Q[6];
-  rad_quantities<R>(a, i, j, Of, Q);
+  rad_quantities<T, R>(a, i, j, Of, Q);
   const R fin = dt * dtheta * (R)a.Rinf[i], fout = dt * dtheta * (R)a.Rsup[i], invs = (R)a.InvSurf[i];
   void *outp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
 #pragma unroll
@@ -1164,7 +1215,7 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
     // VanLeerRadial (src/TransportEuler.c:205-217)
     R varq = fin * q0 * rs0 * v0;
     varq -= fout * q1 * rs1 * v1;
-    ((R *)outp[k])[l] = Q[k] + varq * invs;
+    ((T *)outp[k])[l] = sr_round<T>((double)Q[k] + (double)(varq * invs), sr_key(c, l, SR_RAD + k));
     if (k == 5 && i == 0) a.lost_col[j] = (double)varq;
   }
 }
@@ -1224,9 +1275,10 @@ __device__ __forceinline__ void azi_star(const R (&X)[K], const R (&H)[4], const
 // QuantitiesAdvection (src/TransportEuler.c:375-400) with velocity v at the
 // cells j0 .. j0+K: the density star first, then every quantity's star,
 // flux and update.  Q[5] is the density; RI is its value before the pass.
-template <typename R, int K>
+template <typename ST, typename R, int K>
 __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int j0, R (&Q)[6][K], const R (&v)[K + 1],
-                                         R dt, R dxtheta, R invdxtheta, R dxrad, R invsurf) {
+                                         R dt, R dxtheta, R invdxtheta, R dxrad, R invsurf, long base,
+                                         unsigned salt) {
   const int ns = a.ns;
   R RI[K], RS[K + 1], H[4];
 #pragma unroll
@@ -1249,7 +1301,7 @@ __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int 
     for (int k = 0; k < K; k++) {
       R varq = f[k];
       varq -= f[k + 1];
-      Q[q][k] += varq * invsurf;
+      Q[q][k] = (R)sr_round<ST>((double)Q[q][k] + (double)(varq * invsurf), sr_key(a.ctrl, base + j0 + k, salt + q));
     }
   }
 }
@@ -1257,7 +1309,7 @@ __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int 
 // The FARGO mean velocity, the integer shift and the constant residual stay
 // in FP64 whatever the storage type.
 template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_transport_azi(KArgs a) {
-  using R = T;
+  using R = typename TransportArith<T>::type;
   extern __shared__ double lds[];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
@@ -1279,7 +1331,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
 #pragma unroll
   for (int q = 0; q < 6; q++)
 #pragma unroll
-    for (int k = 0; k < K; k++) Q[q][k] = (q == 4 && !a.energy) ? R(0) : ((const R *)qp[q])[base + j0 + k];
+    for (int k = 0; k < K; k++) Q[q][k] = (q == 4 && !a.energy) ? R(0) : (R)((const T *)qp[q])[base + j0 + k];
   // ComputeThetaElongations / ComputeAverageThetaVelocities / Residuals on
   // the velocity before transport; v[K] is the first cell of the next thread
   double vel[K + 1];
@@ -1315,13 +1367,13 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
   R *ring = (R *)lds;
   int flip = 0;
   // residual pass (the only pass of the standard transport)
-  azi_pass<R, K>(a, ring, flip, j0, Q, vres, rdt, dxtheta, invdxtheta, dxrad, invsurf);
+  azi_pass<T, R, K>(a, ring, flip, j0, Q, vres, rdt, dxtheta, invdxtheta, dxrad, invsurf, base, SR_AZI);
   // uniform pass with the constant residual
   if (a.fast) {
     R uu[K + 1];
 #pragma unroll
     for (int k = 0; k < K + 1; k++) uu[k] = (R)u;
-    azi_pass<R, K>(a, ring, flip, j0, Q, uu, rdt, dxtheta, invdxtheta, dxrad, invsurf);
+    azi_pass<T, R, K>(a, ring, flip, j0, Q, uu, rdt, dxtheta, invdxtheta, dxrad, invsurf, base, SR_AZI + 8);
   }
   // AdvectSHIFT (src/TransportEuler.c:350-373): new[j] = old[j - Nshift]
   long sh_mod = nshift % ns;
@@ -1333,7 +1385,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
     for (int k = 0; k < K; k++) {
       long jn = j0 + k + sh_mod;
       if (jn >= ns) jn -= ns;
-      ((R *)qp[q])[base + jn] = Q[q][k];
+      ((T *)qp[q])[base + jn] = (T)Q[q][k];
     }
   }
 }
@@ -1492,7 +1544,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       break;
     }
     case KID_TRANSPORT_AZI: {
-      size_t lds = sizeof(T) * 2 * (size_t)a.ns + sizeof(double) * 1024;
+      size_t lds = sizeof(typename TransportArith<T>::type) * 2 * (size_t)a.ns + sizeof(double) * 1024;
       RING_DISPATCH(k_transport_azi, T, a, s, lds);
       break;
     }
@@ -1523,7 +1575,7 @@ void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s) {
 }
 
 template <typename T> void prepare_kernels(const KArgs &a) {
-  const size_t lds = sizeof(T) * 2 * (size_t)a.ns + sizeof(double) * 1024;
+  const size_t lds = sizeof(typename TransportArith<T>::type) * 2 * (size_t)a.ns + sizeof(double) * 1024;
   if (lds > 65536) {
     const void *f = nullptr;
     switch (ring_k(a.ns)) {
