@@ -268,6 +268,15 @@ static int build_context(dfargo_ctx *c) {
         (rc = upload_table(c, &a.open_vt_out, &o3, 1)))
       return rc;
   }
+  {
+    // 1 / dxtheta of SubStep1/2 (src/SourceEuler.c:933-934), per ring
+    std::vector<double> inv(n1, 0.0);
+    for (int i = 0; i < nr; i++) {
+      const double dxtheta = (p.pmax - p.pmin) / (double)ns * t.Rmed[i];
+      inv[i] = 1.0 / dxtheta;
+    }
+    if ((rc = upload_table(c, &a.InvDxTheta, inv.data(), n1))) return rc;
+  }
   c->VthetaMed.assign(n1, 0.0);
   if ((rc = upload_table(c, &a.VthetaMed, c->VthetaMed.data(), n1))) return rc;
   // fields: (nr+2) rings so that ring nr (vr outer face) exists with slack

@@ -38,6 +38,7 @@ struct KArgs {
   const double *InvRinf, *CosAz, *SinAz, *SigmaMed, *EnergyMed, *VthetaMed, *cs_iso;
   const double *powmin2, *r15, *visc_mult, *beta, *supp_torque, *sg_sqrt, *sg_normr, *sg_normt;
   const double *open_vt_in1, *open_vt_in2, *open_vt_out;
+  const double *InvDxTheta; // 1 / (dphi Rmed[i]) per ring
   // fields (T)
   void *rho, *vr, *vt, *en, *cs, *vri, *vti, *trr, *trp, *tpp, *divv, *vh;
   void *t_rho, *t_rp, *t_rm, *t_tp, *t_tm, *t_en, *sgr, *sgt, *pot, *e_int;

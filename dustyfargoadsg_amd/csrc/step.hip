@@ -62,7 +62,10 @@ template <> __device__ __forceinline__ float sr_round<float>(double v, unsigned 
     hi = f;
     lo = (f == 0.0f) ? -__uint_as_float(1u) : __uint_as_float(f > 0.0f ? u - 1u : u + 1u);
   }
-  const double p_up = (v - (double)lo) / ((double)hi - (double)lo);
+  // hi - lo is a power of two: its reciprocal is exact, from the exponent
+  const double ulp = (double)hi - (double)lo;
+  const double inv = __longlong_as_double((2046LL - ((__double_as_longlong(ulp) >> 52) & 0x7ff)) << 52);
+  const double p_up = (v - (double)lo) * inv;
   const double draw = (double)(mix32(key) >> 8) * (1.0 / 16777216.0);
   return (draw < p_up) ? hi : lo;
 }
@@ -384,6 +387,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
     const R dxtheta = (R)(a.Rmed[i] * a.pm / (double)ns);
     const R vm = (R)vmoy, dxmin = (dxrad < dxtheta) ? dxrad : dxtheta;
     const R invdt5 = (viscosity != R(0)) ? qdiv(viscosity * R(4), (R)a.powmin2[i]) : R(1e-10);
+    R smax = R(0);
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int j = t + k * TB, jp = (j == ns - 1) ? 0 : j + 1;
@@ -401,10 +405,13 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
       R invdt4 = (q5 > q4) ? q5 : q4;
       invdt4 *= R(4.0 * DF_CVNR * DF_CVNR);
       const R invdt6 = R(1e-10);
-      const R dt = qdiv(R(DF_CFLSECURITY), sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 +
-                                                   invdt4 * invdt4 + invdt5 * invdt5 + invdt6 * invdt6));
-      dtmin = dmin2(dtmin, (double)dt);
+      // dt = CFLSECURITY / sqrt(S) decreases with S, and rounding keeps
+      // that order: the smallest dt of the ring is that of its largest S
+      const R S = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 +
+                  invdt6 * invdt6;
+      smax = (S > smax) ? S : smax;
     }
+    dtmin = qdiv(R(DF_CFLSECURITY), sqrt(smax));
   }
   // per-ring partials, stored partial-major (ring_part[p * nr + i]):
   // negative-density and negative-energy counts, disc mass, barycentre.
@@ -901,7 +908,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
       }
     }
     const long ljm = (long)i * ns + sCol[p];
-    const double dxtheta = a.pm / (double)ns * a.Rmed[i], invdxtheta = 1.0 / dxtheta;
+    const double invdxtheta = a.InvDxTheta[i];
     const double pljm = src_press<T>(a, ljm, i);
     double gradp = (pl - pljm) * 2.0 / (rl + ld<T>(a.rho, ljm)) * invdxtheta;
     double gradphi = (potl - sPot[q + 1][p]) * invdxtheta;
@@ -1013,7 +1020,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
       const double qrm = (dv < 0.0) ? C2 * rhoim * dv * dv : 0.0;
       vrn = vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * a.InvDiffRmed[i];
     }
-    const double dxtheta = a.pm / (double)ns * a.Rmed[i], invdxtheta = 1.0 / dxtheta;
+    const double invdxtheta = a.InvDxTheta[i];
     st_sr<T>(a.vri, l, vrn, sr_key(c, l, SR_VR));
     st_sr<T>(a.vti, l, vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta, sr_key(c, l, SR_VT));
     if (a.energy) {
@@ -1181,8 +1188,10 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
     }
     R Q[6];
     rad_quantities<T, R>(a, r, jc, Of, Q);
+    // the Work values Q / (rho + 1e-20) share one reciprocal per cell
+    const R inv = qdiv(R(1), Q[5] + R(1e-20));
 #pragma unroll
-    for (int k = 0; k < 6; k++) sW[k][q][tx] = qdiv(Q[k], Q[5] + R(1e-20));
+    for (int k = 0; k < 6; k++) sW[k][q][tx] = Q[k] * inv;
     sW[6][q][tx] = Q[5];
   }
   __syncthreads();
@@ -1275,10 +1284,9 @@ __device__ __forceinline__ void azi_star(const R (&X)[K], const R (&H)[4], const
 // QuantitiesAdvection (src/TransportEuler.c:375-400) with velocity v at the
 // cells j0 .. j0+K: the density star first, then every quantity's star,
 // flux and update.  Q[5] is the density; RI is its value before the pass.
-template <typename ST, typename R, int K>
+template <typename R, int K>
 __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int j0, R (&Q)[6][K], const R (&v)[K + 1],
-                                         R dt, R dxtheta, R invdxtheta, R dxrad, R invsurf, long base,
-                                         unsigned salt) {
+                                         R dt, R dxtheta, R invdxtheta, R dxrad, R invsurf) {
   const int ns = a.ns;
   R RI[K], RS[K + 1], H[4];
 #pragma unroll
@@ -1286,12 +1294,16 @@ __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int 
   ring_halo<R, K>(lds + flip * ns, RI, j0, ns, H);
   flip ^= 1;
   azi_star<R, K>(RI, H, v, dt, dxtheta, invdxtheta, RS);
+  // the Work values Q / (rho + 1e-20) share one reciprocal per cell
+  R inv[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) inv[k] = qdiv(R(1), RI[k] + R(1e-20));
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     if (q == 4 && !a.energy) continue;
     R W[K], S[K + 1], f[K + 1];
 #pragma unroll
-    for (int k = 0; k < K; k++) W[k] = qdiv(Q[q][k], RI[k] + R(1e-20));
+    for (int k = 0; k < K; k++) W[k] = Q[q][k] * inv[k];
     ring_halo<R, K>(lds + flip * ns, W, j0, ns, H);
     flip ^= 1;
     azi_star<R, K>(W, H, v, dt, dxtheta, invdxtheta, S);
@@ -1301,7 +1313,7 @@ __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int 
     for (int k = 0; k < K; k++) {
       R varq = f[k];
       varq -= f[k + 1];
-      Q[q][k] = (R)sr_round<ST>((double)Q[q][k] + (double)(varq * invsurf), sr_key(a.ctrl, base + j0 + k, salt + q));
+      Q[q][k] += varq * invsurf;
     }
   }
 }
@@ -1367,13 +1379,13 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
   R *ring = (R *)lds;
   int flip = 0;
   // residual pass (the only pass of the standard transport)
-  azi_pass<T, R, K>(a, ring, flip, j0, Q, vres, rdt, dxtheta, invdxtheta, dxrad, invsurf, base, SR_AZI);
+  azi_pass<R, K>(a, ring, flip, j0, Q, vres, rdt, dxtheta, invdxtheta, dxrad, invsurf);
   // uniform pass with the constant residual
   if (a.fast) {
     R uu[K + 1];
 #pragma unroll
     for (int k = 0; k < K + 1; k++) uu[k] = (R)u;
-    azi_pass<T, R, K>(a, ring, flip, j0, Q, uu, rdt, dxtheta, invdxtheta, dxrad, invsurf, base, SR_AZI + 8);
+    azi_pass<R, K>(a, ring, flip, j0, Q, uu, rdt, dxtheta, invdxtheta, dxrad, invsurf);
   }
   // AdvectSHIFT (src/TransportEuler.c:350-373): new[j] = old[j - Nshift]
   long sh_mod = nshift % ns;
@@ -1385,7 +1397,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
     for (int k = 0; k < K; k++) {
       long jn = j0 + k + sh_mod;
       if (jn >= ns) jn -= ns;
-      ((T *)qp[q])[base + jn] = (T)Q[q][k];
+      ((T *)qp[q])[base + jn] = sr_round<T>((double)Q[q][k], sr_key(c, base + jn, SR_AZI + q));
     }
   }
 }
