@@ -53,13 +53,12 @@ template <typename T> __device__ __forceinline__ void dft4(const cx<T> (&x)[4], 
 // quotient j / Ns comes from a 32-bit reciprocal (mg = 2^32/Ns rounded up,
 // exact for j < 2^12 and Ns <= 2^12), not an integer division.
 template <typename T, int R>
-__device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int Ns, unsigned mg, int span,
-                                         const cx<T> *tw, int dir) {
-  const int nb = L / R;
+__device__ __forceinline__ void fft_butterfly(const cx<T> *in, cx<T> *out, int j, int nb, int Ns, unsigned mg,
+                                              int span, const cx<T> *tw, int dir) {
   const T s3 = (T)(dir * 0.86602540378443864676); // dir * sqrt(3)/2
   const T c8 = (T)0.70710678118654752440;         // sqrt(1/2)
   const T fd = (T)dir;
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+  {
     const int g = (Ns == 1) ? j : (int)__umulhi((unsigned)j, mg);
     const int k = j - g * Ns;
     cx<T> v[R], y[R];
@@ -111,6 +110,27 @@ __device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int
   }
 }
 
+template <typename T, int R>
+__device__ __forceinline__ void fft_pass(const cx<T> *in, cx<T> *out, int L, int Ns, unsigned mg, int span,
+                                         const cx<T> *tw, int dir) {
+  const int nb = L / R;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) fft_butterfly<T, R>(in, out, j, nb, Ns, mg, span, tw, dir);
+}
+
+// the same pass over two independent transforms of the same length at once:
+// the workgroup's threads share their butterflies, and the two transforms
+// share the pass's barrier
+template <typename T, int R>
+__device__ __forceinline__ void fft_pass2(const cx<T> *in0, cx<T> *out0, const cx<T> *in1, cx<T> *out1, int L,
+                                          int Ns, unsigned mg, int span, const cx<T> *tw, int dir) {
+  const int nb = L / R;
+  for (int jj = threadIdx.x; jj < 2 * nb; jj += blockDim.x) {
+    const bool second = jj >= nb;
+    fft_butterfly<T, R>(second ? in1 : in0, second ? out1 : out0, second ? jj - nb : jj, nb, Ns, mg, span, tw,
+                        dir);
+  }
+}
+
 // Stockham transform of length L = 2^a 3^b <= 4096 held in LDS buffers A
 // and B (padded layout fp) (radix 3 passes first, then radix 8, then one radix 4 or 2 if
 // needed); returns the buffer holding the result.  dir = -1 is the forward
@@ -136,6 +156,34 @@ __device__ __forceinline__ cx<T> *fft_lds(cx<T> *A, cx<T> *B, int L, const cx<T>
     rem /= R;
   }
   return in;
+}
+
+// two transforms of length L in lockstep (buffers A0/B0 and A1/B1); the
+// results are in *r0 and *r1
+template <typename T>
+__device__ __forceinline__ void fft_lds2(cx<T> *A0, cx<T> *B0, cx<T> *A1, cx<T> *B1, int L, const cx<T> *tw,
+                                         int dir, cx<T> **r0, cx<T> **r1) {
+  int Ns = 1, rem = L;
+  cx<T> *in0 = A0, *out0 = B0, *in1 = A1, *out1 = B1;
+  while (rem > 1) {
+    const int R = (rem % 3 == 0) ? 3 : ((rem % 8 == 0) ? 8 : ((rem % 4 == 0) ? 4 : 2));
+    const int span = L / (Ns * R);
+    const unsigned mg = (Ns > 1) ? 0xFFFFFFFFu / (unsigned)Ns + 1u : 0u;
+    if (R == 8) fft_pass2<T, 8>(in0, out0, in1, out1, L, Ns, mg, span, tw, dir);
+    else if (R == 4) fft_pass2<T, 4>(in0, out0, in1, out1, L, Ns, mg, span, tw, dir);
+    else if (R == 3) fft_pass2<T, 3>(in0, out0, in1, out1, L, Ns, mg, span, tw, dir);
+    else fft_pass2<T, 2>(in0, out0, in1, out1, L, Ns, mg, span, tw, dir);
+    __syncthreads();
+    cx<T> *t0 = in0, *t1 = in1;
+    in0 = out0;
+    out0 = t0;
+    in1 = out1;
+    out1 = t1;
+    Ns *= R;
+    rem /= R;
+  }
+  *r0 = in0;
+  *r1 = in1;
 }
 
 template <typename T> __device__ __forceinline__ double ldd(const void *p, long i) {
@@ -209,8 +257,12 @@ __global__ void __launch_bounds__(256) k_sg_cols(KArgs a, int force, const cx<T>
   }
   __syncthreads();
   const cx<T> *tw = (const cx<T> *)a.tw_2n;
-  cx<T> *ra = fft_lds<T>(A1, B1, L, tw, -1);
-  cx<T> *rb = self ? ra : fft_lds<T>(A2, B2, L, tw, -1);
+  cx<T> *ra, *rb;
+  if (self) {
+    ra = rb = fft_lds<T>(A1, B1, L, tw, -1);
+  } else {
+    fft_lds2<T>(A1, B1, A2, B2, L, tw, -1, &ra, &rb);
+  }
   cx<T> *wa = (ra == A1) ? B1 : A1;
   cx<T> *wb = self ? wa : ((rb == A2) ? B2 : A2);
   const cx<T> *khr = (const cx<T> *)a.sg_kr + (long)p * L, *kht = (const cx<T> *)a.sg_kt + (long)p * L;
@@ -237,8 +289,12 @@ __global__ void __launch_bounds__(256) k_sg_cols(KArgs a, int force, const cx<T>
     }
   }
   __syncthreads();
-  cx<T> *oa = fft_lds<T>(wa, (wa == A1) ? B1 : A1, L, tw, +1);
-  cx<T> *ob = self ? oa : fft_lds<T>(wb, (wb == A2) ? B2 : A2, L, tw, +1);
+  cx<T> *oa, *ob;
+  if (self) {
+    oa = ob = fft_lds<T>(wa, (wa == A1) ? B1 : A1, L, tw, +1);
+  } else {
+    fft_lds2<T>(wa, (wa == A1) ? B1 : A1, wb, (wb == A2) ? B2 : A2, L, tw, +1, &oa, &ob);
+  }
   cx<T> *wpa = wt + (long)ka * nr, *wpb = wt + (long)kb * nr;
   for (int i = threadIdx.x; i < nr; i += blockDim.x) {
     wpa[i] = oa[fp(i)];
