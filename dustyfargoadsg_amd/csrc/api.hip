@@ -41,6 +41,10 @@ struct dfargo_ctx {
   double kt_ms[KID_COUNT];
   long kt_n[KID_COUNT];
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the second stream and the events of the self-gravity branch (graph
+  // capture only; default priority)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<hipEvent_t> evpool;
   std::vector<double> VthetaMed, gcs_host;
   std::vector<float> fstage;
@@ -129,11 +133,17 @@ static void ctrl_to_planets(const dfargo_ctrl *k, dfargo_planets *s) {
   }
 }
 
-template <typename T> static void launch_one(dfargo_ctx *c, int id) { launch_kernel<T>(id, c->a, c->stream); }
+template <typename T> static void launch_one(dfargo_ctx *c, int id, hipStream_t s) { launch_kernel<T>(id, c->a, s); }
 
-static void launch_id(dfargo_ctx *c, int id) {
-  if (c->prec == DFARGO_FP64) launch_one<double>(c, id);
-  else launch_one<float>(c, id);
+static void launch_id(dfargo_ctx *c, int id, hipStream_t s) {
+  if (c->prec == DFARGO_FP64) launch_one<double>(c, id, s);
+  else launch_one<float>(c, id, s);
+}
+static void launch_id(dfargo_ctx *c, int id) { launch_id(c, id, c->stream); }
+
+static bool is_sg_kernel(int id) {
+  return id == KID_SG_ROWS_FWD || id == KID_SG_TRANSPOSE_Z || id == KID_SG_COLS || id == KID_SG_TRANSPOSE_W ||
+         id == KID_SG_ROWS_INV;
 }
 
 static int run_substep(dfargo_ctx *c) {
@@ -161,8 +171,24 @@ static int run_substep(dfargo_ctx *c) {
     return DFARGO_OK;
   }
   if (!c->gexec) {
+    // The self-gravity transforms read only the density the sub-step starts
+    // with and feed only the source kernel: in the graph they form a branch
+    // of their own, beside the CFL, control and particle kernels, joined
+    // before k_source.
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int id : c->seq) launch_id(c, id);
+    const bool fork = c->p.self_gravity != 0;
+    if (fork) {
+      HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+      HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+      for (int id : c->seq)
+        if (is_sg_kernel(id)) launch_id(c, id, c->stream2);
+      HIPCHK(hipEventRecord(c->ev_join, c->stream2));
+    }
+    for (int id : c->seq) {
+      if (fork && is_sg_kernel(id)) continue;
+      if (fork && id == KID_SOURCE) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+      launch_id(c, id);
+    }
     hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
     if (e != hipSuccess) return dfargo_set_error(DFARGO_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
@@ -382,6 +408,9 @@ static int build_context(dfargo_ctx *c) {
   build_sequence(c);
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   return DFARGO_OK;
 }
 
@@ -438,6 +467,9 @@ extern "C" int dfargo_destroy(dfargo_ctx *c) {
   if (c->h_ctrl) hipHostFree(c->h_ctrl);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->stream2) hipStreamDestroy(c->stream2);
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   dfargo_free_tables(&c->tab);

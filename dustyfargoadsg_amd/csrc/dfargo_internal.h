@@ -68,6 +68,7 @@ typedef struct dfargo_ctrl {
   int tg_planet[DF_MAXTG];
   int ntarget;
   int go, active, gas_timesteps_cfl, substeps, crashed_d, crashed_e;
+  int go_next; /* the loop condition for the next sub-step, set by the control kernel */
   long substeps_total;
   /* dust particles: super-particle mass (tapered like the planets) */
   double particles_mass, particles_mass0;

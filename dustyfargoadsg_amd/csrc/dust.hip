@@ -219,7 +219,7 @@ __device__ __forceinline__ void stencil_gather(const KArgs &a, const double (&wr
 
 // does this sub-step (or the start-up) need the buckets and the deposit?
 __device__ __forceinline__ bool dust_deposit_needed(const KArgs &a, const dfargo_ctrl *c, int force) {
-  return force || a.d_feedback || !c->go;
+  return force || a.d_feedback || !c->go_next;
 }
 
 // One particle per thread.  In a sub-step: SemiUpdateDustPositions over
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(256) k_dust_move(KArgs a, int startup, double 
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int nr = a.nr, ns = a.ns;
   const double timestep = startup ? timestep0 : c->dt;
-  const bool store01 = a.d_feedback, store2 = startup || !c->go || (a.d_feedback && a.energy);
+  const bool store01 = a.d_feedback, store2 = startup || !c->go_next || (a.d_feedback && a.energy);
   const DustSet P = a.pset[c->pset];
   unsigned long long ts_bits = ~0ull;
   bool outside = false;
@@ -541,7 +541,7 @@ template <typename T, int S> __global__ void __launch_bounds__(DEP_BX) k_dust_de
   }
   __syncthreads();
   if (j >= ns) return;
-  const bool want_dens = force || !c->go;
+  const bool want_dens = force || !c->go_next;
   const bool want_fb = a.d_feedback, want_e = a.d_feedback && a.energy;
   const long l = (long)i * ns + j, np = a.d_np;
   const double mp = force ? c->particles_mass : c->dust_mp, surf = a.Surf[i];

@@ -336,12 +336,16 @@ __device__ void begin_substep(const KArgs &a, dfargo_ctrl *c) {
 }
 
 // AlgoGas entry: loop variables, and the first sub-step's start.  The
-// sub-step kernels read two flags of the control block: `go` (the loop
-// condition, read by the CFL kernel that opens a sub-step and by the
-// control kernel) and `active` (set by the control kernel from `go`, read
-// by every kernel after it).  The control kernel re-evaluates `go` for the
-// next sub-step at its end, so the flags never change under a kernel that
-// reads them.
+// sub-step kernels read three flags of the control block: `go` (the loop
+// condition of the current sub-step: read by the kernels that may run
+// before or beside the control kernel, the CFL and self-gravity kernels),
+// `active` (set by the control kernel from `go`, read by every kernel after
+// it) and `go_next` (the condition of the next sub-step, set by the control
+// kernel, which also prepares that sub-step's start; it tells the kernels
+// after the control kernel whether theirs is the last sub-step of the
+// call).  The last kernel of a sub-step copies go_next into go, so no flag
+// changes under a kernel that reads it, even with the self-gravity kernels
+// running concurrently with the control kernel.
 __global__ void k_algogas_begin(KArgs a, double DT) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     dfargo_ctrl *c = a.ctrl;
@@ -351,6 +355,7 @@ __global__ void k_algogas_begin(KArgs a, double DT) {
     c->gas_timesteps_cfl = 1;
     c->active = 0;
     c->go = c->dtemp < 0.999999999 * c->DT;
+    c->go_next = c->go;
     if (c->go) begin_substep(a, c);
   }
 }
@@ -758,8 +763,8 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
     c->out_count = 0;
   }
   // loop condition and start of the next sub-step
-  c->go = c->dtemp < 0.999999999 * c->DT;
-  if (c->go) begin_substep(a, c);
+  c->go_next = c->dtemp < 0.999999999 * c->DT;
+  if (c->go_next) begin_substep(a, c);
 }
 
 // ComputeSoundSpeed at the start of AlgoGas (src/SourceEuler.c:2340); in
@@ -834,6 +839,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs
 #define SRC_VC (SRC_TC + 5)
 #define SRC_SR (SRC_TR + 3)
 #define SRC_SC (SRC_TC + 3)
+#define SRC_FC (SRC_TC + 7)  // state staged on rings i0-4 .. i0+TR+1, columns j0-4 .. j0+TC+2
 
 template <typename T> __device__ __forceinline__ double src_press(const KArgs &a, long l, int i) {
   if (!a.energy) {
@@ -849,7 +855,11 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
   // increments are kept in full until the final, unbiased rounding
   __shared__ double sVr[SRC_VR][SRC_VC], sVt[SRC_VR][SRC_VC];
   __shared__ T sTrr[SRC_SR][SRC_SC], sTpp[SRC_SR][SRC_SC], sTrp[SRC_SR][SRC_SC];
-  __shared__ int sCol[SRC_PC];
+  // the state the stages read, staged once: density and vtheta as stored,
+  // pressure in FP64 (slot = V slot + 1 = S slot + 2 in both directions)
+  __shared__ T sRho[SRC_PR][SRC_FC], sVth[SRC_PR][SRC_FC];
+  __shared__ double sP[SRC_PR][SRC_FC];
+  __shared__ int sCol[SRC_FC];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
   const int ns = a.ns, nr = a.nr, tid = threadIdx.x, TB = blockDim.x;
@@ -857,16 +867,22 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
   const double dt = c->dt;
   // the particles of this sub-step were re-sorted into the other set
   if (a.d_on && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) a.ctrl->pset ^= 1;
-  // columns j0-4 .. j0+SRC_TC+1 of the potential region, wrapped
-  for (int p = tid; p < SRC_PC; p += TB) {
+  // columns j0-4 .. j0+SRC_TC+2 of the staged region, wrapped
+  for (int p = tid; p < SRC_FC; p += TB) {
     int jj = (j0 - 4 + p) % ns;
     sCol[p] = (jj < 0) ? jj + ns : jj;
   }
   __syncthreads();
-  // stage 0: potential on rings i0-4 .. i0+SRC_TR+1
-  for (int idx = tid; idx < SRC_PR * SRC_PC; idx += TB) {
-    const int q = idx / SRC_PC, p = idx - q * SRC_PC, r = i0 - 4 + q;
-    sPot[q][p] = (r >= 0 && r <= nr - 1) ? pot_at(a, c, r, sCol[p]) : 0.0;
+  // stage 0: the state of rings i0-4 .. i0+SRC_TR+1 into LDS, and the
+  // potential there (rings outside the grid hold zeros no valid cell reads)
+  for (int idx = tid; idx < SRC_PR * SRC_FC; idx += TB) {
+    const int q = idx / SRC_FC, p = idx - q * SRC_FC, r = i0 - 4 + q;
+    const bool in = (r >= 0 && r <= nr - 1);
+    const long l = (long)r * ns + sCol[p];
+    sRho[q][p] = in ? ((const T *)a.rho)[l] : (T)0;
+    sVth[q][p] = in ? ((const T *)a.vt)[l] : (T)0;
+    sP[q][p] = in ? src_press<T>(a, l, r) : 0.0;
+    if (p < SRC_PC) sPot[q][p] = in ? pot_at(a, c, r, sCol[p]) : 0.0;
   }
   __syncthreads();
   // stage 1: SubStep1 on rings i0-3 .. i0+SRC_TR+1 (VradInt and VthetaInt
@@ -879,18 +895,19 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
       continue;
     }
     const long l = (long)i * ns + j;
-    const double rl = ld<T>(a.rho, l), pl = src_press<T>(a, l, i), potl = sPot[q + 1][p + 1];
+    const double rl = (double)sRho[q + 1][p + 1], pl = sP[q + 1][p + 1], potl = sPot[q + 1][p + 1];
     // the particles' grids (pressure gradients, indirect and self-gravity
     // accelerations, src/SourceEuler.c:893-948) on the tile's own cells
     const int jt = j0 - 3 + p;
     const bool dgrid = a.d_on && q >= 3 && q < 3 + SRC_TR && jt >= j0 && jt < j0 + SRC_TC && jt < ns;
     double vri = 0.0;
     if (i >= 1) {
-      const long lim = l - ns, ljp = (j == ns - 1) ? (long)i * ns : l + 1;
-      const double plim = src_press<T>(a, lim, i - 1);
-      double gradp = (pl - plim) * 2.0 / (rl + ld<T>(a.rho, lim)) * a.InvDiffRmed[i];
+      const long lim = l - ns;
+      const double plim = sP[q][p + 1];
+      double gradp = (pl - plim) * 2.0 / (rl + (double)sRho[q][p + 1]) * a.InvDiffRmed[i];
       double gradphi = (potl - sPot[q][p + 1]) * a.InvDiffRmed[i];
-      double vt2 = ld<T>(a.vt, l) + ld<T>(a.vt, ljp) + ld<T>(a.vt, lim) + ld<T>(a.vt, ljp - ns);
+      double vt2 = (double)sVth[q + 1][p + 1] + (double)sVth[q + 1][p + 2] + (double)sVth[q][p + 1] +
+                   (double)sVth[q][p + 2];
       vt2 = vt2 / 4.0 + a.Rinf[i] * c->omega_frame;
       vt2 = vt2 * vt2;
       vri = ld<T>(a.vr, l) + dt * (-gradp - gradphi + vt2 * a.InvRinf[i]);
@@ -909,10 +926,10 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
     }
     const long ljm = (long)i * ns + sCol[p];
     const double invdxtheta = a.InvDxTheta[i];
-    const double pljm = src_press<T>(a, ljm, i);
-    double gradp = (pl - pljm) * 2.0 / (rl + ld<T>(a.rho, ljm)) * invdxtheta;
+    const double pljm = sP[q + 1][p];
+    double gradp = (pl - pljm) * 2.0 / (rl + (double)sRho[q + 1][p]) * invdxtheta;
     double gradphi = (potl - sPot[q + 1][p]) * invdxtheta;
-    double vti = ld<T>(a.vt, l) - dt * (gradp + gradphi);
+    double vti = (double)sVth[q + 1][p + 1] - dt * (gradp + gradphi);
     vti += dt * a.supp_torque[i];
     if (a.d_on && a.d_feedback) vti += dt * ld<T>(a.azifbacc, l);
     if (dgrid) {
@@ -938,7 +955,6 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
         sTrr[q][p] = sTpp[q][p] = sTrp[q][p] = (T)0;
         continue;
       }
-      const long l = (long)i * ns + j;
       const double nu = a.nu_ring[i];
       const double vrl = (double)sVr[q + 1][p + 1], vrp = (double)sVr[q + 2][p + 1];
       const double vtl = (double)sVt[q + 1][p + 1], vtp = (double)sVt[q + 1][p + 2];
@@ -946,16 +962,17 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
       dv += (vtp - vtl) * a.invdphi * a.InvRmed[i];
       const double Drr = (vrp - vrl) * a.InvDiffRsup[i];
       const double Dpp = (vtp - vtl) * a.invdphi * a.InvRmed[i] + 0.5 * (vrp + vrl) * a.InvRmed[i];
-      const double rho = ld<T>(a.rho, l);
+      const double rho = (double)sRho[q + 2][p + 2];
       sTrr[q][p] = (T)(2.0 * rho * nu * (Drr - onethird * dv));
       sTpp[q][p] = (T)(2.0 * rho * nu * (Dpp - onethird * dv));
       double Trp = 0.0;
       if (i >= 1) {
-        const long lim = l - ns, ljm = (long)i * ns + sCol[p + 1], ljmim = ljm - ns;
         const double Drp = 0.5 * (a.Rinf[i] * (vtl * a.InvRmed[i] - (double)sVt[q][p + 1] * a.InvRmed[i - 1]) *
                                       a.InvDiffRmed[i] +
                                   (vrl - (double)sVr[q + 1][p]) * a.invdphi * a.InvRinf[i]);
-        Trp = 2.0 * 0.25 * (rho + ld<T>(a.rho, lim) + ld<T>(a.rho, ljm) + ld<T>(a.rho, ljmim)) * nu * Drp;
+        Trp = 2.0 * 0.25 *
+              (rho + (double)sRho[q + 1][p + 2] + (double)sRho[q + 2][p + 1] + (double)sRho[q + 1][p + 1]) * nu *
+              Drp;
       }
       sTrp[q][p] = (T)Trp;
     }
@@ -967,14 +984,13 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
       const int q = idx / (SRC_TC + 2), p = idx - q * (SRC_TC + 2), i = i0 + q;
       const int qv = q + 3, pv = p + 2, qs = qv - 1, ps = pv - 1;
       if (i >= 1 && i < nr - 1) {
-        const long l = (long)i * ns + sCol[pv + 1], ljm = (long)i * ns + sCol[pv];
-        const double rho = ld<T>(a.rho, l);
+        const double rho = (double)sRho[qv + 1][pv + 1];
         const double trpl = (double)sTrp[qs][ps], trpp = (double)sTrp[qs + 1][ps];
         double v = (double)sVt[qv][pv];
         v += dt * a.InvRmed[i] *
              ((a.Rsup[i] * trpp - a.Rinf[i] * trpl) * a.InvDiffRsup[i] +
               ((double)sTpp[qs][ps] - (double)sTpp[qs][ps - 1]) * a.invdphi + 0.5 * (trpl + trpp)) /
-             (0.5 * (rho + ld<T>(a.rho, ljm)));
+             (0.5 * (rho + (double)sRho[qv + 1][pv]));
         sVt[qv][pv] = v;
       }
     }
@@ -982,14 +998,13 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
       const int q = idx / SRC_TC, p = idx - q * SRC_TC, i = i0 - 1 + q;
       const int qv = q + 2, pv = p + 3, qs = qv - 1, ps = pv - 1;
       if (i >= 1 && i <= nr - 1) {
-        const long l = (long)i * ns + sCol[pv + 1], lim = l - ns;
-        const double rho = ld<T>(a.rho, l);
+        const double rho = (double)sRho[qv + 1][pv + 1];
         double v = (double)sVr[qv][pv];
         v += dt * a.InvRinf[i] *
              ((a.Rmed[i] * (double)sTrr[qs][ps] - a.Rmed[i - 1] * (double)sTrr[qs - 1][ps]) * a.InvDiffRmed[i] +
               ((double)sTrp[qs][ps + 1] - (double)sTrp[qs][ps]) * a.invdphi -
               0.5 * ((double)sTpp[qs][ps] + (double)sTpp[qs - 1][ps])) /
-             (0.5 * (rho + ld<T>(a.rho, lim)));
+             (0.5 * (rho + (double)sRho[qv][pv + 1]));
         sVr[qv][pv] = v;
       }
     }
@@ -1001,21 +1016,20 @@ template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
     const int q = idx / SRC_TC, p = idx - q * SRC_TC, i = i0 + q, jt = j0 + p;
     if (i > nr - 1 || jt >= ns) continue;
     const int qv = q + 3, pv = p + 3, j = sCol[pv + 1];
-    const long l = (long)i * ns + j, ljm = (long)i * ns + sCol[pv];
-    const double rho = ld<T>(a.rho, l);
+    const long l = (long)i * ns + j;
+    const double rho = (double)sRho[qv + 1][pv + 1];
     const double vrl = (double)sVr[qv][pv], vrp = (double)sVr[qv + 1][pv];
     const double vtl = (double)sVt[qv][pv], vtp = (double)sVt[qv][pv + 1], vtm = (double)sVt[qv][pv - 1];
     double dv = vrp - vrl;
     const double qr = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
     dv = vtp - vtl;
     const double qt = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
-    const double rhojm = ld<T>(a.rho, ljm);
+    const double rhojm = (double)sRho[qv + 1][pv];
     dv = vtl - vtm;
     const double qtm = (dv < 0.0) ? C2 * rhojm * dv * dv : 0.0;
     double vrn = 0.0;
     if (i >= 1) {
-      const long lim = l - ns;
-      const double rhoim = ld<T>(a.rho, lim);
+      const double rhoim = (double)sRho[qv][pv + 1];
       dv = vrl - (double)sVr[qv - 1][pv];
       const double qrm = (dv < 0.0) ? C2 * rhoim * dv * dv : 0.0;
       vrn = vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * a.InvDiffRmed[i];
@@ -1061,7 +1075,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
   const double dt = c->dt;
   // the stresses, divergence and heating are diagnostics read between
   // AlgoGas calls: stored on the last sub-step of a call only
-  const bool last = !c->go;
+  const bool last = !c->go_next;
   double vh[K], dv[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -1442,6 +1456,8 @@ template <typename T> __device__ CellState vel_cell(const KArgs &a, const dfargo
 template <typename T> __global__ void __launch_bounds__(256) k_finish(KArgs a) {
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
+  // the sub-step ends here: the next one's loop condition takes effect
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctrl->go = c->go_next;
   const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ns) return;
   const long l = (long)i * ns + j;
