@@ -46,8 +46,6 @@ def kernel_bytes_per_cell(name, w, energy, sg):
         # each, 4 words per cell), W columns out
         "sg_cols": 2 * w + 4 * w + 2 * w,
         "sg_rows_inv": 2 * w + w + 2 * w,                # W row, Sigma -> g_r, g_theta
-        "sg_transpose_z": 2 * w + 2 * w,                 # Z row-major -> column-major
-        "sg_transpose_w": 2 * w + 2 * w,
         "control": 0,
     }
     return table.get(name, 0)

@@ -142,8 +142,7 @@ static void launch_id(dfargo_ctx *c, int id, hipStream_t s) {
 static void launch_id(dfargo_ctx *c, int id) { launch_id(c, id, c->stream); }
 
 static bool is_sg_kernel(int id) {
-  return id == KID_SG_ROWS_FWD || id == KID_SG_TRANSPOSE_Z || id == KID_SG_COLS || id == KID_SG_TRANSPOSE_W ||
-         id == KID_SG_ROWS_INV;
+  return id == KID_SG_ROWS_FWD || id == KID_SG_COLS || id == KID_SG_ROWS_INV;
 }
 
 static int run_substep(dfargo_ctx *c) {
@@ -209,8 +208,7 @@ static int run_substep(dfargo_ctx *c) {
 static void build_sequence(dfargo_ctx *c) {
   c->seq.clear();
   if (c->p.self_gravity)
-    for (int id : {KID_SG_ROWS_FWD, KID_SG_TRANSPOSE_Z, KID_SG_COLS, KID_SG_TRANSPOSE_W, KID_SG_ROWS_INV})
-      c->seq.push_back(id);
+    for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV}) c->seq.push_back(id);
   for (int id : {KID_CFL, KID_CONTROL}) c->seq.push_back(id);
   if (c->dust_on)
     for (int id : {KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT,

@@ -86,11 +86,20 @@ struct KArgs {
 
 // launchers (step.hip, sg.hip); each enqueues on stream s
 enum KernelId {
-  KID_SG_ROWS_FWD = 0, KID_SG_TRANSPOSE_Z, KID_SG_COLS, KID_SG_TRANSPOSE_W, KID_SG_ROWS_INV,
+  KID_SG_ROWS_FWD = 0, KID_SG_COLS, KID_SG_ROWS_INV,
   KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
   KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
+
+// One block per ring; with xcd_remap the rings of one XCD (blocks are
+// dealt to the 8 XCDs round-robin) are contiguous, so the radial neighbours
+// a block reads, and the lines a ring-strided access shares, stay in one L2.
+__device__ __forceinline__ int ring_of_block(const KArgs &a) {
+  int b = blockIdx.x, n = gridDim.x;
+  if (a.xcd_remap && (n % 8 == 0)) return (b % 8) * (n / 8) + b / 8;
+  return b;
+}
 
 template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s);
 // one-time kernel attributes (dynamic LDS above 64 KiB); call before any capture

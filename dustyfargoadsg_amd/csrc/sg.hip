@@ -191,12 +191,15 @@ template <typename T> __device__ __forceinline__ double ldd(const void *p, long 
 }
 
 // compute_fftdensity (src/sgdens.c:54-70): Sr = Sigma sqrt(r/r0),
-// St = Sr r/r0 on the first Nrad rows, then the azimuthal transform; Z is
-// stored row-major [Nrad][Nsec]
-template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs a, int force) {
+// St = Sr r/r0 on the first Nrad rows, then the azimuthal transform.  Z is
+// stored transposed, [Nsec][Nrad] (zt), so that the radial transforms read
+// contiguous columns.  The rings of one XCD are contiguous (ring_of_block):
+// the 16 (FP32) or 8 (FP64) rings that share a line of zt are written in
+// one L2, which merges the lines before they go to HBM.
+template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs a, int force, cx<T> *zt) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->go) return;
-  const int i = blockIdx.x, ns = a.ns;
+  const int i = ring_of_block(a), ns = a.ns, nr = a.nr;
   cx<T> *A = (cx<T> *)smem, *B = A + fpsz(ns);
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
     double sr = ldd<T>(a.rho, (long)i * ns + j) * a.sg_sqrt[i];
@@ -205,28 +208,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs
   }
   __syncthreads();
   cx<T> *r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, -1);
-  cx<T> *z = (cx<T> *)a.sg_z + (long)i * ns;
-  for (int j = threadIdx.x; j < ns; j += blockDim.x) z[j] = r[fp(j)];
-}
-
-// dst[c][r] = src[r][c] for a rows x cols complex array, through 32 x 32
-// LDS tiles so that both sides are read and written along rows
-#define TP_T 32
-template <typename T>
-__global__ void __launch_bounds__(TP_T * 8) k_sg_transpose(KArgs a, int force, const cx<T> *src, cx<T> *dst,
-                                                           int rows, int cols) {
-  __shared__ cx<T> tile[TP_T][TP_T + 1];
-  if (!force && !a.ctrl->go) return;
-  const int c0 = blockIdx.x * TP_T, r0 = blockIdx.y * TP_T, tx = threadIdx.x, ty = threadIdx.y;
-  for (int q = ty; q < TP_T; q += 8) {
-    const int r = r0 + q, c = c0 + tx;
-    if (r < rows && c < cols) tile[q][tx] = src[(long)r * cols + c];
-  }
-  __syncthreads();
-  for (int q = ty; q < TP_T; q += 8) {
-    const int c = c0 + q, r = r0 + tx;
-    if (r < rows && c < cols) dst[(long)c * rows + r] = tile[tx][q];
-  }
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) zt[(long)j * nr + i] = r[fp(j)];
 }
 
 // Radial transforms and spectral product (src/sgacc.c:25-42) for one pair
@@ -303,14 +285,14 @@ __global__ void __launch_bounds__(256) k_sg_cols(KArgs a, int force, const cx<T>
 }
 
 // inverse azimuthal transform, normalisation and self-force compensation
-// (src/sgacc.c:137-161); W row-major
-template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_inv(KArgs a, int force, const cx<T> *wrow) {
+// (src/sgacc.c:137-161); W stored transposed ([Nsec][Nrad], wt), read
+// with the rings of one XCD contiguous as in k_sg_rows_fwd
+template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_inv(KArgs a, int force, const cx<T> *wt) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->go) return;
-  const int i = blockIdx.x, ns = a.ns;
+  const int i = ring_of_block(a), ns = a.ns, nr = a.nr;
   cx<T> *A = (cx<T> *)smem, *B = A + fpsz(ns);
-  const cx<T> *w = wrow + (long)i * ns;
-  for (int j = threadIdx.x; j < ns; j += blockDim.x) A[fp(j)] = w[j];
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) A[fp(j)] = wt[(long)j * nr + i];
   __syncthreads();
   cx<T> *r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, +1);
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
@@ -358,25 +340,16 @@ template <typename K> static void allow_lds(K kernel, size_t bytes) {
 static size_t rows_lds_bytes(const KArgs &a, size_t csz) { return 2 * csz * fpsz(a.ns); }
 static size_t cols_lds_bytes(const KArgs &a, size_t csz) { return 4 * csz * fpsz(2 * a.nr); }
 
-// Z (row-major, sg_z) -> Z^T (sg_w) -> pair transforms -> W^T (sg_z) -> W
-// (row-major, sg_w) -> rows_inv
+// rows_fwd: Z^T (sg_z) -> pair transforms -> W^T (sg_w) -> rows_inv
 template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
   const size_t rows_lds = rows_lds_bytes(a, sizeof(cx<T>)), cols_lds = cols_lds_bytes(a, sizeof(cx<T>));
   cx<T> *z = (cx<T> *)a.sg_z, *w = (cx<T> *)a.sg_w;
   switch (id) {
     case KID_SG_ROWS_FWD:
-      hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force);
-      break;
-    case KID_SG_TRANSPOSE_Z:
-      hipLaunchKernelGGL(k_sg_transpose<T>, dim3((a.ns + TP_T - 1) / TP_T, (a.nr + TP_T - 1) / TP_T),
-                         dim3(TP_T, 8), 0, s, a, force, (const cx<T> *)z, w, a.nr, a.ns);
+      hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force, z);
       break;
     case KID_SG_COLS:
-      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(256), cols_lds, s, a, force, (const cx<T> *)w, z);
-      break;
-    case KID_SG_TRANSPOSE_W:
-      hipLaunchKernelGGL(k_sg_transpose<T>, dim3((a.nr + TP_T - 1) / TP_T, (a.ns + TP_T - 1) / TP_T),
-                         dim3(TP_T, 8), 0, s, a, force, (const cx<T> *)z, w, a.ns, a.nr);
+      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(256), cols_lds, s, a, force, (const cx<T> *)z, w);
       break;
     case KID_SG_ROWS_INV:
       hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force, (const cx<T> *)w);
@@ -397,7 +370,7 @@ template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t 
   sg_pipeline<T>(id, a, s, 0);
 }
 template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s) {
-  for (int id : {KID_SG_ROWS_FWD, KID_SG_TRANSPOSE_Z, KID_SG_COLS, KID_SG_TRANSPOSE_W, KID_SG_ROWS_INV})
+  for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV})
     sg_pipeline<T>(id, a, s, 1);
 }
 template <typename T>

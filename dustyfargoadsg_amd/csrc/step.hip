@@ -13,7 +13,7 @@
 #include "kernels.h"
 
 const char *kKernelNames[KID_COUNT] = {
-    "sg_rows_fwd", "sg_transpose_z", "sg_cols", "sg_transpose_w", "sg_rows_inv",
+    "sg_rows_fwd", "sg_cols", "sg_rows_inv",
     "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
     "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
     "dust_sort",     "dust_deposit", "dust_reorder"};
@@ -75,9 +75,31 @@ template <typename T> __device__ __forceinline__ void st_sr(void *p, long i, dou
 // salts of the stochastically rounded stores
 enum { SR_VR = 1, SR_VT, SR_E, SR_EN, SR_RAD = 8, SR_AZI = 16 };
 
-// arithmetic type of the transport stencils for a storage type
-template <typename T> struct TransportArith { using type = T; };
-template <> struct TransportArith<float> { using type = double; };
+// a / b from the hardware reciprocal, two Newton steps and one correction
+// of the quotient: within an ulp or two of the IEEE quotient for operands
+// well inside the exponent range (no scaling steps)
+__device__ __forceinline__ double div_nr(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  const double q = a * r;
+  return fma(r, fma(-b, q, a), q);
+}
+
+// arithmetic type of the transport stencils for a storage type, and its
+// division: with FP64 storage the reference's IEEE double quotient; with
+// FP32 storage FP64 arithmetic whose quotients need not be correctly
+// rounded (their ulp is 2^-29 of the stored values' rounding)
+template <typename T> struct TransportArith {
+  using type = T;
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+};
+template <> struct TransportArith<float> {
+  using type = double;
+  static __device__ __forceinline__ double div(double a, double b) { return div_nr(a, b); }
+};
 
 // a / b in the arithmetic of the storage type: the IEEE quotient for
 // double; for float the hardware reciprocal (1 ulp) times a
@@ -222,14 +244,6 @@ struct CSum {
 // VGPRs), from K = 3 on 512 threads (256 VGPRs), so that the K-wide working
 // set stays in registers.
 #define RING_LB(K) ((K) == 1 ? 1024 : ((K) == 2 ? 768 : 512))
-
-// One block per ring; with xcd_remap the rings of one XCD are contiguous, so
-// the radial neighbours a block reads are in the same L2.
-__device__ __forceinline__ int ring_of_block(const KArgs &a) {
-  int b = blockIdx.x, n = gridDim.x;
-  if (a.xcd_remap && (n % 8 == 0)) return (b % 8) * (n / 8) + b / 8;
-  return b;
-}
 
 // AspectRatio (src/Viscosity.c:107-122) and compute_smoothing
 // (src/Force.c:255-263), LAMBDADOUBLING = 0
@@ -392,6 +406,9 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
     const R dxtheta = (R)(a.Rmed[i] * a.pm / (double)ns);
     const R vm = (R)vmoy, dxmin = (dxrad < dxtheta) ? dxrad : dxtheta;
     const R invdt5 = (viscosity != R(0)) ? qdiv(viscosity * R(4), (R)a.powmin2[i]) : R(1e-10);
+    // the ring's cell sizes divide every cell's terms: one reciprocal each
+    // (within an ulp of the quotients, which only moves dt by as much)
+    const R idxmin = R(1) / dxmin, idxrad = R(1) / dxrad, idxtheta = R(1) / dxtheta;
     R smax = R(0);
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -399,14 +416,14 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
       const R cs = a.energy ? (R)((const T *)a.cs)[base + j] : (R)a.cs_iso[i];
       const R vres = a.fast ? vt_own[k] - vm : vt_own[k];
       const R vrl = (R)vr[j];
-      const R invdt1 = qdiv(cs, dxmin);
-      const R invdt2 = qdiv(fabs(vrl), dxrad);
-      const R invdt3 = qdiv(fabs(vres), dxtheta);
+      const R invdt1 = cs * idxmin;
+      const R invdt2 = fabs(vrl) * idxrad;
+      const R invdt3 = fabs(vres) * idxtheta;
       R dvr = (R)vr[j + ns] - vrl;
       R dvt = (R)vt[jp] - vt_own[k];
       if (dvr >= R(0)) dvr = R(1e-10); else dvr = -dvr;
       if (dvt >= R(0)) dvt = R(1e-10); else dvt = -dvt;
-      const R q4 = qdiv(dvr, dxrad), q5 = qdiv(dvt, dxtheta);
+      const R q4 = dvr * idxrad, q5 = dvt * idxtheta;
       R invdt4 = (q5 > q4) ? q5 : q4;
       invdt4 *= R(4.0 * DF_CVNR * DF_CVNR);
       const R invdt6 = R(1e-10);
@@ -840,6 +857,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs
 #define SRC_SR (SRC_TR + 3)
 #define SRC_SC (SRC_TC + 3)
 #define SRC_FC (SRC_TC + 7)  // state staged on rings i0-4 .. i0+TR+1, columns j0-4 .. j0+TC+2
+#define SRC_TB 256
 
 template <typename T> __device__ __forceinline__ double src_press(const KArgs &a, long l, int i) {
   if (!a.energy) {
@@ -849,7 +867,7 @@ template <typename T> __device__ __forceinline__ double src_press(const KArgs &a
   return (a.gamma - 1.0) * ld<T>(a.en, l);
 }
 
-template <typename T> __global__ void __launch_bounds__(256) k_source(KArgs a) {
+template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a) {
   __shared__ double sPot[SRC_PR][SRC_PC];
   // VradInt / VthetaInt in FP64 whatever the storage type: their sub-step
   // increments are kept in full until the final, unbiased rounding
@@ -1151,20 +1169,42 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_ene
 #define RT_R 8
 
 // slope of ComputeStarRad at ring r (zero on the first and last ring)
-template <typename R>
+template <typename T, typename R = typename TransportArith<T>::type>
 __device__ __forceinline__ R rad_slope(const KArgs &a, int r, R qm, R q0, R qp) {
   if (r <= 0 || r >= a.nr - 1) return R(0);
   R dqm = (q0 - qm) * (R)a.InvDiffRmed[r];
   R dqp = (qp - q0) * (R)a.InvDiffRmed[r + 1];
-  return (dqp * dqm > R(0)) ? qdiv(R(2) * dqp * dqm, dqp + dqm) : R(0);
+  // branch-free: the quotient is formed on every lane and selected (where
+  // it is not used it may be an infinity or a NaN of a zero denominator,
+  // which the select discards; nothing traps)
+  const bool use = dqp * dqm > R(0);
+  const R d = TransportArith<T>::div(R(2) * dqp * dqm, dqp + dqm);
+  return use ? d : R(0);
 }
-// QRStar at face r (between rings r-1 and r) from the Work values of rings
-// r-1, r and the slopes there; zero on faces 0 and nr
+// The upwind side of face r (between rings r-1 and r) for velocity v and
+// the weight of its slope in QRStar: (Rmed[r]-Rmed[r-1]-v dt)/2 from ring
+// r-1 when v > 0, else -(Rmed[r+1]-Rmed[r]+v dt)/2 from ring r (the same
+// products and sums as the reference's two branches); shared by every
+// quantity at the face.  Faces 0 and nr have no star (open = false).
+template <typename R> struct RadFace {
+  bool open, up;
+  R w;
+};
+template <typename R> __device__ __forceinline__ RadFace<R> rad_face(const KArgs &a, int r, R v, R dt) {
+  RadFace<R> f;
+  f.open = (r > 0 && r < a.nr);
+  f.up = v > R(0);
+  const int rc = f.open ? r : 1; // keeps the table reads in range on closed faces
+  const R wu = ((R)a.Rmed[rc] - (R)a.Rmed[rc - 1] - v * dt) * R(0.5);
+  const R wd = ((R)a.Rmed[rc + 1] - (R)a.Rmed[rc] + v * dt) * R(0.5);
+  f.w = f.up ? wu : -wd;
+  return f;
+}
+// QRStar at face r from the Work values of rings r-1, r and the slopes there
 template <typename R>
-__device__ __forceinline__ R rad_star(const KArgs &a, int r, R v, R dt, R wm, R w0, R dqm, R dq0) {
-  if (r <= 0 || r >= a.nr) return R(0);
-  if (v > R(0)) return wm + ((R)a.Rmed[r] - (R)a.Rmed[r - 1] - v * dt) * R(0.5) * dqm;
-  return w0 - ((R)a.Rmed[r + 1] - (R)a.Rmed[r] + v * dt) * R(0.5) * dq0;
+__device__ __forceinline__ R rad_star(const RadFace<R> &f, R wm, R w0, R dqm, R dq0) {
+  const R s = (f.up ? wm : w0) + f.w * (f.up ? dqm : dq0);
+  return f.open ? s : R(0);
 }
 // the six transported quantities of cell (r, j) before the pass
 template <typename T, typename R>
@@ -1203,7 +1243,7 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
     R Q[6];
     rad_quantities<T, R>(a, r, jc, Of, Q);
     // the Work values Q / (rho + 1e-20) share one reciprocal per cell
-    const R inv = qdiv(R(1), Q[5] + R(1e-20));
+    const R inv = TransportArith<T>::div(R(1), Q[5] + R(1e-20));
 #pragma unroll
     for (int k = 0; k < 6; k++) sW[k][q][tx] = Q[k] * inv;
     sW[6][q][tx] = Q[5];
@@ -1213,7 +1253,7 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
   for (int q = ty; q < RT_R + 2; q += RT_R) {
     const int r = i0 - 1 + q;
 #pragma unroll
-    for (int k = 0; k < 7; k++) sD[k][q][tx] = rad_slope<R>(a, r, sW[k][q][tx], sW[k][q + 1][tx], sW[k][q + 2][tx]);
+    for (int k = 0; k < 7; k++) sD[k][q][tx] = rad_slope<T>(a, r, sW[k][q][tx], sW[k][q + 1][tx], sW[k][q + 2][tx]);
   }
   __syncthreads();
   const int i = i0 + ty;
@@ -1222,9 +1262,9 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
   const R v0 = (R)((const T *)a.vri)[l], v1 = (R)((const T *)a.vri)[l + ns];
   // stars at faces i (Work of rings i-1, i: slots ty+1, ty+2; slopes of
   // rings i-1, i: slots ty, ty+1) and i+1
-  const R rs0 = rad_star<R>(a, i, v0, dt, sW[6][ty + 1][tx], sW[6][ty + 2][tx], sD[6][ty][tx], sD[6][ty + 1][tx]);
-  const R rs1 =
-      rad_star<R>(a, i + 1, v1, dt, sW[6][ty + 2][tx], sW[6][ty + 3][tx], sD[6][ty + 1][tx], sD[6][ty + 2][tx]);
+  const RadFace<R> f0 = rad_face<R>(a, i, v0, dt), f1 = rad_face<R>(a, i + 1, v1, dt);
+  const R rs0 = rad_star<R>(f0, sW[6][ty + 1][tx], sW[6][ty + 2][tx], sD[6][ty][tx], sD[6][ty + 1][tx]);
+  const R rs1 = rad_star<R>(f1, sW[6][ty + 2][tx], sW[6][ty + 3][tx], sD[6][ty + 1][tx], sD[6][ty + 2][tx]);
   R Q[6];
   rad_quantities<T, R>(a, i, j, Of, Q);
   const R fin = dt * dtheta * (R)a.Rinf[i], fout = dt * dtheta * (R)a.Rsup[i], invs = (R)a.InvSurf[i];
@@ -1232,9 +1272,8 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     if (k == 4 && !a.energy) continue;
-    const R q0 = rad_star<R>(a, i, v0, dt, sW[k][ty + 1][tx], sW[k][ty + 2][tx], sD[k][ty][tx], sD[k][ty + 1][tx]);
-    const R q1 =
-        rad_star<R>(a, i + 1, v1, dt, sW[k][ty + 2][tx], sW[k][ty + 3][tx], sD[k][ty + 1][tx], sD[k][ty + 2][tx]);
+    const R q0 = rad_star<R>(f0, sW[k][ty + 1][tx], sW[k][ty + 2][tx], sD[k][ty][tx], sD[k][ty + 1][tx]);
+    const R q1 = rad_star<R>(f1, sW[k][ty + 2][tx], sW[k][ty + 3][tx], sD[k][ty + 1][tx], sD[k][ty + 2][tx]);
     // VanLeerRadial (src/TransportEuler.c:205-217)
     R varq = fin * q0 * rs0 * v0;
     varq -= fout * q1 * rs1 * v1;
@@ -1269,7 +1308,7 @@ __device__ __forceinline__ void ring_halo(R *buf, const R (&X)[K], int j0, int n
 // ComputeStarTheta (src/TransportEuler.c:419-444) at the cells j0 .. j0+K
 // (the K own cells and the first cell of the next thread) from the own
 // values X and the halo H = {X[j0-2], X[j0-1], X[j0+K], X[j0+K+1]}.
-template <typename R, int K>
+template <typename T, int K, typename R = typename TransportArith<T>::type>
 __device__ __forceinline__ void azi_star(const R (&X)[K], const R (&H)[4], const R (&v)[K + 1], R dt, R dxtheta,
                                          R invdxtheta, R (&S)[K + 1]) {
   // values at cells j0-2 .. j0+K+1 (offset 2)
@@ -1285,7 +1324,7 @@ __device__ __forceinline__ void azi_star(const R (&X)[K], const R (&H)[4], const
 #pragma unroll
   for (int c = 0; c < K + 2; c++) {
     const R dqm = x[c + 1] - x[c], dqp = x[c + 2] - x[c + 1];
-    dq[c] = (dqp * dqm > R(0)) ? qdiv(dqp * dqm, dqp + dqm) * invdxtheta : R(0);
+    dq[c] = (dqp * dqm > R(0)) ? TransportArith<T>::div(dqp * dqm, dqp + dqm) * invdxtheta : R(0);
   }
 #pragma unroll
   for (int c = 0; c < K + 1; c++) {
@@ -1298,7 +1337,7 @@ __device__ __forceinline__ void azi_star(const R (&X)[K], const R (&H)[4], const
 // QuantitiesAdvection (src/TransportEuler.c:375-400) with velocity v at the
 // cells j0 .. j0+K: the density star first, then every quantity's star,
 // flux and update.  Q[5] is the density; RI is its value before the pass.
-template <typename R, int K>
+template <typename T, int K, typename R = typename TransportArith<T>::type>
 __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int j0, R (&Q)[6][K], const R (&v)[K + 1],
                                          R dt, R dxtheta, R invdxtheta, R dxrad, R invsurf) {
   const int ns = a.ns;
@@ -1307,11 +1346,11 @@ __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int 
   for (int k = 0; k < K; k++) RI[k] = Q[5][k];
   ring_halo<R, K>(lds + flip * ns, RI, j0, ns, H);
   flip ^= 1;
-  azi_star<R, K>(RI, H, v, dt, dxtheta, invdxtheta, RS);
+  azi_star<T, K>(RI, H, v, dt, dxtheta, invdxtheta, RS);
   // the Work values Q / (rho + 1e-20) share one reciprocal per cell
   R inv[K];
 #pragma unroll
-  for (int k = 0; k < K; k++) inv[k] = qdiv(R(1), RI[k] + R(1e-20));
+  for (int k = 0; k < K; k++) inv[k] = TransportArith<T>::div(R(1), RI[k] + R(1e-20));
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     if (q == 4 && !a.energy) continue;
@@ -1320,7 +1359,7 @@ __device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int 
     for (int k = 0; k < K; k++) W[k] = Q[q][k] * inv[k];
     ring_halo<R, K>(lds + flip * ns, W, j0, ns, H);
     flip ^= 1;
-    azi_star<R, K>(W, H, v, dt, dxtheta, invdxtheta, S);
+    azi_star<T, K>(W, H, v, dt, dxtheta, invdxtheta, S);
 #pragma unroll
     for (int c = 0; c < K + 1; c++) f[c] = dxrad * S[c] * RS[c] * v[c];
 #pragma unroll
@@ -1393,13 +1432,13 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_tra
   R *ring = (R *)lds;
   int flip = 0;
   // residual pass (the only pass of the standard transport)
-  azi_pass<R, K>(a, ring, flip, j0, Q, vres, rdt, dxtheta, invdxtheta, dxrad, invsurf);
+  azi_pass<T, K>(a, ring, flip, j0, Q, vres, rdt, dxtheta, invdxtheta, dxrad, invsurf);
   // uniform pass with the constant residual
   if (a.fast) {
     R uu[K + 1];
 #pragma unroll
     for (int k = 0; k < K + 1; k++) uu[k] = (R)u;
-    azi_pass<R, K>(a, ring, flip, j0, Q, uu, rdt, dxtheta, invdxtheta, dxrad, invsurf);
+    azi_pass<T, K>(a, ring, flip, j0, Q, uu, rdt, dxtheta, invdxtheta, dxrad, invsurf);
   }
   // AdvectSHIFT (src/TransportEuler.c:350-373): new[j] = old[j - Nshift]
   long sh_mod = nshift % ns;
@@ -1562,7 +1601,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
     case KID_CFL: RING_DISPATCH(k_cfl_ring, T, a, s, 0); break;
     case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
     case KID_SOURCE:
-      hipLaunchKernelGGL(k_source<T>, dim3((a.ns + SRC_TC - 1) / SRC_TC, (a.nr + SRC_TR - 1) / SRC_TR), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_source<T>, dim3((a.ns + SRC_TC - 1) / SRC_TC, (a.nr + SRC_TR - 1) / SRC_TR), dim3(SRC_TB), 0, s,
                          a);
       break;
     case KID_ENERGY: RING_DISPATCH(k_energy, T, a, s, 0); break;
