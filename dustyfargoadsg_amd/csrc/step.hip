@@ -859,6 +859,23 @@ template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs
 #define SRC_FC (SRC_TC + 7)  // state staged on rings i0-4 .. i0+TR+1, columns j0-4 .. j0+TC+2
 #define SRC_TB 256
 
+// the 1-D tables k_source reads, staged per tile
+enum { ST_IDRM, ST_RINF, ST_IRINF, ST_RMED, ST_SUPP, ST_IDXT, ST_NU, ST_RSUP, ST_IDRS, ST_IRMED, ST_N };
+__device__ __forceinline__ const double *src_table(const KArgs &a, int k) {
+  switch (k) {
+    case ST_IDRM: return a.InvDiffRmed;
+    case ST_RINF: return a.Rinf;
+    case ST_IRINF: return a.InvRinf;
+    case ST_RMED: return a.Rmed;
+    case ST_SUPP: return a.supp_torque;
+    case ST_IDXT: return a.InvDxTheta;
+    case ST_NU: return a.nu_ring;
+    case ST_RSUP: return a.Rsup;
+    case ST_IDRS: return a.InvDiffRsup;
+    default: return a.InvRmed;
+  }
+}
+
 template <typename T> __device__ __forceinline__ double src_press(const KArgs &a, long l, int i) {
   if (!a.energy) {
     const double cs = (double)(T)a.cs_iso[i];
@@ -878,17 +895,52 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
   __shared__ T sRho[SRC_PR][SRC_FC], sVth[SRC_PR][SRC_FC];
   __shared__ double sP[SRC_PR][SRC_FC];
   __shared__ int sCol[SRC_FC];
+  // the 1-D tables on the staged rings (zero outside the grid)
+  __shared__ double sTab[ST_N][SRC_PR];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
-  const int ns = a.ns, nr = a.nr, tid = threadIdx.x, TB = blockDim.x;
+  constexpr int TB = SRC_TB;
+  const int ns = a.ns, nr = a.nr, tid = threadIdx.x;
   const int j0 = blockIdx.x * SRC_TC, i0 = blockIdx.y * SRC_TR;
   const double dt = c->dt;
+#define STAB(k, i) sTab[k][(i) - i0 + 4]
   // the particles of this sub-step were re-sorted into the other set
   if (a.d_on && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) a.ctrl->pset ^= 1;
+  // The global values stage 1 reads at its cells (rings i0-3 .. i0+SRC_TR+1,
+  // columns j0-3 .. j0+SRC_TC+1): loaded first, so that their latency
+  // overlaps the staging below.
+  constexpr int S1_N = SRC_VR * SRC_VC, S1_IT = (S1_N + TB - 1) / TB;
+  T pf_vr[S1_IT], pf_sgr[S1_IT], pf_sgrm[S1_IT], pf_sgt[S1_IT], pf_sgtm[S1_IT];
+#pragma unroll
+  for (int it = 0; it < S1_IT; it++) {
+    const int idx = tid + it * TB, q = idx / SRC_VC, p = idx - q * SRC_VC, i = i0 - 3 + q;
+    pf_vr[it] = pf_sgr[it] = pf_sgrm[it] = pf_sgt[it] = pf_sgtm[it] = (T)0;
+    if (idx < S1_N && i >= 0 && i <= nr - 1) {
+      int j = (j0 - 3 + p) % ns, jm = (j0 - 4 + p) % ns;
+      j = (j < 0) ? j + ns : j;
+      jm = (jm < 0) ? jm + ns : jm;
+      const long l = (long)i * ns + j;
+      if (i >= 1) {
+        pf_vr[it] = ((const T *)a.vr)[l];
+        if (a.sg) {
+          pf_sgr[it] = ((const T *)a.sgr)[l];
+          pf_sgrm[it] = ((const T *)a.sgr)[l - ns];
+        }
+      }
+      if (a.sg) {
+        pf_sgt[it] = ((const T *)a.sgt)[l];
+        pf_sgtm[it] = ((const T *)a.sgt)[(long)i * ns + jm];
+      }
+    }
+  }
   // columns j0-4 .. j0+SRC_TC+2 of the staged region, wrapped
   for (int p = tid; p < SRC_FC; p += TB) {
     int jj = (j0 - 4 + p) % ns;
     sCol[p] = (jj < 0) ? jj + ns : jj;
+  }
+  for (int idx = tid; idx < ST_N * SRC_PR; idx += TB) {
+    const int k = idx / SRC_PR, q = idx - k * SRC_PR, r = i0 - 4 + q;
+    sTab[k][q] = (r >= 0 && r <= nr - 1) ? src_table(a, k)[r] : 0.0;
   }
   __syncthreads();
   // stage 0: the state of rings i0-4 .. i0+SRC_TR+1 into LDS, and the
@@ -905,7 +957,10 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
   __syncthreads();
   // stage 1: SubStep1 on rings i0-3 .. i0+SRC_TR+1 (VradInt and VthetaInt
   // of ring nrad are zero, as in the reference where they are never set)
-  for (int idx = tid; idx < SRC_VR * SRC_VC; idx += TB) {
+#pragma unroll
+  for (int it = 0; it < S1_IT; it++) {
+    const int idx = tid + it * TB;
+    if (idx >= S1_N) break;
     const int q = idx / SRC_VC, p = idx - q * SRC_VC, i = i0 - 3 + q, j = sCol[p + 1];
     if (i < 0 || i > nr - 1) {
       sVr[q][p] = 0.0;
@@ -920,35 +975,33 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
     const bool dgrid = a.d_on && q >= 3 && q < 3 + SRC_TR && jt >= j0 && jt < j0 + SRC_TC && jt < ns;
     double vri = 0.0;
     if (i >= 1) {
-      const long lim = l - ns;
-      const double plim = sP[q][p + 1];
-      double gradp = (pl - plim) * 2.0 / (rl + (double)sRho[q][p + 1]) * a.InvDiffRmed[i];
-      double gradphi = (potl - sPot[q][p + 1]) * a.InvDiffRmed[i];
+      const double plim = sP[q][p + 1], idrm = STAB(ST_IDRM, i);
+      double gradp = (pl - plim) * 2.0 / (rl + (double)sRho[q][p + 1]) * idrm;
+      double gradphi = (potl - sPot[q][p + 1]) * idrm;
       double vt2 = (double)sVth[q + 1][p + 1] + (double)sVth[q + 1][p + 2] + (double)sVth[q][p + 1] +
                    (double)sVth[q][p + 2];
-      vt2 = vt2 / 4.0 + a.Rinf[i] * c->omega_frame;
+      vt2 = vt2 / 4.0 + STAB(ST_RINF, i) * c->omega_frame;
       vt2 = vt2 * vt2;
-      vri = ld<T>(a.vr, l) + dt * (-gradp - gradphi + vt2 * a.InvRinf[i]);
+      vri = (double)pf_vr[it] + dt * (-gradp - gradphi + vt2 * STAB(ST_IRINF, i));
       if (a.d_on && a.d_feedback) vri += dt * ld<T>(a.radfbacc, l);
       if (dgrid) {
-        st<T>(a.radgradp, l, (pl - plim) * a.InvDiffRmed[i]);
+        st<T>(a.radgradp, l, (pl - plim) * idrm);
         if (a.indirect_term)
-          st<T>(a.radindacc, l, -(indpot_at(a, c, i, j) - indpot_at(a, c, i - 1, j)) * a.InvDiffRmed[i]);
+          st<T>(a.radindacc, l, -(indpot_at(a, c, i, j) - indpot_at(a, c, i - 1, j)) * idrm);
       }
       if (a.sg) {
-        double radsgacc = ((a.Rinf[i] - a.Rmed[i - 1]) * ld<T>(a.sgr, l) +
-                           (a.Rmed[i] - a.Rinf[i]) * ld<T>(a.sgr, lim)) * a.InvDiffRmed[i];
+        double radsgacc = ((STAB(ST_RINF, i) - STAB(ST_RMED, i - 1)) * (double)pf_sgr[it] +
+                           (STAB(ST_RMED, i) - STAB(ST_RINF, i)) * (double)pf_sgrm[it]) * idrm;
         if (dgrid) st<T>(a.radsgacc, l, radsgacc);
         vri = vri + dt * radsgacc;
       }
     }
-    const long ljm = (long)i * ns + sCol[p];
-    const double invdxtheta = a.InvDxTheta[i];
+    const double invdxtheta = STAB(ST_IDXT, i);
     const double pljm = sP[q + 1][p];
     double gradp = (pl - pljm) * 2.0 / (rl + (double)sRho[q + 1][p]) * invdxtheta;
     double gradphi = (potl - sPot[q + 1][p]) * invdxtheta;
     double vti = (double)sVth[q + 1][p + 1] - dt * (gradp + gradphi);
-    vti += dt * a.supp_torque[i];
+    vti += dt * STAB(ST_SUPP, i);
     if (a.d_on && a.d_feedback) vti += dt * ld<T>(a.azifbacc, l);
     if (dgrid) {
       st<T>(a.azigradp, l, (pl - pljm) * invdxtheta);
@@ -956,7 +1009,7 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
         st<T>(a.aziindacc, l, -(indpot_at(a, c, i, j) - indpot_at(a, c, i, sCol[p])) * invdxtheta);
     }
     if (a.sg) {
-      const double azisgacc = 0.5 * (ld<T>(a.sgt, l) + ld<T>(a.sgt, ljm));
+      const double azisgacc = 0.5 * ((double)pf_sgt[it] + (double)pf_sgtm[it]);
       if (dgrid) st<T>(a.azisgacc, l, azisgacc);
       vti = vti + dt * azisgacc;
     }
@@ -973,21 +1026,21 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
         sTrr[q][p] = sTpp[q][p] = sTrp[q][p] = (T)0;
         continue;
       }
-      const double nu = a.nu_ring[i];
+      const double nu = STAB(ST_NU, i), irm = STAB(ST_IRMED, i), idrs = STAB(ST_IDRS, i);
       const double vrl = (double)sVr[q + 1][p + 1], vrp = (double)sVr[q + 2][p + 1];
       const double vtl = (double)sVt[q + 1][p + 1], vtp = (double)sVt[q + 1][p + 2];
-      double dv = (vrp * a.Rsup[i] - vrl * a.Rinf[i]) * a.InvDiffRsup[i] * a.InvRmed[i];
-      dv += (vtp - vtl) * a.invdphi * a.InvRmed[i];
-      const double Drr = (vrp - vrl) * a.InvDiffRsup[i];
-      const double Dpp = (vtp - vtl) * a.invdphi * a.InvRmed[i] + 0.5 * (vrp + vrl) * a.InvRmed[i];
+      double dv = (vrp * STAB(ST_RSUP, i) - vrl * STAB(ST_RINF, i)) * idrs * irm;
+      dv += (vtp - vtl) * a.invdphi * irm;
+      const double Drr = (vrp - vrl) * idrs;
+      const double Dpp = (vtp - vtl) * a.invdphi * irm + 0.5 * (vrp + vrl) * irm;
       const double rho = (double)sRho[q + 2][p + 2];
       sTrr[q][p] = (T)(2.0 * rho * nu * (Drr - onethird * dv));
       sTpp[q][p] = (T)(2.0 * rho * nu * (Dpp - onethird * dv));
       double Trp = 0.0;
       if (i >= 1) {
-        const double Drp = 0.5 * (a.Rinf[i] * (vtl * a.InvRmed[i] - (double)sVt[q][p + 1] * a.InvRmed[i - 1]) *
-                                      a.InvDiffRmed[i] +
-                                  (vrl - (double)sVr[q + 1][p]) * a.invdphi * a.InvRinf[i]);
+        const double Drp = 0.5 * (STAB(ST_RINF, i) * (vtl * irm - (double)sVt[q][p + 1] * STAB(ST_IRMED, i - 1)) *
+                                      STAB(ST_IDRM, i) +
+                                  (vrl - (double)sVr[q + 1][p]) * a.invdphi * STAB(ST_IRINF, i));
         Trp = 2.0 * 0.25 *
               (rho + (double)sRho[q + 1][p + 2] + (double)sRho[q + 2][p + 1] + (double)sRho[q + 1][p + 1]) * nu *
               Drp;
@@ -1005,8 +1058,8 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
         const double rho = (double)sRho[qv + 1][pv + 1];
         const double trpl = (double)sTrp[qs][ps], trpp = (double)sTrp[qs + 1][ps];
         double v = (double)sVt[qv][pv];
-        v += dt * a.InvRmed[i] *
-             ((a.Rsup[i] * trpp - a.Rinf[i] * trpl) * a.InvDiffRsup[i] +
+        v += dt * STAB(ST_IRMED, i) *
+             ((STAB(ST_RSUP, i) * trpp - STAB(ST_RINF, i) * trpl) * STAB(ST_IDRS, i) +
               ((double)sTpp[qs][ps] - (double)sTpp[qs][ps - 1]) * a.invdphi + 0.5 * (trpl + trpp)) /
              (0.5 * (rho + (double)sRho[qv + 1][pv]));
         sVt[qv][pv] = v;
@@ -1018,8 +1071,9 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
       if (i >= 1 && i <= nr - 1) {
         const double rho = (double)sRho[qv + 1][pv + 1];
         double v = (double)sVr[qv][pv];
-        v += dt * a.InvRinf[i] *
-             ((a.Rmed[i] * (double)sTrr[qs][ps] - a.Rmed[i - 1] * (double)sTrr[qs - 1][ps]) * a.InvDiffRmed[i] +
+        v += dt * STAB(ST_IRINF, i) *
+             ((STAB(ST_RMED, i) * (double)sTrr[qs][ps] - STAB(ST_RMED, i - 1) * (double)sTrr[qs - 1][ps]) *
+                  STAB(ST_IDRM, i) +
               ((double)sTrp[qs][ps + 1] - (double)sTrp[qs][ps]) * a.invdphi -
               0.5 * ((double)sTpp[qs][ps] + (double)sTpp[qs - 1][ps])) /
              (0.5 * (rho + (double)sRho[qv][pv + 1]));
@@ -1050,14 +1104,14 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
       const double rhoim = (double)sRho[qv][pv + 1];
       dv = vrl - (double)sVr[qv - 1][pv];
       const double qrm = (dv < 0.0) ? C2 * rhoim * dv * dv : 0.0;
-      vrn = vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * a.InvDiffRmed[i];
+      vrn = vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * STAB(ST_IDRM, i);
     }
-    const double invdxtheta = a.InvDxTheta[i];
+    const double invdxtheta = STAB(ST_IDXT, i);
     st_sr<T>(a.vri, l, vrn, sr_key(c, l, SR_VR));
     st_sr<T>(a.vti, l, vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta, sr_key(c, l, SR_VT));
     if (a.energy) {
       double e = ld<T>(a.en, l);
-      e = e - dt * qr * (vrp - vrl) * a.InvDiffRsup[i] - dt * qt * (vtp - vtl) * invdxtheta;
+      e = e - dt * qr * (vrp - vrl) * STAB(ST_IDRS, i) - dt * qt * (vtp - vtl) * invdxtheta;
       st_sr<T>(a.e_int, l, e, sr_key(c, l, SR_E));
     }
     if (i == nr - 1) {
@@ -1067,6 +1121,7 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
     }
   }
 }
+#undef STAB
 
 // Energy source terms, per ring: ComputeDivergenceVelocity
 // (src/SourceEuler.c:2373-2411), the stresses and ComputeViscousHeating
