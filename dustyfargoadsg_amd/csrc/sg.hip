@@ -15,6 +15,7 @@
 
 #include "kernels.h"
 
+
 template <typename T> struct cx {
   T x, y;
 };
@@ -186,6 +187,10 @@ __device__ __forceinline__ void fft_lds2(cx<T> *A0, cx<T> *B0, cx<T> *A1, cx<T> 
   *r1 = in1;
 }
 
+// The transform kernels take up to SG_TB_MAX threads; sg_threads picks the
+// block size at launch.
+#define SG_TB_MAX 1024
+
 template <typename T> __device__ __forceinline__ double ldd(const void *p, long i) {
   return (double)((const T *)p)[i];
 }
@@ -196,7 +201,7 @@ template <typename T> __device__ __forceinline__ double ldd(const void *p, long 
 // contiguous columns.  The rings of one XCD are contiguous (ring_of_block):
 // the 16 (FP32) or 8 (FP64) rings that share a line of zt are written in
 // one L2, which merges the lines before they go to HBM.
-template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs a, int force, cx<T> *zt) {
+template <typename T> __global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_fwd(KArgs a, int force, cx<T> *zt) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->go) return;
   const int i = ring_of_block(a), ns = a.ns, nr = a.nr;
@@ -224,7 +229,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_fwd(KArgs
 //      A_r + i A_t,
 //   4. inverse-transforms both columns and keeps rows < Nrad.
 template <typename T>
-__global__ void __launch_bounds__(256) k_sg_cols(KArgs a, int force, const cx<T> *zt, cx<T> *wt) {
+__global__ void __launch_bounds__(SG_TB_MAX) k_sg_cols(KArgs a, int force, const cx<T> *zt, cx<T> *wt) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->go) return;
   const int ns = a.ns, nr = a.nr, L = 2 * nr, p = blockIdx.x, S = fpsz(L);
@@ -287,7 +292,7 @@ __global__ void __launch_bounds__(256) k_sg_cols(KArgs a, int force, const cx<T>
 // inverse azimuthal transform, normalisation and self-force compensation
 // (src/sgacc.c:137-161); W stored transposed ([Nsec][Nrad], wt), read
 // with the rings of one XCD contiguous as in k_sg_rows_fwd
-template <typename T> __global__ void __launch_bounds__(256) k_sg_rows_inv(KArgs a, int force, const cx<T> *wt) {
+template <typename T> __global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_inv(KArgs a, int force, const cx<T> *wt) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->go) return;
   const int i = ring_of_block(a), ns = a.ns, nr = a.nr;
@@ -340,19 +345,32 @@ template <typename K> static void allow_lds(K kernel, size_t bytes) {
 static size_t rows_lds_bytes(const KArgs &a, size_t csz) { return 2 * csz * fpsz(a.ns); }
 static size_t cols_lds_bytes(const KArgs &a, size_t csz) { return 4 * csz * fpsz(2 * a.nr); }
 
+// Block size of a transform kernel using lds bytes per workgroup: the LDS
+// lets 160 KiB / lds workgroups share a CU, and together they should hold
+// about 16 wavefronts, so that the SIMDs stay busy through the barriers of
+// the passes (256 .. SG_TB_MAX threads, a power of two).
+static int sg_threads(size_t lds) {
+  const size_t per_cu = 160 * 1024;
+  const int groups = (int)(per_cu / (lds ? lds : 1));
+  const int want = 16 * 64 / (groups > 0 ? groups : 1);
+  int t = 256;
+  while (t < want && t < SG_TB_MAX) t *= 2;
+  return t;
+}
+
 // rows_fwd: Z^T (sg_z) -> pair transforms -> W^T (sg_w) -> rows_inv
 template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
   const size_t rows_lds = rows_lds_bytes(a, sizeof(cx<T>)), cols_lds = cols_lds_bytes(a, sizeof(cx<T>));
   cx<T> *z = (cx<T> *)a.sg_z, *w = (cx<T> *)a.sg_w;
   switch (id) {
     case KID_SG_ROWS_FWD:
-      hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force, z);
+      hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(sg_threads(rows_lds)), rows_lds, s, a, force, z);
       break;
     case KID_SG_COLS:
-      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(256), cols_lds, s, a, force, (const cx<T> *)z, w);
+      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(sg_threads(cols_lds)), cols_lds, s, a, force, (const cx<T> *)z, w);
       break;
     case KID_SG_ROWS_INV:
-      hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(256), rows_lds, s, a, force, (const cx<T> *)w);
+      hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(sg_threads(rows_lds)), rows_lds, s, a, force, (const cx<T> *)w);
       break;
   }
 }
