@@ -201,6 +201,9 @@ def main():
     ap.add_argument("--nsec", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device three times (the known-size "
+                         "reference for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--error-orbits", type=float, default=1.0,
                     help="orbits over which the FP32 error against FP64 is measured (0: skip)")
     args = ap.parse_args()
@@ -208,6 +211,17 @@ def main():
         print("bench.py: this build runs one disc on one GPU; multi-GPU runs are not "
               "implemented (the ring-slab sharding is a design only)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = 1 << 26  # 64 Mi floats = 256 MiB
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"copied 3 x {n * 4} bytes")
+        return
     from dustyfargoadsg_amd import _capi
     from dustyfargoadsg_amd.workloads import WORKLOADS
     kw = {}
