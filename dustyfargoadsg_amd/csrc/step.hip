@@ -1629,9 +1629,20 @@ static int ring_k(int ns) {
 
 int ring_kcheck(int ns) { return ring_k(ns); }
 
-#define RING_DISPATCH(KERNEL, T, a, s, lds, ...)                                            \
+// Six cells per thread (four waves per ring, two rings per CU) for the
+// azimuthal transport, which recomputes two slopes at each end of a
+// thread's K cells, and for the CFL kernel, whose block reductions then
+// run over fewer waves: measured faster than two or three cells per thread
+// at more waves.  Rings of fewer than 1536 cells keep the smallest K.
+static int wide_k(int ns) {
+  if (ns % 6 == 0 && ns / 6 >= 256 && ns / 6 <= RING_LB(6)) return 6;
+  return ring_k(ns);
+}
+
+#define RING_DISPATCH(KERNEL, T, a, s, lds, ...) RING_DISPATCH_K(ring_k((a).ns), KERNEL, T, a, s, lds, ##__VA_ARGS__)
+#define RING_DISPATCH_K(KSEL, KERNEL, T, a, s, lds, ...)                                    \
   do {                                                                                      \
-    int K_ = ring_k((a).ns);                                                                \
+    int K_ = (KSEL);                                                                        \
     dim3 g_((a).nr), b_((a).ns / K_);                                                       \
     switch (K_) {                                                                           \
       case 1: hipLaunchKernelGGL((KERNEL<T, 1>), g_, b_, lds, s, a, ##__VA_ARGS__); break;  \
@@ -1653,7 +1664,7 @@ static dim3 cell_block(const KArgs &a) { return dim3(a.ns < 256 ? a.ns : 256); }
 
 template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) {
   switch (id) {
-    case KID_CFL: RING_DISPATCH(k_cfl_ring, T, a, s, 0); break;
+    case KID_CFL: RING_DISPATCH_K(wide_k(a.ns), k_cfl_ring, T, a, s, 0); break;
     case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
     case KID_SOURCE:
       hipLaunchKernelGGL(k_source<T>, dim3((a.ns + SRC_TC - 1) / SRC_TC, (a.nr + SRC_TR - 1) / SRC_TR), dim3(SRC_TB), 0, s,
@@ -1667,7 +1678,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
     }
     case KID_TRANSPORT_AZI: {
       size_t lds = sizeof(typename TransportArith<T>::type) * 2 * (size_t)a.ns + sizeof(double) * 1024;
-      RING_DISPATCH(k_transport_azi, T, a, s, lds);
+      RING_DISPATCH_K(wide_k(a.ns), k_transport_azi, T, a, s, lds);
       break;
     }
     case KID_FINISH:

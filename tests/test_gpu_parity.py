@@ -59,7 +59,9 @@ def compare(o, d, p, tol_rho, tol_vr, tol_vt, tol_e=None):
     return errs, go, oo
 
 
-@pytest.mark.parametrize("nr,ns,calls", [(32, 96, 3), (128, 384, 2)])
+# the 1536- and 3072-cell rings take the six-cells-per-thread launches of the
+# ring kernels (K = 6) that the full-size discs use
+@pytest.mark.parametrize("nr,ns,calls", [(32, 96, 3), (128, 384, 2), (16, 1536, 1), (16, 3072, 1)])
 def test_gas_planet_fp64(nr, ns, calls):
     p, pl = gas_planet(nrad=nr, nsec=ns)
     o, so, d, sg = pair(p, pl)
@@ -81,7 +83,7 @@ def test_gas_planet_fp64(nr, ns, calls):
     assert abs(go["planets"].y[0] - oo["planets"].y[0]) <= 1e-14
 
 
-@pytest.mark.parametrize("nr,ns,calls", [(32, 96, 2), (64, 192, 1)])
+@pytest.mark.parametrize("nr,ns,calls", [(32, 96, 2), (64, 192, 1), (16, 1536, 1), (16, 3072, 1)])
 def test_energy_sg_fp64(nr, ns, calls):
     p, pl = energy_sg(nrad=nr, nsec=ns)
     o, so, d, sg = pair(p, pl)
