@@ -49,25 +49,16 @@ __device__ __forceinline__ unsigned sr_key(const dfargo_ctrl *c, long l, unsigne
 }
 template <typename T> __device__ __forceinline__ T sr_round(double v, unsigned key);
 template <> __device__ __forceinline__ double sr_round<double>(double v, unsigned) { return v; }
+// The 29 mantissa bits of the double below the float's last bit are the
+// distance to the float of smaller magnitude, in units of 2^-29 ulp: adding
+// 29 random bits (the top bits of the key) and clearing them carries into
+// the float's last bit with exactly that probability, and the remaining
+// double converts to float exactly (for normal floats; exact floats, zeros,
+// infinities and NaNs come out unchanged).
 template <> __device__ __forceinline__ float sr_round<float>(double v, unsigned key) {
-  const float f = (float)v;
-  const double df = (double)f;
-  if (df == v || !(fabs(v) < 3.0e38)) return f;
-  const unsigned u = __float_as_uint(f);
-  float lo, hi;
-  if (df < v) {
-    lo = f;
-    hi = (f == 0.0f) ? __uint_as_float(1u) : __uint_as_float(f > 0.0f ? u + 1u : u - 1u);
-  } else {
-    hi = f;
-    lo = (f == 0.0f) ? -__uint_as_float(1u) : __uint_as_float(f > 0.0f ? u - 1u : u + 1u);
-  }
-  // hi - lo is a power of two: its reciprocal is exact, from the exponent
-  const double ulp = (double)hi - (double)lo;
-  const double inv = __longlong_as_double((2046LL - ((__double_as_longlong(ulp) >> 52) & 0x7ff)) << 52);
-  const double p_up = (v - (double)lo) * inv;
-  const double draw = (double)(mix32(key) >> 8) * (1.0 / 16777216.0);
-  return (draw < p_up) ? hi : lo;
+  const long long b = __double_as_longlong(v);
+  const long long t = (b + (long long)(key >> 3)) & ~0x1FFFFFFFLL;
+  return (float)__longlong_as_double(t);
 }
 template <typename T> __device__ __forceinline__ void st_sr(void *p, long i, double v, unsigned key) {
   ((T *)p)[i] = sr_round<T>(v, key);
