@@ -239,6 +239,9 @@ def load(path=None):
         "dfargo_attach_dust": (c_int, [vp, ctypes.POINTER(DustParams), ctypes.POINTER(DustState), c_double]),
         "dfargo_download_dust": (c_int, [vp, ctypes.POINTER(DustState)]),
         "dfargo_upload_dust": (c_int, [vp, ctypes.POINTER(DustState)]),
+        "dfargo_gas_budgets": (c_int, [vp, dp]),
+        "dfargo_update_log": (c_int, [vp, c_int, dp, dp, ctypes.POINTER(c_int)]),
+        "dfargo_orbital_elements": (c_int, [c_double] * 5 + [dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -259,4 +262,5 @@ EXPORTED = [
     "dfargo_set_profiling", "dfargo_get_kernel_times", "dfargo_stream",
     "dfargo_dust_default_params", "dfargo_check_dust_params", "dfargo_init_dust_system",
     "dfargo_attach_dust", "dfargo_download_dust", "dfargo_upload_dust",
+    "dfargo_gas_budgets", "dfargo_update_log", "dfargo_orbital_elements",
 ]

@@ -48,6 +48,8 @@ struct dfargo_ctx {
   std::vector<hipEvent_t> evpool;
   std::vector<double> VthetaMed, gcs_host;
   std::vector<float> fstage;
+  // per-DT diagnostics: ring partials and sums (allocated on first use)
+  double *diag_part = nullptr, *diag_out = nullptr;
   // dust particles (dfargo_attach_dust)
   int dust_on = 0;
   dfargo_dust_params dp;
@@ -730,6 +732,95 @@ extern "C" int dfargo_get_status(dfargo_ctx *c, dfargo_status *st) {
   st->crashed_density = k->crashed_d;
   st->crashed_energy = k->crashed_e;
   st->crashed = k->crashed_e;
+  return DFARGO_OK;
+}
+
+// ------------------------------------------------------------------------
+// per-DT diagnostics
+
+static int run_diagnostics(dfargo_ctx *c, const DiagBodies &b, std::vector<double> &out) {
+  int rc;
+  if (!c->diag_part) {
+    if ((rc = dalloc(c, (void **)&c->diag_part, sizeof(double) * DIAG_NPART(DF_MAXPL) * (size_t)c->p.nrad)) ||
+        (rc = dalloc(c, (void **)&c->diag_out, sizeof(double) * DIAG_NOUT(DF_MAXPL))))
+      return rc;
+  }
+  if (c->prec == DFARGO_FP64) launch_diagnostics<double>(c->a, b, c->diag_part, c->diag_out, c->stream);
+  else launch_diagnostics<float>(c->a, b, c->diag_part, c->diag_out, c->stream);
+  HIPCHK(hipGetLastError());
+  out.assign(DIAG_NOUT(b.n), 0.0);
+  HIPCHK(hipMemcpyAsync(out.data(), c->diag_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_gas_budgets(dfargo_ctx *c, double *out) {
+  if (!c || !out) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
+  HIPCHK(hipSetDevice(c->dev));
+  DiagBodies b;
+  memset(&b, 0, sizeof b);
+  std::vector<double> r;
+  int rc = run_diagnostics(c, b, r);
+  if (rc) return rc;
+  if ((rc = ctrl_pull(c))) return rc;
+  out[0] = r[0];
+  out[1] = r[1];
+  out[2] = r[2];
+  // CheckMomentumConservation's planet term (src/SideEuler.c:196-206)
+  const dfargo_ctrl *k = c->h_ctrl;
+  double planetsmom = 0.0;
+  for (int q = 0; q < k->nb; q++) {
+    const double xp = k->pl_x[q], yp = k->pl_y[q];
+    const double rpl = sqrt(xp * xp + yp * yp), thetapl = atan2(yp, xp);
+    const double vazimpl = -k->pl_vx[q] * sin(thetapl) + k->pl_vy[q] * cos(thetapl);
+    planetsmom += k->pl_m[q] * rpl * vazimpl;
+  }
+  out[3] = planetsmom;
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_update_log(dfargo_ctx *c, int max_planets, double *torques, double *indirect, int *nb) {
+  if (!c || !torques || !indirect || !nb) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
+  HIPCHK(hipSetDevice(c->dev));
+  int rc = ctrl_pull(c);
+  if (rc) return rc;
+  const dfargo_ctrl *k = c->h_ctrl;
+  if (k->nb > max_planets)
+    return dfargo_set_error(DFARGO_ERR_ARG, "%d planets, room for %d", k->nb, max_planets);
+  // UpdateLog's smoothing and ComputeForce's Hill radius (src/Force.c:
+  // 287-292, 118-119) on the host, in FP64
+  DiagBodies b;
+  memset(&b, 0, sizeof b);
+  b.n = k->nb;
+  for (int q = 0; q < k->nb; q++) {
+    const double x = k->pl_x[q], y = k->pl_y[q], m = k->pl_m[q], r = sqrt(x * x + y * y);
+    b.x[q] = x;
+    b.y[q] = y;
+    b.a[q] = sqrt(x * x + y * y);
+    b.rsm[q] = (c->p.roche_smoothing != 0.0)
+                   ? r * pow(m / 3., 1. / 3.) * c->p.roche_smoothing
+                   : c->p.thickness_smoothing * dfargo_aspect_ratio(&c->p, r) * pow(r, 1.0 + c->p.flaring_index);
+    b.rh[q] = pow(m / 3., 1. / 3.) * b.a[q] + 1e-15;
+  }
+  std::vector<double> r;
+  if ((rc = run_diagnostics(c, b, r))) return rc;
+  for (int q = 0; q < k->nb; q++) {
+    const double x = k->pl_x[q], y = k->pl_y[q], vx = k->pl_vx[q], vy = k->pl_vy[q];
+    const double *f = r.data() + 3 + 8 * q; // fx i, fx ex i, fx o, fx ex o, fy i, fy ex i, fy o, fy ex o
+    double *o = torques + 8 * q;
+    o[0] = x * f[4] - y * f[0];
+    o[1] = x * f[6] - y * f[2];
+    o[2] = x * f[5] - y * f[1];
+    o[3] = x * f[7] - y * f[3];
+    o[4] = vx * f[0] + vy * f[4];
+    o[5] = vx * f[2] + vy * f[6];
+    o[6] = vx * f[1] + vy * f[5];
+    o[7] = vx * f[3] + vy * f[7];
+    indirect[q] = -x * k->dop_y + y * k->dop_x;
+  }
+  *nb = k->nb;
   return DFARGO_OK;
 }
 

@@ -440,3 +440,29 @@ int dfargo_init_dust_system(const dfargo_params *p, const dfargo_dust_params *d,
   dfargo_free_tables(&t);
   return DFARGO_OK;
 }
+
+/* FindOrbitalElements (src/Planet.c:129-180), G = 1 */
+int dfargo_orbital_elements(double x, double y, double vx, double vy, double m, double *out) {
+  if (!out) return dfargo_set_error(DFARGO_ERR_ARG, "null output");
+  if (!(m > 0.0)) return dfargo_set_error(DFARGO_ERR_ARG, "mass %g is not positive", m);
+  const double h = x * vy - y * vx;
+  const double d = sqrt(x * x + y * y);
+  const double Ax = x * vy * vy - y * vx * vy - m * x / d;
+  const double Ay = y * vx * vx - x * vx * vy - m * y / d;
+  const double e = sqrt(Ax * Ax + Ay * Ay) / m;
+  const double a = h * h / m / (1 - e * e);
+  double E = (e != 0.0) ? acos((1.0 - d / a) / e) : 0.0;
+  if ((x * y * (vy * vy - vx * vx) + vx * vy * (x * x - y * y)) < 0) E = -E;
+  const double M = E - e * sin(E);
+  double V = (e != 0.0) ? acos((a * (1.0 - e * e) / d - 1.0) / e) : 0.0;
+  if (E < 0.0) V = -V;
+  const double lambda = atan2(y, x);
+  out[0] = e;
+  out[1] = a;
+  out[2] = M;
+  out[3] = V;
+  out[4] = (e != 0.0) ? atan2(Ay, Ax) : atan2(y, x);
+  out[5] = lambda;
+  out[6] = (e != 0.0) ? 2.0 * atan2(Ay, 1. + Ax) : lambda;
+  return DFARGO_OK;
+}

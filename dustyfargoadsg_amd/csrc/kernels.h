@@ -115,6 +115,18 @@ template <typename T> void launch_potential(const KArgs &a, hipStream_t s);
 template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s); // rows_fwd..rows_inv
 template <typename T> void launch_visc_stress_state(const KArgs &a, hipStream_t s);
 void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s);
+// per-DT diagnostics: the bodies whose disc force is logged (position,
+// distance to the star, smoothing length, Hill radius)
+struct DiagBodies {
+  int n;
+  double x[DF_MAXPL], y[DF_MAXPL], a[DF_MAXPL], rsm[DF_MAXPL], rh[DF_MAXPL];
+};
+#define DIAG_NPART(n) (3 + 4 * (n))
+#define DIAG_NOUT(n) (3 + 8 * (n))
+// ring partials into part (DIAG_NPART(DF_MAXPL) x nr doubles), sums into out
+// (DIAG_NOUT(b.n) doubles)
+template <typename T>
+void launch_diagnostics(const KArgs &a, const DiagBodies &b, double *part, double *out, hipStream_t s);
 int ring_kcheck(int ns); // threads-per-ring split used by the ring kernels, -1 if none
 
 // FFT utilities (sg.hip)

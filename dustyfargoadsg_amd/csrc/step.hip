@@ -1726,3 +1726,105 @@ template void launch_potential<double>(const KArgs &, hipStream_t);
 template void launch_potential<float>(const KArgs &, hipStream_t);
 template void launch_visc_stress_state<double>(const KArgs &, hipStream_t);
 template void launch_visc_stress_state<float>(const KArgs &, hipStream_t);
+
+// ------------------------------------------------------------------------
+// per-DT diagnostics (src/main.c:291-394): outside AlgoGas, on the state
+// the last call left
+// ------------------------------------------------------------------------
+
+// Per ring i, part[q * nr + i] for q = 0 GasTotalMass (src/SideEuler.c:
+// 15-36), 1 GasMomentum (src/SideEuler.c:114-146), 2 GasTotalEnergy
+// (src/SideEuler.c:149-180; the energy grid of an isothermal run holds
+// zeros), then for body k, 3 + 4k + {0 fx, 1 fx Hill-cut, 2 fy, 3 fy
+// Hill-cut} of ComputeForce (src/Force.c:155-214); the ring's inner/outer
+// side is decided in k_diag_reduce.  Compensated FP64 sums, fixed block
+// reduction order.
+template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_diag_ring(KArgs a, DiagBodies b,
+                                                                                       double *part) {
+  __shared__ double sh[1024];
+  const dfargo_ctrl *c = a.ctrl;
+  const int i = blockIdx.x, ns = a.ns, nr = a.nr, t = threadIdx.x, TB = blockDim.x;
+  const long base = (long)i * ns;
+  const double Rm = a.Rmed[i], S = a.Surf[i], Of = c->omega_frame;
+  const double wr = (a.Rsup[i] - a.Rinf[i]);
+  CSum m = {0, 0}, mom = {0, 0}, en = {0, 0};
+  for (int k = 0; k < K; k++) {
+    const int j = t + k * TB;
+    const long l = base + j;
+    const double d = ld<T>(a.rho, l);
+    const double vtl = ld<T>(a.vt, l), vtp = ld<T>(a.vt, (j < ns - 1) ? l + 1 : base);
+    const double vt_cent = 0.5 * (vtl + vtp) + Rm * Of;
+    double vr_cent = (Rm - a.Rinf[i]) * ld<T>(a.vr, l + ns) + (a.Rsup[i] - Rm) * ld<T>(a.vr, l);
+    vr_cent /= wr;
+    const double e = a.energy ? ld<T>(a.en, l) : 0.0;
+    m.add(S * d);
+    mom.add(S * d * Rm * vt_cent);
+    en.add(0.5 * S * d * (vr_cent * vr_cent + vt_cent * vt_cent) + S * e - S * d * pot_at(a, c, i, j));
+  }
+  double v[3] = {m.value(), mom.value(), en.value()};
+  block_sum_n<3>(v, sh);
+  if (t == 0)
+    for (int q = 0; q < 3; q++) part[(long)q * nr + i] = v[q];
+  for (int bd = 0; bd < b.n; bd++) {
+    const double x = b.x[bd], y = b.y[bd], rsm = b.rsm[bd], cutoff = a.exclude_hill_factor * b.rh[bd];
+    CSum f[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int k = 0; k < K; k++) {
+      const int j = t + k * TB;
+      const double xc = Rm * a.CosAz[j], yc = Rm * a.SinAz[j];
+      const double cellmass = S * ld<T>(a.rho, base + j);
+      const double dx = xc - x, dy = yc - y;
+      const double d2 = dx * dx + dy * dy;
+      const double planet_distance = sqrt(d2);
+      const double dist2 = d2 + rsm * rsm;
+      const double distance = sqrt(dist2);
+      const double InvDist3 = 1.0 / dist2 / distance;
+      double hill_cut;
+      if (planet_distance / cutoff < 0.5) hill_cut = 0.0;
+      else if (planet_distance > cutoff) hill_cut = 1.0;
+      else hill_cut = pow(sin((planet_distance / cutoff - .5) * DF_PI), 2.);
+      f[0].add(DF_G * cellmass * dx * InvDist3 * 1.0);
+      f[1].add(DF_G * cellmass * dx * InvDist3 * hill_cut);
+      f[2].add(DF_G * cellmass * dy * InvDist3 * 1.0);
+      f[3].add(DF_G * cellmass * dy * InvDist3 * hill_cut);
+    }
+    double w[4] = {f[0].value(), f[1].value(), f[2].value(), f[3].value()};
+    block_sum_n<4>(w, sh);
+    if (t == 0)
+      for (int q = 0; q < 4; q++) part[(long)(3 + 4 * bd + q) * nr + i] = w[q];
+  }
+}
+
+// Ring sums in ring order (compensated): out[0..2] the budgets, then for
+// body k out[3 + 8k ..] = fx inner, fx Hill-cut inner, fx outer, fx
+// Hill-cut outer, fy inner, fy Hill-cut inner, fy outer, fy Hill-cut outer
+// (GlobalForce's order; a ring is inner when Rmed < a_k).
+__global__ void __launch_bounds__(64) k_diag_reduce(KArgs a, DiagBodies b, const double *part, double *out) {
+  const int q = threadIdx.x, nr = a.nr;
+  if (q < 3) {
+    CSum s = {0, 0};
+    for (int i = 0; i < nr; i++) s.add(part[(long)q * nr + i]);
+    out[q] = s.value();
+  }
+  const int bd = q / 4, comp = q % 4;
+  if (bd < b.n) {
+    CSum in = {0, 0}, outer = {0, 0};
+    const double *pp = part + (long)(3 + 4 * bd + comp) * nr;
+    for (int i = 0; i < nr; i++) {
+      if (a.Rmed[i] < b.a[bd]) in.add(pp[i]);
+      else outer.add(pp[i]);
+    }
+    // comp: 0 fx, 1 fx Hill-cut, 2 fy, 3 fy Hill-cut -> GlobalForce slots
+    const int axis = comp / 2, hill = comp % 2;
+    out[3 + 8 * bd + 4 * axis + hill] = in.value();
+    out[3 + 8 * bd + 4 * axis + 2 + hill] = outer.value();
+  }
+}
+
+template <typename T> void launch_diagnostics(const KArgs &a, const DiagBodies &b, double *part, double *out,
+                                              hipStream_t s) {
+  RING_DISPATCH(k_diag_ring, T, a, s, 0, b, part);
+  hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(64), 0, s, a, b, (const double *)part, out);
+}
+template void launch_diagnostics<double>(const KArgs &, const DiagBodies &, double *, double *, hipStream_t);
+template void launch_diagnostics<float>(const KArgs &, const DiagBodies &, double *, double *, hipStream_t);
+

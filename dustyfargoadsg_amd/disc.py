@@ -155,6 +155,18 @@ class Disc:
         _check(self.lib, self.lib.dfargo_get_fargo_shift(self.ctx, ns, _ptr(vm)))
         return np.array(ns[:], dtype=np.int64), vm
 
+    def gas_budgets(self):
+        """GasTotalMass, GasMomentum, GasTotalEnergy, planets' angular momentum."""
+        out = np.zeros(4)
+        _check(self.lib, self.lib.dfargo_gas_budgets(self.ctx, _ptr(out)))
+        return out
+
+    def update_log(self):
+        """UpdateLog: (nb, 8) torques and powers, (nb,) indirect torques."""
+        tq, ind, nb = np.zeros(8 * 8), np.zeros(8), ctypes.c_int()
+        _check(self.lib, self.lib.dfargo_update_log(self.ctx, 8, _ptr(tq), _ptr(ind), ctypes.byref(nb)))
+        return tq[: 8 * nb.value].reshape(nb.value, 8), ind[: nb.value]
+
     def fviscosity(self, r):
         return self.lib.dfargo_fviscosity(self.ctx, r)
 
@@ -212,3 +224,10 @@ def init_dust_system(params, d, axi_sg_accr=None, lib=None):
                                             ctypes.byref(_capi.dust_state(arrs))))
     return {k: v[:n].copy() for k, v in arrs.items()}
 
+
+def orbital_elements(x, y, vx, vy, m, lib=None):
+    """FindOrbitalElements: e, a, M, V, PerihelionPA, lambda, varpi."""
+    lib = lib or _capi.load()
+    out = np.zeros(7)
+    _check(lib, lib.dfargo_orbital_elements(x, y, vx, vy, m, _ptr(out)))
+    return out

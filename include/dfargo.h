@@ -279,6 +279,30 @@ double dfargo_fviscosity(dfargo_ctx *ctx, double r);
  * ring-mean azimuthal velocities VMed[i] of the last sub-step (nrad each). */
 int dfargo_get_fargo_shift(dfargo_ctx *ctx, long *nshift, double *vmed);
 
+/* ---- per-DT diagnostics (src/main.c:291-394, between AlgoGas calls) -------- */
+
+/* The MonitorIntegral budgets of the device state, in FP64 on the device:
+ * out[0] GasTotalMass (src/SideEuler.c:15-36), out[1] GasMomentum
+ * (src/SideEuler.c:114-146), out[2] GasTotalEnergy (src/SideEuler.c:149-180,
+ * with the potential of the last sub-step; an isothermal run's energy grid
+ * is zero, as CreatePolarGrid leaves it), out[3] the planets' angular
+ * momentum of CheckMomentumConservation (src/SideEuler.c:182-210). */
+int dfargo_gas_budgets(dfargo_ctx *ctx, double *out);
+
+/* UpdateLog (src/Force.c:266-328): the disc force on each planet
+ * (ComputeForce, src/Force.c:77-253, FP64 on the device), turned into the
+ * columns of tqwk<k>.dat: torques[8k .. 8k+7] = x fy - y fx of the inner,
+ * outer, Hill-excluded inner and outer discs, then vx fx + vy fy of the same
+ * four; indirect[k] = the indirect torque of indtq<k>.dat from the last
+ * sub-step's DiskOnPrimaryAcceleration.  *nb receives the planet count
+ * (DFARGO_ERR_ARG if it exceeds max_planets). */
+int dfargo_update_log(dfargo_ctx *ctx, int max_planets, double *torques, double *indirect, int *nb);
+
+/* FindOrbitalElements (src/Planet.c:129-180) for a body at (x, y) moving
+ * with (vx, vy) about a total mass m: out = e, a, M, V, PerihelionPA,
+ * lambda, varpi (the columns of orbit<n>.dat after the time). */
+int dfargo_orbital_elements(double x, double y, double vx, double vy, double m, double *out);
+
 /* Per-kernel timing with HIP events on the context's stream.  While
  * enabled, sub-steps launch eagerly and every kernel is bracketed by a pair
  * of events.  names: up to max entries of at most 47 characters. */

@@ -25,6 +25,9 @@
  *   FFT             naive DFT on mixed-radix sizes
  *   RK5             Cash-Karp tableau identities, 5th-order convergence,
  *                   Kepler energy and angular momentum
+ *   diagnostics     budgets of a uniform Keplerian disc in closed form, a
+ *                   one-cell disc's force on a planet, Hill exclusion,
+ *                   orbital elements at pericentre and at E = pi/2
  */
 #include <math.h>
 #include <stdio.h>
@@ -2177,4 +2180,111 @@ void or_get_field(or_ctx *c, int field, double *out) {
 void or_get_fargo_shift(or_ctx *c, long *nshift, double *vmed) {
   if (nshift) memcpy(nshift, c->nshift, sizeof(long) * c->nr);
   if (vmed) memcpy(vmed, c->vmed, sizeof(double) * c->nr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-DT diagnostics (src/main.c:291-394, outside AlgoGas)                 */
+/* ------------------------------------------------------------------------ */
+
+/* out[0] GasTotalMass (src/SideEuler.c:15-36), out[1] GasMomentum
+ * (src/SideEuler.c:114-146), out[2] GasTotalEnergy (src/SideEuler.c:149-180;
+ * the energy grid of an isothermal run is CreatePolarGrid's zeros,
+ * src/LowTasks.c:108-113), out[3] the planets' angular momentum of
+ * CheckMomentumConservation (src/SideEuler.c:182-210).  The potential is
+ * the Potential grid of the last sub-step. */
+void or_gas_budgets(or_ctx *c, double out[4]) {
+  int nr = c->nr, ns = c->ns;
+  const double *density = c->rho, *vrad = c->vr, *vtheta = c->vt, *pot = c->pot;
+  double total = 0.0;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) total += c->Surf[i] * density[j + i * ns];
+  out[0] = total;
+  total = 0.0;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = i * ns + j;
+      double vt_cent;
+      if (j < ns - 1) vt_cent = 0.5 * (vtheta[l] + vtheta[l + 1]) + c->Rmed[i] * c->OmegaFrame;
+      else vt_cent = 0.5 * (vtheta[l] + vtheta[i * ns]) + c->Rmed[i] * c->OmegaFrame;
+      total += c->Surf[i] * density[l] * c->Rmed[i] * vt_cent;
+    }
+  out[1] = total;
+  total = 0.0;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      int l = j + i * ns;
+      double vr_cent = (c->Rmed[i] - c->Rinf[i]) * vrad[l + ns] + (c->Rsup[i] - c->Rmed[i]) * vrad[l];
+      vr_cent /= (c->Rsup[i] - c->Rinf[i]);
+      double vt_cent;
+      if (j < ns - 1) vt_cent = 0.5 * (vtheta[l] + vtheta[l + 1]) + c->Rmed[i] * c->OmegaFrame;
+      else vt_cent = 0.5 * (vtheta[l] + vtheta[i * ns]) + c->Rmed[i] * c->OmegaFrame;
+      double e = c->p.energy_equation ? c->en[l] : 0.0;
+      total += 0.5 * c->Surf[i] * density[l] * (vr_cent * vr_cent + vt_cent * vt_cent) + c->Surf[i] * e -
+               c->Surf[i] * density[l] * pot[l];
+    }
+  out[2] = total;
+  double planetsmom = 0.0;
+  for (int k = 0; k < c->sys.nb; k++) {
+    double xp = c->sys.x[k], yp = c->sys.y[k];
+    double rpl = sqrt(xp * xp + yp * yp), thetapl = atan2(yp, xp);
+    double vazimpl = -c->sys.vx[k] * sin(thetapl) + c->sys.vy[k] * cos(thetapl);
+    planetsmom += c->sys.mass[k] * rpl * vazimpl;
+  }
+  out[3] = planetsmom;
+}
+
+/* UpdateLog (src/Force.c:266-328): for planet k, tq[8k..8k+7] are the
+ * columns of tqwk<k>.dat (torques x fy - y fx and powers vx fx + vy fy of
+ * the inner, outer, Hill-excluded inner and outer discs) and ind[k] the
+ * indirect torque of indtq<k>.dat, from the DiskOnPrimaryAcceleration of
+ * the last sub-step.  Returns the number of planets. */
+int or_update_log(or_ctx *c, double *tq, double *ind) {
+  int nb = c->sys.nb;
+  for (int k = 0; k < nb; k++) {
+    double x = c->sys.x[k], y = c->sys.y[k], vx = c->sys.vx[k], vy = c->sys.vy[k];
+    double r = sqrt(x * x + y * y), m = c->sys.mass[k], smoothing, gf[8];
+    if (c->p.roche_smoothing != 0.0) smoothing = r * pow(m / 3., 1. / 3.) * c->p.roche_smoothing;
+    else smoothing = compute_smoothing(&c->p, r);
+    compute_force(c, c->rho, x, y, smoothing, m, gf);
+    /* gf: fx_inner, fx_ex_inner, fx_outer, fx_ex_outer, fy_inner, ... */
+    double fxi = gf[0], fxei = gf[1], fxo = gf[2], fxeo = gf[3];
+    double fyi = gf[4], fyei = gf[5], fyo = gf[6], fyeo = gf[7];
+    double *o = tq + 8 * k;
+    o[0] = x * fyi - y * fxi;
+    o[1] = x * fyo - y * fxo;
+    o[2] = x * fyei - y * fxei;
+    o[3] = x * fyeo - y * fxeo;
+    o[4] = vx * fxi + vy * fyi;
+    o[5] = vx * fxo + vy * fyo;
+    o[6] = vx * fxei + vy * fyei;
+    o[7] = vx * fxeo + vy * fyeo;
+    ind[k] = -x * c->dop.y + y * c->dop.x;
+  }
+  return nb;
+}
+
+/* FindOrbitalElements (src/Planet.c:129-180): out = e, a, M, V,
+ * PerihelionPA, lambda, varpi (the columns of orbit<n>.dat after the time) */
+void or_orbital_elements(double x, double y, double vx, double vy, double m, double out[7]) {
+  double Ax, Ay, e, d, h, a, E, M, V, lambda, varpi, PerihelionPA;
+  h = x * vy - y * vx;
+  d = sqrt(x * x + y * y);
+  Ax = x * vy * vy - y * vx * vy - G * m * x / d;
+  Ay = y * vx * vx - x * vx * vy - G * m * y / d;
+  e = sqrt(Ax * Ax + Ay * Ay) / G / m;
+  a = h * h / G / m / (1 - e * e);
+  if (e != 0.0) E = acos((1.0 - d / a) / e);
+  else E = 0.0;
+  if ((x * y * (vy * vy - vx * vx) + vx * vy * (x * x - y * y)) < 0) E = -E;
+  M = E - e * sin(E);
+  if (e != 0.0) V = acos((a * (1.0 - e * e) / d - 1.0) / e);
+  else V = 0.0;
+  if (E < 0.0) V = -V;
+  lambda = atan2(y, x);
+  if (e != 0.0) PerihelionPA = atan2(Ay, Ax);
+  else PerihelionPA = atan2(y, x);
+  if (e != 0.0) varpi = 2.0 * atan2(Ay, 1. + Ax);
+  else varpi = lambda;
+  out[0] = e; out[1] = a; out[2] = M; out[3] = V;
+  out[4] = PerihelionPA; out[5] = lambda; out[6] = varpi;
 }

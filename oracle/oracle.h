@@ -66,6 +66,13 @@ void or_rk5(const double *q0, double dt, const double *masses, double *q1,
             int n, const int *feelothers, int indirect_term);
 double or_fviscosity(or_ctx *c, double r);
 
+/* per-DT diagnostics: GasTotalMass, GasMomentum, GasTotalEnergy, planets'
+   angular momentum; UpdateLog's torques/powers (8 per planet) and indirect
+   torque (returns the number of planets); FindOrbitalElements */
+void or_gas_budgets(or_ctx *c, double out[4]);
+int or_update_log(or_ctx *c, double *tq, double *ind);
+void or_orbital_elements(double x, double y, double vx, double vy, double m, double out[7]);
+
 /* dust particles */
 int or_dust_init_system(const dfargo_params *p, const dfargo_dust_params *d, const double *axi_sg_accr,
                         double *size, double *r, double *th, double *vr, double *vth, double *l);

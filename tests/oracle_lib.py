@@ -63,6 +63,9 @@ def lib():
             "or_dust_step": (None, [vp, ctypes.c_double]),
             "or_dust_min_stopping_time": (ctypes.c_double, [vp]),
             "or_dust_particles_mass": (ctypes.c_double, [vp]),
+            "or_gas_budgets": (None, [vp, dp]),
+            "or_update_log": (ctypes.c_int, [vp, dp, dp]),
+            "or_orbital_elements": (None, [ctypes.c_double] * 5 + [dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -126,6 +129,18 @@ class Oracle:
         vr, vt = self.zeros(), self.zeros()
         self.L.or_init_euler(self.ctx, ptr(rho), ptr(vr), ptr(vt), ptr(e), ctypes.byref(sys_))
         return dict(rho=rho, vr=vr, vt=vt, energy=e, planets=sys_)
+
+    def budgets(self):
+        """GasTotalMass, GasMomentum, GasTotalEnergy, planets' angular momentum."""
+        out = np.zeros(4)
+        self.L.or_gas_budgets(self.ctx, ptr(out))
+        return out
+
+    def update_log(self):
+        """UpdateLog: (nb, 8) torques and powers, (nb,) indirect torques."""
+        tq, ind = np.zeros(8 * 8), np.zeros(8)
+        nb = self.L.or_update_log(self.ctx, ptr(tq), ptr(ind))
+        return tq[: 8 * nb].reshape(nb, 8), ind[:nb]
 
     def set_state(self, st):
         self.L.or_set_state(self.ctx, ptr(st["rho"]), ptr(st["vr"]), ptr(st["vt"]),
@@ -208,3 +223,10 @@ class Oracle:
     def dust_step(self, dt):
         """particle part of a sub-step over the frozen gas state"""
         self.L.or_dust_step(self.ctx, dt)
+
+
+def orbital_elements(x, y, vx, vy, m):
+    """FindOrbitalElements: e, a, M, V, PerihelionPA, lambda, varpi."""
+    out = np.zeros(7)
+    lib().or_orbital_elements(x, y, vx, vy, m, ptr(out))
+    return out

@@ -3,6 +3,7 @@ entry point include/dfargo.h declares, its structures match the ctypes
 mirror, and its host-side reference functions (geometry, initial profiles,
 planets) agree bit for bit with the oracle's restatement."""
 import ctypes
+import math
 import os
 import re
 import subprocess
@@ -209,3 +210,21 @@ def test_dust_parameter_checks(lib):
     q.omega_frame = 1.0
     d = _capi.dust_default_params(10, sizepartslope=0.0)
     assert lib.dfargo_check_dust_params(ctypes.byref(q), ctypes.byref(d)) == _capi.ERR_UNSUPPORTED
+
+
+def test_orbital_elements_match_oracle(lib):
+    """dfargo_orbital_elements is FindOrbitalElements, bit for bit the
+    oracle's restatement on random bound orbits; a non-positive mass is an
+    argument error."""
+    from dustyfargoadsg_amd.disc import orbital_elements
+    from tests.oracle_lib import orbital_elements as or_elements
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        r, phi = rng.uniform(0.5, 2.0), rng.uniform(-3, 3)
+        v = math.sqrt(1.0 / r) * rng.uniform(0.8, 1.2)
+        ang = phi + math.pi / 2 + rng.uniform(-0.3, 0.3)
+        args = (r * math.cos(phi), r * math.sin(phi), v * math.cos(ang), v * math.sin(ang), 1.0 + 1e-3)
+        a, b = orbital_elements(*args, lib=lib), or_elements(*args)
+        assert np.array_equal(a, b) or np.allclose(a, b, rtol=0, atol=0, equal_nan=True)
+    with pytest.raises(Exception):
+        orbital_elements(1.0, 0.0, 0.0, 1.0, 0.0, lib=lib)
