@@ -247,6 +247,11 @@ static int build_context(dfargo_ctx *c) {
   a.roche = p.roche_smoothing != 0.0;
   a.forced_circular = p.forced_circular;
   a.xcd_remap = 1;
+  {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->dev));
+    a.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  }
   a.pm = p.pmax - p.pmin;
   a.dphi = a.pm / (double)ns;
   a.invdphi = 1.0 / a.dphi;

@@ -27,6 +27,7 @@ struct KArgs {
   int energy, visc_on, alpha_on, fast, floors, bc, sg, beta_cooling, viscous_heating;
   int gas_indirect, indirect_term, alt_indirect, exclude_hill, roche, forced_circular;
   int xcd_remap;
+  int num_cu; // compute units of the device
   // constants
   double pm, dphi, invdphi, gamma, alpha, visc_const, floor_dens;
   double sg_eps, sg_rstep, sg_tstep, vh_lr1, vh_lr2;

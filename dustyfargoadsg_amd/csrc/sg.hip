@@ -345,16 +345,20 @@ template <typename K> static void allow_lds(K kernel, size_t bytes) {
 static size_t rows_lds_bytes(const KArgs &a, size_t csz) { return 2 * csz * fpsz(a.ns); }
 static size_t cols_lds_bytes(const KArgs &a, size_t csz) { return 4 * csz * fpsz(2 * a.nr); }
 
-// Block size of a transform kernel using lds bytes per workgroup: the LDS
-// lets 160 KiB / lds workgroups share a CU, and together they should hold
-// about 16 wavefronts, so that the SIMDs stay busy through the barriers of
-// the passes (256 .. SG_TB_MAX threads, a power of two).
-static int sg_threads(size_t lds) {
+// Block size of a transform kernel of nwg workgroups using lds bytes each:
+// the LDS lets 160 KiB / lds workgroups share a CU, the grid gives each CU
+// nwg / CUs of them, and together they should hold about 16 wavefronts, so
+// that the SIMDs stay busy through the barriers of the passes (256 ..
+// SG_TB_MAX threads, a power of two, and no more than the butterflies of
+// the widest pass, cap).
+static int sg_threads(const KArgs &a, size_t lds, int nwg, int cap) {
   const size_t per_cu = 160 * 1024;
-  const int groups = (int)(per_cu / (lds ? lds : 1));
+  int groups = (int)(per_cu / (lds ? lds : 1));
+  const int per_grid = (nwg + a.num_cu - 1) / a.num_cu;
+  if (per_grid < groups) groups = per_grid;
   const int want = 16 * 64 / (groups > 0 ? groups : 1);
   int t = 256;
-  while (t < want && t < SG_TB_MAX) t *= 2;
+  while (t < want && t < SG_TB_MAX && 2 * t <= cap) t *= 2;
   return t;
 }
 
@@ -364,13 +368,13 @@ template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_
   cx<T> *z = (cx<T> *)a.sg_z, *w = (cx<T> *)a.sg_w;
   switch (id) {
     case KID_SG_ROWS_FWD:
-      hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(sg_threads(rows_lds)), rows_lds, s, a, force, z);
+      hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(sg_threads(a, rows_lds, a.nr, a.ns / 2)), rows_lds, s, a, force, z);
       break;
     case KID_SG_COLS:
-      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(sg_threads(cols_lds)), cols_lds, s, a, force, (const cx<T> *)z, w);
+      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(sg_threads(a, cols_lds, a.ns / 2 + 1, 2 * a.nr)), cols_lds, s, a, force, (const cx<T> *)z, w);
       break;
     case KID_SG_ROWS_INV:
-      hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(sg_threads(rows_lds)), rows_lds, s, a, force, (const cx<T> *)w);
+      hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(sg_threads(a, rows_lds, a.nr, a.ns / 2)), rows_lds, s, a, force, (const cx<T> *)w);
       break;
   }
 }
