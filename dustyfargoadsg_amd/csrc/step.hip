@@ -848,7 +848,12 @@ template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs
 #define SRC_SR (SRC_TR + 3)
 #define SRC_SC (SRC_TC + 3)
 #define SRC_FC (SRC_TC + 7)  // state staged on rings i0-4 .. i0+TR+1, columns j0-4 .. j0+TC+2
-#define SRC_TB 256
+// Threads per tile: the LDS (48 KiB with FP32 storage, 64 KiB with FP64)
+// lets three or two tiles share a CU; 512 (FP32) and 768 (FP64) threads
+// then fill it with 24 wavefronts, which hide the latency of the stages'
+// LDS and global reads (measured faster than 256 and 1024).
+template <typename T> struct SrcBlock { static constexpr int n = 512; };
+template <> struct SrcBlock<double> { static constexpr int n = 768; };
 
 // the 1-D tables k_source reads, staged per tile
 enum { ST_IDRM, ST_RINF, ST_IRINF, ST_RMED, ST_SUPP, ST_IDXT, ST_NU, ST_RSUP, ST_IDRS, ST_IRMED, ST_N };
@@ -875,7 +880,7 @@ template <typename T> __device__ __forceinline__ double src_press(const KArgs &a
   return (a.gamma - 1.0) * ld<T>(a.en, l);
 }
 
-template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a) {
+template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source(KArgs a) {
   __shared__ double sPot[SRC_PR][SRC_PC];
   // VradInt / VthetaInt in FP64 whatever the storage type: their sub-step
   // increments are kept in full until the final, unbiased rounding
@@ -890,7 +895,7 @@ template <typename T> __global__ void __launch_bounds__(SRC_TB) k_source(KArgs a
   __shared__ double sTab[ST_N][SRC_PR];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
-  constexpr int TB = SRC_TB;
+  constexpr int TB = SrcBlock<T>::n;
   const int ns = a.ns, nr = a.nr, tid = threadIdx.x;
   const int j0 = blockIdx.x * SRC_TC, i0 = blockIdx.y * SRC_TR;
   const double dt = c->dt;
@@ -1658,7 +1663,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
     case KID_CFL: RING_DISPATCH_K(wide_k(a.ns), k_cfl_ring, T, a, s, 0); break;
     case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
     case KID_SOURCE:
-      hipLaunchKernelGGL(k_source<T>, dim3((a.ns + SRC_TC - 1) / SRC_TC, (a.nr + SRC_TR - 1) / SRC_TR), dim3(SRC_TB), 0, s,
+      hipLaunchKernelGGL(k_source<T>, dim3((a.ns + SRC_TC - 1) / SRC_TC, (a.nr + SRC_TR - 1) / SRC_TR), dim3(SrcBlock<T>::n), 0, s,
                          a);
       break;
     case KID_ENERGY: RING_DISPATCH(k_energy, T, a, s, 0); break;
