@@ -1,0 +1,29 @@
+"""FP32 storage against FP64 storage over 10 orbits of the 128x384 planet
+disc (the north star's precision comparison, bench.precision_error): the
+stochastically rounded FP32 state stays within the stated bars of the FP64
+run.  Measured on MI355X: ring-mean Sigma 5e-6, Sigma L2 2.4e-5, Sigma
+L-infinity 2.7e-5, v_theta 3.7e-6, v_r 4.7e-6 (on the v_theta scale), the
+same sub-step count and planet position; the bars leave a factor of about
+two.  A biased rounding (round-to-nearest of float stencils) fails the
+ring-mean bar by two orders of magnitude.
+"""
+import pytest
+
+import bench
+from dustyfargoadsg_amd.workloads import gas_planet
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fp32_storage_ten_orbits_planet_disc():
+    p, pl = gas_planet()
+    print("case gas_planet 128x384, 10 orbits, FP32 vs FP64", flush=True)
+    r = bench.precision_error(p, pl, 10, control=False)["fp32_vs_fp64"]
+    print(r, flush=True)
+    assert r["substeps"][0] == r["substeps"][1]
+    assert r["sigma_ring_mean_linf"] <= 1e-5
+    assert r["sigma_l2"] <= 5e-5
+    assert r["sigma_linf"] <= 1e-4
+    assert r["vtheta_linf"] <= 1e-5
+    assert r["vr_linf_on_vtheta"] <= 1e-5
+    assert r["planet_position"] <= 1e-10
