@@ -314,7 +314,8 @@ static int build_context(dfargo_ctx *c) {
   c->esz = (c->prec == DFARGO_FP64) ? sizeof(double) : sizeof(float);
   c->nfield = (size_t)(nr + 2) * ns;
   void **fields[] = {&a.rho, &a.vr, &a.vt, &a.en, &a.cs, &a.vri, &a.vti, &a.trr, &a.trp, &a.tpp,
-                     &a.divv, &a.vh, &a.t_rho, &a.t_rp, &a.t_rm, &a.t_tp, &a.t_tm, &a.t_en, &a.pot, &a.e_int};
+                     &a.divv, &a.vh, &a.t_rho, &a.t_rp, &a.t_rm, &a.t_tp, &a.t_tm, &a.t_en, &a.pot, &a.e_int,
+                     &a.z_rho, &a.z_rp, &a.z_rm, &a.z_tp, &a.z_tm, &a.z_en};
   for (void **f : fields)
     if ((rc = dalloc(c, f, c->esz * c->nfield))) return rc;
   void **rings[] = {(void **)&a.vmoy, (void **)&a.ring_dt, (void **)&a.nu_ring, (void **)&a.gcs,
@@ -323,6 +324,7 @@ static int build_context(dfargo_ctx *c) {
     if ((rc = dalloc(c, r, sizeof(double) * (size_t)(std::max(nr, ns) + 2)))) return rc;
   if ((rc = dalloc(c, (void **)&a.ring_part, sizeof(double) * (size_t)(nr + 2) * DF_NPART(DF_MAXTG))) ||
       (rc = dalloc(c, (void **)&a.nshift, sizeof(long) * (size_t)(nr + 2))) ||
+      (rc = dalloc(c, (void **)&a.vt_part, sizeof(double) * (size_t)nr * ((ns + DF_SRC_COLS - 1) / DF_SRC_COLS))) ||
       (rc = dalloc(c, (void **)&c->d_ctrl, sizeof(dfargo_ctrl))))
     return rc;
   a.ctrl = c->d_ctrl;

@@ -43,8 +43,14 @@ struct KArgs {
   // fields (T)
   void *rho, *vr, *vt, *en, *cs, *vri, *vti, *trr, *trp, *tpp, *divv, *vh;
   void *t_rho, *t_rp, *t_rm, *t_tp, *t_tm, *t_en, *sgr, *sgt, *pot, *e_int;
+  // the transported quantities after the azimuthal pass (the segments of a
+  // ring read t_* around their cells while others store: separate arrays)
+  void *z_rho, *z_rp, *z_rm, *z_tp, *z_tm, *z_en;
   // ring buffers (FP64 unless noted)
   double *vmoy, *ring_dt, *ring_part, *nu_ring, *gcs, *vmed, *lost_col;
+  // per ring and DF_SRC_COLS-column tile of the source kernel, the sum of
+  // vt dt / dt (nr x ceil(ns / DF_SRC_COLS))
+  double *vt_part;
   long *nshift;
   dfargo_ctrl *ctrl;
   // dust particles (FP64 whatever the field type) and the grids they share
@@ -84,6 +90,9 @@ struct KArgs {
   // sg_kr, sg_kt: kernel spectra for wavenumbers 0..Ns/2, [k2][k1] (transposed)
   void *sg_z, *sg_w, *sg_kr, *sg_kt, *tw_ns, *tw_2n;
 };
+
+// columns of a source-kernel tile: one wavefront per ring of the tile
+#define DF_SRC_COLS 64
 
 // launchers (step.hip, sg.hip); each enqueues on stream s
 enum KernelId {

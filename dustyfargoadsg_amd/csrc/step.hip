@@ -99,6 +99,12 @@ __device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_a
 __device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax2(double a, double b) { return (b > a) ? b : a; }
 
+// the sum over a wavefront, the same in every lane (xor butterfly)
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // deterministic block reductions (fixed tree for a given blockDim)
 __device__ double block_sum(double v, double *sh) {
   int t = threadIdx.x, TB = blockDim.x, P = 1;
@@ -840,7 +846,8 @@ template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs
 // copies of src/SourceEuler.c:598-630 are deferred to the end of the
 // sub-step), so no workgroup overwrites what another one still reads.
 #define SRC_TR 8
-#define SRC_TC 64
+#define SRC_TC DF_SRC_COLS
+static_assert(SRC_TC == 64, "the SubStep2 stage sums one tile ring per wavefront");
 #define SRC_PR (SRC_TR + 6)
 #define SRC_PC (SRC_TC + 6)
 #define SRC_VR (SRC_TR + 5)
@@ -1078,43 +1085,56 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
     }
     __syncthreads();
   }
-  // stage 4: SubStep2 on the tile (V slot = tile slot + 3)
-  const double C2 = DF_CVNR * DF_CVNR;
-  for (int idx = tid; idx < SRC_TR * SRC_TC; idx += TB) {
+  // stage 4: SubStep2 on the tile (V slot = tile slot + 3).  Each
+  // wavefront holds one ring of the tile (SRC_TC = 64 columns) and leaves
+  // the tile's sum of the elongation rates vt dt / dt of the stored vtheta
+  // (ComputeThetaElongations and ComputeAverageThetaVelocities,
+  // src/TransportEuler.c:249-290), from which the azimuthal transport forms
+  // the ring mean.
+  const double C2 = DF_CVNR * DF_CVNR, invdt = 1.0 / dt;
+  const int ntile = (ns + SRC_TC - 1) / SRC_TC;
+  for (int b0 = 0; b0 < SRC_TR * SRC_TC; b0 += TB) {
+    const int idx = b0 + tid;
     const int q = idx / SRC_TC, p = idx - q * SRC_TC, i = i0 + q, jt = j0 + p;
-    if (i > nr - 1 || jt >= ns) continue;
-    const int qv = q + 3, pv = p + 3, j = sCol[pv + 1];
-    const long l = (long)i * ns + j;
-    const double rho = (double)sRho[qv + 1][pv + 1];
-    const double vrl = (double)sVr[qv][pv], vrp = (double)sVr[qv + 1][pv];
-    const double vtl = (double)sVt[qv][pv], vtp = (double)sVt[qv][pv + 1], vtm = (double)sVt[qv][pv - 1];
-    double dv = vrp - vrl;
-    const double qr = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
-    dv = vtp - vtl;
-    const double qt = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
-    const double rhojm = (double)sRho[qv + 1][pv];
-    dv = vtl - vtm;
-    const double qtm = (dv < 0.0) ? C2 * rhojm * dv * dv : 0.0;
-    double vrn = 0.0;
-    if (i >= 1) {
-      const double rhoim = (double)sRho[qv][pv + 1];
-      dv = vrl - (double)sVr[qv - 1][pv];
-      const double qrm = (dv < 0.0) ? C2 * rhoim * dv * dv : 0.0;
-      vrn = vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * STAB(ST_IDRM, i);
+    double rate = 0.0;
+    if (idx < SRC_TR * SRC_TC && i <= nr - 1 && jt < ns) {
+      const int qv = q + 3, pv = p + 3, j = sCol[pv + 1];
+      const long l = (long)i * ns + j;
+      const double rho = (double)sRho[qv + 1][pv + 1];
+      const double vrl = (double)sVr[qv][pv], vrp = (double)sVr[qv + 1][pv];
+      const double vtl = (double)sVt[qv][pv], vtp = (double)sVt[qv][pv + 1], vtm = (double)sVt[qv][pv - 1];
+      double dv = vrp - vrl;
+      const double qr = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
+      dv = vtp - vtl;
+      const double qt = (dv < 0.0) ? C2 * rho * dv * dv : 0.0;
+      const double rhojm = (double)sRho[qv + 1][pv];
+      dv = vtl - vtm;
+      const double qtm = (dv < 0.0) ? C2 * rhojm * dv * dv : 0.0;
+      double vrn = 0.0;
+      if (i >= 1) {
+        const double rhoim = (double)sRho[qv][pv + 1];
+        dv = vrl - (double)sVr[qv - 1][pv];
+        const double qrm = (dv < 0.0) ? C2 * rhoim * dv * dv : 0.0;
+        vrn = vrl - dt * 2.0 / (rho + rhoim) * (qr - qrm) * STAB(ST_IDRM, i);
+      }
+      const double invdxtheta = STAB(ST_IDXT, i);
+      st_sr<T>(a.vri, l, vrn, sr_key(c, l, SR_VR));
+      const T vts = sr_round<T>(vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta, sr_key(c, l, SR_VT));
+      ((T *)a.vti)[l] = vts;
+      rate = ((double)vts * dt) * invdt;
+      if (a.energy) {
+        double e = ld<T>(a.en, l);
+        e = e - dt * qr * (vrp - vrl) * STAB(ST_IDRS, i) - dt * qt * (vtp - vtl) * invdxtheta;
+        st_sr<T>(a.e_int, l, e, sr_key(c, l, SR_E));
+      }
+      if (i == nr - 1) {
+        // vr and vt of ring nrad are zero after ActualiseGas
+        st<T>(a.vr, l + ns, 0.0);
+        st<T>(a.vt, l + ns, 0.0);
+      }
     }
-    const double invdxtheta = STAB(ST_IDXT, i);
-    st_sr<T>(a.vri, l, vrn, sr_key(c, l, SR_VR));
-    st_sr<T>(a.vti, l, vtl - dt * 2.0 / (rho + rhojm) * (qt - qtm) * invdxtheta, sr_key(c, l, SR_VT));
-    if (a.energy) {
-      double e = ld<T>(a.en, l);
-      e = e - dt * qr * (vrp - vrl) * STAB(ST_IDRS, i) - dt * qt * (vtp - vtl) * invdxtheta;
-      st_sr<T>(a.e_int, l, e, sr_key(c, l, SR_E));
-    }
-    if (i == nr - 1) {
-      // vr and vt of ring nrad are zero after ActualiseGas
-      st<T>(a.vr, l + ns, 0.0);
-      st<T>(a.vt, l + ns, 0.0);
-    }
+    rate = wave_sum(rate);
+    if ((tid & 63) == 0 && idx < SRC_TR * SRC_TC && i <= nr - 1) a.vt_part[(long)i * ntile + blockIdx.x] = rate;
   }
 }
 #undef STAB
@@ -1333,176 +1353,128 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
   }
 }
 
-// Azimuthal transport (OneWindTheta, src/TransportEuler.c:225-465), one ring
-// per workgroup, K consecutive cells j0 .. j0+K-1 per thread.  The j-1 /
-// j+1 neighbours of a thread's cells belong to other threads: each
-// exchange publishes the first two and last two cells of every thread in
-// one of two ring-sized LDS buffers (alternating, so one barrier per
-// exchange suffices) and reads back the two cells either side.
-template <typename R, int K>
-__device__ __forceinline__ void ring_halo(R *buf, const R (&X)[K], int j0, int ns, R (&H)[4]) {
-  buf[j0] = X[0];
-  buf[j0 + K - 1] = X[K - 1];
-  if (K > 1) {
-    buf[j0 + 1] = X[K > 1 ? 1 : 0];
-    buf[j0 + K - 2] = X[K > 1 ? K - 2 : 0];
-  }
-  __syncthreads();
-  const int jm2 = (j0 >= 2) ? j0 - 2 : j0 - 2 + ns, jm1 = (j0 >= 1) ? j0 - 1 : j0 - 1 + ns;
-  const int jk0 = (j0 + K < ns) ? j0 + K : j0 + K - ns, jk1 = (j0 + K + 1 < ns) ? j0 + K + 1 : j0 + K + 1 - ns;
-  H[0] = buf[jm2];
-  H[1] = buf[jm1];
-  H[2] = buf[jk0];
-  H[3] = buf[jk1];
+// Azimuthal transport (OneWindTheta, src/TransportEuler.c:225-465) on ring
+// segments, one wavefront each: the residual and uniform passes and the
+// shift.  The ring quantities of the FARGO scheme come first, in FP64
+// whatever the storage type, from the per-tile sums k_source leaves of the
+// velocity before transport: the mean VMed
+// (ComputeAverageThetaVelocities), the integer shift Nshift and the
+// constant residual (ComputeConstantResidual, src/TransportEuler.c:292-348).
+// Every wavefront of a ring forms them from the same sums in the same
+// order, so all agree on the shift.
+
+// The cells a wavefront owns: its 64 lanes hold 64 consecutive cells of a
+// ring, and each of the two passes leaves two fewer valid cells at either
+// end (a cell's update reads its neighbours up to two cells away), so the
+// middle 56 are complete after both.
+#define AZ_LANES 64
+#define AZ_HALO 4
+#define AZ_OWN (AZ_LANES - 2 * AZ_HALO)
+
+// The neighbouring lanes' values: lane L receives lane L - 1's (prev) or
+// lane L + 1's (next); the end lanes receive their own, which only feeds
+// cells outside the valid range.
+template <typename R> __device__ __forceinline__ R lane_prev(R v) { return __shfl_up(v, 1, AZ_LANES); }
+template <typename R> __device__ __forceinline__ R lane_next(R v) { return __shfl_down(v, 1, AZ_LANES); }
+
+// ComputeStarTheta (src/TransportEuler.c:419-444) at this lane's cell (the
+// face between the previous cell and this one), for the values x, the
+// velocity v of the cell and the time step dt
+template <typename T, typename R = typename TransportArith<T>::type>
+__device__ __forceinline__ R azi_star_lane(R x, R v, R dt, R dxtheta, R invdxtheta) {
+  const R xm = lane_prev(x), xp = lane_next(x);
+  const R dqm = x - xm, dqp = xp - x;
+  const R dq = (dqp * dqm > R(0)) ? TransportArith<T>::div(dqp * dqm, dqp + dqm) * invdxtheta : R(0);
+  const R dqprev = lane_prev(dq);
+  const R ksi = v * dt;
+  if (ksi > R(0)) return xm + (dxtheta - ksi) * dqprev;
+  return x - (dxtheta + ksi) * dq;
 }
 
-// ComputeStarTheta (src/TransportEuler.c:419-444) at the cells j0 .. j0+K
-// (the K own cells and the first cell of the next thread) from the own
-// values X and the halo H = {X[j0-2], X[j0-1], X[j0+K], X[j0+K+1]}.
-template <typename T, int K, typename R = typename TransportArith<T>::type>
-__device__ __forceinline__ void azi_star(const R (&X)[K], const R (&H)[4], const R (&v)[K + 1], R dt, R dxtheta,
-                                         R invdxtheta, R (&S)[K + 1]) {
-  // values at cells j0-2 .. j0+K+1 (offset 2)
-  R x[K + 4];
-  x[0] = H[0];
-  x[1] = H[1];
-#pragma unroll
-  for (int k = 0; k < K; k++) x[k + 2] = X[k];
-  x[K + 2] = H[2];
-  x[K + 3] = H[3];
-  // slopes at cells j0-1 .. j0+K (offset 1)
-  R dq[K + 2];
-#pragma unroll
-  for (int c = 0; c < K + 2; c++) {
-    const R dqm = x[c + 1] - x[c], dqp = x[c + 2] - x[c + 1];
-    dq[c] = (dqp * dqm > R(0)) ? TransportArith<T>::div(dqp * dqm, dqp + dqm) * invdxtheta : R(0);
-  }
-#pragma unroll
-  for (int c = 0; c < K + 1; c++) {
-    const R ksi = v[c] * dt;
-    if (ksi > R(0)) S[c] = x[c + 1] + (dxtheta - ksi) * dq[c];
-    else S[c] = x[c + 2] - (dxtheta + ksi) * dq[c + 1];
-  }
-}
-
-// QuantitiesAdvection (src/TransportEuler.c:375-400) with velocity v at the
-// cells j0 .. j0+K: the density star first, then every quantity's star,
-// flux and update.  Q[5] is the density; RI is its value before the pass.
-template <typename T, int K, typename R = typename TransportArith<T>::type>
-__device__ __forceinline__ void azi_pass(const KArgs &a, R *lds, int &flip, int j0, R (&Q)[6][K], const R (&v)[K + 1],
-                                         R dt, R dxtheta, R invdxtheta, R dxrad, R invsurf) {
-  const int ns = a.ns;
-  R RI[K], RS[K + 1], H[4];
-#pragma unroll
-  for (int k = 0; k < K; k++) RI[k] = Q[5][k];
-  ring_halo<R, K>(lds + flip * ns, RI, j0, ns, H);
-  flip ^= 1;
-  azi_star<T, K>(RI, H, v, dt, dxtheta, invdxtheta, RS);
-  // the Work values Q / (rho + 1e-20) share one reciprocal per cell
-  R inv[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) inv[k] = TransportArith<T>::div(R(1), RI[k] + R(1e-20));
+// QuantitiesAdvection (src/TransportEuler.c:375-400) with velocity v: the
+// density star first, then every quantity's star, flux and update.  Q[5]
+// is the density.  The flux through this lane's face is dxrad S RS v; the
+// cell's update is its own face's flux minus the next lane's.
+template <typename T, typename R = typename TransportArith<T>::type>
+__device__ __forceinline__ void azi_pass_lane(const KArgs &a, R (&Q)[6], R v, R dt, R dxtheta, R invdxtheta,
+                                              R dxrad, R invsurf) {
+  const R RI = Q[5];
+  const R RS = azi_star_lane<T>(RI, v, dt, dxtheta, invdxtheta);
+  // the Work values Q / (rho + 1e-20) share one reciprocal
+  const R inv = TransportArith<T>::div(R(1), RI + R(1e-20));
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     if (q == 4 && !a.energy) continue;
-    R W[K], S[K + 1], f[K + 1];
-#pragma unroll
-    for (int k = 0; k < K; k++) W[k] = Q[q][k] * inv[k];
-    ring_halo<R, K>(lds + flip * ns, W, j0, ns, H);
-    flip ^= 1;
-    azi_star<T, K>(W, H, v, dt, dxtheta, invdxtheta, S);
-#pragma unroll
-    for (int c = 0; c < K + 1; c++) f[c] = dxrad * S[c] * RS[c] * v[c];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      R varq = f[k];
-      varq -= f[k + 1];
-      Q[q][k] += varq * invsurf;
-    }
+    const R S = azi_star_lane<T>(Q[q] * inv, v, dt, dxtheta, invdxtheta);
+    const R f = dxrad * S * RS * v;
+    const R fn = lane_next(f);
+    R varq = f;
+    varq -= fn;
+    Q[q] += varq * invsurf;
   }
 }
 
-// The FARGO mean velocity, the integer shift and the constant residual stay
-// in FP64 whatever the storage type.
-template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_transport_azi(KArgs a) {
+template <typename T> __global__ void __launch_bounds__(256) k_transport_azi(KArgs a) {
   using R = typename TransportArith<T>::type;
-  extern __shared__ double lds[];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
-  const int i = ring_of_block(a), ns = a.ns, t = threadIdx.x, TB = blockDim.x;
-  const int j0 = t * K;
-  double *sh = lds + 2 * ns * sizeof(R) / sizeof(double); // after the two ring buffers of R
+  const int ns = a.ns, i = blockIdx.y, lane = threadIdx.x & (AZ_LANES - 1);
+  const int seg = blockIdx.x * (blockDim.x / AZ_LANES) + threadIdx.x / AZ_LANES;
+  if (seg * AZ_OWN >= ns) return; // a whole wavefront
+  int j = (seg * AZ_OWN - AZ_HALO + lane) % ns;
+  if (j < 0) j += ns;
   const long base = (long)i * ns;
   const double dt = c->dt, invdt = 1.0 / dt;
-  // mass through the inner edge (VanLeerRadial's LostByDisk, ring 0)
-  if (i == 0) {
-    double lost = 0.0;
-    for (int j = t; j < ns; j += TB) lost += a.lost_col[j];
-    lost = block_sum_fast(lost, sh);
-    if (t == 0) a.ctrl->lost_mass += lost;
-  }
-  // quantities after the radial pass: rp, rm, tp, tm, [e], rho
-  void *qp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
-  R Q[6][K];
-#pragma unroll
-  for (int q = 0; q < 6; q++)
-#pragma unroll
-    for (int k = 0; k < K; k++) Q[q][k] = (q == 4 && !a.energy) ? R(0) : (R)((const T *)qp[q])[base + j0 + k];
-  // ComputeThetaElongations / ComputeAverageThetaVelocities / Residuals on
-  // the velocity before transport; v[K] is the first cell of the next thread
-  double vel[K + 1];
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < K + 1; k++) {
-    const int j = (j0 + k < ns) ? j0 + k : j0 + k - ns;
-    vel[k] = ld<T>(a.vti, base + j) * dt;
-    if (k < K) s += vel[k] * invdt;
-  }
-  s = block_sum_fast(s, sh);
-  const double vmed = s / (double)ns;
-  // ComputeConstantResidual (src/TransportEuler.c:292-348)
+  // the ring's mean velocity, shift and residual
+  const int ntile = (ns + DF_SRC_COLS - 1) / DF_SRC_COLS;
+  double sum = 0.0;
+  for (int t = lane; t < ntile; t += AZ_LANES) sum += a.vt_part[(long)i * ntile + t];
+  const double vmed = wave_sum(sum) / (double)ns;
   const double dpinvns = a.pm / (double)ns;
   const double Ntilde = vmed * a.InvRmed[i] * dt * (double)ns / a.pm;
   const double Nround = floor(Ntilde + 0.5);
   const long nshift = (long)Nround;
-  double u = (Ntilde - Nround) * a.Rmed[i] * invdt * dpinvns;
-  if (t == 0) {
+  const double u = (Ntilde - Nround) * a.Rmed[i] * invdt * dpinvns;
+  if (seg == 0 && lane == 0) {
     a.vmed[i] = vmed;
     a.nshift[i] = nshift;
   }
-  R vres[K + 1];
-#pragma unroll
-  for (int k = 0; k < K + 1; k++) {
-    double v = vel[k] * invdt - vmed;
-    if (!a.fast) v = u + v;
-    vres[k] = (R)v;
+  // mass through the inner edge (VanLeerRadial's LostByDisk, ring 0)
+  if (i == 0 && seg == 0) {
+    double lost = 0.0;
+    for (int jj = lane; jj < ns; jj += AZ_LANES) lost += a.lost_col[jj];
+    lost = wave_sum(lost);
+    if (lane == 0) a.ctrl->lost_mass += lost;
   }
+  // quantities after the radial pass: rp, rm, tp, tm, [e], rho; the
+  // results go to z_*, since other wavefronts of the ring may still read
+  // the cells this one stores
+  const void *qp[6] = {a.t_rp, a.t_rm, a.t_tp, a.t_tm, a.t_en, a.t_rho};
+  void *zp[6] = {a.z_rp, a.z_rm, a.z_tp, a.z_tm, a.z_en, a.z_rho};
+  R Q[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) Q[q] = (q == 4 && !a.energy) ? R(0) : (R)((const T *)qp[q])[base + j];
+  // the residual velocity of the cell (src/TransportEuler.c:280-290)
+  double v = (ld<T>(a.vti, base + j) * dt) * invdt - vmed;
+  if (!a.fast) v = u + v;
   const double dxth = a.pm / (double)ns * a.Rmed[i];
   const R dxtheta = (R)dxth, invdxtheta = (R)(1.0 / dxth);
   const R dxrad = (R)((a.Rsup[i] - a.Rinf[i]) * dt), invsurf = (R)(1.0 / a.Surf[i]), rdt = (R)dt;
-  R *ring = (R *)lds;
-  int flip = 0;
-  // residual pass (the only pass of the standard transport)
-  azi_pass<T, K>(a, ring, flip, j0, Q, vres, rdt, dxtheta, invdxtheta, dxrad, invsurf);
+  // residual pass (the only pass of the standard transport), then the
   // uniform pass with the constant residual
-  if (a.fast) {
-    R uu[K + 1];
-#pragma unroll
-    for (int k = 0; k < K + 1; k++) uu[k] = (R)u;
-    azi_pass<T, K>(a, ring, flip, j0, Q, uu, rdt, dxtheta, invdxtheta, dxrad, invsurf);
-  }
+  azi_pass_lane<T>(a, Q, (R)v, rdt, dxtheta, invdxtheta, dxrad, invsurf);
+  if (a.fast) azi_pass_lane<T>(a, Q, (R)u, rdt, dxtheta, invdxtheta, dxrad, invsurf);
   // AdvectSHIFT (src/TransportEuler.c:350-373): new[j] = old[j - Nshift]
+  const int own = lane - AZ_HALO;
+  if (own < 0 || own >= AZ_OWN || seg * AZ_OWN + own >= ns) return;
   long sh_mod = nshift % ns;
   if (sh_mod < 0) sh_mod += ns;
+  long jn = j + sh_mod;
+  if (jn >= ns) jn -= ns;
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     if (q == 4 && !a.energy) continue;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      long jn = j0 + k + sh_mod;
-      if (jn >= ns) jn -= ns;
-      ((T *)qp[q])[base + jn] = sr_round<T>((double)Q[q][k], sr_key(c, base + jn, SR_AZI + q));
-    }
+    ((T *)zp[q])[base + jn] = sr_round<T>((double)Q[q], sr_key(c, base + jn, SR_AZI + q));
   }
 }
 
@@ -1515,17 +1487,17 @@ struct CellState {
 template <typename T> __device__ CellState vel_cell(const KArgs &a, const dfargo_ctrl *c, int i, int j) {
   const int ns = a.ns;
   const long l = (long)i * ns + j, lim = l - ns, ljm = (j == 0) ? (long)i * ns + ns - 1 : l - 1;
-  const double rho = ld<T>(a.t_rho, l);
+  const double rho = ld<T>(a.z_rho, l);
   CellState s;
   s.vr = 0.0;
-  if (i != 0) s.vr = (ld<T>(a.t_rp, lim) + ld<T>(a.t_rm, l)) / (rho + ld<T>(a.t_rho, lim) + 1e-20);
-  s.vt = (ld<T>(a.t_tp, ljm) + ld<T>(a.t_tm, l)) / (rho + ld<T>(a.t_rho, ljm) + 1e-20) / a.Rmed[i] -
+  if (i != 0) s.vr = (ld<T>(a.z_rp, lim) + ld<T>(a.z_rm, l)) / (rho + ld<T>(a.z_rho, lim) + 1e-20);
+  s.vt = (ld<T>(a.z_tp, ljm) + ld<T>(a.z_tm, l)) / (rho + ld<T>(a.z_rho, ljm) + 1e-20) / a.Rmed[i] -
          a.Rmed[i] * c->omega_frame;
   s.rho = rho;
   if (a.floors && s.rho < a.floor_dens) s.rho = a.floor_dens;
   s.e = 0.0;
   if (a.energy) {
-    s.e = ld<T>(a.t_en, l);
+    s.e = ld<T>(a.z_en, l);
     const double e_bottom = 1e-3 * a.EnergyMed[i];
     if (a.floors && s.e < e_bottom) s.e = e_bottom;
   }
@@ -1625,11 +1597,10 @@ static int ring_k(int ns) {
 
 int ring_kcheck(int ns) { return ring_k(ns); }
 
-// Six cells per thread (four waves per ring, two rings per CU) for the
-// azimuthal transport, which recomputes two slopes at each end of a
-// thread's K cells, and for the CFL kernel, whose block reductions then
-// run over fewer waves: measured faster than two or three cells per thread
-// at more waves.  Rings of fewer than 1536 cells keep the smallest K.
+// Six cells per thread (four waves per ring) for the ring reductions of
+// the CFL kernel: its block reductions then run over fewer waves, measured
+// faster than two or three cells per thread at more waves.  Rings of fewer
+// than 1536 cells keep the smallest K.
 static int wide_k(int ns) {
   if (ns % 6 == 0 && ns / 6 >= 256 && ns / 6 <= RING_LB(6)) return 6;
   return ring_k(ns);
@@ -1673,8 +1644,8 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       break;
     }
     case KID_TRANSPORT_AZI: {
-      size_t lds = sizeof(typename TransportArith<T>::type) * 2 * (size_t)a.ns + sizeof(double) * 1024;
-      RING_DISPATCH_K(wide_k(a.ns), k_transport_azi, T, a, s, lds);
+      const int nseg = (a.ns + AZ_OWN - 1) / AZ_OWN, per = 256 / AZ_LANES;
+      hipLaunchKernelGGL(k_transport_azi<T>, dim3((nseg + per - 1) / per, a.nr), dim3(256), 0, s, a);
       break;
     }
     case KID_FINISH:
@@ -1704,21 +1675,6 @@ void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s) {
 }
 
 template <typename T> void prepare_kernels(const KArgs &a) {
-  const size_t lds = sizeof(typename TransportArith<T>::type) * 2 * (size_t)a.ns + sizeof(double) * 1024;
-  if (lds > 65536) {
-    const void *f = nullptr;
-    switch (ring_k(a.ns)) {
-      case 1: f = (const void *)k_transport_azi<T, 1>; break;
-      case 2: f = (const void *)k_transport_azi<T, 2>; break;
-      case 3: f = (const void *)k_transport_azi<T, 3>; break;
-      case 4: f = (const void *)k_transport_azi<T, 4>; break;
-      case 6: f = (const void *)k_transport_azi<T, 6>; break;
-      case 8: f = (const void *)k_transport_azi<T, 8>; break;
-      case 12: f = (const void *)k_transport_azi<T, 12>; break;
-      case 16: f = (const void *)k_transport_azi<T, 16>; break;
-    }
-    if (f) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  }
   if (a.sg) sg_prepare<T>(a);
 }
 template void prepare_kernels<double>(const KArgs &);
