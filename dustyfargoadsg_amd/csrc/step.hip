@@ -1373,9 +1373,18 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
 
 // The neighbouring lanes' values: lane L receives lane L - 1's (prev) or
 // lane L + 1's (next); the end lanes receive their own, which only feeds
-// cells outside the valid range.
-template <typename R> __device__ __forceinline__ R lane_prev(R v) { return __shfl_up(v, 1, AZ_LANES); }
-template <typename R> __device__ __forceinline__ R lane_next(R v) { return __shfl_down(v, 1, AZ_LANES); }
+// cells outside the valid range.  CDNA's whole-wavefront DPP shifts
+// (wave_shr:1, wave_shl:1) move each 32-bit half in one VALU operation,
+// without the LDS crossbar of a general lane permute.
+template <int CTRL> __device__ __forceinline__ double dpp_shift(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false));
+}
+#define DPP_WAVE_SHL1 0x130
+#define DPP_WAVE_SHR1 0x138
+__device__ __forceinline__ double lane_prev(double v) { return dpp_shift<DPP_WAVE_SHR1>(v); }
+__device__ __forceinline__ double lane_next(double v) { return dpp_shift<DPP_WAVE_SHL1>(v); }
 
 // ComputeStarTheta (src/TransportEuler.c:419-444) at this lane's cell (the
 // face between the previous cell and this one), for the values x, the
