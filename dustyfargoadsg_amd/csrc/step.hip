@@ -92,12 +92,7 @@ template <> struct TransportArith<float> {
   static __device__ __forceinline__ double div(double a, double b) { return div_nr(a, b); }
 };
 
-// a / b in the arithmetic of the storage type: the IEEE quotient for
-// double; for float the hardware reciprocal (1 ulp) times a
-__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
-__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double dmax2(double a, double b) { return (b > a) ? b : a; }
 
 // the sum over a wavefront, the same in every lane (xor butterfly)
 __device__ __forceinline__ double wave_sum(double v) {
@@ -402,7 +397,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
     const R dxrad = (R)(a.Rsup[i] - a.Rinf[i]);
     const R dxtheta = (R)(a.Rmed[i] * a.pm / (double)ns);
     const R vm = (R)vmoy, dxmin = (dxrad < dxtheta) ? dxrad : dxtheta;
-    const R invdt5 = (viscosity != R(0)) ? qdiv(viscosity * R(4), (R)a.powmin2[i]) : R(1e-10);
+    const R invdt5 = (viscosity != R(0)) ? viscosity * R(4) / (R)a.powmin2[i] : R(1e-10);
     // the ring's cell sizes divide every cell's terms: one reciprocal each
     // (within an ulp of the quotients, which only moves dt by as much)
     const R idxmin = R(1) / dxmin, idxrad = R(1) / dxrad, idxtheta = R(1) / dxtheta;
@@ -430,7 +425,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
                   invdt6 * invdt6;
       smax = (S > smax) ? S : smax;
     }
-    dtmin = qdiv(R(DF_CFLSECURITY), sqrt(smax));
+    dtmin = R(DF_CFLSECURITY) / sqrt(smax);
   }
   // per-ring partials, stored partial-major (ring_part[p * nr + i]):
   // negative-density and negative-energy counts, disc mass, barycentre.
