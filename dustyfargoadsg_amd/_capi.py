@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(_HERE, "libdfargo.so")
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC, ERR_STATE = range(6)
 FP64, FP32 = 0, 1
 BC_WALL, BC_OPEN = 0, 1
+FRAME_FIXED, FRAME_COROTATING, FRAME_GUIDING_CENTER, FRAME_BINARY_CENTER = 0, 1, 2, 3
+DAMP_TO_AXI, DAMP_TO_INI, DAMP_TO_VISCOUS = 0, 1, 2
 
 FIELDS = {
     "potential": 0, "soundspeed": 1, "pressure": 2, "temperature": 3,
@@ -57,6 +59,9 @@ class Params(ctypes.Structure):
         ("corotating", c_int), ("omega_frame", c_double),
         ("fast_transport", c_int), ("add_floors", c_int), ("density_floor", c_double),
         ("boundary", c_int),
+        ("corotate_with_outer_planet", c_int),
+        ("wave_killing_zones", c_int), ("wkz_rmin", c_double), ("wkz_rmax", c_double),
+        ("damping_ratio", c_double), ("damp_to", c_int), ("zero_vrad_bc", c_int),
     ]
 
     def copy(self):
@@ -182,6 +187,9 @@ def default_params(**overrides):
     p.corotating, p.omega_frame = 0, 0.0
     p.fast_transport, p.add_floors, p.density_floor = 1, 1, 1e-9
     p.boundary = BC_WALL
+    p.corotate_with_outer_planet = 0
+    p.wave_killing_zones, p.wkz_rmin, p.wkz_rmax = 0, 0.0, 0.0
+    p.damping_ratio, p.damp_to, p.zero_vrad_bc = 0.3, DAMP_TO_AXI, 0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(f"unknown parameter {k!r}")

@@ -46,7 +46,7 @@ struct dfargo_ctx {
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<hipEvent_t> evpool;
-  std::vector<double> VthetaMed, gcs_host;
+  std::vector<double> VthetaMed, VradMed, gcs_host;
   std::vector<float> fstage;
   // per-DT diagnostics: ring partials and sums (allocated on first use)
   double *diag_part = nullptr, *diag_out = nullptr;
@@ -111,6 +111,16 @@ static int ctrl_push(dfargo_ctx *c) {
 static int ctrl_pull(dfargo_ctx *c) {
   HIPCHK(hipMemcpyAsync(c->h_ctrl, c->d_ctrl, sizeof(dfargo_ctrl), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return DFARGO_OK;
+}
+
+// a corotating frame follows planet 0 (or the outermost), a binary
+// frame planets 0 and 1: GetPsysInfo reads them unchecked (src/Psys.c:246-271)
+static int check_frame_planets(const dfargo_params &p, const dfargo_planets *s) {
+  const int need = (p.corotating == DFARGO_FRAME_BINARY_CENTER) ? 2 : (p.corotating ? 1 : 0);
+  if (s->nb < need)
+    return dfargo_set_error(DFARGO_ERR_ARG, "FRAME %s needs %d planet(s), got %d",
+                            p.corotating == DFARGO_FRAME_BINARY_CENTER ? "B" : "C/G", need, s->nb);
   return DFARGO_OK;
 }
 
@@ -219,6 +229,8 @@ static void build_sequence(dfargo_ctx *c) {
   c->seq.push_back(KID_SOURCE);
   if (c->p.energy_equation) c->seq.push_back(KID_ENERGY);
   for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_FINISH}) c->seq.push_back(id);
+  if (c->p.wave_killing_zones && (c->a.wkz_n_in > 0 || c->a.wkz_i_out < c->p.nrad))
+    c->seq.push_back(KID_EVANESCENT);
 }
 
 static int build_context(dfargo_ctx *c) {
@@ -246,6 +258,20 @@ static int build_context(dfargo_ctx *c) {
   a.exclude_hill = p.exclude_hill;
   a.roche = p.roche_smoothing != 0.0;
   a.forced_circular = p.forced_circular;
+  a.frame = p.corotating;
+  a.frame_outer = p.corotate_with_outer_planet;
+  // the wave-killing zones as ring ranges (Rmed increases with i)
+  a.wkz_n_in = 0;
+  a.wkz_i_out = nr;
+  if (p.wave_killing_zones) {
+    while (a.wkz_n_in < nr && t.Rmed[a.wkz_n_in] < p.wkz_rmin) a.wkz_n_in++;
+    while (a.wkz_i_out > 0 && t.Rmed[a.wkz_i_out - 1] > p.wkz_rmax) a.wkz_i_out--;
+  }
+  a.wkz_rmin = p.wkz_rmin;
+  a.wkz_rmax = p.wkz_rmax;
+  a.damp_ratio = p.damping_ratio;
+  a.damp_ini = p.damp_to == DFARGO_DAMP_TO_INI;
+  a.zero_vrad = p.zero_vrad_bc;
   a.xcd_remap = 1;
   {
     hipDeviceProp_t prop;
@@ -310,6 +336,8 @@ static int build_context(dfargo_ctx *c) {
   }
   c->VthetaMed.assign(n1, 0.0);
   if ((rc = upload_table(c, &a.VthetaMed, c->VthetaMed.data(), n1))) return rc;
+  c->VradMed.assign(n1, 0.0);
+  if ((rc = upload_table(c, &a.VradMed, c->VradMed.data(), n1))) return rc;
   // fields: (nr+2) rings so that ring nr (vr outer face) exists with slack
   c->esz = (c->prec == DFARGO_FP64) ? sizeof(double) : sizeof(float);
   c->nfield = (size_t)(nr + 2) * ns;
@@ -502,6 +530,7 @@ extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad,
   dfargo_planets sys;
   if (sys_in) sys = *sys_in;
   else memset(&sys, 0, sizeof sys);
+  if ((rc = check_frame_planets(p, &sys))) return rc;
   std::vector<double> zero(n, 0.0);
   if ((rc = put_field(c, a.rho, rho, n)) || (rc = put_field(c, a.vr, zero.data(), n)) ||
       (rc = put_field(c, a.vt, zero.data(), n)))
@@ -589,8 +618,14 @@ extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad,
         vr[l] -= 3.0 * viscosity / r * (-p.sigma_slope + .5);
     }
   }
-  for (int i = 0; i < nr; i++) c->VthetaMed[i] = vt[(size_t)i * ns] + t.Rmed[i] * p.omega_frame;
+  // the initial profiles the wave-killing zones may damp towards
+  // (src/Pframeforce.c:635-640)
+  for (int i = 0; i < nr; i++) {
+    c->VthetaMed[i] = vt[(size_t)i * ns] + t.Rmed[i] * p.omega_frame;
+    c->VradMed[i] = vr[(size_t)i * ns];
+  }
   HIPCHK(hipMemcpy((void *)a.VthetaMed, c->VthetaMed.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)a.VradMed, c->VradMed.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
   memcpy(vrad, vr.data(), sizeof(double) * n);
   memcpy(vtheta, vt.data(), sizeof(double) * n);
   if ((rc = put_field(c, a.vr, vr.data(), n)) || (rc = put_field(c, a.vt, vt.data(), n))) return rc;
@@ -671,6 +706,7 @@ extern "C" int dfargo_upload_state(dfargo_ctx *c, const double *rho, const doubl
   if (vtheta && (rc = put_field(c, c->a.vt, vtheta, n))) return rc;
   if (energy && c->p.energy_equation && (rc = put_field(c, c->a.en, energy, n))) return rc;
   if (sys) {
+    if ((rc = check_frame_planets(c->p, sys))) return rc;
     if ((rc = ctrl_pull(c))) return rc;
     planets_to_ctrl(c->h_ctrl, sys);
     if ((rc = ctrl_push(c))) return rc;

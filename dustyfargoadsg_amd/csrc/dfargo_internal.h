@@ -79,6 +79,9 @@ typedef struct dfargo_ctrl {
   /* the particle set holding the current state, and the particles outside
      the grid counted by this sub-step's move */
   int pset, out_count;
+  /* corotating frame: the change of OmegaFrame this sub-step
+     (CorrectVtheta's domega) and GetPsysInfo's mark */
+  double domega, mark_x, mark_y;
 } dfargo_ctrl;
 
 #endif

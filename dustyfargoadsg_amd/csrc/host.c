@@ -34,6 +34,9 @@ void dfargo_default_params(dfargo_params *p) {
   p->eccentricity = 0.0; p->corotating = 0; p->omega_frame = 0.0;
   p->fast_transport = 1; p->add_floors = 1; p->density_floor = 1e-9;
   p->boundary = DFARGO_BC_WALL;
+  p->corotate_with_outer_planet = 0;
+  p->wave_killing_zones = 0; p->wkz_rmin = 0.0; p->wkz_rmax = 0.0;
+  p->damping_ratio = 0.3; p->damp_to = DFARGO_DAMP_TO_AXI; p->zero_vrad_bc = 0;
 }
 
 int dfargo_check_params(const dfargo_params *p) {
@@ -55,8 +58,17 @@ int dfargo_check_params(const dfargo_params *p) {
     return dfargo_set_error(DFARGO_ERR_ARG, "self-gravity needs a logarithmic radial grid");
   if (p->energy_equation && p->adiabatic_index == 1.0)
     return dfargo_set_error(DFARGO_ERR_ARG, "the energy equation needs ADIABATICINDEX != 1");
-  if (p->corotating)
-    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "FRAME C (corotating frame) is not implemented");
+  if (p->corotating < DFARGO_FRAME_FIXED || p->corotating > DFARGO_FRAME_BINARY_CENTER)
+    return dfargo_set_error(DFARGO_ERR_ARG, "unknown FRAME %d", p->corotating);
+  if (p->wave_killing_zones && (p->wkz_rmin == 0.0 || p->wkz_rmax == 0.0)) /* src/Interpret.c:471-476 */
+    return dfargo_set_error(DFARGO_ERR_ARG, "wave-killing zones need WKZRMIN and WKZRMAX");
+  if (p->wave_killing_zones && p->wkz_rmin > p->wkz_rmax)
+    return dfargo_set_error(DFARGO_ERR_ARG, "WKZRMIN must not exceed WKZRMAX");
+  if (p->wave_killing_zones && p->damp_to == DFARGO_DAMP_TO_VISCOUS)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
+                            "DAMPTOVISCOUS (1-D viscous evolution profiles) is not implemented");
+  if (p->wave_killing_zones && p->damp_to != DFARGO_DAMP_TO_AXI && p->damp_to != DFARGO_DAMP_TO_INI)
+    return dfargo_set_error(DFARGO_ERR_ARG, "unknown damping target %d", p->damp_to);
   if (p->boundary != DFARGO_BC_WALL && p->boundary != DFARGO_BC_OPEN)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "only WALL and OPEN boundaries are implemented");
   if (p->self_gravity && (p->nrad & (p->nrad - 1)))
@@ -367,9 +379,9 @@ int dfargo_check_dust_params(const dfargo_params *p, const dfargo_dust_params *d
   if (d->zz_integrator) return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "ZZINTEGRATOR is not implemented");
   if (d->dust_feel_sg_zero_mode)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "DUSTFEELSG Z (axisymmetric SG only) is not implemented");
-  if (p->omega_frame != 0.0)
+  if (p->omega_frame != 0.0 || p->corotating)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
-                            "particles in a rotating frame (OMEGAFRAME != 0) are not implemented");
+                            "particles in a rotating frame (OMEGAFRAME != 0, FRAME C/G/B) are not implemented");
   return DFARGO_OK;
 }
 

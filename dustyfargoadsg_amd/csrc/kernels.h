@@ -25,6 +25,12 @@ struct KArgs {
   int nr, ns;
   // switches decoded from dfargo_params
   int energy, visc_on, alpha_on, fast, floors, bc, sg, beta_cooling, viscous_heating;
+  // frame (enum dfargo_frame) and whether it follows the outermost planet
+  int frame, frame_outer;
+  // wave-killing zones: rings 0 .. wkz_n_in-1 and wkz_i_out .. nr-1
+  int wkz_n_in, wkz_i_out, damp_ini, zero_vrad;
+  double wkz_rmin, wkz_rmax, damp_ratio;
+  const double *VradMed;
   int gas_indirect, indirect_term, alt_indirect, exclude_hill, roche, forced_circular;
   int xcd_remap;
   int num_cu; // compute units of the device
@@ -98,7 +104,8 @@ struct KArgs {
 enum KernelId {
   KID_SG_ROWS_FWD = 0, KID_SG_COLS, KID_SG_ROWS_INV,
   KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
-  KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER, KID_COUNT
+  KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER,
+  KID_EVANESCENT, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 

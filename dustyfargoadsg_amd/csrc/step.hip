@@ -16,7 +16,7 @@ const char *kKernelNames[KID_COUNT] = {
     "sg_rows_fwd", "sg_cols", "sg_rows_inv",
     "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
     "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
-    "dust_sort",     "dust_deposit", "dust_reorder"};
+    "dust_sort",     "dust_deposit", "dust_reorder", "evanescent"};
 
 // ------------------------------------------------------------------------
 // helpers
@@ -64,7 +64,7 @@ template <typename T> __device__ __forceinline__ void st_sr(void *p, long i, dou
   ((T *)p)[i] = sr_round<T>(v, key);
 }
 // salts of the stochastically rounded stores
-enum { SR_VR = 1, SR_VT, SR_E, SR_EN, SR_RAD = 8, SR_AZI = 16 };
+enum { SR_VR = 1, SR_VT, SR_E, SR_EN, SR_RAD = 8, SR_AZI = 16, SR_EVA = 24 };
 
 // a / b from the hardware reciprocal, two Newton steps and one correction
 // of the quotient: within an ulp or two of the IEEE quotient for operands
@@ -288,6 +288,54 @@ __device__ double indpot_at(const KArgs &a, const dfargo_ctrl *c, int i, int j) 
 // ------------------------------------------------------------------------
 // per-call and per-sub-step control
 // ------------------------------------------------------------------------
+
+// GetPsysInfo (src/Psys.c:233-350), actions MARK (get = 0) and GET
+// (get = 1).  The frame's centre is planet 0 (the outermost planet with
+// COROTATEWITHOUTERPLANET) or, with FRAME B, the barycentre of planets 0 and
+// 1, moved with FRAME G to the guiding centre of its orbit unless e < 1e-8.
+// GET returns the angle the centre turned through since the mark.  The
+// orbital frequency the reference also forms (with its self-gravity
+// correction) serves only the FREQUENCY action and the guiding centre's
+// velocity, which MARK and GET do not use.
+__device__ double psys_info(const KArgs &a, dfargo_ctrl *c, int get) {
+  const int k = a.frame_outer ? c->nb - 1 : 0;
+  double x = c->pl_x[k], y = c->pl_y[k], vx = c->pl_vx[k], vy = c->pl_vy[k], m = c->pl_m[k] + 1.;
+  if (a.frame == DFARGO_FRAME_BINARY_CENTER) {
+    double m0 = c->pl_m[0], m1 = c->pl_m[1];
+    if (m0 == 0.0) m0 = c->pl_m1[0];
+    if (m1 == 0.0) m1 = c->pl_m1[1];
+    x = (m0 * c->pl_x[0] + m1 * c->pl_x[1]) / (m0 + m1);
+    y = (m0 * c->pl_y[0] + m1 * c->pl_y[1]) / (m0 + m1);
+    vx = (m0 * c->pl_vx[0] + m1 * c->pl_vx[1]) / (m0 + m1);
+    vy = (m0 * c->pl_vy[0] + m1 * c->pl_vy[1]) / (m0 + m1);
+    m = m0 + m1 + 1.;
+  }
+  double xc = x, yc = y;
+  if (a.frame == DFARGO_FRAME_GUIDING_CENTER) {
+    const double h = x * vy - y * vx, d = sqrt(x * x + y * y);
+    const double Ax = x * vy * vy - y * vx * vy - DF_G * m * x / d;
+    const double Ay = y * vx * vx - x * vx * vy - DF_G * m * y / d;
+    const double e = sqrt(Ax * Ax + Ay * Ay) / m, sa = h * h / DF_G / m / (1. - e * e);
+    const double arg = (e == 0.0) ? 1.0 : (1.0 - d / sa) / e;
+    double E = (fabs(arg) >= 1.0) ? DF_PI * (1. - arg / fabs(arg)) / 2. : acos((1.0 - d / sa) / e);
+    if ((x * y * (vy * vy - vx * vx) + vx * vy * (x * x - y * y)) < 0) E = -E;
+    const double M = E - e * sin(E), pa = atan2(Ay, Ax);
+    if (!(e < 1e-8)) {
+      xc = sa * cos(M + pa);
+      yc = sa * sin(M + pa);
+    }
+  }
+  if (!get) {
+    c->mark_x = xc;
+    c->mark_y = yc;
+    return 0.0;
+  }
+  const double d2 = sqrt(xc * xc + yc * yc), d1 = sqrt(c->mark_x * c->mark_x + c->mark_y * c->mark_y);
+  const double cross = c->mark_x * yc - xc * c->mark_y;
+  c->mark_x = xc;
+  c->mark_y = yc;
+  return asin(cross / (d1 * d2));
+}
 
 // start of a sub-step: mass taper, n-body limit and the force-sum targets
 // (src/SourceEuler.c:398-440, src/Pframeforce.c:131-161); one thread, run
@@ -698,6 +746,9 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
     c->pot_invd3[k] = invd3;
     c->pot_smooth[k] = sm * sm;
   }
+  // the corotating frame's mark (src/SourceEuler.c:476), before the
+  // disc's kick changes the velocities
+  if (a.frame) psys_info(a, c, 0);
   // AdvanceSystemFromDisk (src/Pframeforce.c:131-161)
   for (int tg = 1; tg < c->ntarget; tg++) {
     int k = c->tg_planet[tg];
@@ -745,6 +796,14 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
       c->pl_y[0] = new_r * sin(dtheta + theta);
       c->pl_vx[0] = vx * cos(dtheta + theta) - vy * sin(dtheta + theta);
       c->pl_vy[0] = vx * sin(dtheta + theta) + vy * cos(dtheta + theta);
+    }
+    // the frame follows its centre (src/SourceEuler.c:516-524); the
+    // source kernel applies CorrectVtheta (vt -= domega Rmed) where it
+    // reads vt
+    if (a.frame) {
+      const double omega_new = psys_info(a, c, 1) / dt;
+      c->domega = omega_new - c->omega_frame;
+      c->omega_frame = omega_new;
     }
     // RotatePsys (src/Psys.c:352-370)
     double angle = c->omega_frame * dt, sint = sin(angle), cost = cos(angle);
@@ -953,6 +1012,8 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
     if (p < SRC_PC) sPot[q][p] = in ? pot_at(a, c, r, sCol[p]) : 0.0;
   }
   __syncthreads();
+  // the change of the corotating frame's rate this sub-step
+  const double dom = a.frame ? c->domega : 0.0;
   // stage 1: SubStep1 on rings i0-3 .. i0+SRC_TR+1 (VradInt and VthetaInt
   // of ring nrad are zero, as in the reference where they are never set)
 #pragma unroll
@@ -976,8 +1037,16 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
       const double plim = sP[q][p + 1], idrm = STAB(ST_IDRM, i);
       double gradp = (pl - plim) * 2.0 / (rl + (double)sRho[q][p + 1]) * idrm;
       double gradphi = (potl - sPot[q][p + 1]) * idrm;
-      double vt2 = (double)sVth[q + 1][p + 1] + (double)sVth[q + 1][p + 2] + (double)sVth[q][p + 1] +
-                   (double)sVth[q][p + 2];
+      double vt2;
+      if (a.frame) {
+        // CorrectVtheta (src/SideEuler.c:1133-1148) on the values read
+        const double ci = dom * STAB(ST_RMED, i), cim = dom * STAB(ST_RMED, i - 1);
+        vt2 = ((double)sVth[q + 1][p + 1] - ci) + ((double)sVth[q + 1][p + 2] - ci) +
+              ((double)sVth[q][p + 1] - cim) + ((double)sVth[q][p + 2] - cim);
+      } else {
+        vt2 = (double)sVth[q + 1][p + 1] + (double)sVth[q + 1][p + 2] + (double)sVth[q][p + 1] +
+              (double)sVth[q][p + 2];
+      }
       vt2 = vt2 / 4.0 + STAB(ST_RINF, i) * c->omega_frame;
       vt2 = vt2 * vt2;
       vri = (double)pf_vr[it] + dt * (-gradp - gradphi + vt2 * STAB(ST_IRINF, i));
@@ -998,7 +1067,9 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
     const double pljm = sP[q + 1][p];
     double gradp = (pl - pljm) * 2.0 / (rl + (double)sRho[q + 1][p]) * invdxtheta;
     double gradphi = (potl - sPot[q + 1][p]) * invdxtheta;
-    double vti = (double)sVth[q + 1][p + 1] - dt * (gradp + gradphi);
+    double vti = (double)sVth[q + 1][p + 1];
+    if (a.frame) vti = vti - dom * STAB(ST_RMED, i);
+    vti = vti - dt * (gradp + gradphi);
     vti += dt * STAB(ST_SUPP, i);
     if (a.d_on && a.d_feedback) vti += dt * ld<T>(a.azifbacc, l);
     if (dgrid) {
@@ -1581,6 +1652,74 @@ template <typename T> __global__ void __launch_bounds__(256) k_finish(KArgs a) {
   if (a.energy) st<T>(a.en, l, s.e);
 }
 
+// EvanescentBoundary (src/SideEuler.c:796-950) for the gas, after the
+// boundary conditions: in the wave-killing zones Rmed < WKZRMIN and
+// Rmed > WKZRMAX each ring relaxes as q <- (q + lambda q0) / (1 + lambda)
+// towards its azimuthal mean (DampToAxi, FP64 sums) or the initial profile
+// (DampToIni), lambda = x^2 dt / (ratio Rmed^1.5) with x rising from 0 at
+// the zone's edge to 1 at the grid's edge.  With ZEROVRADBC the two
+// innermost and the outermost ring take vr = 0 and the target density and
+// vtheta.  One 256-thread workgroup per ring of the zones; each thread
+// reads and rewrites the same cells, so the update is in place.
+enum { EV_TB = 256 };
+template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int b = blockIdx.x, nr = a.nr, ns = a.ns, tid = threadIdx.x;
+  const int i = (b < a.wkz_n_in) ? b : a.wkz_i_out + (b - a.wkz_n_in);
+  const long base = (long)i * ns;
+  const double R = a.Rmed[i];
+  const bool in_zone = R < a.wkz_rmin;
+  const double damping = in_zone ? (R - a.wkz_rmin) / (a.Rmed[0] - a.wkz_rmin)
+                                 : (R - a.wkz_rmax) / (a.Rmed[nr - 1] - a.wkz_rmax);
+  const double Tz = a.damp_ratio * pow(R, 1.5);
+  const double lambda = damping * damping * c->dt / Tz;
+  double q0[4];
+  if (a.damp_ini) {
+    q0[0] = a.VradMed[i];
+    q0[1] = a.VthetaMed[i] - R * c->omega_frame;
+    q0[2] = a.SigmaMed[i];
+    q0[3] = a.EnergyMed[i];
+  } else {
+    __shared__ double part[EV_TB / 64][4];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = tid; j < ns; j += EV_TB) {
+      s[0] += ld<T>(a.vr, base + j);
+      s[1] += ld<T>(a.vt, base + j);
+      s[2] += ld<T>(a.rho, base + j);
+      if (a.energy) s[3] += ld<T>(a.en, base + j);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) s[k] = wave_sum(s[k]);
+    if ((tid & 63) == 0)
+#pragma unroll
+      for (int k = 0; k < 4; k++) part[tid >> 6][k] = s[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      double t = 0.0;
+      for (int w = 0; w < EV_TB / 64; w++) t += part[w][k];
+      q0[k] = t / (double)ns;
+    }
+  }
+  const bool edge = a.zero_vrad && ((in_zone && (i == 0 || i == 1)) || (!in_zone && i == nr - 1));
+  for (int j = tid; j < ns; j += EV_TB) {
+    const long l = base + j;
+    double vr = (ld<T>(a.vr, l) + lambda * q0[0]) / (1.0 + lambda);
+    double vt = (ld<T>(a.vt, l) + lambda * q0[1]) / (1.0 + lambda);
+    double rho = (ld<T>(a.rho, l) + lambda * q0[2]) / (1.0 + lambda);
+    if (edge) {
+      vr = 0.0;
+      rho = q0[2];
+      vt = q0[1];
+    }
+    st_sr<T>(a.vr, l, vr, sr_key(c, l, SR_EVA));
+    st_sr<T>(a.vt, l, vt, sr_key(c, l, SR_EVA + 1));
+    st_sr<T>(a.rho, l, rho, sr_key(c, l, SR_EVA + 2));
+    if (a.energy) st_sr<T>(a.en, l, (ld<T>(a.en, l) + lambda * q0[3]) / (1.0 + lambda), sr_key(c, l, SR_EVA + 3));
+  }
+}
+
 // Potential grid on demand (the field of FillForcesArrays)
 template <typename T> __global__ void k_potential(KArgs a) {
   const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1637,6 +1776,9 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
   switch (id) {
     case KID_CFL: RING_DISPATCH_K(wide_k(a.ns), k_cfl_ring, T, a, s, 0); break;
     case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
+    case KID_EVANESCENT:
+      hipLaunchKernelGGL(k_evanescent<T>, dim3(a.wkz_n_in + (a.nr - a.wkz_i_out)), dim3(EV_TB), 0, s, a);
+      break;
     case KID_SOURCE:
       hipLaunchKernelGGL(k_source<T>, dim3((a.ns + SRC_TC - 1) / SRC_TC, (a.nr + SRC_TR - 1) / SRC_TR), dim3(SrcBlock<T>::n), 0, s,
                          a);

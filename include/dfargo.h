@@ -39,6 +39,24 @@ enum dfargo_boundary {
   DFARGO_BC_OPEN = 1   /* INNERBOUNDARY OPEN (src/SideEuler.c:300-408)     */
 };
 
+/* FRAME (src/Interpret.c:225-233): fixed (rotating at OMEGAFRAME) or
+ * corotating with a planet, its guiding centre or a binary's barycentre
+ * (GetPsysInfo, src/Psys.c:233-350). */
+enum dfargo_frame {
+  DFARGO_FRAME_FIXED = 0,           /* FRAME F                               */
+  DFARGO_FRAME_COROTATING = 1,      /* FRAME C                               */
+  DFARGO_FRAME_GUIDING_CENTER = 2,  /* FRAME G                               */
+  DFARGO_FRAME_BINARY_CENTER = 3    /* FRAME B                               */
+};
+
+/* What the wave-killing zones damp towards (src/Interpret.c:344-358). */
+enum dfargo_damp {
+  DFARGO_DAMP_TO_AXI = 0,      /* the ring's instantaneous mean (default)  */
+  DFARGO_DAMP_TO_INI = 1,      /* the initial profiles (DAMPTOINI)         */
+  DFARGO_DAMP_TO_VISCOUS = 2   /* a 1-D viscous evolution (DAMPTOVISCOUS;
+                                  rejected: not implemented)               */
+};
+
 /* Parameter block: the .par keys the hydro step reads (src/var.c:15-202),
  * already decoded the way src/Interpret.c:165-476 decodes them. */
 typedef struct dfargo_params {
@@ -79,13 +97,23 @@ typedef struct dfargo_params {
   double release_date, release_radius;
   double eccentricity;               /* ECCENTRICITY                        */
   /* frame */
-  int    corotating;                 /* FRAME C (only 0 is implemented)     */
-  double omega_frame;                /* OMEGAFRAME                          */
+  int    corotating;                 /* FRAME, enum dfargo_frame            */
+  double omega_frame;                /* OMEGAFRAME, or with a corotating
+                                        frame the start-up frequency the host
+                                        sets (src/main.c:229-246)           */
   /* numerics */
   int    fast_transport;             /* TRANSPORT FARGO (1) or STANDARD (0) */
   int    add_floors;                 /* ADDFLOORS (default YES)             */
   double density_floor;              /* DENSITYFLOOR                        */
   int    boundary;                   /* enum dfargo_boundary                */
+  /* frame (continued) */
+  int    corotate_with_outer_planet; /* COROTATEWITHOUTERPLANET             */
+  /* wave-killing zones (EvanescentBoundary, src/SideEuler.c:796-950) */
+  int    wave_killing_zones;         /* WAVEKILLINGZONES                    */
+  double wkz_rmin, wkz_rmax;         /* WKZRMIN, WKZRMAX                    */
+  double damping_ratio;              /* DAMPINGTODYNAMICALTIMERATIO (0.3)   */
+  int    damp_to;                    /* enum dfargo_damp                    */
+  int    zero_vrad_bc;               /* ZEROVRADBC                          */
 } dfargo_params;
 
 /* PlanetarySystem (src/types.h:97-111) plus the two mass tables that

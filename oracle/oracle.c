@@ -73,6 +73,7 @@ struct or_ctx {
   /* scalars */
   double PhysicalTime, PhysicalTimeInitial, OmegaFrame, MINDT, MAXDT, LostMass;
   double MassTaper;
+  double Xplanet, Yplanet; /* GetPsysInfo's mark (src/Psys.c:334-344) */
   pair_t dop, dop_alt, indirect;
   double barx[3], bary[3];
   int GasTimeStepsCFL, last_substeps, crashed_d, crashed_e, crashed;
@@ -788,6 +789,61 @@ static void rotate_psys(or_ctx *c, double angle) {
     c->sys.vx[i] = xt * cost + yt * sint;
     c->sys.vy[i] = -xt * sint + yt * cost;
   }
+}
+
+/* GetPsysInfo (src/Psys.c:233-350), actions MARK (get = 0) and GET
+ * (get = 1).  The frame's centre is planet 0 (the outermost planet with
+ * COROTATEWITHOUTERPLANET) or, with FRAME B, the barycentre of planets 0 and
+ * 1, moved with FRAME G to the guiding centre of its orbit unless e < 1e-8.
+ * GET returns the angle the centre turned through since the mark.  The
+ * orbital frequency the reference also forms (with its self-gravity
+ * correction) serves only the FREQUENCY action and the guiding centre's
+ * velocity, which MARK and GET do not use. */
+static double psys_info(or_ctx *c, int get) {
+  const dfargo_planets *s = &c->sys;
+  const int k = c->p.corotate_with_outer_planet ? s->nb - 1 : 0;
+  double x = s->x[k], y = s->y[k], vx = s->vx[k], vy = s->vy[k], m = s->mass[k] + 1.;
+  if (c->p.corotating == DFARGO_FRAME_BINARY_CENTER) {
+    double m0 = s->mass[0], m1 = s->mass[1];
+    if (m0 == 0.0) m0 = s->final_mass[0];
+    if (m1 == 0.0) m1 = s->final_mass[1];
+    x = (m0 * s->x[0] + m1 * s->x[1]) / (m0 + m1);
+    y = (m0 * s->y[0] + m1 * s->y[1]) / (m0 + m1);
+    vx = (m0 * s->vx[0] + m1 * s->vx[1]) / (m0 + m1);
+    vy = (m0 * s->vy[0] + m1 * s->vy[1]) / (m0 + m1);
+    m = m0 + m1 + 1.;
+  }
+  double xc = x, yc = y;
+  if (c->p.corotating == DFARGO_FRAME_GUIDING_CENTER) {
+    const double h = x * vy - y * vx, d = sqrt(x * x + y * y);
+    const double Ax = x * vy * vy - y * vx * vy - G * m * x / d;
+    const double Ay = y * vx * vx - x * vx * vy - G * m * y / d;
+    const double e = sqrt(Ax * Ax + Ay * Ay) / m, a = h * h / G / m / (1. - e * e);
+    const double arg = (e == 0.0) ? 1.0 : (1.0 - d / a) / e;
+    double E = (fabs(arg) >= 1.0) ? M_PI * (1. - arg / fabs(arg)) / 2. : acos((1.0 - d / a) / e);
+    if ((x * y * (vy * vy - vx * vx) + vx * vy * (x * x - y * y)) < 0) E = -E;
+    const double M = E - e * sin(E), pa = atan2(Ay, Ax);
+    if (!(e < 1e-8)) {
+      xc = a * cos(M + pa);
+      yc = a * sin(M + pa);
+    }
+  }
+  if (!get) {
+    c->Xplanet = xc;
+    c->Yplanet = yc;
+    return 0.0;
+  }
+  const double d2 = sqrt(xc * xc + yc * yc), d1 = sqrt(c->Xplanet * c->Xplanet + c->Yplanet * c->Yplanet);
+  const double cross = c->Xplanet * yc - xc * c->Yplanet;
+  c->Xplanet = xc;
+  c->Yplanet = yc;
+  return asin(cross / (d1 * d2));
+}
+
+/* CorrectVtheta (src/SideEuler.c:1133-1148) */
+static void correct_vtheta(or_ctx *c, double domega) {
+  for (int i = 0; i < c->nr; i++)
+    for (int j = 0; j < c->ns; j++) c->vt[i * c->ns + j] -= domega * c->Rmed[i];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1560,6 +1616,69 @@ static void apply_boundary_condition(or_ctx *c) {
   if (c->p.boundary == DFARGO_BC_OPEN) open_boundary(c);
 }
 
+/* EvanescentBoundary (src/SideEuler.c:796-950) for the gas: in the
+ * wave-killing zones Rmed < WKZRMIN and Rmed > WKZRMAX each ring relaxes as
+ * q <- (q + lambda q0) / (1 + lambda) towards its azimuthal mean (DampToAxi)
+ * or the initial profile (DampToIni), lambda = x^2 step / (ratio Rmed^1.5)
+ * with x rising from 0 at the zone's edge to 1 at the grid's edge.  With
+ * ZEROVRADBC the two innermost and the outermost ring of the zones take
+ * vr = 0 and the target density and vtheta. */
+static void evanescent_boundary(or_ctx *c, double step) {
+  const dfargo_params *p = &c->p;
+  const int nr = c->nr, ns = c->ns;
+  double lambda = 0.0;
+  for (int i = 0; i < nr; i++) {
+    const double R = c->Rmed[i];
+    if (!((R < p->wkz_rmin) || (R > p->wkz_rmax))) continue;
+    if (R < p->wkz_rmin) {
+      const double damping = (R - p->wkz_rmin) / (c->Rmed[0] - p->wkz_rmin);
+      const double Tin = p->damping_ratio * pow(R, 1.5);
+      lambda = damping * damping * step / Tin;
+    }
+    if (R > p->wkz_rmax) {
+      const double damping = (R - p->wkz_rmax) / (c->Rmed[nr - 1] - p->wkz_rmax);
+      const double Tout = p->damping_ratio * pow(R, 1.5);
+      lambda = damping * damping * step / Tout;
+    }
+    double vr0, vt0, d0, e0;
+    if (p->damp_to == DFARGO_DAMP_TO_INI) {
+      vt0 = c->VthetaMed[i] - R * c->OmegaFrame;
+      vr0 = c->VradMed[i];
+      d0 = c->SigmaMed[i];
+      e0 = c->EnergyMed[i];
+    } else {
+      vr0 = vt0 = d0 = e0 = 0.0;
+      for (int j = 0; j < ns; j++) {
+        const int l = i * ns + j;
+        vr0 += c->vr[l];
+        vt0 += c->vt[l];
+        d0 += c->rho[l];
+        e0 += c->en[l];
+      }
+      vr0 /= (double)ns;
+      vt0 /= (double)ns;
+      d0 /= (double)ns;
+      e0 /= (double)ns;
+    }
+    const int edge = p->zero_vrad_bc && (((R < p->wkz_rmin) && (i == 0 || i == 1)) ||
+                                         ((R > p->wkz_rmax) && (i == nr - 1)));
+    for (int j = 0; j < ns; j++) {
+      const int l = i * ns + j;
+      c->vr[l] = (c->vr[l] + lambda * vr0) / (1.0 + lambda);
+      c->vt[l] = (c->vt[l] + lambda * vt0) / (1.0 + lambda);
+      c->rho[l] = (c->rho[l] + lambda * d0) / (1.0 + lambda);
+      if (p->energy_equation) c->en[l] = (c->en[l] + lambda * e0) / (1.0 + lambda);
+      if (edge) {
+        c->vr[l] = 0.0;
+        c->rho[l] = d0;
+        c->vt[l] = vt0;
+      }
+    }
+  }
+}
+
+void or_evanescent_boundary(or_ctx *c, double step) { evanescent_boundary(c, step); }
+
 /* AddFloorDensity, AddFloorEnergy (src/SourceEuler.c:2506-2541) */
 static void add_floors(or_ctx *c) {
   for (int i = 0; i < c->nr; i++)
@@ -2079,6 +2198,7 @@ int or_algogas(or_ctx *c, double DT) {
     if (dt < c->MINDT) c->MINDT = dt;
     if (dt > c->MAXDT) c->MAXDT = dt;
     /* forces and planets (src/SourceEuler.c:472-500) */
+    if (p->corotating) psys_info(c, 0);
     c->dop = compute_accel(c, c->rho, 0.0, 0.0, p->indirect_term_smoothing, 0.0);
     c->dop_alt = opposite_acc_barycenter(c, dt);
     fill_forces_arrays(c);
@@ -2089,6 +2209,12 @@ int or_algogas(or_ctx *c, double DT) {
     if (c->dust_on && c->np != 0) {
       dust_semi_update_positions(c, 2. * dt);
       dust_update_velocities(c, dt);
+    }
+    /* the frame follows its centre (src/SourceEuler.c:516-524) */
+    if (p->corotating) {
+      const double omega_new = psys_info(c, 1) / dt;
+      correct_vtheta(c, omega_new - c->OmegaFrame);
+      c->OmegaFrame = omega_new;
     }
     rotate_psys(c, c->OmegaFrame * dt);
     if (c->dust_on && c->np != 0) dust_rotate(c, c->OmegaFrame * dt);
@@ -2119,6 +2245,7 @@ int or_algogas(or_ctx *c, double DT) {
     or_transport(c, dt);
     if (p->add_floors) add_floors(c);
     apply_boundary_condition(c);
+    if (p->wave_killing_zones) evanescent_boundary(c, dt);
     compute_temperature(c);
     c->PhysicalTime += dt;
     nsub++;

@@ -47,6 +47,8 @@ void or_set_state(or_ctx *c, const double *rho, const double *vrad,
 void or_get_state(or_ctx *c, double *rho, double *vrad, double *vtheta,
                   double *energy, dfargo_planets *sys);
 void or_set_time(or_ctx *c, double t, double t0, double omega_frame);
+/* EvanescentBoundary on the current state, for the pins of its formula */
+void or_evanescent_boundary(or_ctx *c, double step);
 
 /* AlgoGas; returns the number of sub-steps taken */
 int or_algogas(or_ctx *c, double DT);

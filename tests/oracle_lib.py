@@ -40,6 +40,7 @@ def lib():
             "or_set_state": (None, [vp, dp, dp, dp, dp, PP]),
             "or_get_state": (None, [vp, dp, dp, dp, dp, PP]),
             "or_set_time": (None, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "or_evanescent_boundary": (None, [vp, ctypes.c_double]),
             "or_algogas": (ctypes.c_int, [vp, ctypes.c_double]),
             "or_get_status": (None, [vp, ctypes.POINTER(Status)]),
             "or_get_field": (None, [vp, ctypes.c_int, dp]),
@@ -155,6 +156,12 @@ class Oracle:
 
     def algogas(self, DT):
         return self.L.or_algogas(self.ctx, DT)
+
+    def evanescent(self, step):
+        self.L.or_evanescent_boundary(self.ctx, step)
+
+    def set_time(self, t, t0=0.0, omega_frame=0.0):
+        self.L.or_set_time(self.ctx, t, t0, omega_frame)
 
     def status(self):
         st = Status()

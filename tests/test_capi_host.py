@@ -113,7 +113,13 @@ def test_parameter_checks(lib):
     assert check(viscosity=1e-5, alpha_viscosity=1e-3) == _capi.ERR_ARG
     assert check(exclude_hill_factor=1.5) == _capi.ERR_ARG
     assert check(self_gravity=1, log_grid=0) == _capi.ERR_ARG
-    assert check(corotating=1) == _capi.ERR_UNSUPPORTED
+    assert check(corotating=_capi.FRAME_BINARY_CENTER) == _capi.OK
+    assert check(corotating=4) == _capi.ERR_ARG
+    assert check(wave_killing_zones=1) == _capi.ERR_ARG
+    assert check(wave_killing_zones=1, wkz_rmin=0.5, wkz_rmax=0.4) == _capi.ERR_ARG
+    assert check(wave_killing_zones=1, wkz_rmin=0.5, wkz_rmax=2.2,
+                 damp_to=_capi.DAMP_TO_VISCOUS) == _capi.ERR_UNSUPPORTED
+    assert check(wave_killing_zones=1, wkz_rmin=0.5, wkz_rmax=2.2, damp_to=_capi.DAMP_TO_INI) == _capi.OK
     assert check(self_gravity=1, log_grid=1, nsec=40) == _capi.ERR_UNSUPPORTED
     assert b"VISCOSITY" in (lib.dfargo_check_params(ctypes.byref(_capi.default_params(
         nrad=16, nsec=32, rmin=0.5, rmax=2.0, viscosity=1, alpha_viscosity=1))) and

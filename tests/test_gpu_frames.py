@@ -1,0 +1,105 @@
+"""Corotating frames (FRAME C, G, B and COROTATEWITHOUTERPLANET;
+src/SourceEuler.c:476,516-524, src/Psys.c:233-350, src/SideEuler.c:1133-1148)
+and the wave-killing zones (EvanescentBoundary, src/SideEuler.c:796-950) of
+the HIP step against the FP64 oracle, through the C ABI.  Same bars as the
+other parity tests: FP64 fields within 1e-12 of the oracle after two AlgoGas
+calls, sub-step counts and FARGO shifts exact; FP32 storage within 2e-5
+(Sigma) and 2e-6 (velocities).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from dustyfargoadsg_amd import _capi
+from dustyfargoadsg_amd.workloads import gas_planet, energy_sg
+from tests.test_gpu_parity import DT, compare, pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _frame_case(name):
+    m = 1e-3
+    if name == "corotating":
+        # starts in the inertial frame: the first sub-step's CorrectVtheta
+        # takes the whole OmegaFrame Rmed off v_theta; the planet feels the
+        # disc, so the frame's rate moves every sub-step
+        p, pl = gas_planet(nrad=32, nsec=96, forced_circular=0, corotating=_capi.FRAME_COROTATING)
+        pl = dict(pl, feel_disk=[1], feel_others=[1])
+    elif name == "guiding":
+        p, pl = gas_planet(nrad=32, nsec=96, forced_circular=0, eccentricity=0.1,
+                           corotating=_capi.FRAME_GUIDING_CENTER, omega_frame=math.sqrt(1 + m))
+        pl = dict(pl, feel_disk=[1], feel_others=[1])
+    else:
+        outer = name == "outer"
+        frame = _capi.FRAME_COROTATING if outer else _capi.FRAME_BINARY_CENTER
+        p, _ = gas_planet(nrad=32, nsec=96, forced_circular=0, corotating=frame,
+                          corotate_with_outer_planet=int(outer), omega_frame=math.sqrt(1 + m))
+        pl = dict(dist=[1.0, 1.6], mass=[m, 3e-4], acc=[0.0, 0.0], feel_disk=[1, 0],
+                  feel_others=[1, 1])
+    return p, pl
+
+
+@pytest.mark.parametrize("name", ["corotating", "guiding", "binary", "outer"])
+def test_frames_fp64(name):
+    p, pl = _frame_case(name)
+    o, so, d, sg = pair(p, pl)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+        assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
+    compare(o, d, p, 1e-12, 1e-12, 1e-12)
+    w_g, w_o = d.status().omega_frame, o.status().omega_frame
+    assert abs(w_g - w_o) <= 1e-12 * abs(w_o)
+    go, oo = d.state()["planets"], o.state()["planets"]
+    for k in ("x", "y", "vx", "vy"):
+        for i in range(oo.nb):
+            assert abs(getattr(go, k)[i] - getattr(oo, k)[i]) <= 1e-13, (k, i)
+
+
+def _wkz_case(name):
+    zones = dict(wave_killing_zones=1, wkz_rmin=0.55, wkz_rmax=2.2)
+    if name == "axi":
+        return gas_planet(nrad=32, nsec=96, **zones)
+    if name == "ini":
+        return gas_planet(nrad=32, nsec=96, damp_to=_capi.DAMP_TO_INI, **zones)
+    if name == "zero_vrad":
+        return gas_planet(nrad=32, nsec=96, zero_vrad_bc=1, **zones)
+    # the energy field is damped too; the K = 6 ring width of the large discs
+    return energy_sg(nrad=16, nsec=1536, wave_killing_zones=1, wkz_rmin=0.55, wkz_rmax=3.5)
+
+
+@pytest.mark.parametrize("name", ["axi", "ini", "zero_vrad", "energy_sg"])
+def test_wave_killing_zones_fp64(name):
+    p, pl = _wkz_case(name)
+    o, so, d, sg = pair(p, pl)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+        assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
+    compare(o, d, p, 1e-12, 1e-12, 1e-12, 1e-12)
+
+
+def test_frame_and_zones_fp32():
+    p, pl = gas_planet(nrad=128, nsec=384, forced_circular=0, corotating=_capi.FRAME_COROTATING,
+                       omega_frame=math.sqrt(1 + 1e-3), wave_killing_zones=1, wkz_rmin=0.55,
+                       wkz_rmax=2.2)
+    pl = dict(pl, feel_disk=[1], feel_others=[1])
+    o, so, d, sg = pair(p, pl, _capi.FP32)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+    compare(o, d, p, 2e-5, 2e-6, 2e-6)
+    assert abs(d.status().omega_frame - o.status().omega_frame) <= 1e-9
+
+
+def test_frame_needs_its_planets():
+    p, _ = gas_planet(nrad=32, nsec=96, corotating=_capi.FRAME_BINARY_CENTER)
+    pl = dict(dist=[1.0], mass=[1e-3], acc=[0.0], feel_disk=[0], feel_others=[0])
+    from dustyfargoadsg_amd.disc import Disc, DfargoError
+    d = Disc(p, _capi.FP64)
+    with pytest.raises(DfargoError):
+        d.initial_state(pl)

@@ -1,0 +1,209 @@
+"""Pins the oracle's corotating frames (GetPsysInfo MARK/GET, CorrectVtheta,
+src/SourceEuler.c:476,516-524) and its wave-killing zones
+(EvanescentBoundary, src/SideEuler.c:796-950) by invariants and closed
+forms.  Each test names the mistake it would catch.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from dustyfargoadsg_amd import _capi
+from dustyfargoadsg_amd.workloads import gas_planet
+from tests.oracle_lib import Oracle
+
+DT = 2 * math.pi / 20
+
+
+def _disc(nr=32, ns=96, planets=None, **over):
+    over.setdefault("forced_circular", 0)
+    p, pl = gas_planet(nrad=nr, nsec=ns, **over)
+    if planets is not None:
+        pl = planets
+    o = Oracle(p)
+    st = o.initial_state(pl)
+    return p, o, st
+
+
+def _one(dist, mass):
+    return dict(dist=[dist], mass=[mass], acc=[0.0], feel_disk=[0], feel_others=[0])
+
+
+def test_corotating_frame_holds_a_circular_planet(oracle_built):
+    """FRAME C with a planet on a circular orbit that feels no disc: GET's
+    angle over dt is its angular velocity sqrt(1 + m) (r = 1), and the
+    rotation by it keeps the planet at (1, 0).  A sign error in GET's cross
+    product, or the angle left undivided by dt, fails."""
+    m = 1e-3
+    w = math.sqrt(1 + m)
+    p, o, st = _disc(corotating=_capi.FRAME_COROTATING, omega_frame=w,
+                     planets=_one(1.0, m))
+    for _ in range(3):
+        o.algogas(DT)
+    pl = o.state()["planets"]
+    # the planet's RK5 truncation error is ~1e-9 per orbit fraction here
+    assert abs(pl.x[0] - 1.0) < 1e-7 and abs(pl.y[0]) < 1e-7
+    assert abs(o.status().omega_frame - w) < 1e-7
+
+
+def test_corotating_equals_the_fixed_frame_at_the_same_rate(oracle_built):
+    """Start in the inertial frame (OmegaFrame 0) with FRAME C: the first
+    sub-step finds OmegaFrame = sqrt(1 + m) and CorrectVtheta takes
+    OmegaFrame R off v_theta.  The run must then equal one in a frame fixed
+    at that rate from the start (the same disc with v_theta - OmegaFrame R),
+    up to the RK5 error in the measured rate.  CorrectVtheta with the wrong
+    sign or left out is off by O(1)."""
+    m = 1e-3
+    w = math.sqrt(1 + m)
+    pa, oa, sa = _disc(corotating=_capi.FRAME_COROTATING, omega_frame=0.0,
+                       planets=_one(1.0, m))
+    pb, ob, sb = _disc(omega_frame=w, planets=_one(1.0, m))
+    nr, ns = pa.nrad, pa.nsec
+    assert np.abs(sa["vt"][: nr * ns] - w * np.repeat(oa.tables()[1], ns) - sb["vt"][: nr * ns]).max() < 1e-14
+    for _ in range(2):
+        assert oa.algogas(DT) == ob.algogas(DT)
+    a, b = oa.state(), ob.state()
+    n = nr * ns
+    vscale = np.abs(b["vt"][:n]).max()
+    assert np.abs(a["rho"][:n] - b["rho"][:n]).max() <= 1e-8 * np.abs(b["rho"][:n]).max()
+    assert np.abs(a["vt"][:n] - b["vt"][:n]).max() <= 1e-8 * vscale
+    assert np.abs(a["vr"][:n] - b["vr"][:n]).max() <= 1e-8 * vscale
+    assert abs(oa.status().omega_frame - w) < 1e-7
+
+
+def test_guiding_centre_turns_at_the_mean_motion(oracle_built):
+    """FRAME G on an e = 0.1 orbit: the guiding centre a (cos, sin)(M + PA)
+    turns at the mean motion n = sqrt((1 + m) / a^3), while the planet
+    itself (FRAME C) turns at its instantaneous rate, 20 % off n near
+    apocentre.  A wrong mean anomaly (E for M, or E's sign) fails."""
+    m = 1e-3
+    n = math.sqrt(1 + m)
+    p, o, st = _disc(corotating=_capi.FRAME_GUIDING_CENTER, omega_frame=n, eccentricity=0.1,
+                     planets=_one(1.0, m))
+    for _ in range(4):
+        o.algogas(DT)
+    assert abs(o.status().omega_frame - n) < 1e-7 * n
+    p, o, st = _disc(corotating=_capi.FRAME_COROTATING, omega_frame=n, eccentricity=0.1,
+                     planets=_one(1.0, m))
+    o.algogas(DT)
+    assert abs(o.status().omega_frame - n) > 0.05 * n
+
+
+@pytest.mark.parametrize("frame,outer", [(_capi.FRAME_BINARY_CENTER, 0), (_capi.FRAME_COROTATING, 1)])
+def test_frame_centre_choice(oracle_built, frame, outer):
+    """Two planets on circular orbits at r = 1 and r = 1.6, one of 1e-3 and
+    one of 1e-12 (whose indirect term perturbs the other by 1e-12 only).
+    FRAME B with the heavy planet 0 follows the barycentre, planet 0 to 1e-9:
+    the rate is sqrt(1 + 1e-3).  FRAME C with COROTATEWITHOUTERPLANET and
+    the heavy planet 1 follows it at sqrt(1 + 1e-3) / 1.6^1.5.  Swapped
+    barycentre weights or the wrong planet index fail."""
+    mass = [1e-12, 1e-3] if outer else [1e-3, 1e-12]
+    pl = dict(dist=[1.0, 1.6], mass=mass, acc=[0.0, 0.0], feel_disk=[0, 0], feel_others=[0, 0])
+    w = math.sqrt(1 + 1e-3) if not outer else math.sqrt(1 + 1e-3) / 1.6**1.5
+    p, o, st = _disc(corotating=frame, corotate_with_outer_planet=outer, omega_frame=w,
+                     planets=pl)
+    for _ in range(2):
+        o.algogas(DT)
+    assert abs(o.status().omega_frame - w) < 1e-7 * w
+    k = 1 if outer else 0
+    r = 1.6 if outer else 1.0
+    pl = o.state()["planets"]
+    assert abs(pl.x[k] - r) < 1e-7 and abs(pl.y[k]) < 1e-7
+
+
+def _noisy_state(o, st, seed):
+    rng = np.random.default_rng(seed)
+    for k in ("rho", "vt"):
+        st[k] = st[k] * (1 + 0.1 * rng.standard_normal(st[k].shape))
+    st["vr"] = st["vr"] + 0.01 * rng.standard_normal(st["vr"].shape)
+    o.set_state(st)
+    return st
+
+
+def _zones(o, p):
+    rmed = o.tables()[1]
+    inner = np.nonzero(rmed < p.wkz_rmin)[0]
+    outer = np.nonzero(rmed > p.wkz_rmax)[0]
+    return rmed, inner, outer
+
+
+def test_wkz_damp_to_axi_keeps_ring_means(oracle_built):
+    """DampToAxi relaxes each zone ring towards its own azimuthal mean, so the
+    mean stays (to rounding), the deviations shrink by exactly 1/(1 + lambda)
+    and rings between WKZRMIN and WKZRMAX are untouched.  lambda at the
+    grid's edge ring is step / (ratio Rmed^1.5) (x = 1 there), and inside the
+    zone lambda / lambda_edge = x^2 (Rmed_edge / Rmed)^1.5 with x the
+    fractional depth into the zone.  A linear profile, a missing power 1.5
+    or damping towards zero instead of the mean fails."""
+    p, o, st = _disc(wave_killing_zones=1, wkz_rmin=0.6, wkz_rmax=2.1)
+    st = _noisy_state(o, st, 1)
+    nr, ns = p.nrad, p.nsec
+    step = 0.05
+    o.evanescent(step)
+    after = o.state()
+    rmed, inner, outer = _zones(o, p)
+    assert len(inner) >= 3 and len(outer) >= 3
+    lam = {}
+    for k in ("rho", "vr", "vt"):
+        a0 = st[k][: nr * ns].reshape(nr, ns)
+        a1 = after[k][: nr * ns].reshape(nr, ns)
+        for i in range(nr):
+            if i not in inner and i not in outer:
+                assert np.array_equal(a0[i], a1[i]), (k, i)
+                continue
+            m0 = a0[i].mean()
+            assert abs(a1[i].mean() - m0) <= 1e-14 * abs(a0[i]).max(), (k, i)
+            ratio = (a1[i] - m0) / (a0[i] - m0)
+            assert np.ptp(ratio) < 1e-9, (k, i)
+            lam.setdefault(i, []).append(1 / ratio.mean() - 1)
+    lam = {i: np.mean(v) for i, v in lam.items()}
+    for i, edge, wz in ((inner[0], 0, p.wkz_rmin), (outer[-1], nr - 1, p.wkz_rmax)):
+        assert abs(lam[edge] / (step / (0.3 * rmed[edge] ** 1.5)) - 1) < 1e-9
+    for zone, edge, wz in ((inner, 0, p.wkz_rmin), (outer, nr - 1, p.wkz_rmax)):
+        for i in zone:
+            x = (rmed[i] - wz) / (rmed[edge] - wz)
+            expect = x * x * (rmed[edge] / rmed[i]) ** 1.5
+            assert abs(lam[i] / lam[edge] - expect) < 1e-8, i
+
+
+def test_wkz_damp_to_ini_converges_to_initial_profile(oracle_built):
+    """DampToIni with a step far above the damping time drives the grid's
+    edge rings to the initial profiles whatever the state: Sigma to the
+    noiseless power law Sigma0 Rmed^-0.5 (SigmaMed), vr and v_theta to the
+    initial velocities (VradMed, VthetaMed - Rmed OmegaFrame).  Damping
+    towards the ring mean instead fails on Sigma (the state carries 10 %
+    noise)."""
+    p, o, st0 = _disc(wave_killing_zones=1, wkz_rmin=0.6, wkz_rmax=2.1, damp_to=_capi.DAMP_TO_INI)
+    nr, ns = p.nrad, p.nsec
+    rmed = o.tables()[1]
+    init = {k: st0[k].copy() for k in ("vr", "vt")}
+    _noisy_state(o, st0, 2)
+    o.evanescent(1e9)
+    after = o.state()
+    for i in (0, nr - 1):
+        sl = slice(i * ns, (i + 1) * ns)
+        ref = p.sigma0 * rmed[i] ** -0.5
+        assert np.abs(after["rho"][sl] - ref).max() <= 1e-8 * ref
+        for k in ("vr", "vt"):
+            ref = init[k][i * ns]
+            # vr: on the scale of the 0.01 noise it was given
+            scale = 0.01 if k == "vr" else abs(ref)
+            assert np.abs(after[k][sl] - ref).max() <= 1e-8 * scale, (k, i)
+
+
+def test_wkz_zero_vrad_edges(oracle_built):
+    """ZEROVRADBC: rings 0, 1 (inner zone) and nrad-1 (outer zone) end with
+    vr = 0 and the damping target (the ring mean) for Sigma and v_theta;
+    ring 2 keeps a non-zero vr."""
+    p, o, st = _disc(wave_killing_zones=1, wkz_rmin=0.6, wkz_rmax=2.1, zero_vrad_bc=1)
+    st = _noisy_state(o, st, 3)
+    nr, ns = p.nrad, p.nsec
+    o.evanescent(0.05)
+    after = o.state()
+    for i in (0, 1, nr - 1):
+        sl = slice(i * ns, (i + 1) * ns)
+        assert np.all(after["vr"][sl] == 0.0)
+        assert np.ptp(after["rho"][sl]) == 0.0
+        assert abs(after["rho"][sl][0] - st["rho"][sl].mean()) <= 1e-15 * st["rho"][sl].mean()
+        assert abs(after["vt"][sl][0] - st["vt"][sl].mean()) <= 1e-14 * abs(st["vt"][sl].mean())
+    assert np.any(after["vr"][2 * ns:3 * ns] != 0.0)
