@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libdfargo.so")
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC, ERR_STATE = range(6)
 FP64, FP32 = 0, 1
-BC_WALL, BC_OPEN = 0, 1
+BC_WALL, BC_OPEN, BC_NONREFLECTING = 0, 1, 2
 FRAME_FIXED, FRAME_COROTATING, FRAME_GUIDING_CENTER, FRAME_BINARY_CENTER = 0, 1, 2, 3
 DAMP_TO_AXI, DAMP_TO_INI, DAMP_TO_VISCOUS = 0, 1, 2
 

@@ -121,6 +121,10 @@ static int check_frame_planets(const dfargo_params &p, const dfargo_planets *s) 
   if (s->nb < need)
     return dfargo_set_error(DFARGO_ERR_ARG, "FRAME %s needs %d planet(s), got %d",
                             p.corotating == DFARGO_FRAME_BINARY_CENTER ? "B" : "C/G", need, s->nb);
+  // the non-reflecting boundary takes its pattern speed from the innermost
+  // and outermost planets (src/SideEuler.c:590-600,702-712)
+  if (p.boundary == DFARGO_BC_NONREFLECTING && s->nb < 1)
+    return dfargo_set_error(DFARGO_ERR_ARG, "the non-reflecting boundary needs a planet");
   return DFARGO_OK;
 }
 
@@ -229,6 +233,7 @@ static void build_sequence(dfargo_ctx *c) {
   c->seq.push_back(KID_SOURCE);
   if (c->p.energy_equation) c->seq.push_back(KID_ENERGY);
   for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_FINISH}) c->seq.push_back(id);
+  if (c->p.boundary == DFARGO_BC_NONREFLECTING) c->seq.push_back(KID_NONREFLECTING);
   if (c->p.wave_killing_zones && (c->a.wkz_n_in > 0 || c->a.wkz_i_out < c->p.nrad))
     c->seq.push_back(KID_EVANESCENT);
 }

@@ -69,8 +69,15 @@ int dfargo_check_params(const dfargo_params *p) {
                             "DAMPTOVISCOUS (1-D viscous evolution profiles) is not implemented");
   if (p->wave_killing_zones && p->damp_to != DFARGO_DAMP_TO_AXI && p->damp_to != DFARGO_DAMP_TO_INI)
     return dfargo_set_error(DFARGO_ERR_ARG, "unknown damping target %d", p->damp_to);
-  if (p->boundary != DFARGO_BC_WALL && p->boundary != DFARGO_BC_OPEN)
-    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "only WALL and OPEN boundaries are implemented");
+  if (p->boundary != DFARGO_BC_WALL && p->boundary != DFARGO_BC_OPEN && p->boundary != DFARGO_BC_NONREFLECTING)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
+                            "only WALL, OPEN and NONREFLECTING boundaries are implemented");
+  /* with the energy equation the reference recomputes the sound speed inside
+     the boundary call (src/SideEuler.c:1121-1122), which the next sub-step's
+     CFL condition then reads */
+  if (p->boundary == DFARGO_BC_NONREFLECTING && p->energy_equation)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
+                            "the non-reflecting boundary is implemented for isothermal discs only");
   if (p->self_gravity && (p->nrad & (p->nrad - 1)))
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
                             "self-gravity needs NRAD a power of two (2*NRAD FFT), got %d", p->nrad);

@@ -105,7 +105,7 @@ enum KernelId {
   KID_SG_ROWS_FWD = 0, KID_SG_COLS, KID_SG_ROWS_INV,
   KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
   KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER,
-  KID_EVANESCENT, KID_COUNT
+  KID_EVANESCENT, KID_NONREFLECTING, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 

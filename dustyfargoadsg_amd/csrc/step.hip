@@ -16,7 +16,7 @@ const char *kKernelNames[KID_COUNT] = {
     "sg_rows_fwd", "sg_cols", "sg_rows_inv",
     "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
     "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
-    "dust_sort",     "dust_deposit", "dust_reorder", "evanescent"};
+    "dust_sort",     "dust_deposit", "dust_reorder", "evanescent", "nonreflecting"};
 
 // ------------------------------------------------------------------------
 // helpers
@@ -1720,6 +1720,75 @@ template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArg
   }
 }
 
+// NonReflectingBoundary (src/SideEuler.c:569-794), barotropic case without
+// the accreting variant, after the sub-Keplerian reset (which it leaves
+// alone).  Workgroup 0 does the inner edge, workgroup 1 the outer one.
+// Each edge takes a pattern speed from the innermost (outermost) planet
+// inside the grid, turns it into an integer azimuthal shift i_angle over the
+// first (last) ring spacing at the mean sound speed, and
+//   inner: rho(0, j + i_angle) = rho(1, j), shifted by a constant so that
+//          ring 0's mean is SigmaMed[0]; vr(1, j) = 2 vr_med - vr(2, j) with
+//          vr_med = -cs1 (rho(1, j) - <rho(1)>) / <rho(1)>;
+//   outer: rho(nr-1, j) = rho(nr-2, j - i_angle) + SigmaMed[nr-1] - <...>;
+//          vr(nr-1, j) = 2 vr_med - vr(nr-2, j), vr_med = +cs (...).
+// The new edge ring is a cyclic shift of its neighbour, so its mean is the
+// neighbour's: the constant is formed from that mean.  The sound speed is
+// the isothermal ring value, the mean of its identical cells.
+enum { NR_TB = 256 };
+template <typename T> __global__ void __launch_bounds__(NR_TB) k_nonreflecting(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const bool inner = blockIdx.x == 0;
+  const int nr = a.nr, ns = a.ns, tid = threadIdx.x;
+  // the ring the edge copies from, and the edge ring
+  const int src = inner ? 1 : nr - 2, dst = inner ? 0 : nr - 1;
+  __shared__ double part[NR_TB / 64];
+  double s = 0.0;
+  for (int j = tid; j < ns; j += NR_TB) s += ld<T>(a.rho, (long)src * ns + j);
+  s = wave_sum(s);
+  if ((tid & 63) == 0) part[tid >> 6] = s;
+  __syncthreads();
+  double mean = 0.0;
+  for (int w = 0; w < NR_TB / 64; w++) mean += part[w];
+  mean /= (double)ns;
+  // the planet that sets the pattern speed (the loop of src/SideEuler.c:
+  // 590-600 and 702-712; the reference leaves r undefined when planet 0 is
+  // outside the grid, here the grid's edge stands in)
+  const double r0 = a.Rinf[0], r1 = a.Rsup[nr - 1];
+  double buf = inner ? r1 : r0, rp_sel = buf, omega = pow(buf, -1.5);
+  for (int k = 0; k < c->nb; k++) {
+    const double xp = c->pl_x[k], yp = c->pl_y[k], rp = sqrt(xp * xp + yp * yp);
+    if (inner ? ((rp < buf) && (rp >= r0)) : ((rp > buf) && (rp <= r1))) {
+      rp_sel = rp;
+      omega = pow(rp, -1.5);
+    }
+    buf = rp_sel;
+  }
+  const double cs_a = a.cs_iso[inner ? 0 : nr - 1], cs_b = a.cs_iso[inner ? 1 : nr - 2];
+  double dangle = (pow(a.Rinf[inner ? 1 : nr - 2], -1.5) - omega) / (.5 * (cs_a + cs_b));
+  dangle *= inner ? (a.Rmed[1] - a.Rmed[0]) : (a.Rmed[nr - 1] - a.Rmed[nr - 2]);
+  const int i_angle = (int)(dangle / a.pm * (double)ns + .5);
+  const double shift = a.SigmaMed[dst] - mean;
+  const double cs_v = inner ? -a.cs_iso[1] : a.cs_iso[nr - 1];
+  for (int j = tid; j < ns; j += NR_TB) {
+    const long ls = (long)src * ns + j;
+    const double rs = ld<T>(a.rho, ls);
+    const double vr_med = cs_v * (rs - mean) / mean;
+    if (inner) {
+      int jp = (j + i_angle) % ns;
+      jp = (jp < 0) ? jp + ns : jp;
+      st<T>(a.rho, jp, rs + shift);
+      st<T>(a.vr, ls, 2. * vr_med - ld<T>(a.vr, ls + ns));
+    } else {
+      int jp = (j - i_angle) % ns;
+      jp = (jp < 0) ? jp + ns : jp;
+      const long l = (long)dst * ns + j;
+      st<T>(a.rho, l, ld<T>(a.rho, (long)src * ns + jp) + shift);
+      st<T>(a.vr, l, 2. * vr_med - ld<T>(a.vr, ls));
+    }
+  }
+}
+
 // Potential grid on demand (the field of FillForcesArrays)
 template <typename T> __global__ void k_potential(KArgs a) {
   const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1776,6 +1845,9 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
   switch (id) {
     case KID_CFL: RING_DISPATCH_K(wide_k(a.ns), k_cfl_ring, T, a, s, 0); break;
     case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
+    case KID_NONREFLECTING:
+      hipLaunchKernelGGL(k_nonreflecting<T>, dim3(2), dim3(NR_TB), 0, s, a);
+      break;
     case KID_EVANESCENT:
       hipLaunchKernelGGL(k_evanescent<T>, dim3(a.wkz_n_in + (a.nr - a.wkz_i_out)), dim3(EV_TB), 0, s, a);
       break;

@@ -1609,12 +1609,107 @@ static void open_boundary(or_ctx *c) {
   }
 }
 
+/* NonReflectingBoundary (src/SideEuler.c:569-794), barotropic case, no
+ * accreting variant.  The pattern speed is that of the innermost
+ * (outermost) planet inside the grid; where the reference reads an
+ * undefined radius (planet 0 outside the grid) the grid's edge stands in.
+ * The energy grid of a barotropic disc holds zeros, which the outer edge's
+ * copy and mean adjustment leave at zero. */
+static void nonreflecting_boundary(or_ctx *c) {
+  const dfargo_params *p = &c->p;
+  const int nr = c->nr, ns = c->ns;
+  double *rho = c->rho, *vr = c->vr, *cs = c->cs;
+  /* inner edge */
+  {
+    const int i = 1;
+    double buf = c->Radii[nr], r_in = c->Radii[nr], omega_in = pow(c->Radii[nr], -1.5);
+    for (int k = 0; k < c->sys.nb; k++) {
+      const double xp = c->sys.x[k], yp = c->sys.y[k], rp = sqrt(xp * xp + yp * yp);
+      if ((rp < buf) && (rp >= c->Radii[0])) {
+        r_in = rp;
+        omega_in = pow(rp, -1.5);
+      }
+      buf = r_in;
+    }
+    double cs0 = 0.0, cs1 = 0.0, dens1 = 0.0;
+    for (int j = 0; j < ns; j++) {
+      cs0 += cs[j];
+      cs1 += cs[ns + j];
+      dens1 += rho[ns + j];
+    }
+    cs0 /= (double)ns;
+    cs1 /= (double)ns;
+    dens1 /= (double)ns;
+    double dangle = (pow(c->Rinf[1], -1.5) - omega_in) / (.5 * (cs0 + cs1));
+    dangle *= (c->Rmed[1] - c->Rmed[0]);
+    const int i_angle = (int)(dangle / (p->pmax - p->pmin) * (double)ns + .5);
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns;
+      int jp = j + i_angle;
+      if (jp >= ns) jp -= ns;
+      if (jp < 0) jp += ns;
+      rho[jp] = rho[l];
+      const double vr_med = -cs1 * (rho[l] - dens1) / dens1;
+      vr[l] = 2. * vr_med - vr[l + ns];
+    }
+    double mean = 0.0;
+    for (int j = 0; j < ns; j++) mean += rho[j];
+    mean /= (double)ns;
+    for (int j = 0; j < ns; j++) rho[j] += (c->SigmaMed[0] - mean);
+  }
+  /* outer edge */
+  {
+    double buf = c->Radii[0], r_out = c->Radii[0], omega_out = pow(c->Radii[0], -1.5);
+    for (int k = 0; k < c->sys.nb; k++) {
+      const double xp = c->sys.x[k], yp = c->sys.y[k], rp = sqrt(xp * xp + yp * yp);
+      if ((rp > buf) && (rp <= c->Radii[nr])) {
+        r_out = rp;
+        omega_out = pow(rp, -1.5);
+      }
+      buf = r_out;
+    }
+    double csnrm2 = 0.0, csnrm1 = 0.0, densnrm2 = 0.0;
+    for (int j = 0; j < ns; j++) {
+      csnrm2 += cs[(nr - 2) * ns + j];
+      csnrm1 += cs[(nr - 1) * ns + j];
+      densnrm2 += rho[(nr - 2) * ns + j];
+    }
+    csnrm1 /= (double)ns;
+    csnrm2 /= (double)ns;
+    densnrm2 /= (double)ns;
+    const int i = nr - 1;
+    double dangle = (pow(c->Rinf[nr - 2], -1.5) - omega_out) / (.5 * (csnrm1 + csnrm2));
+    dangle *= (c->Rmed[nr - 1] - c->Rmed[nr - 2]);
+    const int i_angle = (int)(dangle / (p->pmax - p->pmin) * (double)ns + .5);
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns;
+      int jp = j - i_angle;
+      if (jp >= ns) jp -= ns;
+      if (jp < 0) jp += ns;
+      const int lp = jp + (i - 1) * ns;
+      rho[l] = rho[lp];
+      const double vr_med = csnrm1 * (rho[l - ns] - densnrm2) / densnrm2;
+      vr[l] = 2. * vr_med - vr[l - ns];
+    }
+    double mean = 0.0;
+    for (int j = 0; j < ns; j++) mean += rho[j + ns * (nr - 1)];
+    mean /= (double)ns;
+    for (int j = 0; j < ns; j++) rho[j + (nr - 1) * ns] += c->SigmaMed[nr - 1] - mean;
+  }
+}
+
 /* ApplyBoundaryCondition (src/SideEuler.c:1102-1131) */
 static void apply_boundary_condition(or_ctx *c) {
   if (c->p.boundary == DFARGO_BC_WALL) wall_boundaries(c);
   subkeplerian_boundary(c); /* unconditional: src/SideEuler.c:1110-1112 */
   if (c->p.boundary == DFARGO_BC_OPEN) open_boundary(c);
+  if (c->p.boundary == DFARGO_BC_NONREFLECTING) {
+    nonreflecting_boundary(c);
+    subkeplerian_boundary(c);
+  }
 }
+
+void or_nonreflecting_boundary(or_ctx *c) { nonreflecting_boundary(c); }
 
 /* EvanescentBoundary (src/SideEuler.c:796-950) for the gas: in the
  * wave-killing zones Rmed < WKZRMIN and Rmed > WKZRMAX each ring relaxes as

@@ -49,6 +49,8 @@ void or_get_state(or_ctx *c, double *rho, double *vrad, double *vtheta,
 void or_set_time(or_ctx *c, double t, double t0, double omega_frame);
 /* EvanescentBoundary on the current state, for the pins of its formula */
 void or_evanescent_boundary(or_ctx *c, double step);
+/* NonReflectingBoundary on the current state (barotropic), for its pins */
+void or_nonreflecting_boundary(or_ctx *c);
 
 /* AlgoGas; returns the number of sub-steps taken */
 int or_algogas(or_ctx *c, double DT);

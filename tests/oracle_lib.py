@@ -41,6 +41,7 @@ def lib():
             "or_get_state": (None, [vp, dp, dp, dp, dp, PP]),
             "or_set_time": (None, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "or_evanescent_boundary": (None, [vp, ctypes.c_double]),
+            "or_nonreflecting_boundary": (None, [vp]),
             "or_algogas": (ctypes.c_int, [vp, ctypes.c_double]),
             "or_get_status": (None, [vp, ctypes.POINTER(Status)]),
             "or_get_field": (None, [vp, ctypes.c_int, dp]),
@@ -159,6 +160,9 @@ class Oracle:
 
     def evanescent(self, step):
         self.L.or_evanescent_boundary(self.ctx, step)
+
+    def nonreflecting(self):
+        self.L.or_nonreflecting_boundary(self.ctx)
 
     def set_time(self, t, t0=0.0, omega_frame=0.0):
         self.L.or_set_time(self.ctx, t, t0, omega_frame)

@@ -1,7 +1,8 @@
 """Corotating frames (FRAME C, G, B and COROTATEWITHOUTERPLANET;
-src/SourceEuler.c:476,516-524, src/Psys.c:233-350, src/SideEuler.c:1133-1148)
-and the wave-killing zones (EvanescentBoundary, src/SideEuler.c:796-950) of
-the HIP step against the FP64 oracle, through the C ABI.  Same bars as the
+src/SourceEuler.c:476,516-524, src/Psys.c:233-350, src/SideEuler.c:1133-1148),
+the wave-killing zones (EvanescentBoundary, src/SideEuler.c:796-950) and the
+non-reflecting boundary (src/SideEuler.c:569-794) of the HIP step against
+the FP64 oracle, through the C ABI.  Same bars as the
 other parity tests: FP64 fields within 1e-12 of the oracle after two AlgoGas
 calls, sub-step counts and FARGO shifts exact; FP32 storage within 2e-5
 (Sigma) and 2e-6 (velocities).
@@ -103,3 +104,30 @@ def test_frame_needs_its_planets():
     d = Disc(p, _capi.FP64)
     with pytest.raises(DfargoError):
         d.initial_state(pl)
+
+
+@pytest.mark.parametrize("nr,ns,calls,zones", [(32, 96, 2, 0), (16, 1536, 1, 0), (32, 96, 2, 1)])
+def test_nonreflecting_fp64(nr, ns, calls, zones):
+    """NonReflectingBoundary (src/SideEuler.c:569-794), barotropic, with a
+    planet that feels the disc (its pattern speed moves), alone and under
+    the wave-killing zones."""
+    over = dict(wave_killing_zones=1, wkz_rmin=0.55, wkz_rmax=2.2) if zones else {}
+    p, pl = gas_planet(nrad=nr, nsec=ns, forced_circular=0, boundary=_capi.BC_NONREFLECTING, **over)
+    pl = dict(pl, feel_disk=[1], feel_others=[1])
+    o, so, d, sg = pair(p, pl)
+    for _ in range(calls):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+        assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
+    compare(o, d, p, 1e-12, 1e-12, 1e-12)
+
+
+def test_nonreflecting_fp32():
+    p, pl = gas_planet(nrad=128, nsec=384, boundary=_capi.BC_NONREFLECTING)
+    o, so, d, sg = pair(p, pl, _capi.FP32)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+    compare(o, d, p, 2e-5, 2e-6, 2e-6)

@@ -207,3 +207,76 @@ def test_wkz_zero_vrad_edges(oracle_built):
         assert abs(after["rho"][sl][0] - st["rho"][sl].mean()) <= 1e-15 * st["rho"][sl].mean()
         assert abs(after["vt"][sl][0] - st["vt"][sl].mean()) <= 1e-14 * abs(st["vt"][sl].mean())
     assert np.any(after["vr"][2 * ns:3 * ns] != 0.0)
+
+
+def _best_shift(a, b):
+    """The cyclic shift s for which a[(j + s) % ns] - b[j] is most nearly
+    constant, and the spread of that difference."""
+    ns = len(a)
+    spreads = [np.ptp(np.roll(a, -s) - b) for s in range(ns)]
+    s = int(np.argmin(spreads))
+    return (s if s <= ns // 2 else s - ns), spreads[s]
+
+
+def _nonreflecting_disc(planets, seed):
+    p, o, st = _disc(boundary=_capi.BC_NONREFLECTING, planets=planets)
+    o.algogas(1e-9)  # fills the sound-speed grid the boundary reads
+    st = o.state()
+    st = _noisy_state(o, st, seed)
+    return p, o, st
+
+
+def test_nonreflecting_edges(oracle_built):
+    """The non-reflecting edges are shifted copies of their neighbour rings
+    carrying outgoing acoustic waves: ring 0 is ring 1 turned by a whole
+    number of cells and offset so that its mean is SigmaMed[0] (the noiseless
+    Sigma0 Rmed^-0.5), likewise ring nrad-1 from ring nrad-2; and the radial
+    velocity at the face between each edge pair follows v = -/+ cs
+    dSigma/Sigma for a wave leaving through the inner/outer edge.  The turn
+    is the spiral pitch between the rings at the planet's pattern speed,
+    (Omega(Rinf) - Omega_p) / cs dR in radians.  A wave sign, a dropped mean
+    adjustment or the wrong ring spacing fails."""
+    p, o, st = _nonreflecting_disc(_one(1.0, 1e-3), 4)
+    nr, ns = p.nrad, p.nsec
+    radii, rmed = o.tables()[0], o.tables()[1]
+    cs = p.aspect_ratio * rmed ** -0.5
+    o.nonreflecting()
+    af = o.state()
+    ring = lambda d, k, i: d[k][i * ns:(i + 1) * ns]
+    for dst, src, pitch_r, dr in ((0, 1, radii[1], rmed[1] - rmed[0]),
+                                  (nr - 1, nr - 2, radii[nr - 2], rmed[nr - 1] - rmed[nr - 2])):
+        assert abs(ring(af, "rho", dst).mean() / (p.sigma0 * rmed[dst] ** -0.5) - 1) < 1e-13
+        if dst == 0:
+            s, spread = _best_shift(ring(af, "rho", 0), ring(st, "rho", 1))
+        else:
+            s, spread = _best_shift(ring(st, "rho", nr - 2), ring(af, "rho", nr - 1))
+            s = -s
+        assert spread < 1e-15 * ring(st, "rho", src).max()
+        dangle = (pitch_r ** -1.5 - 1.0) / (0.5 * (cs[dst] + cs[src])) * dr
+        assert s == int(dangle / (2 * math.pi) * ns + 0.5), (dst, s)
+        rs = ring(st, "rho", src)
+        wave = (-1 if dst == 0 else 1) * cs[1 if dst == 0 else nr - 1] * (rs - rs.mean()) / rs.mean()
+        face = 1 if dst == 0 else nr - 1
+        other = 2 if dst == 0 else nr - 2
+        mid = 0.5 * (ring(af, "vr", face) + ring(st, "vr", other))
+        assert np.abs(mid - wave).max() <= 1e-12 * np.abs(wave).max()
+    assert s < 0  # the outer edge trails the planet inside it
+
+
+def test_nonreflecting_pattern_speed_planets(oracle_built):
+    """With planets at Rinf[1] and Rinf[nrad-2] the inner edge takes the
+    innermost one's frequency, equal to Omega at its pitch radius, and the
+    outer edge the outermost one's: both turns are zero.  Swapping the
+    innermost/outermost choice gives turns of many cells."""
+    p0, o0, _ = _disc(nr=32, ns=96)
+    radii = o0.tables()[0]
+    nr = p0.nrad
+    pl = dict(dist=[float(radii[1]), float(radii[nr - 2])], mass=[1e-6, 1e-6], acc=[0.0, 0.0],
+              feel_disk=[0, 0], feel_others=[0, 0])
+    p, o, st = _nonreflecting_disc(pl, 5)
+    ns = p.nsec
+    o.nonreflecting()
+    af = o.state()
+    s_in, _ = _best_shift(af["rho"][:ns], st["rho"][ns:2 * ns])
+    s_out, _ = _best_shift(st["rho"][(nr - 2) * ns:(nr - 1) * ns], af["rho"][(nr - 1) * ns:nr * ns])
+    assert s_in == 0 and s_out == 0
