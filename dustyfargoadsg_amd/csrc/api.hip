@@ -125,6 +125,38 @@ static int check_frame_planets(const dfargo_params &p, const dfargo_planets *s) 
   // and outermost planets (src/SideEuler.c:590-600,702-712)
   if (p.boundary == DFARGO_BC_NONREFLECTING && s->nb < 1)
     return dfargo_set_error(DFARGO_ERR_ARG, "the non-reflecting boundary needs a planet");
+  // accretion (src/Planet.c:11-126) runs in the control kernel after the
+  // force sums were formed; it corrects those sums, which holds for the
+  // plain columns with a mass-independent smoothing, and it changes the
+  // density that the self-gravity transforms (running beside the control
+  // kernel) and the particles read
+  int nacc = 0;
+  for (int k = 0; k < s->nb; k++) nacc += s->acc[k] > 1e-10;
+  if (nacc > 1) return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "at most one accreting planet is supported");
+  if (nacc && (p.self_gravity || p.exclude_hill || p.roche_smoothing != 0.0))
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
+                            "planet accretion with self-gravity, EXCLUDEHILL or ROCHESMOOTHING is not implemented");
+  return DFARGO_OK;
+}
+
+// the planets' accretion switch is part of the captured sub-step: a change
+// drops the graph so that the next call captures it again
+static int set_accretion(dfargo_ctx *c, const dfargo_planets *s) {
+  int acc = 0;
+  for (int k = 0; k < s->nb; k++) acc |= s->acc[k] > 1e-10;
+  if (acc && c->dust_on)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "planet accretion with particles is not implemented");
+  if (acc != c->a.accrete) {
+    c->a.accrete = acc;
+    if (c->gexec) {
+      HIPCHK(hipGraphExecDestroy(c->gexec));
+      c->gexec = nullptr;
+    }
+    if (c->graph) {
+      HIPCHK(hipGraphDestroy(c->graph));
+      c->graph = nullptr;
+    }
+  }
   return DFARGO_OK;
 }
 
@@ -284,6 +316,7 @@ static int build_context(dfargo_ctx *c) {
     a.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   }
   a.pm = p.pmax - p.pmin;
+  a.pmin = p.pmin;
   a.dphi = a.pm / (double)ns;
   a.invdphi = 1.0 / a.dphi;
   a.gamma = p.adiabatic_index;
@@ -535,7 +568,7 @@ extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad,
   dfargo_planets sys;
   if (sys_in) sys = *sys_in;
   else memset(&sys, 0, sizeof sys);
-  if ((rc = check_frame_planets(p, &sys))) return rc;
+  if ((rc = check_frame_planets(p, &sys)) || (rc = set_accretion(c, &sys))) return rc;
   std::vector<double> zero(n, 0.0);
   if ((rc = put_field(c, a.rho, rho, n)) || (rc = put_field(c, a.vr, zero.data(), n)) ||
       (rc = put_field(c, a.vt, zero.data(), n)))
@@ -711,7 +744,7 @@ extern "C" int dfargo_upload_state(dfargo_ctx *c, const double *rho, const doubl
   if (vtheta && (rc = put_field(c, c->a.vt, vtheta, n))) return rc;
   if (energy && c->p.energy_equation && (rc = put_field(c, c->a.en, energy, n))) return rc;
   if (sys) {
-    if ((rc = check_frame_planets(c->p, sys))) return rc;
+    if ((rc = check_frame_planets(c->p, sys)) || (rc = set_accretion(c, sys))) return rc;
     if ((rc = ctrl_pull(c))) return rc;
     planets_to_ctrl(c->h_ctrl, sys);
     if ((rc = ctrl_push(c))) return rc;
@@ -1048,6 +1081,8 @@ extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, co
   if (c->dust_on) return dfargo_set_error(DFARGO_ERR_STATE, "particles are already attached to this context");
   int rc = dfargo_check_dust_params(&c->p, d);
   if (rc) return rc;
+  if (c->a.accrete)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "planet accretion with particles is not implemented");
   const int n = d->nbpart;
   if (n == 0) return DFARGO_OK; // NBPART = 0: no particles, nothing changes
   if (!st->r || !st->th || !st->vr || !st->vth || !st->l || !st->dustsize)

@@ -301,8 +301,6 @@ int dfargo_init_planetary_system(const dfargo_params *p, int nb, const double *d
   for (int i = 0; i < nb; i++) {
     float dist = (float)dist_in[i], mass = (float)mass_in[i], accret = (float)acc_in[i];
     double massinvelocity;
-    if (accret > 1e-10)
-      return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "planet accretion (Kley) is not implemented");
     if (p->forced_circular && (feel_disk[i] || feel_others[i])) /* src/Psys.c:146-155 */
       return dfargo_set_error(DFARGO_ERR_ARG, "FeelDisk/FeelOthers cannot be Yes with ForcedCircular");
     sys->mass[i] = (double)mass;

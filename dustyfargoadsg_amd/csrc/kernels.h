@@ -31,6 +31,9 @@ struct KArgs {
   int wkz_n_in, wkz_i_out, damp_ini, zero_vrad;
   double wkz_rmin, wkz_rmax, damp_ratio;
   const double *VradMed;
+  // planet accretion on (some planet has acc > 1e-10), and PMIN
+  int accrete;
+  double pmin;
   int gas_indirect, indirect_term, alt_indirect, exclude_hill, roche, forced_circular;
   int xcd_remap;
   int num_cu; // compute units of the device

@@ -628,7 +628,116 @@ __device__ void d_rk5(const double *q0, double dt, const double *m, double *q1, 
 // barycentre acceleration (src/SideEuler.c:91-112), the potential
 // parameters of FillForcesArrays (src/Pframeforce.c:44-90), and the planet
 // update (AdvanceSystemFromDisk, AdvanceSystemRK5, RotatePsys).
-__global__ void __launch_bounds__(1024) k_control(KArgs a) {
+// AccreteOntoPlanets (src/Planet.c:11-126) by the whole control workgroup,
+// once dt is known.  Each cell of a planet's window belongs to one thread,
+// which updates its density in place.  The planet's mass and momentum gains
+// are block sums over the accreted cells, and so are the changes of the
+// disc's mass, barycentre and force sums: the CFL kernel formed those
+// partials before the accretion, the reference forms them after it, and
+// they are linear in the density.  At most one planet accretes (checked on
+// the host), so no cell is accreted by two threads.  red holds 16 doubles
+// per reduced value.
+template <typename T>
+__device__ void accrete_planets(const KArgs &a, dfargo_ctrl *c, double dt, double *sums, double *red) {
+  const int t = threadIdx.x, TB = blockDim.x, lane = t & 63, wv = t >> 6, nw = TB >> 6;
+  const int nr = a.nr, ns = a.ns, ntg = c->ntarget;
+  constexpr int NV = 6 + 2 * DF_MAXTG;
+  for (int k = 0; k < c->nb; k++) {
+    if (!(c->pl_acc[k] > 1e-10)) continue;
+    const double facc = dt * c->pl_acc[k], facc1 = 1.0 / 3.0 * facc, facc2 = 2.0 / 3.0 * facc;
+    const double Xp = c->pl_x[k], Yp = c->pl_y[k], Mp = c->pl_m[k];
+    const double Rp = sqrt(Xp * Xp + Yp * Yp), RRoche = pow((1.0 / 3.0 * Mp), (1.0 / 3.0)) * Rp;
+    int i_min = 0, i_max = nr - 1;
+    while ((i_min < nr) && (a.Rsup[i_min] < Rp - RRoche)) i_min++;
+    while ((i_max > 0) && (a.Rinf[i_max] > Rp + RRoche)) i_max--;
+    const double angle = atan2(Yp, Xp);
+    const int j_min = (int)((double)ns / a.pm * (angle - 2.0 * RRoche / Rp - a.pmin));
+    const int j_max = (int)((double)ns / a.pm * (angle + 2.0 * RRoche / Rp - a.pmin));
+    const int wj = j_max - j_min + 1;
+    const int ncell = (i_max >= i_min && wj > 0) ? (i_max - i_min + 1) * wj : 0;
+    // 0-2: the planet's mass and momentum gains; 3-5: the disc's mass and
+    // barycentre changes; then fx, fy per force target
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = 0.0;
+    for (int idx = t; idx < ncell; idx += TB) {
+      const int i = i_min + idx / wj, j = j_min + idx % wj;
+      int jf = j % ns;
+      jf = (jf < 0) ? jf + ns : jf;
+      const long l = (long)i * ns + jf, lip = l + ns, ljp = (jf == ns - 1) ? (long)i * ns : l + 1;
+      const double xc = a.Rmed[i] * a.CosAz[jf], yc = a.Rmed[i] * a.SinAz[jf];
+      const double dx = Xp - xc, dy = Yp - yc, distance = sqrt(dx * dx + dy * dy);
+      // the inner radius (0.45) lies inside the outer one (0.75)
+      if (!(distance < 0.75 * RRoche)) continue;
+      const double vtcell = 0.5 * (ld<T>(a.vt, l) + ld<T>(a.vt, ljp)) + a.Rmed[i] * c->omega_frame;
+      const double vrcell = 0.5 * (ld<T>(a.vr, l) + ld<T>(a.vr, lip));
+      const double vxcell = (vrcell * xc - vtcell * yc) / a.Rmed[i];
+      const double vycell = (vrcell * yc + vtcell * xc) / a.Rmed[i];
+      const double d0 = ld<T>(a.rho, l);
+      double dens = d0;
+      double deltaM = facc1 * dens * a.Surf[i];
+      dens *= (1.0 - facc1);
+      v[0] += deltaM;
+      v[1] += deltaM * vxcell;
+      v[2] += deltaM * vycell;
+      if (distance < 0.45 * RRoche) {
+        deltaM = facc2 * dens * a.Surf[i];
+        dens *= (1.0 - facc2);
+        v[0] += deltaM;
+        v[1] += deltaM * vxcell;
+        v[2] += deltaM * vycell;
+      }
+      st<T>(a.rho, l, dens);
+      // the change of the cell's mass as stored, in the partials' terms
+      const double dcm = a.Surf[i] * ((double)(T)dens - d0);
+      v[3] += dcm;
+      v[4] += dcm * xc;
+      v[5] += dcm * yc;
+#pragma unroll
+      for (int tg = 0; tg < DF_MAXTG; tg++) {
+        if (tg >= ntg) break;
+        const double ddx = xc - c->tg_x[tg], ddy = yc - c->tg_y[tg], rsm = c->tg_rsm[tg];
+        const double dist2 = ddx * ddx + ddy * ddy + rsm * rsm, dd = sqrt(dist2);
+        const double InvDist3 = 1.0 / dist2 / dd;
+        v[6 + 2 * tg] += DF_G * dcm * ddx * InvDist3 * 1.0;
+        v[7 + 2 * tg] += DF_G * dcm * ddy * InvDist3 * 1.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      const double w = wave_sum(v[q]);
+      if (lane == 0) red[q * 16 + wv] = w;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double s[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        s[q] = 0.0;
+        for (int w = 0; w < nw; w++) s[q] += red[q * 16 + w];
+      }
+      double Px = Mp * c->pl_vx[k], Py = Mp * c->pl_vy[k], M = Mp;
+      Px += s[1];
+      Py += s[2];
+      M += s[0];
+      if (c->pl_feel_disk[k]) {
+        c->pl_vx[k] = Px / M;
+        c->pl_vy[k] = Py / M;
+      }
+      c->pl_m[k] = M;
+      sums[2] += s[3];
+      sums[3] += s[4];
+      sums[4] += s[5];
+      for (int tg = 0; tg < ntg; tg++) {
+        sums[5 + 4 * tg] += s[6 + 2 * tg];
+        sums[5 + 4 * tg + 2] += s[7 + 2 * tg];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T> __global__ void __launch_bounds__(1024) k_control(KArgs a) {
   __shared__ double sh[1024];
   __shared__ double sums[DF_NPART(DF_MAXTG)];
   dfargo_ctrl *c = a.ctrl;
@@ -676,33 +785,43 @@ __global__ void __launch_bounds__(1024) k_control(KArgs a) {
     if (lane == 0) sums[p] = v;
   }
   __syncthreads();
-  if (t != 0) return;
   // sub-step dt
-  double dt;
-  if (!a.d_fixed_dt) {
-    double deltaT = c->DT - c->dtemp;
-    // the minimum stopping time of the last interpolation, resolved by two
-    // steps (src/SourceEuler.c:1501-1507)
-    if (a.d_on && a.d_np > 0 && a.d_feedback) {
-      double dt_fb = 0.5 * dust_min_ts;
-      if (dt_fb == 0.0) dt_fb = 1e10;
-      if (dt_fb < nd) nd = dt_fb;
+  __shared__ double s_dt;
+  if (t == 0) {
+    double dt;
+    if (!a.d_fixed_dt) {
+      double deltaT = c->DT - c->dtemp;
+      // the minimum stopping time of the last interpolation, resolved by two
+      // steps (src/SourceEuler.c:1501-1507)
+      if (a.d_on && a.d_np > 0 && a.d_feedback) {
+        double dt_fb = 0.5 * dust_min_ts;
+        if (dt_fb == 0.0) dt_fb = 1e10;
+        if (dt_fb < nd) nd = dt_fb;
+      }
+      if (deltaT < nd) nd = deltaT;
+      int gts = (int)(ceil(deltaT / nd));
+      double dthydro = deltaT / (double)gts;
+      dt = dmin2(dthydro, c->dtnbody);
+      c->gas_timesteps_cfl = gts;
+    } else {
+      // the particles' fixed time step (src/SourceEuler.c:457-463)
+      dt = a.d_dust_timestep;
+      c->mindt = a.d_dust_timestep;
+      c->maxdt = a.d_dust_timestep;
     }
-    if (deltaT < nd) nd = deltaT;
-    int gts = (int)(ceil(deltaT / nd));
-    double dthydro = deltaT / (double)gts;
-    dt = dmin2(dthydro, c->dtnbody);
-    c->gas_timesteps_cfl = gts;
-  } else {
-    // the particles' fixed time step (src/SourceEuler.c:457-463)
-    dt = a.d_dust_timestep;
-    c->mindt = a.d_dust_timestep;
-    c->maxdt = a.d_dust_timestep;
+    c->dtemp += dt;
+    if (dt < c->mindt) c->mindt = dt;
+    if (dt > c->maxdt) c->maxdt = dt;
+    c->dt = dt;
+    s_dt = dt;
   }
-  c->dtemp += dt;
-  if (dt < c->mindt) c->mindt = dt;
-  if (dt > c->maxdt) c->maxdt = dt;
-  c->dt = dt;
+  if (a.accrete) {
+    // AccreteOntoPlanets right after dt (src/SourceEuler.c:454)
+    __syncthreads();
+    accrete_planets<T>(a, c, s_dt, sums, sh);
+  }
+  if (t != 0) return;
+  const double dt = s_dt;
   c->crashed_d = sums[0] > 0.0;
   c->crashed_e = sums[1] > 0.0;
   // gas indirect term: ComputeAccel(star), src/SideEuler.c:282-298
@@ -1844,7 +1963,7 @@ static dim3 cell_block(const KArgs &a) { return dim3(a.ns < 256 ? a.ns : 256); }
 template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) {
   switch (id) {
     case KID_CFL: RING_DISPATCH_K(wide_k(a.ns), k_cfl_ring, T, a, s, 0); break;
-    case KID_CONTROL: hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, a); break;
+    case KID_CONTROL: hipLaunchKernelGGL(k_control<T>, dim3(1), dim3(1024), 0, s, a); break;
     case KID_NONREFLECTING:
       hipLaunchKernelGGL(k_nonreflecting<T>, dim3(2), dim3(NR_TB), 0, s, a);
       break;

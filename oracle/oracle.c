@@ -840,6 +840,74 @@ static double psys_info(or_ctx *c, int get) {
   return asin(cross / (d1 * d2));
 }
 
+/* AccreteOntoPlanets (src/Planet.c:11-126), one process (no ghost rings):
+ * gas within 0.75 and 0.45 of the Roche radius loses the fractions dt acc/3
+ * and then 2 dt acc/3 of its density; the planet gains that mass, and the
+ * momentum it carried at the cell-centred velocity when it feels the disc. */
+static void accrete_onto_planets(or_ctx *c, double dt) {
+  const dfargo_params *p = &c->p;
+  const int nr = c->nr, ns = c->ns;
+  double *dens = c->rho, *vrad = c->vr, *vtheta = c->vt;
+  dfargo_planets *sys = &c->sys;
+  for (int k = 0; k < sys->nb; k++) {
+    if (!(sys->acc[k] > 1e-10)) continue;
+    double dMplanet = 0.0, dPxPlanet = 0.0, dPyPlanet = 0.0;
+    const double facc = dt * sys->acc[k], facc1 = 1.0 / 3.0 * facc, facc2 = 2.0 / 3.0 * facc;
+    const double frac1 = 0.75, frac2 = 0.45;
+    const double Xplanet = sys->x[k], Yplanet = sys->y[k];
+    const double VXplanet = sys->vx[k], VYplanet = sys->vy[k];
+    double Mplanet = sys->mass[k];
+    const double Rplanet = sqrt(Xplanet * Xplanet + Yplanet * Yplanet);
+    const double RRoche = pow((1.0 / 3.0 * Mplanet), (1.0 / 3.0)) * Rplanet;
+    int i_min = 0, i_max = nr - 1;
+    while ((c->Rsup[i_min] < Rplanet - RRoche) && (i_min < nr)) i_min++;
+    while ((c->Rinf[i_max] > Rplanet + RRoche) && (i_max > 0)) i_max--;
+    const double angle = atan2(Yplanet, Xplanet);
+    const int j_min = (int)((double)ns / (p->pmax - p->pmin) * (angle - 2.0 * RRoche / Rplanet - p->pmin));
+    const int j_max = (int)((double)ns / (p->pmax - p->pmin) * (angle + 2.0 * RRoche / Rplanet - p->pmin));
+    double PxPlanet = Mplanet * VXplanet, PyPlanet = Mplanet * VYplanet;
+    for (int i = i_min; i <= i_max; i++)
+      for (int j = j_min; j <= j_max; j++) {
+        int jf = j;
+        while (jf < 0) jf += ns;
+        while (jf >= ns) jf -= ns;
+        const int l = jf + i * ns, lip = l + ns;
+        const int ljp = (jf == ns - 1) ? i * ns : l + 1;
+        const double xc = c->cabs[l], yc = c->cord[l];
+        const double dx = Xplanet - xc, dy = Yplanet - yc;
+        const double distance = sqrt(dx * dx + dy * dy);
+        const double vtcell = 0.5 * (vtheta[l] + vtheta[ljp]) + c->Rmed[i] * c->OmegaFrame;
+        const double vrcell = 0.5 * (vrad[l] + vrad[lip]);
+        const double vxcell = (vrcell * xc - vtcell * yc) / c->Rmed[i];
+        const double vycell = (vrcell * yc + vtcell * xc) / c->Rmed[i];
+        if (distance < frac1 * RRoche) {
+          const double deltaM = facc1 * dens[l] * c->Surf[i];
+          dens[l] *= (1.0 - facc1);
+          dPxPlanet += deltaM * vxcell;
+          dPyPlanet += deltaM * vycell;
+          dMplanet += deltaM;
+        }
+        if (distance < frac2 * RRoche) {
+          const double deltaM = facc2 * dens[l] * c->Surf[i];
+          dens[l] *= (1.0 - facc2);
+          dPxPlanet += deltaM * vxcell;
+          dPyPlanet += deltaM * vycell;
+          dMplanet += deltaM;
+        }
+      }
+    PxPlanet += dPxPlanet;
+    PyPlanet += dPyPlanet;
+    Mplanet += dMplanet;
+    if (sys->feel_disk[k]) {
+      sys->vx[k] = PxPlanet / Mplanet;
+      sys->vy[k] = PyPlanet / Mplanet;
+    }
+    sys->mass[k] = Mplanet;
+  }
+}
+
+void or_accrete_onto_planets(or_ctx *c, double dt) { accrete_onto_planets(c, dt); }
+
 /* CorrectVtheta (src/SideEuler.c:1133-1148) */
 static void correct_vtheta(or_ctx *c, double domega) {
   for (int i = 0; i < c->nr; i++)
@@ -2283,6 +2351,7 @@ int or_algogas(or_ctx *c, double DT) {
       c->GasTimeStepsCFL = or_condition_cfl(c, DT - dtemp);
       dthydro = (DT - dtemp) / (double)c->GasTimeStepsCFL;
       dt = min2(dthydro, dtnbody);
+      accrete_onto_planets(c, dt); /* src/SourceEuler.c:454 */
     } else {
       /* fixed particle time step (src/SourceEuler.c:457-463) */
       dt = c->dp.dust_timestep;

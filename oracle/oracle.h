@@ -51,6 +51,8 @@ void or_set_time(or_ctx *c, double t, double t0, double omega_frame);
 void or_evanescent_boundary(or_ctx *c, double step);
 /* NonReflectingBoundary on the current state (barotropic), for its pins */
 void or_nonreflecting_boundary(or_ctx *c);
+/* AccreteOntoPlanets on the current state, for its pins */
+void or_accrete_onto_planets(or_ctx *c, double dt);
 
 /* AlgoGas; returns the number of sub-steps taken */
 int or_algogas(or_ctx *c, double DT);

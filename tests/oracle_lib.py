@@ -42,6 +42,7 @@ def lib():
             "or_set_time": (None, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "or_evanescent_boundary": (None, [vp, ctypes.c_double]),
             "or_nonreflecting_boundary": (None, [vp]),
+            "or_accrete_onto_planets": (None, [vp, ctypes.c_double]),
             "or_algogas": (ctypes.c_int, [vp, ctypes.c_double]),
             "or_get_status": (None, [vp, ctypes.POINTER(Status)]),
             "or_get_field": (None, [vp, ctypes.c_int, dp]),
@@ -163,6 +164,9 @@ class Oracle:
 
     def nonreflecting(self):
         self.L.or_nonreflecting_boundary(self.ctx)
+
+    def accrete(self, dt):
+        self.L.or_accrete_onto_planets(self.ctx, dt)
 
     def set_time(self, t, t0=0.0, omega_frame=0.0):
         self.L.or_set_time(self.ctx, t, t0, omega_frame)

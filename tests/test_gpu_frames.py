@@ -131,3 +131,42 @@ def test_nonreflecting_fp32():
         d.algogas(DT)
         assert d.status().last_call_substeps == n_o
     compare(o, d, p, 2e-5, 2e-6, 2e-6)
+
+
+@pytest.mark.parametrize("nr,ns,frame", [(32, 96, 0), (128, 384, 0), (64, 192, 1)])
+def test_accretion_fp64(nr, ns, frame):
+    """AccreteOntoPlanets (src/Planet.c:11-126) in the control kernel: the
+    density near the planet, the planet's mass and velocity, and through the
+    corrected force sums everything after, against the oracle."""
+    p, pl = gas_planet(nrad=nr, nsec=ns, forced_circular=0,
+                       corotating=_capi.FRAME_COROTATING if frame else _capi.FRAME_FIXED,
+                       omega_frame=math.sqrt(1 + 1e-3) if frame else 0.0)
+    pl = dict(pl, acc=[2.0], feel_disk=[1], feel_others=[1])
+    o, so, d, sg = pair(p, pl)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+        assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
+    compare(o, d, p, 1e-12, 1e-12, 1e-12)
+    go, oo = d.state()["planets"], o.state()["planets"]
+    assert oo.mass[0] > 1e-3
+    assert abs(go.mass[0] - oo.mass[0]) <= 1e-12 * oo.mass[0]
+    for k in ("x", "y", "vx", "vy"):
+        assert abs(getattr(go, k)[0] - getattr(oo, k)[0]) <= 1e-13, k
+
+
+def test_accretion_fp32_and_limits():
+    p, pl = gas_planet(nrad=128, nsec=384, forced_circular=0)
+    pl = dict(pl, acc=[2.0], feel_disk=[1], feel_others=[1])
+    o, so, d, sg = pair(p, pl, _capi.FP32)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+    compare(o, d, p, 2e-5, 2e-6, 2e-6)
+    assert abs(d.state()["planets"].mass[0] / o.state()["planets"].mass[0] - 1) <= 1e-9
+    from dustyfargoadsg_amd.disc import Disc, DfargoError
+    p, pl = gas_planet(nrad=32, nsec=96, exclude_hill=1)
+    with pytest.raises(DfargoError):
+        Disc(p, _capi.FP64).initial_state(dict(pl, acc=[1.0]))

@@ -280,3 +280,48 @@ def test_nonreflecting_pattern_speed_planets(oracle_built):
     s_in, _ = _best_shift(af["rho"][:ns], st["rho"][ns:2 * ns])
     s_out, _ = _best_shift(st["rho"][(nr - 2) * ns:(nr - 1) * ns], af["rho"][(nr - 1) * ns:nr * ns])
     assert s_in == 0 and s_out == 0
+
+
+def test_accretion_conserves_mass_and_momentum(oracle_built):
+    """AccreteOntoPlanets (src/Planet.c:11-126): the planet gains exactly the
+    mass the gas loses, and (feeling the disc) the momentum that gas carried
+    at its cell-centred velocity; cells beyond 0.75 Roche radii keep their
+    density, those within 0.75 lose the fraction dt acc / 3, and those within
+    0.45 then a further 2 dt acc / 3 of what is left.  A planet mass left
+    unchanged, the two fractions swapped or the radii exchanged fail."""
+    acc, dt, m = 2.0, 0.01, 1e-3
+    pl = dict(dist=[1.0], mass=[m], acc=[acc], feel_disk=[1], feel_others=[0])
+    p, o, st = _disc(nr=64, ns=192, planets=pl)
+    st = _noisy_state(o, st, 6)
+    before = o.state()
+    o.accrete(dt)
+    after = o.state()
+    nr, ns = p.nrad, p.nsec
+    radii, rmed, surf, az = o.tables()
+    pb, pa = before["planets"], after["planets"]
+    gas_b = (before["rho"][: nr * ns].reshape(nr, ns) * surf[:nr, None]).sum()
+    gas_a = (after["rho"][: nr * ns].reshape(nr, ns) * surf[:nr, None]).sum()
+    assert pa.mass[0] > pb.mass[0]
+    assert abs((pa.mass[0] - pb.mass[0]) - (gas_b - gas_a)) <= 1e-14 * (pa.mass[0] - pb.mass[0]) + 1e-18
+    # per-cell fractions by distance to the planet
+    rroche = (pb.mass[0] / 3) ** (1 / 3) * math.hypot(pb.x[0], pb.y[0])
+    xc = rmed[:nr, None] * np.cos(az[None, :ns])
+    yc = rmed[:nr, None] * np.sin(az[None, :ns])
+    d = np.hypot(pb.x[0] - xc, pb.y[0] - yc)
+    ratio = after["rho"][: nr * ns].reshape(nr, ns) / before["rho"][: nr * ns].reshape(nr, ns)
+    f1, f2 = dt * acc / 3, 2 * dt * acc / 3
+    expect = np.where(d < 0.45 * rroche, (1 - f1) * (1 - f2), np.where(d < 0.75 * rroche, 1 - f1, 1.0))
+    assert np.any(d < 0.45 * rroche)
+    assert np.abs(ratio - expect).max() < 1e-14
+    # momentum: what the removed gas carried, at the cell-centred velocity
+    dm = (before["rho"][: nr * ns] - after["rho"][: nr * ns]).reshape(nr, ns) * surf[:nr, None]
+    vt = before["vt"][: nr * ns].reshape(nr, ns)
+    vr = before["vr"][: (nr + 1) * ns].reshape(nr + 1, ns)
+    vtc = 0.5 * (vt + np.roll(vt, -1, axis=1)) + rmed[:nr, None] * o.status().omega_frame
+    vrc = 0.5 * (vr[:nr] + vr[1:nr + 1])
+    vx = (vrc * xc - vtc * yc) / rmed[:nr, None]
+    vy = (vrc * yc + vtc * xc) / rmed[:nr, None]
+    px = pb.mass[0] * pb.vx[0] + (dm * vx).sum()
+    py = pb.mass[0] * pb.vy[0] + (dm * vy).sum()
+    assert abs(pa.mass[0] * pa.vx[0] - px) <= 1e-12 * abs(px) + 1e-16
+    assert abs(pa.mass[0] * pa.vy[0] - py) <= 1e-12 * abs(py)
