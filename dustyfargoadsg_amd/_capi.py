@@ -62,6 +62,7 @@ class Params(ctypes.Structure):
         ("corotate_with_outer_planet", c_int),
         ("wave_killing_zones", c_int), ("wkz_rmin", c_double), ("wkz_rmax", c_double),
         ("damping_ratio", c_double), ("damp_to", c_int), ("zero_vrad_bc", c_int),
+        ("sg_zero_mode", c_int),
     ]
 
     def copy(self):
@@ -190,6 +191,7 @@ def default_params(**overrides):
     p.corotate_with_outer_planet = 0
     p.wave_killing_zones, p.wkz_rmin, p.wkz_rmax = 0, 0.0, 0.0
     p.damping_ratio, p.damp_to, p.zero_vrad_bc = 0.3, DAMP_TO_AXI, 0
+    p.sg_zero_mode = 0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(f"unknown parameter {k!r}")

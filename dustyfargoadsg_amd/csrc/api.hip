@@ -190,7 +190,8 @@ static void launch_id(dfargo_ctx *c, int id, hipStream_t s) {
 static void launch_id(dfargo_ctx *c, int id) { launch_id(c, id, c->stream); }
 
 static bool is_sg_kernel(int id) {
-  return id == KID_SG_ROWS_FWD || id == KID_SG_COLS || id == KID_SG_ROWS_INV;
+  return id == KID_SG_ROWS_FWD || id == KID_SG_COLS || id == KID_SG_ROWS_INV || id == KID_SGZ_MEAN ||
+         id == KID_SGZ_CONV || id == KID_SGZ_FILL;
 }
 
 static int run_substep(dfargo_ctx *c) {
@@ -255,7 +256,9 @@ static int run_substep(dfargo_ctx *c) {
 // feedback.
 static void build_sequence(dfargo_ctx *c) {
   c->seq.clear();
-  if (c->p.self_gravity)
+  if (c->p.self_gravity && c->p.sg_zero_mode)
+    for (int id : {KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL}) c->seq.push_back(id);
+  else if (c->p.self_gravity)
     for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV}) c->seq.push_back(id);
   for (int id : {KID_CFL, KID_CONTROL}) c->seq.push_back(id);
   if (c->dust_on)
@@ -394,8 +397,32 @@ static int build_context(dfargo_ctx *c) {
       (rc = dalloc(c, (void **)&c->d_ctrl, sizeof(dfargo_ctrl))))
     return rc;
   a.ctrl = c->d_ctrl;
+  a.sg_zero = p.self_gravity && p.sg_zero_mode;
+  if (a.sg_zero) {
+    // SELFGRAVITY Z (src/sgzero.c:3-84): the reduced kernel summed over
+    // the azimuths, times tstep, on the 2 Nrad log-radius cells, and the
+    // ring's own term of the self-force compensation, FP64 on the host
+    const int L2 = 2 * nr;
+    const double eps = t.sg_eps;
+    if ((rc = dalloc(c, &a.sgr, c->esz * c->nfield)) || (rc = dalloc(c, &a.sgt, c->esz * c->nfield)) ||
+        (rc = dalloc(c, (void **)&a.sgz_mean, sizeof(double) * (size_t)(nr + 2))) ||
+        (rc = dalloc(c, (void **)&a.sgz_acc, sizeof(double) * (size_t)(nr + 2))))
+      return rc;
+    std::vector<double> kr(L2, 0.0);
+    for (int i = 0; i < L2; i++) {
+      const double u = (i < nr) ? log(t.Radii[i] / t.Radii[0]) : -log(t.Radii[2 * nr - i] / t.Radii[0]);
+      for (int j = 0; j < ns; j++)
+        kr[i] += (1.0 + eps * eps - t.CosAz[j] * exp(-u)) * pow(eps * eps * exp(u) + 2.0 * (cosh(u) - t.CosAz[j]), -1.5);
+      kr[i] *= t.sg_tstep;
+    }
+    if ((rc = upload_table(c, &a.sgz_kr, kr.data(), L2))) return rc;
+    double integrale = 0.0;
+    for (int j = 0; j < ns; j++)
+      integrale += (1.0 + eps * eps - t.CosAz[j]) / pow(2.0 * (1.0 - t.CosAz[j]) + eps * eps, 1.5);
+    a.sgz_integrale = integrale * t.sg_tstep;
+  }
   // self-gravity: twiddles and kernel spectra
-  if (p.self_gravity) {
+  if (p.self_gravity && !a.sg_zero) {
     const int L2 = 2 * nr;
     const size_t csz = 2 * c->esz;
     if ((rc = dalloc(c, &a.sgr, c->esz * c->nfield)) || (rc = dalloc(c, &a.sgt, c->esz * c->nfield)) ||
@@ -581,12 +608,19 @@ extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad,
     if (c->prec == DFARGO_FP64) launch_sg_accel<double>(a, c->stream);
     else launch_sg_accel<float>(a, c->stream);
     HIPCHK(hipGetLastError());
-    std::vector<double> accr(n);
-    if ((rc = get_field_raw(c, accr.data(), a.sgr, (size_t)nr * ns))) return rc;
-    for (int i = 0; i < nr; i++) {
-      double s = 0.;
-      for (int j = 0; j < ns; j++) s += accr[(size_t)i * ns + j];
-      axisg[i] = s / (double)ns;
+    if (a.sg_zero) {
+      // the axisymmetric mode's SG_Accr is the profile itself
+      // (src/sgsysinit.c:12-15, src/Pframeforce.c:483-486)
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipMemcpy(axisg.data(), a.sgz_acc, sizeof(double) * nr, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<double> accr(n);
+      if ((rc = get_field_raw(c, accr.data(), a.sgr, (size_t)nr * ns))) return rc;
+      for (int i = 0; i < nr; i++) {
+        double s = 0.;
+        for (int j = 0; j < ns; j++) s += accr[(size_t)i * ns + j];
+        axisg[i] = s / (double)ns;
+      }
     }
     // init_planetarysys_withSG (src/sgsysinit.c:3-49), non-binary
     for (int k = 0; k < sys.nb; k++) {

@@ -37,6 +37,7 @@ void dfargo_default_params(dfargo_params *p) {
   p->corotate_with_outer_planet = 0;
   p->wave_killing_zones = 0; p->wkz_rmin = 0.0; p->wkz_rmax = 0.0;
   p->damping_ratio = 0.3; p->damp_to = DFARGO_DAMP_TO_AXI; p->zero_vrad_bc = 0;
+  p->sg_zero_mode = 0;
 }
 
 int dfargo_check_params(const dfargo_params *p) {
@@ -78,13 +79,16 @@ int dfargo_check_params(const dfargo_params *p) {
   if (p->boundary == DFARGO_BC_NONREFLECTING && p->energy_equation)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
                             "the non-reflecting boundary is implemented for isothermal discs only");
-  if (p->self_gravity && (p->nrad & (p->nrad - 1)))
+  /* the two-dimensional transforms; the axisymmetric mode is a direct
+     radial sum with no size condition */
+  const int sg2d = p->self_gravity && !p->sg_zero_mode;
+  if (sg2d && (p->nrad & (p->nrad - 1)))
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
                             "self-gravity needs NRAD a power of two (2*NRAD FFT), got %d", p->nrad);
-  if (p->self_gravity && !dfargo_fft_length_ok(p->nsec))
+  if (sg2d && !dfargo_fft_length_ok(p->nsec))
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
                             "self-gravity needs NSEC = 2^a 3^b <= 4096, got %d", p->nsec);
-  if (p->self_gravity && 2 * p->nrad > 4096)
+  if (sg2d && 2 * p->nrad > 4096)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "self-gravity needs 2*NRAD <= 4096");
   if (fabs(p->pmax - p->pmin - 2. * M_PI) >= 0.01 && p->self_gravity)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "self-gravity needs a full 2 pi azimuth");
@@ -384,6 +388,8 @@ int dfargo_check_dust_params(const dfargo_params *p, const dfargo_dust_params *d
   if (d->zz_integrator) return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "ZZINTEGRATOR is not implemented");
   if (d->dust_feel_sg_zero_mode)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "DUSTFEELSG Z (axisymmetric SG only) is not implemented");
+  if (p->self_gravity && p->sg_zero_mode)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "particles with SELFGRAVITY Z are not implemented");
   if (p->omega_frame != 0.0 || p->corotating)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
                             "particles in a rotating frame (OMEGAFRAME != 0, FRAME C/G/B) are not implemented");

@@ -34,6 +34,12 @@ struct KArgs {
   // planet accretion on (some planet has acc > 1e-10), and PMIN
   int accrete;
   double pmin;
+  // SELFGRAVITY Z: ring means of Sigma, the radial acceleration per ring,
+  // the reduced kernel (2 nr values, times tstep) and the ring's own term
+  int sg_zero;
+  double sgz_integrale;
+  double *sgz_mean, *sgz_acc;
+  const double *sgz_kr;
   int gas_indirect, indirect_term, alt_indirect, exclude_hill, roche, forced_circular;
   int xcd_remap;
   int num_cu; // compute units of the device
@@ -108,7 +114,8 @@ enum KernelId {
   KID_SG_ROWS_FWD = 0, KID_SG_COLS, KID_SG_ROWS_INV,
   KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
   KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER,
-  KID_EVANESCENT, KID_NONREFLECTING, KID_COUNT
+  KID_EVANESCENT, KID_NONREFLECTING,
+  KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 

@@ -362,6 +362,89 @@ static int sg_threads(const KArgs &a, size_t lds, int nwg, int cap) {
   return t;
 }
 
+// ------------------------------------------------------------------------
+// SELFGRAVITY Z: compute_SGZeroMode (src/sgzero.c:3-84), the axisymmetric
+// part only.  The reference convolves, cyclically over 2 Nrad log-radius
+// cells, the reduced ring-mean density Sr[k] = <Sigma>_k sqrt(Rmed_k/Rmed_0)
+// (zero beyond Nrad) with the reduced kernel summed over the azimuths,
+// through real FFTs of length 2 Nrad.  Only Nrad outputs and Nrad non-zero
+// inputs count, so the direct sum -- Nrad^2 products, one thread per output
+// ring with Sr staged in LDS and the kernel read from L2 -- is the cheaper
+// form on the GPU; it equals the transform product in exact arithmetic.
+// The result goes into every cell of its ring of the SG_Accr grid, so the
+// velocity update reads it as it reads the full field; SG_Acct stays zero.
+// ------------------------------------------------------------------------
+enum { SGZ_TB = 256, SGZ_CHUNK = 2048 };
+
+__device__ __forceinline__ double sgz_wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ring means of Sigma (mpi_make1Dprofile), one workgroup per ring
+template <typename T> __global__ void __launch_bounds__(SGZ_TB) k_sgz_mean(KArgs a, int force) {
+  if (!force && !a.ctrl->go) return;
+  const int i = blockIdx.x, ns = a.ns, tid = threadIdx.x;
+  __shared__ double part[SGZ_TB / 64];
+  double s = 0.0;
+  for (int j = tid; j < ns; j += SGZ_TB) s += ldd<T>(a.rho, (long)i * ns + j);
+  s = sgz_wave_sum(s);
+  if ((tid & 63) == 0) part[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) {
+    double m = 0.0;
+    for (int w = 0; w < SGZ_TB / 64; w++) m += part[w];
+    a.sgz_mean[i] = m / (double)ns;
+  }
+}
+
+// SG_Accr[i] = -G rstep (K * Sr)[i] / sqrt(Rmed_i/Rmed_0)
+//              + G <Sigma>_i rstep integrale
+template <typename T> __global__ void __launch_bounds__(SGZ_TB) k_sgz_conv(KArgs a, int force) {
+  if (!force && !a.ctrl->go) return;
+  __shared__ double sS[SGZ_CHUNK];
+  const int nr = a.nr, n2 = 2 * nr, tid = threadIdx.x, i = blockIdx.x * SGZ_TB + tid;
+  double acc = 0.0;
+  for (int k0 = 0; k0 < nr; k0 += SGZ_CHUNK) {
+    const int kn = (nr - k0 < SGZ_CHUNK) ? nr - k0 : SGZ_CHUNK;
+    __syncthreads();
+    for (int k = tid; k < kn; k += SGZ_TB) sS[k] = a.sgz_mean[k0 + k] * a.sg_sqrt[k0 + k];
+    __syncthreads();
+    if (i < nr)
+      for (int k = 0; k < kn; k++) {
+        int d = i - (k0 + k);
+        d += (d < 0) ? n2 : 0;
+        acc += a.sgz_kr[d] * sS[k];
+      }
+  }
+  if (i < nr) {
+    double v = -DF_G * acc * a.sg_rstep / a.sg_sqrt[i];
+    v += DF_G * a.sgz_mean[i] * a.sg_rstep * a.sgz_integrale;
+    a.sgz_acc[i] = v;
+  }
+}
+
+// the ring values into the SG_Accr grid
+template <typename T> __global__ void __launch_bounds__(256) k_sgz_fill(KArgs a, int force) {
+  if (!force && !a.ctrl->go) return;
+  const int i = blockIdx.y, ns = a.ns, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  ((T *)a.sgr)[(long)i * ns + j] = (T)a.sgz_acc[i];
+}
+
+template <typename T> static void sgz_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
+  const int bx = a.ns < 256 ? a.ns : 256;
+  switch (id) {
+    case KID_SGZ_MEAN: hipLaunchKernelGGL(k_sgz_mean<T>, dim3(a.nr), dim3(SGZ_TB), 0, s, a, force); break;
+    case KID_SGZ_CONV:
+      hipLaunchKernelGGL(k_sgz_conv<T>, dim3((a.nr + SGZ_TB - 1) / SGZ_TB), dim3(SGZ_TB), 0, s, a, force);
+      break;
+    case KID_SGZ_FILL:
+      hipLaunchKernelGGL(k_sgz_fill<T>, dim3((a.ns + bx - 1) / bx, a.nr), dim3(bx), 0, s, a, force);
+      break;
+  }
+}
+
 // rows_fwd: Z^T (sg_z) -> pair transforms -> W^T (sg_w) -> rows_inv
 template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
   const size_t rows_lds = rows_lds_bytes(a, sizeof(cx<T>)), cols_lds = cols_lds_bytes(a, sizeof(cx<T>));
@@ -389,9 +472,14 @@ template <typename T> void sg_prepare(const KArgs &a) {
 }
 
 template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s) {
-  sg_pipeline<T>(id, a, s, 0);
+  if (a.sg_zero) sgz_pipeline<T>(id, a, s, 0);
+  else sg_pipeline<T>(id, a, s, 0);
 }
 template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s) {
+  if (a.sg_zero) {
+    for (int id : {KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL}) sgz_pipeline<T>(id, a, s, 1);
+    return;
+  }
   for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV})
     sg_pipeline<T>(id, a, s, 1);
 }

@@ -16,7 +16,8 @@ const char *kKernelNames[KID_COUNT] = {
     "sg_rows_fwd", "sg_cols", "sg_rows_inv",
     "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
     "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
-    "dust_sort",     "dust_deposit", "dust_reorder", "evanescent", "nonreflecting"};
+    "dust_sort",     "dust_deposit", "dust_reorder", "evanescent", "nonreflecting",
+    "sgz_mean",      "sgz_conv",     "sgz_fill"};
 
 // ------------------------------------------------------------------------
 // helpers
@@ -2012,7 +2013,7 @@ void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s) {
 }
 
 template <typename T> void prepare_kernels(const KArgs &a) {
-  if (a.sg) sg_prepare<T>(a);
+  if (a.sg && !a.sg_zero) sg_prepare<T>(a);
 }
 template void prepare_kernels<double>(const KArgs &);
 template void prepare_kernels<float>(const KArgs &);

@@ -83,7 +83,7 @@ typedef struct dfargo_params {
   double adiabatic_index;            /* ADIABATICINDEX                      */
   double beta_cooling_time, beta_cooling_slope;
   /* self-gravity */
-  int    self_gravity;               /* SELFGRAVITY YES (Z mode not here)   */
+  int    self_gravity;               /* SELFGRAVITY YES or Z (sg_zero_mode) */
   double sg_thickness_smoothing;     /* SGTHICKNESSSMOOTHING                */
   /* planets and forces */
   double thickness_smoothing;        /* THICKNESSSMOOTHING                  */
@@ -116,6 +116,9 @@ typedef struct dfargo_params {
   double damping_ratio;              /* DAMPINGTODYNAMICALTIMERATIO (0.3)   */
   int    damp_to;                    /* enum dfargo_damp                    */
   int    zero_vrad_bc;               /* ZEROVRADBC                          */
+  /* self-gravity (continued) */
+  int    sg_zero_mode;               /* SELFGRAVITY Z: axisymmetric part
+                                        only (src/sgzero.c)                 */
 } dfargo_params;
 
 /* PlanetarySystem (src/types.h:97-111) plus the two mass tables that

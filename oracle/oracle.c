@@ -1102,11 +1102,70 @@ void or_sg_accel_direct(or_ctx *c, const double *rho, double *accr, double *acct
   free(rawt);
 }
 
+/* compute_SGZeroMode (src/sgzero.c:3-84): the axisymmetric part of the
+ * disc's self-gravity, SG_Accr per ring.  The reduced ring-mean density
+ * (zero-padded to 2 Nrad log-radius cells) is convolved cyclically with the
+ * reduced kernel summed over the azimuths, through transforms of length
+ * 2 Nrad (unnormalised, so the product carries a factor 2 Nrad that the
+ * normalisation rstep / (2 Nrad) removes), then the whole ring's own
+ * contribution is compensated. */
+void or_sg_zero_mode(or_ctx *c, const double *rho, double *accr) {
+  const int nr = c->nr, ns = c->ns, n2 = 2 * nr;
+  const double eps = c->p.sg_thickness_smoothing * c->p.aspect_ratio;
+  const double rstep = log(c->Radii[nr] / c->Radii[0]) / (double)nr;
+  const double tstep = (c->p.pmax - c->p.pmin) / (double)ns;
+  double *axi = (double *)calloc(nr, sizeof(double));
+  double *sr = (double *)calloc(n2, sizeof(double)), *si = (double *)calloc(n2, sizeof(double));
+  double *kr = (double *)calloc(n2, sizeof(double)), *ki = (double *)calloc(n2, sizeof(double));
+  double *ar = (double *)calloc(n2, sizeof(double)), *ai = (double *)calloc(n2, sizeof(double));
+  make_1d_profile(c, rho, axi);
+  for (int i = 0; i < nr; i++) sr[i] = axi[i] * sqrt(c->GlobalRmed[i] / c->GlobalRmed[0]);
+  for (int i = 0; i < n2; i++) {
+    const double u = (i < nr) ? log(c->Radii[i] / c->Radii[0]) : -log(c->Radii[2 * nr - i] / c->Radii[0]);
+    for (int j = 0; j < ns; j++)
+      kr[i] += (1.0 + eps * eps - c->CosAz[j] * exp(-u)) *
+               pow(eps * eps * exp(u) + 2.0 * (cosh(u) - c->CosAz[j]), -1.5);
+    kr[i] *= tstep;
+  }
+  or_fft(sr, si, n2, -1);
+  or_fft(kr, ki, n2, -1);
+  for (int m = 0; m < n2; m++) {
+    ar[m] = -G * (kr[m] * sr[m] - ki[m] * si[m]);
+    ai[m] = -G * (kr[m] * si[m] + ki[m] * sr[m]);
+  }
+  or_fft(ar, ai, n2, +1);
+  for (int i = 0; i < nr; i++) {
+    double normaccr = rstep / (2.0 * (double)nr);
+    normaccr /= sqrt(c->GlobalRmed[i] / c->GlobalRmed[0]);
+    accr[i] = ar[i] * normaccr;
+  }
+  double integrale = 0.0;
+  for (int j = 0; j < ns; j++) {
+    const double num = 1.0 + eps * eps - c->CosAz[j];
+    const double den = pow(2.0 * (1.0 - c->CosAz[j]) + eps * eps, 1.5);
+    integrale += num / den;
+  }
+  integrale *= tstep;
+  for (int i = 0; i < nr; i++) accr[i] += G * axi[i] * rstep * integrale;
+  free(axi); free(sr); free(si); free(kr); free(ki); free(ar); free(ai);
+}
+
 /* compute_selfgravity (src/sgmain.c:3-35) + update_sgvelocity
- * (src/sgupdate.c:3-73) */
+ * (src/sgupdate.c:3-73).  In the axisymmetric mode SG_Accr is the ring
+ * profile (kept in AxiSG, which the start-up uses as it stands) and there
+ * is no azimuthal acceleration: the grids hold the ring values and zeros,
+ * which the same update reads. */
 static void compute_selfgravity(or_ctx *c, double *vrad, double *vtheta, double Dt, int update) {
   int nr = c->nr, ns = c->ns;
-  or_sg_accel(c, c->rho, c->sg_accr, c->sg_acct);
+  if (c->p.sg_zero_mode) {
+    or_sg_zero_mode(c, c->rho, c->AxiSG);
+    for (int i = 0; i < nr; i++)
+      for (int j = 0; j < ns; j++) {
+        c->sg_accr[i * ns + j] = c->AxiSG[i];
+        c->sg_acct[i * ns + j] = 0.0;
+      }
+  } else
+    or_sg_accel(c, c->rho, c->sg_accr, c->sg_acct);
   memcpy(c->gr, c->sg_accr, sizeof(double) * nr * ns);
   memcpy(c->gtheta, c->sg_acct, sizeof(double) * nr * ns);
   if (!update) return;
@@ -1142,7 +1201,7 @@ static void init_gas_velocities(or_ctx *c) {
     for (int j = 0; j < ns; j++) vt[j + i * ns] = myvtheta;
   }
   if (p->self_gravity) {
-    make_1d_profile(c, c->sg_accr, c->AxiSG);
+    if (!p->sg_zero_mode) make_1d_profile(c, c->sg_accr, c->AxiSG);
     for (int i = 0; i < nr; i++) {
       double r = c->Rmed[i];
       for (int j = 0; j < ns; j++) {
@@ -1177,7 +1236,7 @@ static void init_gas_velocities(or_ctx *c) {
 
 /* init_planetarysys_withSG (src/sgsysinit.c:3-49), non-binary */
 static void init_planetarysys_with_sg(or_ctx *c) {
-  make_1d_profile(c, c->sg_accr, c->AxiSG);
+  if (!c->p.sg_zero_mode) make_1d_profile(c, c->sg_accr, c->AxiSG);
   for (int k = 0; k < c->sys.nb; k++) {
     double x = c->sys.x[k], y = c->sys.y[k], r = sqrt(x * x + y * y), sgacc;
     double dist = (double)(r / (1. + c->p.eccentricity));

@@ -53,6 +53,8 @@ void or_evanescent_boundary(or_ctx *c, double step);
 void or_nonreflecting_boundary(or_ctx *c);
 /* AccreteOntoPlanets on the current state, for its pins */
 void or_accrete_onto_planets(or_ctx *c, double dt);
+/* compute_SGZeroMode: SG_Accr per ring (nrad values) of the density rho */
+void or_sg_zero_mode(or_ctx *c, const double *rho, double *accr);
 
 /* AlgoGas; returns the number of sub-steps taken */
 int or_algogas(or_ctx *c, double DT);

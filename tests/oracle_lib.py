@@ -43,6 +43,7 @@ def lib():
             "or_evanescent_boundary": (None, [vp, ctypes.c_double]),
             "or_nonreflecting_boundary": (None, [vp]),
             "or_accrete_onto_planets": (None, [vp, ctypes.c_double]),
+            "or_sg_zero_mode": (None, [vp, dp, dp]),
             "or_algogas": (ctypes.c_int, [vp, ctypes.c_double]),
             "or_get_status": (None, [vp, ctypes.POINTER(Status)]),
             "or_get_field": (None, [vp, ctypes.c_int, dp]),
@@ -167,6 +168,11 @@ class Oracle:
 
     def accrete(self, dt):
         self.L.or_accrete_onto_planets(self.ctx, dt)
+
+    def sg_zero_mode(self, rho):
+        out = np.zeros(self.nr)
+        self.L.or_sg_zero_mode(self.ctx, ptr(rho), ptr(out))
+        return out
 
     def set_time(self, t, t0=0.0, omega_frame=0.0):
         self.L.or_set_time(self.ctx, t, t0, omega_frame)

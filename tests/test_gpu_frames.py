@@ -14,7 +14,7 @@ import pytest
 
 from dustyfargoadsg_amd import _capi
 from dustyfargoadsg_amd.workloads import gas_planet, energy_sg
-from tests.test_gpu_parity import DT, compare, pair
+from tests.test_gpu_parity import DT, compare, pair, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -170,3 +170,42 @@ def test_accretion_fp32_and_limits():
     p, pl = gas_planet(nrad=32, nsec=96, exclude_hill=1)
     with pytest.raises(DfargoError):
         Disc(p, _capi.FP64).initial_state(dict(pl, acc=[1.0]))
+
+
+@pytest.mark.parametrize("nr,ns,calls", [(32, 96, 2), (40, 120, 1), (16, 1536, 1)])
+def test_sg_zero_mode_fp64(nr, ns, calls):
+    """SELFGRAVITY Z (src/sgzero.c): the axisymmetric self-gravity, also on a
+    grid the two-dimensional transforms do not take (Nrad = 40)."""
+    p, pl = energy_sg(nrad=nr, nsec=ns, sg_zero_mode=1)
+    o, so, d, sg = pair(p, pl)
+    assert rel(sg["vt"], so["vt"], nr, ns) <= 1e-14
+    for _ in range(calls):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+        assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
+    compare(o, d, p, 1e-12, 1e-12, 1e-12, 1e-12)
+    assert rel(d.field("sg_accr"), o.field("sg_accr"), nr, ns) <= 1e-11
+
+
+def test_sg_zero_mode_planet_and_fp32():
+    """A planet on the start-up orbit corrected by the axisymmetric field
+    (init_planetarysys_withSG with GLOBAL_AxiSGAccr = SG_Accr), FP64; then
+    FP32 storage."""
+    p, pl = gas_planet(nrad=32, nsec=96, self_gravity=1, sg_zero_mode=1, sg_thickness_smoothing=0.3)
+    o, so, d, sg = pair(p, pl)
+    vy_g, vy_o = d.state()["planets"].vy[0], o.state()["planets"].vy[0]
+    assert vy_o != so["planets"].vy[0]  # the start-up correction acted
+    assert abs(vy_g - vy_o) <= 1e-15
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+    compare(o, d, p, 1e-12, 1e-12, 1e-12)
+    p, pl = energy_sg(nrad=64, nsec=192, sg_zero_mode=1)
+    o, so, d, sg = pair(p, pl, _capi.FP32)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+    compare(o, d, p, 2e-5, 2e-6, 2e-6, 2e-5)

@@ -325,3 +325,33 @@ def test_accretion_conserves_mass_and_momentum(oracle_built):
     py = pb.mass[0] * pb.vy[0] + (dm * vy).sum()
     assert abs(pa.mass[0] * pa.vx[0] - px) <= 1e-12 * abs(px) + 1e-16
     assert abs(pa.mass[0] * pa.vy[0] - py) <= 1e-12 * abs(py)
+
+
+def test_sg_zero_mode_is_the_axisymmetric_part(oracle_built):
+    """SELFGRAVITY Z (src/sgzero.c): for an axisymmetric disc the full
+    two-dimensional self-gravity (pinned against direct summation elsewhere)
+    is axisymmetric too, and the zero mode must equal its ring values up to
+    the two self-force compensations: the full field removes the cell's own
+    term G Sigma rstep tstep / eps, the zero mode the whole ring's own
+    u = 0 row, G Sigma rstep tstep sum_j K(0, phi_j).  A transform
+    normalisation off by 2 Nrad, a missing sqrt(Rmed) reduction or the
+    wrong compensation fails."""
+    from dustyfargoadsg_amd.workloads import energy_sg
+    p, pl = energy_sg(nrad=32, nsec=96, add_noise=0)
+    o = Oracle(p)
+    st = o.initial_state(pl)
+    nr, ns = p.nrad, p.nsec
+    rho = st["rho"]
+    ar2, _ = o.sg_accel(rho)
+    ring2 = ar2[: nr * ns].reshape(nr, ns)
+    assert np.ptp(ring2, axis=1).max() <= 1e-12 * np.abs(ring2).max()
+    zero = o.sg_zero_mode(rho)
+    radii, rmed, surf, az = o.tables()
+    eps = p.sg_thickness_smoothing * p.aspect_ratio
+    rstep = math.log(radii[nr] / radii[0]) / nr
+    tstep = 2 * math.pi / ns
+    c = np.cos(az[:ns])
+    k0 = (1 + eps * eps - c) / (2 * (1 - c) + eps * eps) ** 1.5
+    sig = rho[: nr * ns].reshape(nr, ns).mean(axis=1)
+    expect = ring2.mean(axis=1) + sig * rstep * tstep * (k0.sum() - 1 / eps)
+    assert np.abs(zero - expect).max() <= 1e-11 * np.abs(zero).max()
