@@ -74,6 +74,12 @@ class Params(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class RunParams(ctypes.Structure):
+    """dfargo_run_params: DT, NTOT, NINTERM, PLANETCONFIG, OUTPUTDIR."""
+    _fields_ = [("dt", c_double), ("ntot", c_int), ("ninterm", c_int),
+                ("planet_config", ctypes.c_char * 512), ("output_dir", ctypes.c_char * 512)]
+
+
 class Planets(ctypes.Structure):
     """dfargo_planets (PlanetarySystem, src/types.h:97-111)."""
     _fields_ = [
@@ -252,6 +258,8 @@ def load(path=None):
         "dfargo_gas_budgets": (c_int, [vp, dp]),
         "dfargo_update_log": (c_int, [vp, c_int, dp, dp, ctypes.POINTER(c_int)]),
         "dfargo_orbital_elements": (c_int, [c_double] * 5 + [dp]),
+        "dfargo_read_par": (c_int, [ctypes.c_char_p, P, ctypes.POINTER(RunParams)]),
+        "dfargo_read_planet_config": (c_int, [ctypes.c_char_p, P, PP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -273,4 +281,5 @@ EXPORTED = [
     "dfargo_dust_default_params", "dfargo_check_dust_params", "dfargo_init_dust_system",
     "dfargo_attach_dust", "dfargo_download_dust", "dfargo_upload_dust",
     "dfargo_gas_budgets", "dfargo_update_log", "dfargo_orbital_elements",
+    "dfargo_read_par", "dfargo_read_planet_config",
 ]

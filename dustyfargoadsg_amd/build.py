@@ -20,7 +20,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
            "-I", CSRC]
 HIP_SOURCES = ["step.hip", "sg.hip", "dust.hip", "api.hip"]
-C_SOURCES = ["host.c"]
+C_SOURCES = ["host.c", "parfile.c"]
 
 
 def _run(cmd):

@@ -629,7 +629,7 @@ __device__ void d_rk5(const double *q0, double dt, const double *m, double *q1, 
 // barycentre acceleration (src/SideEuler.c:91-112), the potential
 // parameters of FillForcesArrays (src/Pframeforce.c:44-90), and the planet
 // update (AdvanceSystemFromDisk, AdvanceSystemRK5, RotatePsys).
-// AccreteOntoPlanets (src/Planet.c:11-126) by the whole control workgroup,
+// AccreteOntoPlanets (src/Planet.c:11-126) by the whole control workgroup
 // once dt is known.  Each cell of a planet's window belongs to one thread,
 // which updates its density in place.  The planet's mass and momentum gains
 // are block sums over the accreted cells, and so are the changes of the
@@ -657,88 +657,108 @@ __device__ void accrete_planets(const KArgs &a, dfargo_ctrl *c, double dt, doubl
     const int wj = j_max - j_min + 1;
     const int ncell = (i_max >= i_min && wj > 0) ? (i_max - i_min + 1) * wj : 0;
     // 0-2: the planet's mass and momentum gains; 3-5: the disc's mass and
-    // barycentre changes; then fx, fy per force target
-    double v[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = 0.0;
-    for (int idx = t; idx < ncell; idx += TB) {
-      const int i = i_min + idx / wj, j = j_min + idx % wj;
-      int jf = j % ns;
-      jf = (jf < 0) ? jf + ns : jf;
-      const long l = (long)i * ns + jf, lip = l + ns, ljp = (jf == ns - 1) ? (long)i * ns : l + 1;
-      const double xc = a.Rmed[i] * a.CosAz[jf], yc = a.Rmed[i] * a.SinAz[jf];
-      const double dx = Xp - xc, dy = Yp - yc, distance = sqrt(dx * dx + dy * dy);
-      // the inner radius (0.45) lies inside the outer one (0.75)
-      if (!(distance < 0.75 * RRoche)) continue;
-      const double vtcell = 0.5 * (ld<T>(a.vt, l) + ld<T>(a.vt, ljp)) + a.Rmed[i] * c->omega_frame;
-      const double vrcell = 0.5 * (ld<T>(a.vr, l) + ld<T>(a.vr, lip));
-      const double vxcell = (vrcell * xc - vtcell * yc) / a.Rmed[i];
-      const double vycell = (vrcell * yc + vtcell * xc) / a.Rmed[i];
-      const double d0 = ld<T>(a.rho, l);
-      double dens = d0;
-      double deltaM = facc1 * dens * a.Surf[i];
-      dens *= (1.0 - facc1);
-      v[0] += deltaM;
-      v[1] += deltaM * vxcell;
-      v[2] += deltaM * vycell;
-      if (distance < 0.45 * RRoche) {
-        deltaM = facc2 * dens * a.Surf[i];
-        dens *= (1.0 - facc2);
-        v[0] += deltaM;
-        v[1] += deltaM * vxcell;
-        v[2] += deltaM * vycell;
+    // barycentre changes (per thread); then fx, fy per force target, summed
+    // per wavefront after each chunk of TB cells into the wave's own slots
+    // of red (few live registers: no spills in the control kernel)
+    for (int q = t; q < NV * 16; q += TB) red[q] = 0.0;
+    __syncthreads();
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int base = 0; base < ncell; base += TB) {
+      const int idx = base + t;
+      double dcm = 0.0, xc = 0.0, yc = 0.0;
+      if (idx < ncell) {
+        const int i = i_min + idx / wj, j = j_min + idx % wj;
+        int jf = j % ns;
+        jf = (jf < 0) ? jf + ns : jf;
+        const long l = (long)i * ns + jf, lip = l + ns, ljp = (jf == ns - 1) ? (long)i * ns : l + 1;
+        xc = a.Rmed[i] * a.CosAz[jf];
+        yc = a.Rmed[i] * a.SinAz[jf];
+        const double dx = Xp - xc, dy = Yp - yc, distance = sqrt(dx * dx + dy * dy);
+        // the inner radius (0.45) lies inside the outer one (0.75)
+        if (distance < 0.75 * RRoche) {
+          const double vtcell = 0.5 * (ld<T>(a.vt, l) + ld<T>(a.vt, ljp)) + a.Rmed[i] * c->omega_frame;
+          const double vrcell = 0.5 * (ld<T>(a.vr, l) + ld<T>(a.vr, lip));
+          const double vxcell = (vrcell * xc - vtcell * yc) / a.Rmed[i];
+          const double vycell = (vrcell * yc + vtcell * xc) / a.Rmed[i];
+          const double d0 = ld<T>(a.rho, l);
+          double dens = d0;
+          double deltaM = facc1 * dens * a.Surf[i];
+          dens *= (1.0 - facc1);
+          v[0] += deltaM;
+          v[1] += deltaM * vxcell;
+          v[2] += deltaM * vycell;
+          if (distance < 0.45 * RRoche) {
+            deltaM = facc2 * dens * a.Surf[i];
+            dens *= (1.0 - facc2);
+            v[0] += deltaM;
+            v[1] += deltaM * vxcell;
+            v[2] += deltaM * vycell;
+          }
+          st<T>(a.rho, l, dens);
+          // the change of the cell's mass as stored, in the partials' terms
+          dcm = a.Surf[i] * ((double)(T)dens - d0);
+          v[3] += dcm;
+          v[4] += dcm * xc;
+          v[5] += dcm * yc;
+        }
       }
-      st<T>(a.rho, l, dens);
-      // the change of the cell's mass as stored, in the partials' terms
-      const double dcm = a.Surf[i] * ((double)(T)dens - d0);
-      v[3] += dcm;
-      v[4] += dcm * xc;
-      v[5] += dcm * yc;
-#pragma unroll
-      for (int tg = 0; tg < DF_MAXTG; tg++) {
-        if (tg >= ntg) break;
-        const double ddx = xc - c->tg_x[tg], ddy = yc - c->tg_y[tg], rsm = c->tg_rsm[tg];
-        const double dist2 = ddx * ddx + ddy * ddy + rsm * rsm, dd = sqrt(dist2);
-        const double InvDist3 = 1.0 / dist2 / dd;
-        v[6 + 2 * tg] += DF_G * dcm * ddx * InvDist3 * 1.0;
-        v[7 + 2 * tg] += DF_G * dcm * ddy * InvDist3 * 1.0;
+      for (int tg = 0; tg < ntg; tg++) {
+        double fx = 0.0, fy = 0.0;
+        if (dcm != 0.0) {
+          const double ddx = xc - c->tg_x[tg], ddy = yc - c->tg_y[tg], rsm = c->tg_rsm[tg];
+          const double dist2 = ddx * ddx + ddy * ddy + rsm * rsm, dd = sqrt(dist2);
+          const double InvDist3 = 1.0 / dist2 / dd;
+          fx = DF_G * dcm * ddx * InvDist3 * 1.0;
+          fy = DF_G * dcm * ddy * InvDist3 * 1.0;
+        }
+        fx = wave_sum(fx);
+        fy = wave_sum(fy);
+        if (lane == 0) {
+          red[(6 + 2 * tg) * 16 + wv] += fx;
+          red[(7 + 2 * tg) * 16 + wv] += fy;
+        }
       }
     }
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
+    for (int q = 0; q < 6; q++) {
       const double w = wave_sum(v[q]);
       if (lane == 0) red[q * 16 + wv] = w;
     }
     __syncthreads();
     if (t == 0) {
-      double s[NV];
-#pragma unroll
-      for (int q = 0; q < NV; q++) {
-        s[q] = 0.0;
-        for (int w = 0; w < nw; w++) s[q] += red[q * 16 + w];
-      }
+      // the wavefronts' sums of value q, in wavefront order
+      auto total = [&](int q) {
+        double x = 0.0;
+        for (int w = 0; w < nw; w++) x += red[q * 16 + w];
+        return x;
+      };
       double Px = Mp * c->pl_vx[k], Py = Mp * c->pl_vy[k], M = Mp;
-      Px += s[1];
-      Py += s[2];
-      M += s[0];
+      Px += total(1);
+      Py += total(2);
+      M += total(0);
       if (c->pl_feel_disk[k]) {
         c->pl_vx[k] = Px / M;
         c->pl_vy[k] = Py / M;
       }
       c->pl_m[k] = M;
-      sums[2] += s[3];
-      sums[3] += s[4];
-      sums[4] += s[5];
+      sums[2] += total(3);
+      sums[3] += total(4);
+      sums[4] += total(5);
       for (int tg = 0; tg < ntg; tg++) {
-        sums[5 + 4 * tg] += s[6 + 2 * tg];
-        sums[5 + 4 * tg + 2] += s[7 + 2 * tg];
+        sums[5 + 4 * tg] += total(6 + 2 * tg);
+        sums[5 + 4 * tg + 2] += total(7 + 2 * tg);
       }
     }
     __syncthreads();
   }
 }
 
-template <typename T> __global__ void __launch_bounds__(1024) k_control(KArgs a) {
+// Two builds: without accretion, 1024 threads; with it, 512, whose larger
+// register budget keeps the accretion path and the one-thread n-body code
+// free of spills.
+template <bool ACC> struct CtrlBlock { static constexpr int n = 1024; };
+template <> struct CtrlBlock<true> { static constexpr int n = 512; };
+template <typename T, bool ACC> __global__ void __launch_bounds__(CtrlBlock<ACC>::n) k_control(KArgs a) {
   __shared__ double sh[1024];
   __shared__ double sums[DF_NPART(DF_MAXTG)];
   dfargo_ctrl *c = a.ctrl;
@@ -816,7 +836,7 @@ template <typename T> __global__ void __launch_bounds__(1024) k_control(KArgs a)
     c->dt = dt;
     s_dt = dt;
   }
-  if (a.accrete) {
+  if constexpr (ACC) {
     // AccreteOntoPlanets right after dt (src/SourceEuler.c:454)
     __syncthreads();
     accrete_planets<T>(a, c, s_dt, sums, sh);
@@ -1964,7 +1984,10 @@ static dim3 cell_block(const KArgs &a) { return dim3(a.ns < 256 ? a.ns : 256); }
 template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) {
   switch (id) {
     case KID_CFL: RING_DISPATCH_K(wide_k(a.ns), k_cfl_ring, T, a, s, 0); break;
-    case KID_CONTROL: hipLaunchKernelGGL(k_control<T>, dim3(1), dim3(1024), 0, s, a); break;
+    case KID_CONTROL:
+      if (a.accrete) hipLaunchKernelGGL((k_control<T, true>), dim3(1), dim3(CtrlBlock<true>::n), 0, s, a);
+      else hipLaunchKernelGGL((k_control<T, false>), dim3(1), dim3(CtrlBlock<false>::n), 0, s, a);
+      break;
     case KID_NONREFLECTING:
       hipLaunchKernelGGL(k_nonreflecting<T>, dim3(2), dim3(NR_TB), 0, s, a);
       break;

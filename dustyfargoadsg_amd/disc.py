@@ -15,7 +15,11 @@ from ._capi import Params, Planets, Status, FIELDS
 
 
 class DfargoError(RuntimeError):
-    pass
+    """A failed C-ABI call; `code` is its DFARGO_ERR_* return code."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def _ptr(a):
@@ -28,7 +32,7 @@ def _ptr(a):
 
 def _check(lib, rc):
     if rc != _capi.OK:
-        raise DfargoError(f"dfargo error {rc}: {lib.dfargo_last_error().decode()}")
+        raise DfargoError(f"dfargo error {rc}: {lib.dfargo_last_error().decode()}", rc)
 
 
 def make_planets(params, spec, lib=None):
@@ -223,6 +227,24 @@ def init_dust_system(params, d, axi_sg_accr=None, lib=None):
     _check(lib, lib.dfargo_init_dust_system(ctypes.byref(params), ctypes.byref(d), _ptr(axi_sg_accr),
                                             ctypes.byref(_capi.dust_state(arrs))))
     return {k: v[:n].copy() for k, v in arrs.items()}
+
+
+def read_par(path, lib=None):
+    """A .par file as (dfargo_params, dfargo_run_params): ReadVariables and
+    its decoding (src/Interpret.c:82-476), then the parameter checks."""
+    lib = lib or _capi.load()
+    p, run = _capi.Params(), _capi.RunParams()
+    _check(lib, lib.dfargo_read_par(str(path).encode(), ctypes.byref(p), ctypes.byref(run)))
+    return p, run
+
+
+def read_planet_config(path, params, lib=None):
+    """A planet .cfg file as the initial planetary system
+    (InitPlanetarySystem, src/Psys.c:96-196)."""
+    lib = lib or _capi.load()
+    sys_ = _capi.Planets()
+    _check(lib, lib.dfargo_read_planet_config(str(path).encode(), ctypes.byref(params), ctypes.byref(sys_)))
+    return sys_
 
 
 def orbital_elements(x, y, vx, vy, m, lib=None):

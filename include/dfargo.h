@@ -336,6 +336,30 @@ int dfargo_update_log(dfargo_ctx *ctx, int max_planets, double *torques, double 
  * lambda, varpi (the columns of orbit<n>.dat after the time). */
 int dfargo_orbital_elements(double x, double y, double vx, double vy, double m, double *out);
 
+/* The run-level keys of a .par file that dfargo_params does not carry
+ * (src/var.c:15-24). */
+typedef struct dfargo_run_params {
+  double dt;                  /* DT: the interval of one AlgoGas call     */
+  int    ntot, ninterm;       /* NTOT, NINTERM                            */
+  char   planet_config[512];  /* PLANETCONFIG                             */
+  char   output_dir[512];     /* OUTPUTDIR                                */
+} dfargo_run_params;
+
+/* ReadVariables (src/Interpret.c:82-160) and the decoding after it
+ * (src/Interpret.c:165-476) for the keys the hydro step reads, with the
+ * defaults of src/var.c; then dfargo_check_params.  A missing mandatory key
+ * is DFARGO_ERR_ARG; a file that switches on physics not implemented here
+ * (thermal cooling, diffusion, the dust fluid, ...) is
+ * DFARGO_ERR_UNSUPPORTED.  run may be NULL.  The particle keys are not read:
+ * dfargo_dust_params describes the particle set.  Replaces the host's
+ * ReadVariables for callers without the reference's host. */
+int dfargo_read_par(const char *path, dfargo_params *p, dfargo_run_params *run);
+
+/* InitPlanetarySystem (src/Psys.c:96-196) from a planet .cfg file: lines
+ * starting with a letter are "name distance mass accretion FeelDisk
+ * FeelOthers [Binary]", read with %f as the reference reads them. */
+int dfargo_read_planet_config(const char *path, const dfargo_params *p, dfargo_planets *sys);
+
 /* Per-kernel timing with HIP events on the context's stream.  While
  * enabled, sub-steps launch eagerly and every kernel is bracketed by a pair
  * of events.  names: up to max entries of at most 47 characters. */
