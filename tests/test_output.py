@@ -1,0 +1,119 @@
+"""The reference's on-disk formats and restart path (src/Output.c,
+src/Init.c:18-166, src/main.c:266-272), restated in
+dustyfargoadsg_amd/output.py.  Host code only: the oracle supplies the
+states.  Checks: byte layout and sizes of the field files, the planet-file
+line format and GetfromPlanetFile's column rules, the text of dims.dat and
+used_rad.dat, and a restart that continues a run of the oracle exactly
+(fields are binary; the planets pass through %.14g text, so they are
+compared to that precision)."""
+import ctypes
+import math
+
+import numpy as np
+
+from dustyfargoadsg_amd import output
+from dustyfargoadsg_amd.workloads import gas_planet, energy_sg
+from tests.oracle_lib import Oracle
+
+DT = 2 * math.pi / 20
+
+
+def test_field_files_round_trip(tmp_path, oracle_built):
+    p, pl = gas_planet(nrad=16, nsec=48)
+    o = Oracle(p)
+    st = o.initial_state(pl)
+    nr, ns = p.nrad, p.nsec
+    output.write_outputs(tmp_path, 3, st, nr, ns)
+    for name in ("gasdens", "gasvrad", "gasvtheta"):
+        assert (tmp_path / f"{name}3.dat").stat().st_size == nr * ns * 8
+    back = output.read_disk_polar(tmp_path, "gasvrad", 3, nr, ns)
+    assert np.array_equal(back[: nr * ns], st["vr"][: nr * ns]) and not back[nr * ns:].any()
+    assert output.read_disk_polar(tmp_path, "gasdens", 4, nr, ns) is None
+
+
+def test_planet_file_lines_and_columns(tmp_path, oracle_built):
+    p, pl = gas_planet(nrad=16, nsec=48)
+    o = Oracle(p)
+    o.initial_state(pl)
+    o.algogas(DT)
+    s, sys_ = o.status(), o.state()["planets"]
+    output.write_planet_files(tmp_path, 0, sys_, s)
+    o.algogas(DT)
+    s1, sys1 = o.status(), o.state()["planets"]
+    output.write_planet_files(tmp_path, 1, sys1, s1)
+    lines = (tmp_path / "planet0.dat").read_text().splitlines()
+    assert len(lines) == 2
+    # the C library's own "%d" and "%#.14g" give the same text
+    libc = ctypes.CDLL(None)
+    buf = ctypes.create_string_buffer(64)
+    cols = lines[1].split("\t")
+    assert len(cols) == 11 and cols[0] == "1"
+    for c in cols[1:]:
+        libc.snprintf(buf, 64, b"%#.14g", ctypes.c_double(float(c)))
+        assert buf.value.decode() == c
+    # GetfromPlanetFile picks the line of the asked timestep
+    assert output.get_from_planet_file(tmp_path, 1, 2, 0) == float("%.14g" % sys1.x[0])
+    assert output.get_from_planet_file(tmp_path, 0, 8, 0) == float("%.14g" % s.physical_time)
+    assert output.get_from_planet_file(tmp_path, 1, 9, 0) == 0.0
+
+
+def test_dims_and_used_rad(tmp_path, oracle_built):
+    output.write_dims(tmp_path, 2.5, 100, 10, 128, 384)
+    assert (tmp_path / "dims.dat").read_text() == "0\t0\t\t0\t0\t2.500000\t10\t128\t384\n"
+    p, _ = gas_planet(nrad=16, nsec=48)
+    radii = Oracle(p).tables()[0][:17]
+    output.write_used_rad(tmp_path, radii)
+    back = np.array([float(x) for x in (tmp_path / "used_rad.dat").read_text().split()])
+    assert np.array_equal(back, radii)  # %.18g is exact for doubles
+
+
+def test_restart_continues_the_run(tmp_path, oracle_built):
+    """Write output 1 after one call, restart a fresh oracle from it (fields,
+    planets, LostMass, PhysicalTime) and take the next call on both: the
+    fields agree to the planets' 14-digit text precision."""
+    p, pl = gas_planet(nrad=32, nsec=96, forced_circular=0)
+    pl = dict(pl, feel_disk=[1], feel_others=[1])
+    o = Oracle(p)
+    o.initial_state(pl)
+    o.algogas(DT)
+    st, s = o.state(), o.status()
+    output.write_outputs(tmp_path, 1, st, p.nrad, p.nsec)
+    output.write_planet_files(tmp_path, 1, st["planets"], s)
+    r = Oracle(p)
+    rst = r.initial_state(pl)
+    back = output.read_restart(tmp_path, 1, p, r.tables()[1])
+    planets = output.restart_planetary_system(tmp_path, 1, rst["planets"])
+    lost, t, w = output.restart_scalars(tmp_path, 1, p.corotating)
+    assert w is None and t == float("%.14g" % s.physical_time)
+    for k in ("rho", "vr", "vt"):
+        rst[k][:] = back[k]
+    rst["planets"] = planets
+    r.set_state(rst)
+    r.set_time(s.physical_time, 0.0, 0.0)
+    assert r.algogas(DT) == o.algogas(DT)
+    a, b = r.state(), o.state()
+    n = p.nrad * p.nsec
+    for k in ("rho", "vt"):
+        assert np.abs(a[k][:n] - b[k][:n]).max() <= 1e-11 * np.abs(b[k][:n]).max(), k
+
+
+def test_restart_energy_from_temperature(tmp_path, oracle_built):
+    """With the energy equation the restart reads the Temperature grid and
+    forms e = Sigma T / (gamma - 1) (src/Init.c:155-166); a planet0.dat must
+    exist, as GetfromPlanetFile(..., 9, 0) is read in every restart."""
+    p, pl = energy_sg(nrad=16, nsec=48)
+    o = Oracle(p)
+    st = o.initial_state(pl)
+    temp = o.field("temperature")
+    output.write_outputs(tmp_path, 2, st, p.nrad, p.nsec, temperature=temp)
+
+    class S:
+        lost_mass = physical_time = omega_frame = 0.0
+
+    class One:
+        nb, x, y, vx, vy, mass = 1, [1.0], [0.0], [0.0], [1.0], [0.0]
+
+    output.write_planet_files(tmp_path, 2, One, S)
+    back = output.read_restart(tmp_path, 2, p, o.tables()[1])
+    n = p.nrad * p.nsec
+    assert np.abs(back["energy"][:n] - st["energy"][:n]).max() <= 1e-14 * np.abs(st["energy"][:n]).max()
