@@ -268,7 +268,10 @@ static void build_sequence(dfargo_ctx *c) {
   c->seq.push_back(KID_SOURCE);
   if (c->p.energy_equation) c->seq.push_back(KID_ENERGY);
   for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_FINISH}) c->seq.push_back(id);
-  if (c->p.boundary == DFARGO_BC_NONREFLECTING) c->seq.push_back(KID_NONREFLECTING);
+  if (c->p.boundary == DFARGO_BC_NONREFLECTING) {
+    if (c->p.energy_equation) c->seq.push_back(KID_EOS_STEP);
+    c->seq.push_back(KID_NONREFLECTING);
+  }
   if (c->p.wave_killing_zones && (c->a.wkz_n_in > 0 || c->a.wkz_i_out < c->p.nrad))
     c->seq.push_back(KID_EVANESCENT);
 }

@@ -73,12 +73,6 @@ int dfargo_check_params(const dfargo_params *p) {
   if (p->boundary != DFARGO_BC_WALL && p->boundary != DFARGO_BC_OPEN && p->boundary != DFARGO_BC_NONREFLECTING)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
                             "only WALL, OPEN and NONREFLECTING boundaries are implemented");
-  /* with the energy equation the reference recomputes the sound speed inside
-     the boundary call (src/SideEuler.c:1121-1122), which the next sub-step's
-     CFL condition then reads */
-  if (p->boundary == DFARGO_BC_NONREFLECTING && p->energy_equation)
-    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
-                            "the non-reflecting boundary is implemented for isothermal discs only");
   /* the two-dimensional transforms; the axisymmetric mode is a direct
      radial sum with no size condition */
   const int sg2d = p->self_gravity && !p->sg_zero_mode;

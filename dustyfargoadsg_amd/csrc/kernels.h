@@ -115,7 +115,7 @@ enum KernelId {
   KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
   KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER,
   KID_EVANESCENT, KID_NONREFLECTING,
-  KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL, KID_COUNT
+  KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL, KID_EOS_STEP, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 

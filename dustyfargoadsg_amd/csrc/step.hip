@@ -17,7 +17,7 @@ const char *kKernelNames[KID_COUNT] = {
     "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
     "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
     "dust_sort",     "dust_deposit", "dust_reorder", "evanescent", "nonreflecting",
-    "sgz_mean",      "sgz_conv",     "sgz_fill"};
+    "sgz_mean",      "sgz_conv",     "sgz_fill",     "eos_step"};
 
 // ------------------------------------------------------------------------
 // helpers
@@ -1875,26 +1875,31 @@ template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArg
 // neighbour's: the constant is formed from that mean.  The sound speed is
 // the isothermal ring value, the mean of its identical cells.
 enum { NR_TB = 256 };
-template <typename T> __global__ void __launch_bounds__(NR_TB) k_nonreflecting(KArgs a) {
-  const dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
-  const bool inner = blockIdx.x == 0;
-  const int nr = a.nr, ns = a.ns, tid = threadIdx.x;
-  // the ring the edge copies from, and the edge ring
-  const int src = inner ? 1 : nr - 2, dst = inner ? 0 : nr - 1;
-  __shared__ double part[NR_TB / 64];
-  double s = 0.0;
-  for (int j = tid; j < ns; j += NR_TB) s += ld<T>(a.rho, (long)src * ns + j);
-  s = wave_sum(s);
-  if ((tid & 63) == 0) part[tid >> 6] = s;
+
+// block sums of N values per thread, in a fixed order (wavefront tree, then
+// the wavefronts in order); every thread gets the totals
+template <int N> __device__ void nr_block_sums(double (&v)[N], double (*part)[NR_TB / 64]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    const double w = wave_sum(v[q]);
+    if ((tid & 63) == 0) part[q][tid >> 6] = w;
+  }
   __syncthreads();
-  double mean = 0.0;
-  for (int w = 0; w < NR_TB / 64; w++) mean += part[w];
-  mean /= (double)ns;
-  // the planet that sets the pattern speed (the loop of src/SideEuler.c:
-  // 590-600 and 702-712; the reference leaves r undefined when planet 0 is
-  // outside the grid, here the grid's edge stands in)
-  const double r0 = a.Rinf[0], r1 = a.Rsup[nr - 1];
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    double s = 0.0;
+    for (int w = 0; w < NR_TB / 64; w++) s += part[q][w];
+    v[q] = s;
+  }
+}
+
+// the pattern speed: the innermost (outermost) planet inside the grid (the
+// loops of src/SideEuler.c:590-600 and 702-712; the reference leaves r
+// undefined when planet 0 is outside the grid, here the grid's edge stands
+// in)
+__device__ double nr_pattern_omega(const KArgs &a, const dfargo_ctrl *c, bool inner) {
+  const double r0 = a.Rinf[0], r1 = a.Rsup[a.nr - 1];
   double buf = inner ? r1 : r0, rp_sel = buf, omega = pow(buf, -1.5);
   for (int k = 0; k < c->nb; k++) {
     const double xp = c->pl_x[k], yp = c->pl_y[k], rp = sqrt(xp * xp + yp * yp);
@@ -1904,28 +1909,124 @@ template <typename T> __global__ void __launch_bounds__(NR_TB) k_nonreflecting(K
     }
     buf = rp_sel;
   }
-  const double cs_a = a.cs_iso[inner ? 0 : nr - 1], cs_b = a.cs_iso[inner ? 1 : nr - 2];
+  return omega;
+}
+
+__device__ __forceinline__ int nr_wrap(int j, int ns) {
+  int jp = j % ns;
+  return (jp < 0) ? jp + ns : jp;
+}
+
+// NonReflectingBoundary (src/SideEuler.c:569-794) without the accreting
+// variant, after the sub-Keplerian reset (which it leaves alone).
+// Workgroup 0 does the inner edge, workgroup 1 the outer one.  Each edge
+// takes a pattern speed from the innermost (outermost) planet inside the
+// grid and turns it into an integer azimuthal shift i_angle over the first
+// (last) ring spacing at a mean sound speed.
+//   barotropic: rho(0, j + i_angle) = rho(1, j) and
+//     rho(nr-1, j) = rho(nr-2, j - i_angle), each shifted by a constant so
+//     that the edge ring's mean is SigmaMed; vr at the faces 1 and nr-1 is
+//     2 vr_med - vr of the next face in, vr_med = -/+ cs drho / <rho>;
+//   adiabatic (inner edge): the energy is copied with the shift, the
+//     density follows it acoustically, SigmaMed[0] + (gamma-1)(e -
+//     EnergyMed[0]) / csinit0^2, and both are shifted to the initial means;
+//     vr_med uses the initial profiles; the outer edge is the barotropic one
+//     with the energy copied and shifted to EnergyMed too and vr_med from the
+//     cell's own sound speed.
+// A shifted edge ring is a cyclic copy of its neighbour, so its mean is the
+// neighbour's: the constants are formed from that mean.  The sound speed is
+// the isothermal ring value (the mean of its identical cells), or, with the
+// energy equation, the grid k_eos_step has just formed from the state the
+// boundary sees (src/SideEuler.c:1121-1122).
+template <typename T> __global__ void __launch_bounds__(NR_TB) k_nonreflecting(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const bool inner = blockIdx.x == 0;
+  const int nr = a.nr, ns = a.ns, tid = threadIdx.x;
+  // the ring the edge copies from, and the edge ring
+  const int src = inner ? 1 : nr - 2, dst = inner ? 0 : nr - 1;
+  const long bs = (long)src * ns, bd = (long)dst * ns;
+  __shared__ double part[4][NR_TB / 64];
+  const double omega = nr_pattern_omega(a, c, inner);
+  const double g = a.gamma;
+  if (a.energy && inner) {
+    const double csinit0 = sqrt(g) * a.aspect_ratio * pow(a.Rmed[0], -0.5 + a.flaring);
+    const double csinit1 = sqrt(g) * a.aspect_ratio * pow(a.Rmed[1], -0.5 + a.flaring);
+    double dangle = (pow(a.Rinf[1], -1.5) - omega) / (.5 * (csinit0 + csinit1));
+    dangle *= (a.Rmed[1] - a.Rmed[0]);
+    const int i_angle = (int)(dangle / a.pm * (double)ns + .5);
+    double s[2] = {0.0, 0.0};
+    for (int j = tid; j < ns; j += NR_TB) {
+      const double e = ld<T>(a.en, bs + j);
+      s[0] += e;
+      s[1] += a.SigmaMed[0] + (g - 1.0) * (e - a.EnergyMed[0]) / csinit0 / csinit0;
+    }
+    nr_block_sums<2>(s, part);
+    const double mean_e = s[0] / (double)ns, mean_rho = s[1] / (double)ns;
+    for (int j = tid; j < ns; j += NR_TB) {
+      const double e = ld<T>(a.en, bs + j);
+      const double rho = a.SigmaMed[0] + (g - 1.0) * (e - a.EnergyMed[0]) / csinit0 / csinit0;
+      const int jp = nr_wrap(j + i_angle, ns);
+      st<T>(a.en, jp, e + (a.EnergyMed[0] - mean_e));
+      st<T>(a.rho, jp, rho + (a.SigmaMed[0] - mean_rho));
+      const double vr_med = -csinit1 * (ld<T>(a.rho, bs + j) - a.SigmaMed[1]) / a.SigmaMed[1];
+      st<T>(a.vr, bs + j, 2. * vr_med - ld<T>(a.vr, bs + j + ns));
+    }
+    return;
+  }
+  // ring means: rho of the source ring; the sound speed of both rings
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = tid; j < ns; j += NR_TB) {
+    s[0] += ld<T>(a.rho, bs + j);
+    if (a.energy) {
+      s[1] += ld<T>(a.cs, bd + j);
+      s[2] += ld<T>(a.cs, bs + j);
+      s[3] += ld<T>(a.en, bs + j);
+    }
+  }
+  nr_block_sums<4>(s, part);
+  const double mean = s[0] / (double)ns;
+  // isothermal: the ring value, the mean of the ring's identical cells
+  const double cs_a = a.energy ? s[1] / (double)ns : a.cs_iso[dst];
+  const double cs_b = a.energy ? s[2] / (double)ns : a.cs_iso[src];
   double dangle = (pow(a.Rinf[inner ? 1 : nr - 2], -1.5) - omega) / (.5 * (cs_a + cs_b));
   dangle *= inner ? (a.Rmed[1] - a.Rmed[0]) : (a.Rmed[nr - 1] - a.Rmed[nr - 2]);
   const int i_angle = (int)(dangle / a.pm * (double)ns + .5);
   const double shift = a.SigmaMed[dst] - mean;
-  const double cs_v = inner ? -a.cs_iso[1] : a.cs_iso[nr - 1];
+  const double eshift = a.EnergyMed[nr - 1] - s[3] / (double)ns;
   for (int j = tid; j < ns; j += NR_TB) {
-    const long ls = (long)src * ns + j;
+    const long ls = bs + j;
     const double rs = ld<T>(a.rho, ls);
-    const double vr_med = cs_v * (rs - mean) / mean;
     if (inner) {
-      int jp = (j + i_angle) % ns;
-      jp = (jp < 0) ? jp + ns : jp;
-      st<T>(a.rho, jp, rs + shift);
+      const double vr_med = -a.cs_iso[1] * (rs - mean) / mean;
+      st<T>(a.rho, nr_wrap(j + i_angle, ns), rs + shift);
       st<T>(a.vr, ls, 2. * vr_med - ld<T>(a.vr, ls + ns));
     } else {
-      int jp = (j - i_angle) % ns;
-      jp = (jp < 0) ? jp + ns : jp;
-      const long l = (long)dst * ns + j;
-      st<T>(a.rho, l, ld<T>(a.rho, (long)src * ns + jp) + shift);
+      const long l = bd + j;
+      const long lp = bs + nr_wrap(j - i_angle, ns);
+      // the outer face's own sound speed with the energy equation
+      const double csv = a.energy ? ld<T>(a.cs, l) : a.cs_iso[nr - 1];
+      const double vr_med = csv * (rs - mean) / mean;
+      st<T>(a.rho, l, ld<T>(a.rho, lp) + shift);
+      if (a.energy) st<T>(a.en, l, ld<T>(a.en, lp) + eshift);
       st<T>(a.vr, l, 2. * vr_med - ld<T>(a.vr, ls));
     }
+  }
+}
+
+// ComputeSoundSpeed inside ApplyBoundaryCondition (src/SideEuler.c:
+// 1121-1122), for the non-reflecting boundary with the energy equation: cs
+// of the state after transport and floors, which the boundary and the next
+// sub-step's CFL condition read
+template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_eos_step(KArgs a) {
+  if (!a.ctrl->active) return;
+  const int i = ring_of_block(a), ns = a.ns, t = threadIdx.x, TB = blockDim.x;
+  const long base = (long)i * ns;
+  const double g = a.gamma;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long l = base + t + k * TB;
+    st<T>(a.cs, l, sqrt(g * (g - 1.0) * ld<T>(a.en, l) / ld<T>(a.rho, l)));
   }
 }
 
@@ -1988,6 +2089,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       if (a.accrete) hipLaunchKernelGGL((k_control<T, true>), dim3(1), dim3(CtrlBlock<true>::n), 0, s, a);
       else hipLaunchKernelGGL((k_control<T, false>), dim3(1), dim3(CtrlBlock<false>::n), 0, s, a);
       break;
+    case KID_EOS_STEP: RING_DISPATCH(k_eos_step, T, a, s, 0); break;
     case KID_NONREFLECTING:
       hipLaunchKernelGGL(k_nonreflecting<T>, dim3(2), dim3(NR_TB), 0, s, a);
       break;

@@ -38,7 +38,7 @@ enum dfargo_boundary {
   DFARGO_BC_WALL = 0,  /* INNERBOUNDARY WALL (default, src/var.c:20)       */
   DFARGO_BC_OPEN = 1,  /* INNERBOUNDARY OPEN (src/SideEuler.c:300-408)     */
   DFARGO_BC_NONREFLECTING = 2 /* INNERBOUNDARY NONREFLECTING
-                                 (src/SideEuler.c:569-794), isothermal only */
+                                 (src/SideEuler.c:569-794)                 */
 };
 
 /* FRAME (src/Interpret.c:225-233): fixed (rotating at OMEGAFRAME) or

@@ -1736,18 +1736,54 @@ static void open_boundary(or_ctx *c) {
   }
 }
 
-/* NonReflectingBoundary (src/SideEuler.c:569-794), barotropic case, no
- * accreting variant.  The pattern speed is that of the innermost
- * (outermost) planet inside the grid; where the reference reads an
- * undefined radius (planet 0 outside the grid) the grid's edge stands in.
- * The energy grid of a barotropic disc holds zeros, which the outer edge's
- * copy and mean adjustment leave at zero. */
+/* NonReflectingBoundary (src/SideEuler.c:569-794), no accreting variant.
+ * The pattern speed is that of the innermost (outermost) planet inside the
+ * grid; where the reference reads an undefined radius (planet 0 outside
+ * the grid) the grid's edge stands in.  The energy grid of a barotropic
+ * disc holds zeros, which the outer edge's copy and mean adjustment leave
+ * at zero. */
 static void nonreflecting_boundary(or_ctx *c) {
   const dfargo_params *p = &c->p;
   const int nr = c->nr, ns = c->ns;
-  double *rho = c->rho, *vr = c->vr, *cs = c->cs;
-  /* inner edge */
-  {
+  double *rho = c->rho, *vr = c->vr, *cs = c->cs, *energy = c->en;
+  /* inner edge, adiabatic case (src/SideEuler.c:654-699) */
+  if (p->energy_equation) {
+    const int i = 1;
+    double buf = c->Radii[nr], r_in = c->Radii[nr], omega_in = pow(c->Radii[nr], -1.5);
+    for (int k = 0; k < c->sys.nb; k++) {
+      const double xp = c->sys.x[k], yp = c->sys.y[k], rp = sqrt(xp * xp + yp * yp);
+      if ((rp < buf) && (rp >= c->Radii[0])) {
+        r_in = rp;
+        omega_in = pow(rp, -1.5);
+      }
+      buf = r_in;
+    }
+    const double g = p->adiabatic_index;
+    const double csinit0 = sqrt(g) * p->aspect_ratio * pow(c->Rmed[0], -0.5 + p->flaring_index);
+    const double csinit1 = sqrt(g) * p->aspect_ratio * pow(c->Rmed[1], -0.5 + p->flaring_index);
+    double dangle = (pow(c->Rinf[1], -1.5) - omega_in) / (.5 * (csinit0 + csinit1));
+    dangle *= (c->Rmed[1] - c->Rmed[0]);
+    const int i_angle = (int)(dangle / (p->pmax - p->pmin) * (double)ns + .5);
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns;
+      int jp = j + i_angle;
+      if (jp >= ns) jp -= ns;
+      if (jp < 0) jp += ns;
+      energy[jp] = energy[l];
+      const double vr_med = -csinit1 * (rho[l] - c->SigmaMed[1]) / c->SigmaMed[1];
+      vr[l] = 2. * vr_med - vr[l + ns];
+    }
+    for (int j = 0; j < ns; j++)
+      rho[j] = c->SigmaMed[0] + (g - 1.0) * (energy[j] - c->EnergyMed[0]) / csinit0 / csinit0;
+    double mean = 0.0;
+    for (int j = 0; j < ns; j++) mean += rho[j];
+    mean /= (double)ns;
+    for (int j = 0; j < ns; j++) rho[j] += (c->SigmaMed[0] - mean);
+    mean = 0.0;
+    for (int j = 0; j < ns; j++) mean += energy[j];
+    mean /= (double)ns;
+    for (int j = 0; j < ns; j++) energy[j] += (c->EnergyMed[0] - mean);
+  } else {
     const int i = 1;
     double buf = c->Radii[nr], r_in = c->Radii[nr], omega_in = pow(c->Radii[nr], -1.5);
     for (int k = 0; k < c->sys.nb; k++) {
@@ -1784,7 +1820,7 @@ static void nonreflecting_boundary(or_ctx *c) {
     mean /= (double)ns;
     for (int j = 0; j < ns; j++) rho[j] += (c->SigmaMed[0] - mean);
   }
-  /* outer edge */
+  /* outer edge (both cases) */
   {
     double buf = c->Radii[0], r_out = c->Radii[0], omega_out = pow(c->Radii[0], -1.5);
     for (int k = 0; k < c->sys.nb; k++) {
@@ -1815,13 +1851,18 @@ static void nonreflecting_boundary(or_ctx *c) {
       if (jp < 0) jp += ns;
       const int lp = jp + (i - 1) * ns;
       rho[l] = rho[lp];
-      const double vr_med = csnrm1 * (rho[l - ns] - densnrm2) / densnrm2;
+      energy[l] = energy[lp];
+      const double vr_med = (!p->energy_equation ? csnrm1 : cs[l]) * (rho[l - ns] - densnrm2) / densnrm2;
       vr[l] = 2. * vr_med - vr[l - ns];
     }
     double mean = 0.0;
     for (int j = 0; j < ns; j++) mean += rho[j + ns * (nr - 1)];
     mean /= (double)ns;
     for (int j = 0; j < ns; j++) rho[j + (nr - 1) * ns] += c->SigmaMed[nr - 1] - mean;
+    mean = 0.0;
+    for (int j = 0; j < ns; j++) mean += energy[j + ns * (nr - 1)];
+    mean /= (double)ns;
+    for (int j = 0; j < ns; j++) energy[j + (nr - 1) * ns] += c->EnergyMed[nr - 1] - mean;
   }
 }
 
@@ -1831,6 +1872,7 @@ static void apply_boundary_condition(or_ctx *c) {
   subkeplerian_boundary(c); /* unconditional: src/SideEuler.c:1110-1112 */
   if (c->p.boundary == DFARGO_BC_OPEN) open_boundary(c);
   if (c->p.boundary == DFARGO_BC_NONREFLECTING) {
+    if (c->p.energy_equation) compute_sound_speed(c); /* src/SideEuler.c:1121-1122 */
     nonreflecting_boundary(c);
     subkeplerian_boundary(c);
   }

@@ -116,7 +116,7 @@ def test_parameter_checks(lib):
     assert check(corotating=_capi.FRAME_BINARY_CENTER) == _capi.OK
     assert check(corotating=4) == _capi.ERR_ARG
     assert check(boundary=_capi.BC_NONREFLECTING) == _capi.OK
-    assert check(boundary=_capi.BC_NONREFLECTING, energy_equation=1) == _capi.ERR_UNSUPPORTED
+    assert check(boundary=_capi.BC_NONREFLECTING, energy_equation=1) == _capi.OK
     assert check(boundary=3) == _capi.ERR_UNSUPPORTED
     assert check(wave_killing_zones=1) == _capi.ERR_ARG
     assert check(wave_killing_zones=1, wkz_rmin=0.5, wkz_rmax=0.4) == _capi.ERR_ARG

@@ -209,3 +209,20 @@ def test_sg_zero_mode_planet_and_fp32():
         d.algogas(DT)
         assert d.status().last_call_substeps == n_o
     compare(o, d, p, 2e-5, 2e-6, 2e-6, 2e-5)
+
+
+@pytest.mark.parametrize("nr,ns,calls", [(32, 96, 2), (16, 1536, 1)])
+def test_nonreflecting_energy_fp64(nr, ns, calls):
+    """The adiabatic non-reflecting boundary (src/SideEuler.c:654-699 and
+    the outer edge with the cell's own sound speed), the sound speed formed
+    inside the boundary call and read by the next CFL condition, with
+    self-gravity and a planet that feels the disc."""
+    p, _ = energy_sg(nrad=nr, nsec=ns, boundary=_capi.BC_NONREFLECTING)
+    pl = dict(dist=[1.5], mass=[1e-4], acc=[0.0], feel_disk=[1], feel_others=[1])
+    o, so, d, sg = pair(p, pl)
+    for _ in range(calls):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+        assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
+    compare(o, d, p, 1e-12, 1e-12, 1e-12, 1e-12)

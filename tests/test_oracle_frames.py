@@ -355,3 +355,43 @@ def test_sg_zero_mode_is_the_axisymmetric_part(oracle_built):
     sig = rho[: nr * ns].reshape(nr, ns).mean(axis=1)
     expect = ring2.mean(axis=1) + sig * rstep * tstep * (k0.sum() - 1 / eps)
     assert np.abs(zero - expect).max() <= 1e-11 * np.abs(zero).max()
+
+
+def test_nonreflecting_adiabatic_edges(oracle_built):
+    """With the energy equation the inner edge copies the energy with the
+    shift and makes the density follow it acoustically: in ring 0,
+    Sigma - SigmaMed[0] = (gamma - 1)(e - EnergyMed[0]) / csinit0^2 up to
+    one constant, with csinit0 = sqrt(gamma) h Rmed0^(f - 1/2), and both
+    rings end at the initial means; the outer edge ring is a shifted copy of
+    its neighbour in density and energy, with means SigmaMed and EnergyMed.
+    A wrong slope, a missing mean adjustment or an unshifted copy fails."""
+    from dustyfargoadsg_amd.workloads import energy_sg
+    p, _ = energy_sg(nrad=32, nsec=96, self_gravity=0, boundary=_capi.BC_NONREFLECTING)
+    o = Oracle(p)
+    o.initial_state(_one(1.5, 1e-4))
+    o.algogas(1e-9)
+    st = o.state()
+    rng = np.random.default_rng(7)
+    for k in ("rho", "energy"):
+        st[k] = st[k] * (1 + 0.05 * rng.standard_normal(st[k].shape))
+    o.set_state(st)
+    o.nonreflecting()
+    af = o.state()
+    nr, ns = p.nrad, p.nsec
+    rmed = o.tables()[1]
+    ring = lambda d, k, i: d[k][i * ns:(i + 1) * ns]
+    g, h = p.adiabatic_index, p.aspect_ratio
+    cs0 = math.sqrt(g) * h * rmed[0] ** -0.5
+    sig = lambda i: p.sigma0 * rmed[i] ** -1.0
+    e0 = ring(af, "energy", 0)
+    rho0 = ring(af, "rho", 0)
+    lin = rho0 - (g - 1) * e0 / cs0 ** 2
+    assert np.ptp(lin) <= 1e-12 * rho0.max()
+    assert abs(rho0.mean() / sig(0) - 1) < 1e-13
+    s, spread = _best_shift(e0, ring(st, "energy", 1))
+    assert spread <= 1e-12 * e0.max() and s > 0
+    s_out, spread = _best_shift(ring(st, "energy", nr - 2), ring(af, "energy", nr - 1))
+    assert spread <= 1e-12 * ring(af, "energy", nr - 1).max()
+    s_rho, _ = _best_shift(ring(st, "rho", nr - 2), ring(af, "rho", nr - 1))
+    assert s_rho == s_out
+    assert abs(ring(af, "rho", nr - 1).mean() / sig(nr - 1) - 1) < 1e-13
