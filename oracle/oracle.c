@@ -28,6 +28,23 @@
  *   diagnostics     budgets of a uniform Keplerian disc in closed form, a
  *                   one-cell disc's force on a planet, Hill exclusion,
  *                   orbital elements at pericentre and at E = pi/2
+ *   frames          (psys_info, correct_vtheta) a circular planet stays put
+ *                   in FRAME C at rate sqrt(1+m); a run started inertial
+ *                   equals one in a frame fixed at that rate; FRAME G turns
+ *                   at the mean motion of an eccentric orbit; FRAME B and
+ *                   the outer-planet option follow the right body
+ *   accretion       (accrete_onto_planets) mass conservation planet+gas,
+ *                   per-cell fractions by distance, momentum conservation
+ *   wave-killing    (evanescent_boundary) ring means kept, deviations
+ *                   scaled by 1/(1+lambda) with lambda's closed-form
+ *                   profile; convergence to the initial profiles; ZEROVRADBC
+ *   non-reflecting  (nonreflecting_boundary) edge rings are shifted copies
+ *                   with the initial means, vr at the faces follows the
+ *                   outgoing acoustic relation, zero turn at the planets'
+ *                   pitch radii, the adiabatic density/energy relation
+ *   SG zero mode    (or_sg_zero_mode) equals the full field's ring values of
+ *                   an axisymmetric disc up to the two self-force terms
+ *   (tests/test_oracle_frames.py for the last five)
  */
 #include <math.h>
 #include <stdio.h>
