@@ -63,6 +63,7 @@ class Params(ctypes.Structure):
         ("wave_killing_zones", c_int), ("wkz_rmin", c_double), ("wkz_rmax", c_double),
         ("damping_ratio", c_double), ("damp_to", c_int), ("zero_vrad_bc", c_int),
         ("sg_zero_mode", c_int),
+        ("nrad1d", c_int), ("rmin1d", c_double), ("rmax1d", c_double), ("bc1d", c_int),
     ]
 
     def copy(self):
@@ -198,6 +199,7 @@ def default_params(**overrides):
     p.wave_killing_zones, p.wkz_rmin, p.wkz_rmax = 0, 0.0, 0.0
     p.damping_ratio, p.damp_to, p.zero_vrad_bc = 0.3, DAMP_TO_AXI, 0
     p.sg_zero_mode = 0
+    p.nrad1d, p.rmin1d, p.rmax1d, p.bc1d = 100, 0.01, 100.0, 0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(f"unknown parameter {k!r}")

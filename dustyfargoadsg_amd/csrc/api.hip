@@ -272,6 +272,9 @@ static void build_sequence(dfargo_ctx *c) {
     if (c->p.energy_equation) c->seq.push_back(KID_EOS_STEP);
     c->seq.push_back(KID_NONREFLECTING);
   }
+  // SolveOneDViscousEvolution runs whenever DAMPTOVISCOUS is on
+  // (src/SourceEuler.c:540-541); nothing reads it before the damping
+  if (c->p.damp_to == DFARGO_DAMP_TO_VISCOUS) c->seq.push_back(KID_VISC1D);
   if (c->p.wave_killing_zones && (c->a.wkz_n_in > 0 || c->a.wkz_i_out < c->p.nrad))
     c->seq.push_back(KID_EVANESCENT);
 }
@@ -315,6 +318,37 @@ static int build_context(dfargo_ctx *c) {
   a.damp_ratio = p.damping_ratio;
   a.damp_ini = p.damp_to == DFARGO_DAMP_TO_INI;
   a.zero_vrad = p.zero_vrad_bc;
+  a.damp_visc = p.damp_to == DFARGO_DAMP_TO_VISCOUS;
+  if (a.damp_visc) {
+    // InitializeOneDViscousEvolution (src/SideEuler.c:1444-1487), FP64 on
+    // the host: a log grid over [RMIN1D, RMAX1D], Sigma and the viscosity
+    // of the initial profiles, vr of the steady viscous drift
+    const int n = p.nrad1d;
+    std::vector<double> radii(n + 1), rmed(n), sig(n), nu(n, 0.0), vr(n);
+    for (int i = 0; i <= n; i++) radii[i] = p.rmin1d * exp((double)i / (double)n * log(p.rmax1d / p.rmin1d));
+    for (int i = 0; i < n; i++) {
+      rmed[i] = 2.0 / 3.0 * (radii[i + 1] * radii[i + 1] * radii[i + 1] - radii[i] * radii[i] * radii[i]);
+      rmed[i] = rmed[i] / (radii[i + 1] * radii[i + 1] - radii[i] * radii[i]);
+      sig[i] = dfargo_sigma_profile(&p, rmed[i]);
+      const double cs = dfargo_aspect_ratio(&p, rmed[i]) / sqrt(rmed[i]) * pow(rmed[i], p.flaring_index);
+      if (p.viscosity != 0.0) nu[i] = p.viscosity;
+      if (p.alpha_viscosity != 0.0) {
+        nu[i] = p.alpha_viscosity * cs * cs * pow(rmed[i], 1.5);
+        vr[i] = -3.0 * nu[i] / rmed[i] * (-p.sigma_slope + 2.0 * p.flaring_index + 1.0);
+      } else
+        vr[i] = -3.0 * nu[i] / rmed[i] * (-p.sigma_slope + 0.5);
+    }
+    // the damping interpolates at every zone ring's Rmed
+    for (int i = 0; i < nr; i++)
+      if (!(rmed[0] < t.Rmed[i] && t.Rmed[i] <= rmed[n - 1]))
+        return dfargo_set_error(DFARGO_ERR_ARG, "the 1-D grid's cell centres must enclose every Rmed");
+    a.n1d = n;
+    a.bc1d = p.bc1d;
+    if ((rc = upload_table(c, &a.v1d_rmed, rmed.data(), n)) || (rc = upload_table(c, &a.v1d_nu, nu.data(), n)) ||
+        (rc = upload_table(c, (const double **)&a.v1d_sigma, sig.data(), n)) ||
+        (rc = upload_table(c, (const double **)&a.v1d_vrad, vr.data(), n)))
+      return rc;
+  }
   a.xcd_remap = 1;
   {
     hipDeviceProp_t prop;

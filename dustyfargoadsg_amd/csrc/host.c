@@ -38,6 +38,7 @@ void dfargo_default_params(dfargo_params *p) {
   p->wave_killing_zones = 0; p->wkz_rmin = 0.0; p->wkz_rmax = 0.0;
   p->damping_ratio = 0.3; p->damp_to = DFARGO_DAMP_TO_AXI; p->zero_vrad_bc = 0;
   p->sg_zero_mode = 0;
+  p->nrad1d = 100; p->rmin1d = 0.01; p->rmax1d = 100.0; p->bc1d = 0;
 }
 
 int dfargo_check_params(const dfargo_params *p) {
@@ -65,11 +66,20 @@ int dfargo_check_params(const dfargo_params *p) {
     return dfargo_set_error(DFARGO_ERR_ARG, "wave-killing zones need WKZRMIN and WKZRMAX");
   if (p->wave_killing_zones && p->wkz_rmin > p->wkz_rmax)
     return dfargo_set_error(DFARGO_ERR_ARG, "WKZRMIN must not exceed WKZRMAX");
-  if (p->wave_killing_zones && p->damp_to == DFARGO_DAMP_TO_VISCOUS)
-    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
-                            "DAMPTOVISCOUS (1-D viscous evolution profiles) is not implemented");
-  if (p->wave_killing_zones && p->damp_to != DFARGO_DAMP_TO_AXI && p->damp_to != DFARGO_DAMP_TO_INI)
+  if (p->damp_to < DFARGO_DAMP_TO_AXI || p->damp_to > DFARGO_DAMP_TO_VISCOUS)
     return dfargo_set_error(DFARGO_ERR_ARG, "unknown damping target %d", p->damp_to);
+  if (p->damp_to == DFARGO_DAMP_TO_VISCOUS) {
+    /* the 1-D disc's viscosity is set only by VISCOSITY or ALPHAVISCOSITY
+       (src/SideEuler.c:1477-1480), and its grid must cover the 2-D one for
+       the interpolation of src/SideEuler.c:879-886 */
+    if (p->viscosity == 0.0 && p->alpha_viscosity == 0.0)
+      return dfargo_set_error(DFARGO_ERR_ARG, "DAMPTOVISCOUS needs VISCOSITY or ALPHAVISCOSITY");
+    if (p->nrad1d < 3 || p->nrad1d > 1024)
+      return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "NRAD1D must lie in [3, 1024], got %d", p->nrad1d);
+    if (!(p->rmin1d > 0.0) || !(p->rmin1d < p->rmin) || !(p->rmax1d > p->rmax))
+      return dfargo_set_error(DFARGO_ERR_ARG, "RMIN1D/RMAX1D must enclose [RMIN, RMAX]");
+    if (p->bc1d < 0 || p->bc1d > 2) return dfargo_set_error(DFARGO_ERR_ARG, "unknown BOUNDARY1DGRID %d", p->bc1d);
+  }
   if (p->boundary != DFARGO_BC_WALL && p->boundary != DFARGO_BC_OPEN && p->boundary != DFARGO_BC_NONREFLECTING)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
                             "only WALL, OPEN and NONREFLECTING boundaries are implemented");

@@ -40,6 +40,10 @@ struct KArgs {
   double sgz_integrale;
   double *sgz_mean, *sgz_acc;
   const double *sgz_kr;
+  // DAMPTOVISCOUS: the 1-D viscous disc (n1d cells), its boundary rule
+  int n1d, bc1d, damp_visc;
+  double *v1d_sigma, *v1d_vrad;
+  const double *v1d_rmed, *v1d_nu;
   int gas_indirect, indirect_term, alt_indirect, exclude_hill, roche, forced_circular;
   int xcd_remap;
   int num_cu; // compute units of the device
@@ -115,7 +119,7 @@ enum KernelId {
   KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
   KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER,
   KID_EVANESCENT, KID_NONREFLECTING,
-  KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL, KID_EOS_STEP, KID_COUNT
+  KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL, KID_EOS_STEP, KID_VISC1D, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 

@@ -53,6 +53,8 @@ static const par_key kKeys[] = {
     {"WKZRMAX", K_REAL, 0, "0.0"}, {"DAMPTOINI", K_STR, 0, "NO"}, {"DAMPTOAXI", K_STR, 0, "YES"},
     {"DAMPTOVISCOUS", K_STR, 0, "NO"}, {"COROTATEWITHOUTERPLANET", K_STR, 0, "NO"},
     {"ADDFLOORS", K_STR, 0, "YES"}, {"DENSITYFLOOR", K_REAL, 0, "1e-9"}, {"ZEROVRADBC", K_STR, 0, "NO"},
+    {"NRAD1D", K_INT, 0, "100"}, {"RMIN1D", K_REAL, 0, "0.01"}, {"RMAX1D", K_REAL, 0, "100.0"},
+    {"BOUNDARY1DGRID", K_STR, 0, "T"},
     /* switches of physics not implemented here (refused when on) */
     {"OUTERSOURCEMASS", K_STR, 0, "NO"}, {"ZMPLUS", K_STR, 0, "NO"}, {"ENTROPYDIFFUSION", K_STR, 0, "NO"},
     {"RADIATIVEDIFFUSION", K_STR, 0, "NO"}, {"THERMALCOOLING", K_STR, 0, "NO"},
@@ -207,6 +209,10 @@ int dfargo_read_par(const char *path, dfargo_params *p, dfargo_run_params *run) 
   if (C(v, "DAMPTOAXI") == 'N') p->damp_to = DFARGO_DAMP_TO_INI;
   if (C(v, "DAMPTOVISCOUS") == 'Y') p->damp_to = DFARGO_DAMP_TO_VISCOUS;
   p->zero_vrad_bc = C(v, "ZEROVRADBC") == 'Y';
+  p->nrad1d = I(v, "NRAD1D");
+  p->rmin1d = R(v, "RMIN1D");
+  p->rmax1d = R(v, "RMAX1D");
+  p->bc1d = C(v, "BOUNDARY1DGRID") == 'Z' ? 1 : C(v, "BOUNDARY1DGRID") == 'D' ? 2 : 0;
   if (run) {
     run->dt = R(v, "DT");
     run->ntot = I(v, "NTOT");

@@ -17,7 +17,7 @@ const char *kKernelNames[KID_COUNT] = {
     "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
     "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
     "dust_sort",     "dust_deposit", "dust_reorder", "evanescent", "nonreflecting",
-    "sgz_mean",      "sgz_conv",     "sgz_fill",     "eos_step"};
+    "sgz_mean",      "sgz_conv",     "sgz_fill",     "eos_step",      "visc1d"};
 
 // ------------------------------------------------------------------------
 // helpers
@@ -1802,6 +1802,65 @@ template <typename T> __global__ void __launch_bounds__(256) k_finish(KArgs a) {
 // vtheta.  One 256-thread workgroup per ring of the zones; each thread
 // reads and rewrites the same cells, so the update is in place.
 enum { EV_TB = 256 };
+
+// SolveOneDViscousEvolution (src/SideEuler.c:1518-1565): one explicit step
+// of the 1-D viscous disc the DAMPTOVISCOUS zones relax towards -- the
+// diffusion dSigma/dt = 3[(nu Sigma)'' + 3/(2R)(nu Sigma)'] on the
+// non-uniform 1-D grid, its radial velocity, then the boundary rule.  One
+// workgroup; the profile is staged in LDS, updated there, written back.
+enum { V1D_MAX = 1024, V1D_TB = 256 };
+__global__ void __launch_bounds__(V1D_TB) k_visc1d(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  __shared__ double sS[V1D_MAX], sV[V1D_MAX], sN[V1D_MAX], sR[V1D_MAX], sT[V1D_MAX];
+  const int n = a.n1d, tid = threadIdx.x;
+  const double dt = c->dt;
+  for (int i = tid; i < n; i += V1D_TB) {
+    sS[i] = a.v1d_sigma[i];
+    sV[i] = a.v1d_vrad[i];
+    sN[i] = a.v1d_nu[i];
+    sR[i] = a.v1d_rmed[i];
+  }
+  __syncthreads();
+  for (int i = 1 + tid; i <= n - 2; i += V1D_TB) {
+    const double drm = sR[i] - sR[i - 1], drp = sR[i + 1] - sR[i];
+    const double drnu = (drm * drm * sN[i + 1] - drp * drp * sN[i - 1] + (drp * drp - drm * drm) * sN[i]) /
+                        (drm * drm * drp + drp * drp * drm);
+    const double drsigma = (drm * drm * sS[i + 1] - drp * drp * sS[i - 1] + (drp * drp - drm * drm) * sS[i]) /
+                           (drm * drm * drp + drp * drp * drm);
+    const double d2rnu = (drm * sN[i + 1] + drp * sN[i - 1] - (drm + drp) * sN[i]) /
+                         (0.5 * drm * drp * drp + 0.5 * drp * drm * drm);
+    const double d2rsigma = (drm * sS[i + 1] + drp * sS[i - 1] - (drm + drp) * sS[i]) /
+                            (0.5 * drm * drp * drp + 0.5 * drp * drm * drm);
+    const double term1 = 3.0 * sS[i] * (1.5 * drnu / sR[i] + d2rnu);
+    const double term2 = 3.0 * drsigma * (1.5 * sN[i] / sR[i] + 2.0 * drnu);
+    const double term3 = 3.0 * sN[i] * d2rsigma;
+    sT[i] = term1 + term2 + term3;
+    sV[i] = -1.5 * sN[i] / sR[i] - 3.0 * drnu - 1.5 * (sN[i] / sS[i]) * drsigma;
+  }
+  __syncthreads();
+  for (int i = 1 + tid; i <= n - 2; i += V1D_TB) sS[i] += dt * sT[i];
+  __syncthreads();
+  if (tid == 0) {
+    if (a.bc1d == 1) {
+      sS[0] = sS[1] * (sN[1] / sN[0]) * sqrt(sR[1] / sR[0]);
+      sS[n - 1] = sS[n - 2] * (sN[n - 2] / sN[n - 1]) * sqrt(sR[n - 2] / sR[n - 1]);
+    } else if (a.bc1d == 2) {
+      sS[0] = sS[1] * (sN[1] / sN[0]);
+      sS[n - 1] = sS[n - 2] * (sN[n - 2] / sN[n - 1]);
+    } else {
+      sS[0] = 1e-9;
+      sS[n - 1] = 1e-9;
+    }
+    sV[0] = sV[1];
+    sV[n - 1] = sV[n - 2];
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += V1D_TB) {
+    a.v1d_sigma[i] = sS[i];
+    a.v1d_vrad[i] = sV[i];
+  }
+}
 template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArgs a) {
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
@@ -1815,7 +1874,36 @@ template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArg
   const double Tz = a.damp_ratio * pow(R, 1.5);
   const double lambda = damping * damping * c->dt / Tz;
   double q0[4];
-  if (a.damp_ini) {
+  if (a.damp_visc) {
+    // the ring means of vtheta and e, Sigma and vr interpolated linearly in
+    // the 1-D disc (src/SideEuler.c:868-887)
+    __shared__ double partv[EV_TB / 64][2];
+    double s[2] = {0.0, 0.0};
+    for (int j = tid; j < ns; j += EV_TB) {
+      s[0] += ld<T>(a.vt, base + j);
+      if (a.energy) s[1] += ld<T>(a.en, base + j);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) s[k] = wave_sum(s[k]);
+    if ((tid & 63) == 0) {
+      partv[tid >> 6][0] = s[0];
+      partv[tid >> 6][1] = s[1];
+    }
+    __syncthreads();
+    double sv = 0.0, se = 0.0;
+    for (int w = 0; w < EV_TB / 64; w++) {
+      sv += partv[w][0];
+      se += partv[w][1];
+    }
+    q0[1] = sv / (double)ns;
+    q0[3] = se / (double)ns;
+    int m = 0;
+    while (a.v1d_rmed[m] < R) m++;
+    m--;
+    const double r0 = a.v1d_rmed[m], r1 = a.v1d_rmed[m + 1];
+    q0[2] = ((r1 - R) * a.v1d_sigma[m] + (R - r0) * a.v1d_sigma[m + 1]) / (r1 - r0);
+    q0[0] = ((r1 - R) * a.v1d_vrad[m] + (R - r0) * a.v1d_vrad[m + 1]) / (r1 - r0);
+  } else if (a.damp_ini) {
     q0[0] = a.VradMed[i];
     q0[1] = a.VthetaMed[i] - R * c->omega_frame;
     q0[2] = a.SigmaMed[i];
@@ -2090,6 +2178,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       else hipLaunchKernelGGL((k_control<T, false>), dim3(1), dim3(CtrlBlock<false>::n), 0, s, a);
       break;
     case KID_EOS_STEP: RING_DISPATCH(k_eos_step, T, a, s, 0); break;
+    case KID_VISC1D: hipLaunchKernelGGL(k_visc1d, dim3(1), dim3(V1D_TB), 0, s, a); break;
     case KID_NONREFLECTING:
       hipLaunchKernelGGL(k_nonreflecting<T>, dim3(2), dim3(NR_TB), 0, s, a);
       break;

@@ -55,8 +55,9 @@ enum dfargo_frame {
 enum dfargo_damp {
   DFARGO_DAMP_TO_AXI = 0,      /* the ring's instantaneous mean (default)  */
   DFARGO_DAMP_TO_INI = 1,      /* the initial profiles (DAMPTOINI)         */
-  DFARGO_DAMP_TO_VISCOUS = 2   /* a 1-D viscous evolution (DAMPTOVISCOUS;
-                                  rejected: not implemented)               */
+  DFARGO_DAMP_TO_VISCOUS = 2   /* a 1-D viscous disc evolved beside the
+                                  grid (DAMPTOVISCOUS,
+                                  src/SideEuler.c:1444-1565)               */
 };
 
 /* Parameter block: the .par keys the hydro step reads (src/var.c:15-202),
@@ -119,6 +120,12 @@ typedef struct dfargo_params {
   /* self-gravity (continued) */
   int    sg_zero_mode;               /* SELFGRAVITY Z: axisymmetric part
                                         only (src/sgzero.c)                 */
+  /* the 1-D viscous disc of DAMPTOVISCOUS */
+  int    nrad1d;                     /* NRAD1D (100, at most 1024)          */
+  double rmin1d, rmax1d;             /* RMIN1D, RMAX1D (0.01, 100)          */
+  int    bc1d;                       /* BOUNDARY1DGRID: 0 low density (T),
+                                        1 zero-velocity steady state (Z),
+                                        2 steady state vr = -3/2 nu/R (D)   */
 } dfargo_params;
 
 /* PlanetarySystem (src/types.h:97-111) plus the two mass tables that

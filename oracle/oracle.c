@@ -44,7 +44,11 @@
  *                   pitch radii, the adiabatic density/energy relation
  *   SG zero mode    (or_sg_zero_mode) equals the full field's ring values of
  *                   an axisymmetric disc up to the two self-force terms
- *   (tests/test_oracle_frames.py for the last five)
+ *   1-D disc        (solve_1d_viscous) dSigma/dt of power laws under a
+ *                   constant viscosity in closed form, vr = -3/2 nu/R for a
+ *                   uniform disc; its Sigma' coefficient in vr (1.5) is
+ *                   parity unpinned beyond the reference's own formula
+ *   (tests/test_oracle_frames.py for the last six)
  */
 #include <math.h>
 #include <stdio.h>
@@ -91,6 +95,9 @@ struct or_ctx {
   double PhysicalTime, PhysicalTimeInitial, OmegaFrame, MINDT, MAXDT, LostMass;
   double MassTaper;
   double Xplanet, Yplanet; /* GetPsysInfo's mark (src/Psys.c:334-344) */
+  /* the 1-D viscous disc of DAMPTOVISCOUS (src/SideEuler.c:1444-1565) */
+  int n1d;
+  double *rmed1d, *sigma1d, *vrad1d, *nu1d, *t1, *t2, *t3;
   pair_t dop, dop_alt, indirect;
   double barx[3], bary[3];
   int GasTimeStepsCFL, last_substeps, crashed_d, crashed_e, crashed;
@@ -272,6 +279,8 @@ or_ctx *or_create(const dfargo_params *p) {
 
 void or_destroy(or_ctx *c) {
   if (!c) return;
+  free(c->rmed1d); free(c->sigma1d); free(c->vrad1d); free(c->nu1d);
+  free(c->t1); free(c->t2); free(c->t3);
   double **all[] = {&c->Radii, &c->Rinf, &c->Rsup, &c->Rmed, &c->Surf, &c->InvRmed,
                     &c->InvSurf, &c->InvDiffRmed, &c->InvDiffRsup, &c->InvRinf,
                     &c->GlobalRmed, &c->Azimuth, &c->CosAz, &c->SinAz, &c->SigmaMed,
@@ -924,6 +933,79 @@ static void accrete_onto_planets(or_ctx *c, double dt) {
 }
 
 void or_accrete_onto_planets(or_ctx *c, double dt) { accrete_onto_planets(c, dt); }
+
+/* InitializeOneDViscousEvolution (src/SideEuler.c:1444-1487), without
+ * the restart read and the dims1D/used_rad1D files */
+static void init_1d_viscous(or_ctx *c) {
+  const dfargo_params *p = &c->p;
+  const int n = p->nrad1d;
+  double *radii = (double *)calloc(n + 1, sizeof(double));
+  c->n1d = n;
+  c->rmed1d = (double *)calloc(n, sizeof(double));
+  c->sigma1d = (double *)calloc(n, sizeof(double));
+  c->vrad1d = (double *)calloc(n, sizeof(double));
+  c->nu1d = (double *)calloc(n, sizeof(double));
+  c->t1 = (double *)calloc(n, sizeof(double));
+  c->t2 = (double *)calloc(n, sizeof(double));
+  c->t3 = (double *)calloc(n, sizeof(double));
+  for (int i = 0; i <= n; i++) radii[i] = p->rmin1d * exp((double)i / (double)n * log(p->rmax1d / p->rmin1d));
+  for (int i = 0; i < n; i++) {
+    c->rmed1d[i] = 2.0 / 3.0 * (radii[i + 1] * radii[i + 1] * radii[i + 1] - radii[i] * radii[i] * radii[i]);
+    c->rmed1d[i] = c->rmed1d[i] / (radii[i + 1] * radii[i + 1] - radii[i] * radii[i]);
+    c->sigma1d[i] = sigma_profile(p, c->rmed1d[i]);
+    const double cs = aspect_ratio(p, c->rmed1d[i]) / sqrt(c->rmed1d[i]) * pow(c->rmed1d[i], p->flaring_index);
+    if (p->viscosity != 0.0) c->nu1d[i] = p->viscosity;
+    if (p->alpha_viscosity != 0.0) {
+      c->nu1d[i] = p->alpha_viscosity * cs * cs * pow(c->rmed1d[i], 1.5);
+      c->vrad1d[i] = -3.0 * c->nu1d[i] / c->rmed1d[i] * (-p->sigma_slope + 2.0 * p->flaring_index + 1.0);
+    } else
+      c->vrad1d[i] = -3.0 * c->nu1d[i] / c->rmed1d[i] * (-p->sigma_slope + .5);
+  }
+  free(radii);
+}
+
+/* SolveOneDViscousEvolution (src/SideEuler.c:1518-1565) */
+static void solve_1d_viscous(or_ctx *c, double dt) {
+  const int n = c->n1d;
+  double *R = c->rmed1d, *S = c->sigma1d, *V = c->vrad1d, *nu = c->nu1d;
+  for (int i = 1; i <= n - 2; i++) {
+    const double drm = R[i] - R[i - 1], drp = R[i + 1] - R[i];
+    const double drnu = (drm * drm * nu[i + 1] - drp * drp * nu[i - 1] + (drp * drp - drm * drm) * nu[i]) /
+                        (drm * drm * drp + drp * drp * drm);
+    const double drsigma = (drm * drm * S[i + 1] - drp * drp * S[i - 1] + (drp * drp - drm * drm) * S[i]) /
+                           (drm * drm * drp + drp * drp * drm);
+    const double d2rnu = (drm * nu[i + 1] + drp * nu[i - 1] - (drm + drp) * nu[i]) /
+                         (0.5 * drm * drp * drp + 0.5 * drp * drm * drm);
+    const double d2rsigma = (drm * S[i + 1] + drp * S[i - 1] - (drm + drp) * S[i]) /
+                            (0.5 * drm * drp * drp + 0.5 * drp * drm * drm);
+    c->t1[i] = 3.0 * S[i] * (1.5 * drnu / R[i] + d2rnu);
+    c->t2[i] = 3.0 * drsigma * (1.5 * nu[i] / R[i] + 2.0 * drnu);
+    c->t3[i] = 3.0 * nu[i] * d2rsigma;
+    V[i] = -1.5 * nu[i] / R[i] - 3.0 * drnu - 1.5 * (nu[i] / S[i]) * drsigma;
+  }
+  for (int i = 1; i <= n - 2; i++) S[i] += dt * (c->t1[i] + c->t2[i] + c->t3[i]);
+  if (c->p.bc1d == 1) {
+    S[0] = S[1] * (nu[1] / nu[0]) * sqrt(R[1] / R[0]);
+    S[n - 1] = S[n - 2] * (nu[n - 2] / nu[n - 1]) * sqrt(R[n - 2] / R[n - 1]);
+  }
+  if (c->p.bc1d == 2) {
+    S[0] = S[1] * (nu[1] / nu[0]);
+    S[n - 1] = S[n - 2] * (nu[n - 2] / nu[n - 1]);
+  }
+  if (c->p.bc1d == 0) {
+    S[0] = 1e-9;
+    S[n - 1] = 1e-9;
+  }
+  V[0] = V[1];
+  V[n - 1] = V[n - 2];
+}
+
+void or_solve_1d_viscous(or_ctx *c, double dt, double *sigma, double *vrad) {
+  if (!c->n1d) init_1d_viscous(c);
+  if (dt > 0.0) solve_1d_viscous(c, dt);
+  memcpy(sigma, c->sigma1d, sizeof(double) * c->n1d);
+  memcpy(vrad, c->vrad1d, sizeof(double) * c->n1d);
+}
 
 /* CorrectVtheta (src/SideEuler.c:1133-1148) */
 static void correct_vtheta(or_ctx *c, double domega) {
@@ -1922,7 +2004,22 @@ static void evanescent_boundary(or_ctx *c, double step) {
       lambda = damping * damping * step / Tout;
     }
     double vr0, vt0, d0, e0;
-    if (p->damp_to == DFARGO_DAMP_TO_INI) {
+    if (p->damp_to == DFARGO_DAMP_TO_VISCOUS) {
+      vt0 = e0 = 0.0;
+      for (int j = 0; j < ns; j++) {
+        vt0 += c->vt[i * ns + j];
+        e0 += c->en[i * ns + j];
+      }
+      vt0 /= (double)ns;
+      e0 /= (double)ns;
+      int m = 0;
+      while (c->rmed1d[m] < R) m++;
+      m--;
+      d0 = (c->rmed1d[m + 1] - R) * c->sigma1d[m] + (R - c->rmed1d[m]) * c->sigma1d[m + 1];
+      d0 /= (c->rmed1d[m + 1] - c->rmed1d[m]);
+      vr0 = (c->rmed1d[m + 1] - R) * c->vrad1d[m] + (R - c->rmed1d[m]) * c->vrad1d[m + 1];
+      vr0 /= (c->rmed1d[m + 1] - c->rmed1d[m]);
+    } else if (p->damp_to == DFARGO_DAMP_TO_INI) {
       vt0 = c->VthetaMed[i] - R * c->OmegaFrame;
       vr0 = c->VradMed[i];
       d0 = c->SigmaMed[i];
@@ -2500,6 +2597,11 @@ int or_algogas(or_ctx *c, double DT) {
     }
     rotate_psys(c, c->OmegaFrame * dt);
     if (c->dust_on && c->np != 0) dust_rotate(c, c->OmegaFrame * dt);
+    /* the 1-D viscous disc (src/SourceEuler.c:540-541) */
+    if (p->damp_to == DFARGO_DAMP_TO_VISCOUS) {
+      if (!c->n1d) init_1d_viscous(c);
+      solve_1d_viscous(c, dt);
+    }
     /* crash scan (src/SourceEuler.c:545-548) */
     c->crashed_d = detect_crash(c, c->rho);
     c->crashed_e = detect_crash(c, c->en);

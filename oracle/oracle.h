@@ -55,6 +55,9 @@ void or_nonreflecting_boundary(or_ctx *c);
 void or_accrete_onto_planets(or_ctx *c, double dt);
 /* compute_SGZeroMode: SG_Accr per ring (nrad values) of the density rho */
 void or_sg_zero_mode(or_ctx *c, const double *rho, double *accr);
+/* the DAMPTOVISCOUS 1-D disc: one step of dt (none for dt <= 0), then its
+ * Sigma and vr (nrad1d values each) */
+void or_solve_1d_viscous(or_ctx *c, double dt, double *sigma, double *vrad);
 
 /* AlgoGas; returns the number of sub-steps taken */
 int or_algogas(or_ctx *c, double DT);

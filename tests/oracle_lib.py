@@ -44,6 +44,7 @@ def lib():
             "or_nonreflecting_boundary": (None, [vp]),
             "or_accrete_onto_planets": (None, [vp, ctypes.c_double]),
             "or_sg_zero_mode": (None, [vp, dp, dp]),
+            "or_solve_1d_viscous": (None, [vp, ctypes.c_double, dp, dp]),
             "or_algogas": (ctypes.c_int, [vp, ctypes.c_double]),
             "or_get_status": (None, [vp, ctypes.POINTER(Status)]),
             "or_get_field": (None, [vp, ctypes.c_int, dp]),
@@ -168,6 +169,11 @@ class Oracle:
 
     def accrete(self, dt):
         self.L.or_accrete_onto_planets(self.ctx, dt)
+
+    def viscous_1d(self, dt):
+        s, v = np.zeros(self.p.nrad1d), np.zeros(self.p.nrad1d)
+        self.L.or_solve_1d_viscous(self.ctx, dt, ptr(s), ptr(v))
+        return s, v
 
     def sg_zero_mode(self, rho):
         out = np.zeros(self.nr)

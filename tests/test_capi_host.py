@@ -121,7 +121,9 @@ def test_parameter_checks(lib):
     assert check(wave_killing_zones=1) == _capi.ERR_ARG
     assert check(wave_killing_zones=1, wkz_rmin=0.5, wkz_rmax=0.4) == _capi.ERR_ARG
     assert check(wave_killing_zones=1, wkz_rmin=0.5, wkz_rmax=2.2,
-                 damp_to=_capi.DAMP_TO_VISCOUS) == _capi.ERR_UNSUPPORTED
+                 damp_to=_capi.DAMP_TO_VISCOUS) == _capi.ERR_ARG  # needs a viscosity
+    assert check(wave_killing_zones=1, wkz_rmin=0.5, wkz_rmax=2.2, viscosity=1e-5,
+                 damp_to=_capi.DAMP_TO_VISCOUS) == _capi.OK
     assert check(wave_killing_zones=1, wkz_rmin=0.5, wkz_rmax=2.2, damp_to=_capi.DAMP_TO_INI) == _capi.OK
     assert check(self_gravity=1, log_grid=1, nsec=40) == _capi.ERR_UNSUPPORTED
     assert b"VISCOSITY" in (lib.dfargo_check_params(ctypes.byref(_capi.default_params(

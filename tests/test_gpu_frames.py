@@ -67,11 +67,18 @@ def _wkz_case(name):
         return gas_planet(nrad=32, nsec=96, damp_to=_capi.DAMP_TO_INI, **zones)
     if name == "zero_vrad":
         return gas_planet(nrad=32, nsec=96, zero_vrad_bc=1, **zones)
+    if name == "viscous":
+        # DAMPTOVISCOUS: the 1-D viscous disc beside the grid, the boundary
+        # rule of BOUNDARYGRID1D Z
+        return gas_planet(nrad=32, nsec=96, damp_to=_capi.DAMP_TO_VISCOUS, bc1d=1, **zones)
+    if name == "viscous_alpha":
+        return energy_sg(nrad=32, nsec=96, damp_to=_capi.DAMP_TO_VISCOUS, wave_killing_zones=1,
+                         wkz_rmin=0.55, wkz_rmax=3.5)
     # the energy field is damped too; the K = 6 ring width of the large discs
     return energy_sg(nrad=16, nsec=1536, wave_killing_zones=1, wkz_rmin=0.55, wkz_rmax=3.5)
 
 
-@pytest.mark.parametrize("name", ["axi", "ini", "zero_vrad", "energy_sg"])
+@pytest.mark.parametrize("name", ["axi", "ini", "zero_vrad", "energy_sg", "viscous", "viscous_alpha"])
 def test_wave_killing_zones_fp64(name):
     p, pl = _wkz_case(name)
     o, so, d, sg = pair(p, pl)

@@ -395,3 +395,31 @@ def test_nonreflecting_adiabatic_edges(oracle_built):
     s_rho, _ = _best_shift(ring(st, "rho", nr - 2), ring(af, "rho", nr - 1))
     assert s_rho == s_out
     assert abs(ring(af, "rho", nr - 1).mean() / sig(nr - 1) - 1) < 1e-13
+
+
+def test_viscous_1d_disc(oracle_built):
+    """DAMPTOVISCOUS's 1-D disc (src/SideEuler.c:1518-1565): with a constant
+    viscosity, Sigma = R^p evolves as dSigma/dt = 3 nu (p^2 + p/2) R^(p-2)
+    (so the steady Sigma ~ R^-1/2 stays put), to O(dR^2) on its log grid;
+    for a uniform Sigma the radial velocity is -3/2 nu / R.  (Its Sigma'
+    term carries 1.5 where the textbook drift has 3; only the reference's
+    own formula fixes that coefficient.)  A wrong diffusion coefficient or
+    curvature term fails."""
+    nu = 1e-5
+    for slope in (0.5, -1.0, 0.0):
+        p, pl = gas_planet(nrad=16, nsec=48, viscosity=nu, sigma_slope=slope, sigma0=1.0,
+                           damp_to=_capi.DAMP_TO_VISCOUS, nrad1d=400, rmin1d=0.1, rmax1d=10.0)
+        o = Oracle(p)
+        s0, _ = o.viscous_1d(0.0)
+        dt = 1.0
+        s1, v1 = o.viscous_1d(dt)
+        n = p.nrad1d
+        radii = p.rmin1d * np.exp(np.arange(n + 1) / n * math.log(p.rmax1d / p.rmin1d))
+        r = 2 / 3 * (radii[1:] ** 3 - radii[:-1] ** 3) / (radii[1:] ** 2 - radii[:-1] ** 2)
+        q = -slope
+        rate = (s1 - s0)[5:-5] / dt
+        want = 3 * nu * (q * q + q / 2) * r[5:-5] ** (q - 2)
+        scale = 3 * nu * r[5:-5] ** (q - 2)
+        assert np.abs(rate - want).max() <= 1e-3 * scale.max(), slope
+        if slope == 0.0:
+            assert np.abs(v1[5:-5] + 1.5 * nu / r[5:-5]).max() <= 1e-12 * nu
