@@ -163,6 +163,7 @@ def precision_error(params, planets, orbits, control=True):
         qa, qb = a["planets"], b["planets"]
         if qb.nb > 0:
             out["planet_position"] = max(max(abs(qa.x[k] - qb.x[k]), abs(qa.y[k] - qb.y[k])) for k in range(qb.nb))
+            out["planet_mass"] = max(abs(qa.mass[k] / qb.mass[k] - 1.0) for k in range(qb.nb))
         return out
 
     ref = run(_capi.FP64, False)

@@ -54,6 +54,9 @@ struct dfargo_ctx {
   int dust_on = 0;
   dfargo_dust_params dp;
   std::vector<double> dustsize_init;
+  // DAMPTOVISCOUS: the 1-D viscous disc's initial Sigma and vr, uploaded
+  // again by every InitEuler
+  std::vector<double> v1d_sigma0, v1d_vrad0;
 };
 
 // ------------------------------------------------------------------------
@@ -338,10 +341,14 @@ static int build_context(dfargo_ctx *c) {
       } else
         vr[i] = -3.0 * nu[i] / rmed[i] * (-p.sigma_slope + 0.5);
     }
-    // the damping interpolates at every zone ring's Rmed
-    for (int i = 0; i < nr; i++)
-      if (!(rmed[0] < t.Rmed[i] && t.Rmed[i] <= rmed[n - 1]))
-        return dfargo_set_error(DFARGO_ERR_ARG, "the 1-D grid's cell centres must enclose every Rmed");
+    // the damping interpolates the 1-D disc at every zone ring's Rmed
+    // (src/SideEuler.c:868-887); without the zones it only evolves
+    if (p.wave_killing_zones)
+      for (int i = 0; i < nr; i++)
+        if (!(rmed[0] < t.Rmed[i] && t.Rmed[i] <= rmed[n - 1]))
+          return dfargo_set_error(DFARGO_ERR_ARG, "the 1-D grid's cell centres must enclose every Rmed");
+    c->v1d_sigma0 = sig;
+    c->v1d_vrad0 = vr;
     a.n1d = n;
     a.bc1d = p.bc1d;
     if ((rc = upload_table(c, &a.v1d_rmed, rmed.data(), n)) || (rc = upload_table(c, &a.v1d_nu, nu.data(), n)) ||
@@ -735,6 +742,12 @@ extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad,
   }
   HIPCHK(hipMemcpy((void *)a.VthetaMed, c->VthetaMed.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy((void *)a.VradMed, c->VradMed.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
+  // the 1-D viscous disc restarts from its initial profiles
+  // (InitializeOneDViscousEvolution, src/SideEuler.c:1444-1487)
+  if (a.damp_visc) {
+    HIPCHK(hipMemcpy(a.v1d_sigma, c->v1d_sigma0.data(), sizeof(double) * a.n1d, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(a.v1d_vrad, c->v1d_vrad0.data(), sizeof(double) * a.n1d, hipMemcpyHostToDevice));
+  }
   memcpy(vrad, vr.data(), sizeof(double) * n);
   memcpy(vtheta, vt.data(), sizeof(double) * n);
   if ((rc = put_field(c, a.vr, vr.data(), n)) || (rc = put_field(c, a.vt, vt.data(), n))) return rc;
@@ -884,6 +897,12 @@ extern "C" int dfargo_get_status(dfargo_ctx *c, dfargo_status *st) {
   st->crashed_density = k->crashed_d;
   st->crashed_energy = k->crashed_e;
   st->crashed = k->crashed_e;
+  {
+    const unsigned long long b = k->fargo_margin_bits;
+    double m;
+    memcpy(&m, &b, sizeof m);
+    st->fargo_margin = m;
+  }
   return DFARGO_OK;
 }
 
@@ -932,18 +951,11 @@ extern "C" int dfargo_gas_budgets(dfargo_ctx *c, double *out) {
   return DFARGO_OK;
 }
 
-extern "C" int dfargo_update_log(dfargo_ctx *c, int max_planets, double *torques, double *indirect, int *nb) {
-  if (!c || !torques || !indirect || !nb) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
-  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
-  HIPCHK(hipSetDevice(c->dev));
-  int rc = ctrl_pull(c);
-  if (rc) return rc;
+// UpdateLog's bodies: the planets of the control block, with UpdateLog's
+// smoothing and ComputeForce's Hill radius (src/Force.c:287-292, 118-119)
+// on the host, in FP64
+static void log_bodies(const dfargo_ctx *c, DiagBodies &b) {
   const dfargo_ctrl *k = c->h_ctrl;
-  if (k->nb > max_planets)
-    return dfargo_set_error(DFARGO_ERR_ARG, "%d planets, room for %d", k->nb, max_planets);
-  // UpdateLog's smoothing and ComputeForce's Hill radius (src/Force.c:
-  // 287-292, 118-119) on the host, in FP64
-  DiagBodies b;
   memset(&b, 0, sizeof b);
   b.n = k->nb;
   for (int q = 0; q < k->nb; q++) {
@@ -956,11 +968,15 @@ extern "C" int dfargo_update_log(dfargo_ctx *c, int max_planets, double *torques
                    : c->p.thickness_smoothing * dfargo_aspect_ratio(&c->p, r) * pow(r, 1.0 + c->p.flaring_index);
     b.rh[q] = pow(m / 3., 1. / 3.) * b.a[q] + 1e-15;
   }
-  std::vector<double> r;
-  if ((rc = run_diagnostics(c, b, r))) return rc;
+}
+
+// the tqwk columns and indtq of each planet (src/Force.c:293-325) from the
+// diagnostics' sums r
+static void log_torques(const dfargo_ctx *c, const double *r, double *torques, double *indirect) {
+  const dfargo_ctrl *k = c->h_ctrl;
   for (int q = 0; q < k->nb; q++) {
     const double x = k->pl_x[q], y = k->pl_y[q], vx = k->pl_vx[q], vy = k->pl_vy[q];
-    const double *f = r.data() + 3 + 8 * q; // fx i, fx ex i, fx o, fx ex o, fy i, fy ex i, fy o, fy ex o
+    const double *f = r + 3 + 8 * q; // fx i, fx ex i, fx o, fx ex o, fy i, fy ex i, fy o, fy ex o
     double *o = torques + 8 * q;
     o[0] = x * f[4] - y * f[0];
     o[1] = x * f[6] - y * f[2];
@@ -972,6 +988,22 @@ extern "C" int dfargo_update_log(dfargo_ctx *c, int max_planets, double *torques
     o[7] = vx * f[3] + vy * f[7];
     indirect[q] = -x * k->dop_y + y * k->dop_x;
   }
+}
+
+extern "C" int dfargo_update_log(dfargo_ctx *c, int max_planets, double *torques, double *indirect, int *nb) {
+  if (!c || !torques || !indirect || !nb) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
+  HIPCHK(hipSetDevice(c->dev));
+  int rc = ctrl_pull(c);
+  if (rc) return rc;
+  const dfargo_ctrl *k = c->h_ctrl;
+  if (k->nb > max_planets)
+    return dfargo_set_error(DFARGO_ERR_ARG, "%d planets, room for %d", k->nb, max_planets);
+  DiagBodies b;
+  log_bodies(c, b);
+  std::vector<double> r;
+  if ((rc = run_diagnostics(c, b, r))) return rc;
+  log_torques(c, r.data(), torques, indirect);
   *nb = k->nb;
   return DFARGO_OK;
 }
@@ -1084,6 +1116,17 @@ extern "C" int dfargo_get_fargo_shift(dfargo_ctx *c, long *nshift, double *vmed)
   HIPCHK(hipSetDevice(c->dev));
   if (nshift) HIPCHK(hipMemcpy(nshift, c->a.nshift, sizeof(long) * c->p.nrad, hipMemcpyDeviceToHost));
   if (vmed) HIPCHK(hipMemcpy(vmed, c->a.vmed, sizeof(double) * c->p.nrad, hipMemcpyDeviceToHost));
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_get_visc1d(dfargo_ctx *c, double *sigma, double *vrad, int *n1d) {
+  if (!c || !n1d) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  *n1d = c->a.damp_visc ? c->a.n1d : 0;
+  if (!c->a.damp_visc) return DFARGO_OK;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (sigma) HIPCHK(hipMemcpy(sigma, c->a.v1d_sigma, sizeof(double) * c->a.n1d, hipMemcpyDeviceToHost));
+  if (vrad) HIPCHK(hipMemcpy(vrad, c->a.v1d_vrad, sizeof(double) * c->a.n1d, hipMemcpyDeviceToHost));
   return DFARGO_OK;
 }
 

@@ -82,6 +82,9 @@ typedef struct dfargo_ctrl {
   /* corotating frame: the change of OmegaFrame this sub-step
      (CorrectVtheta's domega) and GetPsysInfo's mark */
   double domega, mark_x, mark_y;
+  /* the smallest 1/2 - |Ntilde - Nshift| of this AlgoGas call, as the bits
+     of a non-negative double (they order as unsigned integers) */
+  unsigned long long fargo_margin_bits;
 } dfargo_ctrl;
 
 #endif

@@ -70,14 +70,17 @@ int dfargo_check_params(const dfargo_params *p) {
     return dfargo_set_error(DFARGO_ERR_ARG, "unknown damping target %d", p->damp_to);
   if (p->damp_to == DFARGO_DAMP_TO_VISCOUS) {
     /* the 1-D disc's viscosity is set only by VISCOSITY or ALPHAVISCOSITY
-       (src/SideEuler.c:1477-1480), and its grid must cover the 2-D one for
-       the interpolation of src/SideEuler.c:879-886 */
+       (src/SideEuler.c:1477-1480); with the wave-killing zones its grid
+       must cover the 2-D one for the interpolation of
+       src/SideEuler.c:879-886, without them it only evolves */
     if (p->viscosity == 0.0 && p->alpha_viscosity == 0.0)
       return dfargo_set_error(DFARGO_ERR_ARG, "DAMPTOVISCOUS needs VISCOSITY or ALPHAVISCOSITY");
     if (p->nrad1d < 3 || p->nrad1d > 1024)
       return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "NRAD1D must lie in [3, 1024], got %d", p->nrad1d);
-    if (!(p->rmin1d > 0.0) || !(p->rmin1d < p->rmin) || !(p->rmax1d > p->rmax))
-      return dfargo_set_error(DFARGO_ERR_ARG, "RMIN1D/RMAX1D must enclose [RMIN, RMAX]");
+    if (!(p->rmin1d > 0.0) || !(p->rmax1d > p->rmin1d))
+      return dfargo_set_error(DFARGO_ERR_ARG, "need 0 < RMIN1D < RMAX1D");
+    if (p->wave_killing_zones && (!(p->rmin1d < p->rmin) || !(p->rmax1d > p->rmax)))
+      return dfargo_set_error(DFARGO_ERR_ARG, "RMIN1D/RMAX1D must enclose [RMIN, RMAX] under the wave-killing zones");
     if (p->bc1d < 0 || p->bc1d > 2) return dfargo_set_error(DFARGO_ERR_ARG, "unknown BOUNDARY1DGRID %d", p->bc1d);
   }
   if (p->boundary != DFARGO_BC_WALL && p->boundary != DFARGO_BC_OPEN && p->boundary != DFARGO_BC_NONREFLECTING)

@@ -65,7 +65,7 @@ template <typename T> __device__ __forceinline__ void st_sr(void *p, long i, dou
   ((T *)p)[i] = sr_round<T>(v, key);
 }
 // salts of the stochastically rounded stores
-enum { SR_VR = 1, SR_VT, SR_E, SR_EN, SR_RAD = 8, SR_AZI = 16, SR_EVA = 24 };
+enum { SR_VR = 1, SR_VT, SR_E, SR_EN, SR_RAD = 8, SR_AZI = 16, SR_EVA = 24, SR_ACC = 32 };
 
 // a / b from the hardware reciprocal, two Newton steps and one correction
 // of the quotient: within an ulp or two of the IEEE quotient for operands
@@ -411,6 +411,7 @@ __global__ void k_algogas_begin(KArgs a, double DT) {
     c->active = 0;
     c->go = c->dtemp < 0.999999999 * c->DT;
     c->go_next = c->go;
+    c->fargo_margin_bits = 0x3FE0000000000000ULL; // 0.5
     if (c->go) begin_substep(a, c);
   }
 }
@@ -694,9 +695,10 @@ __device__ void accrete_planets(const KArgs &a, dfargo_ctrl *c, double dt, doubl
             v[1] += deltaM * vxcell;
             v[2] += deltaM * vycell;
           }
-          st<T>(a.rho, l, dens);
+          const T dstored = sr_round<T>(dens, sr_key(c, l, SR_ACC));
+          ((T *)a.rho)[l] = dstored;
           // the change of the cell's mass as stored, in the partials' terms
-          dcm = a.Surf[i] * ((double)(T)dens - d0);
+          dcm = a.Surf[i] * ((double)dstored - d0);
           v[3] += dcm;
           v[4] += dcm * xc;
           v[5] += dcm * yc;
@@ -1653,6 +1655,9 @@ template <typename T> __global__ void __launch_bounds__(256) k_transport_azi(KAr
   if (seg == 0 && lane == 0) {
     a.vmed[i] = vmed;
     a.nshift[i] = nshift;
+    // distance of Ntilde from the nearest rounding tie, min over the call
+    const double margin = 0.5 - fabs(Ntilde - Nround);
+    atomicMin(&a.ctrl->fargo_margin_bits, (unsigned long long)__double_as_longlong(margin > 0.0 ? margin : 0.0));
   }
   // mass through the inner edge (VanLeerRadial's LostByDisk, ring 0)
   if (i == 0 && seg == 0) {

@@ -159,6 +159,13 @@ class Disc:
         _check(self.lib, self.lib.dfargo_get_fargo_shift(self.ctx, ns, _ptr(vm)))
         return np.array(ns[:], dtype=np.int64), vm
 
+    def visc1d(self):
+        """DAMPTOVISCOUS's 1-D viscous disc: (Sigma, vr), NRAD1D values each."""
+        n = self.p.nrad1d
+        s, v, got = np.zeros(max(n, 1)), np.zeros(max(n, 1)), ctypes.c_int()
+        _check(self.lib, self.lib.dfargo_get_visc1d(self.ctx, _ptr(s), _ptr(v), ctypes.byref(got)))
+        return s[: got.value], v[: got.value]
+
     def gas_budgets(self):
         """GasTotalMass, GasMomentum, GasTotalEnergy, planets' angular momentum."""
         out = np.zeros(4)

@@ -100,11 +100,21 @@ def dust_particles(params, nbpart, dust_to_gas=0.01, **over):
     return d
 
 
+# Sigma0 (M_sun / AU^2) at which a 1 cm grain of 2.7 g/cc has the Epstein
+# Stokes number pi/2 s rho_int / Sigma = 1 at r = 1, so that 1 um - 1 cm
+# grains span St ~ 1e-4 - 1
+SIGMA0_ST1_CM = 4.77e-7
+
+
 def gas_dust(nrad=1024, nsec=3072, seed=DEFAULT_SEED, **over):
     """Gas plus Lagrangian dust: the gas disc of gas_sg_large without
     self-gravity (alpha 1e-3, one q=1e-3 planet at r=1, h = 0.05) at
-    1024x3072; the particles are given by gas_dust_particles."""
-    return gas_sg_large(nrad=nrad, nsec=nsec, seed=seed, self_gravity=0, **over)
+    1024x3072, with Sigma0 = 4.77e-7 so that the grains of
+    gas_dust_particles span Stokes numbers ~1e-4 - 1; the particles are
+    given by gas_dust_particles."""
+    kw = dict(sigma0=SIGMA0_ST1_CM)
+    kw.update(over)
+    return gas_sg_large(nrad=nrad, nsec=nsec, seed=seed, self_gravity=0, **kw)
 
 
 def gas_dust_particles(params, nbpart=1_000_000, seed=DEFAULT_SEED, **over):

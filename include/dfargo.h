@@ -155,6 +155,12 @@ typedef struct dfargo_status {
   int    crashed;                    /* the reference's Crashed flag (the
                                         energy scan overwrites the density
                                         scan, src/SourceEuler.c:545-548)     */
+  double fargo_margin;               /* the smallest 1/2 - |Ntilde - Nshift|
+                                        over the rings and sub-steps of the
+                                        last AlgoGas call: how far the FARGO
+                                        shift's argument stayed from a
+                                        rounding tie (src/TransportEuler.c:
+                                        310-316); 0.5 before any sub-step   */
 } dfargo_status;
 
 /* Derived grids the host may read back (src/Output.c:234-284). */
@@ -318,6 +324,11 @@ double dfargo_fviscosity(dfargo_ctx *ctx, double r);
 /* The integer FARGO shifts Nshift[i] (src/TransportEuler.c:310-316) and the
  * ring-mean azimuthal velocities VMed[i] of the last sub-step (nrad each). */
 int dfargo_get_fargo_shift(dfargo_ctx *ctx, long *nshift, double *vmed);
+
+/* DAMPTOVISCOUS's 1-D viscous disc (src/SideEuler.c:1444-1565): its Sigma
+ * and vr (n1d values each, NRAD1D) as the last sub-step left them; *n1d is
+ * 0 and nothing is written when the option is off. */
+int dfargo_get_visc1d(dfargo_ctx *ctx, double *sigma, double *vrad, int *n1d);
 
 /* ---- per-DT diagnostics (src/main.c:291-394, between AlgoGas calls) -------- */
 

@@ -67,6 +67,7 @@ typedef struct { double x, y; } pair_t;
 struct or_ctx {
   dfargo_params p;
   int nr, ns;
+  double fargo_margin; /* smallest 1/2 - |Ntilde - Nshift| of the last AlgoGas call */
   size_t gsize; /* doubles per grid, as CreatePolarGrid (src/LowTasks.c:97) */
   /* 1-D tables (src/global.h:29-37) */
   double *Radii, *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed;
@@ -196,6 +197,7 @@ static void make_1d_profile(or_ctx *c, const double *f, double *axi) {
 
 or_ctx *or_create(const dfargo_params *p) {
   or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
+  c->fargo_margin = 0.5;
   int nr = p->nrad, ns = p->nsec, i, j;
   c->p = *p;
   c->nr = nr;
@@ -1718,6 +1720,9 @@ static void one_wind_theta(or_ctx *c, double dt) {
     double Ntilde = c->vmed[i] * c->InvRmed[i] * dt * (double)ns / pm;
     double Nround = floor(Ntilde + 0.5);
     c->nshift[i] = (long)Nround;
+    /* distance from the rounding tie, the smallest of the call */
+    double margin = 0.5 - fabs(Ntilde - Nround);
+    if (margin < c->fargo_margin) c->fargo_margin = margin;
     for (int j = 0; j < ns; j++) c->vt[j + i * ns] = (Ntilde - Nround) * c->Rmed[i] * invdt * dpinvns;
     if (!c->p.fast_transport) {
       c->nosplit[i] = 1;
@@ -2535,6 +2540,7 @@ int or_algogas(or_ctx *c, double DT) {
   size_t n = (size_t)(c->nr + 1) * c->ns;
   if (p->energy_equation) compute_sound_speed(c);
   c->GasTimeStepsCFL = 1;
+  c->fargo_margin = 0.5;
   while (dtemp < 0.999999999 * DT) {
     /* mass taper (src/SourceEuler.c:400-418) */
     if (p->mass_taper > 1e-2) {
@@ -2662,6 +2668,7 @@ void or_get_status(or_ctx *c, dfargo_status *st) {
   st->crashed_density = c->crashed_d;
   st->crashed_energy = c->crashed_e;
   st->crashed = c->crashed;
+  st->fargo_margin = c->fargo_margin;
 }
 
 void or_get_field(or_ctx *c, int field, double *out) {

@@ -227,3 +227,81 @@ def test_deposit_conserves_mass_at_full_size(interp):
     dd = g.field("dust_density")[: p.nrad * p.nsec].reshape(p.nrad, p.nsec)
     total = (dd * surf[:, None]).sum()
     assert abs(total / (inside.sum() * d.particle_mass) - 1.0) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [_capi.FP64, _capi.FP32])
+def test_gas_dust_workload_reduced(prec):
+    """The gas-plus-dust workload itself (gas_dust + gas_dust_particles: alpha
+    1e-3, planet q = 1e-3, grains 1 um - 1 cm with dn/ds ~ s^-3.5, 2.7 g/cc,
+    CIC, feedback) at 64x192 with 2*10^4 particles, against the oracle over
+    two calls of DT = 2e-4: its grains reach St ~ 1e-4, where the minimum
+    stopping time (src/SourceEuler.c:1500-1506) sets the sub-step
+    (~16 per call) and the short-friction-time update
+    (src/DustUpdate.c:323-360) moves them.  Cells are bit-exact in both
+    precisions; FP64 bars as the other particle tests; FP32 storage: the
+    FP32 particle bars of test_dust_fp32_fields."""
+    from dustyfargoadsg_amd.workloads import gas_dust, gas_dust_particles
+    p, pl = gas_dust(nrad=64, nsec=192)
+    d = gas_dust_particles(p, 20000)
+    print(f"case gas_dust 64x192 np=20000 prec={prec}", flush=True)
+    o, g = Oracle(p), Disc(p, prec)
+    o.initial_state(pl)
+    g.initial_state(pl)
+    parts = init_dust_system(p, d)
+    o.dust_attach(d, parts, DT)
+    g.attach_dust(d, parts, DT)
+    for c in range(2):
+        n_o = o.algogas(2e-4)
+        g.algogas(2e-4)
+        assert g.status().last_call_substeps == n_o
+        assert abs(g.status().mindt - o.status().mindt) <= 1e-9 * o.status().mindt
+    if prec == _capi.FP64:
+        so, sg = compare_particles(o, g, 1e-11, 1e-11)
+        go, oo = g.state(), o.state()
+        n = p.nrad * p.nsec
+        for k in ("rho", "vt"):
+            assert relmax(go[k][:n], oo[k][:n]) <= 1e-11, k
+        compare_grids(o, g, p, ("dust_density", "fb_accr", "fb_acct"), 1e-10)
+    else:
+        so, sg = compare_particles(o, g, 1e-6, 1e-5, tol_st=1e-3)
+    assert so["stokesnb"].min() < 1e-3
+
+
+def test_gas_dust_full_size_fp32_properties():
+    """The full workload: 1024x3072 FP32 gas, 10^6 particles (CIC,
+    feedback), 20 sub-steps.  No oracle at this size; instead: every
+    particle and field stays finite, the dust density integrates to the
+    mass of the particles inside the grid (CIC weights sum to one), the
+    Stokes numbers span the grains' range, and the sub-step follows the
+    minimum stopping time."""
+    import ctypes
+    from dustyfargoadsg_amd.workloads import gas_dust, gas_dust_particles
+    p, pl = gas_dust()
+    d = gas_dust_particles(p)
+    print("case gas_dust 1024x3072 np=1e6 FP32, 20 sub-steps", flush=True)
+    g = Disc(p, _capi.FP32)
+    g.initial_state(pl)
+    parts = init_dust_system(p, d)
+    g.attach_dust(d, parts, DT)
+    g.run_substeps(DT, 20)
+    st = g.status()
+    assert st.last_call_substeps == 20
+    s = g.dust_state()
+    for k in ("r", "th", "vr", "vth", "stokesnb"):
+        assert np.all(np.isfinite(s[k])), k
+    gs = g.state()
+    n = p.nrad * p.nsec
+    for k in ("rho", "vr", "vt"):
+        assert np.all(np.isfinite(gs[k][:n])), k
+    lib = _capi.load()
+    radii, rm, surf, az = np.zeros(p.nrad + 1), np.zeros(p.nrad), np.zeros(p.nrad), np.zeros(p.nsec)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    lib.dfargo_fill_polar_1d_arrays(ctypes.byref(p), dp(radii), dp(rm), dp(surf), dp(az))
+    inside = (s["r"] >= radii[0]) & (s["r"] < radii[-1])
+    dd = g.field("dust_density")[:n].reshape(p.nrad, p.nsec)
+    total = (dd * surf[:, None]).sum()
+    # float storage of the deposit: each cell rounded to ~6e-8
+    assert abs(total / (inside.sum() * d.particle_mass) - 1.0) < 1e-6
+    stk = s["stokesnb"][inside]
+    assert stk.min() < 1e-3 and stk.max() > 5e-3
+    assert 0.0 < st.maxdt < 1e-3

@@ -233,3 +233,68 @@ def test_nonreflecting_energy_fp64(nr, ns, calls):
         assert d.status().last_call_substeps == n_o
         assert np.array_equal(d.fargo_shift()[0], o.fargo_shift()[0])
     compare(o, d, p, 1e-12, 1e-12, 1e-12, 1e-12)
+
+
+@pytest.mark.parametrize("bc1d,prec", [(2, _capi.FP64), (0, _capi.FP64), (1, _capi.FP32)])
+def test_damp_to_viscous_1d_disc(bc1d, prec):
+    """DAMPTOVISCOUS (src/SideEuler.c:1444-1565): the 1-D viscous disc beside
+    the grid, read back directly and compared with the oracle's after two
+    calls, for BOUNDARY1DGRID D (2), T (0) and, in FP32 storage, Z (1); the
+    damped 2-D fields with it.  The 1-D disc is FP64 at either precision."""
+    p, pl = gas_planet(nrad=32, nsec=96, damp_to=_capi.DAMP_TO_VISCOUS, bc1d=bc1d,
+                       wave_killing_zones=1, wkz_rmin=0.55, wkz_rmax=2.2)
+    o, so, d, sg = pair(p, pl, prec)
+    s0, v0 = d.visc1d()
+    so0, vo0 = o.viscous_1d(0.0)
+    assert len(s0) == p.nrad1d and np.array_equal(s0, so0) and np.array_equal(v0, vo0)
+    for _ in range(2):
+        n_o = o.algogas(DT)
+        d.algogas(DT)
+        assert d.status().last_call_substeps == n_o
+    s1, v1 = d.visc1d()
+    so1, vo1 = o.viscous_1d(0.0)
+    assert rel(s1, so1, 1, len(so1)) <= 1e-12
+    assert np.abs(v1 - vo1).max() <= 1e-12 * np.abs(vo1).max()
+    assert rel(s1, s0, 1, len(s0)) > 0.0  # the 1-D disc evolved
+    if prec == _capi.FP64:
+        compare(o, d, p, 1e-12, 1e-12, 1e-12)
+    else:
+        compare(o, d, p, 2e-5, 2e-6, 2e-6)
+    # a second InitEuler on the same context restarts the 1-D disc from its
+    # initial profiles
+    d.initial_state(pl)
+    s2, v2 = d.visc1d()
+    assert np.array_equal(s2, s0) and np.array_equal(v2, v0)
+
+
+def test_damp_to_viscous_without_zones_accepts_any_1d_grid():
+    """Without the wave-killing zones the 1-D disc only evolves (nothing
+    interpolates it), so a 1-D grid that does not enclose the 2-D one is
+    accepted, as in the reference."""
+    p, pl = gas_planet(nrad=32, nsec=96, damp_to=_capi.DAMP_TO_VISCOUS, bc1d=1, rmin1d=0.5, rmax1d=2.0)
+    o, so, d, sg = pair(p, pl)
+    n_o = o.algogas(DT)
+    d.algogas(DT)
+    assert d.status().last_call_substeps == n_o
+    compare(o, d, p, 1e-12, 1e-12, 1e-12)
+
+
+def test_accretion_fp32_ten_orbits():
+    """An accreting planet in FP32 storage against FP64 over 10 orbits: the
+    accretion rewrites the Roche-lobe cells every sub-step through the
+    stochastically rounded store.  The planet feels the disc, so its orbit
+    integrates the FP32 torque error: measured on MI355X, ring-mean Sigma
+    1.4e-5, Sigma L2 7.1e-5, planet position 3.9e-5, planet mass 4.1e-6
+    (the non-accreting planet on a forced circular orbit stays within 1e-11);
+    the bars leave a factor of about two."""
+    import bench
+    p, pl = gas_planet(forced_circular=0)
+    pl = dict(pl, acc=[2.0], feel_disk=[1], feel_others=[1])
+    print("case accretion 128x384, 10 orbits, FP32 vs FP64", flush=True)
+    r = bench.precision_error(p, pl, 10, control=False)["fp32_vs_fp64"]
+    print(r, flush=True)
+    assert r["substeps"][0] == r["substeps"][1]
+    assert r["sigma_ring_mean_linf"] <= 3e-5
+    assert r["sigma_l2"] <= 1.5e-4
+    assert r["planet_position"] <= 1e-4
+    assert r["planet_mass"] <= 1e-5
