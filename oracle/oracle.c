@@ -12,11 +12,33 @@
  *   tables          closed forms of Rmed, Surf, log spacing
  *   init            equilibrium of the initial state (vr stays ~h^2 small)
  *   CFL             closed-form dt for a uniform, static disc
- *   SubStep1        sign/scale: a pure pressure ramp kicks vr by -dt*dP/dr/rho
+ *   SubStep1        free fall at rest is exactly -dt/(R_i R_{i-1}); a
+ *                   linear pressure ramp adds -dt dP/dR/rho; the initial
+ *                   equilibrium's kick is small (test_oracle_stages.py)
  *   viscosity       solid-body rotation is stress-free; Keplerian shear
- *                   stress matches -3/2 nu Sigma Omega to O(dr^2)
- *   SubStep2        von Neumann-Richtmyer q = C^2 rho dv^2 on a single jump
- *   SubStep3        closed-form beta cooling e/(1+dt/t_c)
+ *                   stress matches -3/2 nu Sigma Omega to O(dr^2); a uniform
+ *                   radial flow gives e_phiphi = div v = alpha/r and
+ *                   tau_rr, tau_phiphi = -2/3, 4/3 rho nu alpha/r exactly;
+ *                   an azimuthal sine gives the forward difference
+ *   heating         Keplerian shear on a power-law disc: Q is an exact
+ *                   power law of Rmed including the extrapolated ring 0,
+ *                   first-order convergent to 9/4 nu Sigma Omega^2
+ *   SubStep2        von Neumann-Richtmyer q = C^2 rho dv^2 on single radial
+ *                   and azimuthal jumps, faces pushed apart by dt q/(rho dR),
+ *                   compressional heating dt q |dv|/dR; expansion untouched
+ *   SubStep3        closed-form beta cooling e/(1+dt/t_c); pdV of a uniform
+ *                   radial flow with heating, backward Euler in closed form
+ *                   and convergent to e0 exp(-(gamma-1) alpha t/r)
+ *   potential       Plummer planet + its indirect term in closed form (with
+ *                   and without INDIRECTTERM); the gas indirect term is
+ *                   the disc's softened pull on the star, summed over the
+ *                   sub-step's starting state, as a uniform field
+ *   OPEN edge       ghost rings copy the density, scale v_theta as r^-1/2,
+ *                   pass v_r only for outflow
+ *   floors/crash    floors lift only what is below; the crash flag
+ *                   overwrite, ActualiseGas's zero edges, the unconditional
+ *                   sub-Keplerian reset and the double mass taper as spec
+ *                   tests of their lines
  *   transport       mass conservation, zero-velocity identity, an integer
  *                   solid-body rotation equals numpy.roll, TVD in 1-D
  *   FARGO shift     Nshift = round(Omega dt Ns / 2pi) on solid-body rings
@@ -2805,4 +2827,43 @@ void or_orbital_elements(double x, double y, double vx, double vy, double m, dou
   else varpi = lambda;
   out[0] = e; out[1] = a; out[2] = M; out[3] = V;
   out[4] = PerihelionPA; out[5] = lambda; out[6] = varpi;
+}
+
+/* ------------------------------------------------------------------------ */
+/* test hooks: single stages of the sub-step on the context's own grids, so
+ * that tests can pin each stage against closed forms                        */
+/* ------------------------------------------------------------------------ */
+
+static double *test_grid(or_ctx *c, int which) {
+  double *g[OR_G_COUNT] = {c->rho,       c->vr,        c->vt,       c->en,       c->cs,      c->pres,
+                           c->pot,       c->vradint,   c->vthetaint, c->vradnew, c->vthetanew, c->energyint,
+                           c->energynew, c->rhoint,    c->temperint, c->divv,    c->drr,      c->dpp,
+                           c->drp,       c->trr,       c->tpp,       c->trp,     c->vischeat, c->indpot};
+  return (which >= 0 && which < OR_G_COUNT) ? g[which] : NULL;
+}
+
+int or_test_grid(or_ctx *c, int which, double *buf, int set) {
+  double *g = test_grid(c, which);
+  if (!g) return -1;
+  size_t n = (size_t)(c->nr + 1) * c->ns;
+  if (set) memcpy(g, buf, sizeof(double) * n);
+  else memcpy(buf, g, sizeof(double) * n);
+  return 0;
+}
+
+int or_test_stage(or_ctx *c, int stage, double dt) {
+  switch (stage) {
+    case OR_S_EOS: compute_sound_speed(c); compute_pressure(c); return 0;
+    case OR_S_SUBSTEP1: or_substep1(c, dt); return 0;
+    case OR_S_SUBSTEP2: substep2(c, dt); return 0;
+    case OR_S_VISC_TERMS: compute_viscous_terms(c, c->vr, c->vt); return 0;
+    case OR_S_VISC_HEATING: compute_viscous_heating(c); return 0;
+    case OR_S_DIVERGENCE: compute_divergence(c, c->vr, c->vt); return 0;
+    case OR_S_SUBSTEP3: substep3(c, dt); return 0;
+    case OR_S_POTENTIAL: fill_forces_arrays(c); return 0;
+    case OR_S_BOUNDARY: apply_boundary_condition(c); return 0;
+    case OR_S_FLOORS: add_floors(c); return 0;
+    case OR_S_CRASH: return detect_crash(c, c->rho) | (detect_crash(c, c->en) << 1);
+  }
+  return -1;
 }

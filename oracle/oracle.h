@@ -99,6 +99,31 @@ void or_dust_get_drag(or_ctx *c, double *raddrag, double *azidrag);
 double or_dust_min_stopping_time(or_ctx *c);
 double or_dust_particles_mass(or_ctx *c);
 
+/* ---- test hooks: one stage of the sub-step on the context's grids ------ */
+enum or_test_grid_id {
+  OR_G_RHO, OR_G_VR, OR_G_VT, OR_G_EN, OR_G_CS, OR_G_PRES, OR_G_POT, OR_G_VRADINT, OR_G_VTHETAINT,
+  OR_G_VRADNEW, OR_G_VTHETANEW, OR_G_ENERGYINT, OR_G_ENERGYNEW, OR_G_QR, OR_G_QT, OR_G_DIVV, OR_G_DRR,
+  OR_G_DPP, OR_G_DRP, OR_G_TRR, OR_G_TPP, OR_G_TRP, OR_G_VISCHEAT, OR_G_INDPOT, OR_G_COUNT
+};
+enum or_test_stage_id {
+  OR_S_EOS,           /* ComputeSoundSpeed + ComputePressureField          */
+  OR_S_SUBSTEP1,      /* SubStep1 (+SG, viscosity) into VradInt/VthetaInt  */
+  OR_S_SUBSTEP2,      /* SubStep2 from VradInt/VthetaInt                    */
+  OR_S_VISC_TERMS,    /* ComputeViscousTerms on Vrad/Vtheta                 */
+  OR_S_VISC_HEATING,  /* ComputeViscousHeating from the stress grids        */
+  OR_S_DIVERGENCE,    /* ComputeDivergenceVelocity on Vrad/Vtheta           */
+  OR_S_SUBSTEP3,      /* SubStep3 from EnergyInt, ViscHeat, DivV, Vtheta    */
+  OR_S_POTENTIAL,     /* FillForcesArrays                                   */
+  OR_S_BOUNDARY,      /* ApplyBoundaryCondition                             */
+  OR_S_FLOORS,        /* AddFloorDensity + AddFloorEnergy                   */
+  OR_S_CRASH          /* DetectCrash(Rho) | DetectCrash(Energy) << 1        */
+};
+/* copy (nrad+1)*nsec doubles of grid `which` out of (set = 0) or into
+ * (set = 1) the context; -1 for an unknown grid */
+int or_test_grid(or_ctx *c, int which, double *buf, int set);
+/* run one stage; its return value (OR_S_CRASH: the two flags), -1 unknown */
+int or_test_stage(or_ctx *c, int stage, double dt);
+
 #ifdef __cplusplus
 }
 #endif

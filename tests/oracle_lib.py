@@ -71,6 +71,8 @@ def lib():
             "or_gas_budgets": (None, [vp, dp]),
             "or_update_log": (ctypes.c_int, [vp, dp, dp]),
             "or_orbital_elements": (None, [ctypes.c_double] * 5 + [dp]),
+            "or_test_grid": (ctypes.c_int, [vp, ctypes.c_int, dp, ctypes.c_int]),
+            "or_test_stage": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -257,3 +259,26 @@ def orbital_elements(x, y, vx, vy, m):
     out = np.zeros(7)
     lib().or_orbital_elements(x, y, vx, vy, m, ptr(out))
     return out
+
+
+# oracle/oracle.h: or_test_grid_id and or_test_stage_id
+TEST_GRIDS = ["rho", "vr", "vt", "energy", "cs", "pres", "pot", "vradint", "vthetaint", "vradnew",
+              "vthetanew", "energyint", "energynew", "qr", "qt", "divv", "drr", "dpp", "drp", "trr",
+              "tpp", "trp", "vischeat", "indpot"]
+TEST_STAGES = ["eos", "substep1", "substep2", "visc_terms", "visc_heating", "divergence", "substep3",
+               "potential", "boundary", "floors", "crash"]
+
+
+def stage_grid(o, name, values=None):
+    """A work grid of the oracle context ((nrad+1)*nsec doubles): read, or
+    overwritten with `values`."""
+    buf = np.zeros((o.nr + 1) * o.ns) if values is None else np.ascontiguousarray(values, dtype=np.float64)
+    rc = lib().or_test_grid(o.ctx, TEST_GRIDS.index(name), buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            0 if values is None else 1)
+    assert rc == 0
+    return buf
+
+
+def run_stage(o, name, dt=0.0):
+    """One stage of the sub-step on the oracle context's grids."""
+    return lib().or_test_stage(o.ctx, TEST_STAGES.index(name), dt)
