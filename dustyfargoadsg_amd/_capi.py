@@ -260,6 +260,7 @@ def load(path=None):
         "dfargo_upload_dust": (c_int, [vp, ctypes.POINTER(DustState)]),
         "dfargo_gas_budgets": (c_int, [vp, dp]),
         "dfargo_update_log": (c_int, [vp, c_int, dp, dp, ctypes.POINTER(c_int)]),
+        "dfargo_update_log_noalloc": (c_int, [vp, c_int, dp, dp, ctypes.POINTER(c_int)]),
         "dfargo_orbital_elements": (c_int, [c_double] * 5 + [dp]),
         "dfargo_read_par": (c_int, [ctypes.c_char_p, P, ctypes.POINTER(RunParams)]),
         "dfargo_read_planet_config": (c_int, [ctypes.c_char_p, P, PP]),
@@ -283,6 +284,6 @@ EXPORTED = [
     "dfargo_get_visc1d", "dfargo_set_profiling", "dfargo_get_kernel_times", "dfargo_stream",
     "dfargo_dust_default_params", "dfargo_check_dust_params", "dfargo_init_dust_system",
     "dfargo_attach_dust", "dfargo_download_dust", "dfargo_upload_dust",
-    "dfargo_gas_budgets", "dfargo_update_log", "dfargo_orbital_elements",
+    "dfargo_gas_budgets", "dfargo_update_log", "dfargo_update_log_noalloc", "dfargo_orbital_elements",
     "dfargo_read_par", "dfargo_read_planet_config",
 ]

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -58,6 +59,8 @@ struct dfargo_ctx {
   // again by every InitEuler
   std::vector<double> v1d_sigma0, v1d_vrad0;
 };
+
+static int noalloc_setup(int dev);
 
 // ------------------------------------------------------------------------
 
@@ -581,6 +584,7 @@ extern "C" int dfargo_create(const dfargo_params *p, int precision, int device, 
   if (ring_kcheck(p->nsec) < 0)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "NSEC=%d cannot be split into <=1024-thread rings", p->nsec);
   HIPCHK(hipSetDevice(device));
+  if ((rc = noalloc_setup(device))) return rc;
   dfargo_ctx *c = new dfargo_ctx();
   c->p = *p;
   c->prec = precision;
@@ -1004,6 +1008,73 @@ extern "C" int dfargo_update_log(dfargo_ctx *c, int max_planets, double *torques
   std::vector<double> r;
   if ((rc = run_diagnostics(c, b, r))) return rc;
   log_torques(c, r.data(), torques, indirect);
+  *nb = k->nb;
+  return DFARGO_OK;
+}
+
+// ------------------------------------------------------------------------
+// UpdateLog without allocation
+//
+// The ring partials and sums of the diagnostics kernels live in one
+// file-scope device array instead of buffers allocated on first use.  Each
+// device holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device
+// (noalloc_scratch[dev], set when a context is created there).  Every call
+// on a device shares that device's array: calls that overlap on one device
+// race.  Each call writes every slot it reads (the ring partials of its
+// rings and bodies, then the sums), so nothing needs clearing between calls.
+
+#define DF_NOALLOC_MAX_NRAD 8192
+#define DF_NOALLOC_DOUBLES ((size_t)DIAG_NPART(DF_MAXPL) * DF_NOALLOC_MAX_NRAD + DIAG_NOUT(DF_MAXPL))
+static __device__ __align__(256) double scratch[DF_NOALLOC_DOUBLES];
+#define DF_NOALLOC_MAX_DEV 64
+static double *noalloc_scratch[DF_NOALLOC_MAX_DEV];
+static std::mutex noalloc_mutex;
+
+// the device address of this device's scratch array (the current device
+// is `dev`); once per device
+static int noalloc_setup(int dev) {
+  if (dev < 0 || dev >= DF_NOALLOC_MAX_DEV)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "device %d beyond the %d the library tracks", dev,
+                            DF_NOALLOC_MAX_DEV);
+  std::lock_guard<std::mutex> lock(noalloc_mutex);
+  if (noalloc_scratch[dev]) return DFARGO_OK;
+  double *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return dfargo_set_error(DFARGO_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                            hipGetErrorString(e));
+  noalloc_scratch[dev] = ptr;
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_update_log_noalloc(dfargo_ctx *c, int max_planets, double *torques, double *indirect,
+                                         int *nb) {
+  if (!c || !torques || !indirect || !nb) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
+  const int nr = c->p.nrad;
+  if (nr > DF_NOALLOC_MAX_NRAD)
+    return dfargo_set_error(DFARGO_ERR_ARG, "NRAD=%d exceeds the %d rings of the no-allocation scratch", nr,
+                            DF_NOALLOC_MAX_NRAD);
+  double *part = (c->dev >= 0 && c->dev < DF_NOALLOC_MAX_DEV) ? noalloc_scratch[c->dev] : nullptr;
+  if (!part) return dfargo_set_error(DFARGO_ERR_STATE, "no scratch address for device %d", c->dev);
+  HIPCHK(hipSetDevice(c->dev));
+  int rc = ctrl_pull(c);
+  if (rc) return rc;
+  const dfargo_ctrl *k = c->h_ctrl;
+  if (k->nb > max_planets)
+    return dfargo_set_error(DFARGO_ERR_ARG, "%d planets, room for %d", k->nb, max_planets);
+  DiagBodies b;
+  log_bodies(c, b);
+  // the partials of this disc's rings, then the sums
+  double *out_d = part + (size_t)DIAG_NPART(DF_MAXPL) * nr;
+  if (c->prec == DFARGO_FP64) launch_diagnostics<double>(c->a, b, part, out_d, c->stream);
+  else launch_diagnostics<float>(c->a, b, part, out_d, c->stream);
+  HIPCHK(hipGetLastError());
+  double r[DIAG_NOUT(DF_MAXPL)];
+  HIPCHK(hipMemcpyAsync(r, out_d, sizeof(double) * DIAG_NOUT(b.n), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  log_torques(c, r, torques, indirect);
   *nb = k->nb;
   return DFARGO_OK;
 }

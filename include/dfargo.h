@@ -349,6 +349,15 @@ int dfargo_gas_budgets(dfargo_ctx *ctx, double *out);
  * (DFARGO_ERR_ARG if it exceeds max_planets). */
 int dfargo_update_log(dfargo_ctx *ctx, int max_planets, double *torques, double *indirect, int *nb);
 
+/* dfargo_update_log without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (under HIP
+ * graph capture, in tight serving loops).  The same kernels and results as
+ * dfargo_update_log, with the ring partials and sums in a static device
+ * array of the library (one per device, 2.3 MB, kept until the process
+ * ends): NRAD must be <= 8192 (DFARGO_ERR_ARG otherwise).  Calls that
+ * overlap on one device share that array and race. */
+int dfargo_update_log_noalloc(dfargo_ctx *ctx, int max_planets, double *torques, double *indirect, int *nb);
+
 /* FindOrbitalElements (src/Planet.c:129-180) for a body at (x, y) moving
  * with (vx, vy) about a total mass m: out = e, a, M, V, PerihelionPA,
  * lambda, varpi (the columns of orbit<n>.dat after the time). */
