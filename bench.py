@@ -111,6 +111,81 @@ def run_gpu(params, planets, precision, steps, warmup, prof_steps, dust=None):
     return t1 - t0, kt
 
 
+def batched_sweep(n, seed, calls, warmup_calls=1, tune=True):
+    """The batched parameter sweep: n isothermal FP32 planet discs of uneven
+    grids and parameters (workloads.batch_discs), advanced `calls` AlgoGas
+    calls of DT = 2 pi / 20 each (1/20 orbit at r = 1) through one batch
+    (one graph launch per sub-step for all discs), against a second set of
+    the same discs advanced one disc after another (each disc's own graph).
+    Launch shapes from the autotuner (one tuning per grid shape).  Returns
+    discs x orbits per hour for both and the sub-steps taken."""
+    from dustyfargoadsg_amd import _capi
+    from dustyfargoadsg_amd.disc import Batch, Disc
+    from dustyfargoadsg_amd.workloads import batch_discs
+    DT = 2 * math.pi / 20
+    cases = batch_discs(n=n, seed=seed)
+    sets = []
+    t0 = time.perf_counter()
+    for _ in range(2):
+        ds = []
+        for p, pl in cases:
+            d = Disc(p, _capi.FP32)
+            d.initial_state(pl)
+            ds.append(d)
+        sets.append(ds)
+    t_create = time.perf_counter() - t0
+    tunings = {}
+    t0 = time.perf_counter()
+    if tune:
+        for ds in sets:
+            for d in ds:
+                tunings[f"{d.nr}x{d.ns}"] = d.autotune(substeps=4)
+    t_tune = time.perf_counter() - t0
+    batched, seq = sets
+    b = Batch(batched)
+    for _ in range(warmup_calls):
+        b.algogas(DT)
+        for d in seq:
+            d.algogas(DT)
+
+    def timed(step, discs):
+        sub0 = [d.status().substeps_total for d in discs]
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            step()
+        el = time.perf_counter() - t0
+        steps = [d.status().substeps_total - s0 for d, s0 in zip(discs, sub0)]
+        return el, sum(steps), sum(k * d.nr * d.ns for k, d in zip(steps, discs))
+
+    b.set_mode(Batch.GRAPH)
+    t_g, sub_g, cu_g = timed(lambda: b.algogas(DT), batched)
+    b.set_mode(Batch.STREAMS)
+    t_b, sub_b, cu_b = timed(lambda: b.algogas(DT), batched)
+    t_s, sub_s, cu_s = timed(lambda: [d.algogas(DT) for d in seq], seq)
+    orbits = calls * DT / (2 * math.pi)
+    cells = sum(p.nrad * p.nsec for p, _ in cases)
+    b.close()
+    for ds in sets:
+        for d in ds:
+            d.close()
+    return {
+        "workload": f"{n} isothermal FP32 planet discs, Nrad in {{128, 256, 512}}, Nsec = 3 Nrad, seed {seed}",
+        "grids": {k: sum(1 for p, _ in cases if f"{p.nrad}x{p.nsec}" == k) for k in
+                  sorted({f"{p.nrad}x{p.nsec}" for p, _ in cases})},
+        "calls": calls, "orbits_per_disc": orbits,
+        "batched_discs_orbits_per_hour": n * orbits / t_b * 3600.0,
+        "sequential_discs_orbits_per_hour": n * orbits / t_s * 3600.0,
+        "speedup": t_s / t_b,
+        "batched_s": t_b, "sequential_s": t_s,
+        "substeps": [sub_b, sub_s],
+        "cell_updates_per_s": [cu_b / t_b, cu_s / t_s],
+        "one_graph_mode": {"discs_orbits_per_hour": n * orbits / t_g * 3600.0, "s": t_g, "substeps": sub_g,
+                           "cell_updates_per_s": cu_g / t_g},
+        "cells_total": cells,
+        "tuning": tunings, "create_s": t_create, "tune_s": t_tune,
+    }
+
+
 def precision_error(params, planets, orbits, control=True):
     """FP32 against FP64 storage from the same initial disc after `orbits`
     orbits at r = 1 (AlgoGas calls of DT = 2 pi / 20): relative L-infinity
@@ -207,6 +282,12 @@ def main():
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--error-orbits", type=float, default=1.0,
                     help="orbits over which the FP32 error against FP64 is measured (0: skip)")
+    ap.add_argument("--target-steps", type=int, default=20,
+                    help="sub-steps of the 1024x3072 FP32 gas+SG disc timed beside the default "
+                         "workload (0: skip)")
+    ap.add_argument("--batched", type=int, default=64,
+                    help="discs of the batched sweep timed beside the default workload (0: skip)")
+    ap.add_argument("--batched-calls", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one disc on one GPU; multi-GPU runs are not "
@@ -238,6 +319,7 @@ def main():
     prof_steps = min(args.steps, 60)
     res = {}
     for prec, name in ((_capi.FP64, "fp64"), (_capi.FP32, "fp32")):
+        print(f"bench: {args.workload} {name}", file=sys.stderr, flush=True)
         el, kt = run_gpu(params, planets, prec, args.steps, args.warmup, prof_steps, dust)
         res[name] = dict(elapsed=el, kt=kt)
     w32, w64 = 4, 8
@@ -272,7 +354,11 @@ def main():
         # HBM bytes per launch of the roofline kernel (FETCH_SIZE + WRITE_SIZE,
         # scaled by a plain copy of known size)
         traffic = entry["bytes"] if entry else None
-    ach = kb32 / a32 / 1e9
+    # the roofline of the dominant kernel: the sub-step's algorithmic bytes
+    # per cell times the cells of one launch over its average duration; and
+    # the same with the kernel's own compulsory bytes
+    ach = sb32 / a32 / 1e9
+    ach_own = kb32 / a32 / 1e9
     line = {
         "metric": "cell-updates/s per hydro step (% HBM roofline), FP32 vs FP64 at <=1e-5 error",
         "value": v32,
@@ -294,7 +380,8 @@ def main():
                    "particles": npart},
         "roofline": {"bound": "hbm", "kernel": k32, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": kb32, "avg_launch_us": a32 * 1e6},
+                     "algorithmic_bytes_per_launch": sb32, "avg_launch_us": a32 * 1e6,
+                     "kernel_own_bytes_per_launch": kb32, "kernel_own_frac": ach_own / HBM_PEAK_GBS},
         "roofline_step": {"bytes_per_cell": step_bytes_per_cell(w32, energy, sg),
                           "achieved": sb32 / (ms32 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": sb32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS},
@@ -305,9 +392,29 @@ def main():
         "kernels_fp64": kern64,
         "cpu_baseline": None,
     }
+    if args.target_steps > 0 and args.workload == "energy_sg":
+        # the 1024x3072 FP32 gas+SG disc (with a planet), 40 B per cell
+        from dustyfargoadsg_amd.workloads import gas_sg_large
+        print("bench: gas_sg_large 1024x3072", file=sys.stderr, flush=True)
+        tp, tpl = gas_sg_large()
+        tcells = tp.nrad * tp.nsec
+        el32, _ = run_gpu(tp, tpl, _capi.FP32, args.target_steps, 5, 1)
+        el64, _ = run_gpu(tp, tpl, _capi.FP64, args.target_steps, 5, 1)
+        tb = step_bytes_per_cell(4, False, True)
+        ach_t = tb * tcells * args.target_steps / el32 / 1e9
+        line["target_1024x3072"] = {
+            "workload": "gas_sg_large 1024x3072 (isothermal gas + SG + planet)", "steps": args.target_steps,
+            "value": args.target_steps * tcells / el32, "ms_per_step": el32 / args.target_steps * 1e3,
+            "fp64_value": args.target_steps * tcells / el64, "fp32_over_fp64": el64 / el32,
+            "roofline_step": {"bytes_per_cell": tb, "achieved": ach_t, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": ach_t / HBM_PEAK_GBS}}
+    if args.batched > 0 and args.workload == "energy_sg":
+        print(f"bench: batched sweep of {args.batched} discs", file=sys.stderr, flush=True)
+        line["batched"] = batched_sweep(args.batched, 4, args.batched_calls)
     if args.error_orbits > 0 and dust is None:
         line["fp32_error_vs_fp64"] = precision_error(params, planets, args.error_orbits, control=False)
     if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
         cv, n, el = cpu_baseline(params, planets, args.cpu_seconds, dust)
         line["cpu_baseline"] = {"value": cv, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                                 "sample": f"{n} sub-steps of the same {params.nrad}x{params.nsec} "

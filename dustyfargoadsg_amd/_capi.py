@@ -106,6 +106,11 @@ class Status(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """dfargo_tuning: launch shapes that do not change results."""
+    _fields_ = [("xcd_remap", c_int), ("azi_waves", c_int), ("finish_block", c_int)]
+
+
 class DustParams(ctypes.Structure):
     """dfargo_dust_params (the particle switches of src/var.c:136-181)."""
     _fields_ = [
@@ -263,6 +268,13 @@ def load(path=None):
         "dfargo_update_log_noalloc": (c_int, [vp, c_int, dp, dp, ctypes.POINTER(c_int)]),
         "dfargo_orbital_elements": (c_int, [c_double] * 5 + [dp]),
         "dfargo_read_par": (c_int, [ctypes.c_char_p, P, ctypes.POINTER(RunParams)]),
+        "dfargo_get_tuning": (c_int, [vp, ctypes.POINTER(Tuning)]),
+        "dfargo_set_tuning": (c_int, [vp, ctypes.POINTER(Tuning)]),
+        "dfargo_autotune": (c_int, [vp, c_int, ctypes.POINTER(Tuning)]),
+        "dfargo_batch_create": (c_int, [ctypes.POINTER(vp), c_int, ctypes.POINTER(vp)]),
+        "dfargo_batch_algogas": (c_int, [vp, dp]),
+        "dfargo_batch_destroy": (c_int, [vp]),
+        "dfargo_batch_set_mode": (c_int, [vp, c_int]),
         "dfargo_read_planet_config": (c_int, [ctypes.c_char_p, P, PP]),
     }
     for name, (res, args) in sig.items():
@@ -286,4 +298,6 @@ EXPORTED = [
     "dfargo_attach_dust", "dfargo_download_dust", "dfargo_upload_dust",
     "dfargo_gas_budgets", "dfargo_update_log", "dfargo_update_log_noalloc", "dfargo_orbital_elements",
     "dfargo_read_par", "dfargo_read_planet_config",
+    "dfargo_get_tuning", "dfargo_set_tuning", "dfargo_autotune",
+    "dfargo_batch_create", "dfargo_batch_algogas", "dfargo_batch_destroy", "dfargo_batch_set_mode",
 ]

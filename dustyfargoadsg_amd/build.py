@@ -50,7 +50,36 @@ def build(verbose=False):
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_shim(verbose)
     return OUT
+
+
+SHIM = os.path.join(HERE, "shim")
+SHIM_HOST = os.path.join(SHIM, "fargo_host")
+
+
+def build_shim(verbose=False):
+    """The drop-in shim (shim/fargo_shim.c) and the small C host that drives
+    it (shim/fargo_host.c), linked against libdfargo.so with no undefined
+    symbol allowed: the shim's imports are exactly what a host defines."""
+    flags = ["-std=gnu11", "-O2", "-Wall", "-Wextra", "-Werror", "-ffp-contract=off",
+             "-I", os.path.join(HERE, "..", "include"), "-I", SHIM]
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    pid = os.getpid()  # concurrent builds (test workers) write their own files
+    for src in ("fargo_shim.c", "fargo_host.c"):
+        obj = os.path.join(BUILD, src + ".o")
+        tmp_obj = f"{obj}.{pid}"
+        _run(["gcc"] + flags + ["-c", os.path.join(SHIM, src), "-o", tmp_obj])
+        os.replace(tmp_obj, obj)
+        objs.append(obj)
+    tmp = f"{SHIM_HOST}.tmp{pid}"
+    _run(["gcc", "-o", tmp] + objs + ["-L", HERE, "-ldfargo", "-Wl,-rpath,$ORIGIN/..", "-Wl,--no-undefined",
+                                      "-lm"])
+    os.replace(tmp, SHIM_HOST)
+    if verbose:
+        print("built", SHIM_HOST)
+    return SHIM_HOST
 
 
 if __name__ == "__main__":

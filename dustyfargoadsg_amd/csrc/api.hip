@@ -12,7 +12,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -38,6 +40,7 @@ struct dfargo_ctx {
   std::vector<int> seq; // kernel ids of one sub-step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  long graph_version = 0; // bumped whenever the captured sub-step is dropped
   int initialized = 0, profiling = 0;
   double kt_ms[KID_COUNT];
   long kt_n[KID_COUNT];
@@ -145,6 +148,21 @@ static int check_frame_planets(const dfargo_params &p, const dfargo_planets *s) 
   return DFARGO_OK;
 }
 
+// a change of what the sub-step launches drops the graph, so that the next
+// sub-step captures it again
+static int drop_graph(dfargo_ctx *c) {
+  if (c->gexec) {
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  if (c->graph) {
+    HIPCHK(hipGraphDestroy(c->graph));
+    c->graph = nullptr;
+  }
+  c->graph_version++;
+  return DFARGO_OK;
+}
+
 // the planets' accretion switch is part of the captured sub-step: a change
 // drops the graph so that the next call captures it again
 static int set_accretion(dfargo_ctx *c, const dfargo_planets *s) {
@@ -154,14 +172,7 @@ static int set_accretion(dfargo_ctx *c, const dfargo_planets *s) {
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "planet accretion with particles is not implemented");
   if (acc != c->a.accrete) {
     c->a.accrete = acc;
-    if (c->gexec) {
-      HIPCHK(hipGraphExecDestroy(c->gexec));
-      c->gexec = nullptr;
-    }
-    if (c->graph) {
-      HIPCHK(hipGraphDestroy(c->graph));
-      c->graph = nullptr;
-    }
+    return drop_graph(c);
   }
   return DFARGO_OK;
 }
@@ -200,6 +211,8 @@ static bool is_sg_kernel(int id) {
          id == KID_SGZ_CONV || id == KID_SGZ_FILL;
 }
 
+static int ensure_graph(dfargo_ctx *c);
+
 static int run_substep(dfargo_ctx *c) {
   if (c->profiling) {
     // one event pair per kernel from a pool; collected once per sub-step
@@ -224,6 +237,15 @@ static int run_substep(dfargo_ctx *c) {
     }
     return DFARGO_OK;
   }
+  int rc = ensure_graph(c);
+  if (rc) return rc;
+  HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  return DFARGO_OK;
+}
+
+// The sub-step's kernel list captured once as a graph (c->graph) and
+// instantiated (c->gexec).
+static int ensure_graph(dfargo_ctx *c) {
   if (!c->gexec) {
     // The self-gravity transforms read only the density the sub-step starts
     // with and feed only the source kernel: in the graph they form a branch
@@ -247,7 +269,6 @@ static int run_substep(dfargo_ctx *c) {
     if (e != hipSuccess) return dfargo_set_error(DFARGO_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
   }
-  HIPCHK(hipGraphLaunch(c->gexec, c->stream));
   return DFARGO_OK;
 }
 
@@ -360,6 +381,8 @@ static int build_context(dfargo_ctx *c) {
       return rc;
   }
   a.xcd_remap = 1;
+  a.tune_azi_waves = 4;
+  a.tune_finish_block = 256;
   {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, c->dev));
@@ -1352,14 +1375,7 @@ extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, co
   c->dp = *d;
   c->dust_on = 1;
   // the sub-step now holds the particle kernels: capture it again
-  if (c->gexec) {
-    HIPCHK(hipGraphExecDestroy(c->gexec));
-    c->gexec = nullptr;
-  }
-  if (c->graph) {
-    HIPCHK(hipGraphDestroy(c->graph));
-    c->graph = nullptr;
-  }
+  if ((rc = drop_graph(c))) return rc;
   build_sequence(c);
   return DFARGO_OK;
 }
@@ -1410,5 +1426,287 @@ extern "C" int dfargo_upload_dust(dfargo_ctx *c, const dfargo_dust_state *st) {
   const DustSet &P = c->a.pset[c->h_ctrl->pset];
   if ((rc = put_dust_set(c, P, &cur, np)) || (rc = put_doubles(c, P.stokes, stk.data(), np))) return rc;
   if (st->dustsize_init) c->dustsize_init.assign(st->dustsize_init, st->dustsize_init + np);
+  return DFARGO_OK;
+}
+
+// ------------------------------------------------------------------------
+// launch-shape tuning
+
+extern "C" int dfargo_get_tuning(dfargo_ctx *c, dfargo_tuning *t) {
+  if (!c || !t) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  t->xcd_remap = c->a.xcd_remap;
+  t->azi_waves = c->a.tune_azi_waves;
+  t->finish_block = c->a.tune_finish_block;
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_set_tuning(dfargo_ctx *c, const dfargo_tuning *t) {
+  if (!c || !t) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if ((t->xcd_remap != 0 && t->xcd_remap != 1) || (t->azi_waves != 1 && t->azi_waves != 2 && t->azi_waves != 4) ||
+      (t->finish_block != 64 && t->finish_block != 128 && t->finish_block != 256))
+    return dfargo_set_error(DFARGO_ERR_ARG, "tuning out of range (xcd_remap 0/1, azi_waves 1/2/4, "
+                                            "finish_block 64/128/256)");
+  if (t->xcd_remap == c->a.xcd_remap && t->azi_waves == c->a.tune_azi_waves &&
+      t->finish_block == c->a.tune_finish_block)
+    return DFARGO_OK;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->a.xcd_remap = t->xcd_remap;
+  c->a.tune_azi_waves = t->azi_waves;
+  c->a.tune_finish_block = t->finish_block;
+  return drop_graph(c);
+}
+
+// best launch shapes found per disc shape and switches, for the process
+typedef std::tuple<int, int, int, int, int, int, int> TuneKey;
+static std::map<TuneKey, dfargo_tuning> tune_cache;
+static std::mutex tune_mutex;
+
+static TuneKey tune_key(const dfargo_ctx *c) {
+  return TuneKey(c->p.nrad, c->p.nsec, c->prec, c->p.energy_equation, c->p.self_gravity, c->dust_on, c->dev);
+}
+
+// the average time of one launch of kernel `id` in the scratch disc's
+// last profiled sub-steps
+static double kernel_avg_ms(const dfargo_ctx *s, int id) { return s->kt_n[id] ? s->kt_ms[id] / s->kt_n[id] : 0.0; }
+
+extern "C" int dfargo_autotune(dfargo_ctx *c, int substeps, dfargo_tuning *best_out) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
+  if (substeps < 1 || substeps > 1000) return dfargo_set_error(DFARGO_ERR_ARG, "substeps must lie in [1, 1000]");
+  int rc;
+  {
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    auto it = tune_cache.find(tune_key(c));
+    if (it != tune_cache.end()) {
+      if (best_out) *best_out = it->second;
+      return dfargo_set_tuning(c, &it->second);
+    }
+  }
+  HIPCHK(hipSetDevice(c->dev));
+  // a scratch disc of the same parameters from the initial state, so the
+  // trials leave this disc's state alone
+  const size_t n = (size_t)(c->p.nrad + 1) * c->p.nsec;
+  std::vector<double> rho(n, 0.0), en(n, 0.0), vr(n), vt(n);
+  if ((rc = dfargo_init_gas_density(&c->p, rho.data()))) return rc;
+  if (c->p.energy_equation && (rc = dfargo_init_gas_energy(&c->p, en.data()))) return rc;
+  if ((rc = ctrl_pull(c))) return rc;
+  dfargo_planets pl;
+  memset(&pl, 0, sizeof pl);
+  ctrl_to_planets(c->h_ctrl, &pl);
+  dfargo_ctx *s = nullptr;
+  if ((rc = dfargo_create(&c->p, c->prec, c->dev, &s))) return rc;
+  const double DT = 2.0 * M_PI / 20.0;
+  dfargo_tuning cur, best;
+  dfargo_get_tuning(c, &best);
+  // one knob at a time (each shapes different kernels), the others at the
+  // best values found so far; a warm-up pass before each timed one
+  auto trial = [&](const dfargo_tuning &t, double *ms) -> int {
+    int r;
+    if ((r = dfargo_set_tuning(s, &t)) || (r = dfargo_set_profiling(s, 1)) ||
+        (r = dfargo_run_substeps(s, DT, substeps)) || (r = dfargo_set_profiling(s, 1)) ||
+        (r = dfargo_run_substeps(s, DT, substeps)))
+      return r;
+    ms[0] = kernel_avg_ms(s, KID_CFL) + kernel_avg_ms(s, KID_ENERGY) + kernel_avg_ms(s, KID_EOS_STEP);
+    ms[1] = kernel_avg_ms(s, KID_TRANSPORT_AZI);
+    ms[2] = kernel_avg_ms(s, KID_FINISH);
+    return DFARGO_OK;
+  };
+  rc = dfargo_init_euler(s, rho.data(), vr.data(), vt.data(), c->p.energy_equation ? en.data() : nullptr, &pl);
+  const int xcd[] = {1, 0}, waves[] = {4, 2, 1}, blocks[] = {256, 128, 64};
+  for (int knob = 0; knob < 3 && !rc; knob++) {
+    double best_ms = 1e30;
+    const int nv = knob == 0 ? 2 : 3;
+    for (int v = 0; v < nv && !rc; v++) {
+      cur = best;
+      if (knob == 0) cur.xcd_remap = xcd[v];
+      else if (knob == 1) cur.azi_waves = waves[v];
+      else cur.finish_block = blocks[v];
+      double ms[3];
+      if ((rc = trial(cur, ms))) break;
+      if (ms[knob] < best_ms) {
+        best_ms = ms[knob];
+        if (knob == 0) best.xcd_remap = cur.xcd_remap;
+        else if (knob == 1) best.azi_waves = cur.azi_waves;
+        else best.finish_block = cur.finish_block;
+      }
+    }
+  }
+  dfargo_destroy(s);
+  HIPCHK(hipSetDevice(c->dev));
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    tune_cache[tune_key(c)] = best;
+  }
+  if (best_out) *best_out = best;
+  return dfargo_set_tuning(c, &best);
+}
+
+// ------------------------------------------------------------------------
+// batched discs: one graph holds every disc's sub-step as a child graph,
+// without dependencies between discs; each launch of it advances every
+// disc whose AlgoGas call is still running by one sub-step (the discs'
+// kernels read their own device loop condition, so a finished disc's
+// kernels return at once)
+
+struct dfargo_batch {
+  int mode = DFARGO_BATCH_STREAMS;
+  std::vector<dfargo_ctx *> d;
+  std::vector<long> version; // each disc's graph_version at capture
+  std::vector<double> DT;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+extern "C" int dfargo_batch_destroy(dfargo_batch *b) {
+  if (!b) return DFARGO_OK;
+  hipSetDevice(b->dev);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_batch_create(dfargo_ctx *const *discs, int n, dfargo_batch **out) {
+  if (!discs || !out || n < 1 || n > 4096) return dfargo_set_error(DFARGO_ERR_ARG, "need 1..4096 discs");
+  *out = nullptr;
+  for (int i = 0; i < n; i++) {
+    if (!discs[i]) return dfargo_set_error(DFARGO_ERR_ARG, "disc %d is null", i);
+    if (!discs[i]->initialized)
+      return dfargo_set_error(DFARGO_ERR_STATE, "disc %d: dfargo_init_euler has not been called", i);
+    if (discs[i]->dev != discs[0]->dev) return dfargo_set_error(DFARGO_ERR_ARG, "the discs must share a device");
+    if (discs[i]->profiling) return dfargo_set_error(DFARGO_ERR_STATE, "disc %d has profiling on", i);
+    for (int j = 0; j < i; j++)
+      if (discs[j] == discs[i]) return dfargo_set_error(DFARGO_ERR_ARG, "disc %d appears twice", i);
+  }
+  dfargo_batch *b = new dfargo_batch();
+  b->dev = discs[0]->dev;
+  b->d.assign(discs, discs + n);
+  b->DT.assign(n, 0.0);
+  int rc = DFARGO_OK;
+  hipError_t e = hipSetDevice(b->dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipGraphCreate(&b->graph, 0);
+  for (int i = 0; i < n && e == hipSuccess && !rc; i++) {
+    dfargo_ctx *c = b->d[i];
+    if ((rc = ensure_graph(c))) break;
+    b->version.push_back(c->graph_version);
+    hipGraphNode_t node;
+    e = hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, c->graph);
+  }
+  if (e == hipSuccess && !rc) e = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess && !rc) rc = dfargo_set_error(DFARGO_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
+  if (rc) {
+    dfargo_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_batch_set_mode(dfargo_batch *b, int mode) {
+  if (!b) return dfargo_set_error(DFARGO_ERR_ARG, "null batch");
+  if (mode != DFARGO_BATCH_GRAPH && mode != DFARGO_BATCH_STREAMS)
+    return dfargo_set_error(DFARGO_ERR_ARG, "unknown batch mode %d", mode);
+  b->mode = mode;
+  return DFARGO_OK;
+}
+
+// each disc's sub-step graph on the disc's own stream, as many times as the
+// disc's own estimate of its remaining sub-steps, the discs interleaved so
+// that consecutive launches go to different streams (and hardware queues);
+// a finished disc launches nothing
+static int batch_streams(dfargo_batch *b) {
+  const int n = (int)b->d.size();
+  for (int i = 0; i < n; i++) {
+    dfargo_ctx *c = b->d[i];
+    launch_algogas_begin(c->a, b->DT[i], c->stream);
+    if (c->p.energy_equation) {
+      if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, c->stream);
+      else launch_eos_pre<float>(c->a, c->stream);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  std::vector<int> rem(n, 1);
+  long launched = 0;
+  const long cap = 50000000L;
+  for (;;) {
+    int most = 0;
+    for (int i = 0; i < n; i++) most = std::max(most, rem[i]);
+    if (most == 0) break;
+    for (int r = 0; r < most; r++)
+      for (int i = 0; i < n; i++)
+        if (rem[i] > r) HIPCHK(hipGraphLaunch(b->d[i]->gexec, b->d[i]->stream));
+    launched += most;
+    for (int i = 0; i < n; i++)
+      if (rem[i] > 0)
+        HIPCHK(hipMemcpyAsync(b->d[i]->h_ctrl, b->d[i]->d_ctrl, sizeof(dfargo_ctrl), hipMemcpyDeviceToHost,
+                              b->d[i]->stream));
+    for (int i = 0; i < n; i++)
+      if (rem[i] > 0) HIPCHK(hipStreamSynchronize(b->d[i]->stream));
+    for (int i = 0; i < n; i++) {
+      if (rem[i] == 0) continue;
+      const dfargo_ctrl *k = b->d[i]->h_ctrl;
+      rem[i] = 0;
+      if (k->dtemp < 0.999999999 * b->DT[i])
+        rem[i] = (int)std::min(std::max(ceil((b->DT[i] - k->dtemp) / (k->dt > 0.0 ? k->dt : b->DT[i])), 1.0),
+                               4096.0);
+    }
+    if (launched > cap)
+      return dfargo_set_error(DFARGO_ERR_STATE, "more than %ld sub-steps in one batched AlgoGas call", cap);
+  }
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_batch_algogas(dfargo_batch *b, const double *DT) {
+  if (!b || !DT) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  const int n = (int)b->d.size();
+  for (int i = 0; i < n; i++) {
+    if (!(DT[i] > 0.0)) return dfargo_set_error(DFARGO_ERR_ARG, "disc %d: DT must be positive", i);
+    if (b->d[i]->graph_version != b->version[i])
+      return dfargo_set_error(DFARGO_ERR_STATE, "disc %d's sub-step changed since the batch was made", i);
+  }
+  HIPCHK(hipSetDevice(b->dev));
+  if (b->mode == DFARGO_BATCH_STREAMS) {
+    for (int i = 0; i < n; i++) b->DT[i] = DT[i];
+    return batch_streams(b);
+  }
+  // each disc's own stream holds no pending work once the batch starts
+  for (dfargo_ctx *c : b->d) HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++) {
+    dfargo_ctx *c = b->d[i];
+    b->DT[i] = DT[i];
+    launch_algogas_begin(c->a, DT[i], b->stream);
+    if (c->p.energy_equation) {
+      if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, b->stream);
+      else launch_eos_pre<float>(c->a, b->stream);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  long launched = 0;
+  int batch = 1;
+  const long cap = 50000000L;
+  for (;;) {
+    for (int k = 0; k < batch; k++) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    launched += batch;
+    for (dfargo_ctx *c : b->d)
+      HIPCHK(hipMemcpyAsync(c->h_ctrl, c->d_ctrl, sizeof(dfargo_ctrl), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    double rem = 0.0;
+    for (int i = 0; i < n; i++) {
+      const dfargo_ctrl *k = b->d[i]->h_ctrl;
+      if (k->dtemp < 0.999999999 * b->DT[i])
+        rem = std::max(rem, ceil((b->DT[i] - k->dtemp) / (k->dt > 0.0 ? k->dt : b->DT[i])));
+    }
+    if (rem <= 0.0) break;
+    if (launched > cap)
+      return dfargo_set_error(DFARGO_ERR_STATE, "more than %ld sub-steps in one batched AlgoGas call", cap);
+    batch = (int)std::min(std::max(rem, 1.0), 4096.0);
+  }
   return DFARGO_OK;
 }

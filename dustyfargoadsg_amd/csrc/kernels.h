@@ -46,6 +46,10 @@ struct KArgs {
   const double *v1d_rmed, *v1d_nu;
   int gas_indirect, indirect_term, alt_indirect, exclude_hill, roche, forced_circular;
   int xcd_remap;
+  // launch shapes the autotuner picks (results do not depend on them):
+  // wavefronts per workgroup of the azimuthal transport (1, 2, 4) and
+  // threads per workgroup of the velocity/boundary kernel (64, 128, 256)
+  int tune_azi_waves, tune_finish_block;
   int num_cu; // compute units of the device
   // constants
   double pm, dphi, invdphi, gamma, alpha, visc_const, floor_dens;

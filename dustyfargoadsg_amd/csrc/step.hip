@@ -2174,6 +2174,11 @@ static dim3 cell_grid(const KArgs &a, int rings) {
   return dim3((a.ns + bx - 1) / bx, rings);
 }
 static dim3 cell_block(const KArgs &a) { return dim3(a.ns < 256 ? a.ns : 256); }
+// the velocity/boundary kernel's workgroup: the tuned width, at most a ring
+static int finish_bx(const KArgs &a) {
+  const int w = (a.tune_finish_block == 64 || a.tune_finish_block == 128) ? a.tune_finish_block : 256;
+  return a.ns < w ? a.ns : w;
+}
 
 template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) {
   switch (id) {
@@ -2201,13 +2206,16 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       break;
     }
     case KID_TRANSPORT_AZI: {
-      const int nseg = (a.ns + AZ_OWN - 1) / AZ_OWN, per = 256 / AZ_LANES;
-      hipLaunchKernelGGL(k_transport_azi<T>, dim3((nseg + per - 1) / per, a.nr), dim3(256), 0, s, a);
+      const int per = (a.tune_azi_waves == 1 || a.tune_azi_waves == 2) ? a.tune_azi_waves : 4;
+      const int nseg = (a.ns + AZ_OWN - 1) / AZ_OWN;
+      hipLaunchKernelGGL(k_transport_azi<T>, dim3((nseg + per - 1) / per, a.nr), dim3(per * AZ_LANES), 0, s, a);
       break;
     }
-    case KID_FINISH:
-      hipLaunchKernelGGL(k_finish<T>, cell_grid(a, a.nr + 1), cell_block(a), 0, s, a);
+    case KID_FINISH: {
+      const int bx = finish_bx(a);
+      hipLaunchKernelGGL(k_finish<T>, dim3((a.ns + bx - 1) / bx, a.nr + 1), dim3(bx), 0, s, a);
       break;
+    }
     case KID_DUST_MOVE:
     case KID_DUST_SCAN:
     case KID_DUST_SCATTER:

@@ -208,6 +208,23 @@ class Disc:
             setattr(st, k, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         _check(self.lib, self.lib.dfargo_upload_dust(self.ctx, ctypes.byref(st)))
 
+    # -- launch shapes --------------------------------------------------------
+    def tuning(self):
+        t = _capi.Tuning()
+        _check(self.lib, self.lib.dfargo_get_tuning(self.ctx, ctypes.byref(t)))
+        return dict(xcd_remap=t.xcd_remap, azi_waves=t.azi_waves, finish_block=t.finish_block)
+
+    def set_tuning(self, **kw):
+        t = _capi.Tuning(**dict(self.tuning(), **kw))
+        _check(self.lib, self.lib.dfargo_set_tuning(self.ctx, ctypes.byref(t)))
+
+    def autotune(self, substeps=5):
+        """Times the launch shapes on a scratch disc of the same parameters
+        and applies the fastest (kept for later discs of this shape)."""
+        t = _capi.Tuning()
+        _check(self.lib, self.lib.dfargo_autotune(self.ctx, substeps, ctypes.byref(t)))
+        return dict(xcd_remap=t.xcd_remap, azi_waves=t.azi_waves, finish_block=t.finish_block)
+
     # -- measurement --------------------------------------------------------
     def set_profiling(self, on):
         _check(self.lib, self.lib.dfargo_set_profiling(self.ctx, 1 if on else 0))
@@ -224,6 +241,42 @@ class Disc:
 
     def stream(self):
         return self.lib.dfargo_stream(self.ctx)
+
+
+class Batch:
+    """Discs advanced together: one graph launch advances every unfinished
+    disc by one sub-step, each with its own CFL step."""
+
+    def __init__(self, discs):
+        self.discs = list(discs)
+        self.lib = self.discs[0].lib
+        arr = (ctypes.c_void_p * len(self.discs))(*[d.ctx for d in self.discs])
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.dfargo_batch_create(arr, len(self.discs), ctypes.byref(h)))
+        self.handle = h
+
+    GRAPH, STREAMS = 0, 1
+
+    def set_mode(self, mode):
+        """GRAPH: one graph of all discs per sub-step; STREAMS: each disc's
+        graph on its own stream as often as it needs (the default)."""
+        _check(self.lib, self.lib.dfargo_batch_set_mode(self.handle, mode))
+
+    def algogas(self, DT):
+        """One AlgoGas call per disc, of length DT (a number or one per disc)."""
+        dts = np.ascontiguousarray(np.broadcast_to(np.asarray(DT, dtype=np.float64), (len(self.discs),)))
+        _check(self.lib, self.lib.dfargo_batch_algogas(self.handle, _ptr(dts)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.dfargo_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def init_dust_system(params, d, axi_sg_accr=None, lib=None):

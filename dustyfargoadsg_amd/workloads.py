@@ -128,6 +128,31 @@ def gas_dust_particles(params, nbpart=1_000_000, seed=DEFAULT_SEED, **over):
     return dust_particles(params, nbpart, **kw)
 
 
+def batch_discs(n=64, seed=DEFAULT_SEED, nrads=(128, 256, 512)):
+    """n independent isothermal planet discs of uneven grids and parameters
+    for the batched sweep: Nrad drawn from `nrads`, Nsec = 3 Nrad, log grid
+    over [0.4, 2.5]; planet mass ratio log-uniform in [1e-5, 3e-3] on a
+    fixed circular orbit at r = 1 (smoothing 0.6 H), alpha log-uniform in
+    [1e-4, 1e-2], h uniform in [0.03, 0.08], flaring index in [0, 0.25],
+    Sigma ~ r^-p with p in [0.5, 1.5]; all drawn from `seed`.  Returns a
+    list of (params, planets)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nrad = int(rng.choice(nrads))
+        q = float(10 ** rng.uniform(math.log10(1e-5), math.log10(3e-3)))
+        alpha = float(10 ** rng.uniform(-4.0, -2.0))
+        p = default_params(
+            nrad=nrad, nsec=3 * nrad, rmin=0.4, rmax=2.5, log_grid=1,
+            sigma0=6.3662e-4, sigma_slope=float(rng.uniform(0.5, 1.5)),
+            aspect_ratio=float(rng.uniform(0.03, 0.08)), flaring_index=float(rng.uniform(0.0, 0.25)),
+            alpha_viscosity=alpha, thickness_smoothing=0.6, forced_circular=1,
+            add_noise=1, noise_seed=seed + k, boundary=BC_WALL, fast_transport=1)
+        out.append((p, dict(dist=[1.0], mass=[q], acc=[0.0], feel_disk=[0], feel_others=[0])))
+    return out
+
+
 DUST_WORKLOADS = {"gas_dust": gas_dust_particles}
 
 WORKLOADS = {

@@ -387,6 +387,45 @@ int dfargo_read_par(const char *path, dfargo_params *p, dfargo_run_params *run);
  * FeelOthers [Binary]", read with %f as the reference reads them. */
 int dfargo_read_planet_config(const char *path, const dfargo_params *p, dfargo_planets *sys);
 
+/* ---- launch shapes and batched discs -------------------------------------- */
+
+/* Launch shapes of the sub-step kernels that do not change results: the
+ * ring kernels' XCD-contiguous block order, the azimuthal transport's
+ * wavefronts per workgroup and the velocity/boundary kernel's workgroup
+ * width.  No counterpart in the reference (its loops have no launch
+ * shape). */
+typedef struct dfargo_tuning {
+  int xcd_remap;     /* 1: the rings of one XCD are contiguous; 0: round-robin */
+  int azi_waves;     /* 1, 2 or 4 wavefronts per workgroup                    */
+  int finish_block;  /* 64, 128 or 256 threads per workgroup                  */
+} dfargo_tuning;
+int dfargo_get_tuning(dfargo_ctx *ctx, dfargo_tuning *t);
+int dfargo_set_tuning(dfargo_ctx *ctx, const dfargo_tuning *t);
+/* Times each knob's values on a scratch disc of the same parameters
+ * (`substeps` profiled sub-steps per trial; this disc's state is not
+ * touched), applies the fastest and keeps it for every later disc of the
+ * same shape, precision, switches and device in this process. */
+int dfargo_autotune(dfargo_ctx *ctx, int substeps, dfargo_tuning *best);
+
+/* Many discs advanced together, each with its own device-side CFL step
+ * and loop condition (src/SourceEuler.c:398-453 per disc), every disc's
+ * sub-step a captured graph (dispatch: dfargo_batch_set_mode).  dfargo_batch_algogas is one AlgoGas call of length DT[i] for
+ * disc i, state left in HBM (as dfargo_algogas_device).  The discs must be
+ * initialised, on one device, and keep the sub-step they had at creation
+ * (attaching particles or switching accretion needs a new batch;
+ * DFARGO_ERR_STATE otherwise).  No counterpart in the reference. */
+typedef struct dfargo_batch dfargo_batch;
+int dfargo_batch_create(dfargo_ctx *const *discs, int n, dfargo_batch **out);
+int dfargo_batch_algogas(dfargo_batch *b, const double *DT);
+int dfargo_batch_destroy(dfargo_batch *b);
+/* How a batch dispatches: DFARGO_BATCH_GRAPH, one graph holding every
+ * disc's sub-step launched once per sub-step for all discs (a finished
+ * disc's kernels return at once); DFARGO_BATCH_STREAMS (the default), each
+ * disc's sub-step graph on the disc's own stream, as many times as that
+ * disc still needs, the discs interleaved across the hardware queues. */
+enum dfargo_batch_mode { DFARGO_BATCH_GRAPH = 0, DFARGO_BATCH_STREAMS = 1 };
+int dfargo_batch_set_mode(dfargo_batch *b, int mode);
+
 /* Per-kernel timing with HIP events on the context's stream.  While
  * enabled, sub-steps launch eagerly and every kernel is bracketed by a pair
  * of events.  names: up to max entries of at most 47 characters. */
