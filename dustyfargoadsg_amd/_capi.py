@@ -228,6 +228,7 @@ def load(path=None):
     lib = ctypes.CDLL(p)
     P, PP = ctypes.POINTER(Params), ctypes.POINTER(Planets)
     dp, vp = ctypes.POINTER(c_double), ctypes.c_void_p
+    fp = ctypes.POINTER(ctypes.c_float)
     sig = {
         "dfargo_default_params": (None, [P]),
         "dfargo_check_params": (c_int, [P]),
@@ -275,6 +276,15 @@ def load(path=None):
         "dfargo_batch_algogas": (c_int, [vp, dp]),
         "dfargo_batch_destroy": (c_int, [vp]),
         "dfargo_batch_set_mode": (c_int, [vp, c_int]),
+        "dfargo_mlp_create": (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.c_ulong, c_int,
+                                      ctypes.POINTER(vp)]),
+        "dfargo_mlp_destroy": (c_int, [vp]),
+        "dfargo_mlp_param_count": (ctypes.c_size_t, [vp]),
+        "dfargo_mlp_get_params": (c_int, [vp, fp]),
+        "dfargo_mlp_set_params": (c_int, [vp, fp]),
+        "dfargo_mlp_forward": (c_int, [vp, c_int, fp, fp]),
+        "dfargo_mlp_gradients": (c_int, [vp, c_int, fp, fp, fp, fp]),
+        "dfargo_mlp_train": (c_int, [vp, c_int, fp, fp, c_int, ctypes.c_float, fp]),
         "dfargo_read_planet_config": (c_int, [ctypes.c_char_p, P, PP]),
     }
     for name, (res, args) in sig.items():
@@ -300,4 +310,6 @@ EXPORTED = [
     "dfargo_read_par", "dfargo_read_planet_config",
     "dfargo_get_tuning", "dfargo_set_tuning", "dfargo_autotune",
     "dfargo_batch_create", "dfargo_batch_algogas", "dfargo_batch_destroy", "dfargo_batch_set_mode",
+    "dfargo_mlp_create", "dfargo_mlp_destroy", "dfargo_mlp_param_count", "dfargo_mlp_get_params",
+    "dfargo_mlp_set_params", "dfargo_mlp_forward", "dfargo_mlp_gradients", "dfargo_mlp_train",
 ]

@@ -19,7 +19,7 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
            "-I", CSRC]
-HIP_SOURCES = ["step.hip", "sg.hip", "dust.hip", "api.hip"]
+HIP_SOURCES = ["step.hip", "sg.hip", "dust.hip", "api.hip", "mlp.hip"]
 C_SOURCES = ["host.c", "parfile.c"]
 
 

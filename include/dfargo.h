@@ -13,6 +13,8 @@
 #ifndef DFARGO_H
 #define DFARGO_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -425,6 +427,29 @@ int dfargo_batch_destroy(dfargo_batch *b);
  * disc still needs, the discs interleaved across the hardware queues. */
 enum dfargo_batch_mode { DFARGO_BATCH_GRAPH = 0, DFARGO_BATCH_STREAMS = 1 };
 int dfargo_batch_set_mode(dfargo_batch *b, int mode);
+
+/* ---- surrogate warm start: the MLP (no counterpart in the reference) ------- */
+
+/* A small MLP trained and evaluated on the GPU with the library's own BF16
+ * matrix kernels (FP32 accumulation, FP32 master weights, Adam): n_in
+ * inputs (<= 32) -> n_hidden layers of `width` GELU units -> n_out linear
+ * outputs (width, n_out multiples of 32).  Inputs and targets are host
+ * FP32 arrays, row-major, `batch` rows.  Parameters in the flat order
+ * W0, b0, W1, b1, ... with W_l row-major (inputs x outputs).  The loss is
+ * the mean squared error over the batch's outputs. */
+typedef struct dfargo_mlp dfargo_mlp;
+int dfargo_mlp_create(int n_in, int width, int n_hidden, int n_out, int max_batch, unsigned long seed, int device,
+                      dfargo_mlp **out);
+int dfargo_mlp_destroy(dfargo_mlp *m);
+size_t dfargo_mlp_param_count(const dfargo_mlp *m);
+int dfargo_mlp_get_params(dfargo_mlp *m, float *out);
+int dfargo_mlp_set_params(dfargo_mlp *m, const float *in);
+int dfargo_mlp_forward(dfargo_mlp *m, int batch, const float *x, float *y);
+/* the loss and its gradient with respect to every parameter (flat order) */
+int dfargo_mlp_gradients(dfargo_mlp *m, int batch, const float *x, const float *t, float *loss, float *grads);
+/* `steps` full-batch Adam steps (lr; beta 0.9, 0.999) on the device; the
+ * loss after them if loss is not NULL */
+int dfargo_mlp_train(dfargo_mlp *m, int batch, const float *x, const float *t, int steps, float lr, float *loss);
 
 /* Per-kernel timing with HIP events on the context's stream.  While
  * enabled, sub-steps launch eagerly and every kernel is bracketed by a pair
