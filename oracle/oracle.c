@@ -35,6 +35,9 @@
  *                   sub-step's starting state, as a uniform field
  *   OPEN edge       ghost rings copy the density, scale v_theta as r^-1/2,
  *                   pass v_r only for outflow
+ *   particles       a particle feeling only the star and a planet follows
+ *                   the softened point-mass acceleration in vector form;
+ *                   FBedot integrates to the drag dissipation m |dv|^2/t_s
  *   floors/crash    floors lift only what is below; the crash flag
  *                   overwrite, ActualiseGas's zero edges, the unconditional
  *                   sub-Keplerian reset and the double mass taper as spec
