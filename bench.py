@@ -186,7 +186,7 @@ def batched_sweep(n, seed, calls, warmup_calls=1, tune=True):
     }
 
 
-def precision_error(params, planets, orbits, control=True):
+def precision_error(params, planets, orbits, control=True, checkpoints=None):
     """FP32 against FP64 storage from the same initial disc after `orbits`
     orbits at r = 1 (AlgoGas calls of DT = 2 pi / 20): relative L-infinity
     and L2 errors per field (Sigma, v_theta, energy on their own scale; v_r
@@ -202,19 +202,26 @@ def precision_error(params, planets, orbits, control=True):
     nr, ns = params.nrad, params.nsec
     n = nr * ns
 
+    marks = sorted(set(int(round(o * 20)) for o in (checkpoints or []) if 0 < o < orbits))
+
     def run(prec, perturb):
         d = Disc(params, prec)
         st = d.initial_state(planets)
         if perturb:
             st["rho"][:n] *= 1.0 + 1e-14
             d.set_state(dict(rho=st["rho"], vr=st["vr"], vt=st["vt"], energy=st["energy"]))
+        saved = {}
         for c in range(int(round(orbits * 20))):
             d.algogas(DT)
+            if c + 1 in marks:
+                saved[c + 1] = d.state()
+                saved[c + 1]["substeps"] = d.status().substeps_total
             if c % 200 == 199:
                 print(f"precision run {'fp64' if prec == _capi.FP64 else 'fp32'}{' perturbed' if perturb else ''}:"
                       f" {(c + 1) / 20:g} orbits", file=sys.stderr, flush=True)
         out = d.state()
         out["substeps"] = d.status().substeps_total
+        out["saved"] = saved
         d.close()
         return out
 
@@ -242,7 +249,10 @@ def precision_error(params, planets, orbits, control=True):
         return out
 
     ref = run(_capi.FP64, False)
-    res = {"orbits": orbits, "fp32_vs_fp64": compare(run(_capi.FP32, False), ref)}
+    f32 = run(_capi.FP32, False)
+    res = {"orbits": orbits, "fp32_vs_fp64": compare(f32, ref)}
+    for c in marks:
+        res[f"fp32_vs_fp64_at_{c / 20:g}_orbits"] = compare(f32["saved"][c], ref["saved"][c])
     if control:
         res["fp64_perturbed_1e-14_vs_fp64"] = compare(run(_capi.FP64, True), ref)
     return res
@@ -288,6 +298,9 @@ def main():
     ap.add_argument("--batched", type=int, default=64,
                     help="discs of the batched sweep timed beside the default workload (0: skip)")
     ap.add_argument("--batched-calls", type=int, default=2)
+    ap.add_argument("--precision-only", type=float, default=0.0,
+                    help="only run the FP32-vs-FP64 comparison of the workload over this many orbits "
+                         "(with checkpoints every tenth of it and the perturbed FP64 control) and print it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one disc on one GPU; multi-GPU runs are not "
@@ -315,6 +328,12 @@ def main():
     from dustyfargoadsg_amd.workloads import DUST_WORKLOADS
     dust = DUST_WORKLOADS[args.workload](params) if args.workload in DUST_WORKLOADS else None
     npart = dust.nbpart if dust is not None else 0
+    if args.precision_only > 0:
+        o = args.precision_only
+        print(json.dumps({"workload": f"{args.workload} {params.nrad}x{params.nsec}",
+                          **precision_error(params, planets, o, control=True,
+                                            checkpoints=[o * k / 10 for k in range(1, 10)])}))
+        return
     cells = params.nrad * params.nsec
     prof_steps = min(args.steps, 60)
     res = {}
