@@ -186,15 +186,24 @@ def batched_sweep(n, seed, calls, warmup_calls=1, tune=True):
     }
 
 
-def precision_error(params, planets, orbits, control=True, checkpoints=None):
+def precision_error(params, planets, orbits, control=True, checkpoints=None, noise=(1e-14, 2.0 ** -24),
+                    reinject=()):
     """FP32 against FP64 storage from the same initial disc after `orbits`
     orbits at r = 1 (AlgoGas calls of DT = 2 pi / 20): relative L-infinity
     and L2 errors per field (Sigma, v_theta, energy on their own scale; v_r
     on the v_theta scale), the relative L-infinity error of the ring-mean
-    Sigma(r) and of the disc mass.  With `control`, a second FP64 run whose
-    initial Sigma is perturbed by 1e-14 (relative) gives the same figures
-    for FP64 against itself: where the flow amplifies perturbations
-    (gravitoturbulence), that is the floor any pointwise comparison has."""
+    Sigma(r) and of the disc mass, and the largest Sigma error in each of 16
+    radial bands.  With `control`, FP64 runs whose initial Sigma carries
+    independent relative noise of each amplitude in `noise` (uniform in
+    [-eps, eps] per cell, seeded) give the same figures for FP64 against
+    itself: 1e-14 shows how fast the flow amplifies perturbations, 2^-24
+    what one FP32 rounding of the initial state alone grows into.  Where the
+    flow amplifies perturbations (gravitoturbulence, unstable gap edges),
+    that is the floor any pointwise comparison has.  Each amplitude in
+    `reinject` adds an FP64 run that receives fresh noise of that amplitude
+    in Sigma, v_r, v_theta (and e) after every AlgoGas call: perturbations
+    of FP32 size that keep coming, as FP32 storage brings them, but no
+    rounding inside the sub-steps."""
     import numpy as np
     from dustyfargoadsg_amd import _capi
     from dustyfargoadsg_amd.disc import Disc
@@ -204,20 +213,29 @@ def precision_error(params, planets, orbits, control=True, checkpoints=None):
 
     marks = sorted(set(int(round(o * 20)) for o in (checkpoints or []) if 0 < o < orbits))
 
-    def run(prec, perturb):
+    def run(prec, eps, again=0.0):
         d = Disc(params, prec)
         st = d.initial_state(planets)
-        if perturb:
-            st["rho"][:n] *= 1.0 + 1e-14
+        rng = np.random.default_rng(12345)
+        if eps:
+            st["rho"][:n] *= 1.0 + eps * rng.uniform(-1.0, 1.0, n)
             d.set_state(dict(rho=st["rho"], vr=st["vr"], vt=st["vt"], energy=st["energy"]))
         saved = {}
         for c in range(int(round(orbits * 20))):
             d.algogas(DT)
+            if again:
+                s2 = d.state()
+                s2["rho"][:n] *= 1.0 + again * rng.uniform(-1.0, 1.0, n)
+                s2["vt"][:n] *= 1.0 + again * rng.uniform(-1.0, 1.0, n)
+                s2["vr"][:n] += again * np.abs(s2["vt"][:n]) * rng.uniform(-1.0, 1.0, n)
+                if params.energy_equation:
+                    s2["energy"][:n] *= 1.0 + again * rng.uniform(-1.0, 1.0, n)
+                d.set_state(dict(rho=s2["rho"], vr=s2["vr"], vt=s2["vt"], energy=s2["energy"]))
             if c + 1 in marks:
                 saved[c + 1] = d.state()
                 saved[c + 1]["substeps"] = d.status().substeps_total
             if c % 200 == 199:
-                print(f"precision run {'fp64' if prec == _capi.FP64 else 'fp32'}{' perturbed' if perturb else ''}:"
+                print(f"precision run {'fp64' if prec == _capi.FP64 else 'fp32'}{f' noise {eps:g}' if eps else ''}:"
                       f" {(c + 1) / 20:g} orbits", file=sys.stderr, flush=True)
         out = d.state()
         out["substeps"] = d.status().substeps_total
@@ -241,6 +259,8 @@ def precision_error(params, planets, orbits, control=True, checkpoints=None):
         if params.energy_equation:
             out["energy_linf"] = linf("energy")
             out["energy_l2"] = l2("energy")
+        band = np.abs(a["rho"][:n] - b["rho"][:n]).reshape(nr, ns).max(axis=1) / np.abs(b["rho"][:n]).max()
+        out["sigma_linf_by_band"] = [float(x.max()) for x in np.array_split(band, 16)]
         out["substeps"] = [int(a["substeps"]), int(b["substeps"])]
         qa, qb = a["planets"], b["planets"]
         if qb.nb > 0:
@@ -248,13 +268,16 @@ def precision_error(params, planets, orbits, control=True, checkpoints=None):
             out["planet_mass"] = max(abs(qa.mass[k] / qb.mass[k] - 1.0) for k in range(qb.nb))
         return out
 
-    ref = run(_capi.FP64, False)
-    f32 = run(_capi.FP32, False)
+    ref = run(_capi.FP64, 0.0)
+    f32 = run(_capi.FP32, 0.0)
     res = {"orbits": orbits, "fp32_vs_fp64": compare(f32, ref)}
     for c in marks:
         res[f"fp32_vs_fp64_at_{c / 20:g}_orbits"] = compare(f32["saved"][c], ref["saved"][c])
     if control:
-        res["fp64_perturbed_1e-14_vs_fp64"] = compare(run(_capi.FP64, True), ref)
+        for eps in noise:
+            res[f"fp64_noise_{eps:.3g}_vs_fp64"] = compare(run(_capi.FP64, eps), ref)
+    for eps in reinject:
+        res[f"fp64_noise_{eps:.3g}_every_call_vs_fp64"] = compare(run(_capi.FP64, 0.0, eps), ref)
     return res
 
 
@@ -298,6 +321,10 @@ def main():
     ap.add_argument("--batched", type=int, default=64,
                     help="discs of the batched sweep timed beside the default workload (0: skip)")
     ap.add_argument("--batched-calls", type=int, default=2)
+    ap.add_argument("--reinject", action="store_true",
+                    help="with --precision-only, add the FP64 run that receives FP32-size noise every call")
+    ap.add_argument("--no-controls", action="store_true",
+                    help="with --precision-only, skip the perturbed FP64 control runs")
     ap.add_argument("--precision-only", type=float, default=0.0,
                     help="only run the FP32-vs-FP64 comparison of the workload over this many orbits "
                          "(with checkpoints every tenth of it and the perturbed FP64 control) and print it")
@@ -331,7 +358,8 @@ def main():
     if args.precision_only > 0:
         o = args.precision_only
         print(json.dumps({"workload": f"{args.workload} {params.nrad}x{params.nsec}",
-                          **precision_error(params, planets, o, control=True,
+                          **precision_error(params, planets, o, control=not args.no_controls,
+                                            reinject=(2.0 ** -24,) if args.reinject else (),
                                             checkpoints=[o * k / 10 for k in range(1, 10)])}))
         return
     cells = params.nrad * params.nsec

@@ -49,6 +49,9 @@ struct dfargo_ctx {
   // capture only; default priority)
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // the next sub-step's row transforms of self-gravity are formed by
+  // k_finish (a.sg_rows_in_finish), and once at the start of each call
+  bool sg_post = false;
   std::vector<hipEvent_t> evpool;
   std::vector<double> VthetaMed, VradMed, gcs_host;
   std::vector<float> fstage;
@@ -206,9 +209,25 @@ static void launch_id(dfargo_ctx *c, int id, hipStream_t s) {
 }
 static void launch_id(dfargo_ctx *c, int id) { launch_id(c, id, c->stream); }
 
+// the kernels of the self-gravity branch that starts with the sub-step
 static bool is_sg_kernel(int id) {
   return id == KID_SG_ROWS_FWD || id == KID_SG_COLS || id == KID_SG_ROWS_INV || id == KID_SGZ_MEAN ||
          id == KID_SGZ_CONV || id == KID_SGZ_FILL;
+}
+
+// Start of an AlgoGas call on stream s: the device loop condition, the
+// sound speed of the state, and (when k_finish's partner forms them) the
+// row spectra of the state's density for the first sub-step.
+static void call_start(dfargo_ctx *c, double DT, hipStream_t s) {
+  launch_algogas_begin(c->a, DT, s);
+  if (c->p.energy_equation) {
+    if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, s);
+    else launch_eos_pre<float>(c->a, s);
+  }
+  if (c->sg_post) {
+    if (c->prec == DFARGO_FP64) launch_sg_rows_forced<double>(c->a, s);
+    else launch_sg_rows_forced<float>(c->a, s);
+  }
 }
 
 static int ensure_graph(dfargo_ctx *c);
@@ -281,10 +300,27 @@ static int ensure_graph(dfargo_ctx *c) {
 // The particles move after the planets (src/SourceEuler.c:504-531), i.e.
 // after the control kernel, and before the source terms that read their
 // feedback.
+//
+// With the full self-gravity and an edge that k_finish completes (wall or
+// open, no wave-killing zones or other boundary kernel after it), k_finish
+// can also form the row transforms for the next sub-step from the density
+// it stores (sg_post); the sub-step's self-gravity branch then starts with
+// the radial transforms.  Measured on MI355X (512x1536 energy+SG disc) the
+// ring-per-workgroup k_finish takes the row transforms' time back (21 us
+// against 9 us) and the CFL branch, which the radial transforms now meet
+// sooner, slows by as much, so the sub-step is no faster (0.172 against
+// 0.170 ms): it stays off (kSgRowsInFinish).
+static constexpr bool kSgRowsInFinish = false;
 static void build_sequence(dfargo_ctx *c) {
   c->seq.clear();
+  const bool wkz = c->p.wave_killing_zones && (c->a.wkz_n_in > 0 || c->a.wkz_i_out < c->p.nrad);
+  c->sg_post = kSgRowsInFinish && c->p.self_gravity && !c->p.sg_zero_mode &&
+               (c->p.boundary == DFARGO_BC_WALL || c->p.boundary == DFARGO_BC_OPEN) && !wkz &&
+               c->p.damp_to != DFARGO_DAMP_TO_VISCOUS;
   if (c->p.self_gravity && c->p.sg_zero_mode)
     for (int id : {KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL}) c->seq.push_back(id);
+  else if (c->p.self_gravity && c->sg_post)
+    for (int id : {KID_SG_COLS, KID_SG_ROWS_INV}) c->seq.push_back(id);
   else if (c->p.self_gravity)
     for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV}) c->seq.push_back(id);
   for (int id : {KID_CFL, KID_CONTROL}) c->seq.push_back(id);
@@ -294,6 +330,7 @@ static void build_sequence(dfargo_ctx *c) {
       c->seq.push_back(id);
   c->seq.push_back(KID_SOURCE);
   if (c->p.energy_equation) c->seq.push_back(KID_ENERGY);
+  c->a.sg_rows_in_finish = c->sg_post ? 1 : 0;
   for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_FINISH}) c->seq.push_back(id);
   if (c->p.boundary == DFARGO_BC_NONREFLECTING) {
     if (c->p.energy_equation) c->seq.push_back(KID_EOS_STEP);
@@ -581,6 +618,7 @@ static int build_context(dfargo_ctx *c) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+
   return DFARGO_OK;
 }
 
@@ -640,6 +678,7 @@ extern "C" int dfargo_destroy(dfargo_ctx *c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+
   if (c->stream2) hipStreamDestroy(c->stream2);
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -802,11 +841,7 @@ extern "C" int dfargo_algogas_device(dfargo_ctx *c, double DT) {
   if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
   if (!(DT > 0.0)) return dfargo_set_error(DFARGO_ERR_ARG, "DT must be positive");
   HIPCHK(hipSetDevice(c->dev));
-  launch_algogas_begin(c->a, DT, c->stream);
-  if (c->p.energy_equation) {
-    if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, c->stream);
-    else launch_eos_pre<float>(c->a, c->stream);
-  }
+  call_start(c, DT, c->stream);
   HIPCHK(hipGetLastError());
   long launched = 0;
   int batch = 1, rc;
@@ -831,11 +866,7 @@ extern "C" int dfargo_run_substeps(dfargo_ctx *c, double DT, int nsub) {
   if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
   if (!(DT > 0.0) || nsub < 0) return dfargo_set_error(DFARGO_ERR_ARG, "need DT > 0 and nsub >= 0");
   HIPCHK(hipSetDevice(c->dev));
-  launch_algogas_begin(c->a, DT, c->stream);
-  if (c->p.energy_equation) {
-    if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, c->stream);
-    else launch_eos_pre<float>(c->a, c->stream);
-  }
+  call_start(c, DT, c->stream);
   HIPCHK(hipGetLastError());
   int rc;
   for (int b = 0; b < nsub; b++)
@@ -1625,11 +1656,7 @@ static int batch_streams(dfargo_batch *b) {
   const int n = (int)b->d.size();
   for (int i = 0; i < n; i++) {
     dfargo_ctx *c = b->d[i];
-    launch_algogas_begin(c->a, b->DT[i], c->stream);
-    if (c->p.energy_equation) {
-      if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, c->stream);
-      else launch_eos_pre<float>(c->a, c->stream);
-    }
+    call_start(c, b->DT[i], c->stream);
   }
   HIPCHK(hipGetLastError());
   std::vector<int> rem(n, 1);
@@ -1681,11 +1708,7 @@ extern "C" int dfargo_batch_algogas(dfargo_batch *b, const double *DT) {
   for (int i = 0; i < n; i++) {
     dfargo_ctx *c = b->d[i];
     b->DT[i] = DT[i];
-    launch_algogas_begin(c->a, DT[i], b->stream);
-    if (c->p.energy_equation) {
-      if (c->prec == DFARGO_FP64) launch_eos_pre<double>(c->a, b->stream);
-      else launch_eos_pre<float>(c->a, b->stream);
-    }
+    call_start(c, DT[i], b->stream);
   }
   HIPCHK(hipGetLastError());
   long launched = 0;

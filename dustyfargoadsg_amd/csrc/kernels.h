@@ -50,6 +50,10 @@ struct KArgs {
   // wavefronts per workgroup of the azimuthal transport (1, 2, 4) and
   // threads per workgroup of the velocity/boundary kernel (64, 128, 256)
   int tune_azi_waves, tune_finish_block;
+  // with the full self-gravity: k_finish also forms the next sub-step's row
+  // spectra (one workgroup per ring), and the self-gravity branch of a
+  // sub-step starts with the radial transforms
+  int sg_rows_in_finish;
   int num_cu; // compute units of the device
   // constants
   double pm, dphi, invdphi, gamma, alpha, visc_const, floor_dens;
@@ -141,6 +145,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s);
 template <typename T> void prepare_kernels(const KArgs &a);
 template <typename T> void sg_prepare(const KArgs &a);
 template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s);
+template <typename T> void launch_sg_rows_forced(const KArgs &a, hipStream_t s);
 // dust.hip: the particle kernels of a sub-step (after the control kernel),
 // and the start-up interpolation outside sub-steps
 template <typename T> void launch_dust_kernel(int id, const KArgs &a, hipStream_t s);

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "fft_lds.h"
 #include "kernels.h"
 
 const char *kKernelNames[KID_COUNT] = {
@@ -65,7 +66,7 @@ template <typename T> __device__ __forceinline__ void st_sr(void *p, long i, dou
   ((T *)p)[i] = sr_round<T>(v, key);
 }
 // salts of the stochastically rounded stores
-enum { SR_VR = 1, SR_VT, SR_E, SR_EN, SR_RAD = 8, SR_AZI = 16, SR_EVA = 24, SR_ACC = 32 };
+enum { SR_VR = 1, SR_VT, SR_E, SR_EN, SR_RAD = 8, SR_AZI = 16, SR_EVA = 24, SR_ACC = 32, SR_FIN = 40 };
 
 // a / b from the hardware reciprocal, two Newton steps and one correction
 // of the quotient: within an ulp or two of the IEEE quotient for operands
@@ -755,6 +756,9 @@ __device__ void accrete_planets(const KArgs &a, dfargo_ctrl *c, double dt, doubl
   }
 }
 
+template <typename T>
+__device__ void control_serial(const KArgs &a, dfargo_ctrl *c, const double *sums, double *sh, double s_dt);
+
 // Two builds: without accretion, 1024 threads; with it, 512, whose larger
 // register budget keeps the accretion path and the one-thread n-body code
 // free of spills.
@@ -763,12 +767,24 @@ template <> struct CtrlBlock<true> { static constexpr int n = 512; };
 template <typename T, bool ACC> __global__ void __launch_bounds__(CtrlBlock<ACC>::n) k_control(KArgs a) {
   __shared__ double sh[1024];
   __shared__ double sums[DF_NPART(DF_MAXTG)];
-  dfargo_ctrl *c = a.ctrl;
+  // The control block is staged in LDS for the kernel's run: the one-thread
+  // part below reads and writes dozens of its fields in dependent order,
+  // which from global memory costs a memory latency each.  It is copied in
+  // and back by all threads; no kernel that may run beside this one writes
+  // the block (the self-gravity transforms only read its go flag, which
+  // this kernel leaves as it is).
+  __shared__ dfargo_ctrl cs;
+  static_assert(sizeof(dfargo_ctrl) % 8 == 0, "the control block is copied as 8-byte words");
+  constexpr int NW = (int)(sizeof(dfargo_ctrl) / 8);
+  dfargo_ctrl *const cg = a.ctrl;
   const int t = threadIdx.x, TB = blockDim.x, nr = a.nr, ns = a.ns;
-  if (!c->go) {
-    if (t == 0) c->active = 0;
+  if (!cg->go) {
+    if (t == 0) cg->active = 0;
     return;
   }
+  for (int w = t; w < NW; w += TB) ((unsigned long long *)&cs)[w] = ((const unsigned long long *)cg)[w];
+  __syncthreads();
+  dfargo_ctrl *c = &cs;
   double nd = 1e30;
   for (int i = 1 + t; i < nr; i += TB) nd = dmin2(nd, a.ring_dt[i]);
   for (int i = t; i < nr - 1; i += TB) {
@@ -843,7 +859,18 @@ template <typename T, bool ACC> __global__ void __launch_bounds__(CtrlBlock<ACC>
     __syncthreads();
     accrete_planets<T>(a, c, s_dt, sums, sh);
   }
-  if (t != 0) return;
+  __syncthreads();
+  if (t == 0) control_serial<T>(a, c, sums, sh, s_dt);
+  __syncthreads();
+  for (int w = t; w < NW; w += TB) ((unsigned long long *)cg)[w] = ((const unsigned long long *)&cs)[w];
+}
+
+// The one-thread part of the control kernel on the LDS copy c of the
+// control block: crash flags, indirect terms, barycentre, FillForcesArrays'
+// parameters, AdvanceSystemFromDisk, AdvanceSystemRK5, the frame and the
+// loop condition.
+template <typename T>
+__device__ void control_serial(const KArgs &a, dfargo_ctrl *c, const double *sums, double *sh, double s_dt) {
   const double dt = s_dt;
   c->crashed_d = sums[0] > 0.0;
   c->crashed_e = sums[1] > 0.0;
@@ -1700,7 +1727,11 @@ template <typename T> __global__ void __launch_bounds__(256) k_transport_azi(KAr
 
 // ComputeVelocities (src/TransportEuler.c:515-547) and the floors
 // (src/SourceEuler.c:2506-2541) of one cell from the transported momenta,
-// rounded to the storage type as the stored state would be
+// rounded to the storage type as the stored state would be.  The
+// velocities are quotients formed here, so with FP32 storage they are
+// rounded stochastically like every other state store (keyed by the cell,
+// so an edge ring that copies a neighbour's value recomputes exactly the
+// value stored there); density and energy are already stored values.
 struct CellState {
   double rho, vr, vt, e;
 };
@@ -1722,8 +1753,8 @@ template <typename T> __device__ CellState vel_cell(const KArgs &a, const dfargo
     if (a.floors && s.e < e_bottom) s.e = e_bottom;
   }
   s.rho = (double)(T)s.rho;
-  s.vr = (double)(T)s.vr;
-  s.vt = (double)(T)s.vt;
+  s.vr = (double)sr_round<T>(s.vr, sr_key(c, l, SR_FIN));
+  s.vt = (double)sr_round<T>(s.vt, sr_key(c, l, SR_FIN + 1));
   s.e = (double)(T)s.e;
   return s;
 }
@@ -1734,19 +1765,12 @@ template <typename T> __device__ CellState vel_cell(const KArgs &a, const dfargo
 // every boundary type (src/SideEuler.c:1110-1112).  Each cell of the edge
 // rings takes the values the boundary routines would copy into it from the
 // rings they read (2, 3 and nrad-2), recomputed here from the momenta.
-// Rings 0 .. nrad, the last only for the wall's vr at the outer face.
-template <typename T> __global__ void __launch_bounds__(256) k_finish(KArgs a) {
-  const dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
-  // the sub-step ends here: the next one's loop condition takes effect
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctrl->go = c->go_next;
-  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
-  const long l = (long)i * ns + j;
-  if (i == nr) {
-    if (a.bc == DFARGO_BC_WALL) st<T>(a.vr, l, -vel_cell<T>(a, c, nr - 2, j).vr);
-    return;
-  }
+// Stores cell (i, j) of rings 0 .. nrad-1 and returns its density as
+// stored.
+template <typename T> __device__ __forceinline__ double finish_cell(const KArgs &a, const dfargo_ctrl *c, int i,
+                                                                    int j) {
+  const int nr = a.nr;
+  const long l = (long)i * a.ns + j;
   CellState s = vel_cell<T>(a, c, i, j);
   if (a.bc == DFARGO_BC_WALL) {
     if (i == 0) {
@@ -1795,6 +1819,46 @@ template <typename T> __global__ void __launch_bounds__(256) k_finish(KArgs a) {
   st<T>(a.vr, l, s.vr);
   st<T>(a.vt, l, s.vt);
   if (a.energy) st<T>(a.en, l, s.e);
+  return (double)(T)s.rho;
+}
+// the wall's radial velocity at the outer face (ring nrad)
+template <typename T> __device__ __forceinline__ void finish_outer_face(const KArgs &a, const dfargo_ctrl *c, int j) {
+  if (a.bc == DFARGO_BC_WALL) st<T>(a.vr, (long)a.nr * a.ns + j, -vel_cell<T>(a, c, a.nr - 2, j).vr);
+}
+
+// Rings 0 .. nrad, the last only for the wall's vr at the outer face.
+template <typename T> __global__ void __launch_bounds__(256) k_finish(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  // the sub-step ends here: the next one's loop condition takes effect
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctrl->go = c->go_next;
+  const int i = blockIdx.y, ns = a.ns, nr = a.nr, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  if (i == nr) {
+    finish_outer_face<T>(a, c, j);
+    return;
+  }
+  finish_cell<T>(a, c, i, j);
+}
+
+// The same with self-gravity, one workgroup per ring: after storing its
+// ring the workgroup forms the ring's reduced densities from the values it
+// stored and transforms them (compute_fftdensity and the azimuthal FFT of
+// the next sub-step, src/sgdens.c:54-70), so that the next sub-step's
+// self-gravity starts from the radial transforms.  The workgroup of ring
+// nrad-1 also stores the outer face.
+template <typename T> __global__ void __launch_bounds__(SG_TB_MAX) k_finish_rows(KArgs a, cx<T> *zt) {
+  extern __shared__ unsigned char smem[];
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->go = c->go_next;
+  const int i = ring_of_block(a), ns = a.ns, nr = a.nr;
+  cx<T> *A = (cx<T> *)smem, *B = A + fpsz(ns);
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    A[fp(j)] = sg_reduced<T>(a, i, finish_cell<T>(a, c, i, j));
+    if (i == nr - 1) finish_outer_face<T>(a, c, j);
+  }
+  sg_row_transform<T>(a, i, A, B, zt);
 }
 
 // EvanescentBoundary (src/SideEuler.c:796-950) for the gas, after the
@@ -2212,6 +2276,12 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
       break;
     }
     case KID_FINISH: {
+      if (a.sg_rows_in_finish) {
+        const size_t lds = sg_rows_lds_bytes(a, sizeof(cx<T>));
+        hipLaunchKernelGGL(k_finish_rows<T>, dim3(a.nr), dim3(sg_threads(a, lds, a.nr, a.ns / 2)), lds, s, a,
+                           (cx<T> *)a.sg_z);
+        break;
+      }
       const int bx = finish_bx(a);
       hipLaunchKernelGGL(k_finish<T>, dim3((a.ns + bx - 1) / bx, a.nr + 1), dim3(bx), 0, s, a);
       break;
@@ -2240,7 +2310,13 @@ void launch_algogas_begin(const KArgs &a, double DT, hipStream_t s) {
 }
 
 template <typename T> void prepare_kernels(const KArgs &a) {
-  if (a.sg && !a.sg_zero) sg_prepare<T>(a);
+  if (a.sg && !a.sg_zero) {
+    sg_prepare<T>(a);
+    // dynamic LDS above 64 KiB must be requested (gfx950 has 160 KiB per CU)
+    const size_t lds = sg_rows_lds_bytes(a, sizeof(cx<T>));
+    if (lds > 65536)
+      hipFuncSetAttribute((const void *)k_finish_rows<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  }
 }
 template void prepare_kernels<double>(const KArgs &);
 template void prepare_kernels<float>(const KArgs &);
