@@ -1399,8 +1399,8 @@ extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, co
   c->h_ctrl->out_count = 0;
   if ((rc = ctrl_push(c))) return rc;
   // start-up interpolation (src/main.c:287)
-  if (c->prec == DFARGO_FP64) launch_dust_startup<double>(a, timestep, c->stream);
-  else launch_dust_startup<float>(a, timestep, c->stream);
+  if (c->prec == DFARGO_FP64) launch_dust_startup<double>(a, timestep, c->h_ctrl->omega_frame, c->stream);
+  else launch_dust_startup<float>(a, timestep, c->h_ctrl->omega_frame, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   c->dp = *d;

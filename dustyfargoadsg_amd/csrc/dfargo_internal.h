@@ -76,6 +76,10 @@ typedef struct dfargo_ctrl {
      the control kernel for the particle kernels that run after it (it
      already tapers the masses for the next sub-step) */
   double dust_mp, dust_pl_m[DF_MAXPL];
+  /* ... and the planets' positions the particles feel (after the n-body
+     step, before RotatePsys) and the frame rate their interpolation reads
+     the gas in (OmegaFrame before this sub-step's frame update) */
+  double dust_pl_x[DF_MAXPL], dust_pl_y[DF_MAXPL], omega_prev;
   /* the particle set holding the current state, and the particles outside
      the grid counted by this sub-step's move */
   int pset, out_count;

@@ -178,12 +178,14 @@ __device__ __forceinline__ void stencil_weights(const KArgs &a, int ip, int jp, 
 // rings inner) over the nodes inside the grid.  Grid g1 of the
 // cell-centred stencil may be the per-ring isothermal sound speed; g2 and
 // g3 are read always (a valid grid stands in for an unused one) and summed
-// only when use2 / use3.
+// only when use2 / use3.  With a frame rate, g0 is vtheta and is read as
+// vt + Rmed frame at each node.
 template <typename T, int S, int KIND>
 __device__ __forceinline__ void stencil_gather(const KArgs &a, const double (&wr)[3], const double (&wt)[3],
                                                const int (&col)[3], int imin, const void *g0, const void *g1,
                                                const void *g2, const void *g3, bool iso1, bool use2, bool use3,
-                                               double &s0, double &s1, double &s2, double &s3) {
+                                               double &s0, double &s1, double &s2, double &s3,
+                                               double frame = 0.0) {
   constexpr int NI = nodes_r(S, KIND), NJ = nodes_t(S, KIND);
   const int nr = a.nr, ns = a.ns;
   bool inr[NI];
@@ -201,6 +203,7 @@ __device__ __forceinline__ void stencil_gather(const KArgs &a, const double (&wr
     for (int p = 0; p < NI; p++) {
       const long l = (long)ic[p] * ns + col[q];
       x0[q][p] = ldd<T>(g0, l);
+      if (frame != 0.0) x0[q][p] = x0[q][p] + a.Rmed[ic[p]] * frame;
       x1[q][p] = iso1 ? (double)(T)((const double *)g1)[ic[p]] : ldd<T>(g1, l);
       x2[q][p] = ldd<T>(g2, l);
       x3[q][p] = ldd<T>(g3, l);
@@ -224,10 +227,13 @@ __device__ __forceinline__ bool dust_deposit_needed(const KArgs &a, const dfargo
 
 // One particle per thread.  In a sub-step: SemiUpdateDustPositions over
 // 2 dt, interpolation (gathers, Stokes number, drag), UpdateDustVelocities
-// and RotateDsys by OmegaFrame dt = 0; the particle is counted into its
-// cell's bucket.  At start-up: the interpolation only, with the given time
-// step.  A particle outside the grid keeps its state and has no cell
-// (p_cell = -1).
+// and RotateDsys by OmegaFrame dt; the particle is counted into its cell's
+// bucket at the position before the rotation, where the reference's
+// interpolation deposits it.  At start-up: the interpolation only, with the
+// given time step.  The gas vtheta is read in the fixed frame, vt + Rmed
+// OmegaFrame per cell (src/Dsys.c:316), with the frame rate before this
+// sub-step's frame update.  A particle outside the grid keeps its state
+// and has no cell (p_cell = -1).
 template <typename T, int S>
 __global__ void __launch_bounds__(256) k_dust_move(KArgs a, int startup, double timestep0) {
   dfargo_ctrl *c = a.ctrl;
@@ -250,8 +256,10 @@ __global__ void __launch_bounds__(256) k_dust_move(KArgs a, int startup, double 
       r = rnew;
       P.r[k] = r;
     }
-    // RotateDsys by a zero angle still wraps every particle
-    if (!startup) P.th[k] = d_wrap(a, th);
+    // RotateDsys (src/Dsys.c:1153-1169) by the frame's new rate (in the
+    // fixed frame a wrap only); the stencils below use the azimuth before it
+    if (!startup) P.th[k] = d_wrap(a, th - c->omega_frame * timestep);
+    const double frame = startup ? c->omega_frame : c->omega_prev;
     int ip, jp;
     if (!d_in_grid(a, r) || !d_cell(a, r, th, ip, jp)) {
       a.p_cell[k] = -1;
@@ -275,7 +283,7 @@ __global__ void __launch_bounds__(256) k_dust_move(KArgs a, int startup, double 
       stencil_weights<S, 1>(a, ip, jp, rp, tp, az_jp, rmed_ip, wr, wt, col, imin, jmin, store01, k);
       base |= ((imin - ip + 1) << 4) | ((jmin - jp + 1) << 6);
       stencil_gather<T, S, 1>(a, wr, wt, col, imin, a.vt, a.azigradp, a.aziindacc, a.azisgacc, false,
-                              a.indirect_term, a.sg, vtg, azigradp, aziind, azisg);
+                              a.indirect_term, a.sg, vtg, azigradp, aziind, azisg, frame);
       // cell-centred stencil: density and the sound speed the sub-step
       // started with
       double densg = 0.0, csg = 0.0, unused0 = 0.0, unused1 = 0.0;
@@ -335,7 +343,7 @@ __global__ void __launch_bounds__(256) k_dust_move(KArgs a, int startup, double 
             FgravR = -vtd * vtd / rd;
           if (a.d_feel_planets) {
             for (int q = 0; q < c->nb; q++) {
-              const double xp = c->pl_x[q], yp = c->pl_y[q];
+              const double xp = c->dust_pl_x[q], yp = c->dust_pl_y[q];
               const double rpl = sqrt(xp * xp + yp * yp), tpl = atan2(yp, xp), mp = c->dust_pl_m[q];
               const double eps = a.thickness_smoothing * a.aspect_ratio * pow(0.5 * rpl, 1.0 + a.flaring) *
                                  pow(2.0, 1.0 + a.flaring);
@@ -605,6 +613,16 @@ __global__ void __launch_bounds__(256) k_dust_reorder(KArgs a) {
   B.id[s] = A.id[src];
 }
 
+// The gas vtheta after the start-up interpolation in a rotating frame:
+// vt + Rmed OmegaFrame and back (src/Dsys.c:316,1082), one cell per thread
+template <typename T> __global__ void __launch_bounds__(256) k_dust_frame_trip(KArgs a) {
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.ns) return;
+  const long l = (long)i * a.ns + j;
+  const double rw = a.Rmed[i] * a.ctrl->omega_frame;
+  ((T *)a.vt)[l] = (T)((ldd<T>(a.vt, l) + rw) - rw);
+}
+
 }  // namespace
 
 template <typename T, int S> static void dust_launch_s(int id, const KArgs &a, hipStream_t s, int force,
@@ -644,13 +662,17 @@ template <typename T> static void dust_launch(int id, const KArgs &a, hipStream_
 
 template <typename T> void launch_dust_kernel(int id, const KArgs &a, hipStream_t s) { dust_launch<T>(id, a, s, 0, 0.0); }
 
-template <typename T> void launch_dust_startup(const KArgs &a, double timestep, hipStream_t s) {
+template <typename T> void launch_dust_startup(const KArgs &a, double timestep, double omega_frame, hipStream_t s) {
   hipLaunchKernelGGL(k_dust_zero, dim3(1024), dim3(256), 0, s, a);
   for (int id : {KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT})
     dust_launch<T>(id, a, s, 1, timestep);
+  if (omega_frame != 0.0 && a.d_np > 0) {
+    const int bx = a.ns < 256 ? a.ns : 256;
+    hipLaunchKernelGGL(k_dust_frame_trip<T>, dim3((a.ns + bx - 1) / bx, a.nr), dim3(bx), 0, s, a);
+  }
 }
 
 template void launch_dust_kernel<double>(int, const KArgs &, hipStream_t);
 template void launch_dust_kernel<float>(int, const KArgs &, hipStream_t);
-template void launch_dust_startup<double>(const KArgs &, double, hipStream_t);
-template void launch_dust_startup<float>(const KArgs &, double, hipStream_t);
+template void launch_dust_startup<double>(const KArgs &, double, double, hipStream_t);
+template void launch_dust_startup<float>(const KArgs &, double, double, hipStream_t);

@@ -397,9 +397,6 @@ int dfargo_check_dust_params(const dfargo_params *p, const dfargo_dust_params *d
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "DUSTFEELSG Z (axisymmetric SG only) is not implemented");
   if (p->self_gravity && p->sg_zero_mode)
     return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "particles with SELFGRAVITY Z are not implemented");
-  if (p->omega_frame != 0.0 || p->corotating)
-    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED,
-                            "particles in a rotating frame (OMEGAFRAME != 0, FRAME C/G/B) are not implemented");
   return DFARGO_OK;
 }
 

@@ -149,7 +149,7 @@ template <typename T> void launch_sg_rows_forced(const KArgs &a, hipStream_t s);
 // dust.hip: the particle kernels of a sub-step (after the control kernel),
 // and the start-up interpolation outside sub-steps
 template <typename T> void launch_dust_kernel(int id, const KArgs &a, hipStream_t s);
-template <typename T> void launch_dust_startup(const KArgs &a, double timestep, hipStream_t s);
+template <typename T> void launch_dust_startup(const KArgs &a, double timestep, double omega_frame, hipStream_t s);
 template <typename T> void launch_eos_pre(const KArgs &a, hipStream_t s);
 template <typename T> void launch_potential(const KArgs &a, hipStream_t s);
 template <typename T> void launch_sg_accel(const KArgs &a, hipStream_t s); // rows_fwd..rows_inv

@@ -872,6 +872,7 @@ template <typename T, bool ACC> __global__ void __launch_bounds__(CtrlBlock<ACC>
 template <typename T>
 __device__ void control_serial(const KArgs &a, dfargo_ctrl *c, const double *sums, double *sh, double s_dt) {
   const double dt = s_dt;
+  c->omega_prev = c->omega_frame;
   c->crashed_d = sums[0] > 0.0;
   c->crashed_e = sums[1] > 0.0;
   // gas indirect term: ComputeAccel(star), src/SideEuler.c:282-298
@@ -974,6 +975,13 @@ __device__ void control_serial(const KArgs &a, dfargo_ctrl *c, const double *sum
       c->domega = omega_new - c->omega_frame;
       c->omega_frame = omega_new;
     }
+    // the particles of this sub-step feel the planets where the n-body
+    // step left them (src/SourceEuler.c:504-513 come before RotatePsys)
+    if (a.d_on)
+      for (int i = 0; i < nb; i++) {
+        c->dust_pl_x[i] = c->pl_x[i];
+        c->dust_pl_y[i] = c->pl_y[i];
+      }
     // RotatePsys (src/Psys.c:352-370)
     double angle = c->omega_frame * dt, sint = sin(angle), cost = cos(angle);
     for (int i = 0; i < nb; i++) {
@@ -1171,12 +1179,21 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
   __syncthreads();
   // stage 0: the state of rings i0-4 .. i0+SRC_TR+1 into LDS, and the
   // potential there (rings outside the grid hold zeros no valid cell reads)
+  const bool frame_trip = a.d_on && a.d_np > 0 && c->omega_prev != 0.0;
   for (int idx = tid; idx < SRC_PR * SRC_FC; idx += TB) {
     const int q = idx / SRC_FC, p = idx - q * SRC_FC, r = i0 - 4 + q;
     const bool in = (r >= 0 && r <= nr - 1);
     const long l = (long)r * ns + sCol[p];
     sRho[q][p] = in ? ((const T *)a.rho)[l] : (T)0;
-    sVth[q][p] = in ? ((const T *)a.vt)[l] : (T)0;
+    if (in && frame_trip) {
+      // the particles' interpolation holds vt + Rmed OmegaFrame in the gas
+      // grid and subtracts it again (src/Dsys.c:316,1082): SubStep1 reads
+      // the round trip's result
+      const double rw = sTab[ST_RMED][q] * c->omega_prev;
+      sVth[q][p] = (T)((ld<T>(a.vt, l) + rw) - rw);
+    } else {
+      sVth[q][p] = in ? ((const T *)a.vt)[l] : (T)0;
+    }
     sP[q][p] = in ? src_press<T>(a, l, r) : 0.0;
     if (p < SRC_PC) sPot[q][p] = in ? pot_at(a, c, r, sCol[p]) : 0.0;
   }

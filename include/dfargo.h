@@ -467,8 +467,8 @@ void *dfargo_stream(dfargo_ctx *ctx);
  * INTERPOLATION TSC, SFTAPPROX YES) with nbpart particles. */
 void dfargo_dust_default_params(dfargo_dust_params *d, int nbpart);
 /* Checks the particle switches; DFARGO_ERR_UNSUPPORTED for the options the
- * GPU path does not implement (and for a rotating frame, whose
- * interpolation moves vtheta by +R*OmegaFrame and back). */
+ * GPU path does not implement (turbulent kicks, growth, Hill-sphere
+ * removal, the ZZ integrator, the axisymmetric-only SG options). */
 int dfargo_check_dust_params(const dfargo_params *p, const dfargo_dust_params *d);
 /* InitDustSystem (src/Dsys.c:128-216): sizes, radii and azimuths drawn
  * with drand48 seeded by d->seed (all sizes, then all radii, then all

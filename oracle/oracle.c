@@ -2286,15 +2286,18 @@ static double dust_dr(or_ctx *c, int kind, int i, double rp) {
 
 /* interpolation (src/Dsys.c:265-1149), single process: gas values at the
  * particles, Stokes numbers and drag accelerations, feedback grids and the
- * dust density.  vtheta is read in the fixed frame (the reference adds
- * Rmed*OmegaFrame to the gas grid and removes it afterwards; a rotating
- * frame is not supported with particles). */
+ * dust density.  vtheta is read in the fixed frame: the gas grid holds
+ * vt + Rmed*OmegaFrame while the particles are processed and vt again
+ * afterwards (src/Dsys.c:316,1082), the round trip leaving the rounding of
+ * both operations in the gas grid. */
 void or_dust_interpolation(or_ctx *c, double timestep) {
   const dfargo_params *p = &c->p;
   const dfargo_dust_params *d = &c->dp;
   int nr = c->nr, ns = c->ns, scheme = d->interpolation;
   const double *vr = c->vr, *vt = c->vt, *dens = c->rho, *cs = c->cs;
   double dust_density_codeunits = d->rhopart * 1000.0 * pow(c->unit_length, 3.) / c->unit_mass;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) c->vt[j + i * ns] += c->Rmed[i] * c->OmegaFrame;
   for (int l = 0; l < nr * ns; l++) {
     if (d->dust_feedback) {
       c->radfbacc[l] = 0.0;
@@ -2443,6 +2446,8 @@ void or_dust_interpolation(or_ctx *c, double timestep) {
       }
     }
   }
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) c->vt[j + i * ns] -= c->Rmed[i] * c->OmegaFrame;
 }
 
 /* keep an azimuth in [AziInf[0], AziSup[ns-1]] (src/DustUpdate.c:38-45) */
@@ -2542,10 +2547,13 @@ void or_dust_get_gas_at_particles(or_ctx *c, double *vrg, double *vtg, double *d
   if (csg) memcpy(csg, c->d_csg, n);
 }
 
-/* the particle part of one sub-step over a frozen gas state, for pinning */
+/* the particle part of one sub-step over a frozen gas state, for pinning:
+ * positions, velocities, and RotateDsys by OmegaFrame dt (a wrap only in
+ * the fixed frame) */
 void or_dust_step(or_ctx *c, double dt) {
   dust_semi_update_positions(c, 2. * dt);
   dust_update_velocities(c, dt);
+  dust_rotate(c, c->OmegaFrame * dt);
 }
 
 void or_dust_get_drag(or_ctx *c, double *raddrag, double *azidrag) {

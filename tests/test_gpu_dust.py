@@ -305,3 +305,35 @@ def test_gas_dust_full_size_fp32_properties():
     stk = s["stokesnb"][inside]
     assert stk.min() < 1e-3 and stk.max() > 5e-3
     assert 0.0 < st.maxdt < 1e-3
+
+
+def _planet_disc_fixed_frame(nrad, nsec):
+    return gas_planet(nrad=nrad, nsec=nsec, omega_frame=1.0)
+
+
+def _planet_disc_corotating(nrad, nsec):
+    return gas_planet(nrad=nrad, nsec=nsec, omega_frame=1.0, corotating=1)
+
+
+@pytest.mark.parametrize("factory", [_planet_disc_fixed_frame, _planet_disc_corotating])
+def test_dust_rotating_frame_fp64(factory):
+    """Particles in a rotating frame: a fixed OMEGAFRAME, and FRAME C (the
+    frame follows the planet, so OmegaFrame changes every sub-step).  The
+    interpolation reads the gas in the fixed frame with the rate before the
+    frame update and leaves the round trip's rounding in the gas vtheta
+    (src/Dsys.c:316,1082), the particles feel the planet before RotatePsys
+    and turn with RotateDsys by the new rate (src/SourceEuler.c:504-530,
+    src/Dsys.c:1153-1169)."""
+    p, d, o, g = setup(factory, 32, 96, 3000, interpolation=_capi.INTERP_TSC)
+    for c in range(2):
+        n_o = o.algogas(DT)
+        g.algogas(DT)
+        assert g.status().last_call_substeps == n_o
+    compare_particles(o, g, 1e-11, 1e-11)
+    go, oo = g.state(), o.state()
+    n = p.nrad * p.nsec
+    for k in ("rho", "vt"):
+        assert relmax(go[k][:n], oo[k][:n]) <= 1e-11, k
+    assert np.abs(go["vr"][:n] - oo["vr"][:n]).max() <= 1e-11 * np.abs(oo["vt"][:n]).max()
+    assert abs(g.status().omega_frame - o.status().omega_frame) <= 1e-12
+    compare_grids(o, g, p, ("dust_density", "fb_accr", "fb_acct"), 1e-10)

@@ -212,3 +212,37 @@ def test_fbedot_is_the_drag_dissipation():
     fb = o.field("fb_edot")[: o.nr * o.ns].reshape(o.nr, o.ns)
     total = (fb * surf[:, None]).sum()
     assert power > 0 and abs(total / power - 1.0) < 1e-12
+
+
+def test_rotating_frame_is_a_rotation_of_the_fixed_frame():
+    """The same gas disc, frozen, and the same particle set advanced in the
+    fixed frame and in a frame rotating at OMEGAFRAME = 0.7 (gas vtheta
+    lower by Rmed OmegaFrame): the interpolation reads vt + Rmed OmegaFrame
+    (src/Dsys.c:316,1082) and RotateDsys turns the particles back by
+    OmegaFrame dt every sub-step (src/Dsys.c:1153-1169), so the particles'
+    radii and velocities agree and their azimuths differ by OmegaFrame t.
+    A wrong sign of either frame term, or a missing rotation, fails by
+    orders of magnitude.  (The gas itself is not frame-invariant in the
+    reference's discretisation -- the centrifugal term averages vtheta at
+    Rmed and adds the frame at Rinf -- hence the frozen gas.)"""
+    omega, dt, nstep = 0.7, 2e-3, 500
+    runs = []
+    for om in (0.0, omega):
+        p, _ = gas_planet(nrad=32, nsec=96, add_noise=0, omega_frame=om)
+        o = Oracle(p)
+        o.initial_state(dict(dist=[], mass=[], acc=[], feel_disk=[], feel_others=[]))
+        d = dust(300, dust_feel_planets=0, dust_feedback=0, sizeminpart=1e-2, sizemaxpart=1.0)
+        a = o.dust_init(d)
+        o.dust_attach(d, a, 1e-3)
+        for _ in range(nstep):
+            o.dust_step(dt)
+        runs.append(o.dust_state())
+    a, b = runs
+    inside = (a["r"] > 0.45) & (a["r"] < 2.4)
+    assert inside.sum() > 250
+    assert np.abs(a["vr"]).max() > 1e-5  # the drag acts
+    np.testing.assert_allclose(b["r"], a["r"], rtol=1e-11)
+    np.testing.assert_allclose(b["vr"], a["vr"], rtol=1e-8, atol=1e-13)
+    np.testing.assert_allclose(b["vth"], a["vth"], rtol=1e-11)
+    dth = np.angle(np.exp(1j * (a["th"] - b["th"] - omega * dt * nstep)))
+    assert np.abs(dth).max() < 1e-10
