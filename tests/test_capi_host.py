@@ -217,10 +217,11 @@ def test_dust_parameter_checks(lib):
     for k in ("dust_feel_turb", "dust_growth", "remove_dust_from_hill", "zz_integrator",
               "dust_feel_sg_zero_mode"):
         assert check(**{k: 1}) == _capi.ERR_UNSUPPORTED, k
+    # particles in a rotating frame are supported
     q = p.copy()
     q.omega_frame = 1.0
     d = _capi.dust_default_params(10, sizepartslope=0.0)
-    assert lib.dfargo_check_dust_params(ctypes.byref(q), ctypes.byref(d)) == _capi.ERR_UNSUPPORTED
+    assert lib.dfargo_check_dust_params(ctypes.byref(q), ctypes.byref(d)) == _capi.OK
 
 
 def test_orbital_elements_match_oracle(lib):
