@@ -37,7 +37,9 @@
  *                   pass v_r only for outflow
  *   particles       a particle feeling only the star and a planet follows
  *                   the softened point-mass acceleration in vector form;
- *                   FBedot integrates to the drag dissipation m |dv|^2/t_s
+ *                   FBedot integrates to the drag dissipation m |dv|^2/t_s;
+ *                   TSC off the nearest node sums the textbook kernel at
+ *                   the reference's three nodes
  *   floors/crash    floors lift only what is below; the crash flag
  *                   overwrite, ActualiseGas's zero edges, the unconditional
  *                   sub-Keplerian reset and the double mass taper as spec
@@ -56,8 +58,12 @@
  *   frames          (psys_info, correct_vtheta) a circular planet stays put
  *                   in FRAME C at rate sqrt(1+m); a run started inertial
  *                   equals one in a frame fixed at that rate; FRAME G turns
- *                   at the mean motion of an eccentric orbit; FRAME B and
- *                   the outer-planet option follow the right body
+ *                   at the mean motion of an eccentric orbit, also from an
+ *                   apocentre start in the anomaly's |arg| >= 1 branch;
+ *                   FRAME B (also with massless planets, on the final
+ *                   masses) and the outer-planet option follow the right
+ *                   body; particles in a rotating frame are the fixed
+ *                   frame's turned by OmegaFrame t
  *   accretion       (accrete_onto_planets) mass conservation planet+gas,
  *                   per-cell fractions by distance, momentum conservation
  *   wave-killing    (evanescent_boundary) ring means kept, deviations
@@ -71,8 +77,8 @@
  *                   an axisymmetric disc up to the two self-force terms
  *   1-D disc        (solve_1d_viscous) dSigma/dt of power laws under a
  *                   constant viscosity in closed form, vr = -3/2 nu/R for a
- *                   uniform disc; its Sigma' coefficient in vr (1.5) is
- *                   parity unpinned beyond the reference's own formula
+ *                   uniform disc, and vr of power-law discs against the
+ *                   reference's formula with its 1.5 on the Sigma' term
  *   (tests/test_oracle_frames.py for the last six)
  */
 #include <math.h>

@@ -62,7 +62,7 @@ def test_uniform_fields_interpolate_exactly(interp):
     st["vr"][:] = -1.5e-3
     st["vt"][:] = 0.875
     o.set_state(st)
-    d = dust(500, interpolation=interp, rmindust=0.7, rmaxdust=2.1)
+    d = dust(4000, interpolation=interp, rmindust=0.7, rmaxdust=2.1)
     a = o.dust_init(d)
     o.dust_attach(d, a, 0.01)
     g = o.dust_gas_at_particles()
@@ -81,6 +81,21 @@ def test_uniform_fields_interpolate_exactly(interp):
     np.testing.assert_allclose(g["vtg"][sel_c], 0.875, rtol=1e-13)
     # elsewhere the weights fall short of one, never above it
     assert np.all(g["densg"] <= 7.25e-4 * (1 + 1e-13))
+    if interp == _capi.INTERP_TSC:
+        # Off the nearest node the three radial nodes are the textbook TSC
+        # kernel W(x) = 3/4 - x^2 (|x| < 1/2), (3/2 - |x|)^2 / 2 (|x| <= 3/2)
+        # at distances 1 - |d|, |d|, 1 + |d| cells, d the (log) distance to
+        # the middle node: they sum to 7/8 + |d| (1 - |d|) / 2 for |d| > 1/2,
+        # and the azimuthal nodes (the nearest one in the middle) to one.
+        def tsc_sum(dd):
+            dd = np.abs(dd)
+            return np.where(dd <= 0.5, 1.0, 0.875 + 0.5 * dd * (1.0 - dd))
+        d_c = np.log(a["r"] / rmed[ip]) / dlog
+        d_f = np.log(a["r"] / radii[ip]) / dlog
+        assert (~near_c).sum() >= 10 and (~near_f).sum() > 1000
+        np.testing.assert_allclose(g["densg"], 7.25e-4 * tsc_sum(d_c), rtol=1e-13)
+        np.testing.assert_allclose(g["vtg"], 0.875 * tsc_sum(d_c), rtol=1e-13)
+        np.testing.assert_allclose(g["vrg"], -1.5e-3 * tsc_sum(d_f), rtol=1e-12)
 
 
 @pytest.mark.parametrize("interp", [_capi.INTERP_NGP, _capi.INTERP_CIC])

@@ -111,6 +111,38 @@ def test_frame_centre_choice(oracle_built, frame, outer):
     assert abs(pl.x[k] - r) < 1e-7 and abs(pl.y[k]) < 1e-7
 
 
+def test_binary_frame_with_massless_planets(oracle_built):
+    """FRAME B while the mass taper holds both planets at zero mass: the
+    barycentre is formed with the final masses instead (src/Psys.c:271-274),
+    1e-3 at r = 1 and 1e-12 at r = 1.6, so the frame follows planet 0.  Left
+    at zero weights the rate is 0/0; swapped weights follow planet 1 at a
+    rate 1.6^-1.5 times smaller."""
+    pl = dict(dist=[1.0, 1.6], mass=[1e-3, 1e-12], acc=[0.0, 0.0], feel_disk=[0, 0], feel_others=[0, 0])
+    p, o, st = _disc(corotating=_capi.FRAME_BINARY_CENTER, omega_frame=1.0, mass_taper=5.0, planets=pl)
+    s0 = st["planets"]
+    w0 = s0.vy[0] / s0.x[0]
+    o.algogas(1e-3)
+    assert list(o.state()["planets"].mass[:2]) == [0.0, 0.0]
+    assert abs(o.status().omega_frame - w0) < 1e-6 * w0
+
+
+def test_guiding_centre_at_apocentre(oracle_built):
+    """FRAME G from apocentre (e = 0.05): (1 - d/a)/e rounds to just below
+    -1, where the anomaly's branch sets E = pi (src/Psys.c:292-293) and the
+    guiding centre lies on the planet's side at distance a.  One short
+    sub-step then turns the frame at the mean motion, to the conditioning
+    of the anomaly at an apse (1e-3 here); the other branch value, E = 0,
+    puts the centre on the far side and the rate off by orders of
+    magnitude."""
+    m = 1e-3
+    n = math.sqrt(1 + m)
+    p, o, st = _disc(corotating=_capi.FRAME_GUIDING_CENTER, omega_frame=n, eccentricity=0.05,
+                     planets=_one(1.0, m))
+    assert st["planets"].x[0] > 1.04
+    o.algogas(1e-4)
+    assert abs(o.status().omega_frame - n) < 1e-3 * n
+
+
 def _noisy_state(o, st, seed):
     rng = np.random.default_rng(seed)
     for k in ("rho", "vt"):
@@ -423,3 +455,10 @@ def test_viscous_1d_disc(oracle_built):
         assert np.abs(rate - want).max() <= 1e-3 * scale.max(), slope
         if slope == 0.0:
             assert np.abs(v1[5:-5] + 1.5 * nu / r[5:-5]).max() <= 1e-12 * nu
+        else:
+            # spec of src/SideEuler.c:1538: vr = -1.5 nu/R - 3 nu' - 1.5 (nu/Sigma)
+            # Sigma', so -1.5 nu (1 + q)/R for Sigma = R^q (the textbook drift,
+            # with 3 on the Sigma' term, would give -1.5 nu (1 + 2q)/R: zero
+            # for q = -1/2); to O(dR^2) of the three-point derivative
+            want_v = -1.5 * nu * (1 + q) / r[5:-5]
+            assert np.abs(v1[5:-5] - want_v).max() <= 1e-3 * 1.5 * nu / r[5:-5].min(), slope
