@@ -157,11 +157,14 @@ def batched_sweep(n, seed, calls, warmup_calls=1, tune=True):
         steps = [d.status().substeps_total - s0 for d, s0 in zip(discs, sub0)]
         return el, sum(steps), sum(k * d.nr * d.ns for k, d in zip(steps, discs))
 
-    b.set_mode(Batch.GRAPH)
-    t_g, sub_g, cu_g = timed(lambda: b.algogas(DT), batched)
+    # the default dispatch and the sequential runs over the same calls of
+    # the same discs (so the same sub-steps), then the one-graph dispatch
+    # over the calls after them
     b.set_mode(Batch.STREAMS)
     t_b, sub_b, cu_b = timed(lambda: b.algogas(DT), batched)
     t_s, sub_s, cu_s = timed(lambda: [d.algogas(DT) for d in seq], seq)
+    b.set_mode(Batch.GRAPH)
+    t_g, sub_g, cu_g = timed(lambda: b.algogas(DT), batched)
     orbits = calls * DT / (2 * math.pi)
     cells = sum(p.nrad * p.nsec for p, _ in cases)
     b.close()
