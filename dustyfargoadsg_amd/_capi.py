@@ -253,7 +253,8 @@ def load(path=None):
         "dfargo_compute_selfgravity": (c_int, [vp]),
         "dfargo_fviscosity": (c_double, [vp, c_double]),
         "dfargo_get_fargo_shift": (c_int, [vp, ctypes.POINTER(c_long), dp]),
-        "dfargo_get_visc1d": (c_int, [vp, dp, dp, ctypes.POINTER(c_int)]),
+        "dfargo_get_visc1d": (c_int, [vp, dp, dp, dp, ctypes.POINTER(c_int)]),
+        "dfargo_set_visc1d": (c_int, [vp, dp, dp, dp, c_int]),
         "dfargo_set_profiling": (c_int, [vp, c_int]),
         "dfargo_get_kernel_times": (c_int, [vp, c_int, ctypes.c_void_p, dp,
                                             ctypes.POINTER(c_long), ctypes.POINTER(c_int)]),
@@ -303,7 +304,7 @@ EXPORTED = [
     "dfargo_init_euler", "dfargo_algogas", "dfargo_algogas_device", "dfargo_run_substeps", "dfargo_upload_state",
     "dfargo_download_state", "dfargo_get_status", "dfargo_set_time", "dfargo_get_field",
     "dfargo_compute_selfgravity", "dfargo_fviscosity", "dfargo_get_fargo_shift",
-    "dfargo_get_visc1d", "dfargo_set_profiling", "dfargo_get_kernel_times", "dfargo_stream",
+    "dfargo_get_visc1d", "dfargo_set_visc1d", "dfargo_set_profiling", "dfargo_get_kernel_times", "dfargo_stream",
     "dfargo_dust_default_params", "dfargo_check_dust_params", "dfargo_init_dust_system",
     "dfargo_attach_dust", "dfargo_download_dust", "dfargo_upload_dust",
     "dfargo_gas_budgets", "dfargo_update_log", "dfargo_update_log_noalloc", "dfargo_orbital_elements",

@@ -61,9 +61,9 @@ struct dfargo_ctx {
   int dust_on = 0;
   dfargo_dust_params dp;
   std::vector<double> dustsize_init;
-  // DAMPTOVISCOUS: the 1-D viscous disc's initial Sigma and vr, uploaded
-  // again by every InitEuler
-  std::vector<double> v1d_sigma0, v1d_vrad0;
+  // DAMPTOVISCOUS: the 1-D viscous disc's initial cell centres, Sigma and
+  // vr, uploaded again by every InitEuler
+  std::vector<double> v1d_rmed0, v1d_sigma0, v1d_vrad0;
 };
 
 static int noalloc_setup(int dev);
@@ -408,6 +408,7 @@ static int build_context(dfargo_ctx *c) {
       for (int i = 0; i < nr; i++)
         if (!(rmed[0] < t.Rmed[i] && t.Rmed[i] <= rmed[n - 1]))
           return dfargo_set_error(DFARGO_ERR_ARG, "the 1-D grid's cell centres must enclose every Rmed");
+    c->v1d_rmed0 = rmed;
     c->v1d_sigma0 = sig;
     c->v1d_vrad0 = vr;
     a.n1d = n;
@@ -811,6 +812,7 @@ extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad,
   // the 1-D viscous disc restarts from its initial profiles
   // (InitializeOneDViscousEvolution, src/SideEuler.c:1444-1487)
   if (a.damp_visc) {
+    HIPCHK(hipMemcpy((void *)a.v1d_rmed, c->v1d_rmed0.data(), sizeof(double) * a.n1d, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(a.v1d_sigma, c->v1d_sigma0.data(), sizeof(double) * a.n1d, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(a.v1d_vrad, c->v1d_vrad0.data(), sizeof(double) * a.n1d, hipMemcpyHostToDevice));
   }
@@ -1244,14 +1246,48 @@ extern "C" int dfargo_get_fargo_shift(dfargo_ctx *c, long *nshift, double *vmed)
   return DFARGO_OK;
 }
 
-extern "C" int dfargo_get_visc1d(dfargo_ctx *c, double *sigma, double *vrad, int *n1d) {
+extern "C" int dfargo_get_visc1d(dfargo_ctx *c, double *rmed, double *sigma, double *vrad, int *n1d) {
   if (!c || !n1d) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
   *n1d = c->a.damp_visc ? c->a.n1d : 0;
   if (!c->a.damp_visc) return DFARGO_OK;
+  const size_t bytes = sizeof(double) * c->a.n1d;
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (sigma) HIPCHK(hipMemcpy(sigma, c->a.v1d_sigma, sizeof(double) * c->a.n1d, hipMemcpyDeviceToHost));
-  if (vrad) HIPCHK(hipMemcpy(vrad, c->a.v1d_vrad, sizeof(double) * c->a.n1d, hipMemcpyDeviceToHost));
+  if (rmed) HIPCHK(hipMemcpy(rmed, c->a.v1d_rmed, bytes, hipMemcpyDeviceToHost));
+  if (sigma) HIPCHK(hipMemcpy(sigma, c->a.v1d_sigma, bytes, hipMemcpyDeviceToHost));
+  if (vrad) HIPCHK(hipMemcpy(vrad, c->a.v1d_vrad, bytes, hipMemcpyDeviceToHost));
+  return DFARGO_OK;
+}
+
+// The restart branch of InitializeOneDViscousEvolution
+// (src/SideEuler.c:1489-1502) replaces Rmed1D and Sigma1D with what it
+// reads; this takes any of the three profiles from the host.  The cell
+// centres must increase, and under the wave-killing zones enclose every
+// zone ring's Rmed, the bracket search of k_evanescent relies on both.
+extern "C" int dfargo_set_visc1d(dfargo_ctx *c, const double *rmed, const double *sigma, const double *vrad,
+                                 int n1d) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  if (!c->a.damp_visc) return dfargo_set_error(DFARGO_ERR_ARG, "the 1-D viscous disc is off (DAMPTOVISCOUS)");
+  const int n = c->a.n1d;
+  if (n1d != n) return dfargo_set_error(DFARGO_ERR_ARG, "the 1-D disc has NRAD1D = %d cells, got %d", n, n1d);
+  for (int i = 0; i < n; i++)
+    if ((rmed && !(rmed[i] > 0.0 && isfinite(rmed[i]))) || (sigma && !isfinite(sigma[i])) ||
+        (vrad && !isfinite(vrad[i])))
+      return dfargo_set_error(DFARGO_ERR_ARG, "1-D profile value %d is not finite (or Rmed1D <= 0)", i);
+  if (rmed) {
+    for (int i = 1; i < n; i++)
+      if (!(rmed[i] > rmed[i - 1])) return dfargo_set_error(DFARGO_ERR_ARG, "Rmed1D must increase (cell %d)", i);
+    if (c->p.wave_killing_zones)
+      for (int i = 0; i < c->p.nrad; i++)
+        if (!(rmed[0] < c->tab.Rmed[i] && c->tab.Rmed[i] <= rmed[n - 1]))
+          return dfargo_set_error(DFARGO_ERR_ARG, "the 1-D grid's cell centres must enclose every Rmed");
+  }
+  const size_t bytes = sizeof(double) * n;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (rmed) HIPCHK(hipMemcpy((void *)c->a.v1d_rmed, rmed, bytes, hipMemcpyHostToDevice));
+  if (sigma) HIPCHK(hipMemcpy(c->a.v1d_sigma, sigma, bytes, hipMemcpyHostToDevice));
+  if (vrad) HIPCHK(hipMemcpy(c->a.v1d_vrad, vrad, bytes, hipMemcpyHostToDevice));
   return DFARGO_OK;
 }
 

@@ -123,18 +123,27 @@ class Disc:
         _check(self.lib, self.lib.dfargo_run_substeps(self.ctx, DT, nsub))
 
     def set_state(self, st):
+        """Upload the fields and planets of `st`; with DAMPTOVISCOUS, an
+        entry "visc1d" = (Rmed1D, Sigma1D, Vrad1D) also sets the 1-D disc."""
         e = st.get("energy")
         _check(self.lib, self.lib.dfargo_upload_state(
             self.ctx, _ptr(st["rho"]), _ptr(st["vr"]), _ptr(st["vt"]),
             _ptr(e if (e is not None and self.p.energy_equation) else None),
             ctypes.byref(st["planets"]) if st.get("planets") is not None else None))
+        if st.get("visc1d") is not None:
+            self.set_visc1d(*st["visc1d"])
 
     def state(self):
+        """The fields and planets; with DAMPTOVISCOUS also "visc1d" =
+        (Rmed1D, Sigma1D, Vrad1D)."""
         rho, vr, vt, e = self.zeros(), self.zeros(), self.zeros(), self.zeros()
         sys_ = Planets()
         _check(self.lib, self.lib.dfargo_download_state(self.ctx, _ptr(rho), _ptr(vr), _ptr(vt),
                                                         _ptr(e), ctypes.byref(sys_)))
-        return dict(rho=rho, vr=vr, vt=vt, energy=e, planets=sys_)
+        out = dict(rho=rho, vr=vr, vt=vt, energy=e, planets=sys_)
+        if self.p.damp_to == _capi.DAMP_TO_VISCOUS:
+            out["visc1d"] = self.visc1d(rmed=True)
+        return out
 
     def status(self):
         st = Status()
@@ -159,12 +168,23 @@ class Disc:
         _check(self.lib, self.lib.dfargo_get_fargo_shift(self.ctx, ns, _ptr(vm)))
         return np.array(ns[:], dtype=np.int64), vm
 
-    def visc1d(self):
-        """DAMPTOVISCOUS's 1-D viscous disc: (Sigma, vr), NRAD1D values each."""
-        n = self.p.nrad1d
-        s, v, got = np.zeros(max(n, 1)), np.zeros(max(n, 1)), ctypes.c_int()
-        _check(self.lib, self.lib.dfargo_get_visc1d(self.ctx, _ptr(s), _ptr(v), ctypes.byref(got)))
-        return s[: got.value], v[: got.value]
+    def visc1d(self, rmed=False):
+        """DAMPTOVISCOUS's 1-D viscous disc: (Sigma, vr), NRAD1D values each,
+        or (Rmed1D, Sigma, vr) with rmed=True; empty when the option is off."""
+        n = max(self.p.nrad1d, 1)
+        r, s, v, got = np.zeros(n), np.zeros(n), np.zeros(n), ctypes.c_int()
+        _check(self.lib, self.lib.dfargo_get_visc1d(self.ctx, _ptr(r), _ptr(s), _ptr(v), ctypes.byref(got)))
+        m = got.value
+        return (r[:m], s[:m], v[:m]) if rmed else (s[:m], v[:m])
+
+    def set_visc1d(self, rmed=None, sigma=None, vrad=None):
+        """Replace any of the 1-D disc's Rmed1D, Sigma1D, Vrad1D (the restart
+        branch of src/SideEuler.c:1489-1502 replaces the first two)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (rmed, sigma, vrad)]
+        n = next((len(a) for a in arrs if a is not None), self.p.nrad1d)
+        if any(a is not None and len(a) != n for a in arrs):
+            raise ValueError("the 1-D profiles differ in length")
+        _check(self.lib, self.lib.dfargo_set_visc1d(self.ctx, *[_ptr(a) for a in arrs], n))
 
     def gas_budgets(self):
         """GasTotalMass, GasMomentum, GasTotalEnergy, planets' angular momentum."""

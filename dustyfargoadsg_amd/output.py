@@ -11,7 +11,9 @@ other and a run of one can restart from the other's files:
 * planet<k>.dat: one line per output, "%d" then ten "%#.14g" columns
   (timestep, x, y, vx, vy, mass, LostMass, PhysicalTime, OmegaFrame, mdcp,
   exces_mdcp), tab-separated, appended;
-* dims.dat and used_rad.dat as the reference writes them.
+* dims.dat and used_rad.dat as the reference writes them;
+* DAMPTOVISCOUS's 1-D disc: gasdens/gasvrad.ascii_rad.<N>.dat ("%g\t%g"
+  lines of Rmed1D and Sigma1D / Vrad1D), dims1D.dat and used_rad1D.dat.
 """
 import os
 
@@ -147,6 +149,57 @@ def write_used_rad(outdir, radii):
     with open(_name(outdir, "used_rad.dat"), "w") as f:
         for r in radii:
             f.write("%.18g\n" % r)
+
+
+def radii_1d(rmin1d, rmax1d, nrad1d):
+    """The 1-D disc's NRAD1D + 1 interface radii, logarithmically spaced
+    (src/SideEuler.c:1465-1468)."""
+    i = np.arange(nrad1d + 1, dtype=np.float64)
+    return rmin1d * np.exp(i / nrad1d * np.log(rmax1d / rmin1d))
+
+
+def write_dims_1d(outdir, nrad1d):
+    """dims1D.dat of InitializeOneDViscousEvolution (src/SideEuler.c:1504-1508):
+    NRAD1D eight times, each followed by a tab."""
+    with open(_name(outdir, "dims1D.dat"), "w") as f:
+        f.write("%d\t" * 8 % ((nrad1d,) * 8) + "\n")
+
+
+def write_used_rad_1d(outdir, radii1d):
+    """used_rad1D.dat (src/SideEuler.c:1509-1513): the NRAD1D + 1 interface
+    radii, "%f" one per line."""
+    with open(_name(outdir, "used_rad1D.dat"), "w") as f:
+        for r in radii1d:
+            f.write("%f\n" % r)
+
+
+def write_visc1d_profiles(outdir, number, rmed1d, sigma1d, vrad1d):
+    """Write1DViscProfiles (src/SideEuler.c:1564-1585), written with the gas
+    fields when the wave-killing zones damp towards the 1-D disc
+    (src/main.c:341-342): "%g\t%g" lines of (Rmed1D, Sigma1D) to
+    gasdens.ascii_rad.<number>.dat and of (Rmed1D, Vrad1D) to
+    gasvrad.ascii_rad.<number>.dat."""
+    for base, vals in (("gasdens", sigma1d), ("gasvrad", vrad1d)):
+        with open(_name(outdir, f"{base}.ascii_rad.{number}.dat"), "w") as f:
+            for r, v in zip(rmed1d, vals):
+                f.write("%g\t%g\n" % (r, v))
+
+
+def read_visc1d_restart(outdir, number, nrad1d):
+    """The restart branch of InitializeOneDViscousEvolution
+    (src/SideEuler.c:1489-1502): the first NRAD1D (Rmed1D, Sigma1D) pairs of
+    gasdens.ascii_rad.<number>.dat, which replace the 1-D disc's cell
+    centres and surface density.  Vrad1D keeps its t = 0 profile: the
+    reference opens gasvrad.ascii_rad.<number>.dat for writing, so its reads
+    there return nothing (and the file is emptied; this reader leaves it as
+    it is).  Returns (rmed1d, sigma1d)."""
+    path = _name(outdir, f"gasdens.ascii_rad.{number}.dat")
+    with open(path) as f:
+        vals = f.read().split()
+    if len(vals) < 2 * nrad1d:
+        raise ValueError(f"{path}: {len(vals) // 2} pairs, expected {nrad1d}")
+    a = np.array([float(v) for v in vals[: 2 * nrad1d]], dtype=np.float64).reshape(nrad1d, 2)
+    return a[:, 0].copy(), a[:, 1].copy()
 
 
 def write_outputs(outdir, number, state, nrad, nsec, temperature=None):

@@ -327,10 +327,22 @@ double dfargo_fviscosity(dfargo_ctx *ctx, double r);
  * ring-mean azimuthal velocities VMed[i] of the last sub-step (nrad each). */
 int dfargo_get_fargo_shift(dfargo_ctx *ctx, long *nshift, double *vmed);
 
-/* DAMPTOVISCOUS's 1-D viscous disc (src/SideEuler.c:1444-1565): its Sigma
- * and vr (n1d values each, NRAD1D) as the last sub-step left them; *n1d is
- * 0 and nothing is written when the option is off. */
-int dfargo_get_visc1d(dfargo_ctx *ctx, double *sigma, double *vrad, int *n1d);
+/* DAMPTOVISCOUS's 1-D viscous disc (src/SideEuler.c:1444-1565): its cell
+ * centres Rmed1D, Sigma1D and Vrad1D (n1d values each, NRAD1D) as the last
+ * sub-step left them, the state Write1DViscProfiles writes
+ * (src/SideEuler.c:1567-1585); any pointer may be NULL.  *n1d is 0 and
+ * nothing is written when the option is off. */
+int dfargo_get_visc1d(dfargo_ctx *ctx, double *rmed, double *sigma, double *vrad, int *n1d);
+
+/* Sets the 1-D disc's Rmed1D, Sigma1D and/or Vrad1D (n1d must be NRAD1D;
+ * a NULL pointer keeps that profile), as the restart branch of
+ * InitializeOneDViscousEvolution (src/SideEuler.c:1489-1502) replaces
+ * Rmed1D and Sigma1D with the values of gasdens.ascii_rad.<N>.dat.  Call it
+ * after dfargo_init_euler, which resets the profiles to their t = 0 values
+ * (dfargo_upload_state leaves them).  DFARGO_ERR_ARG when the option is off, on a
+ * length mismatch, a non-finite value, Rmed1D not increasing, or (with the
+ * wave-killing zones) Rmed1D not enclosing the grid's Rmed. */
+int dfargo_set_visc1d(dfargo_ctx *ctx, const double *rmed, const double *sigma, const double *vrad, int n1d);
 
 /* ---- per-DT diagnostics (src/main.c:291-394, between AlgoGas calls) -------- */
 

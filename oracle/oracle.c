@@ -968,7 +968,7 @@ static void accrete_onto_planets(or_ctx *c, double dt) {
 void or_accrete_onto_planets(or_ctx *c, double dt) { accrete_onto_planets(c, dt); }
 
 /* InitializeOneDViscousEvolution (src/SideEuler.c:1444-1487), without
- * the restart read and the dims1D/used_rad1D files */
+ * the dims1D/used_rad1D files; its restart read is or_set_visc1d */
 static void init_1d_viscous(or_ctx *c) {
   const dfargo_params *p = &c->p;
   const int n = p->nrad1d;
@@ -1038,6 +1038,26 @@ void or_solve_1d_viscous(or_ctx *c, double dt, double *sigma, double *vrad) {
   if (dt > 0.0) solve_1d_viscous(c, dt);
   memcpy(sigma, c->sigma1d, sizeof(double) * c->n1d);
   memcpy(vrad, c->vrad1d, sizeof(double) * c->n1d);
+}
+
+/* The restart branch of InitializeOneDViscousEvolution
+ * (src/SideEuler.c:1489-1502) overwrites Rmed1D and Sigma1D with the values
+ * it reads, after viscosity1D was computed from the t = 0 cell centres; a
+ * NULL pointer keeps that profile. */
+void or_set_visc1d(or_ctx *c, const double *rmed, const double *sigma, const double *vrad) {
+  if (!c->n1d) init_1d_viscous(c);
+  const size_t b = sizeof(double) * c->n1d;
+  if (rmed) memcpy(c->rmed1d, rmed, b);
+  if (sigma) memcpy(c->sigma1d, sigma, b);
+  if (vrad) memcpy(c->vrad1d, vrad, b);
+}
+
+void or_get_visc1d(or_ctx *c, double *rmed, double *sigma, double *vrad) {
+  if (!c->n1d) init_1d_viscous(c);
+  const size_t b = sizeof(double) * c->n1d;
+  if (rmed) memcpy(rmed, c->rmed1d, b);
+  if (sigma) memcpy(sigma, c->sigma1d, b);
+  if (vrad) memcpy(vrad, c->vrad1d, b);
 }
 
 /* CorrectVtheta (src/SideEuler.c:1133-1148) */

@@ -58,6 +58,10 @@ void or_sg_zero_mode(or_ctx *c, const double *rho, double *accr);
 /* the DAMPTOVISCOUS 1-D disc: one step of dt (none for dt <= 0), then its
  * Sigma and vr (nrad1d values each) */
 void or_solve_1d_viscous(or_ctx *c, double dt, double *sigma, double *vrad);
+/* the 1-D disc's Rmed1D, Sigma1D, Vrad1D (nrad1d values each; NULL skips):
+ * set as the restart read sets them, and read back */
+void or_set_visc1d(or_ctx *c, const double *rmed, const double *sigma, const double *vrad);
+void or_get_visc1d(or_ctx *c, double *rmed, double *sigma, double *vrad);
 
 /* AlgoGas; returns the number of sub-steps taken */
 int or_algogas(or_ctx *c, double DT);

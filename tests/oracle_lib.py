@@ -45,6 +45,8 @@ def lib():
             "or_accrete_onto_planets": (None, [vp, ctypes.c_double]),
             "or_sg_zero_mode": (None, [vp, dp, dp]),
             "or_solve_1d_viscous": (None, [vp, ctypes.c_double, dp, dp]),
+            "or_set_visc1d": (None, [vp, dp, dp, dp]),
+            "or_get_visc1d": (None, [vp, dp, dp, dp]),
             "or_algogas": (ctypes.c_int, [vp, ctypes.c_double]),
             "or_get_status": (None, [vp, ctypes.POINTER(Status)]),
             "or_get_field": (None, [vp, ctypes.c_int, dp]),
@@ -176,6 +178,16 @@ class Oracle:
         s, v = np.zeros(self.p.nrad1d), np.zeros(self.p.nrad1d)
         self.L.or_solve_1d_viscous(self.ctx, dt, ptr(s), ptr(v))
         return s, v
+
+    def set_visc1d(self, rmed=None, sigma=None, vrad=None):
+        self.L.or_set_visc1d(self.ctx, *[None if a is None else ptr(np.ascontiguousarray(a, dtype=np.float64))
+                                          for a in (rmed, sigma, vrad)])
+
+    def visc1d(self):
+        """(Rmed1D, Sigma1D, Vrad1D) with no step taken."""
+        r, s, v = (np.zeros(self.p.nrad1d) for _ in range(3))
+        self.L.or_get_visc1d(self.ctx, ptr(r), ptr(s), ptr(v))
+        return r, s, v
 
     def sg_zero_mode(self, rho):
         out = np.zeros(self.nr)

@@ -117,3 +117,81 @@ def test_restart_energy_from_temperature(tmp_path, oracle_built):
     back = output.read_restart(tmp_path, 2, p, o.tables()[1])
     n = p.nrad * p.nsec
     assert np.abs(back["energy"][:n] - st["energy"][:n]).max() <= 1e-14 * np.abs(st["energy"][:n]).max()
+
+
+def _libc_format(fmt, *vals):
+    libc = ctypes.CDLL(None)
+    buf = ctypes.create_string_buffer(256)
+    args = [ctypes.c_double(v) if isinstance(v, float) else ctypes.c_int(v) for v in vals]
+    libc.snprintf(buf, 256, fmt, *args)
+    return buf.value.decode()
+
+
+def _visc1d_case():
+    return gas_planet(nrad=32, nsec=96, damp_to=2, bc1d=1, wave_killing_zones=1, wkz_rmin=0.55, wkz_rmax=2.2)
+
+
+def test_visc1d_files_match_the_c_formats(tmp_path, oracle_built):
+    """Write1DViscProfiles' "%lg\\t%lg\\n" lines, dims1D.dat's eight "%d\\t"
+    and used_rad1D.dat's "%f\\n" (src/SideEuler.c:1504-1513,1564-1585),
+    compared byte for byte with the C library's own formatting of the same
+    values; the 1-D radii give back the oracle's Rmed1D."""
+    p, pl = _visc1d_case()
+    o = Oracle(p)
+    o.initial_state(pl)
+    o.algogas(DT)
+    rmed, sig, vr = o.visc1d()
+    output.write_visc1d_profiles(tmp_path, 4, rmed, sig, vr)
+    for base, vals in (("gasdens", sig), ("gasvrad", vr)):
+        want = "".join(_libc_format(b"%lg\t%lg\n", float(a), float(b)) for a, b in zip(rmed, vals))
+        assert (tmp_path / f"{base}.ascii_rad.4.dat").read_text() == want
+    output.write_dims_1d(tmp_path, p.nrad1d)
+    assert (tmp_path / "dims1D.dat").read_text() == _libc_format(b"%d\t" * 8 + b"\n", *([p.nrad1d] * 8))
+    radii = output.radii_1d(p.rmin1d, p.rmax1d, p.nrad1d)
+    output.write_used_rad_1d(tmp_path, radii)
+    assert (tmp_path / "used_rad1D.dat").read_text() == "".join(_libc_format(b"%f\n", float(r)) for r in radii)
+    # the cell centres of src/SideEuler.c:1472-1473 from these radii
+    a, b = radii[:-1], radii[1:]
+    rm = 2.0 / 3.0 * (b * b * b - a * a * a) / (b * b - a * a)
+    assert (np.abs(rm - rmed) / rmed).max() <= 4e-15
+
+
+def test_visc1d_restart_read(tmp_path, oracle_built):
+    """The restart branch of InitializeOneDViscousEvolution
+    (src/SideEuler.c:1489-1502): Rmed1D and Sigma1D become the values the C
+    library's strtod reads from the 6-digit gasdens.ascii_rad file, Vrad1D
+    keeps its t = 0 profile (the reference opens the vr file for writing).
+    A restarted oracle continues within the text's precision of the run it
+    came from."""
+    p, pl = _visc1d_case()
+    o = Oracle(p)
+    o.initial_state(pl)
+    for _ in range(2):
+        o.algogas(DT)
+    rmed, sig, vr = o.visc1d()
+    output.write_visc1d_profiles(tmp_path, 2, rmed, sig, vr)
+    rm2, sg2 = output.read_visc1d_restart(tmp_path, 2, p.nrad1d)
+    libc = ctypes.CDLL(None)
+    libc.strtod.restype = ctypes.c_double
+    libc.strtod.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    toks = (tmp_path / "gasdens.ascii_rad.2.dat").read_bytes().split()
+    want = np.array([libc.strtod(t, None) for t in toks]).reshape(-1, 2)
+    assert np.array_equal(rm2, want[:, 0]) and np.array_equal(sg2, want[:, 1])
+    assert np.abs(rm2 - rmed).max() <= 5e-6 * rmed.max() and np.abs(sg2 - sig).max() <= 5e-6 * sig.max()
+    assert not np.array_equal(rm2, rmed)  # the text rounds Rmed1D
+    r = Oracle(p)
+    r.initial_state(pl)
+    _, _, vr0 = r.visc1d()
+    r.set_state(o.state())
+    r.set_visc1d(rm2, sg2)
+    a_r, a_s, a_v = r.visc1d()
+    assert np.array_equal(a_r, rm2) and np.array_equal(a_s, sg2) and np.array_equal(a_v, vr0)
+    assert not np.array_equal(vr0, vr)
+    assert r.algogas(DT) == o.algogas(DT)
+    rs, os_ = r.state(), o.state()
+    n = p.nrad * p.nsec
+    for k in ("rho", "vt"):
+        assert np.abs(rs[k][:n] - os_[k][:n]).max() <= 1e-4 * np.abs(os_[k][:n]).max(), k
+    _, s_r, _ = r.visc1d()
+    _, s_o, _ = o.visc1d()
+    assert np.abs(s_r - s_o).max() <= 1e-4 * s_o.max()
