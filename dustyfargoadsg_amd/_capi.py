@@ -134,6 +134,37 @@ class DustParams(ctypes.Structure):
         return q
 
 
+DUST_SG_NO, DUST_SG_YES, DUST_SG_ZERO_MODE = 0, 1, 2
+
+
+class DustFluidParams(ctypes.Structure):
+    """dfargo_dust_fluid_params (the dust-fluid switches of src/var.c:136-186)."""
+    _fields_ = [
+        ("sizepart", c_double), ("rhopart", c_double),
+        ("factor_unit_mass", c_double), ("factor_unit_length", c_double),
+        ("dust_to_gas_density_ratio", c_double),
+        ("dviscosity", c_double), ("dalpha_viscosity", c_double),
+        ("daspect_ratio", c_double), ("dflaring_index", c_double), ("dust_diff_inn", c_double),
+        ("sfta", c_int), ("dust_feel_disk", c_int), ("dust_feedback", c_int),
+        ("dust_feel_sg", c_int), ("dust_diffusion", c_int),
+    ]
+
+
+def dust_fluid_default_params(**overrides):
+    """dfargo_dust_fluid_default_params() restated (src/var.c defaults)."""
+    f = DustFluidParams()
+    f.sizepart, f.rhopart = 1e-3, 1.0
+    f.factor_unit_mass, f.factor_unit_length = 1.0, 1.0
+    f.dust_to_gas_density_ratio = 0.01
+    f.daspect_ratio = 0.05
+    f.sfta, f.dust_feel_disk, f.dust_feel_sg = 1, 1, DUST_SG_YES
+    for k, v in overrides.items():
+        if not hasattr(f, k):
+            raise KeyError(f"unknown dust-fluid parameter {k!r}")
+        setattr(f, k, v)
+    return f
+
+
 class DustState(ctypes.Structure):
     """dfargo_dust_state: pointers to the nbpart-long particle arrays."""
     _fields_ = [(n, ctypes.POINTER(c_double)) for n in
@@ -264,6 +295,12 @@ def load(path=None):
         "dfargo_init_dust_system": (c_int, [P, ctypes.POINTER(DustParams), dp, ctypes.POINTER(DustState)]),
         "dfargo_attach_dust": (c_int, [vp, ctypes.POINTER(DustParams), ctypes.POINTER(DustState), c_double]),
         "dfargo_download_dust": (c_int, [vp, ctypes.POINTER(DustState)]),
+        "dfargo_dust_fluid_default_params": (None, [ctypes.POINTER(DustFluidParams)]),
+        "dfargo_check_dust_fluid_params": (c_int, [P, ctypes.POINTER(DustFluidParams), c_int]),
+        "dfargo_init_dust_fluid_density": (c_int, [P, ctypes.POINTER(DustFluidParams), dp]),
+        "dfargo_attach_dust_fluid": (c_int, [vp, ctypes.POINTER(DustFluidParams), dp]),
+        "dfargo_download_dust_fluid": (c_int, [vp, dp, dp, dp, dp]),
+        "dfargo_upload_dust_fluid": (c_int, [vp, dp, dp, dp]),
         "dfargo_upload_dust": (c_int, [vp, ctypes.POINTER(DustState)]),
         "dfargo_gas_budgets": (c_int, [vp, dp]),
         "dfargo_update_log": (c_int, [vp, c_int, dp, dp, ctypes.POINTER(c_int)]),
@@ -307,6 +344,8 @@ EXPORTED = [
     "dfargo_get_visc1d", "dfargo_set_visc1d", "dfargo_set_profiling", "dfargo_get_kernel_times", "dfargo_stream",
     "dfargo_dust_default_params", "dfargo_check_dust_params", "dfargo_init_dust_system",
     "dfargo_attach_dust", "dfargo_download_dust", "dfargo_upload_dust",
+    "dfargo_dust_fluid_default_params", "dfargo_check_dust_fluid_params", "dfargo_init_dust_fluid_density",
+    "dfargo_attach_dust_fluid", "dfargo_download_dust_fluid", "dfargo_upload_dust_fluid",
     "dfargo_gas_budgets", "dfargo_update_log", "dfargo_update_log_noalloc", "dfargo_orbital_elements",
     "dfargo_read_par", "dfargo_read_planet_config",
     "dfargo_get_tuning", "dfargo_set_tuning", "dfargo_autotune",

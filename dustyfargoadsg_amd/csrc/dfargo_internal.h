@@ -52,6 +52,7 @@ int dfargo_set_error(int code, const char *fmt, ...);
  * sub-step.  Lives in device memory; one copy per context. */
 typedef struct dfargo_ctrl {
   double DT, dtemp, dt, time, time0, omega_frame, mindt, maxdt, lost_mass, mass_taper;
+  double lost_massd; /* LostMassd: the dust fluid's mass through the inner edge */
   double dtnbody;
   double dop_x, dop_y, dopalt_x, dopalt_y, ind_x, ind_y;
   double barx[3], bary[3];

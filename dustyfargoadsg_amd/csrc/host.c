@@ -400,6 +400,57 @@ int dfargo_check_dust_params(const dfargo_params *p, const dfargo_dust_params *d
   return DFARGO_OK;
 }
 
+void dfargo_dust_fluid_default_params(dfargo_dust_fluid_params *f) {
+  memset(f, 0, sizeof(*f));
+  f->sizepart = 1e-3;
+  f->rhopart = 1.0;
+  f->factor_unit_mass = 1.0;
+  f->factor_unit_length = 1.0;
+  f->dust_to_gas_density_ratio = 0.01;
+  f->daspect_ratio = 0.05;
+  f->sfta = 1;
+  f->dust_feel_disk = 1;
+  f->dust_feel_sg = DFARGO_DUST_SG_YES;
+}
+
+int dfargo_check_dust_fluid_params(const dfargo_params *p, const dfargo_dust_fluid_params *f, int precision) {
+  if (!p || !f) return dfargo_set_error(DFARGO_ERR_ARG, "null parameter block");
+  int rc = dfargo_check_params(p);
+  if (rc) return rc;
+  if (!(f->sizepart > 0.0) || !(f->rhopart > 0.0))
+    return dfargo_set_error(DFARGO_ERR_ARG, "SIZEPART and RHOPART must be positive");
+  if (!(f->factor_unit_mass > 0.0) || !(f->factor_unit_length > 0.0))
+    return dfargo_set_error(DFARGO_ERR_ARG, "FACTORUNITMASS and FACTORUNITLENGTH must be positive");
+  if (!(f->dust_to_gas_density_ratio > 0.0))
+    return dfargo_set_error(DFARGO_ERR_ARG, "DUSTTOGASDENSITYRATIO must be positive");
+  if (f->dust_feel_sg < DFARGO_DUST_SG_NO || f->dust_feel_sg > DFARGO_DUST_SG_ZERO_MODE)
+    return dfargo_set_error(DFARGO_ERR_ARG, "unknown DUSTFEELSG %d", f->dust_feel_sg);
+  if (precision != DFARGO_FP64)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "the dust fluid runs with FP64 storage");
+  if (f->sfta && !f->dust_feel_disk) /* src/SourceEuler.c:906-913 leaves DVradInt unset */
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "SFTAPPROX YES with DUSTFEELDISK NO is not implemented");
+  if (p->nrad < 6) return dfargo_set_error(DFARGO_ERR_ARG, "the dust fluid needs NRAD >= 6");
+  return DFARGO_OK;
+}
+
+/* InitDustDensity (src/Pframeforce.c:344-380) with FillDSigma
+ * (src/Theo.c:95-101), no noise options */
+int dfargo_init_dust_fluid_density(const dfargo_params *p, const dfargo_dust_fluid_params *f, double *drho) {
+  dfargo_tables t;
+  int rc = dfargo_check_params(p);
+  if (rc) return rc;
+  if (!f || !drho) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  rc = dfargo_host_tables(p, &t);
+  if (rc) return rc;
+  memset(drho, 0, sizeof(double) * (size_t)(p->nrad + 1) * p->nsec);
+  for (int i = 0; i < p->nrad; i++) {
+    const double dsig = dfargo_sigma_profile(p, t.Rmed[i]) * f->dust_to_gas_density_ratio;
+    for (int j = 0; j < p->nsec; j++) drho[j + i * p->nsec] = dsig;
+  }
+  dfargo_free_tables(&t);
+  return DFARGO_OK;
+}
+
 /* InitDustSystem (src/Dsys.c:128-216), one process: sizes, then radii, then
  * azimuths from one drand48 stream seeded like srand48(seed) */
 int dfargo_init_dust_system(const dfargo_params *p, const dfargo_dust_params *d, const double *axi_sg_accr,

@@ -113,6 +113,22 @@ struct KArgs {
   // energy equation: the sound speed the sub-step started with, kept for
   // the particles when the CFL kernel recomputes cs
   void *cs_old;
+  // the dust fluid (DUSTFLUID YES, FP64 storage): switches, grain
+  // constants (internal density and radius in code units, the code-to-cgs
+  // surface density factor), state, the sub-step's Stokes numbers,
+  // DVradInt / DVthetaInt, stresses, the transport's buffers (dt_*, dz_*
+  // as t_*, z_* for the gas), ring buffers and ring tables
+  int df_on, df_sfta, df_feel_disk, df_feedback, df_feel_sg, df_diffusion, df_visc;
+  double df_rho_int, df_radius, df_to_cgs, df_diff_inn;
+  double *drho, *dvr, *dvt, *dstokes, *dvri, *dvti, *dtrr, *dtrp, *dtpp;
+  double *dt_rho, *dt_rp, *dt_rm, *dt_tp, *dt_tm, *dz_rho, *dz_rp, *dz_rm, *dz_tp, *dz_tm;
+  double *dvmoy, *dvmed, *dvt_part, *dlost_col, *df_axisg;
+  long *dnshift;
+  const double *df_cs, *df_nu, *df_sup15, *df_cavsup, *DSigmaMed, *DVthetaMed, *DVradMed;
+  // the transport kernels run on the dust fluid (Transportd): its lost
+  // mass, no FARGO margin; in the short-friction-time approximation the
+  // momenta only take the integer shift
+  int tr_dust, tr_skip_mom;
   // self-gravity buffers (complex T)
   // sg_kr, sg_kt: kernel spectra for wavenumbers 0..Ns/2, [k2][k1] (transposed)
   void *sg_z, *sg_w, *sg_kr, *sg_kt, *tw_ns, *tw_2n;
@@ -127,7 +143,9 @@ enum KernelId {
   KID_CFL, KID_CONTROL, KID_SOURCE, KID_ENERGY, KID_TRANSPORT_RAD, KID_TRANSPORT_AZI,
   KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER,
   KID_EVANESCENT, KID_NONREFLECTING,
-  KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL, KID_EOS_STEP, KID_VISC1D, KID_COUNT
+  KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL, KID_EOS_STEP, KID_VISC1D,
+  KID_DF_AXISG, KID_DF_SOURCE, KID_DF_STRESS, KID_DF_VISC, KID_DF_SUBSTEP2, KID_DF_TRANSPORT_RAD,
+  KID_DF_TRANSPORT_AZI, KID_DF_FINISH, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];
 

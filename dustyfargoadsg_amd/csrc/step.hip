@@ -18,7 +18,9 @@ const char *kKernelNames[KID_COUNT] = {
     "cfl_ring",    "control",        "source",  "energy",         "transport_rad",
     "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
     "dust_sort",     "dust_deposit", "dust_reorder", "evanescent", "nonreflecting",
-    "sgz_mean",      "sgz_conv",     "sgz_fill",     "eos_step",      "visc1d"};
+    "sgz_mean",      "sgz_conv",     "sgz_fill",     "eos_step",      "visc1d",
+    "df_axisg",      "df_source",    "df_stress",    "df_visc",       "df_substep2",
+    "df_transport_rad", "df_transport_azi", "df_finish"};
 
 // ------------------------------------------------------------------------
 // helpers

@@ -214,6 +214,30 @@ typedef struct dfargo_dust_params {
   int    dust_feel_sg_zero_mode;
 } dfargo_dust_params;
 
+/* ---- the dust fluid (DUSTFLUID YES) -------------------------------------- */
+
+/* The dust-fluid switches of the .par file (src/var.c:136-186, decoded as
+ * src/Interpret.c:253-342 does; dfargo_dust_fluid_default_params holds the
+ * defaults).  The fluid is one grain size, SIZEPART, whose Stokes number
+ * follows from the local gas state (src/SourceEuler.c:820-858). */
+enum dfargo_dust_feel_sg { DFARGO_DUST_SG_NO = 0, DFARGO_DUST_SG_YES = 1, DFARGO_DUST_SG_ZERO_MODE = 2 };
+typedef struct dfargo_dust_fluid_params {
+  double sizepart;                   /* SIZEPART (m, 1e-3)                    */
+  double rhopart;                    /* RHOPART (g/cm^3, 1)                   */
+  double factor_unit_mass, factor_unit_length; /* FACTORUNITMASS/LENGTH (1)  */
+  double dust_to_gas_density_ratio;  /* DUSTTOGASDENSITYRATIO (0.01)          */
+  double dviscosity, dalpha_viscosity; /* DVISCOSITY, DALPHAVISCOSITY (0)    */
+  double daspect_ratio;              /* DASPECTRATIO (0.05)                   */
+  double dflaring_index;             /* DFLARINGINDEX (0)                     */
+  double dust_diff_inn;              /* DUSTDIFFINN (0)                       */
+  int    sfta;                       /* SFTAPPROX (YES): dust velocities from
+                                        the short-friction-time approximation */
+  int    dust_feel_disk;             /* DUSTFEELDISK (YES)                    */
+  int    dust_feedback;              /* DUSTFEEDBACK (NO): drag on the gas    */
+  int    dust_feel_sg;               /* DUSTFEELSG, enum dfargo_dust_feel_sg  */
+  int    dust_diffusion;             /* DUSTDIFFUSION (NO)                    */
+} dfargo_dust_fluid_params;
+
 /* The particle arrays of DustSystem (src/types.h:114-146) the step reads
  * and writes, nbpart entries each. */
 typedef struct dfargo_dust_state {
@@ -500,6 +524,33 @@ int dfargo_attach_dust(dfargo_ctx *ctx, const dfargo_dust_params *d, const dfarg
 /* Particle arrays back to the host / from the host (nbpart entries). */
 int dfargo_download_dust(dfargo_ctx *ctx, dfargo_dust_state *st);
 int dfargo_upload_dust(dfargo_ctx *ctx, const dfargo_dust_state *st);
+
+/* ---- the dust fluid --------------------------------------------------------- */
+
+/* Defaults of src/var.c:136-186 for the dust fluid. */
+void dfargo_dust_fluid_default_params(dfargo_dust_fluid_params *f);
+/* Checks the dust-fluid switches against the gas parameters;
+ * DFARGO_ERR_UNSUPPORTED for what the GPU path does not implement: FP32
+ * storage, dust particles beside the fluid, SFTAPPROX YES with DUSTFEELDISK
+ * NO (the reference then leaves DVradInt unset), INNERBOUNDARYDUST OPEN,
+ * the energy equation's radiative/entropy diffusion (not built). */
+int dfargo_check_dust_fluid_params(const dfargo_params *p, const dfargo_dust_fluid_params *f, int precision);
+/* InitDustDensity (src/Pframeforce.c:344-380) without its noise options:
+ * DSigma(Rmed[i]) = Sigma(Rmed[i]) * DUSTTOGASDENSITYRATIO on every cell
+ * ((nrad + 1) * nsec values, ring nrad zero). */
+int dfargo_init_dust_fluid_density(const dfargo_params *p, const dfargo_dust_fluid_params *f, double *drho);
+/* Switches the dust fluid on for a context (before dfargo_init_euler) with
+ * its initial surface density ((nrad + 1) * nsec values, DRho of
+ * src/main.c:167-168).  dfargo_init_euler then runs InitDustStokesNumber
+ * and InitDustVelocities (src/SourceEuler.c:265-269) and, with
+ * DUSTFEEDBACK, the gas velocities of src/Pframeforce.c:609-634; every
+ * AlgoGas call advances the fluid with the gas. */
+int dfargo_attach_dust_fluid(dfargo_ctx *ctx, const dfargo_dust_fluid_params *f, const double *drho);
+/* The fluid's fields DRho, DVrad, DVtheta and the Stokes grid of the last
+ * sub-step ((nrad + 1) * nsec values each; any pointer may be NULL) to the
+ * host, and DRho, DVrad, DVtheta from the host. */
+int dfargo_download_dust_fluid(dfargo_ctx *ctx, double *drho, double *dvrad, double *dvtheta, double *stokes);
+int dfargo_upload_dust_fluid(dfargo_ctx *ctx, const double *drho, const double *dvrad, const double *dvtheta);
 
 #ifdef __cplusplus
 }

@@ -145,9 +145,31 @@ struct or_ctx {
   double *d_radsg, *d_azisg, *d_raddrag, *d_azidrag, *d_stokes;
   double *AziInf, *AziSup;
   double particles_mass_initial, particles_mass, min_stopping_time, unit_mass, unit_length;
+  /* the dust fluid (DUSTFLUID YES): DRho, DVrad, DVtheta and the module
+     grids of src/SourceEuler.c, src/Viscosity.c and src/TransportEuler.c
+     that carry its terms */
+  int df_on;
+  dfargo_dust_fluid_params fp;
+  double *drho, *dvr, *dvt, *stokes, *dpres, *dvradint, *dvthetaint, *dvradnew, *dvthetanew;
+  double *dqr, *dqt, *ddrr, *ddrp, *ddpp, *ddivv, *dtrr, *dtrp, *dtpp, *fdiffrp, *fdifftp;
+  double *dcs1, *DSigmaMed, *DVthetaMed, *DVradMed, *GlobalDCs, *AxiSGd, *dvmed, *dvmoy;
+  long *dnshift;
+  double LostMassd;
+  int df_skip_momenta; /* Transportd in the short-friction-time approximation */
 };
 
 static double min2(double a, double b) { return (b < a) ? b : a; }
+static void df_stokes(or_ctx *c);
+static void df_viscous_terms(or_ctx *c, const double *vr, const double *vt);
+static void df_update_viscosity(or_ctx *c, double *vr, double *vt, double DeltaT);
+static void df_init_gas_feedback(or_ctx *c);
+static double dfviscosity(const or_ctx *c, double rad);
+static int df_has_viscosity(const or_ctx *c);
+static void df_sound_speed_pressure(or_ctx *c);
+static void df_init_stokes(or_ctx *c);
+static void df_init_velocities(or_ctx *c);
+static void df_wall_boundaries(or_ctx *c);
+static void df_subkeplerian_boundary(or_ctx *c);
 static double max2(double a, double b) { return (b > a) ? b : a; }
 
 static double *grid(or_ctx *c) { return (double *)calloc(c->gsize, sizeof(double)); }
@@ -333,9 +355,15 @@ void or_destroy(or_ctx *c) {
                     &c->d_vth, &c->d_l, &c->d_vrg, &c->d_vtg, &c->d_densg, &c->d_csg,
                     &c->d_radgradp, &c->d_azigradp, &c->d_radind, &c->d_aziind, &c->d_radsg,
                     &c->d_azisg, &c->d_raddrag, &c->d_azidrag, &c->d_stokes, &c->AziInf,
-                    &c->AziSup};
+                    &c->AziSup, &c->drho, &c->dvr, &c->dvt, &c->stokes, &c->dpres,
+                    &c->dvradint, &c->dvthetaint, &c->dvradnew, &c->dvthetanew, &c->dqr,
+                    &c->dqt, &c->ddrr, &c->ddrp, &c->ddpp, &c->ddivv, &c->dtrr, &c->dtrp,
+                    &c->dtpp, &c->fdiffrp, &c->fdifftp, &c->dcs1, &c->DSigmaMed,
+                    &c->DVthetaMed, &c->DVradMed, &c->GlobalDCs, &c->AxiSGd, &c->dvmed,
+                    &c->dvmoy};
   for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); k++) free(*all[k]);
   free(c->nshift);
+  free(c->dnshift);
   free(c->nosplit);
   free(c);
 }
@@ -1064,6 +1092,9 @@ void or_get_visc1d(or_ctx *c, double *rmed, double *sigma, double *vrad) {
 static void correct_vtheta(or_ctx *c, double domega) {
   for (int i = 0; i < c->nr; i++)
     for (int j = 0; j < c->ns; j++) c->vt[i * c->ns + j] -= domega * c->Rmed[i];
+  if (c->df_on) /* CorrectVtheta (DVtheta, domega), src/SourceEuler.c:521 */
+    for (int i = 0; i < c->nr; i++)
+      for (int j = 0; j < c->ns; j++) c->dvt[i * c->ns + j] -= domega * c->Rmed[i];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1321,6 +1352,11 @@ static void compute_selfgravity(or_ctx *c, double *vrad, double *vtheta, double 
   memcpy(c->gr, c->sg_accr, sizeof(double) * nr * ns);
   memcpy(c->gtheta, c->sg_acct, sizeof(double) * nr * ns);
   if (!update) return;
+  /* update_sgvelocity (src/sgupdate.c:3-72) with the dust fluid's kicks:
+     DUSTFEELSG Y takes the full accelerations, Z the radial one of the
+     ring-mean SG_Accr */
+  const int dsg = c->df_on ? c->fp.dust_feel_sg : DFARGO_DUST_SG_NO;
+  if (dsg == DFARGO_DUST_SG_ZERO_MODE && !c->p.sg_zero_mode) make_1d_profile(c, c->sg_accr, c->AxiSGd);
   for (int i = 0; i < nr; i++)
     for (int j = 0; j < ns; j++) {
       int l = i * ns + j;
@@ -1329,11 +1365,22 @@ static void compute_selfgravity(or_ctx *c, double *vrad, double *vtheta, double 
                            (c->Rmed[i] - c->Rinf[i]) * c->sg_accr[l - ns]) * c->InvDiffRmed[i];
         c->radsgacc[l] = radsgacc;
         vrad[l] += Dt * radsgacc;
+        if (c->p.sg_zero_mode) {
+          if (dsg != DFARGO_DUST_SG_NO) c->dvradint[l] += Dt * radsgacc;
+        } else if (dsg == DFARGO_DUST_SG_YES) {
+          c->dvradint[l] += Dt * radsgacc;
+        } else if (dsg == DFARGO_DUST_SG_ZERO_MODE) {
+          const double axi = ((c->Rinf[i] - c->Rmed[i - 1]) * c->AxiSGd[i] +
+                              (c->Rmed[i] - c->Rinf[i]) * c->AxiSGd[i - 1]) * c->InvDiffRmed[i];
+          c->radsgacc[l] = axi;
+          c->dvradint[l] += Dt * axi;
+        }
       }
       int lm1 = i * ns + (j == 0 ? ns - 1 : j - 1);
       double azisgacc = 0.5 * (c->sg_acct[l] + c->sg_acct[lm1]);
       c->azisgacc[l] = azisgacc;
       vtheta[l] += Dt * azisgacc;
+      if (!c->p.sg_zero_mode && dsg == DFARGO_DUST_SG_YES) c->dvthetaint[l] += Dt * azisgacc;
     }
 }
 
@@ -1380,6 +1427,7 @@ static void init_gas_velocities(or_ctx *c) {
         vr[l] -= 3.0 * viscosity / r * (-p->sigma_slope + .5);
     }
   }
+  if (c->df_on && c->fp.dust_feedback) df_init_gas_feedback(c);
   for (int i = 0; i < nr; i++) {
     c->VthetaMed[i] = vt[i * ns] + c->Rmed[i] * c->OmegaFrame;
     c->VradMed[i] = vr[i * ns];
@@ -1415,6 +1463,11 @@ int or_init_euler(or_ctx *c, const double *rho, double *vrad, double *vtheta,
   if (c->p.self_gravity) { /* src/main.c:178-186 */
     compute_selfgravity(c, c->vr, c->vt, 0.0, 0);
     init_planetarysys_with_sg(c);
+  }
+  if (c->df_on) { /* src/SourceEuler.c:265-269 */
+    df_sound_speed_pressure(c);
+    df_init_stokes(c);
+    df_init_velocities(c);
   }
   compute_sound_speed(c);
   compute_pressure(c);
@@ -1468,15 +1521,28 @@ int or_condition_cfl(or_ctx *c, double deltaT) {
     for (int j = 0; j < ns; j++) Vmoy[i] += vt[j + i * ns];
     Vmoy[i] /= (double)ns;
   }
+  /* the dust fluid's terms (src/SourceEuler.c:1352-1361,1376-1384,1430-1471,1486-1491) */
+  const int df = c->df_on, sfta = c->fp.sfta;
+  const double *dustvt = c->dvt, *dustvr = c->dvr;
+  double *DVres = (double *)malloc(sizeof(double) * (ns + 1)), dviscosity = 0.0;
+  if (df)
+    for (int i = 0; i < nr; i++) {
+      c->dvmoy[i] = 0.0;
+      for (int j = 0; j < ns; j++) c->dvmoy[i] += dustvt[j + i * ns];
+      c->dvmoy[i] /= (double)ns;
+    }
   for (int i = 1; i < nr; i++) {
     if (has_viscosity(&c->p)) viscosity = or_fviscosity(c, c->Rmed[i]);
+    if (df_has_viscosity(c)) dviscosity = dfviscosity(c, c->Rmed[i]);
     double dxrad = c->Rsup[i] - c->Rinf[i];
     double dxtheta = c->Rmed[i] * pm / (double)ns;
     for (int j = 0; j < ns; j++) {
       int l = j + i * ns;
       Vres[j] = c->p.fast_transport ? vt[l] - Vmoy[i] : vt[l];
+      if (df) DVres[j] = c->p.fast_transport ? dustvt[l] - c->dvmoy[i] : dustvt[l];
     }
     Vres[ns] = Vres[0];
+    DVres[ns] = DVres[0];
     for (int j = 0; j < ns; j++) {
       int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
       double cs = c->cs[l];
@@ -1492,8 +1558,34 @@ int or_condition_cfl(or_ctx *c, double deltaT) {
         invdt5 = viscosity * 4.0 / pow(min2(dxrad, dxtheta), 2.0);
       else
         invdt5 = 1e-10;
-      dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 +
-                              invdt4 * invdt4 + invdt5 * invdt5 + invdt6 * invdt6);
+      if (df) {
+        double invdt7 = 1.e-10, invdt8 = 1.e-10, invdt9 = 1.e-10, invdt10 = 1.e-10, invdt11, invdt12;
+        const double csd = c->dcs1[i];
+        if (!sfta) {
+          invdt7 = csd / (min2(dxrad, dxtheta));
+          invdt8 = fabs(dustvr[l] + 1e-15) / dxrad;
+          invdt9 = fabs(DVres[j]) / dxtheta;
+          invdt10 = c->fp.dust_feel_disk ? 1. / (c->stokes[l] + 1.e-10) * sqrt(G * 1.0 / c->Rmed[i]) / c->Rmed[i]
+                                         : 1.e-10;
+        }
+        double ddvr = dustvr[lip] - dustvr[l], ddvt = dustvt[ljp] - dustvt[l];
+        if (ddvr >= 0.0) ddvr = 1e-10; else ddvr = -ddvr;
+        if (ddvt >= 0.0) ddvt = 1e-10; else ddvt = -ddvt;
+        invdt11 = max2(ddvr / dxrad, ddvt / dxtheta);
+        invdt11 *= 4.0 * CVNR * CVNR;
+        if (dviscosity != 0.0)
+          invdt12 = dviscosity * 4.0 / pow(min2(dxrad, dxtheta), 2.0);
+        else
+          invdt12 = 1e-10;
+        const double dtg = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 +
+                                              invdt4 * invdt4 + invdt5 * invdt5 + invdt6 * invdt6);
+        const double dtd = CFLSECURITY / sqrt(invdt7 * invdt7 + invdt8 * invdt8 + invdt9 * invdt9 +
+                                              invdt10 * invdt10 + invdt11 * invdt11 + invdt12 * invdt12);
+        dt = min2(dtg, dtd);
+        dt = max2(dt, 1.e-10);
+      } else
+        dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 +
+                                invdt4 * invdt4 + invdt5 * invdt5 + invdt6 * invdt6);
       if (dt < newdt) newdt = dt;
     }
   }
@@ -1502,6 +1594,13 @@ int or_condition_cfl(or_ctx *c, double deltaT) {
                 fabs(Vmoy[i] * c->InvRmed[i] - Vmoy[i + 1] * c->InvRmed[i + 1]);
     if (dt < newdt) newdt = dt;
   }
+  if (df && !sfta)
+    for (int i = 0; i < nr - 1; i++) {
+      double dt = pm * CFLSECURITY / (double)c->ns /
+                  fabs(c->dvmoy[i] * c->InvRmed[i] - c->dvmoy[i + 1] * c->InvRmed[i + 1]);
+      if (dt < newdt) newdt = dt;
+    }
+  free(DVres);
   /* the minimum stopping time of the last interpolation resolved by two
      steps (src/SourceEuler.c:1501-1507) */
   if (c->dust_on && c->np != 0 && c->dp.dust_feedback) {
@@ -1518,13 +1617,20 @@ int or_condition_cfl(or_ctx *c, double deltaT) {
 /* source steps                                                               */
 /* ------------------------------------------------------------------------ */
 
-/* SubStep1 (src/SourceEuler.c:765-1009), gas without dust fluid */
+/* SubStep1 (src/SourceEuler.c:765-1009), with the dust fluid's Stokes
+   number, kicks and drag (src/SourceEuler.c:805-858,885-922,961-999) */
 void or_substep1(or_ctx *c, double dt) {
   const dfargo_params *p = &c->p;
+  const dfargo_dust_fluid_params *fp = &c->fp;
   int nr = c->nr, ns = c->ns;
   const double *rho = c->rho, *vrad = c->vr, *vtheta = c->vt, *Pot = c->pot, *Press = c->pres;
   double *vradint = c->vradint, *vthetaint = c->vthetaint;
-  for (int i = 1; i < nr; i++)
+  const int df = c->df_on;
+  const double *drho = c->drho, *dvrad = c->dvr, *dvtheta = c->dvt, *dPress = c->dpres, *St = c->stokes;
+  double *dvradint = c->dvradint, *dvthetaint = c->dvthetaint;
+  if (df) df_stokes(c);
+  for (int i = 1; i < nr; i++) {
+    const double invomega = pow(c->Rmed[i], 1.5);
     for (int j = 0; j < ns; j++) {
       int l = j + i * ns, lim = l - ns, ljp = (j == ns - 1) ? i * ns : l + 1;
       double gradp = (Press[l] - Press[lim]) * 2.0 / (rho[l] + rho[lim]) * c->InvDiffRmed[i];
@@ -1536,11 +1642,32 @@ void or_substep1(or_ctx *c, double dt) {
       vt2 = vt2 * vt2;
       vradint[l] = vrad[l] + dt * (-gradp - gradphi + vt2 * c->InvRinf[i]);
       if (c->dust_on && c->dp.dust_feedback) vradint[l] += dt * c->radfbacc[l];
+      if (df) {
+        double ts = St[l] * invomega;
+        if (!fp->sfta) {
+          double dgradp = (dPress[l] - dPress[lim]) * 2.0 / (drho[l] + drho[lim]) * c->InvDiffRmed[i];
+          double dvt2 = dvtheta[l] + dvtheta[ljp] + dvtheta[lim] + dvtheta[ljp - ns];
+          dvt2 = dvt2 / 4.0 + c->Rinf[i] * c->OmegaFrame;
+          dvt2 = dvt2 * dvt2;
+          dvradint[l] = dvrad[l] + dt * (-gradphi - dgradp + dvt2 * c->InvRinf[i]);
+          if (fp->dust_feel_disk) dvradint[l] = (dvradint[l] * ts + vrad[l] * dt) / (ts + dt);
+        } else if (fp->dust_feel_disk) {
+          if (fp->dust_feedback) ts /= (1.0 + drho[l] / rho[l]);
+          dvradint[l] = vrad[l] + ts * gradp;
+        }
+        if (fp->dust_feedback) {
+          ts = St[l] * invomega;
+          c->radfbacc[l] = (drho[l] / rho[l]) * (dvrad[l] - vrad[l]) / (ts + dt / 2.);
+          vradint[l] = (vradint[l] * ts + (drho[l] / rho[l]) * dvrad[l] * dt) / (ts + dt * (drho[l] / rho[l]));
+        }
+      }
     }
+  }
   for (int i = 0; i < nr; i++) {
     double dxtheta = (p->pmax - p->pmin) / (double)ns * c->Rmed[i];
     double invdxtheta = 1.0 / dxtheta;
     double supp_torque = p->imposed_disk_drift * 0.5 * pow(c->Rmed[i], -2.5) * p->sigma0 / c->SigmaMed[i];
+    const double invomega = pow(c->Rmed[i], 1.5);
     for (int j = 0; j < ns; j++) {
       int l = j + i * ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
       double gradp = (Press[l] - Press[ljm]) * 2.0 / (rho[l] + rho[ljm]) * invdxtheta;
@@ -1550,12 +1677,31 @@ void or_substep1(or_ctx *c, double dt) {
       vthetaint[l] = vtheta[l] - dt * (gradp + gradphi);
       vthetaint[l] += dt * supp_torque;
       if (c->dust_on && c->dp.dust_feedback) vthetaint[l] += dt * c->azifbacc[l];
+      if (df) {
+        double ts = St[l] * invomega;
+        if (!fp->sfta) {
+          double dgradp = (dPress[l] - dPress[ljm]) * 2.0 / (drho[l] + drho[ljm]) * invdxtheta;
+          dvthetaint[l] = dvtheta[l] - dt * (gradphi + dgradp);
+          if (fp->dust_feel_disk) dvthetaint[l] = (dvthetaint[l] * ts + vtheta[l] * dt) / (ts + dt);
+        } else if (fp->dust_feel_disk) {
+          if (fp->dust_feedback) ts /= (1.0 + drho[l] / rho[l]);
+          dvthetaint[l] = vtheta[l] + ts * gradp;
+        }
+        if (fp->dust_feedback) {
+          ts = St[l] * invomega;
+          c->azifbacc[l] = (drho[l] / rho[l]) * (dvtheta[l] - vtheta[l]) / (ts + dt / 2.);
+          vthetaint[l] =
+              (vthetaint[l] * ts + (drho[l] / rho[l]) * dvtheta[l] * dt) / (ts + dt * (drho[l] / rho[l]));
+        }
+      }
     }
   }
   if (p->self_gravity) compute_selfgravity(c, vradint, vthetaint, dt, 1);
-  if (has_viscosity(p)) {
+  if (has_viscosity(p) || df_has_viscosity(c)) {
     compute_viscous_terms(c, vradint, vthetaint);
+    if (df) df_viscous_terms(c, dvradint, dvthetaint);
     update_velocities_viscosity(c, vradint, vthetaint, dt);
+    if (df) df_update_viscosity(c, dvradint, dvthetaint, dt);
   }
 }
 
@@ -1733,10 +1879,12 @@ static void van_leer_theta(or_ctx *c, const double *vt, double *qb, double dt, i
 static void quantities_advection(or_ctx *c, const double *vt, double dt, int uniform) {
   compute_star_theta(c, c->rho, vt, c->rhostar, dt);
   memcpy(c->rhoint, c->rho, sizeof(double) * (c->nr + 1) * c->ns);
-  van_leer_theta(c, vt, c->rp, dt, uniform);
-  van_leer_theta(c, vt, c->rm, dt, uniform);
-  van_leer_theta(c, vt, c->tp, dt, uniform);
-  van_leer_theta(c, vt, c->tm, dt, uniform);
+  if (!c->df_skip_momenta) {
+    van_leer_theta(c, vt, c->rp, dt, uniform);
+    van_leer_theta(c, vt, c->rm, dt, uniform);
+    van_leer_theta(c, vt, c->tp, dt, uniform);
+    van_leer_theta(c, vt, c->tm, dt, uniform);
+  }
   if (c->p.energy_equation) van_leer_theta(c, vt, c->en, dt, uniform);
   van_leer_theta(c, vt, c->rho, dt, uniform);
 }
@@ -2023,13 +2171,18 @@ static void nonreflecting_boundary(or_ctx *c) {
 
 /* ApplyBoundaryCondition (src/SideEuler.c:1102-1131) */
 static void apply_boundary_condition(or_ctx *c) {
-  if (c->p.boundary == DFARGO_BC_WALL) wall_boundaries(c);
+  if (c->p.boundary == DFARGO_BC_WALL) {
+    wall_boundaries(c);
+    if (c->df_on) df_wall_boundaries(c);
+  }
   subkeplerian_boundary(c); /* unconditional: src/SideEuler.c:1110-1112 */
+  if (c->df_on) df_subkeplerian_boundary(c);
   if (c->p.boundary == DFARGO_BC_OPEN) open_boundary(c);
   if (c->p.boundary == DFARGO_BC_NONREFLECTING) {
     if (c->p.energy_equation) compute_sound_speed(c); /* src/SideEuler.c:1121-1122 */
     nonreflecting_boundary(c);
     subkeplerian_boundary(c);
+    if (c->df_on) df_subkeplerian_boundary(c);
   }
 }
 
@@ -2094,6 +2247,26 @@ static void evanescent_boundary(or_ctx *c, double step) {
       d0 /= (double)ns;
       e0 /= (double)ns;
     }
+    /* the dust fluid's targets (src/SideEuler.c:844-848,887-926): the
+       initial profiles, or with DampToAxi its ring means */
+    double dvr0 = 0.0, dvt0 = 0.0, dd0 = 0.0;
+    if (c->df_on) {
+      if (p->damp_to == DFARGO_DAMP_TO_AXI) {
+        for (int j = 0; j < ns; j++) {
+          const int l = i * ns + j;
+          dvr0 += c->dvr[l];
+          dvt0 += c->dvt[l];
+          dd0 += c->drho[l];
+        }
+        dvr0 /= (double)ns;
+        dvt0 /= (double)ns;
+        dd0 /= (double)ns;
+      } else {
+        dvt0 = c->DVthetaMed[i] - R * c->OmegaFrame;
+        dvr0 = c->DVradMed[i];
+        dd0 = c->DSigmaMed[i];
+      }
+    }
     const int edge = p->zero_vrad_bc && (((R < p->wkz_rmin) && (i == 0 || i == 1)) ||
                                          ((R > p->wkz_rmax) && (i == nr - 1)));
     for (int j = 0; j < ns; j++) {
@@ -2102,6 +2275,11 @@ static void evanescent_boundary(or_ctx *c, double step) {
       c->vt[l] = (c->vt[l] + lambda * vt0) / (1.0 + lambda);
       c->rho[l] = (c->rho[l] + lambda * d0) / (1.0 + lambda);
       if (p->energy_equation) c->en[l] = (c->en[l] + lambda * e0) / (1.0 + lambda);
+      if (c->df_on) {
+        c->dvr[l] = (c->dvr[l] + lambda * dvr0) / (1.0 + lambda);
+        c->dvt[l] = (c->dvt[l] + lambda * dvt0) / (1.0 + lambda);
+        c->drho[l] = (c->drho[l] + lambda * dd0) / (1.0 + lambda);
+      }
       if (edge) {
         c->vr[l] = 0.0;
         c->rho[l] = d0;
@@ -2591,6 +2769,403 @@ void or_dust_get_drag(or_ctx *c, double *raddrag, double *azidrag) {
 double or_dust_min_stopping_time(or_ctx *c) { return c->min_stopping_time; }
 double or_dust_particles_mass(or_ctx *c) { return c->particles_mass; }
 
+/* ------------------------------------------------------------------------ */
+/* the dust fluid (DUSTFLUID YES)                                             */
+/* ------------------------------------------------------------------------ */
+
+/* DAspectRatio (src/Viscosity.c:123-138), LAMBDADOUBLING = 0 */
+static double daspect_ratio(const or_ctx *c, double rad) {
+  const dfargo_params *p = &c->p;
+  double ar = c->fp.daspect_ratio;
+  double rmin = p->transition_radius - p->transition_width * c->fp.daspect_ratio;
+  double rmax = p->transition_radius + p->transition_width * c->fp.daspect_ratio;
+  if (rad < rmin) ar *= p->transition_ratio;
+  if ((rad >= rmin) && (rad <= rmax)) ar *= exp((rmax - rad) / (rmax - rmin) * log(p->transition_ratio));
+  return ar;
+}
+
+/* DFViscosity (src/Viscosity.c:72-104), RELEASEDATEVISCOSITY = 0 */
+static double dfviscosity(const or_ctx *c, double rad) {
+  const dfargo_params *p = &c->p;
+  double viscosity = c->fp.dviscosity;
+  int i = 0;
+  if (c->fp.dalpha_viscosity != 0.0) {
+    while (c->GlobalRmed[i] < rad) i++;
+    viscosity = c->fp.dalpha_viscosity * c->GlobalDCs[i] * c->GlobalDCs[i] * pow(rad, 1.5);
+  }
+  double rmin = p->cavity_radius - p->cavity_width * c->fp.daspect_ratio;
+  double rmax = p->cavity_radius + p->cavity_width * c->fp.daspect_ratio;
+  if (rad < rmin) viscosity *= p->cavity_ratio;
+  if ((rad >= rmin) && (rad <= rmax)) viscosity *= exp((rmax - rad) / (rmax - rmin) * log(p->cavity_ratio));
+  return viscosity;
+}
+
+static int df_has_viscosity(const or_ctx *c) {
+  return c->df_on && ((c->fp.dalpha_viscosity != 0.0) || (c->fp.dviscosity != 0.0));
+}
+
+/* DSigma (src/Theo.c:66-85), no RestartWithNewDust */
+static double dsigma(const or_ctx *c, double r) { return sigma_profile(&c->p, r) * c->fp.dust_to_gas_density_ratio; }
+
+/* ComputeDustSoundSpeedAndPressure (src/SourceEuler.c:2353-2371); the
+   sound speed is one value per ring.  Its azimuthal mean
+   (mpi_make1Dprofile, GLOBAL_DustSoundSpeed of DFViscosity) is summed
+   cell by cell as the reference sums it. */
+static void df_sound_speed_pressure(or_ctx *c) {
+  for (int i = 0; i < c->nr; i++) {
+    const double r = c->Rmed[i];
+    c->dcs1[i] = daspect_ratio(c, r) * sqrt(G * 1.0 / r) * pow(r, c->fp.dflaring_index);
+    double s = 0.0;
+    for (int j = 0; j < c->ns; j++) {
+      const int l = i * c->ns + j;
+      c->dpres[l] = c->drho[l] * c->dcs1[i] * c->dcs1[i];
+      s += c->dcs1[i];
+    }
+    c->GlobalDCs[i] = s / (double)c->ns;
+  }
+}
+
+/* the code units of ComputeCodeUnits (src/LowTasks.c:204-212) and the
+   grain's internal density and radius in them (src/SourceEuler.c:812-818) */
+static void df_grain(const or_ctx *c, double *rho_int, double *radius, double *to_cgs) {
+  const double um = 1.9891e30 * c->fp.factor_unit_mass, ul = 1.49598e11 * c->fp.factor_unit_length;
+  *rho_int = c->fp.rhopart * 1000.0 * pow(ul, 3.) / um;
+  *radius = c->fp.sizepart / ul;
+  *to_cgs = 0.1 * um * pow(ul, -2.0);
+}
+
+/* InitDustStokesNumber (src/Pframeforce.c:836-874): zero relative velocity
+   (f_D = k_D = 1) */
+static void df_init_stokes(or_ctx *c) {
+  double rho_int, radius, to_cgs;
+  df_grain(c, &rho_int, &radius, &to_cgs);
+  const double um = 1.9891e30 * c->fp.factor_unit_mass, ul = 1.49598e11 * c->fp.factor_unit_length;
+  for (int i = 0; i < c->nr; i++)
+    for (int j = 0; j < c->ns; j++) {
+      const int l = j + i * c->ns;
+      double densg_cgs = 0.1 * c->rho[l] * um * pow(ul, -2.0);
+      double lambda = 3.34e-8 * pow(densg_cgs, -1.) * aspect_ratio(&c->p, c->Rmed[i]) * c->Rmed[i];
+      double Kn = 0.5 * lambda / radius, f_D = 1.0, k_D = 1.0;
+      double Cdrag = pow(3.0 * Kn + 1.0, 2.0) * pow(9.0 * Kn * Kn * f_D + 3.0 * Kn * k_D, -1.0);
+      c->stokes[l] = 0.5 * M_PI * Cdrag * rho_int * radius / c->rho[l];
+    }
+}
+
+/* The Stokes number of SubStep1 (src/SourceEuler.c:820-858): the
+   Paardekooper (2007) drag coefficient from the Knudsen number, the
+   relative Mach number and the Reynolds number, evaluated on the state
+   the sub-step starts from */
+static void df_stokes(or_ctx *c) {
+  double rho_int, radius, to_cgs;
+  df_grain(c, &rho_int, &radius, &to_cgs);
+  for (int i = 0; i < c->nr; i++)
+    for (int j = 0; j < c->ns; j++) {
+      const int l = j + i * c->ns;
+      const double densg_cgs = c->rho[l] * to_cgs;
+      const double lambda = 3.34e-8 * pow(densg_cgs, -1.) * aspect_ratio(&c->p, c->Rmed[i]) * c->Rmed[i];
+      const double Kn = 0.5 * lambda / radius;
+      const double dvr = c->dvr[l] - c->vr[l], dvt = c->dvt[l] - c->vt[l];
+      const double Mach = sqrt(dvr * dvr + dvt * dvt) / c->cs[l];
+      const double f_D = sqrt(1.0 + 9.0 * M_PI * Mach * Mach / 128.0);
+      const double Re = 3.0 * sqrt(M_PI / 8.0) * Mach / Kn;
+      double k_D = 0.0;
+      if (Re <= 500.0)
+        k_D = 1.0 + 0.15 * pow(Re, 0.687);
+      else {
+        if ((Re > 500.0) && (Re <= 1500.0)) k_D = 3.96e-6 * pow(Re, 2.4);
+        if (Re > 1500.0) k_D = 0.11 * Re;
+      }
+      const double Cdrag = (3.0 * Kn + 1.0) * (3.0 * Kn + 1.0) / (9.0 * Kn * Kn * f_D + 3.0 * Kn * k_D);
+      c->stokes[l] = 0.5 * M_PI * Cdrag * rho_int * radius / c->rho[l];
+    }
+}
+
+/* InitDustVelocities (src/Pframeforce.c:644-830) as src/SourceEuler.c:268
+   calls it, with the gas and dust densities in each other's places: in the
+   feedback branch its eps is gas over dust.  Every earlier assignment of
+   the routine is overwritten by its final loops, which alone are
+   restated. */
+static void df_init_velocities(or_ctx *c) {
+  const dfargo_params *p = &c->p;
+  const int nr = c->nr, ns = c->ns;
+  for (int i = 0; i < nr; i++) {
+    const double r = c->Rmed[i];
+    double vk = sqrt(G * 1.0 / r);
+    const double eta = -0.5 * pow(p->aspect_ratio, 2.0) * pow(r, 2.0 * p->flaring_index) *
+                       (-1. - p->sigma_slope + 2.0 * p->flaring_index);
+    if (p->self_gravity) vk = sqrt(G * 1.0 / r - r * c->AxiSG[i]);
+    for (int j = 0; j < ns; j++) {
+      const int l = i * ns + j;
+      const double St = c->stokes[l];
+      if (c->fp.dust_feedback) {
+        const double eps = c->rho[l] / c->drho[l];
+        c->dvr[l] = -2.0 * eta * vk / (St + pow(1.0 + eps, 2.0) / St);
+        c->dvt[l] = vk - eta * vk * (1.0 + eps) / (pow(St, 2.0) + pow(1.0 + eps, 2.0));
+      } else {
+        c->dvr[l] = -2.0 * eta * vk / (St + pow(St, -1.0));
+        c->dvt[l] = vk - eta * vk / (pow(St, 2.0) + 1.0);
+      }
+      c->dvt[l] -= c->OmegaFrame * r;
+    }
+  }
+  for (int i = 0; i < nr; i++) {
+    c->DVthetaMed[i] = c->dvt[i * ns] + c->Rmed[i] * c->OmegaFrame;
+    c->DVradMed[i] = c->dvr[i * ns];
+  }
+}
+
+/* the gas velocities of InitGasVelocities with dust feedback
+   (src/Pframeforce.c:609-634) */
+static void df_init_gas_feedback(or_ctx *c) {
+  const dfargo_params *p = &c->p;
+  for (int i = 0; i < c->nr; i++) {
+    const double r = c->Rmed[i];
+    double vk = sqrt(G * 1.0 / r);
+    const double eta = -0.5 * pow(p->aspect_ratio, 2.0) * pow(r, 2.0 * p->flaring_index) *
+                       (-1. - p->sigma_slope + 2.0 * p->flaring_index);
+    if (p->self_gravity) vk = sqrt(G * 1.0 / r - r * c->AxiSG[i]);
+    for (int j = 0; j < c->ns; j++) {
+      const int l = i * c->ns + j;
+      const double eps = c->drho[l] / c->rho[l], St = c->stokes[l];
+      c->vr[l] = 2.0 * eps * eta * vk * St / (St * St + pow(1.0 + eps, 2.0));
+      c->vt[l] = vk * (1.0 - eta) + eps * eta * vk * (1.0 + eps) / (pow(St, 2.0) + pow(1.0 + eps, 2.0));
+      c->vt[l] -= c->OmegaFrame * r;
+    }
+  }
+}
+
+/* The dust terms of ComputeViscousTerms (src/Viscosity.c:159-298) on the
+   dust velocities vr, vt: the divergence of ComputeDivergenceVelocity
+   (src/SourceEuler.c:2373-2412), the strain rates, then the stresses with
+   DFViscosity */
+static void df_viscous_terms(or_ctx *c, const double *vr, const double *vt) {
+  const int nr = c->nr, ns = c->ns;
+  const double dphi = (c->p.pmax - c->p.pmin) / (double)ns, invdphi = 1.0 / dphi, onethird = 1.0 / 3.0;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      c->ddivv[l] = (vr[lip] * c->Rsup[i] - vr[l] * c->Rinf[i]) * c->InvDiffRsup[i] * c->InvRmed[i];
+      c->ddivv[l] += (vt[ljp] - vt[l]) * invdphi * c->InvRmed[i];
+      c->ddrr[l] = (vr[lip] - vr[l]) * c->InvDiffRsup[i];
+      c->ddpp[l] = (vt[ljp] - vt[l]) * invdphi * c->InvRmed[i] + 0.5 * (vr[lip] + vr[l]) * c->InvRmed[i];
+    }
+  for (int i = 1; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1, lim = l - ns;
+      c->ddrp[l] = 0.5 * (c->Rinf[i] * (vt[l] * c->InvRmed[i] - vt[lim] * c->InvRmed[i - 1]) * c->InvDiffRmed[i] +
+                          (vr[l] - vr[ljm]) * invdphi * c->InvRinf[i]);
+    }
+  for (int i = 0; i < nr; i++) {
+    const double nu = df_has_viscosity(c) ? dfviscosity(c, c->Rmed[i]) : 0.0;
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns;
+      c->dtrr[l] = 2.0 * c->drho[l] * nu * (c->ddrr[l] - onethird * c->ddivv[l]);
+      c->dtpp[l] = 2.0 * c->drho[l] * nu * (c->ddpp[l] - onethird * c->ddivv[l]);
+    }
+  }
+  for (int i = 1; i < nr; i++) {
+    const double nu = df_has_viscosity(c) ? dfviscosity(c, c->Rmed[i]) : 0.0;
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, lim = l - ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1, ljmim = ljm - ns;
+      c->dtrp[l] = 2.0 * 0.25 * (c->drho[l] + c->drho[lim] + c->drho[ljm] + c->drho[ljmim]) * nu * c->ddrp[l];
+    }
+  }
+}
+
+/* the dust terms of UpdateVelocitiesWithViscosity (src/Viscosity.c:301-380) */
+static void df_update_viscosity(or_ctx *c, double *vr, double *vt, double DeltaT) {
+  const int nr = c->nr, ns = c->ns;
+  const double invdphi = 1.0 / ((c->p.pmax - c->p.pmin) / (double)ns);
+  const double *Trr = c->dtrr, *Trp = c->dtrp, *Tpp = c->dtpp, *rho = c->drho;
+  for (int i = 1; i < nr - 1; i++)
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, lip = l + ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      vt[l] += DeltaT * c->InvRmed[i] *
+               ((c->Rsup[i] * Trp[lip] - c->Rinf[i] * Trp[l]) * c->InvDiffRsup[i] + (Tpp[l] - Tpp[ljm]) * invdphi +
+                0.5 * (Trp[l] + Trp[lip])) /
+               (0.5 * (rho[l] + rho[ljm]));
+    }
+  for (int i = 1; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, lim = l - ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      vr[l] += DeltaT * c->InvRinf[i] *
+               ((c->Rmed[i] * Trr[l] - c->Rmed[i - 1] * Trr[lim]) * c->InvDiffRmed[i] + (Trp[ljp] - Trp[l]) * invdphi -
+                0.5 * (Tpp[l] + Tpp[lim])) /
+               (0.5 * (rho[l] + rho[lim]));
+    }
+}
+
+/* the dust terms of SubStep2 (src/SourceEuler.c:1035-1116): artificial
+   viscosity on DVradInt, DVthetaInt into DVradNew, DVthetaNew */
+static void df_substep2(or_ctx *c, double dt) {
+  const int nr = c->nr, ns = c->ns;
+  const double *drho = c->drho, *dvrad = c->dvradint, *dvtheta = c->dvthetaint;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      double ddv = dvrad[lip] - dvrad[l];
+      c->dqr[l] = (ddv < 0.0) ? CVNR * CVNR * drho[l] * ddv * ddv : 0.0;
+      ddv = dvtheta[ljp] - dvtheta[l];
+      c->dqt[l] = (ddv < 0.0) ? CVNR * CVNR * drho[l] * ddv * ddv : 0.0;
+    }
+  for (int i = 1; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, lim = l - ns;
+      c->dvradnew[l] =
+          dvrad[l] - dt * 2.0 / (drho[l] + drho[lim]) * (c->dqr[l] - c->dqr[lim]) * c->InvDiffRmed[i];
+    }
+  for (int i = 0; i < nr; i++) {
+    const double dxtheta = (c->p.pmax - c->p.pmin) / (double)ns * c->Rmed[i], invdxtheta = 1.0 / dxtheta;
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      c->dvthetanew[l] = dvtheta[l] - dt * 2.0 / (drho[l] + drho[ljm]) * (c->dqt[l] - c->dqt[ljm]) * invdxtheta;
+    }
+  }
+}
+
+/* Transportd (src/TransportEuler.c:553-568): the gas routines on the dust
+   fields, whose static work grids they share.  In the short-friction-time
+   approximation OneWindRadd and QuantitiesAdvectiond skip the momenta
+   (src/TransportEuler.c:575,778): they only take the integer shift, and
+   ComputeVelocities then divides them by the transported density.
+   VanLeerRadialdd, VanLeerThetad and VanLeerRadiald compute what
+   VanLeerRadial and VanLeerTheta do (their diffusion terms are comments),
+   so the gas routines are called on swapped fields; the FARGO mean and
+   shift are the dust's own. */
+static void swap_ptr(double **a, double **b) {
+  double *t = *a;
+  *a = *b;
+  *b = t;
+}
+static void df_transport(or_ctx *c, double dt) {
+  const int nr = c->nr;
+  long *tl;
+  const int energy = c->p.energy_equation;
+  const double margin = c->fargo_margin;
+  swap_ptr(&c->rho, &c->drho);
+  swap_ptr(&c->vr, &c->dvr);
+  swap_ptr(&c->vt, &c->dvt);
+  swap_ptr(&c->vmed, &c->dvmed);
+  tl = c->nshift; c->nshift = c->dnshift; c->dnshift = tl;
+  c->p.energy_equation = 0;
+  compute_lr_momenta(c);
+  compute_star_rad(c, c->rho, c->vr, c->rhostar, dt);
+  memcpy(c->rhoint, c->rho, sizeof(double) * (nr + 1) * c->ns);
+  if (!c->fp.sfta) {
+    van_leer_radial(c, c->rp, dt);
+    van_leer_radial(c, c->rm, dt);
+    van_leer_radial(c, c->tp, dt);
+    van_leer_radial(c, c->tm, dt);
+  }
+  c->LostMassd += van_leer_radial(c, c->rho, dt);
+  c->df_skip_momenta = c->fp.sfta;
+  one_wind_theta(c, dt);
+  c->df_skip_momenta = 0;
+  compute_velocities(c);
+  c->p.energy_equation = energy;
+  c->fargo_margin = margin;
+  tl = c->nshift; c->nshift = c->dnshift; c->dnshift = tl;
+  swap_ptr(&c->vmed, &c->dvmed);
+  swap_ptr(&c->vt, &c->dvt);
+  swap_ptr(&c->vr, &c->dvr);
+  swap_ptr(&c->rho, &c->drho);
+}
+
+/* Diffd (src/SourceEuler.c:2599-2656): turbulent diffusion of the dust
+   concentration with the gas viscosity and the Schmidt number
+   (1 + 4 St^2) / (1 + St^2)^2 */
+static void df_diffd(or_ctx *c, double dt) {
+  const int nr = c->nr, ns = c->ns;
+  const double *rho = c->rho, *St = c->stokes;
+  double *drho = c->drho;
+  const double dtheta = 2.0 * M_PI / (double)ns, invdtheta = 1.0 / dtheta;
+  double viscosityp = 0.0, viscosity = 0.0;
+  for (int i = 0; i < nr - 1; i++) {
+    if (has_viscosity(&c->p)) {
+      viscosityp = or_fviscosity(c, c->Rsup[i]);
+      viscosity = or_fviscosity(c, c->Rmed[i]);
+    }
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, lip = l + ns, ljp = (j == ns - 1) ? i * ns : l + 1;
+      double Dturb = viscosityp * (1.0 + 4.0 * St[lip] * St[lip]) * pow((1.0 + St[lip] * St[lip]), -2.0);
+      c->fdiffrp[l] = -Dturb * (rho[lip] + rho[l]) / 2. * (drho[lip] / rho[lip] - drho[l] / rho[l]) *
+                      c->InvDiffRmed[i + 1];
+      Dturb = viscosity * (1.0 + 4.0 * St[l] * St[l]) * pow((1.0 + St[l] * St[l]), -2.0);
+      c->fdifftp[l] = -Dturb * (rho[ljp] + rho[l]) / 2. * (drho[ljp] / rho[ljp] - drho[l] / rho[l]) * invdtheta /
+                      c->Rmed[i];
+    }
+  }
+  for (int i = 1; i < nr - 1; i++) {
+    if (!(c->Rmed[i] > c->fp.dust_diff_inn)) continue;
+    for (int j = 0; j < ns; j++) {
+      const int l = j + i * ns, lim = l - ns, ljm = (j == 0) ? i * ns + ns - 1 : l - 1;
+      const double frp = c->fdiffrp[l], frm = c->fdiffrp[lim], ftp = c->fdifftp[l], ftm = c->fdifftp[ljm];
+      drho[l] = drho[l] - (c->Rsup[i] * dtheta * frp - c->Rinf[i] * dtheta * frm + (c->Rsup[i] - c->Rinf[i]) * (ftp - ftm)) *
+                              dt * c->InvSurf[i];
+    }
+  }
+}
+
+/* the dust's part of ApplyWallBoundaries (src/SideEuler.c:1066-1091) */
+static void df_wall_boundaries(or_ctx *c) {
+  const int nr = c->nr, ns = c->ns;
+  int i = 2;
+  for (int j = 0; j < ns; j++) {
+    c->dvr[(i - 1) * ns + j] = 0.0;
+    c->dvr[(i - 2) * ns + j] = -c->dvr[i * ns + j];
+    c->drho[(i - 1) * ns + j] = c->DSigmaMed[i - 1];
+    c->drho[(i - 2) * ns + j] = c->DSigmaMed[i - 2];
+  }
+  i = nr - 3;
+  for (int j = 0; j < ns; j++) {
+    c->dvr[(i + 2) * ns + j] = 0.0;
+    c->dvr[(i + 3) * ns + j] = -c->dvr[(i + 1) * ns + j];
+    c->drho[(i + 1) * ns + j] = c->DSigmaMed[i + 1];
+    c->drho[(i + 2) * ns + j] = c->DSigmaMed[i + 2];
+  }
+}
+
+/* the dust's part of ApplySubKeplerianBoundary (src/SideEuler.c:1025-1041) */
+static void df_subkeplerian_boundary(or_ctx *c) {
+  const int nr = c->nr, ns = c->ns;
+  for (int j = 0; j < ns; j++) c->dvt[j] = c->DVthetaMed[0] - c->Rmed[0] * c->OmegaFrame;
+  for (int j = 0; j < ns; j++) c->dvt[(nr - 1) * ns + j] = c->DVthetaMed[nr - 1] - c->Rmed[nr - 1] * c->OmegaFrame;
+}
+
+int or_dust_fluid_attach(or_ctx *c, const dfargo_dust_fluid_params *f, const double *drho) {
+  c->df_on = 1;
+  c->fp = *f;
+  double **g[] = {&c->drho, &c->dvr, &c->dvt, &c->stokes, &c->dpres, &c->dvradint, &c->dvthetaint,
+                  &c->dvradnew, &c->dvthetanew, &c->dqr, &c->dqt, &c->ddrr, &c->ddrp, &c->ddpp,
+                  &c->ddivv, &c->dtrr, &c->dtrp, &c->dtpp, &c->fdiffrp, &c->fdifftp};
+  for (size_t k = 0; k < sizeof(g) / sizeof(g[0]); k++)
+    if (!*g[k]) *g[k] = grid(c);
+  double **t[] = {&c->dcs1, &c->DSigmaMed, &c->DVthetaMed, &c->DVradMed, &c->GlobalDCs, &c->AxiSGd,
+                  &c->dvmed, &c->dvmoy};
+  for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); k++)
+    if (!*t[k]) *t[k] = (double *)calloc(c->nr + 2, sizeof(double));
+  if (!c->dnshift) c->dnshift = (long *)calloc(c->nr + 2, sizeof(long));
+  for (int i = 0; i < c->nr; i++) c->DSigmaMed[i] = dsigma(c, c->Rmed[i]);
+  memcpy(c->drho, drho, sizeof(double) * (c->nr + 1) * c->ns);
+  return 0;
+}
+
+void or_dust_fluid_get(or_ctx *c, double *drho, double *dvr, double *dvt, double *stokes) {
+  const size_t b = sizeof(double) * (c->nr + 1) * c->ns;
+  if (drho) memcpy(drho, c->drho, b);
+  if (dvr) memcpy(dvr, c->dvr, b);
+  if (dvt) memcpy(dvt, c->dvt, b);
+  if (stokes) memcpy(stokes, c->stokes, b);
+}
+
+void or_dust_fluid_set(or_ctx *c, const double *drho, const double *dvr, const double *dvt) {
+  const size_t b = sizeof(double) * (c->nr + 1) * c->ns;
+  if (drho) memcpy(c->drho, drho, b);
+  if (dvr) memcpy(c->dvr, dvr, b);
+  if (dvt) memcpy(c->dvt, dvt, b);
+}
+
+
 int or_algogas(or_ctx *c, double DT) {
   const dfargo_params *p = &c->p;
   dfargo_planets *sys = &c->sys;
@@ -2673,13 +3248,21 @@ int or_algogas(or_ctx *c, double DT) {
     c->crashed = c->crashed_e;
     compute_sound_speed(c);
     compute_pressure(c);
+    if (c->df_on) df_sound_speed_pressure(c);
     or_substep1(c, dt);
     /* ActualiseGas round trips (src/SourceEuler.c:598-612) */
     memcpy(c->vr, c->vradint, sizeof(double) * n);
     memcpy(c->vt, c->vthetaint, sizeof(double) * n);
     substep2(c, dt);
+    if (c->df_on) df_substep2(c, dt);
     memcpy(c->vr, c->vradnew, sizeof(double) * n);
     memcpy(c->vt, c->vthetanew, sizeof(double) * n);
+    /* the dust velocities: DVradNew, or in the short-friction-time
+       approximation DVradInt (src/SourceEuler.c:600-630) */
+    if (c->df_on) {
+      memcpy(c->dvr, c->fp.sfta ? c->dvradint : c->dvradnew, sizeof(double) * n);
+      memcpy(c->dvt, c->fp.sfta ? c->dvthetaint : c->dvthetanew, sizeof(double) * n);
+    }
     if (p->energy_equation) memcpy(c->en, c->energyint, sizeof(double) * n);
     compute_divergence(c, c->vr, c->vt);
     if (p->energy_equation) {
@@ -2693,6 +3276,14 @@ int or_algogas(or_ctx *c, double DT) {
     }
     or_transport(c, dt);
     if (p->add_floors) add_floors(c);
+    /* the dust fluid's transport, diffusion and floor (src/SourceEuler.c:667-676) */
+    if (c->df_on) {
+      df_transport(c, dt);
+      if (c->fp.dust_diffusion) df_diffd(c, dt);
+      if (p->add_floors)
+        for (int l = 0; l < c->nr * c->ns; l++)
+          if (c->drho[l] < p->density_floor) c->drho[l] = p->density_floor;
+    }
     apply_boundary_condition(c);
     if (p->wave_killing_zones) evanescent_boundary(c, dt);
     compute_temperature(c);

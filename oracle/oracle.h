@@ -63,6 +63,12 @@ void or_solve_1d_viscous(or_ctx *c, double dt, double *sigma, double *vrad);
 void or_set_visc1d(or_ctx *c, const double *rmed, const double *sigma, const double *vrad);
 void or_get_visc1d(or_ctx *c, double *rmed, double *sigma, double *vrad);
 
+/* the dust fluid: switched on with its initial DRho ((nrad + 1) * nsec);
+ * or_init_euler then sets its Stokes number and velocities */
+int or_dust_fluid_attach(or_ctx *c, const dfargo_dust_fluid_params *f, const double *drho);
+void or_dust_fluid_get(or_ctx *c, double *drho, double *dvr, double *dvt, double *stokes);
+void or_dust_fluid_set(or_ctx *c, const double *drho, const double *dvr, const double *dvt);
+
 /* AlgoGas; returns the number of sub-steps taken */
 int or_algogas(or_ctx *c, double DT);
 void or_get_status(or_ctx *c, dfargo_status *st);

@@ -9,6 +9,7 @@ import subprocess
 
 import numpy as np
 
+from dustyfargoadsg_amd import _capi
 from dustyfargoadsg_amd._capi import Params, Planets, Status, FIELDS, DustParams
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -46,6 +47,9 @@ def lib():
             "or_sg_zero_mode": (None, [vp, dp, dp]),
             "or_solve_1d_viscous": (None, [vp, ctypes.c_double, dp, dp]),
             "or_set_visc1d": (None, [vp, dp, dp, dp]),
+            "or_dust_fluid_attach": (ctypes.c_int, [vp, ctypes.POINTER(_capi.DustFluidParams), dp]),
+            "or_dust_fluid_get": (None, [vp, dp, dp, dp, dp]),
+            "or_dust_fluid_set": (None, [vp, dp, dp, dp]),
             "or_get_visc1d": (None, [vp, dp, dp, dp]),
             "or_algogas": (ctypes.c_int, [vp, ctypes.c_double]),
             "or_get_status": (None, [vp, ctypes.POINTER(Status)]),
@@ -178,6 +182,20 @@ class Oracle:
         s, v = np.zeros(self.p.nrad1d), np.zeros(self.p.nrad1d)
         self.L.or_solve_1d_viscous(self.ctx, dt, ptr(s), ptr(v))
         return s, v
+
+    def attach_dust_fluid(self, fp, drho):
+        self.fp = fp
+        self.L.or_dust_fluid_attach(self.ctx, ctypes.byref(fp), ptr(np.ascontiguousarray(drho, dtype=np.float64)))
+
+    def dust_fluid(self):
+        """(DRho, DVrad, DVtheta, Stokes) grids."""
+        out = [self.zeros() for _ in range(4)]
+        self.L.or_dust_fluid_get(self.ctx, *[ptr(a) for a in out])
+        return dict(drho=out[0], dvr=out[1], dvt=out[2], stokes=out[3])
+
+    def set_dust_fluid(self, drho=None, dvr=None, dvt=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (drho, dvr, dvt)]
+        self.L.or_dust_fluid_set(self.ctx, *[ptr(a) for a in arrs])
 
     def set_visc1d(self, rmed=None, sigma=None, vrad=None):
         self.L.or_set_visc1d(self.ctx, *[None if a is None else ptr(np.ascontiguousarray(a, dtype=np.float64))
