@@ -64,6 +64,10 @@ struct dfargo_ctx {
   // DAMPTOVISCOUS: the 1-D viscous disc's initial cell centres, Sigma and
   // vr, uploaded again by every InitEuler
   std::vector<double> v1d_rmed0, v1d_sigma0, v1d_vrad0;
+  // the dust fluid (dfargo_attach_dust_fluid): switches and its initial
+  // surface density, which every InitEuler starts the fluid from
+  dfargo_dust_fluid_params fp;
+  std::vector<double> df_drho0;
 };
 
 static int noalloc_setup(int dev);
@@ -279,9 +283,14 @@ static int ensure_graph(dfargo_ctx *c) {
         if (is_sg_kernel(id)) launch_id(c, id, c->stream2);
       HIPCHK(hipEventRecord(c->ev_join, c->stream2));
     }
+    bool joined = false;
     for (int id : c->seq) {
       if (fork && is_sg_kernel(id)) continue;
-      if (fork && id == KID_SOURCE) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+      // the first kernel that reads the accelerations waits for them
+      if (fork && !joined && (id == KID_SOURCE || id == KID_DF_SOURCE || id == KID_DF_AXISG)) {
+        HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        joined = true;
+      }
       launch_id(c, id);
     }
     hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
@@ -328,10 +337,21 @@ static void build_sequence(dfargo_ctx *c) {
     for (int id : {KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT,
                    KID_DUST_REORDER})
       c->seq.push_back(id);
+  const bool df = c->a.df_on != 0;
+  // the dust fluid's source step starts from the state the sub-step starts
+  // with, before k_source (whose feedback drag reads its Stokes numbers)
+  if (df && c->p.self_gravity && !c->p.sg_zero_mode && c->a.df_feel_sg == DFARGO_DUST_SG_ZERO_MODE)
+    c->seq.push_back(KID_DF_AXISG);
+  if (df) c->seq.push_back(KID_DF_SOURCE);
   c->seq.push_back(KID_SOURCE);
   if (c->p.energy_equation) c->seq.push_back(KID_ENERGY);
+  if (df && c->a.df_visc)
+    for (int id : {KID_DF_STRESS, KID_DF_VISC}) c->seq.push_back(id);
+  if (df) c->seq.push_back(KID_DF_SUBSTEP2);
   c->a.sg_rows_in_finish = c->sg_post ? 1 : 0;
   for (int id : {KID_TRANSPORT_RAD, KID_TRANSPORT_AZI, KID_FINISH}) c->seq.push_back(id);
+  if (df)
+    for (int id : {KID_DF_TRANSPORT_RAD, KID_DF_TRANSPORT_AZI, KID_DF_FINISH}) c->seq.push_back(id);
   if (c->p.boundary == DFARGO_BC_NONREFLECTING) {
     if (c->p.energy_equation) c->seq.push_back(KID_EOS_STEP);
     c->seq.push_back(KID_NONREFLECTING);
@@ -691,6 +711,70 @@ extern "C" int dfargo_destroy(dfargo_ctx *c) {
 // ------------------------------------------------------------------------
 // InitEuler
 
+static int put_doubles(dfargo_ctx *c, double *dst, const double *src, size_t n);
+
+// The dust fluid's start-up in InitEuler (src/SourceEuler.c:265-269):
+// InitDustStokesNumber on the gas density (zero relative velocity,
+// src/Pframeforce.c:836-874) and InitDustVelocities (src/Pframeforce.c:
+// 644-830) as src/SourceEuler.c:268 calls it, with the two densities in each
+// other's places (in the feedback branch its eps is gas over dust).  The
+// routine's earlier assignments are all overwritten by its final loops,
+// which alone are restated.  The fluid's state, Stokes grid and
+// DVthetaMed / DVradMed go to the device; St returns for the gas velocities.
+static int df_init(dfargo_ctx *c, const double *rho, const std::vector<double> &axisg, std::vector<double> &St) {
+  const dfargo_params &p = c->p;
+  const dfargo_tables &t = c->tab;
+  const dfargo_dust_fluid_params &f = c->fp;
+  const int nr = p.nrad, ns = p.nsec;
+  const size_t n = (size_t)(nr + 1) * ns;
+  KArgs &a = c->a;
+  const double *drho = c->df_drho0.data();
+  const double um = 1.9891e30 * f.factor_unit_mass, ul = 1.49598e11 * f.factor_unit_length;
+  const double rho_int = f.rhopart * 1000.0 * pow(ul, 3.) / um, radius = f.sizepart / ul;
+  St.assign(n, 0.0);
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      const size_t l = (size_t)j + (size_t)i * ns;
+      const double densg_cgs = 0.1 * rho[l] * um * pow(ul, -2.0);
+      const double lambda = 3.34e-8 * pow(densg_cgs, -1.) * dfargo_aspect_ratio(&p, t.Rmed[i]) * t.Rmed[i];
+      const double Kn = 0.5 * lambda / radius, f_D = 1.0, k_D = 1.0;
+      const double Cdrag = pow(3.0 * Kn + 1.0, 2.0) * pow(9.0 * Kn * Kn * f_D + 3.0 * Kn * k_D, -1.0);
+      St[l] = 0.5 * M_PI * Cdrag * rho_int * radius / rho[l];
+    }
+  std::vector<double> dvr(n, 0.0), dvt(n, 0.0), dvtm(nr + 2, 0.0), dvrm(nr + 2, 0.0);
+  for (int i = 0; i < nr; i++) {
+    const double r = t.Rmed[i];
+    double vk = sqrt(1.0 * 1.0 / r);
+    const double eta = -0.5 * pow(p.aspect_ratio, 2.0) * pow(r, 2.0 * p.flaring_index) *
+                       (-1. - p.sigma_slope + 2.0 * p.flaring_index);
+    if (p.self_gravity) vk = sqrt(1.0 * 1.0 / r - r * axisg[i]);
+    for (int j = 0; j < ns; j++) {
+      const size_t l = (size_t)i * ns + j;
+      if (f.dust_feedback) {
+        const double eps = rho[l] / drho[l];
+        dvr[l] = -2.0 * eta * vk / (St[l] + pow(1.0 + eps, 2.0) / St[l]);
+        dvt[l] = vk - eta * vk * (1.0 + eps) / (pow(St[l], 2.0) + pow(1.0 + eps, 2.0));
+      } else {
+        dvr[l] = -2.0 * eta * vk / (St[l] + pow(St[l], -1.0));
+        dvt[l] = vk - eta * vk / (pow(St[l], 2.0) + 1.0);
+      }
+      dvt[l] -= p.omega_frame * r;
+    }
+    dvtm[i] = dvt[(size_t)i * ns] + r * p.omega_frame;
+    dvrm[i] = dvr[(size_t)i * ns];
+  }
+  int rc;
+  if ((rc = put_doubles(c, a.drho, drho, n)) || (rc = put_doubles(c, a.dvr, dvr.data(), n)) ||
+      (rc = put_doubles(c, a.dvt, dvt.data(), n)) || (rc = put_doubles(c, a.dstokes, St.data(), n)) ||
+      (rc = put_doubles(c, (double *)a.DVthetaMed, dvtm.data(), nr + 2)) ||
+      (rc = put_doubles(c, (double *)a.DVradMed, dvrm.data(), nr + 2)))
+    return rc;
+  // DVradInt / DVthetaInt of rings 0 and nrad stay zero, as in the reference
+  HIPCHK(hipMemset(a.dvri, 0, sizeof(double) * c->nfield));
+  HIPCHK(hipMemset(a.dvti, 0, sizeof(double) * c->nfield));
+  return DFARGO_OK;
+}
+
 extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad, double *vtheta,
                                  const double *energy, const dfargo_planets *sys_in) {
   if (!c || !rho || !vrad || !vtheta) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
@@ -771,6 +855,8 @@ extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad,
   }
   HIPCHK(hipMemcpy(a.nu_ring, nu.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(a.gcs, c->gcs_host.data(), sizeof(double) * (nr + 2), hipMemcpyHostToDevice));
+  std::vector<double> dfSt;
+  if (a.df_on && (rc = df_init(c, rho, axisg, dfSt))) return rc;
   // InitGasVelocities (src/Pframeforce.c:442-641), no centrifugal balance
   std::vector<double> vr(n, 0.0), vt(n, 0.0);
   for (int i = 0; i < nr; i++) {
@@ -801,6 +887,23 @@ extern "C" int dfargo_init_euler(dfargo_ctx *c, const double *rho, double *vrad,
         vr[l] -= 3.0 * viscosity / r * (-p.sigma_slope + .5);
     }
   }
+  // with the dust fluid's feedback, the gas velocities of the drag
+  // equilibrium (Nakagawa, Sekiya & Hayashi 1986; src/Pframeforce.c:609-634)
+  if (a.df_on && c->fp.dust_feedback)
+    for (int i = 0; i < nr; i++) {
+      const double r = t.Rmed[i];
+      double vk = sqrt(1.0 * 1.0 / r);
+      const double eta = -0.5 * pow(p.aspect_ratio, 2.0) * pow(r, 2.0 * p.flaring_index) *
+                         (-1. - p.sigma_slope + 2.0 * p.flaring_index);
+      if (p.self_gravity) vk = sqrt(1.0 * 1.0 / r - r * axisg[i]);
+      for (int j = 0; j < ns; j++) {
+        const size_t l = (size_t)i * ns + j;
+        const double eps = c->df_drho0[l] / rho[l], St = dfSt[l];
+        vr[l] = 2.0 * eps * eta * vk * St / (St * St + pow(1.0 + eps, 2.0));
+        vt[l] = vk * (1.0 - eta) + eps * eta * vk * (1.0 + eps) / (pow(St, 2.0) + pow(1.0 + eps, 2.0));
+        vt[l] -= p.omega_frame * r;
+      }
+    }
   // the initial profiles the wave-killing zones may damp towards
   // (src/Pframeforce.c:635-640)
   for (int i = 0; i < nr; i++) {
@@ -1354,6 +1457,8 @@ extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, co
   if (!c || !d || !st) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
   if (!c->initialized) return dfargo_set_error(DFARGO_ERR_STATE, "dfargo_init_euler has not been called");
   if (c->dust_on) return dfargo_set_error(DFARGO_ERR_STATE, "particles are already attached to this context");
+  if (c->a.df_on && d->nbpart > 0)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "dust particles beside the dust fluid are not implemented");
   int rc = dfargo_check_dust_params(&c->p, d);
   if (rc) return rc;
   if (c->a.accrete)
@@ -1444,6 +1549,141 @@ extern "C" int dfargo_attach_dust(dfargo_ctx *c, const dfargo_dust_params *d, co
   // the sub-step now holds the particle kernels: capture it again
   if ((rc = drop_graph(c))) return rc;
   build_sequence(c);
+  return DFARGO_OK;
+}
+
+// ------------------------------------------------------------------------
+// the dust fluid
+
+// DAspectRatio (src/Viscosity.c:123-138), LAMBDADOUBLING = 0
+static double df_aspect_ratio(const dfargo_params &p, double dar, double rad) {
+  double ar = dar;
+  const double rmin = p.transition_radius - p.transition_width * dar;
+  const double rmax = p.transition_radius + p.transition_width * dar;
+  if (rad < rmin) ar *= p.transition_ratio;
+  if ((rad >= rmin) && (rad <= rmax)) ar *= exp((rmax - rad) / (rmax - rmin) * log(p.transition_ratio));
+  return ar;
+}
+
+extern "C" int dfargo_attach_dust_fluid(dfargo_ctx *c, const dfargo_dust_fluid_params *f, const double *drho) {
+  if (!c || !f || !drho) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  if (c->a.df_on) return dfargo_set_error(DFARGO_ERR_STATE, "the dust fluid is already attached to this context");
+  if (c->dust_on)
+    return dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "dust particles beside the dust fluid are not implemented");
+  int rc = dfargo_check_dust_fluid_params(&c->p, f, c->prec);
+  if (rc) return rc;
+  const dfargo_params &p = c->p;
+  const dfargo_tables &t = c->tab;
+  const int nr = p.nrad, ns = p.nsec;
+  const size_t n = (size_t)(nr + 1) * ns;
+  for (size_t l = 0; l < (size_t)nr * ns; l++)
+    if (!(drho[l] > 0.0) || !std::isfinite(drho[l]))
+      return dfargo_set_error(DFARGO_ERR_ARG, "the dust density must be positive (cell %zu)", l);
+  HIPCHK(hipSetDevice(c->dev));
+  KArgs &a = c->a;
+  double **grids[] = {&a.drho, &a.dvr, &a.dvt, &a.dstokes, &a.dvri, &a.dvti, &a.dtrr, &a.dtrp, &a.dtpp,
+                      &a.dt_rho, &a.dt_rp, &a.dt_rm, &a.dt_tp, &a.dt_tm, &a.dz_rho, &a.dz_rp, &a.dz_rm,
+                      &a.dz_tp, &a.dz_tm};
+  for (double **g : grids)
+    if ((rc = dalloc(c, (void **)g, sizeof(double) * c->nfield))) return rc;
+  const int ntile = (ns + DF_SRC_COLS - 1) / DF_SRC_COLS;
+  if ((rc = dalloc(c, (void **)&a.dvmoy, sizeof(double) * (nr + 2))) ||
+      (rc = dalloc(c, (void **)&a.dvmed, sizeof(double) * (nr + 2))) ||
+      (rc = dalloc(c, (void **)&a.df_axisg, sizeof(double) * (nr + 2))) ||
+      (rc = dalloc(c, (void **)&a.dnshift, sizeof(long) * (nr + 2))) ||
+      (rc = dalloc(c, (void **)&a.dvt_part, sizeof(double) * (size_t)nr * ntile)) ||
+      (rc = dalloc(c, (void **)&a.dlost_col, sizeof(double) * ns)))
+    return rc;
+  // ring tables: the dust's sound speed (ComputeDustSoundSpeedAndPressure,
+  // src/SourceEuler.c:2353-2371) and its azimuthal mean summed cell by cell
+  // (GLOBAL_DustSoundSpeed), DFViscosity at Rmed (src/Viscosity.c:72-104),
+  // pow(Rsup, 1.5) and the cavity factor of FViscosity at Rsup for Diffd,
+  // DSigmaMed (FillDSigma, src/Theo.c:95-101)
+  const int n1 = nr + 2;
+  std::vector<double> dcs(n1, 0.0), gdcs(n1, 0.0), dnu(n1, 0.0), sup15(n1, 0.0), cavsup(n1, 1.0), dsig(n1, 0.0);
+  for (int i = 0; i < nr; i++) {
+    const double r = t.Rmed[i];
+    dcs[i] = df_aspect_ratio(p, f->daspect_ratio, r) * sqrt(1.0 * 1.0 / r) * pow(r, f->dflaring_index);
+    double sum = 0.0;
+    for (int j = 0; j < ns; j++) sum += dcs[i];
+    gdcs[i] = sum / (double)ns;
+    dsig[i] = dfargo_sigma_profile(&p, r) * f->dust_to_gas_density_ratio;
+  }
+  const bool dvisc = (f->dalpha_viscosity != 0.0) || (f->dviscosity != 0.0);
+  for (int i = 0; i < nr; i++) {
+    const double rad = t.Rmed[i];
+    double v = f->dviscosity;
+    if (f->dalpha_viscosity != 0.0) {
+      int k = 0;
+      while (t.Rmed[k] < rad) k++;
+      v = f->dalpha_viscosity * gdcs[k] * gdcs[k] * pow(rad, 1.5);
+    }
+    const double rmin = p.cavity_radius - p.cavity_width * f->daspect_ratio;
+    const double rmax = p.cavity_radius + p.cavity_width * f->daspect_ratio;
+    if (rad < rmin) v *= p.cavity_ratio;
+    if ((rad >= rmin) && (rad <= rmax)) v *= exp((rmax - rad) / (rmax - rmin) * log(p.cavity_ratio));
+    dnu[i] = dvisc ? v : 0.0;
+  }
+  for (int i = 0; i < nr; i++) {
+    const double rad = t.Rsup[i];
+    sup15[i] = pow(rad, 1.5);
+    const double rmin = p.cavity_radius - p.cavity_width * p.aspect_ratio * pow(p.cavity_radius, 1.0 + p.flaring_index);
+    const double rmax = p.cavity_radius + p.cavity_width * p.aspect_ratio * pow(p.cavity_radius, 1.0 + p.flaring_index);
+    if (rad < rmin) cavsup[i] = p.cavity_ratio;
+    if ((rad >= rmin) && (rad <= rmax)) cavsup[i] = exp((rmax - rad) / (rmax - rmin) * log(p.cavity_ratio));
+  }
+  if ((rc = upload_table(c, &a.df_cs, dcs.data(), n1)) || (rc = upload_table(c, &a.df_nu, dnu.data(), n1)) ||
+      (rc = upload_table(c, &a.df_sup15, sup15.data(), n1)) ||
+      (rc = upload_table(c, &a.df_cavsup, cavsup.data(), n1)) ||
+      (rc = upload_table(c, &a.DSigmaMed, dsig.data(), n1)) ||
+      (rc = upload_table(c, &a.DVthetaMed, dsig.data(), n1)) || (rc = upload_table(c, &a.DVradMed, dsig.data(), n1)))
+    return rc;
+  // the grain in code units (src/SourceEuler.c:812-818; ComputeCodeUnits,
+  // src/LowTasks.c:204-212)
+  const double um = 1.9891e30 * f->factor_unit_mass, ul = 1.49598e11 * f->factor_unit_length;
+  a.df_rho_int = f->rhopart * 1000.0 * pow(ul, 3.) / um;
+  a.df_radius = f->sizepart / ul;
+  a.df_to_cgs = 0.1 * um * pow(ul, -2.0);
+  a.df_diff_inn = f->dust_diff_inn;
+  a.df_sfta = f->sfta ? 1 : 0;
+  a.df_feel_disk = f->dust_feel_disk ? 1 : 0;
+  a.df_feedback = f->dust_feedback ? 1 : 0;
+  a.df_feel_sg = f->dust_feel_sg;
+  a.df_diffusion = f->dust_diffusion ? 1 : 0;
+  a.df_visc = dvisc ? 1 : 0;
+  c->fp = *f;
+  c->df_drho0.assign(drho, drho + n);
+  for (int j = 0; j < ns; j++) c->df_drho0[(size_t)nr * ns + j] = 0.0;
+  if ((rc = put_doubles(c, a.drho, c->df_drho0.data(), n))) return rc;
+  a.df_on = 1;
+  if ((rc = drop_graph(c))) return rc;
+  build_sequence(c);
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_download_dust_fluid(dfargo_ctx *c, double *drho, double *dvrad, double *dvtheta,
+                                          double *stokes) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  if (!c->a.df_on) return dfargo_set_error(DFARGO_ERR_STATE, "no dust fluid is attached to this context");
+  HIPCHK(hipSetDevice(c->dev));
+  const size_t n = (size_t)(c->p.nrad + 1) * c->p.nsec;
+  int rc;
+  if ((rc = get_doubles(c, drho, c->a.drho, n)) || (rc = get_doubles(c, dvrad, c->a.dvr, n)) ||
+      (rc = get_doubles(c, dvtheta, c->a.dvt, n)) || (rc = get_doubles(c, stokes, c->a.dstokes, n)))
+    return rc;
+  return DFARGO_OK;
+}
+
+extern "C" int dfargo_upload_dust_fluid(dfargo_ctx *c, const double *drho, const double *dvrad,
+                                        const double *dvtheta) {
+  if (!c) return dfargo_set_error(DFARGO_ERR_ARG, "null context");
+  if (!c->a.df_on) return dfargo_set_error(DFARGO_ERR_STATE, "no dust fluid is attached to this context");
+  HIPCHK(hipSetDevice(c->dev));
+  const size_t n = (size_t)(c->p.nrad + 1) * c->p.nsec;
+  int rc;
+  if ((rc = put_doubles(c, c->a.drho, drho, n)) || (rc = put_doubles(c, c->a.dvr, dvrad, n)) ||
+      (rc = put_doubles(c, c->a.dvt, dvtheta, n)))
+    return rc;
   return DFARGO_OK;
 }
 

@@ -444,6 +444,16 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
   s = block_sum_fast(s, sh);
   const double vmoy = s / (double)ns;
   if (t == 0) a.vmoy[i] = vmoy;
+  // the dust fluid's ring mean DVmoy (src/SourceEuler.c:1352-1361)
+  double dvmoy = 0.0;
+  if (a.df_on) {
+    double sd = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) sd += a.dvt[base + t + k * TB];
+    sd = block_sum_fast(sd, sh);
+    dvmoy = sd / (double)ns;
+    if (t == 0) a.dvmoy[i] = dvmoy;
+  }
   double dtmin = 1e30;
   if (i >= 1) {
     const R viscosity = (R)a.nu_ring[i];
@@ -474,11 +484,38 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
       const R invdt6 = R(1e-10);
       // dt = CFLSECURITY / sqrt(S) decreases with S, and rounding keeps
       // that order: the smallest dt of the ring is that of its largest S
-      const R S = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 +
-                  invdt6 * invdt6;
+      R S = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 +
+            invdt6 * invdt6;
+      if (a.df_on) {
+        // the dust fluid's limits (src/SourceEuler.c:1430-1471): dt is the
+        // smaller of the two fluids', i.e. the larger S
+        R invdt7 = R(1e-10), invdt8 = R(1e-10), invdt9 = R(1e-10), invdt10 = R(1e-10);
+        const double *dvr = a.dvr + base, *dvt = a.dvt + base;
+        const R dvtl = dvt[j];
+        if (!a.df_sfta) {
+          invdt7 = a.df_cs[i] * idxmin;
+          invdt8 = fabs(dvr[j] + 1e-15) * idxrad;
+          invdt9 = fabs(a.fast ? dvtl - (R)dvmoy : dvtl) * idxtheta;
+          if (a.df_feel_disk)
+            invdt10 = 1. / (a.dstokes[base + j] + 1.e-10) * sqrt(DF_G * 1.0 / a.Rmed[i]) / a.Rmed[i];
+        }
+        R ddvr = dvr[j + ns] - dvr[j], ddvt = dvt[jp] - dvtl;
+        if (ddvr >= R(0)) ddvr = R(1e-10); else ddvr = -ddvr;
+        if (ddvt >= R(0)) ddvt = R(1e-10); else ddvt = -ddvt;
+        const R d4 = ddvr * idxrad, d5 = ddvt * idxtheta;
+        R invdt11 = (d5 > d4) ? d5 : d4;
+        invdt11 *= R(4.0 * DF_CVNR * DF_CVNR);
+        const R dnu = a.df_nu[i];
+        const R invdt12 = (dnu != R(0)) ? dnu * R(4) / (R)a.powmin2[i] : R(1e-10);
+        const R Sd = invdt7 * invdt7 + invdt8 * invdt8 + invdt9 * invdt9 + invdt10 * invdt10 +
+                     invdt11 * invdt11 + invdt12 * invdt12;
+        S = (Sd > S) ? Sd : S;
+      }
       smax = (S > smax) ? S : smax;
     }
     dtmin = R(DF_CFLSECURITY) / sqrt(smax);
+    // max2(dt, 1e-10) of the two-fluid limit
+    if (a.df_on && dtmin < 1.e-10) dtmin = 1.e-10;
   }
   // per-ring partials, stored partial-major (ring_part[p * nr + i]):
   // negative-density and negative-energy counts, disc mass, barycentre.
@@ -793,6 +830,11 @@ template <typename T, bool ACC> __global__ void __launch_bounds__(CtrlBlock<ACC>
     double dt = a.pm * DF_CFLSECURITY / (double)ns /
                 fabs(a.vmoy[i] * a.InvRmed[i] - a.vmoy[i + 1] * a.InvRmed[i + 1]);
     nd = dmin2(nd, dt);
+    // the dust fluid's shear limit (src/SourceEuler.c:1486-1491)
+    if (a.df_on && !a.df_sfta) {
+      dt = a.pm * DF_CFLSECURITY / (double)ns / fabs(a.dvmoy[i] * a.InvRmed[i] - a.dvmoy[i + 1] * a.InvRmed[i + 1]);
+      nd = dmin2(nd, dt);
+    }
   }
   nd = block_min_fast(nd, sh);
   // minimum stopping time of the last particle interpolation (reset to 1e6
@@ -1247,6 +1289,11 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
       vt2 = vt2 * vt2;
       vri = (double)pf_vr[it] + dt * (-gradp - gradphi + vt2 * STAB(ST_IRINF, i));
       if (a.d_on && a.d_feedback) vri += dt * ld<T>(a.radfbacc, l);
+      if (a.df_feedback) {
+        // the dust fluid's drag on the gas, implicit (src/SourceEuler.c:915-921)
+        const double ts = a.dstokes[l] * a.r15[i], eps = a.drho[l] / rl;
+        vri = (vri * ts + eps * a.dvr[l] * dt) / (ts + dt * eps);
+      }
       if (dgrid) {
         st<T>(a.radgradp, l, (pl - plim) * idrm);
         if (a.indirect_term)
@@ -1268,6 +1315,12 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
     vti = vti - dt * (gradp + gradphi);
     vti += dt * STAB(ST_SUPP, i);
     if (a.d_on && a.d_feedback) vti += dt * ld<T>(a.azifbacc, l);
+    if (a.df_feedback) {
+      // src/SourceEuler.c:991-997, with the dust's vtheta after CorrectVtheta
+      const double ts = a.dstokes[l] * a.r15[i], eps = a.drho[l] / rl;
+      const double dvtl = a.frame ? a.dvt[l] - dom * STAB(ST_RMED, i) : a.dvt[l];
+      vti = (vti * ts + eps * dvtl * dt) / (ts + dt * eps);
+    }
     if (dgrid) {
       st<T>(a.azigradp, l, (pl - pljm) * invdxtheta);
       if (a.indirect_term)
@@ -1605,6 +1658,10 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     if (k == 4 && !a.energy) continue;
+    if (k < 4 && a.tr_skip_mom) {
+      ((T *)outp[k])[l] = (T)Q[k];
+      continue;
+    }
     const R q0 = rad_star<R>(f0, sW[k][ty + 1][tx], sW[k][ty + 2][tx], sD[k][ty][tx], sD[k][ty + 1][tx]);
     const R q1 = rad_star<R>(f1, sW[k][ty + 2][tx], sW[k][ty + 3][tx], sD[k][ty + 1][tx], sD[k][ty + 2][tx]);
     // VanLeerRadial (src/TransportEuler.c:205-217)
@@ -1676,6 +1733,7 @@ __device__ __forceinline__ void azi_pass_lane(const KArgs &a, R (&Q)[6], R v, R 
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     if (q == 4 && !a.energy) continue;
+    if (q < 4 && a.tr_skip_mom) continue;
     const R S = azi_star_lane<T>(Q[q] * inv, v, dt, dxtheta, invdxtheta);
     const R f = dxrad * S * RS * v;
     const R fn = lane_next(f);
@@ -1710,15 +1768,21 @@ template <typename T> __global__ void __launch_bounds__(256) k_transport_azi(KAr
     a.vmed[i] = vmed;
     a.nshift[i] = nshift;
     // distance of Ntilde from the nearest rounding tie, min over the call
+    // (the gas's shift)
     const double margin = 0.5 - fabs(Ntilde - Nround);
-    atomicMin(&a.ctrl->fargo_margin_bits, (unsigned long long)__double_as_longlong(margin > 0.0 ? margin : 0.0));
+    if (!a.tr_dust)
+      atomicMin(&a.ctrl->fargo_margin_bits, (unsigned long long)__double_as_longlong(margin > 0.0 ? margin : 0.0));
   }
-  // mass through the inner edge (VanLeerRadial's LostByDisk, ring 0)
+  // mass through the inner edge (VanLeerRadial's LostByDisk, ring 0;
+  // LostMassd for the dust fluid)
   if (i == 0 && seg == 0) {
     double lost = 0.0;
     for (int jj = lane; jj < ns; jj += AZ_LANES) lost += a.lost_col[jj];
     lost = wave_sum(lost);
-    if (lane == 0) a.ctrl->lost_mass += lost;
+    if (lane == 0) {
+      if (a.tr_dust) a.ctrl->lost_massd += lost;
+      else a.ctrl->lost_mass += lost;
+    }
   }
   // quantities after the radial pass: rp, rm, tp, tm, [e], rho; the
   // results go to z_*, since other wavefronts of the ring may still read
@@ -2026,9 +2090,44 @@ template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArg
       q0[k] = t / (double)ns;
     }
   }
+  // the dust fluid's targets (src/SideEuler.c:844-848,887-926): its
+  // initial profiles, or with DampToAxi its ring means
+  double dq0[3] = {0.0, 0.0, 0.0};
+  if (a.df_on) {
+    if (!a.damp_ini && !a.damp_visc) {
+      __shared__ double dpart[EV_TB / 64][3];
+      double s[3] = {0.0, 0.0, 0.0};
+      for (int j = tid; j < ns; j += EV_TB) {
+        s[0] += a.dvr[base + j];
+        s[1] += a.dvt[base + j];
+        s[2] += a.drho[base + j];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) s[k] = wave_sum(s[k]);
+      if ((tid & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 3; k++) dpart[tid >> 6][k] = s[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double t = 0.0;
+        for (int w = 0; w < EV_TB / 64; w++) t += dpart[w][k];
+        dq0[k] = t / (double)ns;
+      }
+    } else {
+      dq0[0] = a.DVradMed[i];
+      dq0[1] = a.DVthetaMed[i] - R * c->omega_frame;
+      dq0[2] = a.DSigmaMed[i];
+    }
+  }
   const bool edge = a.zero_vrad && ((in_zone && (i == 0 || i == 1)) || (!in_zone && i == nr - 1));
   for (int j = tid; j < ns; j += EV_TB) {
     const long l = base + j;
+    if (a.df_on) {
+      a.dvr[l] = (a.dvr[l] + lambda * dq0[0]) / (1.0 + lambda);
+      a.dvt[l] = (a.dvt[l] + lambda * dq0[1]) / (1.0 + lambda);
+      a.drho[l] = (a.drho[l] + lambda * dq0[2]) / (1.0 + lambda);
+    }
     double vr = (ld<T>(a.vr, l) + lambda * q0[0]) / (1.0 + lambda);
     double vt = (ld<T>(a.vt, l) + lambda * q0[1]) / (1.0 + lambda);
     double rho = (ld<T>(a.rho, l) + lambda * q0[2]) / (1.0 + lambda);
@@ -2260,6 +2359,8 @@ static int wide_k(int ns) {
     }                                                                                       \
   } while (0)
 
+#include "dust_fluid.h"
+
 static dim3 cell_grid(const KArgs &a, int rings) {
   int bx = a.ns < 256 ? a.ns : 256;
   return dim3((a.ns + bx - 1) / bx, rings);
@@ -2319,6 +2420,18 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
     case KID_DUST_SORT:
     case KID_DUST_DEPOSIT:
     case KID_DUST_REORDER: launch_dust_kernel<T>(id, a, s); break;
+    case KID_DF_AXISG:
+    case KID_DF_SOURCE:
+    case KID_DF_STRESS:
+    case KID_DF_VISC:
+    case KID_DF_SUBSTEP2:
+    case KID_DF_TRANSPORT_RAD:
+    case KID_DF_TRANSPORT_AZI:
+    case KID_DF_FINISH:
+      // the dust fluid runs with FP64 storage only (the context refuses it
+      // otherwise)
+      if (sizeof(T) == sizeof(double)) launch_df_kernel(id, a, s);
+      break;
     default: launch_sg_kernel<T>(id, a, s); break;
   }
 }

@@ -202,6 +202,26 @@ class Disc:
         return self.lib.dfargo_fviscosity(self.ctx, r)
 
     # -- dust particles (src/Dsys.c, src/DustUpdate.c) ---------------------
+    def attach_dust_fluid(self, fp, drho=None):
+        """The dust fluid (DUSTFLUID YES) from its initial surface density
+        (by default InitDustDensity's); call before initial_state /
+        init_euler, which set its Stokes numbers and velocities."""
+        if drho is None:
+            drho = init_dust_fluid_density(self.p, fp, self.lib)
+        drho = np.ascontiguousarray(drho, dtype=np.float64)
+        _check(self.lib, self.lib.dfargo_attach_dust_fluid(self.ctx, ctypes.byref(fp), _ptr(drho)))
+
+    def dust_fluid(self):
+        """The fluid's DRho, DVrad, DVtheta and the Stokes grid of the last
+        sub-step ((nrad + 1) * nsec values each)."""
+        out = [self.zeros() for _ in range(4)]
+        _check(self.lib, self.lib.dfargo_download_dust_fluid(self.ctx, *[_ptr(a) for a in out]))
+        return dict(drho=out[0], dvr=out[1], dvt=out[2], stokes=out[3])
+
+    def set_dust_fluid(self, drho=None, dvr=None, dvt=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (drho, dvr, dvt)]
+        _check(self.lib, self.lib.dfargo_upload_dust_fluid(self.ctx, *[_ptr(a) for a in arrs]))
+
     def attach_dust(self, d, parts, timestep):
         """Particles from a dict of nbpart-long arrays (r, th, vr, vth, l,
         dustsize) and the start-up interpolation with `timestep`."""
@@ -297,6 +317,15 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def init_dust_fluid_density(params, fp, lib=None):
+    """InitDustDensity (src/Pframeforce.c:344-380) on the host, no noise:
+    DSigma(Rmed) on every cell, ring nrad zero."""
+    lib = lib or _capi.load()
+    drho = np.zeros((params.nrad + 1) * params.nsec)
+    _check(lib, lib.dfargo_init_dust_fluid_density(ctypes.byref(params), ctypes.byref(fp), _ptr(drho)))
+    return drho
 
 
 def init_dust_system(params, d, axi_sg_accr=None, lib=None):
