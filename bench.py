@@ -111,6 +111,41 @@ def run_gpu(params, planets, precision, steps, warmup, prof_steps, dust=None):
     return t1 - t0, kt
 
 
+def dust_fluid_timing(params, planets, mode, steps, warmup, prof_steps=40):
+    """The workload with the dust fluid (FP64 storage; "sfta": the
+    short-friction-time approximation, "full": the dust's momentum equations
+    with viscosity and diffusion; 1 cm grains) beside the gas alone: ms per
+    sub-step of each and the fluid's kernels (HIP events, one launch at a
+    time)."""
+    from dustyfargoadsg_amd import _capi
+    from dustyfargoadsg_amd.disc import Disc
+    DT = 1.0e3
+    out = {}
+    for name in ("gas", mode):
+        d = Disc(params, _capi.FP64)
+        if name != "gas":
+            full = mode == "full"
+            d.attach_dust_fluid(_capi.dust_fluid_default_params(
+                sfta=0 if full else 1, sizepart=1e-2, dviscosity=1e-6 if full else 0.0,
+                dust_diffusion=1 if full else 0))
+        d.initial_state(planets)
+        d.run_substeps(DT, warmup)
+        t0 = time.perf_counter()
+        d.run_substeps(DT, steps)
+        t1 = time.perf_counter()
+        d.set_profiling(True)
+        d.run_substeps(DT, prof_steps)
+        kt = d.kernel_times()
+        d.close()
+        out[name] = dict(ms_per_step=(t1 - t0) / steps * 1e3,
+                         kernels_us={k: round(v[0] / v[1] * 1e3, 2) for k, v in kt.items() if v[1]})
+    cells = params.nrad * params.nsec
+    return {"mode": mode, "cells": cells, "steps": steps,
+            "gas_ms_per_step": out["gas"]["ms_per_step"], "with_fluid_ms_per_step": out[mode]["ms_per_step"],
+            "fluid_share": 1.0 - out["gas"]["ms_per_step"] / out[mode]["ms_per_step"],
+            "fluid_kernels_us": {k: v for k, v in out[mode]["kernels_us"].items() if k.startswith("df_")}}
+
+
 def batched_sweep(n, seed, calls, warmup_calls=1, tune=True):
     """The batched parameter sweep: n isothermal FP32 planet discs of uneven
     grids and parameters (workloads.batch_discs), advanced `calls` AlgoGas
@@ -328,6 +363,8 @@ def main():
                     help="with --precision-only, add the FP64 run that receives FP32-size noise every call")
     ap.add_argument("--no-controls", action="store_true",
                     help="with --precision-only, skip the perturbed FP64 control runs")
+    ap.add_argument("--dust-fluid", choices=("sfta", "full"), default=None,
+                    help="time the workload with the dust fluid (FP64) beside the gas alone, print that, return")
     ap.add_argument("--precision-only", type=float, default=0.0,
                     help="only run the FP32-vs-FP64 comparison of the workload over this many orbits "
                          "(with checkpoints every tenth of it and the perturbed FP64 control) and print it")
@@ -358,6 +395,10 @@ def main():
     from dustyfargoadsg_amd.workloads import DUST_WORKLOADS
     dust = DUST_WORKLOADS[args.workload](params) if args.workload in DUST_WORKLOADS else None
     npart = dust.nbpart if dust is not None else 0
+    if args.dust_fluid:
+        print(json.dumps({"workload": f"{args.workload} {params.nrad}x{params.nsec}",
+                          **dust_fluid_timing(params, planets, args.dust_fluid, args.steps, args.warmup)}))
+        return
     if args.precision_only > 0:
         o = args.precision_only
         print(json.dumps({"workload": f"{args.workload} {params.nrad}x{params.nsec}",

@@ -146,7 +146,7 @@ class DustFluidParams(ctypes.Structure):
         ("dviscosity", c_double), ("dalpha_viscosity", c_double),
         ("daspect_ratio", c_double), ("dflaring_index", c_double), ("dust_diff_inn", c_double),
         ("sfta", c_int), ("dust_feel_disk", c_int), ("dust_feedback", c_int),
-        ("dust_feel_sg", c_int), ("dust_diffusion", c_int),
+        ("dust_feel_sg", c_int), ("dust_diffusion", c_int), ("open_inner_dust", c_int),
     ]
 
 
@@ -301,6 +301,8 @@ def load(path=None):
         "dfargo_attach_dust_fluid": (c_int, [vp, ctypes.POINTER(DustFluidParams), dp]),
         "dfargo_download_dust_fluid": (c_int, [vp, dp, dp, dp, dp]),
         "dfargo_upload_dust_fluid": (c_int, [vp, dp, dp, dp]),
+        "dfargo_read_dust_fluid_par": (c_int, [ctypes.c_char_p, ctypes.POINTER(DustFluidParams),
+                                               ctypes.POINTER(c_int)]),
         "dfargo_upload_dust": (c_int, [vp, ctypes.POINTER(DustState)]),
         "dfargo_gas_budgets": (c_int, [vp, dp]),
         "dfargo_update_log": (c_int, [vp, c_int, dp, dp, ctypes.POINTER(c_int)]),
@@ -346,6 +348,7 @@ EXPORTED = [
     "dfargo_attach_dust", "dfargo_download_dust", "dfargo_upload_dust",
     "dfargo_dust_fluid_default_params", "dfargo_check_dust_fluid_params", "dfargo_init_dust_fluid_density",
     "dfargo_attach_dust_fluid", "dfargo_download_dust_fluid", "dfargo_upload_dust_fluid",
+    "dfargo_read_dust_fluid_par",
     "dfargo_gas_budgets", "dfargo_update_log", "dfargo_update_log_noalloc", "dfargo_orbital_elements",
     "dfargo_read_par", "dfargo_read_planet_config",
     "dfargo_get_tuning", "dfargo_set_tuning", "dfargo_autotune",

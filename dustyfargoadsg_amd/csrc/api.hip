@@ -1651,6 +1651,7 @@ extern "C" int dfargo_attach_dust_fluid(dfargo_ctx *c, const dfargo_dust_fluid_p
   a.df_feel_sg = f->dust_feel_sg;
   a.df_diffusion = f->dust_diffusion ? 1 : 0;
   a.df_visc = dvisc ? 1 : 0;
+  a.df_open = f->open_inner_dust ? 1 : 0;
   c->fp = *f;
   c->df_drho0.assign(drho, drho + n);
   for (int j = 0; j < ns; j++) c->df_drho0[(size_t)nr * ns + j] = 0.0;

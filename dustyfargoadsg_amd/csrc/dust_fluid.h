@@ -17,8 +17,8 @@
 //                  short-friction-time approximation DVradInt as it is;
 //                  the per-tile sums of the new vtheta for the FARGO mean
 //   k_transport_rad / k_transport_azi on the dust fields (Transportd)
-//   k_df_finish    ComputeVelocities, Diffd, the floor, the wall and
-//                  sub-Keplerian boundaries
+//   k_df_finish    ComputeVelocities, Diffd, the floor, the wall,
+//                  sub-Keplerian and open-dust boundaries
 // and the wave-killing zones damp the fluid in k_evanescent.
 
 // ---- the grain's Stokes number: SubStep1 (src/SourceEuler.c:820-858) ----
@@ -306,31 +306,21 @@ __device__ __forceinline__ double df_gas_nu_sup(const KArgs &a, int i) {
   return v * a.df_cavsup[i];
 }
 
-// The end of the dust's sub-step: ComputeVelocities, Diffd
-// (src/SourceEuler.c:2599-2656), the floor (AddFloorDensity), then the
-// dust's part of ApplyBoundaryCondition (src/SideEuler.c:1008-1131): the
-// wall edges with DSigmaMed and the sub-Keplerian vtheta of rings 0 and
-// nrad-1.  Rings 0 .. nrad, the last only for the wall's vr.
-__global__ void __launch_bounds__(256) k_df_finish(KArgs a) {
-  const dfargo_ctrl *c = a.ctrl;
-  if (!c->active) return;
-  const int i = blockIdx.y, ns = a.ns, nr = a.nr;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
+// The dust's state of cell (i, j) at the end of the sub-step before
+// OpenBoundaryd: ComputeVelocities, Diffd (src/SourceEuler.c:2599-2656),
+// the floor (AddFloorDensity), then the dust's part of ApplyWallBoundaries
+// (src/SideEuler.c:1066-1091) with DSigmaMed and of
+// ApplySubKeplerianBoundary (src/SideEuler.c:1025-1041) for rings 0 and
+// nrad-1.  Rings 0 .. nrad-1.
+struct DfCell {
+  double rho, vr, vt;
+};
+__device__ DfCell df_cell(const KArgs &a, const dfargo_ctrl *c, int i, int j) {
+  const int ns = a.ns, nr = a.nr;
   const long l = (long)i * ns + j;
-  const bool wall = (a.bc == DFARGO_BC_WALL);
-  double vr, vt;
-  if (i == nr) {
-    vr = 0.0;
-    if (wall) {
-      df_vel(a, c, nr - 2, j, vr, vt);
-      vr = -vr;
-    }
-    a.dvr[l] = vr;
-    return;
-  }
-  df_vel(a, c, i, j, vr, vt);
-  double drho = a.dz_rho[l];
+  DfCell s;
+  df_vel(a, c, i, j, s.vr, s.vt);
+  s.rho = a.dz_rho[l];
   if (a.df_diffusion && i >= 1 && i <= nr - 2 && a.Rmed[i] > a.df_diff_inn) {
     const double dtheta = 2.0 * DF_PI / (double)ns, invdtheta = 1.0 / dtheta;
     const int jm = (j == 0) ? ns - 1 : j - 1, jp = (j == ns - 1) ? 0 : j + 1;
@@ -340,31 +330,65 @@ __global__ void __launch_bounds__(256) k_df_finish(KArgs a) {
     const double frm = df_flux_r(a, i - 1, lim, df_gas_nu_sup(a, i - 1));
     const double ftp = df_flux_t(a, i, l, ljp, nu, invdtheta);
     const double ftm = df_flux_t(a, i, ljm, l, nu, invdtheta);
-    drho = drho - (a.Rsup[i] * dtheta * frp - a.Rinf[i] * dtheta * frm + (a.Rsup[i] - a.Rinf[i]) * (ftp - ftm)) *
-                      c->dt * a.InvSurf[i];
+    s.rho = s.rho - (a.Rsup[i] * dtheta * frp - a.Rinf[i] * dtheta * frm + (a.Rsup[i] - a.Rinf[i]) * (ftp - ftm)) *
+                        c->dt * a.InvSurf[i];
   }
-  if (a.floors && drho < a.floor_dens) drho = a.floor_dens;
-  if (wall) {
+  if (a.floors && s.rho < a.floor_dens) s.rho = a.floor_dens;
+  if (a.bc == DFARGO_BC_WALL) {
     if (i == 0) {
       double v2, t2;
       df_vel(a, c, 2, j, v2, t2);
-      vr = -v2;
-      drho = a.DSigmaMed[0];
+      s.vr = -v2;
+      s.rho = a.DSigmaMed[0];
     } else if (i == 1) {
-      vr = 0.0;
-      drho = a.DSigmaMed[1];
+      s.vr = 0.0;
+      s.rho = a.DSigmaMed[1];
     }
-    if (i == nr - 2) drho = a.DSigmaMed[nr - 2];
+    if (i == nr - 2) s.rho = a.DSigmaMed[nr - 2];
     if (i == nr - 1) {
-      vr = 0.0;
-      drho = a.DSigmaMed[nr - 1];
+      s.vr = 0.0;
+      s.rho = a.DSigmaMed[nr - 1];
     }
   }
-  if (i == 0) vt = a.DVthetaMed[0] - a.Rmed[0] * c->omega_frame;
-  if (i == nr - 1) vt = a.DVthetaMed[nr - 1] - a.Rmed[nr - 1] * c->omega_frame;
-  a.drho[l] = drho;
-  a.dvr[l] = vr;
-  a.dvt[l] = vt;
+  if (i == 0) s.vt = a.DVthetaMed[0] - a.Rmed[0] * c->omega_frame;
+  if (i == nr - 1) s.vt = a.DVthetaMed[nr - 1] - a.Rmed[nr - 1] * c->omega_frame;
+  return s;
+}
+
+// The end of the dust's sub-step: df_cell, then with INNERBOUNDARYDUST
+// OPEN the edges of OpenBoundaryd (src/SideEuler.c:411-466): rings 0 and 1
+// take ring 2's density, rings 1 and 2 ring 3's vr, ring nrad-1 ring
+// nrad-2's density and vr.  Rings 0 .. nrad, the last only for the wall's
+// vr of the outer face.
+__global__ void __launch_bounds__(256) k_df_finish(KArgs a) {
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->active) return;
+  const int i = blockIdx.y, ns = a.ns, nr = a.nr;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const long l = (long)i * ns + j;
+  if (i == nr) {
+    double vr = 0.0, vt;
+    if (a.bc == DFARGO_BC_WALL) {
+      df_vel(a, c, nr - 2, j, vr, vt);
+      vr = -vr;
+    }
+    a.dvr[l] = vr;
+    return;
+  }
+  DfCell s = df_cell(a, c, i, j);
+  if (a.df_open) {
+    if (i <= 1) s.rho = df_cell(a, c, 2, j).rho;
+    if (i == 1 || i == 2) s.vr = df_cell(a, c, 3, j).vr;
+    if (i == nr - 1) {
+      const DfCell m = df_cell(a, c, nr - 2, j);
+      s.rho = m.rho;
+      s.vr = m.vr;
+    }
+  }
+  a.drho[l] = s.rho;
+  a.dvr[l] = s.vr;
+  a.dvt[l] = s.vt;
 }
 
 static void launch_df_kernel(int id, const KArgs &a, hipStream_t s) {

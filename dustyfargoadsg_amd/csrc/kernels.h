@@ -118,7 +118,7 @@ struct KArgs {
   // surface density factor), state, the sub-step's Stokes numbers,
   // DVradInt / DVthetaInt, stresses, the transport's buffers (dt_*, dz_*
   // as t_*, z_* for the gas), ring buffers and ring tables
-  int df_on, df_sfta, df_feel_disk, df_feedback, df_feel_sg, df_diffusion, df_visc;
+  int df_on, df_sfta, df_feel_disk, df_feedback, df_feel_sg, df_diffusion, df_visc, df_open;
   double df_rho_int, df_radius, df_to_cgs, df_diff_inn;
   double *drho, *dvr, *dvt, *dstokes, *dvri, *dvti, *dtrr, *dtrp, *dtpp;
   double *dt_rho, *dt_rp, *dt_rm, *dt_tp, *dt_tm, *dz_rho, *dz_rp, *dz_rm, *dz_tp, *dz_tm;

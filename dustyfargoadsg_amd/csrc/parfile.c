@@ -8,7 +8,8 @@
  * physics this library does not implement is refused with
  * DFARGO_ERR_UNSUPPORTED naming the key, rather than run without it.  The
  * particles' keys (NBPART, SIZEMINPART, ...) are not read here: the particle
- * set is described by dfargo_dust_params. */
+ * set is described by dfargo_dust_params.  The dust fluid's switches have
+ * a reader of their own (dfargo_read_dust_fluid_par). */
 #include <ctype.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -55,11 +56,20 @@ static const par_key kKeys[] = {
     {"ADDFLOORS", K_STR, 0, "YES"}, {"DENSITYFLOOR", K_REAL, 0, "1e-9"}, {"ZEROVRADBC", K_STR, 0, "NO"},
     {"NRAD1D", K_INT, 0, "100"}, {"RMIN1D", K_REAL, 0, "0.01"}, {"RMAX1D", K_REAL, 0, "100.0"},
     {"BOUNDARY1DGRID", K_STR, 0, "T"},
+    /* the dust fluid (dfargo_read_dust_fluid_par) */
+    {"DUSTFLUID", K_STR, 0, "NO"}, {"SIZEPART", K_REAL, 0, "1e-3"}, {"RHOPART", K_REAL, 0, "1.0"},
+    {"FACTORUNITMASS", K_REAL, 0, "1.0"}, {"FACTORUNITLENGTH", K_REAL, 0, "1.0"},
+    {"DUSTTOGASDENSITYRATIO", K_REAL, 0, "0.01"}, {"DVISCOSITY", K_REAL, 0, "0.0"},
+    {"DALPHAVISCOSITY", K_REAL, 0, "0.0"}, {"DASPECTRATIO", K_REAL, 0, "0.05"},
+    {"DFLARINGINDEX", K_REAL, 0, "0.0"}, {"DUSTDIFFINN", K_REAL, 0, "0.0"}, {"SFTAPPROX", K_STR, 0, "YES"},
+    {"DUSTFEELDISK", K_STR, 0, "YES"}, {"DUSTFEEDBACK", K_STR, 0, "NO"}, {"DUSTFEELSG", K_STR, 0, "YES"},
+    {"DUSTDIFFUSION", K_STR, 0, "NO"}, {"INNERBOUNDARYDUST", K_STR, 0, "WALL"},
+    {"RESTARTWITHNEWDUST", K_STR, 0, "NO"},
     /* switches of physics not implemented here (refused when on) */
     {"OUTERSOURCEMASS", K_STR, 0, "NO"}, {"ZMPLUS", K_STR, 0, "NO"}, {"ENTROPYDIFFUSION", K_STR, 0, "NO"},
     {"RADIATIVEDIFFUSION", K_STR, 0, "NO"}, {"THERMALCOOLING", K_STR, 0, "NO"},
     {"STELLARIRRADIATION", K_STR, 0, "NO"}, {"TEMPPRESC", K_STR, 0, "NO"}, {"ADDMASS", K_STR, 0, "NO"},
-    {"DISCEVAPORATION", K_STR, 0, "NO"}, {"DUSTFLUID", K_STR, 0, "NO"}, {"PHOTOEVAPORATION", K_STR, 0, "NO"},
+    {"DISCEVAPORATION", K_STR, 0, "NO"}, {"PHOTOEVAPORATION", K_STR, 0, "NO"},
     {"CAVITYTORQUE", K_STR, 0, "NO"},
 };
 enum { NKEYS = sizeof(kKeys) / sizeof(kKeys[0]) };
@@ -80,8 +90,8 @@ static double R(const par_values *v, const char *k) { return v->r[key_index(k)];
 static int I(const par_values *v, const char *k) { return (int)v->r[key_index(k)]; }
 static char C(const par_values *v, const char *k) { return (char)toupper((unsigned char)v->s[key_index(k)][0]); }
 
-int dfargo_read_par(const char *path, dfargo_params *p, dfargo_run_params *run) {
-  if (!path || !p) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+/* ReadVariables (src/Interpret.c:82-140) into *out (freed by the caller) */
+static int read_values(const char *path, par_values **out) {
   par_values *v = (par_values *)calloc(1, sizeof(par_values));
   if (!v) return dfargo_set_error(DFARGO_ERR_ALLOC, "out of host memory");
   /* defaults: var() reads the default text with %lf for every type and
@@ -120,10 +130,23 @@ int dfargo_read_par(const char *path, dfargo_params *p, dfargo_run_params *run) 
   for (int i = 0; i < NKEYS && rc == DFARGO_OK; i++)
     if (kKeys[i].necessary && !v->read[i])
       rc = dfargo_set_error(DFARGO_ERR_ARG, "'%s' lacks the mandatory variable %s", path, kKeys[i].name);
+  if (rc != DFARGO_OK) {
+    free(v);
+    return rc;
+  }
+  *out = v;
+  return DFARGO_OK;
+}
+
+int dfargo_read_par(const char *path, dfargo_params *p, dfargo_run_params *run) {
+  if (!path || !p) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  par_values *v = NULL;
+  int rc = read_values(path, &v);
+  if (rc) return rc;
   /* physics this library does not implement */
   static const char *const kOff[] = {"OUTERSOURCEMASS", "ZMPLUS",   "ENTROPYDIFFUSION", "THERMALCOOLING",
-                                     "TEMPPRESC",       "ADDMASS",  "DISCEVAPORATION",  "DUSTFLUID",
-                                     "PHOTOEVAPORATION", "CAVITYTORQUE"};
+                                     "TEMPPRESC",       "ADDMASS",  "DISCEVAPORATION",  "PHOTOEVAPORATION",
+                                     "CAVITYTORQUE"};
   for (size_t i = 0; i < sizeof kOff / sizeof kOff[0] && rc == DFARGO_OK; i++)
     if (C(v, kOff[i]) == 'Y') rc = dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "%s = YES is not implemented", kOff[i]);
   if (rc == DFARGO_OK && (C(v, "RADIATIVEDIFFUSION") == 'E' || C(v, "RADIATIVEDIFFUSION") == 'I'))
@@ -222,6 +245,41 @@ int dfargo_read_par(const char *path, dfargo_params *p, dfargo_run_params *run) 
   }
   free(v);
   return dfargo_check_params(p);
+}
+
+/* The dust-fluid switches of the same file (src/Interpret.c:253-342 and
+ * src/var.c's defaults).  *dust_fluid is DUSTFLUID's value; the switches
+ * are decoded either way.  RESTARTWITHNEWDUST YES is refused (not
+ * implemented). */
+int dfargo_read_dust_fluid_par(const char *path, dfargo_dust_fluid_params *f, int *dust_fluid) {
+  if (!path || !f) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
+  par_values *v = NULL;
+  int rc = read_values(path, &v);
+  if (rc) return rc;
+  dfargo_dust_fluid_default_params(f);
+  if (dust_fluid) *dust_fluid = C(v, "DUSTFLUID") == 'Y';
+  f->sizepart = R(v, "SIZEPART");
+  f->rhopart = R(v, "RHOPART");
+  f->factor_unit_mass = R(v, "FACTORUNITMASS");
+  f->factor_unit_length = R(v, "FACTORUNITLENGTH");
+  f->dust_to_gas_density_ratio = R(v, "DUSTTOGASDENSITYRATIO");
+  f->dviscosity = R(v, "DVISCOSITY");
+  f->dalpha_viscosity = R(v, "DALPHAVISCOSITY");
+  f->daspect_ratio = R(v, "DASPECTRATIO");
+  f->dflaring_index = R(v, "DFLARINGINDEX");
+  f->dust_diff_inn = R(v, "DUSTDIFFINN");
+  f->sfta = C(v, "SFTAPPROX") != 'N';
+  f->dust_feel_disk = C(v, "DUSTFEELDISK") != 'N';
+  f->dust_feedback = C(v, "DUSTFEEDBACK") == 'Y';
+  f->dust_feel_sg = C(v, "DUSTFEELSG") == 'N'   ? DFARGO_DUST_SG_NO
+                    : C(v, "DUSTFEELSG") == 'Z' ? DFARGO_DUST_SG_ZERO_MODE
+                                                : DFARGO_DUST_SG_YES;
+  f->dust_diffusion = C(v, "DUSTDIFFUSION") == 'Y';
+  f->open_inner_dust = C(v, "INNERBOUNDARYDUST") == 'O';
+  if (C(v, "RESTARTWITHNEWDUST") == 'Y')
+    rc = dfargo_set_error(DFARGO_ERR_UNSUPPORTED, "RESTARTWITHNEWDUST is not implemented");
+  free(v);
+  return rc;
 }
 
 int dfargo_read_planet_config(const char *path, const dfargo_params *p, dfargo_planets *sys) {

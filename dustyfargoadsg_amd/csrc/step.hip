@@ -2091,9 +2091,11 @@ template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArg
     }
   }
   // the dust fluid's targets (src/SideEuler.c:844-848,887-926): its
-  // initial profiles, or with DampToAxi its ring means
+  // initial profiles, or with DampToAxi its ring means; with the open dust
+  // edge the fluid is not damped
   double dq0[3] = {0.0, 0.0, 0.0};
-  if (a.df_on) {
+  const bool ddamp = a.df_on && !a.df_open;
+  if (ddamp) {
     if (!a.damp_ini && !a.damp_visc) {
       __shared__ double dpart[EV_TB / 64][3];
       double s[3] = {0.0, 0.0, 0.0};
@@ -2123,7 +2125,7 @@ template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArg
   const bool edge = a.zero_vrad && ((in_zone && (i == 0 || i == 1)) || (!in_zone && i == nr - 1));
   for (int j = tid; j < ns; j += EV_TB) {
     const long l = base + j;
-    if (a.df_on) {
+    if (ddamp) {
       a.dvr[l] = (a.dvr[l] + lambda * dq0[0]) / (1.0 + lambda);
       a.dvt[l] = (a.dvt[l] + lambda * dq0[1]) / (1.0 + lambda);
       a.drho[l] = (a.drho[l] + lambda * dq0[2]) / (1.0 + lambda);

@@ -347,6 +347,14 @@ def read_par(path, lib=None):
     return p, run
 
 
+def read_dust_fluid_par(path, lib=None):
+    """The dust-fluid switches of a .par file: (DustFluidParams, DUSTFLUID on)."""
+    lib = lib or _capi.load()
+    f, on = _capi.DustFluidParams(), ctypes.c_int()
+    _check(lib, lib.dfargo_read_dust_fluid_par(str(path).encode(), ctypes.byref(f), ctypes.byref(on)))
+    return f, bool(on.value)
+
+
 def read_planet_config(path, params, lib=None):
     """A planet .cfg file as the initial planetary system
     (InitPlanetarySystem, src/Psys.c:96-196)."""

@@ -13,7 +13,9 @@ other and a run of one can restart from the other's files:
   exces_mdcp), tab-separated, appended;
 * dims.dat and used_rad.dat as the reference writes them;
 * DAMPTOVISCOUS's 1-D disc: gasdens/gasvrad.ascii_rad.<N>.dat ("%g\t%g"
-  lines of Rmed1D and Sigma1D / Vrad1D), dims1D.dat and used_rad1D.dat.
+  lines of Rmed1D and Sigma1D / Vrad1D), dims1D.dat and used_rad1D.dat;
+* the dust fluid: dustdens, dustvrad, dustvtheta (and Stokes) like the gas
+  fields.
 """
 import os
 
@@ -202,15 +204,25 @@ def read_visc1d_restart(outdir, number, nrad1d):
     return a[:, 0].copy(), a[:, 1].copy()
 
 
-def write_outputs(outdir, number, state, nrad, nsec, temperature=None):
+def write_outputs(outdir, number, state, nrad, nsec, temperature=None, dust_fluid=None, stokes=False):
     """The gas fields of SendOutput (src/Output.c:235-284) at output
     `number`: gasdens, gasvrad, gasvtheta and, given (with the energy
-    equation, where Write_Temperature is on), the Temperature grid."""
+    equation, where Write_Temperature is on), the Temperature grid.  With
+    the dust fluid (a dict of drho, dvr, dvt[, stokes] as
+    Disc.dust_fluid() returns) also dustdens, dustvrad, dustvtheta and, with
+    stokes=True (WRITEDUSTSTOKES), the Stokes grid."""
     write_disk_polar(outdir, "gasdens", number, state["rho"], nrad, nsec)
+    if dust_fluid is not None:
+        write_disk_polar(outdir, "dustdens", number, dust_fluid["drho"], nrad, nsec)
     write_disk_polar(outdir, "gasvrad", number, state["vr"], nrad, nsec)
     write_disk_polar(outdir, "gasvtheta", number, state["vt"], nrad, nsec)
+    if dust_fluid is not None:
+        write_disk_polar(outdir, "dustvrad", number, dust_fluid["dvr"], nrad, nsec)
+        write_disk_polar(outdir, "dustvtheta", number, dust_fluid["dvt"], nrad, nsec)
     if temperature is not None:
         write_disk_polar(outdir, "Temperature", number, temperature, nrad, nsec)
+    if dust_fluid is not None and stokes:
+        write_disk_polar(outdir, "Stokes", number, dust_fluid["stokes"], nrad, nsec)
 
 
 def read_restart(outdir, number, params, rmed):
@@ -233,4 +245,21 @@ def read_restart(outdir, number, params, rmed):
         st["energy"] = None
         if temp is not None and st["rho"] is not None:
             st["energy"] = st["rho"] * temp / (params.adiabatic_index - 1.0)
+    return st
+
+
+def read_restart_dust_fluid(outdir, number, params):
+    """The dust fluid's restart reads (src/Init.c:167-185): dustdens,
+    dustvrad, dustvtheta of output `number` (None where a file is missing).
+    Its vtheta is not shifted back to the fixed frame: the reference's test
+    there assigns instead of comparing (`Corotating = NO`), so it is false,
+    and when planet0.dat records a positive OmegaFrame it also switches the
+    run's frame off; "frame_fixed" is True in that case, and a host that
+    wants the reference's run sets params.corotating = 0 before the
+    restart's first call."""
+    nrad, nsec = params.nrad, params.nsec
+    st = {"drho": read_disk_polar(outdir, "dustdens", number, nrad, nsec),
+          "dvr": read_disk_polar(outdir, "dustvrad", number, nrad, nsec),
+          "dvt": read_disk_polar(outdir, "dustvtheta", number, nrad, nsec)}
+    st["frame_fixed"] = get_from_planet_file(outdir, number, 9, 0) > 0.0
     return st

@@ -236,6 +236,10 @@ typedef struct dfargo_dust_fluid_params {
   int    dust_feedback;              /* DUSTFEEDBACK (NO): drag on the gas    */
   int    dust_feel_sg;               /* DUSTFEELSG, enum dfargo_dust_feel_sg  */
   int    dust_diffusion;             /* DUSTDIFFUSION (NO)                    */
+  int    open_inner_dust;            /* INNERBOUNDARYDUST OPEN: OpenBoundaryd
+                                        (src/SideEuler.c:411-466) instead of
+                                        the wall, and no damping of the fluid
+                                        in the wave-killing zones            */
 } dfargo_dust_fluid_params;
 
 /* The particle arrays of DustSystem (src/types.h:114-146) the step reads
@@ -532,8 +536,7 @@ void dfargo_dust_fluid_default_params(dfargo_dust_fluid_params *f);
 /* Checks the dust-fluid switches against the gas parameters;
  * DFARGO_ERR_UNSUPPORTED for what the GPU path does not implement: FP32
  * storage, dust particles beside the fluid, SFTAPPROX YES with DUSTFEELDISK
- * NO (the reference then leaves DVradInt unset), INNERBOUNDARYDUST OPEN,
- * the energy equation's radiative/entropy diffusion (not built). */
+ * NO (the reference then leaves DVradInt unset). */
 int dfargo_check_dust_fluid_params(const dfargo_params *p, const dfargo_dust_fluid_params *f, int precision);
 /* InitDustDensity (src/Pframeforce.c:344-380) without its noise options:
  * DSigma(Rmed[i]) = Sigma(Rmed[i]) * DUSTTOGASDENSITYRATIO on every cell
@@ -546,6 +549,11 @@ int dfargo_init_dust_fluid_density(const dfargo_params *p, const dfargo_dust_flu
  * DUSTFEEDBACK, the gas velocities of src/Pframeforce.c:609-634; every
  * AlgoGas call advances the fluid with the gas. */
 int dfargo_attach_dust_fluid(dfargo_ctx *ctx, const dfargo_dust_fluid_params *f, const double *drho);
+/* The dust-fluid switches of a .par file (src/Interpret.c:253-342, with
+ * src/var.c's defaults): *dust_fluid takes DUSTFLUID's value.
+ * DFARGO_ERR_UNSUPPORTED for RESTARTWITHNEWDUST YES.  dfargo_read_par
+ * reads the gas's part of the same file. */
+int dfargo_read_dust_fluid_par(const char *path, dfargo_dust_fluid_params *f, int *dust_fluid);
 /* The fluid's fields DRho, DVrad, DVtheta and the Stokes grid of the last
  * sub-step ((nrad + 1) * nsec values each; any pointer may be NULL) to the
  * host, and DRho, DVrad, DVtheta from the host. */

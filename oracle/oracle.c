@@ -170,6 +170,7 @@ static void df_init_stokes(or_ctx *c);
 static void df_init_velocities(or_ctx *c);
 static void df_wall_boundaries(or_ctx *c);
 static void df_subkeplerian_boundary(or_ctx *c);
+static void df_open_boundary(or_ctx *c);
 static double max2(double a, double b) { return (b > a) ? b : a; }
 
 static double *grid(or_ctx *c) { return (double *)calloc(c->gsize, sizeof(double)); }
@@ -2178,6 +2179,7 @@ static void apply_boundary_condition(or_ctx *c) {
   subkeplerian_boundary(c); /* unconditional: src/SideEuler.c:1110-1112 */
   if (c->df_on) df_subkeplerian_boundary(c);
   if (c->p.boundary == DFARGO_BC_OPEN) open_boundary(c);
+  if (c->df_on && c->fp.open_inner_dust) df_open_boundary(c);
   if (c->p.boundary == DFARGO_BC_NONREFLECTING) {
     if (c->p.energy_equation) compute_sound_speed(c); /* src/SideEuler.c:1121-1122 */
     nonreflecting_boundary(c);
@@ -2250,7 +2252,8 @@ static void evanescent_boundary(or_ctx *c, double step) {
     /* the dust fluid's targets (src/SideEuler.c:844-848,887-926): the
        initial profiles, or with DampToAxi its ring means */
     double dvr0 = 0.0, dvt0 = 0.0, dd0 = 0.0;
-    if (c->df_on) {
+    const int ddamp = c->df_on && !c->fp.open_inner_dust;
+    if (ddamp) {
       if (p->damp_to == DFARGO_DAMP_TO_AXI) {
         for (int j = 0; j < ns; j++) {
           const int l = i * ns + j;
@@ -2275,7 +2278,7 @@ static void evanescent_boundary(or_ctx *c, double step) {
       c->vt[l] = (c->vt[l] + lambda * vt0) / (1.0 + lambda);
       c->rho[l] = (c->rho[l] + lambda * d0) / (1.0 + lambda);
       if (p->energy_equation) c->en[l] = (c->en[l] + lambda * e0) / (1.0 + lambda);
-      if (c->df_on) {
+      if (ddamp) {
         c->dvr[l] = (c->dvr[l] + lambda * dvr0) / (1.0 + lambda);
         c->dvt[l] = (c->dvt[l] + lambda * dvt0) / (1.0 + lambda);
         c->drho[l] = (c->drho[l] + lambda * dd0) / (1.0 + lambda);
@@ -3130,6 +3133,27 @@ static void df_subkeplerian_boundary(or_ctx *c) {
   const int nr = c->nr, ns = c->ns;
   for (int j = 0; j < ns; j++) c->dvt[j] = c->DVthetaMed[0] - c->Rmed[0] * c->OmegaFrame;
   for (int j = 0; j < ns; j++) c->dvt[(nr - 1) * ns + j] = c->DVthetaMed[nr - 1] - c->Rmed[nr - 1] * c->OmegaFrame;
+}
+
+/* OpenBoundaryd (src/SideEuler.c:411-466), DontApplySubKeplerian off:
+   zero-gradient density, vr copied from the next ring inwards */
+static void df_open_boundary(or_ctx *c) {
+  const int nr = c->nr, ns = c->ns;
+  double *rho = c->drho, *vr = c->dvr;
+  int i = 2;
+  for (int j = 0; j < ns; j++) {
+    const int l = j + i * ns;
+    rho[l - ns] = rho[l];
+    rho[l - 2 * ns] = rho[l];
+    vr[l] = vr[l + ns];
+    vr[l - ns] = vr[l + ns];
+  }
+  i = nr - 1;
+  for (int j = 0; j < ns; j++) {
+    const int l = j + i * ns;
+    rho[l] = rho[l - ns];
+    vr[l] = vr[l - ns];
+  }
 }
 
 int or_dust_fluid_attach(or_ctx *c, const dfargo_dust_fluid_params *f, const double *drho) {

@@ -26,7 +26,10 @@ and the reference's own structure fix:
   src/Pframeforce.c:644);
 * the CFL limit of the drag term, dt = CFLSECURITY St / Omega, sets the
   sub-step count for small grains without the short-friction-time
-  approximation.
+  approximation;
+* the open dust edge (OpenBoundaryd) leaves the zero-gradient density and
+  copied vr of src/SideEuler.c:411-466, and the wave-killing zones then
+  leave the fluid alone.
 """
 import ctypes
 import math
@@ -225,3 +228,36 @@ def test_drag_time_step_limit(oracle_built):
     dt = 0.5 * (St[1:] / omega[1:, None]).min()  # the CFL loop starts at ring 1
     DT = 37.3 * dt
     assert o.algogas(DT) == math.ceil(DT / dt)
+
+
+def test_open_dust_edge(oracle_built):
+    """INNERBOUNDARYDUST OPEN beside a wall gas edge and damping zones:
+    after a call, rings 0 and 1 hold ring 2's dust density, rings 1 and 2
+    ring 3's vr, ring nrad-1 ring nrad-2's density and vr; the zones damp
+    the gas but not the dust (a dust copy of the gas loses the identity in
+    the zones' interior rings only through the damping)."""
+    p, pl = gas_planet(nrad=24, nsec=48, wave_killing_zones=1, wkz_rmin=0.6, wkz_rmax=2.0,
+                       damp_to=_capi.DAMP_TO_AXI)
+    fp = _capi.dust_fluid_default_params(sfta=0, sizepart=1e-2, open_inner_dust=1)
+    o, st = _oracle(p, pl, fp)
+    o.algogas(2 * math.pi / 40)
+    d = o.dust_fluid()
+    ns, nr = p.nsec, p.nrad
+    R = lambda a, i: a[i * ns:(i + 1) * ns]
+    assert np.array_equal(R(d["drho"], 0), R(d["drho"], 2)) and np.array_equal(R(d["drho"], 1), R(d["drho"], 2))
+    assert np.array_equal(R(d["dvr"], 1), R(d["dvr"], 3)) and np.array_equal(R(d["dvr"], 2), R(d["dvr"], 3))
+    assert np.array_equal(R(d["drho"], nr - 1), R(d["drho"], nr - 2))
+    assert np.array_equal(R(d["dvr"], nr - 1), R(d["dvr"], nr - 2))
+    # no damping of the dust: one sub-step of an undragged fluid with the
+    # zones on and off (the gas differs, the dust does not)
+    fp = _capi.dust_fluid_default_params(sfta=0, dust_feel_disk=0, open_inner_dust=1)
+    o, _ = _oracle(p, pl, fp)
+    assert o.algogas(1e-4) == 1
+    d = o.dust_fluid()
+    q, _ = gas_planet(nrad=24, nsec=48, wave_killing_zones=0)
+    o2, _ = _oracle(q, pl, fp)
+    assert o2.algogas(1e-4) == 1
+    d2 = o2.dust_fluid()
+    zone = slice(3 * ns, 4 * ns)  # Rmed < 0.6, away from the copied edge rings
+    assert np.array_equal(d["drho"][zone], d2["drho"][zone])
+    assert not np.array_equal(o.state()["rho"][zone], o2.state()["rho"][zone])

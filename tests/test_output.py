@@ -195,3 +195,45 @@ def test_visc1d_restart_read(tmp_path, oracle_built):
     _, s_r, _ = r.visc1d()
     _, s_o, _ = o.visc1d()
     assert np.abs(s_r - s_o).max() <= 1e-4 * s_o.max()
+
+
+def test_restart_dust_fluid(tmp_path, oracle_built):
+    """Output 1 with the dust fluid's dustdens/dustvrad/dustvtheta, a fresh
+    oracle restarted from the files, the next call on both: gas and dust
+    agree to the planet files' 14-digit text (short-friction-time
+    approximation, where the stored Stokes grid does not enter the time
+    step)."""
+    from dustyfargoadsg_amd import _capi
+    from dustyfargoadsg_amd.disc import init_dust_fluid_density
+    p, pl = gas_planet(nrad=32, nsec=96, forced_circular=0)
+    pl = dict(pl, feel_disk=[1], feel_others=[1])
+    fp = _capi.dust_fluid_default_params(sizepart=1e-2)
+    drho0 = init_dust_fluid_density(p, fp)
+    o = Oracle(p)
+    o.attach_dust_fluid(fp, drho0)
+    o.initial_state(pl)
+    o.algogas(DT)
+    st, s, df = o.state(), o.status(), o.dust_fluid()
+    output.write_outputs(tmp_path, 1, st, p.nrad, p.nsec, dust_fluid=df, stokes=True)
+    output.write_planet_files(tmp_path, 1, st["planets"], s)
+    assert (tmp_path / "Stokes1.dat").stat().st_size == p.nrad * p.nsec * 8
+    r = Oracle(p)
+    r.attach_dust_fluid(fp, drho0)
+    rst = r.initial_state(pl)
+    back = output.read_restart(tmp_path, 1, p, r.tables()[1])
+    for k in ("rho", "vr", "vt"):
+        rst[k][:] = back[k]
+    rst["planets"] = output.restart_planetary_system(tmp_path, 1, rst["planets"])
+    r.set_state(rst)
+    r.set_time(s.physical_time, 0.0, 0.0)
+    bd = output.read_restart_dust_fluid(tmp_path, 1, p)
+    assert not bd["frame_fixed"]
+    r.set_dust_fluid(bd["drho"], bd["dvr"], bd["dvt"])
+    assert r.algogas(DT) == o.algogas(DT)
+    n = p.nrad * p.nsec
+    a, b = r.state(), o.state()
+    for k in ("rho", "vt"):
+        assert np.abs(a[k][:n] - b[k][:n]).max() <= 1e-11 * np.abs(b[k][:n]).max(), k
+    da, db = r.dust_fluid(), o.dust_fluid()
+    for k in ("drho", "dvt"):
+        assert np.abs(da[k][:n] - db[k][:n]).max() <= 1e-11 * np.abs(db[k][:n]).max(), k

@@ -117,7 +117,6 @@ def test_decoding_rules(tmp_path):
 @pytest.mark.parametrize("extra,code", [
     ("ThermalCooling Yes\n", _capi.ERR_UNSUPPORTED),
     ("RadiativeDiffusion Implicit\n", _capi.ERR_UNSUPPORTED),
-    ("DustFluid Yes\n", _capi.ERR_UNSUPPORTED),
     ("InnerBoundary Accreting\n", _capi.ERR_UNSUPPORTED),
     ("Viscosity 1e-5\nAlphaViscosity 1e-3\n", _capi.ERR_ARG),
 ])
@@ -127,6 +126,28 @@ def test_refusals(tmp_path, extra, code):
     with pytest.raises(DfargoError) as e:
         read_par(_write(tmp_path, base + extra))
     assert e.value.code == code
+
+
+def test_dust_fluid_switches(tmp_path):
+    """DUSTFLUID YES: the gas reader accepts the file, the dust-fluid reader
+    decodes its switches (DUSTFEELSG Z, SFTAPPROX N, DUSTFEEDBACK Y,
+    INNERBOUNDARYDUST OPEN) and refuses RESTARTWITHNEWDUST YES."""
+    from dustyfargoadsg_amd.disc import read_dust_fluid_par
+    base = MANDATORY + "Sigma0 1e-3\nSigmaSlope 0.5\nAspectRatio 0.05\nNrad 32\nNsec 96\n" \
+                       "Rmin 0.4\nRmax 2.5\nAdiabaticIndex 1.4\n"
+    text = base + ("DustFluid Yes\nSizePart 3e-2\nDustFeelSG Z\nSFTApprox N\nDustFeedback Yes\n"
+                   "DViscosity 1e-6\nDustDiffusion Yes\nDustDiffInn 0.6\nDAspectRatio 0.01\n")
+    read_par(_write(tmp_path, text))
+    f, on = read_dust_fluid_par(_write(tmp_path, text))
+    assert on and f.sizepart == 3e-2 and f.dust_feel_sg == _capi.DUST_SG_ZERO_MODE
+    assert (f.sfta, f.dust_feedback, f.dust_diffusion, f.dust_feel_disk) == (0, 1, 1, 1)
+    assert (f.dviscosity, f.dust_diff_inn, f.daspect_ratio, f.rhopart) == (1e-6, 0.6, 0.01, 1.0)
+    assert f.open_inner_dust == 0
+    f, _ = read_dust_fluid_par(_write(tmp_path, text + "InnerBoundaryDust OPEN\n"))
+    assert f.open_inner_dust == 1
+    with pytest.raises(DfargoError) as e:
+        read_dust_fluid_par(_write(tmp_path, text + "RestartWithNewDust Yes\n"))
+    assert e.value.code == _capi.ERR_UNSUPPORTED
 
 
 def test_missing_mandatory_and_binary_planet(tmp_path):

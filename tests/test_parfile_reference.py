@@ -10,7 +10,7 @@ import os
 import pytest
 
 from dustyfargoadsg_amd import _capi
-from dustyfargoadsg_amd.disc import DfargoError, read_par, read_planet_config
+from dustyfargoadsg_amd.disc import DfargoError, read_dust_fluid_par, read_par, read_planet_config
 
 IN = "/root/reference/in"
 pytestmark = pytest.mark.skipif(not os.path.isdir(IN), reason="the reference's in/ files are not present")
@@ -38,6 +38,19 @@ def test_template_par():
     assert (p.nrad, p.nsec, p.log_grid, p.alpha_viscosity, p.mass_taper) == (200, 400, 1, 1e-4, 1.0)
     assert p.corotating == _capi.FRAME_COROTATING and p.boundary == _capi.BC_NONREFLECTING
     assert (run.ntot, run.ninterm, run.output_dir) == (200, 20, b"out1_wall_damp")
+
+
+def test_template_par_dust_fluid_switches():
+    """template.par leaves DustFluid off but sets its switches: they decode
+    as written (DustDiffusion Yes, DustToGasDensityRatio 1e-2, DAspectRatio
+    0, Rhopart 1.3, FactorUnitLength 10, DustFeelSG No, SFTApprox No,
+    InnerBoundaryDust O) with src/var.c's defaults for the rest."""
+    f, on = read_dust_fluid_par(os.path.join(IN, "template.par"))
+    assert not on and f.open_inner_dust == 1
+    assert (f.dust_diffusion, f.dust_to_gas_density_ratio, f.daspect_ratio) == (1, 1e-2, 0.0)
+    assert (f.rhopart, f.factor_unit_length, f.factor_unit_mass) == (1.3, 10.0, 1.0)
+    assert (f.dust_feel_sg, f.sfta, f.dust_feel_disk, f.dust_feedback) == (_capi.DUST_SG_NO, 0, 1, 0)
+    assert (f.sizepart, f.dviscosity, f.dalpha_viscosity, f.dflaring_index) == (1e-3, 0.0, 0.0, 0.0)
 
 
 def test_template_cfg():
