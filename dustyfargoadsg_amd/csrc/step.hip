@@ -26,6 +26,10 @@ const char *kKernelNames[KID_COUNT] = {
 // helpers
 // ------------------------------------------------------------------------
 
+// The dust fluid runs with FP64 storage only: its branches in the shared
+// kernels are compiled out of the FP32 instantiations.
+#define DF_ON(T, a) (sizeof(T) == sizeof(double) && (a).df_on)
+
 template <typename T> __device__ __forceinline__ double ld(const void *p, long i) {
   return (double)((const T *)p)[i];
 }
@@ -446,7 +450,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
   if (t == 0) a.vmoy[i] = vmoy;
   // the dust fluid's ring mean DVmoy (src/SourceEuler.c:1352-1361)
   double dvmoy = 0.0;
-  if (a.df_on) {
+  if (DF_ON(T, a)) {
     double sd = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) sd += a.dvt[base + t + k * TB];
@@ -486,7 +490,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
       // that order: the smallest dt of the ring is that of its largest S
       R S = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 +
             invdt6 * invdt6;
-      if (a.df_on) {
+      if (DF_ON(T, a)) {
         // the dust fluid's limits (src/SourceEuler.c:1430-1471): dt is the
         // smaller of the two fluids', i.e. the larger S
         R invdt7 = R(1e-10), invdt8 = R(1e-10), invdt9 = R(1e-10), invdt10 = R(1e-10);
@@ -515,7 +519,7 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
     }
     dtmin = R(DF_CFLSECURITY) / sqrt(smax);
     // max2(dt, 1e-10) of the two-fluid limit
-    if (a.df_on && dtmin < 1.e-10) dtmin = 1.e-10;
+    if (DF_ON(T, a) && dtmin < 1.e-10) dtmin = 1.e-10;
   }
   // per-ring partials, stored partial-major (ring_part[p * nr + i]):
   // negative-density and negative-energy counts, disc mass, barycentre.
@@ -1289,7 +1293,7 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
       vt2 = vt2 * vt2;
       vri = (double)pf_vr[it] + dt * (-gradp - gradphi + vt2 * STAB(ST_IRINF, i));
       if (a.d_on && a.d_feedback) vri += dt * ld<T>(a.radfbacc, l);
-      if (a.df_feedback) {
+      if (DF_ON(T, a) && a.df_feedback) {
         // the dust fluid's drag on the gas, implicit (src/SourceEuler.c:915-921)
         const double ts = a.dstokes[l] * a.r15[i], eps = a.drho[l] / rl;
         vri = (vri * ts + eps * a.dvr[l] * dt) / (ts + dt * eps);
@@ -1315,7 +1319,7 @@ template <typename T> __global__ void __launch_bounds__(SrcBlock<T>::n) k_source
     vti = vti - dt * (gradp + gradphi);
     vti += dt * STAB(ST_SUPP, i);
     if (a.d_on && a.d_feedback) vti += dt * ld<T>(a.azifbacc, l);
-    if (a.df_feedback) {
+    if (DF_ON(T, a) && a.df_feedback) {
       // src/SourceEuler.c:991-997, with the dust's vtheta after CorrectVtheta
       const double ts = a.dstokes[l] * a.r15[i], eps = a.drho[l] / rl;
       const double dvtl = a.frame ? a.dvt[l] - dom * STAB(ST_RMED, i) : a.dvt[l];
@@ -1658,7 +1662,7 @@ template <typename T> __global__ void __launch_bounds__(RT_C * RT_R) k_transport
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     if (k == 4 && !a.energy) continue;
-    if (k < 4 && a.tr_skip_mom) {
+    if (k < 4 && DF_ON(T, a) && a.tr_skip_mom) {
       ((T *)outp[k])[l] = (T)Q[k];
       continue;
     }
@@ -1733,7 +1737,7 @@ __device__ __forceinline__ void azi_pass_lane(const KArgs &a, R (&Q)[6], R v, R 
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     if (q == 4 && !a.energy) continue;
-    if (q < 4 && a.tr_skip_mom) continue;
+    if (q < 4 && DF_ON(T, a) && a.tr_skip_mom) continue;
     const R S = azi_star_lane<T>(Q[q] * inv, v, dt, dxtheta, invdxtheta);
     const R f = dxrad * S * RS * v;
     const R fn = lane_next(f);
@@ -1770,7 +1774,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_transport_azi(KAr
     // distance of Ntilde from the nearest rounding tie, min over the call
     // (the gas's shift)
     const double margin = 0.5 - fabs(Ntilde - Nround);
-    if (!a.tr_dust)
+    if (!(DF_ON(T, a) && a.tr_dust))
       atomicMin(&a.ctrl->fargo_margin_bits, (unsigned long long)__double_as_longlong(margin > 0.0 ? margin : 0.0));
   }
   // mass through the inner edge (VanLeerRadial's LostByDisk, ring 0;
@@ -1780,7 +1784,7 @@ template <typename T> __global__ void __launch_bounds__(256) k_transport_azi(KAr
     for (int jj = lane; jj < ns; jj += AZ_LANES) lost += a.lost_col[jj];
     lost = wave_sum(lost);
     if (lane == 0) {
-      if (a.tr_dust) a.ctrl->lost_massd += lost;
+      if (DF_ON(T, a) && a.tr_dust) a.ctrl->lost_massd += lost;
       else a.ctrl->lost_mass += lost;
     }
   }
@@ -2094,7 +2098,7 @@ template <typename T> __global__ void __launch_bounds__(EV_TB) k_evanescent(KArg
   // initial profiles, or with DampToAxi its ring means; with the open dust
   // edge the fluid is not damped
   double dq0[3] = {0.0, 0.0, 0.0};
-  const bool ddamp = a.df_on && !a.df_open;
+  const bool ddamp = DF_ON(T, a) && !a.df_open;
   if (ddamp) {
     if (!a.damp_ini && !a.damp_visc) {
       __shared__ double dpart[EV_TB / 64][3];
