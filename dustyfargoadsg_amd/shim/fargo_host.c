@@ -28,6 +28,7 @@
 void FillPolar1DArrays(void);
 void InitGasDensity(PolarGrid *);
 void InitGasEnergy(PolarGrid *);
+void InitDustDensity(PolarGrid *);
 void InitEuler(PolarGrid *, PolarGrid *, PolarGrid *, PolarGrid *, PolarGrid *, PolarGrid *, PolarGrid *,
                PlanetarySystem *);
 void AlgoGas(Force *, PolarGrid *, PolarGrid *, PolarGrid *, PolarGrid *, PolarGrid *, PolarGrid *, PolarGrid *,
@@ -49,6 +50,7 @@ real EXCLUDEHILLFACTOR, RELEASEDATE, RELEASERADIUS, ECCENTRICITY, OMEGAFRAME;
 real WKZRMIN, WKZRMAX, DAMPINGTODYNAMICALTIMERATIO, RMIN1D, RMAX1D;
 real SIZEMINPART, SIZEMAXPART, SIZEPARTSLOPE, RHOPART, DUSTSLOPE, RMINDUST, RMAXDUST;
 real DUSTTIMESTEP, FACTORUNITMASS = 1.0, FACTORUNITLENGTH = 1.0, DASPECTRATIO, LAMBDADOUBLING;
+real DUSTTOGASDENSITYRATIO, DVISCOSITY, DALPHAVISCOSITY, DFLARINGINDEX, DUSTDIFFINN, Sizepart;
 real floordens, MINDT = 100.0, MAXDT = 0.00001;
 char INTERPOLATION[512] = "TSC", OUTPUTDIR[512];
 boolean FastTransport = YES, GuidingCenter, BinaryCenter, Indirect_Term = YES, Discard_GasIndirect_term;
@@ -60,11 +62,12 @@ boolean DustFeelPlanets = YES, DustFeelTurb, DustGrowth, ZZIntegrator, NoTimeste
 boolean DustFeedback, RemoveDustFromPlanetsHillRadius = YES, DustFeelSGZeroMode;
 boolean BC1D_SS_ZeroVel, BC1D_SS_NonZeroVel, ImposeZeroRadialVelocityBC;
 boolean OpenInner, Restart, ExcludeHill, LogGrid;
+boolean DustFluid, DustDiffusion, OpenInnerDust;
 int CPU_Rank = 0, CPU_Number = 1, IMIN = 0, GLOBALNRAD, Zero_or_active, Max_or_active;
 real Rinf[MAX1D], Rsup[MAX1D], Rmed[MAX1D], Surf[MAX1D], InvRmed[MAX1D], InvSurf[MAX1D];
 real InvDiffRmed[MAX1D], InvDiffRsup[MAX1D], InvRinf[MAX1D], Radii[MAX1D], GlobalRmed[MAX1D];
 real Azimuth[MAX1D], CosAzimuth[MAX1D], SinAzimuth[MAX1D], AziInf[MAX1D], AziSup[MAX1D];
-real InitialPlanetMass[MAX1D], FinalPlanetMass[MAX1D], SigmaMed[MAX1D], EnergyMed[MAX1D];
+real InitialPlanetMass[MAX1D], FinalPlanetMass[MAX1D], SigmaMed[MAX1D], EnergyMed[MAX1D], DSigmaMed[MAX1D];
 real OmegaFrame, PhysicalTime, PhysicalTimeInitial, MassTaper;
 real stardiscbarycenter_x[3], stardiscbarycenter_y[3];
 real Particles_Mass, Particles_Mass_Initial;
@@ -73,6 +76,7 @@ PolarGrid *DivergenceVelocity, *TAURR, *TAUPP, *TAURP, *ViscHeat, *gr, *gtheta, 
 PolarGrid *RadFBAcc, *AziFBAcc, *FBedot, *Test;
 
 static dfargo_params P; /* the decoded .par file */
+static dfargo_dust_fluid_params F; /* its dust-fluid switches */
 
 /* ---- the reference host's helpers --------------------------------------- */
 
@@ -141,6 +145,15 @@ void FillSigma(void) {
   p.add_noise = 0;
   if (f && !dfargo_init_gas_density(&p, f))
     for (i = 0; i < NRAD; i++) SigmaMed[i] = f[i * NSEC];
+  free(f);
+}
+
+/* FillDSigma (src/Theo.c:95-101): DSigma at Rmed */
+void FillDSigma(void) {
+  real *f = (real *)calloc((size_t)(NRAD + 1) * NSEC, sizeof(real));
+  int i;
+  if (f && !dfargo_init_dust_fluid_density(&P, &F, f))
+    for (i = 0; i < NRAD; i++) DSigmaMed[i] = f[i * NSEC];
   free(f);
 }
 
@@ -255,6 +268,22 @@ int main(int argc, char **argv) {
     return 1;
   }
   globals_from_params(&P, &run);
+  {
+    int on = 0;
+    if (dfargo_read_dust_fluid_par(argv[1], &F, &on)) {
+      fprintf(stderr, "%s\n", dfargo_last_error());
+      return 1;
+    }
+    /* src/Interpret.c:253-342 */
+    DustFluid = on; Sizepart = F.sizepart; RHOPART = F.rhopart;
+    FACTORUNITMASS = F.factor_unit_mass; FACTORUNITLENGTH = F.factor_unit_length;
+    DUSTTOGASDENSITYRATIO = F.dust_to_gas_density_ratio; DVISCOSITY = F.dviscosity;
+    DALPHAVISCOSITY = F.dalpha_viscosity; DASPECTRATIO = F.daspect_ratio; DFLARINGINDEX = F.dflaring_index;
+    DUSTDIFFINN = F.dust_diff_inn; ShortFrictionTimeApproximation = F.sfta; DustFeelDisk = F.dust_feel_disk;
+    DustFeedback = F.dust_feedback; DustFeelSG = F.dust_feel_sg != DFARGO_DUST_SG_NO;
+    DustFeelSGZeroMode = F.dust_feel_sg == DFARGO_DUST_SG_ZERO_MODE; DustDiffusion = F.dust_diffusion;
+    OpenInnerDust = F.open_inner_dust;
+  }
   ntot = argc > 2 ? atoi(argv[2]) : run.ntot;
   memset(&pl, 0, sizeof pl);
   if (run.planet_config[0] && dfargo_read_planet_config(run.planet_config, &P, &pl)) {
@@ -275,6 +304,7 @@ int main(int argc, char **argv) {
   FillPolar1DArrays();
   sys = planetary_system(&pl);
   InitGasDensity(rho);
+  if (DustFluid) InitDustDensity(drho);
   if (EnergyEquation) InitGasEnergy(en);
   if (SelfGravity) {
     compute_selfgravity(rho, vr, vt, dvr, dvt, 0.0, NO);
@@ -291,6 +321,11 @@ int main(int argc, char **argv) {
   write_field(rho, "gasdens", 0);
   write_field(vr, "gasvrad", 0);
   write_field(vt, "gasvtheta", 0);
+  if (DustFluid) {
+    write_field(drho, "dustdens", 0);
+    write_field(dvr, "dustvrad", 0);
+    write_field(dvt, "dustvtheta", 0);
+  }
   write_planets(sys, 0);
   /* src/main.c:302-397 */
   for (i = 1; i <= ntot; i++) {
@@ -303,6 +338,11 @@ int main(int argc, char **argv) {
       write_field(vr, "gasvrad", out);
       write_field(vt, "gasvtheta", out);
       if (EnergyEquation) write_field(en, "gasenergy", out);
+      if (DustFluid) {
+        write_field(drho, "dustdens", out);
+        write_field(dvr, "dustvrad", out);
+        write_field(dvt, "dustvtheta", out);
+      }
       write_planets(sys, out);
     }
   }

@@ -74,6 +74,7 @@ extern real EXCLUDEHILLFACTOR, RELEASEDATE, RELEASERADIUS, ECCENTRICITY, OMEGAFR
 extern real WKZRMIN, WKZRMAX, DAMPINGTODYNAMICALTIMERATIO, RMIN1D, RMAX1D;
 extern real SIZEMINPART, SIZEMAXPART, SIZEPARTSLOPE, RHOPART, DUSTSLOPE, RMINDUST, RMAXDUST;
 extern real DUSTTIMESTEP, FACTORUNITMASS, FACTORUNITLENGTH, DASPECTRATIO, LAMBDADOUBLING;
+extern real DUSTTOGASDENSITYRATIO, DVISCOSITY, DALPHAVISCOSITY, DFLARINGINDEX, DUSTDIFFINN, Sizepart;
 extern real floordens; /* DENSITYFLOOR (src/var.c) */
 extern char INTERPOLATION[512], OUTPUTDIR[512];
 
@@ -87,6 +88,7 @@ extern boolean DustFeelTurb, DustGrowth, ZZIntegrator, NoTimestepConstraintByPar
 extern boolean DustFeedback, RemoveDustFromPlanetsHillRadius, DustFeelSGZeroMode;
 extern boolean BC1D_SS_ZeroVel, BC1D_SS_NonZeroVel, ImposeZeroRadialVelocityBC;
 extern boolean OpenInner, Restart, ExcludeHill, LogGrid;
+extern boolean DustFluid, DustDiffusion, OpenInnerDust;
 
 /* ---- tables and scalars (src/global.h) ------------------------------------ */
 extern int CPU_Rank, CPU_Number, IMIN, GLOBALNRAD, Zero_or_active, Max_or_active;
@@ -94,6 +96,7 @@ extern real Rinf[MAX1D], Rsup[MAX1D], Rmed[MAX1D], Surf[MAX1D], InvRmed[MAX1D], 
 extern real InvDiffRmed[MAX1D], InvDiffRsup[MAX1D], InvRinf[MAX1D], Radii[MAX1D], GlobalRmed[MAX1D];
 extern real Azimuth[MAX1D], CosAzimuth[MAX1D], SinAzimuth[MAX1D], AziInf[MAX1D], AziSup[MAX1D];
 extern real InitialPlanetMass[MAX1D], FinalPlanetMass[MAX1D];
+extern real DSigmaMed[MAX1D];
 extern real OmegaFrame, PhysicalTime, PhysicalTimeInitial, MassTaper;
 extern real MINDT, MAXDT; /* MINDT, MAXDT of src/var.c */
 extern real stardiscbarycenter_x[3], stardiscbarycenter_y[3];

@@ -31,6 +31,7 @@
 
 void FillSigma(void);  /* src/Theo.c:87-93 */
 void FillEnergy(void); /* src/Theo.c:172-206 */
+void FillDSigma(void); /* src/Theo.c:95-101 */
 
 /* the functions this file defines, with the parameter types of their
  * definitions in the removed objects (src/proto.h declares them without
@@ -69,6 +70,7 @@ real LostMass = 0.0;
 static dfargo_ctx *ctx = NULL;
 static dfargo_dust_params dpar;
 static int dust_attached = 0;
+static int fluid_attached = 0; /* the dust fluid is on the context */
 
 static void die(int rc) {
   if (rc) {
@@ -234,9 +236,30 @@ void InitGasEnergy(PolarGrid *energ) {
   die(dfargo_init_gas_energy(&p, energ->Field));
 }
 
+/* the dust-fluid switches (src/Interpret.c:253-342) as a parameter block */
+static void dust_fluid_params_from_globals(dfargo_dust_fluid_params *f) {
+  dfargo_dust_fluid_default_params(f);
+  f->sizepart = Sizepart; f->rhopart = RHOPART;
+  f->factor_unit_mass = FACTORUNITMASS; f->factor_unit_length = FACTORUNITLENGTH;
+  f->dust_to_gas_density_ratio = DUSTTOGASDENSITYRATIO;
+  f->dviscosity = DVISCOSITY; f->dalpha_viscosity = DALPHAVISCOSITY;
+  f->daspect_ratio = DASPECTRATIO; f->dflaring_index = DFLARINGINDEX; f->dust_diff_inn = DUSTDIFFINN;
+  f->sfta = ShortFrictionTimeApproximation; f->dust_feel_disk = DustFeelDisk;
+  f->dust_feedback = DustFeedback;
+  f->dust_feel_sg = !DustFeelSG ? DFARGO_DUST_SG_NO : DustFeelSGZeroMode ? DFARGO_DUST_SG_ZERO_MODE
+                                                                          : DFARGO_DUST_SG_YES;
+  f->dust_diffusion = DustDiffusion; f->open_inner_dust = OpenInnerDust;
+}
+
+/* InitDustDensity (src/Pframeforce.c:344-380) without its noise options */
 void InitDustDensity(PolarGrid *DRho) {
-  (void)DRho;
-  unsupported("DUSTFLUID");
+  dfargo_params p;
+  dfargo_dust_fluid_params f;
+  if (AddNoise) unsupported("ADDNOISE with DUSTFLUID (the dust's draws continue the gas's drand48 stream)");
+  FillDSigma();
+  params_from_globals(&p);
+  dust_fluid_params_from_globals(&f);
+  die(dfargo_init_dust_fluid_density(&p, &f, DRho->Field));
 }
 
 /* InitEuler (src/SourceEuler.c:190-291): the device context, the host
@@ -245,7 +268,6 @@ void InitDustDensity(PolarGrid *DRho) {
 void InitEuler(PolarGrid *Vr, PolarGrid *Vt, PolarGrid *Rho, PolarGrid *Energy, PolarGrid *DVr, PolarGrid *DVt,
                PolarGrid *DRho, PlanetarySystem *sys) {
   dfargo_planets pl;
-  (void)DVr; (void)DVt; (void)DRho;
   ensure_context();
   grid_once(&Potential, "Potential");
   grid_once(&Pressure, "Pressure");
@@ -263,7 +285,16 @@ void InitEuler(PolarGrid *Vr, PolarGrid *Vt, PolarGrid *Rho, PolarGrid *Energy, 
   InitComputeAccel();
   die(dfargo_set_time(ctx, PhysicalTime, PhysicalTimeInitial, OmegaFrame));
   planets_to_abi(sys, &pl);
+  /* the dust fluid starts from the host's DRho (src/SourceEuler.c:265-269) */
+  if (DustFluid) {
+    dfargo_dust_fluid_params f;
+    dust_fluid_params_from_globals(&f);
+    if (!fluid_attached) die(dfargo_attach_dust_fluid(ctx, &f, DRho->Field));
+    else die(dfargo_upload_dust_fluid(ctx, DRho->Field, NULL, NULL));
+    fluid_attached = 1;
+  }
   die(dfargo_init_euler(ctx, Rho->Field, Vr->Field, Vt->Field, EnergyEquation ? Energy->Field : NULL, &pl));
+  if (DustFluid) die(dfargo_download_dust_fluid(ctx, DRho->Field, DVr->Field, DVt->Field, NULL));
   /* the planets as the device holds them (with self-gravity their
      start-up velocities are corrected, src/sgsysinit.c:3-49) */
   die(dfargo_download_state(ctx, NULL, NULL, NULL, NULL, &pl));
@@ -283,10 +314,12 @@ void AlgoGas(Force *force, PolarGrid *Rho, PolarGrid *Vrad, PolarGrid *Vtheta, P
   dfargo_planets pl;
   dfargo_status st;
   int k;
-  (void)force; (void)Label; (void)DRho; (void)DVrad; (void)DVtheta;
+  (void)force; (void)Label;
   planets_to_abi(sys, &pl);
   die(dfargo_set_time(ctx, PhysicalTime, PhysicalTimeInitial, OmegaFrame));
+  if (DustFluid) die(dfargo_upload_dust_fluid(ctx, DRho->Field, DVrad->Field, DVtheta->Field));
   die(dfargo_algogas(ctx, DT, Rho->Field, Vrad->Field, Vtheta->Field, EnergyEquation ? Energy->Field : NULL, &pl));
+  if (DustFluid) die(dfargo_download_dust_fluid(ctx, DRho->Field, DVrad->Field, DVtheta->Field, NULL));
   planets_from_abi(&pl, sys);
   if (NBPART != 0 && dust_attached && dsys) {
     dfargo_dust_state ds;

@@ -85,7 +85,7 @@ def test_shim_defines_the_26_boundary_symbols(shim_obj):
 def test_shim_imports_only_the_abi_libc_and_host(shim_obj):
     lib_syms = set(subprocess.run(["nm", "-D", "--defined-only", build.OUT], capture_output=True, text=True,
                                   check=True).stdout.split())
-    host = _ref_host_names() | {"FillSigma", "FillEnergy"}
+    host = _ref_host_names() | {"FillSigma", "FillEnergy", "FillDSigma"}
     for name in _symbols(shim_obj, True):
         if name.startswith("dfargo_"):
             assert name in lib_syms, name
