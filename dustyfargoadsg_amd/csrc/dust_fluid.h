@@ -23,11 +23,12 @@
 
 // ---- the grain's Stokes number: SubStep1 (src/SourceEuler.c:820-858) ----
 // Paardekooper (2007): the drag coefficient from the Knudsen number, the
-// Mach number of the relative velocity and the Reynolds number.
-__device__ __forceinline__ double df_stokes_at(const KArgs &a, int i, double rho, double dvr, double dvt,
+// Mach number of the relative velocity and the Reynolds number.  lam is the
+// ring's 3.34e-8 AspectRatio(Rmed) Rmed / df_to_cgs, so that the mean free
+// path is lam / Sigma (pow(x, -1) of the reference taken as 1 / x).
+__device__ __forceinline__ double df_stokes_at(const KArgs &a, double lam, double rho, double dvr, double dvt,
                                                double cs) {
-  const double densg_cgs = rho * a.df_to_cgs;
-  const double lambda = 3.34e-8 * pow(densg_cgs, -1.) * d_aspect_ratio(a, a.Rmed[i]) * a.Rmed[i];
+  const double lambda = lam / rho;
   const double Kn = 0.5 * lambda / a.df_radius;
   const double Mach = sqrt(dvr * dvr + dvt * dvt) / cs;
   const double f_D = sqrt(1.0 + 9.0 * DF_PI * Mach * Mach / 128.0);
@@ -57,45 +58,63 @@ __global__ void __launch_bounds__(256) k_df_axisg(KArgs a) {
 }
 
 // The dust's part of SubStep1 (src/SourceEuler.c:805-999) and of
-// update_sgvelocity (src/sgupdate.c:29-70), one cell per thread, rings
-// 0 .. nrad-1: DVradInt on rings >= 1, DVthetaInt everywhere.  In a
-// corotating frame both fluids' vtheta are read through CorrectVtheta
+// update_sgvelocity (src/sgupdate.c:29-70): DVradInt on rings >= 1,
+// DVthetaInt everywhere.  Tiles of DFS_R rings x DFS_C columns; the
+// potential, both pressures, both densities and the dust's vtheta of the
+// tile and its inner and preceding neighbours are staged in LDS once (the
+// potential, a planet sum per cell, is the costly part).  In a corotating
+// frame both fluids' vtheta are read through CorrectVtheta
 // (src/SourceEuler.c:516-523).
-__global__ void __launch_bounds__(256) k_df_source(KArgs a) {
+enum { DFS_R = 4, DFS_C = 64 };
+__global__ void __launch_bounds__(DFS_R * DFS_C) k_df_source(KArgs a) {
+  constexpr int PR = DFS_R + 1, PC = DFS_C + 2; // rings i0-1 .., columns j0-1 .. j0+DFS_C
+  __shared__ double sPot[PR][PC], sP[PR][PC], sRho[PR][PC], sD[PR][PC], sDP[PR][PC], sDvt[PR][PC];
+  __shared__ double sLam[PR];
   const dfargo_ctrl *c = a.ctrl;
   if (!c->active) return;
-  const int i = blockIdx.y, ns = a.ns;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
-  const long l = (long)i * ns + j, lim = l - ns;
-  const int jm = (j == 0) ? ns - 1 : j - 1, jp = (j == ns - 1) ? 0 : j + 1;
-  const long ljm = (long)i * ns + jm, ljp = (long)i * ns + jp;
+  const int ns = a.ns, nr = a.nr, tid = threadIdx.x;
+  const int j0 = blockIdx.x * DFS_C, i0 = blockIdx.y * DFS_R;
+  const double dt = c->dt, dom = a.frame ? c->domega : 0.0;
   const double *rho = (const double *)a.rho, *vr = (const double *)a.vr, *vt = (const double *)a.vt;
   const double *drho = a.drho, *dvr = a.dvr, *dvt = a.dvt;
-  const double dt = c->dt, dom = a.frame ? c->domega : 0.0;
+  for (int idx = tid; idx < PR * PC; idx += DFS_R * DFS_C) {
+    const int q = idx / PC, p = idx - q * PC, r = i0 - 1 + q;
+    int jj = (j0 - 1 + p) % ns;
+    if (jj < 0) jj += ns;
+    const bool in = (r >= 0 && r <= nr - 1);
+    const long m = (long)r * ns + jj;
+    sPot[q][p] = in ? pot_at(a, c, r, jj) : 0.0;
+    sP[q][p] = in ? src_press<double>(a, m, r) : 0.0;
+    sRho[q][p] = in ? rho[m] : 0.0;
+    sD[q][p] = in ? drho[m] : 0.0;
+    sDP[q][p] = in ? drho[m] * a.df_cs[r] * a.df_cs[r] : 0.0;
+    sDvt[q][p] = in ? (a.frame ? dvt[m] - dom * a.Rmed[r] : dvt[m]) : 0.0;
+  }
+  if (tid < PR) {
+    const int r = i0 - 1 + tid;
+    sLam[tid] = (r >= 0 && r <= nr - 1) ? 3.34e-8 * d_aspect_ratio(a, a.Rmed[r]) * a.Rmed[r] / a.df_to_cgs : 0.0;
+  }
+  __syncthreads();
+  const int ty = tid / DFS_C, tx = tid - ty * DFS_C, i = i0 + ty, j = j0 + tx;
+  if (i > nr - 1 || j >= ns) return;
+  const int q = ty + 1, p = tx + 1; // the cell's LDS slot
+  const long l = (long)i * ns + j, lim = l - ns;
   const double ci = dom * a.Rmed[i];
-  const double rl = rho[l], vtl = a.frame ? vt[l] - ci : vt[l], dvtl = a.frame ? dvt[l] - ci : dvt[l];
+  const double rl = sRho[q][p], vtl = a.frame ? vt[l] - ci : vt[l], dvtl = sDvt[q][p];
   const double cs = a.energy ? ((const double *)a.cs)[l] : a.cs_iso[i];
-  const double St = df_stokes_at(a, i, rl, dvr[l] - vr[l], dvtl - vtl, cs);
+  const double St = df_stokes_at(a, sLam[q], rl, dvr[l] - vr[l], dvtl - vtl, cs);
   a.dstokes[l] = St;
   const double invomega = a.r15[i];
-  const double pl = src_press<double>(a, l, i), potl = pot_at(a, c, i, j);
-  const double dcs = a.df_cs[i], dpl = drho[l] * dcs * dcs;
+  const double pl = sP[q][p], potl = sPot[q][p], dl = sD[q][p], dpl = sDP[q][p];
   const int dsg = a.sg ? a.df_feel_sg : DFARGO_DUST_SG_NO;
   if (i >= 1) {
     const double idrm = a.InvDiffRmed[i];
-    const double gradp = (pl - src_press<double>(a, lim, i - 1)) * 2.0 / (rl + rho[lim]) * idrm;
-    const double gradphi = (potl - pot_at(a, c, i - 1, j)) * idrm;
+    const double gradp = (pl - sP[q - 1][p]) * 2.0 / (rl + sRho[q - 1][p]) * idrm;
+    const double gradphi = (potl - sPot[q - 1][p]) * idrm;
     double ts = St * invomega, dvri;
     if (!a.df_sfta) {
-      const double dcsm = a.df_cs[i - 1], dplim = drho[lim] * dcsm * dcsm;
-      const double dgradp = (dpl - dplim) * 2.0 / (drho[l] + drho[lim]) * idrm;
-      const double cim = dom * a.Rmed[i - 1];
-      double dvt2;
-      if (a.frame)
-        dvt2 = (dvt[l] - ci) + (dvt[ljp] - ci) + (dvt[lim] - cim) + (dvt[ljp - ns] - cim);
-      else
-        dvt2 = dvt[l] + dvt[ljp] + dvt[lim] + dvt[ljp - ns];
+      const double dgradp = (dpl - sDP[q - 1][p]) * 2.0 / (dl + sD[q - 1][p]) * idrm;
+      double dvt2 = dvtl + sDvt[q][p + 1] + sDvt[q - 1][p] + sDvt[q - 1][p + 1];
       dvt2 = dvt2 / 4.0 + a.Rinf[i] * c->omega_frame;
       dvt2 = dvt2 * dvt2;
       dvri = dvr[l] + dt * (-gradphi - dgradp + dvt2 * a.InvRinf[i]);
@@ -103,7 +122,7 @@ __global__ void __launch_bounds__(256) k_df_source(KArgs a) {
     } else {
       // the short-friction-time velocity (Johansen & Klahr 2005); the
       // option needs DUSTFEELDISK
-      if (a.df_feedback) ts /= (1.0 + drho[l] / rl);
+      if (a.df_feedback) ts /= (1.0 + dl / rl);
       dvri = vr[l] + ts * gradp;
     }
     if (dsg != DFARGO_DUST_SG_NO) {
@@ -117,19 +136,20 @@ __global__ void __launch_bounds__(256) k_df_source(KArgs a) {
     a.dvri[l] = dvri;
   }
   const double invdxtheta = a.InvDxTheta[i];
-  const double gradp = (pl - src_press<double>(a, ljm, i)) * 2.0 / (rl + rho[ljm]) * invdxtheta;
-  const double gradphi = (potl - pot_at(a, c, i, jm)) * invdxtheta;
+  const double gradp = (pl - sP[q][p - 1]) * 2.0 / (rl + sRho[q][p - 1]) * invdxtheta;
+  const double gradphi = (potl - sPot[q][p - 1]) * invdxtheta;
   double ts = St * invomega, dvti;
   if (!a.df_sfta) {
-    const double dgradp = (dpl - drho[ljm] * dcs * dcs) * 2.0 / (drho[l] + drho[ljm]) * invdxtheta;
+    const double dgradp = (dpl - sDP[q][p - 1]) * 2.0 / (dl + sD[q][p - 1]) * invdxtheta;
     dvti = dvtl - dt * (gradphi + dgradp);
     if (a.df_feel_disk) dvti = (dvti * ts + vtl * dt) / (ts + dt);
   } else {
-    if (a.df_feedback) ts /= (1.0 + drho[l] / rl);
+    if (a.df_feedback) ts /= (1.0 + dl / rl);
     dvti = vtl + ts * gradp;
   }
   if (dsg == DFARGO_DUST_SG_YES && !a.sg_zero) {
     const double *sgt = (const double *)a.sgt;
+    const long ljm = (long)i * ns + ((j == 0) ? ns - 1 : j - 1);
     dvti = dvti + dt * (0.5 * (sgt[l] + sgt[ljm]));
   }
   a.dvti[l] = dvti;
@@ -288,13 +308,15 @@ __device__ __forceinline__ double df_gas_rho(const KArgs &a, long m) {
 __device__ __forceinline__ double df_flux_r(const KArgs &a, int im, long m, double nup) {
   const long mip = m + a.ns;
   const double St = a.dstokes[mip];
-  const double Dturb = nup * (1.0 + 4.0 * St * St) * pow((1.0 + St * St), -2.0);
+  const double q = 1.0 + St * St;
+  const double Dturb = nup * (1.0 + 4.0 * St * St) * (1.0 / (q * q));
   const double r0 = df_gas_rho(a, m), r1 = df_gas_rho(a, mip);
   return -Dturb * (r1 + r0) / 2. * (a.dz_rho[mip] / r1 - a.dz_rho[m] / r0) * a.InvDiffRmed[im + 1];
 }
 __device__ __forceinline__ double df_flux_t(const KArgs &a, int im, long m, long mjp, double nu, double invdtheta) {
   const double St = a.dstokes[m];
-  const double Dturb = nu * (1.0 + 4.0 * St * St) * pow((1.0 + St * St), -2.0);
+  const double q = 1.0 + St * St;
+  const double Dturb = nu * (1.0 + 4.0 * St * St) * (1.0 / (q * q));
   const double r0 = df_gas_rho(a, m), r1 = df_gas_rho(a, mjp);
   return -Dturb * (r1 + r0) / 2. * (a.dz_rho[mjp] / r1 - a.dz_rho[m] / r0) * invdtheta / a.Rmed[im];
 }
@@ -396,7 +418,10 @@ static void launch_df_kernel(int id, const KArgs &a, hipStream_t s) {
   const dim3 g((a.ns + bx - 1) / bx, a.nr), b(bx);
   switch (id) {
     case KID_DF_AXISG: hipLaunchKernelGGL(k_df_axisg, dim3(a.nr), dim3(256), 0, s, a); break;
-    case KID_DF_SOURCE: hipLaunchKernelGGL(k_df_source, g, b, 0, s, a); break;
+    case KID_DF_SOURCE:
+      hipLaunchKernelGGL(k_df_source, dim3((a.ns + DFS_C - 1) / DFS_C, (a.nr + DFS_R - 1) / DFS_R),
+                         dim3(DFS_R * DFS_C), 0, s, a);
+      break;
     case KID_DF_STRESS: hipLaunchKernelGGL(k_df_stress, g, b, 0, s, a); break;
     case KID_DF_VISC: hipLaunchKernelGGL(k_df_visc, g, b, 0, s, a); break;
     case KID_DF_SUBSTEP2: {
