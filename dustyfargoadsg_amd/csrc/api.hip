@@ -332,7 +332,9 @@ static void build_sequence(dfargo_ctx *c) {
     for (int id : {KID_SG_COLS, KID_SG_ROWS_INV}) c->seq.push_back(id);
   else if (c->p.self_gravity)
     for (int id : {KID_SG_ROWS_FWD, KID_SG_COLS, KID_SG_ROWS_INV}) c->seq.push_back(id);
-  for (int id : {KID_CFL, KID_CONTROL}) c->seq.push_back(id);
+  c->seq.push_back(KID_CFL);
+  if (c->a.df_on) c->seq.push_back(KID_DF_CFL);
+  c->seq.push_back(KID_CONTROL);
   if (c->dust_on)
     for (int id : {KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT,
                    KID_DUST_REORDER})
@@ -1590,6 +1592,7 @@ extern "C" int dfargo_attach_dust_fluid(dfargo_ctx *c, const dfargo_dust_fluid_p
   if ((rc = dalloc(c, (void **)&a.dvmoy, sizeof(double) * (nr + 2))) ||
       (rc = dalloc(c, (void **)&a.dvmed, sizeof(double) * (nr + 2))) ||
       (rc = dalloc(c, (void **)&a.df_axisg, sizeof(double) * (nr + 2))) ||
+      (rc = dalloc(c, (void **)&a.df_ring_dt, sizeof(double) * (nr + 2))) ||
       (rc = dalloc(c, (void **)&a.dnshift, sizeof(long) * (nr + 2))) ||
       (rc = dalloc(c, (void **)&a.dvt_part, sizeof(double) * (size_t)nr * ntile)) ||
       (rc = dalloc(c, (void **)&a.dlost_col, sizeof(double) * ns)))

@@ -44,6 +44,61 @@ __device__ __forceinline__ double df_stokes_at(const KArgs &a, double lam, doubl
   return 0.5 * DF_PI * Cdrag * a.df_rho_int * a.df_radius / rho;
 }
 
+// The dust fluid's part of ConditionCFL (src/SourceEuler.c:1352-1361,
+// 1376-1384, 1430-1471): the ring mean DVmoy and the ring's smallest
+// CFLSECURITY / sqrt(sum of the six limits squared), one workgroup per
+// ring; k_control takes each ring's max2(min2(gas, dust), 1e-10) and the
+// dust's shear limit.  Without the short-friction-time approximation the
+// sound speed, motion, residual and drag limits enter; the artificial and
+// the physical viscosity always.  Like the gas's CFL kernel it multiplies
+// by the ring's reciprocal cell sizes.
+__global__ void __launch_bounds__(256) k_df_cfl(KArgs a) {
+  __shared__ double sh[1024];
+  const dfargo_ctrl *c = a.ctrl;
+  if (!c->go) return;
+  const int i = blockIdx.x, ns = a.ns, tid = threadIdx.x, TB = blockDim.x;
+  const long base = (long)i * ns;
+  const double *dvr = a.dvr + base, *dvt = a.dvt + base;
+  double s = 0.0;
+  for (int j = tid; j < ns; j += TB) s += dvt[j];
+  s = block_sum_fast(s, sh);
+  const double dvmoy = s / (double)ns;
+  if (tid == 0) a.dvmoy[i] = dvmoy;
+  if (i < 1) {
+    if (tid == 0) a.df_ring_dt[i] = 1e30;
+    return;
+  }
+  const double dxrad = a.Rsup[i] - a.Rinf[i], dxtheta = a.Rmed[i] * a.pm / (double)ns;
+  const double dxmin = (dxrad < dxtheta) ? dxrad : dxtheta;
+  const double idxmin = 1.0 / dxmin, idxrad = 1.0 / dxrad, idxtheta = 1.0 / dxtheta;
+  const double dnu = a.df_nu[i];
+  const double invdt12 = (dnu != 0.0) ? dnu * 4.0 / a.powmin2[i] : 1e-10;
+  const double invdt7 = a.df_sfta ? 1e-10 : a.df_cs[i] * idxmin;
+  const double om = sqrt(DF_G * 1.0 / a.Rmed[i]) / a.Rmed[i];
+  double smax = 0.0;
+  for (int j = tid; j < ns; j += TB) {
+    const int jp = (j == ns - 1) ? 0 : j + 1;
+    double invdt8 = 1e-10, invdt9 = 1e-10, invdt10 = 1e-10;
+    if (!a.df_sfta) {
+      invdt8 = fabs(dvr[j] + 1e-15) * idxrad;
+      invdt9 = fabs(a.fast ? dvt[j] - dvmoy : dvt[j]) * idxtheta;
+      if (a.df_feel_disk) invdt10 = 1. / (a.dstokes[base + j] + 1.e-10) * om;
+    }
+    double ddvr = dvr[j + ns] - dvr[j], ddvt = dvt[jp] - dvt[j];
+    if (ddvr >= 0.0) ddvr = 1e-10; else ddvr = -ddvr;
+    if (ddvt >= 0.0) ddvt = 1e-10; else ddvt = -ddvt;
+    const double d4 = ddvr * idxrad, d5 = ddvt * idxtheta;
+    double invdt11 = (d5 > d4) ? d5 : d4;
+    invdt11 *= 4.0 * DF_CVNR * DF_CVNR;
+    const double S = invdt7 * invdt7 + invdt8 * invdt8 + invdt9 * invdt9 + invdt10 * invdt10 +
+                     invdt11 * invdt11 + invdt12 * invdt12;
+    smax = (S > smax) ? S : smax;
+  }
+  // the largest sum gives the smallest dt (the map is monotone)
+  smax = -block_min_fast(-smax, sh);
+  if (tid == 0) a.df_ring_dt[i] = DF_CFLSECURITY / sqrt(smax);
+}
+
 // ring means of SG_Accr (mpi_make1Dprofile for DUSTFEELSG Z,
 // src/sgupdate.c:26-27), one workgroup per ring
 __global__ void __launch_bounds__(256) k_df_axisg(KArgs a) {
@@ -417,6 +472,7 @@ static void launch_df_kernel(int id, const KArgs &a, hipStream_t s) {
   const int bx = a.ns < 256 ? a.ns : 256;
   const dim3 g((a.ns + bx - 1) / bx, a.nr), b(bx);
   switch (id) {
+    case KID_DF_CFL: hipLaunchKernelGGL(k_df_cfl, dim3(a.nr), dim3(256), 0, s, a); break;
     case KID_DF_AXISG: hipLaunchKernelGGL(k_df_axisg, dim3(a.nr), dim3(256), 0, s, a); break;
     case KID_DF_SOURCE:
       hipLaunchKernelGGL(k_df_source, dim3((a.ns + DFS_C - 1) / DFS_C, (a.nr + DFS_R - 1) / DFS_R),

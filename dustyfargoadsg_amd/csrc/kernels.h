@@ -122,7 +122,7 @@ struct KArgs {
   double df_rho_int, df_radius, df_to_cgs, df_diff_inn;
   double *drho, *dvr, *dvt, *dstokes, *dvri, *dvti, *dtrr, *dtrp, *dtpp;
   double *dt_rho, *dt_rp, *dt_rm, *dt_tp, *dt_tm, *dz_rho, *dz_rp, *dz_rm, *dz_tp, *dz_tm;
-  double *dvmoy, *dvmed, *dvt_part, *dlost_col, *df_axisg;
+  double *dvmoy, *dvmed, *dvt_part, *dlost_col, *df_axisg, *df_ring_dt;
   long *dnshift;
   const double *df_cs, *df_nu, *df_sup15, *df_cavsup, *DSigmaMed, *DVthetaMed, *DVradMed;
   // the transport kernels run on the dust fluid (Transportd): its lost
@@ -144,7 +144,7 @@ enum KernelId {
   KID_FINISH, KID_DUST_MOVE, KID_DUST_SCAN, KID_DUST_SCATTER, KID_DUST_SORT, KID_DUST_DEPOSIT, KID_DUST_REORDER,
   KID_EVANESCENT, KID_NONREFLECTING,
   KID_SGZ_MEAN, KID_SGZ_CONV, KID_SGZ_FILL, KID_EOS_STEP, KID_VISC1D,
-  KID_DF_AXISG, KID_DF_SOURCE, KID_DF_STRESS, KID_DF_VISC, KID_DF_SUBSTEP2, KID_DF_TRANSPORT_RAD,
+  KID_DF_CFL, KID_DF_AXISG, KID_DF_SOURCE, KID_DF_STRESS, KID_DF_VISC, KID_DF_SUBSTEP2, KID_DF_TRANSPORT_RAD,
   KID_DF_TRANSPORT_AZI, KID_DF_FINISH, KID_COUNT
 };
 extern const char *kKernelNames[KID_COUNT];

@@ -19,7 +19,7 @@ const char *kKernelNames[KID_COUNT] = {
     "transport_azi", "finish",      "dust_move",      "dust_scan",  "dust_scatter",
     "dust_sort",     "dust_deposit", "dust_reorder", "evanescent", "nonreflecting",
     "sgz_mean",      "sgz_conv",     "sgz_fill",     "eos_step",      "visc1d",
-    "df_axisg",      "df_source",    "df_stress",    "df_visc",       "df_substep2",
+    "df_cfl",        "df_axisg",      "df_source",    "df_stress",    "df_visc",       "df_substep2",
     "df_transport_rad", "df_transport_azi", "df_finish"};
 
 // ------------------------------------------------------------------------
@@ -101,6 +101,7 @@ template <> struct TransportArith<float> {
 };
 
 __device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax2(double a, double b) { return (b > a) ? b : a; }
 
 // the sum over a wavefront, the same in every lane (xor butterfly)
 __device__ __forceinline__ double wave_sum(double v) {
@@ -448,16 +449,6 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
   s = block_sum_fast(s, sh);
   const double vmoy = s / (double)ns;
   if (t == 0) a.vmoy[i] = vmoy;
-  // the dust fluid's ring mean DVmoy (src/SourceEuler.c:1352-1361)
-  double dvmoy = 0.0;
-  if (DF_ON(T, a)) {
-    double sd = 0.0;
-#pragma unroll
-    for (int k = 0; k < K; k++) sd += a.dvt[base + t + k * TB];
-    sd = block_sum_fast(sd, sh);
-    dvmoy = sd / (double)ns;
-    if (t == 0) a.dvmoy[i] = dvmoy;
-  }
   double dtmin = 1e30;
   if (i >= 1) {
     const R viscosity = (R)a.nu_ring[i];
@@ -488,38 +479,11 @@ template <typename T, int K> __global__ void __launch_bounds__(RING_LB(K)) k_cfl
       const R invdt6 = R(1e-10);
       // dt = CFLSECURITY / sqrt(S) decreases with S, and rounding keeps
       // that order: the smallest dt of the ring is that of its largest S
-      R S = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 +
-            invdt6 * invdt6;
-      if (DF_ON(T, a)) {
-        // the dust fluid's limits (src/SourceEuler.c:1430-1471): dt is the
-        // smaller of the two fluids', i.e. the larger S
-        R invdt7 = R(1e-10), invdt8 = R(1e-10), invdt9 = R(1e-10), invdt10 = R(1e-10);
-        const double *dvr = a.dvr + base, *dvt = a.dvt + base;
-        const R dvtl = dvt[j];
-        if (!a.df_sfta) {
-          invdt7 = a.df_cs[i] * idxmin;
-          invdt8 = fabs(dvr[j] + 1e-15) * idxrad;
-          invdt9 = fabs(a.fast ? dvtl - (R)dvmoy : dvtl) * idxtheta;
-          if (a.df_feel_disk)
-            invdt10 = 1. / (a.dstokes[base + j] + 1.e-10) * sqrt(DF_G * 1.0 / a.Rmed[i]) / a.Rmed[i];
-        }
-        R ddvr = dvr[j + ns] - dvr[j], ddvt = dvt[jp] - dvtl;
-        if (ddvr >= R(0)) ddvr = R(1e-10); else ddvr = -ddvr;
-        if (ddvt >= R(0)) ddvt = R(1e-10); else ddvt = -ddvt;
-        const R d4 = ddvr * idxrad, d5 = ddvt * idxtheta;
-        R invdt11 = (d5 > d4) ? d5 : d4;
-        invdt11 *= R(4.0 * DF_CVNR * DF_CVNR);
-        const R dnu = a.df_nu[i];
-        const R invdt12 = (dnu != R(0)) ? dnu * R(4) / (R)a.powmin2[i] : R(1e-10);
-        const R Sd = invdt7 * invdt7 + invdt8 * invdt8 + invdt9 * invdt9 + invdt10 * invdt10 +
-                     invdt11 * invdt11 + invdt12 * invdt12;
-        S = (Sd > S) ? Sd : S;
-      }
+      const R S = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 +
+                  invdt6 * invdt6;
       smax = (S > smax) ? S : smax;
     }
     dtmin = R(DF_CFLSECURITY) / sqrt(smax);
-    // max2(dt, 1e-10) of the two-fluid limit
-    if (DF_ON(T, a) && dtmin < 1.e-10) dtmin = 1.e-10;
   }
   // per-ring partials, stored partial-major (ring_part[p * nr + i]):
   // negative-density and negative-energy counts, disc mass, barycentre.
@@ -829,7 +793,13 @@ template <typename T, bool ACC> __global__ void __launch_bounds__(CtrlBlock<ACC>
   __syncthreads();
   dfargo_ctrl *c = &cs;
   double nd = 1e30;
-  for (int i = 1 + t; i < nr; i += TB) nd = dmin2(nd, a.ring_dt[i]);
+  for (int i = 1 + t; i < nr; i += TB) {
+    double v = a.ring_dt[i];
+    // with the dust fluid each cell's limit is max2(min2(dtg, dtd), 1e-10)
+    // (src/SourceEuler.c:1462-1465); both maps are monotone, so per ring
+    if (a.df_on) v = dmax2(dmin2(v, a.df_ring_dt[i]), 1.e-10);
+    nd = dmin2(nd, v);
+  }
   for (int i = t; i < nr - 1; i += TB) {
     double dt = a.pm * DF_CFLSECURITY / (double)ns /
                 fabs(a.vmoy[i] * a.InvRmed[i] - a.vmoy[i + 1] * a.InvRmed[i + 1]);
@@ -2426,6 +2396,7 @@ template <typename T> void launch_kernel(int id, const KArgs &a, hipStream_t s) 
     case KID_DUST_SORT:
     case KID_DUST_DEPOSIT:
     case KID_DUST_REORDER: launch_dust_kernel<T>(id, a, s); break;
+    case KID_DF_CFL:
     case KID_DF_AXISG:
     case KID_DF_SOURCE:
     case KID_DF_STRESS:
