@@ -443,6 +443,7 @@ static int build_context(dfargo_ctx *c) {
   a.xcd_remap = 1;
   a.tune_azi_waves = 4;
   a.tune_finish_block = 256;
+  a.sg_fft_buffers = 0;
   {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, c->dev));
@@ -1748,23 +1749,26 @@ extern "C" int dfargo_get_tuning(dfargo_ctx *c, dfargo_tuning *t) {
   t->xcd_remap = c->a.xcd_remap;
   t->azi_waves = c->a.tune_azi_waves;
   t->finish_block = c->a.tune_finish_block;
+  t->sg_fft_buffers = c->a.sg_fft_buffers;
   return DFARGO_OK;
 }
 
 extern "C" int dfargo_set_tuning(dfargo_ctx *c, const dfargo_tuning *t) {
   if (!c || !t) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
   if ((t->xcd_remap != 0 && t->xcd_remap != 1) || (t->azi_waves != 1 && t->azi_waves != 2 && t->azi_waves != 4) ||
-      (t->finish_block != 64 && t->finish_block != 128 && t->finish_block != 256))
+      (t->finish_block != 64 && t->finish_block != 128 && t->finish_block != 256) || t->sg_fft_buffers < 0 ||
+      t->sg_fft_buffers > 2)
     return dfargo_set_error(DFARGO_ERR_ARG, "tuning out of range (xcd_remap 0/1, azi_waves 1/2/4, "
-                                            "finish_block 64/128/256)");
+                                            "finish_block 64/128/256, sg_fft_buffers 0/1/2)");
   if (t->xcd_remap == c->a.xcd_remap && t->azi_waves == c->a.tune_azi_waves &&
-      t->finish_block == c->a.tune_finish_block)
+      t->finish_block == c->a.tune_finish_block && t->sg_fft_buffers == c->a.sg_fft_buffers)
     return DFARGO_OK;
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->a.xcd_remap = t->xcd_remap;
   c->a.tune_azi_waves = t->azi_waves;
   c->a.tune_finish_block = t->finish_block;
+  c->a.sg_fft_buffers = t->sg_fft_buffers;
   return drop_graph(c);
 }
 
@@ -1821,25 +1825,30 @@ extern "C" int dfargo_autotune(dfargo_ctx *c, int substeps, dfargo_tuning *best_
     ms[0] = kernel_avg_ms(s, KID_CFL) + kernel_avg_ms(s, KID_ENERGY) + kernel_avg_ms(s, KID_EOS_STEP);
     ms[1] = kernel_avg_ms(s, KID_TRANSPORT_AZI);
     ms[2] = kernel_avg_ms(s, KID_FINISH);
+    ms[3] = kernel_avg_ms(s, KID_SG_ROWS_FWD) + kernel_avg_ms(s, KID_SG_COLS) + kernel_avg_ms(s, KID_SG_ROWS_INV);
     return DFARGO_OK;
   };
   rc = dfargo_init_euler(s, rho.data(), vr.data(), vt.data(), c->p.energy_equation ? en.data() : nullptr, &pl);
-  const int xcd[] = {1, 0}, waves[] = {4, 2, 1}, blocks[] = {256, 128, 64};
-  for (int knob = 0; knob < 3 && !rc; knob++) {
+  const int xcd[] = {1, 0}, waves[] = {4, 2, 1}, blocks[] = {256, 128, 64}, bufs[] = {2, 1};
+  // the transforms' buffers only with the full self-gravity
+  const int nknobs = (c->p.self_gravity && !c->p.sg_zero_mode) ? 4 : 3;
+  for (int knob = 0; knob < nknobs && !rc; knob++) {
     double best_ms = 1e30;
-    const int nv = knob == 0 ? 2 : 3;
+    const int nv = (knob == 0 || knob == 3) ? 2 : 3;
     for (int v = 0; v < nv && !rc; v++) {
       cur = best;
       if (knob == 0) cur.xcd_remap = xcd[v];
       else if (knob == 1) cur.azi_waves = waves[v];
-      else cur.finish_block = blocks[v];
-      double ms[3];
+      else if (knob == 2) cur.finish_block = blocks[v];
+      else cur.sg_fft_buffers = bufs[v];
+      double ms[4];
       if ((rc = trial(cur, ms))) break;
       if (ms[knob] < best_ms) {
         best_ms = ms[knob];
         if (knob == 0) best.xcd_remap = cur.xcd_remap;
         else if (knob == 1) best.azi_waves = cur.azi_waves;
-        else best.finish_block = cur.finish_block;
+        else if (knob == 2) best.finish_block = cur.finish_block;
+        else best.sg_fft_buffers = cur.sg_fft_buffers;
       }
     }
   }

@@ -50,16 +50,18 @@ template <typename T> __device__ __forceinline__ void dft4(const cx<T> (&x)[4], 
 // reading R inputs at stride nb and writing R outputs at stride Ns.  The
 // quotient j / Ns comes from a 32-bit reciprocal (mg = 2^32/Ns rounded up,
 // exact for j < 2^12 and Ns <= 2^12), not an integer division.
+// fft_bfly computes butterfly j into y and returns the index of its first
+// output (the outputs go to idxD + r Ns); fft_butterfly also stores them.
 template <typename T, int R>
-__device__ __forceinline__ void fft_butterfly(const cx<T> *in, cx<T> *out, int j, int nb, int Ns, unsigned mg,
-                                              int span, const cx<T> *tw, int dir) {
+__device__ __forceinline__ int fft_bfly(const cx<T> *in, int j, int nb, int Ns, unsigned mg, int span,
+                                        const cx<T> *tw, int dir, cx<T> (&y)[R]) {
   const T s3 = (T)(dir * 0.86602540378443864676); // dir * sqrt(3)/2
   const T c8 = (T)0.70710678118654752440;         // sqrt(1/2)
   const T fd = (T)dir;
   {
     const int g = (Ns == 1) ? j : (int)__umulhi((unsigned)j, mg);
     const int k = j - g * Ns;
-    cx<T> v[R], y[R];
+    cx<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) v[r] = in[fp(j + r * nb)];
     if (Ns > 1) {
@@ -102,10 +104,16 @@ __device__ __forceinline__ void fft_butterfly(const cx<T> *in, cx<T> *out, int j
       y[1] = cadd(t2, t3);
       y[2] = csub(t2, t3);
     }
-    const int idxD = g * Ns * R + k;
-#pragma unroll
-    for (int r = 0; r < R; r++) out[fp(idxD + r * Ns)] = y[r];
+    return g * Ns * R + k;
   }
+}
+template <typename T, int R>
+__device__ __forceinline__ void fft_butterfly(const cx<T> *in, cx<T> *out, int j, int nb, int Ns, unsigned mg,
+                                              int span, const cx<T> *tw, int dir) {
+  cx<T> y[R];
+  const int idxD = fft_bfly<T, R>(in, j, nb, Ns, mg, span, tw, dir, y);
+#pragma unroll
+  for (int r = 0; r < R; r++) out[fp(idxD + r * Ns)] = y[r];
 }
 
 template <typename T, int R>
@@ -184,6 +192,65 @@ __device__ __forceinline__ void fft_lds2(cx<T> *A0, cx<T> *B0, cx<T> *A1, cx<T> 
   *r1 = in1;
 }
 
+// ---- the same transforms in one buffer per transform (in place) ----
+// Each pass reads all of a thread's butterflies into registers, waits for
+// the workgroup, then writes the outputs over the inputs and waits again:
+// half the LDS of the two-buffer form for one more barrier per pass, so that
+// more workgroups share a CU.  A thread holds at most FFT_IP_ELEMS elements
+// of a pass: the workgroup needs fft_ip_min_threads(ntr, L) threads or
+// more (ntr transforms of length L in lockstep).  The butterflies' arithmetic
+// is that of the two-buffer form, so the results are the same bit for bit.
+#ifndef FFT_IP_ELEMS
+#define FFT_IP_ELEMS 8
+#endif
+__host__ __device__ inline int fft_ip_min_threads(int ntr, int L) { return (ntr * L + FFT_IP_ELEMS - 1) / FFT_IP_ELEMS; }
+
+template <typename T, int R, int NT>
+__device__ __forceinline__ void fft_pass_ip(cx<T> *b0, cx<T> *b1, int L, int Ns, unsigned mg, int span,
+                                            const cx<T> *tw, int dir) {
+  constexpr int MB = (FFT_IP_ELEMS + R - 1) / R; // butterflies per thread
+  const int nb = L / R, tot = NT * nb, TB = blockDim.x;
+  cx<T> y[MB][R];
+  int dst[MB];
+#pragma unroll
+  for (int b = 0; b < MB; b++) {
+    const int jj = threadIdx.x + b * TB;
+    if (jj < tot) {
+      const bool second = NT == 2 && jj >= nb;
+      dst[b] = fft_bfly<T, R>(second ? b1 : b0, second ? jj - nb : jj, nb, Ns, mg, span, tw, dir, y[b]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MB; b++) {
+    const int jj = threadIdx.x + b * TB;
+    if (jj < tot) {
+      cx<T> *out = (NT == 2 && jj >= nb) ? b1 : b0;
+#pragma unroll
+      for (int r = 0; r < R; r++) out[fp(dst[b] + r * Ns)] = y[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// NT (1 or 2) transforms of length L = 2^a 3^b <= 4096 in place in b0 (and
+// b1), the passes in the order of fft_lds; the results stay in b0 and b1.
+template <typename T, int NT>
+__device__ __forceinline__ void fft_lds_ip(cx<T> *b0, cx<T> *b1, int L, const cx<T> *tw, int dir) {
+  int Ns = 1, rem = L;
+  while (rem > 1) {
+    const int R = (rem % 3 == 0) ? 3 : ((rem % 8 == 0) ? 8 : ((rem % 4 == 0) ? 4 : 2));
+    const int span = L / (Ns * R);
+    const unsigned mg = (Ns > 1) ? 0xFFFFFFFFu / (unsigned)Ns + 1u : 0u;
+    if (R == 8) fft_pass_ip<T, 8, NT>(b0, b1, L, Ns, mg, span, tw, dir);
+    else if (R == 4) fft_pass_ip<T, 4, NT>(b0, b1, L, Ns, mg, span, tw, dir);
+    else if (R == 3) fft_pass_ip<T, 3, NT>(b0, b1, L, Ns, mg, span, tw, dir);
+    else fft_pass_ip<T, 2, NT>(b0, b1, L, Ns, mg, span, tw, dir);
+    Ns *= R;
+    rem /= R;
+  }
+}
+
 // The transform kernels take up to SG_TB_MAX threads; sg_threads picks the
 // block size at launch.
 #define SG_TB_MAX 1024
@@ -203,16 +270,18 @@ template <typename T> __device__ __forceinline__ cx<T> sg_reduced(const KArgs &a
 
 // The forward azimuthal transform of ring i's reduced densities, which the
 // workgroup has written to A (padded layout), stored transposed into zt
-// ([Nsec][Nrad]); B is the second transform buffer.
-template <typename T> __device__ __forceinline__ void sg_row_transform(const KArgs &a, int i, cx<T> *A, cx<T> *B,
-                                                                      cx<T> *zt) {
+// ([Nsec][Nrad]); with NB = 2 buffers B is the second one (unused with 1).
+template <typename T, int NB = 2>
+__device__ __forceinline__ void sg_row_transform(const KArgs &a, int i, cx<T> *A, cx<T> *B, cx<T> *zt) {
   __syncthreads();
-  cx<T> *r = fft_lds<T>(A, B, a.ns, (const cx<T> *)a.tw_ns, -1);
+  cx<T> *r = A;
+  if constexpr (NB == 1) fft_lds_ip<T, 1>(A, A, a.ns, (const cx<T> *)a.tw_ns, -1);
+  else r = fft_lds<T>(A, B, a.ns, (const cx<T> *)a.tw_ns, -1);
   for (int j = threadIdx.x; j < a.ns; j += blockDim.x) zt[(long)j * a.nr + i] = r[fp(j)];
 }
 
-// LDS bytes of one row transform (two buffers) for element size csz
-static inline size_t sg_rows_lds_bytes(const KArgs &a, size_t csz) { return 2 * csz * fpsz(a.ns); }
+// LDS bytes of one row transform in nb buffers for element size csz
+static inline size_t sg_rows_lds_bytes(const KArgs &a, size_t csz, int nb = 2) { return nb * csz * fpsz(a.ns); }
 
 // Block size of a transform kernel of nwg workgroups using lds bytes each:
 // the LDS lets 160 KiB / lds workgroups share a CU, the grid gives each CU
@@ -220,7 +289,9 @@ static inline size_t sg_rows_lds_bytes(const KArgs &a, size_t csz) { return 2 * 
 // that the SIMDs stay busy through the barriers of the passes (256 ..
 // SG_TB_MAX threads, a power of two, and no more than the butterflies of
 // the widest pass, cap).
-static inline int sg_threads(const KArgs &a, size_t lds, int nwg, int cap) {
+// In-place transforms also need tmin threads or more (fft_ip_min_threads);
+// the result is 0 when SG_TB_MAX threads are not enough.
+static inline int sg_threads(const KArgs &a, size_t lds, int nwg, int cap, int tmin = 0) {
   const size_t per_cu = 160 * 1024;
   int groups = (int)(per_cu / (lds ? lds : 1));
   const int per_grid = (nwg + a.num_cu - 1) / a.num_cu;
@@ -228,7 +299,8 @@ static inline int sg_threads(const KArgs &a, size_t lds, int nwg, int cap) {
   const int want = 16 * 64 / (groups > 0 ? groups : 1);
   int t = 256;
   while (t < want && t < SG_TB_MAX && 2 * t <= cap) t *= 2;
-  return t;
+  while (t < tmin && t < SG_TB_MAX) t *= 2;
+  return (t < tmin) ? 0 : t;
 }
 
 #endif

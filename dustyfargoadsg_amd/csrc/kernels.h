@@ -50,6 +50,9 @@ struct KArgs {
   // wavefronts per workgroup of the azimuthal transport (1, 2, 4) and
   // threads per workgroup of the velocity/boundary kernel (64, 128, 256)
   int tune_azi_waves, tune_finish_block;
+  // LDS buffers per transform of the self-gravity kernels: 0 the default
+  // (one where that lets more workgroups share a CU), 1 or 2
+  int sg_fft_buffers;
   // with the full self-gravity: k_finish also forms the next sub-step's row
   // spectra (one workgroup per ring), and the self-gravity branch of a
   // sub-step starts with the radial transforms

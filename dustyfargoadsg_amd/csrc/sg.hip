@@ -23,13 +23,44 @@
 // of one XCD are contiguous (ring_of_block): the 16 (FP32) or 8 (FP64)
 // rings that share a line of zt are written in one L2, which merges the
 // lines before they go to HBM.
-template <typename T> __global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_fwd(KArgs a, int force, cx<T> *zt) {
+// NB: LDS buffers per transform (2: Stockham ping-pong; 1: in place, half
+// the LDS).
+template <typename T, int NB>
+__global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_fwd(KArgs a, int force, cx<T> *zt) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->go) return;
   const int i = ring_of_block(a), ns = a.ns;
-  cx<T> *A = (cx<T> *)smem, *B = A + fpsz(ns);
+  cx<T> *A = (cx<T> *)smem, *B = (NB == 2) ? A + fpsz(ns) : A;
   for (int j = threadIdx.x; j < ns; j += blockDim.x) A[fp(j)] = sg_reduced<T>(a, i, ldd<T>(a.rho, (long)i * ns + j));
-  sg_row_transform<T>(a, i, A, B, zt);
+  sg_row_transform<T, NB>(a, i, A, B, zt);
+}
+
+// The spectral product of wavenumber k1 of the pair (src/sgacc.c:25-42):
+// from the transformed columns ra (wavenumber k2) and rb (Ns-k2), the new
+// values of both at k1 (wb only when the pair is two columns).
+template <typename T>
+__device__ __forceinline__ void sg_spectral(const cx<T> *ra, const cx<T> *rb, int k1, int L, const cx<T> *khr,
+                                            const cx<T> *kht, bool self, cx<T> &wa, cx<T> &wb) {
+  const T G = (T)DF_G, h = (T)0.5;
+  const int m1 = (k1 == 0) ? 0 : L - k1;
+  {
+    const cx<T> zz = ra[fp(k1)], zp = rb[fp(m1)];
+    const cx<T> sr = {h * (zz.x + zp.x), h * (zz.y - zp.y)};
+    const cx<T> st = {h * (zz.y + zp.y), h * (zp.x - zz.x)};
+    const cx<T> K1 = khr[k1], K2 = kht[k1];
+    const cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
+    const cx<T> at = {-G * (K2.x * st.x - K2.y * st.y), -G * (K2.x * st.y + K2.y * st.x)};
+    wa = {ar.x - at.y, ar.y + at.x};
+  }
+  if (!self) {
+    const cx<T> zz = rb[fp(k1)], zp = ra[fp(m1)];
+    const cx<T> sr = {h * (zz.x + zp.x), h * (zz.y - zp.y)};
+    const cx<T> st = {h * (zz.y + zp.y), h * (zp.x - zz.x)};
+    const cx<T> K1 = {khr[m1].x, -khr[m1].y}, K2 = {kht[m1].x, -kht[m1].y};
+    const cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
+    const cx<T> at = {-G * (K2.x * st.x - K2.y * st.y), -G * (K2.x * st.y + K2.y * st.x)};
+    wb = {ar.x - at.y, ar.y + at.x};
+  }
 }
 
 // Radial transforms and spectral product (src/sgacc.c:25-42) for one pair
@@ -44,14 +75,15 @@ template <typename T> __global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_fwd
 //      only wavenumbers 0..Ns/2 are stored, transposed) and recombines
 //      A_r + i A_t,
 //   4. inverse-transforms both columns and keeps rows < Nrad.
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(SG_TB_MAX) k_sg_cols(KArgs a, int force, const cx<T> *zt, cx<T> *wt) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->go) return;
   const int ns = a.ns, nr = a.nr, L = 2 * nr, p = blockIdx.x, S = fpsz(L);
   const int ka = p, kb = (ns - p) % ns;
   const bool self = (ka == kb);
-  cx<T> *A1 = (cx<T> *)smem, *B1 = A1 + S, *A2 = B1 + S, *B2 = A2 + S;
+  // two buffers per column (A1/B1, A2/B2), or one (A1, A2)
+  cx<T> *A1 = (cx<T> *)smem, *B1 = A1 + S, *A2 = (NB == 2) ? B1 + S : A1 + S, *B2 = A2 + S;
   const cx<T> *za = zt + (long)ka * nr, *zb = zt + (long)kb * nr;
   const cx<T> zero = {(T)0, (T)0};
   for (int i = threadIdx.x; i < L; i += blockDim.x) {
@@ -60,43 +92,52 @@ __global__ void __launch_bounds__(SG_TB_MAX) k_sg_cols(KArgs a, int force, const
   }
   __syncthreads();
   const cx<T> *tw = (const cx<T> *)a.tw_2n;
-  cx<T> *ra, *rb;
-  if (self) {
-    ra = rb = fft_lds<T>(A1, B1, L, tw, -1);
-  } else {
-    fft_lds2<T>(A1, B1, A2, B2, L, tw, -1, &ra, &rb);
-  }
-  cx<T> *wa = (ra == A1) ? B1 : A1;
-  cx<T> *wb = self ? wa : ((rb == A2) ? B2 : A2);
   const cx<T> *khr = (const cx<T> *)a.sg_kr + (long)p * L, *kht = (const cx<T> *)a.sg_kt + (long)p * L;
-  const T G = (T)DF_G, h = (T)0.5;
-  for (int k1 = threadIdx.x; k1 < L; k1 += blockDim.x) {
-    const int m1 = (k1 == 0) ? 0 : L - k1;
-    {
-      const cx<T> zz = ra[fp(k1)], zp = rb[fp(m1)];
-      const cx<T> sr = {h * (zz.x + zp.x), h * (zz.y - zp.y)};
-      const cx<T> st = {h * (zz.y + zp.y), h * (zp.x - zz.x)};
-      const cx<T> K1 = khr[k1], K2 = kht[k1];
-      const cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
-      const cx<T> at = {-G * (K2.x * st.x - K2.y * st.y), -G * (K2.x * st.y + K2.y * st.x)};
-      wa[fp(k1)] = {ar.x - at.y, ar.y + at.x};
-    }
-    if (!self) {
-      const cx<T> zz = rb[fp(k1)], zp = ra[fp(m1)];
-      const cx<T> sr = {h * (zz.x + zp.x), h * (zz.y - zp.y)};
-      const cx<T> st = {h * (zz.y + zp.y), h * (zp.x - zz.x)};
-      const cx<T> K1 = {khr[m1].x, -khr[m1].y}, K2 = {kht[m1].x, -kht[m1].y};
-      const cx<T> ar = {-G * (K1.x * sr.x - K1.y * sr.y), -G * (K1.x * sr.y + K1.y * sr.x)};
-      const cx<T> at = {-G * (K2.x * st.x - K2.y * st.y), -G * (K2.x * st.y + K2.y * st.x)};
-      wb[fp(k1)] = {ar.x - at.y, ar.y + at.x};
-    }
-  }
-  __syncthreads();
   cx<T> *oa, *ob;
-  if (self) {
-    oa = ob = fft_lds<T>(wa, (wa == A1) ? B1 : A1, L, tw, +1);
+  if constexpr (NB == 1) {
+    if (self) fft_lds_ip<T, 1>(A1, A1, L, tw, -1);
+    else fft_lds_ip<T, 2>(A1, A2, L, tw, -1);
+    cx<T> *rb = self ? A1 : A2;
+    // wavenumbers k1 and L-k1 of both columns read and write the same four
+    // slots, so one thread takes both: all four read before any is stored
+    for (int k1 = threadIdx.x; k1 <= L / 2; k1 += blockDim.x) {
+      const int m1 = (k1 == 0) ? 0 : L - k1;
+      cx<T> va, vb, wa, wb;
+      sg_spectral<T>(A1, rb, k1, L, khr, kht, self, va, vb);
+      if (m1 != k1) sg_spectral<T>(A1, rb, m1, L, khr, kht, self, wa, wb);
+      A1[fp(k1)] = va;
+      if (!self) rb[fp(k1)] = vb;
+      if (m1 != k1) {
+        A1[fp(m1)] = wa;
+        if (!self) rb[fp(m1)] = wb;
+      }
+    }
+    __syncthreads();
+    if (self) fft_lds_ip<T, 1>(A1, A1, L, tw, +1);
+    else fft_lds_ip<T, 2>(A1, A2, L, tw, +1);
+    oa = A1;
+    ob = rb;
   } else {
-    fft_lds2<T>(wa, (wa == A1) ? B1 : A1, wb, (wb == A2) ? B2 : A2, L, tw, +1, &oa, &ob);
+    cx<T> *ra, *rb;
+    if (self) {
+      ra = rb = fft_lds<T>(A1, B1, L, tw, -1);
+    } else {
+      fft_lds2<T>(A1, B1, A2, B2, L, tw, -1, &ra, &rb);
+    }
+    cx<T> *wa = (ra == A1) ? B1 : A1;
+    cx<T> *wb = self ? wa : ((rb == A2) ? B2 : A2);
+    for (int k1 = threadIdx.x; k1 < L; k1 += blockDim.x) {
+      cx<T> va, vb;
+      sg_spectral<T>(ra, rb, k1, L, khr, kht, self, va, vb);
+      wa[fp(k1)] = va;
+      if (!self) wb[fp(k1)] = vb;
+    }
+    __syncthreads();
+    if (self) {
+      oa = ob = fft_lds<T>(wa, (wa == A1) ? B1 : A1, L, tw, +1);
+    } else {
+      fft_lds2<T>(wa, (wa == A1) ? B1 : A1, wb, (wb == A2) ? B2 : A2, L, tw, +1, &oa, &ob);
+    }
   }
   cx<T> *wpa = wt + (long)ka * nr, *wpb = wt + (long)kb * nr;
   for (int i = threadIdx.x; i < nr; i += blockDim.x) {
@@ -108,14 +149,17 @@ __global__ void __launch_bounds__(SG_TB_MAX) k_sg_cols(KArgs a, int force, const
 // inverse azimuthal transform, normalisation and self-force compensation
 // (src/sgacc.c:137-161); W stored transposed ([Nsec][Nrad], wt), read
 // with the rings of one XCD contiguous as in k_sg_rows_fwd
-template <typename T> __global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_inv(KArgs a, int force, const cx<T> *wt) {
+template <typename T, int NB>
+__global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_inv(KArgs a, int force, const cx<T> *wt) {
   extern __shared__ unsigned char smem[];
   if (!force && !a.ctrl->go) return;
   const int i = ring_of_block(a), ns = a.ns, nr = a.nr;
   cx<T> *A = (cx<T> *)smem, *B = A + fpsz(ns);
   for (int j = threadIdx.x; j < ns; j += blockDim.x) A[fp(j)] = wt[(long)j * nr + i];
   __syncthreads();
-  cx<T> *r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, +1);
+  cx<T> *r = A;
+  if constexpr (NB == 1) fft_lds_ip<T, 1>(A, A, ns, (const cx<T> *)a.tw_ns, +1);
+  else r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, +1);
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
     const long l = (long)i * ns + j;
     double accr = (double)r[fp(j)].x * a.sg_normr[i];
@@ -158,8 +202,28 @@ template <typename K> static void allow_lds(K kernel, size_t bytes) {
   if (bytes > 65536) hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-static size_t rows_lds_bytes(const KArgs &a, size_t csz) { return sg_rows_lds_bytes(a, csz); }
-static size_t cols_lds_bytes(const KArgs &a, size_t csz) { return 4 * csz * fpsz(2 * a.nr); }
+static size_t rows_lds_bytes(const KArgs &a, size_t csz, int nb) { return sg_rows_lds_bytes(a, csz, nb); }
+static size_t cols_lds_bytes(const KArgs &a, size_t csz, int nb) { return 2 * nb * csz * fpsz(2 * a.nr); }
+
+// Workgroups of a transform kernel that share a CU: as many as its LDS
+// allows, and no more than the grid gives each CU.
+static int sg_groups(const KArgs &a, size_t lds, int nwg) {
+  const int by_lds = (int)((160 * 1024) / lds), per_grid = (nwg + a.num_cu - 1) / a.num_cu;
+  return by_lds < per_grid ? by_lds : per_grid;
+}
+// Buffers per transform of the row (cols = 0) or column-pair (cols = 1)
+// kernels: the tuning's choice (a.sg_fft_buffers 1 or 2), or by default
+// one buffer where that lets more workgroups share a CU and the thread
+// count it needs fits a workgroup.
+static int sg_nb(const KArgs &a, size_t csz, int cols) {
+  const int L = cols ? 2 * a.nr : a.ns, ntr = cols ? 2 : 1, nwg = cols ? a.ns / 2 + 1 : a.nr;
+  const int cap = cols ? 2 * a.nr : a.ns / 2;
+  const size_t l2 = cols ? cols_lds_bytes(a, csz, 2) : rows_lds_bytes(a, csz, 2);
+  const bool fits = sg_threads(a, l2 / 2, nwg, cap, fft_ip_min_threads(ntr, L)) > 0;
+  if (a.sg_fft_buffers == 1 && fits) return 1;
+  if (a.sg_fft_buffers != 0) return 2;
+  return (fits && sg_groups(a, l2 / 2, nwg) > sg_groups(a, l2, nwg)) ? 1 : 2;
+}
 
 
 // ------------------------------------------------------------------------
@@ -247,29 +311,42 @@ template <typename T> static void sgz_pipeline(int id, const KArgs &a, hipStream
 
 // rows_fwd: Z^T (sg_z) -> pair transforms -> W^T (sg_w) -> rows_inv
 template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
-  const size_t rows_lds = rows_lds_bytes(a, sizeof(cx<T>)), cols_lds = cols_lds_bytes(a, sizeof(cx<T>));
+  const size_t csz = sizeof(cx<T>);
+  const int rnb = sg_nb(a, csz, 0), cnb = sg_nb(a, csz, 1);
+  const size_t rows_lds = rows_lds_bytes(a, csz, rnb), cols_lds = cols_lds_bytes(a, csz, cnb);
+  const int rows_tb = sg_threads(a, rows_lds, a.nr, a.ns / 2, rnb == 1 ? fft_ip_min_threads(1, a.ns) : 0);
+  const int cols_tb =
+      sg_threads(a, cols_lds, a.ns / 2 + 1, 2 * a.nr, cnb == 1 ? fft_ip_min_threads(2, 2 * a.nr) : 0);
   cx<T> *z = (cx<T> *)a.sg_z, *w = (cx<T> *)a.sg_w;
   switch (id) {
-    case KID_SG_ROWS_FWD:
-      hipLaunchKernelGGL(k_sg_rows_fwd<T>, dim3(a.nr), dim3(sg_threads(a, rows_lds, a.nr, a.ns / 2)), rows_lds, s, a,
-                         force, z);
+    case KID_SG_ROWS_FWD: {
+      auto kf = (rnb == 1) ? k_sg_rows_fwd<T, 1> : k_sg_rows_fwd<T, 2>;
+      hipLaunchKernelGGL(kf, dim3(a.nr), dim3(rows_tb), rows_lds, s, a, force, z);
       break;
-    case KID_SG_COLS:
-      hipLaunchKernelGGL(k_sg_cols<T>, dim3(a.ns / 2 + 1), dim3(sg_threads(a, cols_lds, a.ns / 2 + 1, 2 * a.nr)), cols_lds, s, a, force, (const cx<T> *)z, w);
+    }
+    case KID_SG_COLS: {
+      auto kf = (cnb == 1) ? k_sg_cols<T, 1> : k_sg_cols<T, 2>;
+      hipLaunchKernelGGL(kf, dim3(a.ns / 2 + 1), dim3(cols_tb), cols_lds, s, a, force, (const cx<T> *)z, w);
       break;
-    case KID_SG_ROWS_INV:
-      hipLaunchKernelGGL(k_sg_rows_inv<T>, dim3(a.nr), dim3(sg_threads(a, rows_lds, a.nr, a.ns / 2)), rows_lds, s, a, force, (const cx<T> *)w);
+    }
+    case KID_SG_ROWS_INV: {
+      auto kf = (rnb == 1) ? k_sg_rows_inv<T, 1> : k_sg_rows_inv<T, 2>;
+      hipLaunchKernelGGL(kf, dim3(a.nr), dim3(rows_tb), rows_lds, s, a, force, (const cx<T> *)w);
       break;
+    }
   }
 }
 
 // Raise the dynamic LDS limit of the kernels that need more than 64 KiB,
 // once at set-up (not inside a stream capture).
 template <typename T> void sg_prepare(const KArgs &a) {
-  const size_t rows_lds = rows_lds_bytes(a, sizeof(cx<T>)), cols_lds = cols_lds_bytes(a, sizeof(cx<T>));
-  allow_lds(k_sg_rows_fwd<T>, rows_lds);
-  allow_lds(k_sg_cols<T>, cols_lds);
-  allow_lds(k_sg_rows_inv<T>, rows_lds);
+  const size_t csz = sizeof(cx<T>);
+  allow_lds(k_sg_rows_fwd<T, 2>, rows_lds_bytes(a, csz, 2));
+  allow_lds(k_sg_cols<T, 2>, cols_lds_bytes(a, csz, 2));
+  allow_lds(k_sg_rows_inv<T, 2>, rows_lds_bytes(a, csz, 2));
+  allow_lds(k_sg_rows_fwd<T, 1>, rows_lds_bytes(a, csz, 1));
+  allow_lds(k_sg_cols<T, 1>, cols_lds_bytes(a, csz, 1));
+  allow_lds(k_sg_rows_inv<T, 1>, rows_lds_bytes(a, csz, 1));
 }
 
 template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s) {

@@ -252,7 +252,8 @@ class Disc:
     def tuning(self):
         t = _capi.Tuning()
         _check(self.lib, self.lib.dfargo_get_tuning(self.ctx, ctypes.byref(t)))
-        return dict(xcd_remap=t.xcd_remap, azi_waves=t.azi_waves, finish_block=t.finish_block)
+        return dict(xcd_remap=t.xcd_remap, azi_waves=t.azi_waves, finish_block=t.finish_block,
+                    sg_fft_buffers=t.sg_fft_buffers)
 
     def set_tuning(self, **kw):
         t = _capi.Tuning(**dict(self.tuning(), **kw))
@@ -263,7 +264,8 @@ class Disc:
         and applies the fastest (kept for later discs of this shape)."""
         t = _capi.Tuning()
         _check(self.lib, self.lib.dfargo_autotune(self.ctx, substeps, ctypes.byref(t)))
-        return dict(xcd_remap=t.xcd_remap, azi_waves=t.azi_waves, finish_block=t.finish_block)
+        return dict(xcd_remap=t.xcd_remap, azi_waves=t.azi_waves, finish_block=t.finish_block,
+                    sg_fft_buffers=t.sg_fft_buffers)
 
     # -- measurement --------------------------------------------------------
     def set_profiling(self, on):

@@ -433,13 +433,16 @@ int dfargo_read_planet_config(const char *path, const dfargo_params *p, dfargo_p
 
 /* Launch shapes of the sub-step kernels that do not change results: the
  * ring kernels' XCD-contiguous block order, the azimuthal transport's
- * wavefronts per workgroup and the velocity/boundary kernel's workgroup
- * width.  No counterpart in the reference (its loops have no launch
- * shape). */
+ * wavefronts per workgroup, the velocity/boundary kernel's workgroup
+ * width and the LDS buffers of the self-gravity transforms.  No
+ * counterpart in the reference (its loops have no launch shape). */
 typedef struct dfargo_tuning {
   int xcd_remap;     /* 1: the rings of one XCD are contiguous; 0: round-robin */
   int azi_waves;     /* 1, 2 or 4 wavefronts per workgroup                    */
   int finish_block;  /* 64, 128 or 256 threads per workgroup                  */
+  int sg_fft_buffers; /* LDS buffers per transform: 2 (ping-pong), 1 (in
+                         place, half the LDS, one more barrier per pass), or
+                         0: one where that lets more workgroups share a CU   */
 } dfargo_tuning;
 int dfargo_get_tuning(dfargo_ctx *ctx, dfargo_tuning *t);
 int dfargo_set_tuning(dfargo_ctx *ctx, const dfargo_tuning *t);
