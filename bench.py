@@ -353,7 +353,7 @@ def main():
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--error-orbits", type=float, default=1.0,
                     help="orbits over which the FP32 error against FP64 is measured (0: skip)")
-    ap.add_argument("--target-steps", type=int, default=20,
+    ap.add_argument("--target-steps", type=int, default=100,
                     help="sub-steps of the 1024x3072 FP32 gas+SG disc timed beside the default "
                          "workload (0: skip)")
     ap.add_argument("--batched", type=int, default=64,
@@ -489,8 +489,8 @@ def main():
         print("bench: gas_sg_large 1024x3072", file=sys.stderr, flush=True)
         tp, tpl = gas_sg_large()
         tcells = tp.nrad * tp.nsec
-        el32, _ = run_gpu(tp, tpl, _capi.FP32, args.target_steps, 5, 1)
-        el64, _ = run_gpu(tp, tpl, _capi.FP64, args.target_steps, 5, 1)
+        el32, _ = run_gpu(tp, tpl, _capi.FP32, args.target_steps, 20, 1)
+        el64, _ = run_gpu(tp, tpl, _capi.FP64, args.target_steps, 20, 1)
         tb = step_bytes_per_cell(4, False, True)
         ach_t = tb * tcells * args.target_steps / el32 / 1e9
         line["target_1024x3072"] = {
