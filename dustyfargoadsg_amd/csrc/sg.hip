@@ -13,6 +13,8 @@
 // host fills in FP64.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "fft_lds.h"
 #include "kernels.h"
 
@@ -205,24 +207,42 @@ template <typename K> static void allow_lds(K kernel, size_t bytes) {
 static size_t rows_lds_bytes(const KArgs &a, size_t csz, int nb) { return sg_rows_lds_bytes(a, csz, nb); }
 static size_t cols_lds_bytes(const KArgs &a, size_t csz, int nb) { return 2 * nb * csz * fpsz(2 * a.nr); }
 
-// Workgroups of a transform kernel that share a CU: as many as its LDS
-// allows, and no more than the grid gives each CU.
-static int sg_groups(const KArgs &a, size_t lds, int nwg) {
-  const int by_lds = (int)((160 * 1024) / lds), per_grid = (nwg + a.num_cu - 1) / a.num_cu;
-  return by_lds < per_grid ? by_lds : per_grid;
+// VGPRs per lane of the transform kernels, [FP64][column pairs][NB - 1]
+// (the larger of the two row kernels), read at set-up; 0 before
+static std::atomic<int> s_vgprs[2][2][2];
+
+// Workgroups of tb threads of a transform kernel that share a CU: as many
+// as its LDS (160 KiB) and its registers (512 VGPRs per SIMD lane, at most
+// 8 wavefronts per SIMD; a workgroup spreads its wavefronts over the 4
+// SIMDs) allow, and no more than the grid gives each CU.
+static int sg_groups(const KArgs &a, size_t lds, int nwg, int tb, int vgprs) {
+  int g = (int)((160 * 1024) / lds);
+  const int per_grid = (nwg + a.num_cu - 1) / a.num_cu;
+  if (per_grid < g) g = per_grid;
+  if (vgprs > 0 && tb > 0) {
+    int per_simd = 512 / ((vgprs + 7) / 8 * 8);
+    if (per_simd > 8) per_simd = 8;
+    const int need = ((tb + 63) / 64 + 3) / 4;
+    if (per_simd / need < g) g = per_simd / need;
+  }
+  return g;
 }
 // Buffers per transform of the row (cols = 0) or column-pair (cols = 1)
 // kernels: the tuning's choice (a.sg_fft_buffers 1 or 2), or by default
-// one buffer where that lets more workgroups share a CU and the thread
-// count it needs fits a workgroup.
+// one buffer where that lets more workgroups share a CU while they hold
+// no more than 16 wavefronts of it.  The transforms run beside the CFL and
+// control kernels: filling every wavefront slot of the CUs with them
+// (1024x3072, FP32) delays those kernels by more than the transforms gain.
 static int sg_nb(const KArgs &a, size_t csz, int cols) {
   const int L = cols ? 2 * a.nr : a.ns, ntr = cols ? 2 : 1, nwg = cols ? a.ns / 2 + 1 : a.nr;
-  const int cap = cols ? 2 * a.nr : a.ns / 2;
+  const int cap = cols ? 2 * a.nr : a.ns / 2, fp64 = csz == sizeof(cx<double>);
   const size_t l2 = cols ? cols_lds_bytes(a, csz, 2) : rows_lds_bytes(a, csz, 2);
-  const bool fits = sg_threads(a, l2 / 2, nwg, cap, fft_ip_min_threads(ntr, L)) > 0;
-  if (a.sg_fft_buffers == 1 && fits) return 1;
+  const int t1 = sg_threads(a, l2 / 2, nwg, cap, fft_ip_min_threads(ntr, L)), t2 = sg_threads(a, l2, nwg, cap);
+  if (a.sg_fft_buffers == 1 && t1 > 0) return 1;
   if (a.sg_fft_buffers != 0) return 2;
-  return (fits && sg_groups(a, l2 / 2, nwg) > sg_groups(a, l2, nwg)) ? 1 : 2;
+  const int g1 = sg_groups(a, l2 / 2, nwg, t1, s_vgprs[fp64][cols][0].load());
+  const int g2 = sg_groups(a, l2, nwg, t2, s_vgprs[fp64][cols][1].load());
+  return (t1 > 0 && g1 > g2 && g1 * t1 <= 16 * 64) ? 1 : 2;
 }
 
 
@@ -339,8 +359,20 @@ template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_
 
 // Raise the dynamic LDS limit of the kernels that need more than 64 KiB,
 // once at set-up (not inside a stream capture).
+template <typename T> static int kernel_vgprs(const void *k) {
+  hipFuncAttributes fa;
+  return (hipFuncGetAttributes(&fa, k) == hipSuccess) ? fa.numRegs : 0;
+}
+
 template <typename T> void sg_prepare(const KArgs &a) {
   const size_t csz = sizeof(cx<T>);
+  const int fp64 = sizeof(T) == sizeof(double);
+  for (int nb = 1; nb <= 2; nb++) {
+    const int rf = kernel_vgprs<T>(nb == 1 ? (const void *)k_sg_rows_fwd<T, 1> : (const void *)k_sg_rows_fwd<T, 2>);
+    const int ri = kernel_vgprs<T>(nb == 1 ? (const void *)k_sg_rows_inv<T, 1> : (const void *)k_sg_rows_inv<T, 2>);
+    s_vgprs[fp64][0][nb - 1] = rf > ri ? rf : ri;
+    s_vgprs[fp64][1][nb - 1] = kernel_vgprs<T>(nb == 1 ? (const void *)k_sg_cols<T, 1> : (const void *)k_sg_cols<T, 2>);
+  }
   allow_lds(k_sg_rows_fwd<T, 2>, rows_lds_bytes(a, csz, 2));
   allow_lds(k_sg_cols<T, 2>, cols_lds_bytes(a, csz, 2));
   allow_lds(k_sg_rows_inv<T, 2>, rows_lds_bytes(a, csz, 2));
