@@ -107,7 +107,8 @@ def test_sg_acceleration_matches_oracle():
     assert rel(at, at_o, 64, 192) <= 1e-11
 
 
-@pytest.mark.parametrize("variant", ["open", "standard", "feel_disk", "taper", "no_indirect"])
+@pytest.mark.parametrize("variant", ["open", "standard", "feel_disk", "taper", "no_indirect", "discard_gas_indirect",
+                                     "alternative_gas_indirect"])
 def test_parameter_variants_fp64(variant):
     """Boundary, transport and planet options against the oracle."""
     if variant == "feel_disk":
@@ -118,7 +119,8 @@ def test_parameter_variants_fp64(variant):
         pl = dict(pl, feel_disk=[1], feel_others=[0])
     else:
         over = {"open": dict(boundary=_capi.BC_OPEN), "standard": dict(fast_transport=0),
-                "no_indirect": dict(indirect_term=0)}[variant]
+                "no_indirect": dict(indirect_term=0), "discard_gas_indirect": dict(discard_gas_indirect_term=1),
+                "alternative_gas_indirect": dict(alternative_gas_indirect_term=1)}[variant]
         p, pl = gas_planet(nrad=32, nsec=96, **over)
     o, so, d, sg = pair(p, pl)
     for _ in range(2):

@@ -345,6 +345,72 @@ def test_gas_indirect_term_is_the_star_acceleration(oracle_built):
     assert np.abs(pot - phi).max() <= 1e-13 * np.abs(phi).max()
 
 
+def _noisy_planet_disc(seed, **over):
+    p, pl = gas_planet(nrad=24, nsec=64, indirect_term_smoothing=0.05, **over)
+    o = Oracle(p)
+    st = o.initial_state(pl)
+    rng = np.random.default_rng(seed)
+    st["rho"][: o.nr * o.ns] *= 1 + 0.3 * rng.standard_normal(o.nr * o.ns)
+    o.set_state(st)
+    return o, st
+
+
+def _star_and_planet_potential(o, st, x, y):
+    m, eps = st["planets"].mass[0], 0.6 * 0.05
+    return -1.0 / np.sqrt(x * x + y * y) - m / np.sqrt((x - 1.0) ** 2 + y ** 2 + eps * eps) + m * x
+
+
+def test_discarded_gas_indirect_term(oracle_built):
+    """DISCARDGASINDIRECTTERM (src/Pframeforce.c:115-124): the potential
+    keeps the planet's indirect term and drops the disc's uniform field."""
+    o, st = _noisy_planet_disc(12, discard_gas_indirect_term=1)
+    assert o.algogas(1e-4) == 1
+    radii, rmed, surf, az = o.tables()
+    x = rmed[:, None] * np.cos(az)[None, :]
+    y = rmed[:, None] * np.sin(az)[None, :]
+    s = o.status()
+    assert abs(s.disk_on_primary_x) > 0.0
+    phi = _star_and_planet_potential(o, st, x, y)
+    pot = _ring(o, o.field("potential"))
+    assert np.abs(pot - phi).max() <= 1e-13 * np.abs(phi).max()
+
+
+def test_alternative_gas_indirect_term_is_the_barycentre_acceleration(oracle_built):
+    """ALTERNATIVEGASINDIRECTTERM (src/SideEuler.c:72-112,
+    src/SourceEuler.c:485, src/Pframeforce.c:29-35): the disc's field on the
+    star is minus the second difference of the star+disc barycentre
+    b = sum Surf Sigma (x, y) / (1 + sum Surf Sigma) over the last three
+    sub-steps, divided by dt^2, with a history that starts at zero (global
+    arrays of the reference).  Two sub-steps of dt = 1e-4 from a fresh
+    context: -b0 / dt^2, then -(b1 - 2 b0) / dt^2, each carried by that
+    sub-step's potential as the uniform field a . (x, y).  A sign, the star's
+    unit mass in the normalisation, the dt^2 or the history's order fails."""
+    o, st = _noisy_planet_disc(13, alternative_gas_indirect_term=1)
+    radii, rmed, surf, az = o.tables()
+    x = rmed[:, None] * np.cos(az)[None, :]
+    y = rmed[:, None] * np.sin(az)[None, :]
+
+    def bary(rho):
+        cm = surf[:, None] * _ring(o, rho)
+        return np.array([(cm * x).sum(), (cm * y).sum()]) / (1.0 + cm.sum())
+
+    dt = 1e-4
+    b = [bary(st["rho"])]
+    assert o.algogas(dt) == 1
+    st1 = o.state()
+    b.append(bary(st1["rho"]))
+    s = o.status()
+    a0 = -b[0] / dt ** 2
+    assert np.abs([s.disk_on_primary_alt_x - a0[0], s.disk_on_primary_alt_y - a0[1]]).max() <= 1e-12 * np.abs(a0).max()
+    phi = _star_and_planet_potential(o, st, x, y) + a0[0] * x + a0[1] * y
+    pot = _ring(o, o.field("potential"))
+    assert np.abs(pot - phi).max() <= 1e-12 * np.abs(phi).max()
+    assert o.algogas(dt) == 1
+    s = o.status()
+    a1 = -(b[1] - 2.0 * b[0]) / dt ** 2
+    assert np.abs([s.disk_on_primary_alt_x - a1[0], s.disk_on_primary_alt_y - a1[1]]).max() <= 1e-10 * np.abs(a1).max()
+
+
 # ---------------------------------------------------------------------------
 # boundaries (src/SideEuler.c:300-408, 1008-1131)
 
