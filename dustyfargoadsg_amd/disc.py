@@ -272,7 +272,7 @@ class Disc:
         _check(self.lib, self.lib.dfargo_set_profiling(self.ctx, 1 if on else 0))
 
     def kernel_times(self):
-        maxn = 32
+        maxn = 64
         names = (ctypes.c_char * 48 * maxn)()
         tot = np.zeros(maxn)
         cnt = (ctypes.c_long * maxn)()
