@@ -236,8 +236,9 @@ def test_full_size_finite(full):
     of the momentum equations with viscosity and diffusion (1 cm grains
     limit their step to ~4e-5): every field stays finite, the densities and
     Stokes numbers positive.  (Neither fluid's mass is conserved to round-off
-    in this configuration, on either side: the wall's edge rings exchange
-    mass with the interior, 2e-5 of the disc per call at 64x192.)"""
+    in this configuration, on either side: the reference's wall does not
+    hold v_r of ring 0 at zero, so mass crosses the inner edge, 2e-5 of the
+    disc per call at 64x192.)"""
     p, pl = gas_planet(nrad=1024, nsec=3072)
     fp = _capi.dust_fluid_default_params(sfta=0 if full else 1, sizepart=1e-2,
                                          dviscosity=1e-6 if full else 0.0, dust_diffusion=full)
