@@ -1,5 +1,5 @@
 """FP32 storage against FP64 storage over 10 orbits of the 128x384 planet
-disc (the north star's precision comparison, bench.precision_error): the
+disc (the FP32 against FP64 comparison of bench.precision_error): the
 stochastically rounded FP32 state stays within the stated bars of the FP64
 run.  Measured on MI355X: ring-mean Sigma 5e-6, Sigma L2 2.4e-5, Sigma
 L-infinity 2.7e-5, v_theta 3.7e-6, v_r 4.7e-6 (on the v_theta scale), the
