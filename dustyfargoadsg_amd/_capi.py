@@ -108,7 +108,8 @@ class Status(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     """dfargo_tuning: launch shapes that do not change results."""
-    _fields_ = [("xcd_remap", c_int), ("azi_waves", c_int), ("finish_block", c_int), ("sg_fft_buffers", c_int)]
+    _fields_ = [("xcd_remap", c_int), ("azi_waves", c_int), ("finish_block", c_int), ("sg_fft_buffers", c_int),
+                ("sg_rows_pair", c_int)]
 
 
 class DustParams(ctypes.Structure):

@@ -445,6 +445,7 @@ static int build_context(dfargo_ctx *c) {
   a.tune_azi_waves = 4;
   a.tune_finish_block = 256;
   a.sg_fft_buffers = 0;
+  a.sg_rows_pair = 0;
   {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, c->dev));
@@ -1751,6 +1752,7 @@ extern "C" int dfargo_get_tuning(dfargo_ctx *c, dfargo_tuning *t) {
   t->azi_waves = c->a.tune_azi_waves;
   t->finish_block = c->a.tune_finish_block;
   t->sg_fft_buffers = c->a.sg_fft_buffers;
+  t->sg_rows_pair = c->a.sg_rows_pair;
   return DFARGO_OK;
 }
 
@@ -1758,11 +1760,12 @@ extern "C" int dfargo_set_tuning(dfargo_ctx *c, const dfargo_tuning *t) {
   if (!c || !t) return dfargo_set_error(DFARGO_ERR_ARG, "null argument");
   if ((t->xcd_remap != 0 && t->xcd_remap != 1) || (t->azi_waves != 1 && t->azi_waves != 2 && t->azi_waves != 4) ||
       (t->finish_block != 64 && t->finish_block != 128 && t->finish_block != 256) || t->sg_fft_buffers < 0 ||
-      t->sg_fft_buffers > 2)
+      t->sg_fft_buffers > 2 || t->sg_rows_pair < 0 || t->sg_rows_pair > 2)
     return dfargo_set_error(DFARGO_ERR_ARG, "tuning out of range (xcd_remap 0/1, azi_waves 1/2/4, "
-                                            "finish_block 64/128/256, sg_fft_buffers 0/1/2)");
+                                            "finish_block 64/128/256, sg_fft_buffers 0/1/2, sg_rows_pair 0/1/2)");
   if (t->xcd_remap == c->a.xcd_remap && t->azi_waves == c->a.tune_azi_waves &&
-      t->finish_block == c->a.tune_finish_block && t->sg_fft_buffers == c->a.sg_fft_buffers)
+      t->finish_block == c->a.tune_finish_block && t->sg_fft_buffers == c->a.sg_fft_buffers &&
+      t->sg_rows_pair == c->a.sg_rows_pair)
     return DFARGO_OK;
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1770,6 +1773,7 @@ extern "C" int dfargo_set_tuning(dfargo_ctx *c, const dfargo_tuning *t) {
   c->a.tune_azi_waves = t->azi_waves;
   c->a.tune_finish_block = t->finish_block;
   c->a.sg_fft_buffers = t->sg_fft_buffers;
+  c->a.sg_rows_pair = t->sg_rows_pair;
   return drop_graph(c);
 }
 
@@ -1828,36 +1832,38 @@ extern "C" int dfargo_autotune(dfargo_ctx *c, int substeps, dfargo_tuning *best_
     ms[2] = kernel_avg_ms(s, KID_FINISH);
     // the transforms run beside the CFL branch: their knob is judged by the
     // captured sub-step (a warm-up call instantiates the graph)
-    ms[3] = 0.0;
-    if (t.sg_fft_buffers) {
+    ms[3] = ms[4] = 0.0;
+    if (t.sg_fft_buffers || t.sg_rows_pair) {
       if ((r = dfargo_set_profiling(s, 0)) || (r = dfargo_run_substeps(s, DT, substeps))) return r;
       const auto t0 = std::chrono::steady_clock::now();
       if ((r = dfargo_run_substeps(s, DT, substeps))) return r;
-      ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / substeps;
+      ms[3] = ms[4] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / substeps;
     }
     return DFARGO_OK;
   };
   rc = dfargo_init_euler(s, rho.data(), vr.data(), vt.data(), c->p.energy_equation ? en.data() : nullptr, &pl);
-  const int xcd[] = {1, 0}, waves[] = {4, 2, 1}, blocks[] = {256, 128, 64}, bufs[] = {2, 1};
-  // the transforms' buffers only with the full self-gravity
-  const int nknobs = (c->p.self_gravity && !c->p.sg_zero_mode) ? 4 : 3;
+  const int xcd[] = {1, 0}, waves[] = {4, 2, 1}, blocks[] = {256, 128, 64}, bufs[] = {2, 1}, pairs[] = {2, 1};
+  // the transforms' knobs only with the full self-gravity
+  const int nknobs = (c->p.self_gravity && !c->p.sg_zero_mode) ? 5 : 3;
   for (int knob = 0; knob < nknobs && !rc; knob++) {
     double best_ms = 1e30;
-    const int nv = (knob == 0 || knob == 3) ? 2 : 3;
+    const int nv = (knob == 0 || knob >= 3) ? 2 : 3;
     for (int v = 0; v < nv && !rc; v++) {
       cur = best;
       if (knob == 0) cur.xcd_remap = xcd[v];
       else if (knob == 1) cur.azi_waves = waves[v];
       else if (knob == 2) cur.finish_block = blocks[v];
-      else cur.sg_fft_buffers = bufs[v];
-      double ms[4];
+      else if (knob == 3) cur.sg_fft_buffers = bufs[v];
+      else cur.sg_rows_pair = pairs[v];
+      double ms[5];
       if ((rc = trial(cur, ms))) break;
       if (ms[knob] < best_ms) {
         best_ms = ms[knob];
         if (knob == 0) best.xcd_remap = cur.xcd_remap;
         else if (knob == 1) best.azi_waves = cur.azi_waves;
         else if (knob == 2) best.finish_block = cur.finish_block;
-        else best.sg_fft_buffers = cur.sg_fft_buffers;
+        else if (knob == 3) best.sg_fft_buffers = cur.sg_fft_buffers;
+        else best.sg_rows_pair = cur.sg_rows_pair;
       }
     }
   }

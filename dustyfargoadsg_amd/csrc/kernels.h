@@ -53,6 +53,9 @@ struct KArgs {
   // LDS buffers per transform of the self-gravity kernels: 0 the default
   // (one where that lets more workgroups share a CU), 1 or 2
   int sg_fft_buffers;
+  // rings per workgroup of the self-gravity row transforms: 0 the default
+  // (two where the rows run in place), 1 or 2
+  int sg_rows_pair;
   // with the full self-gravity: k_finish also forms the next sub-step's row
   // spectra (one workgroup per ring), and the self-gravity branch of a
   // sub-step starts with the radial transforms

@@ -148,6 +148,20 @@ __global__ void __launch_bounds__(SG_TB_MAX) k_sg_cols(KArgs a, int force, const
   }
 }
 
+// ring i's accelerations from its inverse-transformed row r (padded
+// layout): normalisation and self-force compensation
+template <typename T> __device__ __forceinline__ void sg_accel_store(const KArgs &a, int i, const cx<T> *r) {
+  const int ns = a.ns;
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    const long l = (long)i * ns + j;
+    double accr = (double)r[fp(j)].x * a.sg_normr[i];
+    double acct = (double)r[fp(j)].y * a.sg_normt[i];
+    accr += DF_G * ldd<T>(a.rho, l) * a.sg_rstep * a.sg_tstep / a.sg_eps;
+    ((T *)a.sgr)[l] = (T)accr;
+    ((T *)a.sgt)[l] = (T)acct;
+  }
+}
+
 // inverse azimuthal transform, normalisation and self-force compensation
 // (src/sgacc.c:137-161); W stored transposed ([Nsec][Nrad], wt), read
 // with the rings of one XCD contiguous as in k_sg_rows_fwd
@@ -162,14 +176,46 @@ __global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_inv(KArgs a, int force, c
   cx<T> *r = A;
   if constexpr (NB == 1) fft_lds_ip<T, 1>(A, A, ns, (const cx<T> *)a.tw_ns, +1);
   else r = fft_lds<T>(A, B, ns, (const cx<T> *)a.tw_ns, +1);
+  sg_accel_store<T>(a, i, r);
+}
+
+// Two neighbouring rings per workgroup, each transformed in place in its
+// own buffer: the passes of both share their barriers and the transposed
+// accesses of zt / wt read or write both rings' elements of a wavenumber
+// side by side.  The butterflies are those of the other forms, so the
+// results are the same bit for bit.  Needs Nrad even (self-gravity takes a
+// power of two).
+template <typename T>
+__global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_fwd_pair(KArgs a, int force, cx<T> *zt) {
+  extern __shared__ unsigned char smem[];
+  if (!force && !a.ctrl->go) return;
+  const int i = 2 * ring_of_block(a), ns = a.ns, nr = a.nr;
+  cx<T> *A0 = (cx<T> *)smem, *A1 = A0 + fpsz(ns);
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
-    const long l = (long)i * ns + j;
-    double accr = (double)r[fp(j)].x * a.sg_normr[i];
-    double acct = (double)r[fp(j)].y * a.sg_normt[i];
-    accr += DF_G * ldd<T>(a.rho, l) * a.sg_rstep * a.sg_tstep / a.sg_eps;
-    ((T *)a.sgr)[l] = (T)accr;
-    ((T *)a.sgt)[l] = (T)acct;
+    A0[fp(j)] = sg_reduced<T>(a, i, ldd<T>(a.rho, (long)i * ns + j));
+    A1[fp(j)] = sg_reduced<T>(a, i + 1, ldd<T>(a.rho, (long)(i + 1) * ns + j));
   }
+  __syncthreads();
+  fft_lds_ip<T, 2>(A0, A1, ns, (const cx<T> *)a.tw_ns, -1);
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    zt[(long)j * nr + i] = A0[fp(j)];
+    zt[(long)j * nr + i + 1] = A1[fp(j)];
+  }
+}
+template <typename T>
+__global__ void __launch_bounds__(SG_TB_MAX) k_sg_rows_inv_pair(KArgs a, int force, const cx<T> *wt) {
+  extern __shared__ unsigned char smem[];
+  if (!force && !a.ctrl->go) return;
+  const int i = 2 * ring_of_block(a), ns = a.ns, nr = a.nr;
+  cx<T> *A0 = (cx<T> *)smem, *A1 = A0 + fpsz(ns);
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    A0[fp(j)] = wt[(long)j * nr + i];
+    A1[fp(j)] = wt[(long)j * nr + i + 1];
+  }
+  __syncthreads();
+  fft_lds_ip<T, 2>(A0, A1, ns, (const cx<T> *)a.tw_ns, +1);
+  sg_accel_store<T>(a, i, A0);
+  sg_accel_store<T>(a, i + 1, A1);
 }
 
 // generic in-place batched transforms (kernel spectra at set-up)
@@ -330,8 +376,30 @@ template <typename T> static void sgz_pipeline(int id, const KArgs &a, hipStream
 }
 
 // rows_fwd: Z^T (sg_z) -> pair transforms -> W^T (sg_w) -> rows_inv
+// Rings per workgroup of the row transforms: the tuning's choice
+// (a.sg_rows_pair 1 or 2), or by default two where the rows run in place
+// anyway (sg_nb), Nrad is even and the thread count fits a workgroup.
+// Measured (ms per sub-step, one against two rings): 1024x3072 FP64 (rows
+// in place) 0.620 / 0.613, FP32 (two buffers) 0.461 / 0.465; 512x1536
+// within 0.3 % either way.
+static int sg_rows_per_group(const KArgs &a, size_t csz) {
+  const bool fits = (a.nr % 2 == 0) && fft_ip_min_threads(2, a.ns) <= SG_TB_MAX;
+  if (!fits || a.sg_rows_pair == 1) return 1;
+  if (a.sg_rows_pair == 2) return 2;
+  return sg_nb(a, csz, 0) == 1 ? 2 : 1;
+}
+
 template <typename T> static void sg_pipeline(int id, const KArgs &a, hipStream_t s, int force) {
   const size_t csz = sizeof(cx<T>);
+  if ((id == KID_SG_ROWS_FWD || id == KID_SG_ROWS_INV) && sg_rows_per_group(a, csz) == 2) {
+    const size_t lds = rows_lds_bytes(a, csz, 2);
+    const int tb = sg_threads(a, lds, a.nr / 2, a.ns, fft_ip_min_threads(2, a.ns));
+    if (id == KID_SG_ROWS_FWD)
+      hipLaunchKernelGGL(k_sg_rows_fwd_pair<T>, dim3(a.nr / 2), dim3(tb), lds, s, a, force, (cx<T> *)a.sg_z);
+    else
+      hipLaunchKernelGGL(k_sg_rows_inv_pair<T>, dim3(a.nr / 2), dim3(tb), lds, s, a, force, (const cx<T> *)a.sg_w);
+    return;
+  }
   const int rnb = sg_nb(a, csz, 0), cnb = sg_nb(a, csz, 1);
   const size_t rows_lds = rows_lds_bytes(a, csz, rnb), cols_lds = cols_lds_bytes(a, csz, cnb);
   const int rows_tb = sg_threads(a, rows_lds, a.nr, a.ns / 2, rnb == 1 ? fft_ip_min_threads(1, a.ns) : 0);
@@ -379,6 +447,8 @@ template <typename T> void sg_prepare(const KArgs &a) {
   allow_lds(k_sg_rows_fwd<T, 1>, rows_lds_bytes(a, csz, 1));
   allow_lds(k_sg_cols<T, 1>, cols_lds_bytes(a, csz, 1));
   allow_lds(k_sg_rows_inv<T, 1>, rows_lds_bytes(a, csz, 1));
+  allow_lds(k_sg_rows_fwd_pair<T>, rows_lds_bytes(a, csz, 2));
+  allow_lds(k_sg_rows_inv_pair<T>, rows_lds_bytes(a, csz, 2));
 }
 
 template <typename T> void launch_sg_kernel(int id, const KArgs &a, hipStream_t s) {

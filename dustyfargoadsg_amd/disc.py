@@ -253,7 +253,7 @@ class Disc:
         t = _capi.Tuning()
         _check(self.lib, self.lib.dfargo_get_tuning(self.ctx, ctypes.byref(t)))
         return dict(xcd_remap=t.xcd_remap, azi_waves=t.azi_waves, finish_block=t.finish_block,
-                    sg_fft_buffers=t.sg_fft_buffers)
+                    sg_fft_buffers=t.sg_fft_buffers, sg_rows_pair=t.sg_rows_pair)
 
     def set_tuning(self, **kw):
         t = _capi.Tuning(**dict(self.tuning(), **kw))
@@ -265,7 +265,7 @@ class Disc:
         t = _capi.Tuning()
         _check(self.lib, self.lib.dfargo_autotune(self.ctx, substeps, ctypes.byref(t)))
         return dict(xcd_remap=t.xcd_remap, azi_waves=t.azi_waves, finish_block=t.finish_block,
-                    sg_fft_buffers=t.sg_fft_buffers)
+                    sg_fft_buffers=t.sg_fft_buffers, sg_rows_pair=t.sg_rows_pair)
 
     # -- measurement --------------------------------------------------------
     def set_profiling(self, on):

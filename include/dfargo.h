@@ -434,7 +434,8 @@ int dfargo_read_planet_config(const char *path, const dfargo_params *p, dfargo_p
 /* Launch shapes of the sub-step kernels that do not change results: the
  * ring kernels' XCD-contiguous block order, the azimuthal transport's
  * wavefronts per workgroup, the velocity/boundary kernel's workgroup
- * width and the LDS buffers of the self-gravity transforms.  No
+ * width, the LDS buffers of the self-gravity transforms and their rings
+ * per workgroup.  No
  * counterpart in the reference (its loops have no launch shape). */
 typedef struct dfargo_tuning {
   int xcd_remap;     /* 1: the rings of one XCD are contiguous; 0: round-robin */
@@ -443,6 +444,8 @@ typedef struct dfargo_tuning {
   int sg_fft_buffers; /* LDS buffers per transform: 2 (ping-pong), 1 (in
                          place, half the LDS, one more barrier per pass), or
                          0: one where that lets more workgroups share a CU   */
+  int sg_rows_pair;   /* rings per workgroup of the self-gravity row transforms:
+                         1, 2, or 0: two where the rows run in place         */
 } dfargo_tuning;
 int dfargo_get_tuning(dfargo_ctx *ctx, dfargo_tuning *t);
 int dfargo_set_tuning(dfargo_ctx *ctx, const dfargo_tuning *t);

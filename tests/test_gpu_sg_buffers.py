@@ -1,7 +1,8 @@
 """The self-gravity transforms in one LDS buffer per transform (in place,
 half the LDS) against the two-buffer Stockham form (ping-pong): the
 butterflies and the spectral product do the same arithmetic in the same
-order, so the accelerations and a sub-step's state are equal bit for bit.
+order, so the accelerations and a sub-step's state are equal bit for bit;
+so are the row transforms taken two rings per workgroup.
 The grids take radix 8, 4, 3 and 2 passes (rows of 48 to 192 cells,
 radial transforms of 32 to 256; self-gravity needs NRAD a power of two)."""
 import math
@@ -18,11 +19,11 @@ pytestmark = pytest.mark.gpu
 DT = 2 * math.pi / 20
 
 
-def make(p, pl, prec, bufs):
+def make(p, pl, prec, bufs, pair=1):
     d = Disc(p, prec)
     d.initial_state(pl)
     if bufs is not None:
-        d.set_tuning(sg_fft_buffers=bufs)
+        d.set_tuning(sg_fft_buffers=bufs, sg_rows_pair=pair)
     return d
 
 
@@ -31,17 +32,22 @@ def make(p, pl, prec, bufs):
 def test_one_buffer_equals_two(prec, nr, ns):
     print(f"case sg buffers {nr}x{ns} prec={prec}", flush=True)
     p, pl = energy_sg(nrad=nr, nsec=ns)
-    one, two = make(p, pl, prec, 1), make(p, pl, prec, 2)
-    assert one.tuning()["sg_fft_buffers"] == 1 and two.tuning()["sg_fft_buffers"] == 2
-    for a, b in zip(one.compute_selfgravity(), two.compute_selfgravity()):
-        assert np.array_equal(a, b)
-    one.algogas(DT)
+    two = make(p, pl, prec, 2)
+    assert two.tuning()["sg_fft_buffers"] == 2 and two.tuning()["sg_rows_pair"] == 1
+    g2 = two.compute_selfgravity()
     two.algogas(DT)
-    s1, s2 = one.state(), two.state()
-    for k in ("rho", "vr", "vt", "energy"):
-        assert np.array_equal(s1[k], s2[k]), k
-    for f in ("sg_accr", "sg_acct"):
-        assert np.array_equal(one.field(f), two.field(f)), f
+    s2 = two.state()
+    # one buffer per transform; two rings per row workgroup (in place)
+    for bufs, pair in ((1, 1), (2, 2)):
+        one = make(p, pl, prec, bufs, pair)
+        for a, b in zip(one.compute_selfgravity(), g2):
+            assert np.array_equal(a, b), (bufs, pair)
+        one.algogas(DT)
+        s1 = one.state()
+        for k in ("rho", "vr", "vt", "energy"):
+            assert np.array_equal(s1[k], s2[k]), (bufs, pair, k)
+        for f in ("sg_accr", "sg_acct"):
+            assert np.array_equal(one.field(f), two.field(f)), (bufs, pair, f)
 
 
 def test_default_at_full_size():
@@ -63,3 +69,5 @@ def test_tuning_range():
         d.set_tuning(sg_fft_buffers=3)
     with pytest.raises(DfargoError):
         d.set_tuning(sg_fft_buffers=-1)
+    with pytest.raises(DfargoError):
+        d.set_tuning(sg_rows_pair=3)
